@@ -1,0 +1,143 @@
+/*
+ * ttc.h -- C ABI of the MI355X nonbonded path (libttc.so).
+ *
+ * Plain C types only: pointers, sizes, doubles.  One context per simulation
+ * box, one HIP stream per context, host calls single-threaded per context
+ * (as the reference is).  Every entry point returns TTC_OK (0) or an error
+ * code whose value is the exit code the reference uses for the same
+ * condition; ttc_last_error() gives the message.
+ *
+ * Each entry point names the reference routine it stands behind
+ * (file:line in the reference tree).  Arrays the reference keeps as arrays
+ * of structs are taken as (pointer to the first double, stride in doubles),
+ * so &atom[k][0].x with stride sizeof(struct atoms)/sizeof(double) = 8, or
+ * &ff[k][0].x with stride 3, can be passed straight through.
+ */
+#ifndef TTC_H
+#define TTC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes = the reference's exit codes for the same condition */
+#define TTC_OK                0
+#define TTC_ERR_GRID          1   /* pme_qgrid.c:106-110 grid index out of range */
+#define TTC_ERR_LIST_OVERFLOW 5   /* nblist.c:185-196 neighbour row overflow      */
+#define TTC_ERR_ALLOC         11  /* allocate.c passim                            */
+#define TTC_ERR_NOT_NEUTRAL   20  /* ewald.c:184-194 net charge > 5e-6            */
+#define TTC_ERR_HIP           99  /* HIP runtime failure (no reference analogue)  */
+#define TTC_ERR_ARG           111 /* checks.c flag/mode conflicts                 */
+
+/* which list ttc_get_nblist exports */
+#define TTC_LIST_NLIST    0       /* nlist[k]    (rclist)    */
+#define TTC_LIST_NLIST_EW 1       /* nlist_ew[k] (rclist_ew) */
+
+typedef struct ttc_ctx ttc_ctx;
+
+/* Scalars the path reads from sim / box (defines.h:1228-1299). */
+typedef struct {
+    int    n;              /* box[k].boxns                                  */
+    double lx, ly, lz;     /* box[k].lx, ly, lz                             */
+    double rc;             /* sim.rc                                        */
+    double rclist;         /* sim.rclist                                    */
+    double rclist_ew;      /* sim.rclist_ew (used when ewald_lists != 0)    */
+    int    ewald_lists;    /* 1: EWALD build, nlist nests in nlist_ew       */
+    double kappa;          /* sim.kappa[k]                                  */
+    double eps_rf;         /* sim.epsRF[k]                                  */
+    double e14fac;         /* E14FAC (defines.h:72-76)                      */
+    int    order;          /* sim.order (4 or 6)                            */
+    int    gx, gy, gz;     /* sim.grid[0..2]                                */
+    int    ntypes;         /* number of site types (STYPE tables)           */
+    double dt;             /* sim.dt [ps]                                   */
+} ttc_params;
+
+/* Allocation and set-up: allocate.c:318-506, ewald_setup ewald.c:46-196
+ * (pme_init pme_setup.c:59, pme_bspline 86, pme_dftmod 110).             */
+int  ttc_create(const ttc_params *p, ttc_ctx **out);
+/* pme_clean pme_setup.c:75 + release of every device buffer */
+int  ttc_destroy(ttc_ctx *c);
+const char *ttc_last_error(void);
+
+/* Topology: pott[k][i].qq / .mas, atom[k][i].atomid, ljset / ljset14
+ * (setlj.c:181-191, types x types row-major), exclusions xint[k] in
+ * read_topology.c:847-891 order (a, b, xflag), bonds (cbond.c:65-120).
+ * Fails with TTC_ERR_NOT_NEUTRAL like ewald.c:184-194.                   */
+int  ttc_set_topology(ttc_ctx *c, const int *atomid, const double *qq, const double *mass,
+                      const double *lj_eps, const double *lj_sig,
+                      const double *lj14_eps, const double *lj14_sig,
+                      int nx, const int *xa, const int *xb, const int *xflag,
+                      int nbond, const int *ba, const int *bb,
+                      const double *req, const double *kbond);
+
+/* en[k].sewald (ewald.c:171-178), available after ttc_set_topology */
+double ttc_self_energy(const ttc_ctx *c);
+
+/* Host -> device / device -> host copies of per-site state, original site
+ * order.  x points at the first x coordinate, stride in doubles.          */
+int  ttc_set_positions(ttc_ctx *c, const double *x, int64_t stride);
+int  ttc_get_positions(ttc_ctx *c, double *x, int64_t stride);
+int  ttc_set_velocities(ttc_ctx *c, const double *v, int64_t stride);
+int  ttc_get_velocities(ttc_ctx *c, double *v, int64_t stride);
+
+/* nblist.c:32 nblist(): cell binning + sort + Verlet build on the device
+ * from the current device positions; anchors snapshot and nl_flag = 0
+ * (nblist.c:583-591).                                                     */
+int  ttc_nblist(ttc_ctx *c);
+/* nblist.c:607 nl_check(lb,ub,k): *nl_flag = 1 if a site in [lb,ub)
+ * (original numbering) moved more than (rclist-rc)/4 since ttc_nblist.    */
+int  ttc_nl_check(ttc_ctx *c, int lb, int ub, int *nl_flag);
+/* Export of nlist[k] / nlist_ew[k] in the reference layout: row a holds
+ * count[a] entries b > a, ascending, list[a*max_nb + j], xflag likewise.
+ * Returns TTC_ERR_LIST_OVERFLOW if a row exceeds max_nb.                  */
+int  ttc_get_nblist(ttc_ctx *c, int which, int max_nb, int *count, int *list, int *xflag);
+
+/* cnonbond_nblist.c:51 cnonbondnl(ibox, energy): LJ + SPME real space +
+ * exclusion/1-4 corrections + pme_calc.  Forces are ADDED into ff
+ * (caller zeroes, forces.c:126-130); energy[8] as cnonbond_nblist.c:613-620;
+ * vir_nbond = pvir[k].nbond, vir_kspace = pvir[k].ewald_kspace.  Any of
+ * ff / energy / vir_* may be NULL.                                       */
+int  ttc_cnonbondnl(ttc_ctx *c, double *ff, int64_t stride, double *energy8,
+                    double *vir_nbond6, double *vir_kspace6);
+/* pme_calc.c:18 pme_calc(k, energy): charge_grid + FFTs + convolution +
+ * pme_force; energy[3] = {recip, self, recip - self}.                    */
+int  ttc_pme_calc(ttc_ctx *c, double *ff, int64_t stride, double *energy3, double *vir6);
+/* The pair loop of cnonbondnl alone (cnonbond_nblist.c:134-523):
+ * e5 = {enonbond, enonbonds, eewald_real, eewald_reals, eewald_intra}.  */
+int  ttc_pair(ttc_ctx *c, double *ff, int64_t stride, double *e5, double *vir6);
+/* cbond.c:65-120: returns bond energy through *ebond, forces added.       */
+int  ttc_cbond(ttc_ctx *c, double *ff, int64_t stride, double *ebond, double *vir6);
+
+/* forces.c:118 forces(ibox) for a system whose bonded terms are bonds:
+ * zero, cbond, cnonbondnl; en[k] slots (forces.c:204-219) are returned as
+ * en9 = {bond, nbond, nbonds, coulomb, rewald, rewalds, kewald, sewald, iewald}. */
+int  ttc_forces(ttc_ctx *c, double *ff, int64_t stride, double *en9);
+
+/* integrate_vverlet.c:36-103 repeated nsteps times with nl_check/nblist
+ * on the displacement trigger (nve_md.c:369-370), all on the device as one
+ * captured HIP graph per step.  Every energy_every steps (0: never) a row
+ * {step, kinet, bond, nbond, nbonds, coulomb, kewald, rewald, iewald}
+ * is appended to energies (rows of 9 doubles, capacity max_rows).
+ * vscal multiplies velocities as the reference's vscal argument.
+ * *rebuilds (optional) receives the number of list rebuilds.             */
+int  ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_every,
+                    double *energies, int max_rows, int *nrows, int *rebuilds);
+
+/* kinet.c:18-68 on the device; ken6 optional. Returns en[k].kinet.       */
+int  ttc_kinet(ttc_ctx *c, double *kinet, double *ken6);
+
+/* Average device time of the named kernel class over the last
+ * ttc_md_vverlet call, measured with HIP events on the context's stream:
+ * which = 0 pair kernel, 1 spread, 2 whole step.  Microseconds.          */
+int  ttc_timing(ttc_ctx *c, int which, double *usec);
+
+/* Counters of the last list build: total entries of the full device list
+ * (each pair twice), longest row, rows capacity.                         */
+int  ttc_list_stats(ttc_ctx *c, int64_t *entries, int *max_row, int *capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

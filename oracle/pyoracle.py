@@ -1,0 +1,242 @@
+"""ctypes binding of the CPU oracle (liboracle.so) and of oracle/_ref.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, and there only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class OrcSystem(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+        ("rc", C.c_double), ("rclist", C.c_double), ("rclist_ew", C.c_double),
+        ("kappa", C.c_double), ("eps_rf", C.c_double), ("e14fac", C.c_double),
+        ("order", C.c_int), ("gx", C.c_int), ("gy", C.c_int), ("gz", C.c_int),
+        ("ntypes", C.c_int), ("type", C.POINTER(C.c_int)), ("q", C.POINTER(C.c_double)),
+        ("lj_eps", C.POINTER(C.c_double)), ("lj_sig", C.POINTER(C.c_double)),
+        ("lj14_eps", C.POINTER(C.c_double)), ("lj14_sig", C.POINTER(C.c_double)),
+    ]
+
+
+class OrcExcl(C.Structure):
+    _fields_ = [("n", C.c_int), ("off", C.POINTER(C.c_int)), ("partner", C.POINTER(C.c_int)),
+                ("flag", C.POINTER(C.c_int))]
+
+
+class Ran2State(C.Structure):
+    _fields_ = [("idum", C.c_long), ("idum2", C.c_long), ("iy", C.c_long), ("iv", C.c_long * 32)]
+
+
+def build(fast: bool = False) -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def _load(name: str):
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        build()
+    return C.CDLL(path)
+
+
+_lib = None
+_fast = None
+
+
+def lib(fast: bool = False):
+    global _lib, _fast
+    if fast:
+        if _fast is None:
+            _fast = _load("liboracle_fast.so")
+            _declare(_fast)
+        return _fast
+    if _lib is None:
+        _lib = _load("liboracle.so")
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    P = C.POINTER
+    L.orc_erfc.restype = C.c_double; L.orc_erfc.argtypes = [C.c_double]
+    L.orc_ewald_self.restype = C.c_double
+    L.orc_ewald_self.argtypes = [C.c_int, _dp, C.c_double, C.c_double]
+    L.orc_check_neutral.restype = C.c_int; L.orc_check_neutral.argtypes = [C.c_int, _dp]
+    L.orc_fillspline.argtypes = [C.c_double, C.c_int, _dp, _dp]
+    L.orc_dftmod.argtypes = [_dp, _dp, C.c_int]
+    L.orc_pme_bspline.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+    L.orc_fft3d.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.orc_dft3d_naive.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.orc_excl_build.restype = C.c_int
+    L.orc_excl_build.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip, P(OrcExcl)]
+    L.orc_excl_free.argtypes = [P(OrcExcl)]
+    L.orc_nblist.restype = C.c_int
+    L.orc_nblist.argtypes = [P(OrcSystem), _dp, P(OrcExcl), C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip]
+    L.orc_nblist_cells.restype = C.c_int
+    L.orc_nblist_cells.argtypes = [P(OrcSystem), _dp, P(OrcExcl), C.c_int, _ip, _ip, _ip]
+    L.orc_nl_check.restype = C.c_int
+    L.orc_nl_check.argtypes = [P(OrcSystem), _dp, _dp, C.c_int, C.c_int]
+    L.orc_pme_calc.argtypes = [P(OrcSystem), _dp, C.c_double, _dp, _dp, _dp, C.c_void_p]
+    L.orc_cnonbondnl.argtypes = [P(OrcSystem), _dp, _ip, _ip, _ip, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+    L.orc_cbond.restype = C.c_double
+    L.orc_cbond.argtypes = [P(OrcSystem), _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
+    L.orc_pbc_all.argtypes = [P(OrcSystem), _dp, C.c_int]
+    L.orc_kinet.restype = C.c_double
+    L.orc_kinet.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+    L.orc_vverlet_a.argtypes = [P(OrcSystem), C.c_int, _dp, C.c_double, C.c_double, _dp, C.c_void_p, _dp, _dp]
+    L.orc_vverlet_b.argtypes = [C.c_int, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
+    L.orc_nhc.restype = C.c_double
+    L.orc_nhc.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
+                          C.c_int, C.c_int, _dp, _dp, C.c_double]
+    L.orc_go_energy.restype = C.c_double
+    L.orc_go_energy.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_int, _ip, _ip, _dp,
+                                C.c_double, C.c_double, C.c_double]
+    L.orc_go_range_energy.restype = C.c_double
+    L.orc_go_range_energy.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double, C.c_double, _u8,
+                                      _dp, C.c_double, C.c_double, C.c_double]
+    L.orc_ran2_seed.argtypes = [P(Ran2State), C.c_long]
+    L.orc_ran2.restype = C.c_double; L.orc_ran2.argtypes = [P(Ran2State)]
+    L.orc_mlp_nparams.restype = C.c_int; L.orc_mlp_nparams.argtypes = [C.c_int, C.c_int]
+    L.orc_mlp_forward.restype = C.c_double
+    L.orc_mlp_forward.argtypes = [C.c_int, C.c_int, _dp, _dp]
+    L.orc_mlp_loss_grad.restype = C.c_double
+    L.orc_mlp_loss_grad.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, _dp]
+
+
+class Oracle:
+    """Holds an orc_system built from a ChainSystem-like object and keeps the
+    numpy arrays it points into alive."""
+
+    def __init__(self, sysm, fast: bool = False):
+        self.L = lib(fast)
+        self.s = sysm
+        self._keep = []
+
+        def dp(a):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            self._keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_double))
+
+        t = np.ascontiguousarray(sysm.type, dtype=np.int32)
+        self._keep.append(t)
+        self.cs = OrcSystem(
+            n=sysm.n, lx=sysm.box[0], ly=sysm.box[1], lz=sysm.box[2], rc=sysm.rc,
+            rclist=sysm.rclist, rclist_ew=sysm.rclist_ew, kappa=sysm.kappa, eps_rf=sysm.eps_rf,
+            e14fac=sysm.e14fac, order=sysm.order, gx=sysm.grid[0], gy=sysm.grid[1], gz=sysm.grid[2],
+            ntypes=sysm.ntypes, type=t.ctypes.data_as(C.POINTER(C.c_int)), q=dp(sysm.q),
+            lj_eps=dp(sysm.lj_eps), lj_sig=dp(sysm.lj_sig), lj14_eps=dp(sysm.lj14_eps),
+            lj14_sig=dp(sysm.lj14_sig))
+        x = np.ascontiguousarray(sysm.xint, dtype=np.int32)
+        self.ex = OrcExcl()
+        xa, xb, xf = (np.ascontiguousarray(x[:, i]) for i in range(3))
+        if self.L.orc_excl_build(sysm.n, len(x), xa, xb, xf, C.byref(self.ex)) != 0:
+            raise ValueError("bad exclusion list")
+        self.e_self = self.L.orc_ewald_self(sysm.n, np.ascontiguousarray(sysm.q, dtype=np.float64),
+                                            sysm.kappa, sysm.eps_rf)
+
+    def __del__(self):
+        try:
+            self.L.orc_excl_free(C.byref(self.ex))
+        except Exception:
+            pass
+
+    def nblist(self, pos, max_nb=1500, use_ew=True, cells=False):
+        n = self.s.n
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        cnt = np.zeros(n, np.int32); lst = np.zeros(n * max_nb, np.int32); xf = np.zeros(n * max_nb, np.int32)
+        if cells:
+            rc = self.L.orc_nblist_cells(C.byref(self.cs), pos, C.byref(self.ex), max_nb, cnt, lst, xf)
+            return rc, cnt, lst.reshape(n, max_nb), xf.reshape(n, max_nb)
+        ce = np.zeros(n, np.int32); le = np.zeros(n * max_nb, np.int32); xe = np.zeros(n * max_nb, np.int32)
+        rc = self.L.orc_nblist(C.byref(self.cs), pos, C.byref(self.ex), int(use_ew), max_nb,
+                               cnt, lst, xf, ce, le, xe)
+        return (rc, cnt, lst.reshape(n, max_nb), xf.reshape(n, max_nb),
+                ce, le.reshape(n, max_nb), xe.reshape(n, max_nb))
+
+    def cnonbondnl(self, pos, cnt, lst, xf):
+        n = self.s.n
+        max_nb = lst.shape[1]
+        ff = np.zeros(n * 3)
+        en = np.zeros(8); vn = np.zeros(6); vk = np.zeros(6)
+        self.L.orc_cnonbondnl(C.byref(self.cs), np.ascontiguousarray(pos, dtype=np.float64),
+                              np.ascontiguousarray(cnt, dtype=np.int32),
+                              np.ascontiguousarray(lst, dtype=np.int32).ravel(),
+                              np.ascontiguousarray(xf, dtype=np.int32).ravel(), max_nb,
+                              self.e_self, ff, en, vn, vk)
+        return ff.reshape(n, 3), en, vn, vk
+
+    def pme(self, pos, want_grid=False):
+        n = self.s.n
+        ff = np.zeros(n * 3); en = np.zeros(3); vir = np.zeros(6)
+        g = None
+        if want_grid:
+            g = np.zeros(2 * int(np.prod(self.s.grid)))
+            self.L.orc_pme_calc(C.byref(self.cs), np.ascontiguousarray(pos, dtype=np.float64),
+                                self.e_self, ff, en, vir, g.ctypes.data_as(C.c_void_p))
+            g = g[0::2].reshape(self.s.grid)
+        else:
+            self.L.orc_pme_calc(C.byref(self.cs), np.ascontiguousarray(pos, dtype=np.float64),
+                                self.e_self, ff, en, vir, None)
+        return ff.reshape(n, 3), en, vir, g
+
+    def bonds(self, pos):
+        n = self.s.n
+        ff = np.zeros(n * 3); vir = np.zeros(6)
+        b = np.ascontiguousarray(self.s.bonds, dtype=np.int32)
+        e = self.L.orc_cbond(C.byref(self.cs), np.ascontiguousarray(pos, dtype=np.float64), len(b),
+                             np.ascontiguousarray(b[:, 0]), np.ascontiguousarray(b[:, 1]),
+                             np.ascontiguousarray(self.s.bond_r0, dtype=np.float64),
+                             np.ascontiguousarray(self.s.bond_k, dtype=np.float64), ff, vir)
+        return e, ff.reshape(n, 3), vir
+
+    def forces(self, pos, max_nb=1500, lists=None):
+        """forces.c:118-219 for a system with bonds only: returns (ff, en dict)."""
+        if lists is None:
+            rc, cnt, lst, xf, *_ = self.nblist(pos, max_nb)
+            if rc:
+                raise RuntimeError(f"neighbour list overflow (reference exit {rc})")
+        else:
+            cnt, lst, xf = lists
+        ff, en, vn, vk = self.cnonbondnl(pos, cnt, lst, xf)
+        eb, fb, _ = self.bonds(pos)
+        return ff + fb, {"bond": eb, "nbond": en[0], "nbonds": en[1], "coulomb": en[2],
+                         "rewalds": en[3], "kewald": en[4], "sewald": en[5], "iewald": en[6],
+                         "rewald": en[7], "slots": en, "vir_nbond": vn, "vir_kspace": vk}
+
+
+# --------------------------------------------------------------------------
+# oracle/_ref: the reference's own routines compiled from its sources
+# --------------------------------------------------------------------------
+
+REF_SO = os.path.join(HERE, "_ref", "libttcref.so")
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref_lib():
+    L = C.CDLL(REF_SO)
+    L.ref_setup.restype = C.c_int
+    L.ref_setup.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                            C.c_double, C.c_double, C.c_double, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
+                            C.c_int, _ip, _ip, _ip, C.c_int]
+    L.ref_nblist.argtypes = [_dp, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip]
+    L.ref_nl_check.restype = C.c_int
+    L.ref_nl_check.argtypes = [_dp, C.c_int, C.c_int]
+    L.ref_ewald_setup.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    L.ref_cnonbondnl.argtypes = [_dp, _dp, _dp, _dp]
+    L.ref_ewald_kspace.argtypes = [_dp, _dp, _dp]
+    L.ref_erfc.restype = C.c_double; L.ref_erfc.argtypes = [C.c_double]
+    L.ref_self_energy.restype = C.c_double
+    return L

@@ -1,0 +1,172 @@
+/*
+ * ttc_oracle.h -- FP64 CPU restatement of TheCodeCombo_7's nonbonded path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP path is
+ * compared against (tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg).  The product library (libttc.so) never links, loads or
+ * calls it.
+ *
+ * Every function follows one reference routine; the file:line of the routine
+ * is given beside each declaration.  Conventions kept from the reference:
+ *   - positions are FP64, AoS (x,y,z per site), wrapped into [-L/2, L/2];
+ *   - energies kJ/mol, lengths Angstrom, charges e, masses amu, time ps;
+ *   - forces are ADDED into the caller's array (the caller zeroes it, as
+ *     forces.c:126-130 does);
+ *   - half Verlet list: row a holds b > a (nblist.c:95-307).
+ *
+ * Parity pins (what checks each function against something other than
+ * itself) are listed in the header comment of ttc_oracle.c.
+ */
+#ifndef TTC_ORACLE_H
+#define TTC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Physical constants: defines.h:60-71 (same values, same product order). */
+#define ORC_PI      3.141592653589793238462643383
+#define ORC_KB      1.380662E-23
+#define ORC_NA      6.022045E23
+#define ORC_AU      1.6605655E-27
+#define ORC_ELEC    8.987551789e19
+#define ORC_ELEQ    1.602176462e-19
+#define ORC_QFACTOR (ORC_ELEC * ORC_ELEQ * ORC_ELEQ * ORC_NA * 1.0E-3)
+
+typedef struct {
+    int n;                 /* box[k].boxns                                   */
+    double lx, ly, lz;     /* box[k].lx..lz (hx = lx/2 etc.)                 */
+    double rc;             /* sim.rc                                         */
+    double rclist;         /* sim.rclist                                     */
+    double rclist_ew;      /* sim.rclist_ew (nested list, EWALD builds)      */
+    double kappa;          /* sim.kappa[k]                                   */
+    double eps_rf;         /* sim.epsRF[k]                                   */
+    double e14fac;         /* E14FAC (defines.h:72-76; a parameter here)     */
+    int order;             /* sim.order                                      */
+    int gx, gy, gz;        /* sim.grid[0..2]                                 */
+    int ntypes;            /* number of site types (STYPE)                   */
+    const int *type;       /* atom[k][i].atomid                              */
+    const double *q;       /* pott[k][i].qq                                  */
+    const double *lj_eps;  /* ljset[ta][tb].eps   (ntypes*ntypes, row major) */
+    const double *lj_sig;  /* ljset[ta][tb].sig                              */
+    const double *lj14_eps;/* ljset14[ta][tb].eps                            */
+    const double *lj14_sig;/* ljset14[ta][tb].sig                            */
+} orc_system;
+
+/* Exclusions in compressed-row form, both directions, one resolved flag per
+ * (a,b): the flag of the LAST xint entry naming the pair, which is what the
+ * hash scan of nblist.c:156-175 yields (later matches overwrite).          */
+typedef struct {
+    int n;
+    int *off;      /* n+1 */
+    int *partner;  /* off[n] */
+    int *flag;     /* off[n]; xflag 1 bond, 2 bend, 3 exNB, 4 1-4 */
+} orc_excl;
+
+/* --- ewald.c -------------------------------------------------------------- */
+double orc_erfc(double x);                                   /* ewald.c:817-835 */
+double orc_ewald_self(int n, const double *q, double kappa, double eps_rf);
+                                                             /* ewald.c:171-178 */
+int orc_check_neutral(int n, const double *q);               /* ewald.c:184-194 */
+
+/* --- pme_spline.c / pme_setup.c ----------------------------------------- */
+void orc_fillspline(double w, int order, double *array, double *darray);
+                                                             /* pme_spline.c:10-49 */
+void orc_dftmod(double *bmod, const double *barr, int grid); /* pme_setup.c:110-130 */
+void orc_pme_bspline(int order, int gx, int gy, int gz,
+                     double *bx, double *by, double *bz);    /* pme_setup.c:86-107 */
+
+/* --- FFT (stands in for FFTW3's unnormalised complex 3-D DFT) ----------- */
+void orc_fft3d(double *data, int nx, int ny, int nz, int sign);
+void orc_dft3d_naive(const double *in, double *out, int nx, int ny, int nz, int sign);
+
+/* --- exclusions ---------------------------------------------------------- */
+int  orc_excl_build(int n, int nx, const int *xa, const int *xb, const int *xf,
+                    orc_excl *out);                          /* read_topology.c:847-891 */
+void orc_excl_free(orc_excl *e);
+int  orc_excl_flag(const orc_excl *e, int a, int b);
+
+/* --- nblist.c ------------------------------------------------------------ */
+int orc_nblist(const orc_system *s, const double *pos, const orc_excl *ex,
+               int use_ew, int max_nb,
+               int *count, int *list, int *xflag,
+               int *count_ew, int *list_ew, int *xflag_ew); /* nblist.c:32-307 */
+int orc_nblist_cells(const orc_system *s, const double *pos, const orc_excl *ex,
+                     int max_nb, int *count, int *list, int *xflag);
+                                         /* nblist.c:327-557 (CLIST), rows sorted */
+int orc_nl_check(const orc_system *s, const double *pos, const double *anchor,
+                 int lb, int ub);                            /* nblist.c:607-647 */
+
+/* --- cnonbond_nblist.c ----------------------------------------------------- */
+typedef struct {
+    double enonbond, enonbonds;               /* LJ, LJ cutoff shifts        */
+    double eewald_real, eewald_reals, eewald_intra;
+    double vir[6];                            /* wxx wyx wyy wzx wzy wzz     */
+} orc_pair_out;
+
+void orc_pair(const orc_system *s, const double *pos, const int *count,
+              const int *list, const int *xflag, int max_nb,
+              double *ff, orc_pair_out *out);       /* cnonbond_nblist.c:134-523 */
+
+/* pme_qgrid.c:13-121 + pme_calc.c:18-279.  energy[3] = {recip, self, recip-self};
+ * qgrid_out (optional, 2*gx*gy*gz doubles) receives the spread charge grid. */
+void orc_pme_calc(const orc_system *s, const double *pos, double e_self,
+                  double *ff, double energy[3], double vir[6], double *qgrid_out);
+
+/* cnonbond_nblist.c:51-622 (STYPE+S14+SPME): pair loop + pme_calc, energy[8]
+ * slots as at cnonbond_nblist.c:613-620.                                     */
+void orc_cnonbondnl(const orc_system *s, const double *pos, const int *count,
+                    const int *list, const int *xflag, int max_nb, double e_self,
+                    double *ff, double energy[8], double vir_nbond[6],
+                    double vir_kspace[6]);
+
+/* --- bonded, integrator, kinetic ------------------------------------------- */
+double orc_cbond(const orc_system *s, const double *pos, int nb, const int *ia,
+                 const int *ib, const double *req, const double *kb, double *ff,
+                 double vir[6]);                             /* cbond.c:65-120 */
+void orc_pbc_all(const orc_system *s, double *pos, int n);  /* pbc_all.c:23-88 */
+double orc_kinet(int n, const double *mass, const double *vv, const double *uu,
+                 double ken[6]);                             /* kinet.c:18-68 */
+/* first half of integrate_vverlet.c:52-83 (kick, drift, pbc) */
+void orc_vverlet_a(const orc_system *s, int n, const double *mass, double dt,
+                   double vscal, double *pos, double *pos_nopbc, double *vv,
+                   const double *ff);
+/* second half, integrate_vverlet.c:93-102 (kick, uu = vv) */
+void orc_vverlet_b(int n, const double *mass, double dt, double vscal,
+                   double *vv, double *uu, const double *ff);
+
+/* integrate_nhc.c:40-161; zeta laid out as {r,v,G} per chain element.
+ * Returns the velocity scale; vv and uu are scaled in place. */
+double orc_nhc(int n, double *vv, double *uu, const double *mass, double nfree,
+               double T, double Q, int M, int Nc, int Nys, const double *w,
+               double *zeta, double time_step);
+
+/* --- Go model (cgolik.c / egolik_mc.c) and Wang-Landau helpers ------------- */
+/* full Go energy of one chain: bonds k(r-r0)^2 + native 12-10 + non-native
+ * repulsion (cgolik.c:644-948, 12-10 form cgolik.c:1202-1229).            */
+double orc_go_energy(int n, const double *pos, double kbond, double r0bond,
+                     int nnat, const int *na, const int *nb, const double *nr0,
+                     double eps, double sig_rep, double rc);
+/* energy of the pairs touching [lb,ub) (enbnd_mc.c:98-119 range filter) */
+double orc_go_range_energy(int n, const double *pos, int lb, int ub,
+                           double kbond, double r0bond, const unsigned char *natmask,
+                           const double *r0mat, double eps, double sig_rep, double rc);
+
+/* ran2: L'Ecuyer + Bays-Durham (random.c:37-75, Numerical Recipes 2nd ed. 7.1) */
+typedef struct { long idum, idum2, iy; long iv[32]; } orc_ran2_state;
+void   orc_ran2_seed(orc_ran2_state *st, long seed);
+double orc_ran2(orc_ran2_state *st);
+
+/* --- small MLP (no reference counterpart): FP64 forward / backward --------- */
+/* layers: in -> H (tanh) -> H (tanh) -> 1 (linear); params packed
+ * W1[H*in], b1[H], W2[H*H], b2[H], W3[H], b3[1].                           */
+int    orc_mlp_nparams(int in, int H);
+double orc_mlp_forward(int in, int H, const double *p, const double *x);
+/* loss = 0.5*mean((f(x_i)-y_i)^2) over m samples; grad += dloss/dp */
+double orc_mlp_loss_grad(int in, int H, const double *p, int m, const double *x,
+                         const double *y, double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

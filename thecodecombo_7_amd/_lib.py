@@ -1,0 +1,93 @@
+"""ctypes binding of libttc.so (include/ttc.h).
+
+The library is loaded from this package directory only; if it is missing the
+import fails loudly (there is no CPU fallback for any operation).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libttc.so")
+
+TTC_OK = 0
+TTC_ERR_GRID = 1
+TTC_ERR_LIST_OVERFLOW = 5
+TTC_ERR_ALLOC = 11
+TTC_ERR_NOT_NEUTRAL = 20
+TTC_ERR_HIP = 99
+TTC_ERR_ARG = 111
+TTC_LIST_NLIST = 0
+TTC_LIST_NLIST_EW = 1
+
+class TtcParams(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+        ("rc", C.c_double), ("rclist", C.c_double), ("rclist_ew", C.c_double),
+        ("ewald_lists", C.c_int), ("kappa", C.c_double), ("eps_rf", C.c_double),
+        ("e14fac", C.c_double), ("order", C.c_int), ("gx", C.c_int), ("gy", C.c_int),
+        ("gz", C.c_int), ("ntypes", C.c_int), ("dt", C.c_double),
+    ]
+
+
+class TtcError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ttc error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libttc.so; raises if the HIP library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m thecodecombo_7_amd.build` "
+                          "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    vp = C.c_void_p
+    L.ttc_last_error.restype = C.c_char_p
+    L.ttc_create.argtypes = [C.POINTER(TtcParams), C.POINTER(vp)]
+    L.ttc_destroy.argtypes = [vp]
+    L.ttc_set_topology.argtypes = [vp, ip, dp, dp, dp, dp, dp, dp, C.c_int, ip, ip, ip, C.c_int, ip, ip, dp, dp]
+    L.ttc_self_energy.restype = C.c_double
+    L.ttc_self_energy.argtypes = [vp]
+    for f in ("ttc_set_positions", "ttc_get_positions", "ttc_set_velocities", "ttc_get_velocities"):
+        getattr(L, f).argtypes = [vp, dp, C.c_int64]
+    L.ttc_nblist.argtypes = [vp]
+    L.ttc_nl_check.argtypes = [vp, C.c_int, C.c_int, ip]
+    L.ttc_get_nblist.argtypes = [vp, C.c_int, C.c_int, ip, ip, ip]
+    L.ttc_cnonbondnl.argtypes = [vp, dp, C.c_int64, dp, dp, dp]
+    L.ttc_pme_calc.argtypes = [vp, dp, C.c_int64, dp, dp]
+    L.ttc_pair.argtypes = [vp, dp, C.c_int64, dp, dp]
+    L.ttc_cbond.argtypes = [vp, dp, C.c_int64, dp, dp]
+    L.ttc_forces.argtypes = [vp, dp, C.c_int64, dp]
+    L.ttc_md_vverlet.argtypes = [vp, C.c_int, C.c_double, C.c_int, dp, C.c_int, ip, ip]
+    L.ttc_kinet.argtypes = [vp, dp, dp]
+    L.ttc_timing.argtypes = [vp, C.c_int, dp]
+    L.ttc_list_stats.argtypes = [vp, C.POINTER(C.c_int64), ip, ip]
+    _lib = L
+    return L
+
+
+def check(code: int):
+    if code != TTC_OK:
+        raise TtcError(code, lib().ttc_last_error().decode())
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_int))

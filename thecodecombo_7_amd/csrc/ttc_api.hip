@@ -1,0 +1,631 @@
+// ttc_api.hip -- the C ABI of libttc.so (include/ttc.h): allocation,
+// topology upload, host <-> device copies, the reference-named operations
+// and the graph-captured velocity-Verlet loop.
+#include "ttc_common.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+
+static thread_local std::string g_err;
+
+void ttc_set_error(const std::string &msg) { g_err = msg; }
+extern "C" const char *ttc_last_error(void) { return g_err.c_str(); }
+
+// ewald.c:817-835 (Numerical Recipes erfcc) in FP64 for the host-side
+// constant erfc(kappa rc)/rc.
+double ttc_nr_erfc_host(double x) {
+    double z = fabs(x);
+    double t = 1.0 / (1.0 + 0.5 * z);
+    double v = t * exp(-z * z + -1.26551223 +
+               t * (1.00002368 + t * (0.37409196 + t * (0.09678418 +
+               t * (-0.18628806 + t * (0.27886807 + t * (-1.13520398 +
+               t * (1.48851587 + t * (-0.82215223 + t * 0.17087277)))))))));
+    return x < 0 ? 2.0 - v : v;
+}
+
+#define CHECK(cond, code, msg)            \
+    do {                                  \
+        if (!(cond)) {                    \
+            ttc_set_error(msg);           \
+            return code;                  \
+        }                                 \
+    } while (0)
+
+template <typename T>
+static int dalloc(T **p, size_t n) {
+    if (hipMalloc((void **)p, sizeof(T) * (n ? n : 1)) != hipSuccess) {
+        ttc_set_error("hipMalloc failed");
+        return TTC_ERR_ALLOC;
+    }
+    return TTC_OK;
+}
+
+#define ALLOC(p, n)                      \
+    do {                                 \
+        int r_ = dalloc(&(p), (n));      \
+        if (r_) return r_;               \
+    } while (0)
+
+static int list_capacity(const ttc_ctx *c, double cutoff) {
+    double vol = c->p.lx * c->p.ly * c->p.lz;
+    double rho = c->n / vol;
+    double expect = 4.0 / 3.0 * TTC_PI * cutoff * cutoff * cutoff * rho;
+    int cap = (int)(expect * 1.25) + 64;
+    cap = (cap + 63) & ~63;
+    return std::min(cap, std::max(64, ((c->n + 63) / 64) * 64));
+}
+
+extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
+    CHECK(p && out, TTC_ERR_ARG, "ttc_create: null argument");
+    CHECK(p->n > 0 && p->n < (1 << 30), TTC_ERR_ARG, "ttc_create: bad site count");
+    CHECK(p->lx > 0 && p->ly > 0 && p->lz > 0, TTC_ERR_ARG, "ttc_create: bad box");
+    CHECK(p->rc > 0 && p->rclist >= p->rc, TTC_ERR_ARG, "ttc_create: need rclist >= rc > 0");
+    CHECK(p->order == 4 || p->order == 6, TTC_ERR_ARG, "ttc_create: spline order must be 4 or 6");
+    CHECK(p->gx >= p->order && p->gy >= p->order && p->gz >= p->order, TTC_ERR_ARG,
+          "ttc_create: PME grid smaller than the spline order");
+    CHECK(p->gx <= 1024 && p->gy <= 1024 && p->gz <= 1024, TTC_ERR_ARG, "ttc_create: PME grid too large");
+    CHECK(p->ntypes > 0 && p->ntypes <= TTC_MAX_TYPES, TTC_ERR_ARG, "ttc_create: 1..64 site types supported");
+    double lmin = std::min(p->lx, std::min(p->ly, p->lz));
+    CHECK(2.0 * p->rc <= lmin, TTC_ERR_ARG, "ttc_create: cutoff exceeds half the box");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        ttc_set_error("ttc_create: no HIP device");
+        return TTC_ERR_HIP;
+    }
+    ttc_ctx *c = new ttc_ctx();
+    c->p = *p;
+    c->n = p->n;
+    c->G = (p->n + 63) / 64;
+    c->ntypes = p->ntypes;
+    c->order = p->order;
+    TTC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    int n = c->n;
+    ALLOC(c->pos4, n); ALLOC(c->vel4, n); ALLOC(c->f4, n); ALLOC(c->anchor, n);
+    ALLOC(c->pos4b, n); ALLOC(c->vel4b, n); ALLOC(c->f4b, n);
+    ALLOC(c->type, n); ALLOC(c->typeb, n); ALLOC(c->orig, n); ALLOC(c->origb, n); ALLOC(c->inv, n);
+    ALLOC(c->fpair, n); ALLOC(c->g4, n);
+    TTC_HIP(hipMemset(c->pos4, 0, sizeof(double4) * n));
+    TTC_HIP(hipMemset(c->f4, 0, sizeof(double4) * n));
+    TTC_HIP(hipMemset(c->fpair, 0, sizeof(double4) * n));
+    TTC_HIP(hipMemset(c->g4, 0, sizeof(double4) * n));
+    std::vector<int> ident(n);
+    for (int i = 0; i < n; i++) ident[i] = i;
+    TTC_HIP(hipMemcpy(c->orig, ident.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    TTC_HIP(hipMemcpy(c->inv, ident.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    std::vector<double4> v0(n, make_double4(0, 0, 0, 100.0));
+    TTC_HIP(hipMemcpy(c->vel4, v0.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+
+    // cells at least (widest list cutoff)/2 wide
+    double rmax = p->ewald_lists ? std::max(p->rclist, p->rclist_ew) : p->rclist;
+    c->ncx = std::max(1, (int)(p->lx / (0.5 * rmax)));
+    c->ncy = std::max(1, (int)(p->ly / (0.5 * rmax)));
+    c->ncz = std::max(1, (int)(p->lz / (0.5 * rmax)));
+    c->ncell = c->ncx * c->ncy * c->ncz;
+    ALLOC(c->cell_of, n); ALLOC(c->cell_cnt, c->ncell); ALLOC(c->cell_start, c->ncell);
+    ALLOC(c->cell_acc, c->ncell);
+    TTC_HIP(hipMemset(c->cell_acc, 0, sizeof(int) * c->ncell));
+    ALLOC(c->cell_members, (size_t)c->ncell * ttc_cellcap(c));
+
+    c->cap = list_capacity(c, ttc_list_cutoff(c));
+    ALLOC(c->nbl, (size_t)c->G * c->cap * 64);
+    ALLOC(c->glen, c->G);
+    TTC_HIP(hipMemset(c->glen, 0, sizeof(int) * c->G));
+    ALLOC(c->status, ST_NSTAT);
+    TTC_HIP(hipMemset(c->status, 0, sizeof(int) * ST_NSTAT));
+    TTC_HIP(hipHostMalloc((void **)&c->h_status, sizeof(int) * ST_NSTAT, hipHostMallocDefault));
+    memset(c->h_status, 0, sizeof(int) * ST_NSTAT);
+
+    int r = ttc_pme_setup(c);
+    if (r) return r;
+    c->nblk_max = std::max(std::max(ttc_pair_blocks(c), ttc_pme_conv_blocks(c)),
+                           std::max(ttc_gather_blocks(c), ttc_kick_blocks(c)));
+    ALLOC(c->partials, (size_t)EV_N * c->nblk_max);
+    TTC_HIP(hipMemset(c->partials, 0, sizeof(double) * EV_N * c->nblk_max));
+    ALLOC(c->ev, EV_N + 4);
+    double chk[4] = {p->lx, p->ly, p->lz, (p->rclist - p->rc) / 4.0};
+    TTC_HIP(hipMemcpy(c->ev + EV_N, chk, sizeof(chk), hipMemcpyHostToDevice));
+    TTC_HIP(hipHostMalloc((void **)&c->h_ev, sizeof(double) * (EV_N + 4), hipHostMallocDefault));
+    *out = c;
+    return TTC_OK;
+}
+
+static void destroy_graphs(ttc_ctx *c) {
+    if (c->g_plain) hipGraphExecDestroy(c->g_plain);
+    if (c->g_energy) hipGraphExecDestroy(c->g_energy);
+    c->g_plain = c->g_energy = nullptr;
+}
+
+extern "C" int ttc_destroy(ttc_ctx *c) {
+    if (!c) return TTC_OK;
+    hipStreamSynchronize(c->stream);
+    destroy_graphs(c);
+    void *bufs[] = {c->pos4, c->vel4, c->f4, c->anchor, c->pos4b, c->vel4b, c->f4b, c->type, c->typeb, c->orig,
+                    c->origb, c->inv, c->fpair, c->g4, c->cell_of, c->cell_cnt, c->cell_start, c->cell_acc,
+                    c->cell_members, c->nbl, c->glen, c->status, c->ex_off, c->ex_part, c->ex_flag, c->ljtab,
+                    c->b_off, c->b_part, c->b_r0, c->b_k, c->partials, c->ev, c->rows};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    ttc_pme_free(c);
+    if (c->h_status) hipHostFree(c->h_status);
+    if (c->h_ev) hipHostFree(c->h_ev);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return TTC_OK;
+}
+
+extern "C" double ttc_self_energy(const ttc_ctx *c) { return c ? c->e_self : 0.0; }
+
+// device sorted order -> host original order helpers
+static int fetch_orig(ttc_ctx *c, std::vector<int> &orig) {
+    orig.resize(c->n);
+    TTC_HIP(hipMemcpyAsync(orig.data(), c->orig, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    return TTC_OK;
+}
+
+extern "C" int ttc_set_topology(ttc_ctx *c, const int *atomid, const double *qq, const double *mass,
+                                const double *lj_eps, const double *lj_sig, const double *lj14_eps,
+                                const double *lj14_sig, int nx, const int *xa, const int *xb, const int *xflag,
+                                int nbond, const int *ba, const int *bb, const double *req, const double *kbond) {
+    CHECK(c && atomid && qq && mass && lj_eps && lj_sig && lj14_eps && lj14_sig, TTC_ERR_ARG,
+          "ttc_set_topology: null argument");
+    int n = c->n, nt = c->ntypes;
+    for (int i = 0; i < n; i++) {
+        CHECK(atomid[i] >= 0 && atomid[i] < nt, TTC_ERR_ARG, "ttc_set_topology: atomid out of range");
+        CHECK(mass[i] > 0, TTC_ERR_ARG, "ttc_set_topology: non-positive mass");
+    }
+    // ewald.c:184-194 neutrality, ewald.c:171-178 self energy
+    double charge = 0.0, q2 = 0.0;
+    for (int i = 0; i < n; i++) { charge += qq[i]; q2 += qq[i] * qq[i]; }
+    if (fabs(charge) > 0.000005) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "The charge of the system is %f: Ewald summation needs a neutral system", charge);
+        ttc_set_error(buf);
+        return TTC_ERR_NOT_NEUTRAL;
+    }
+    double fconv = (8.987551789e19 * 1.602176462e-19 * 1.602176462e-19 * 6.022045E23 * 0.001) / c->p.eps_rf;
+    c->e_self = q2 * (c->p.kappa / sqrt(TTC_PI)) * fconv;
+
+    std::vector<int> orig;
+    int r = fetch_orig(c, orig);
+    if (r) return r;
+    std::vector<int> typ(n);
+    std::vector<double4> pos(n), vel(n);
+    TTC_HIP(hipMemcpy(pos.data(), c->pos4, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    TTC_HIP(hipMemcpy(vel.data(), c->vel4, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    for (int s = 0; s < n; s++) {
+        int o = orig[s];
+        typ[s] = atomid[o];
+        pos[s].w = qq[o];
+        vel[s].w = 100.0 / mass[o];
+    }
+    TTC_HIP(hipMemcpy(c->type, typ.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    TTC_HIP(hipMemcpy(c->pos4, pos.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    TTC_HIP(hipMemcpy(c->vel4, vel.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+
+    // LJ tables
+    std::vector<float4> lj((size_t)nt * nt);
+    for (int a = 0; a < nt * nt; a++)
+        lj[a] = make_float4((float)lj_eps[a], (float)(lj_sig[a] * lj_sig[a]), (float)lj14_eps[a],
+                            (float)(lj14_sig[a] * lj14_sig[a]));
+    if (c->ljtab) hipFree(c->ljtab);
+    ALLOC(c->ljtab, (size_t)nt * nt);
+    TTC_HIP(hipMemcpy(c->ljtab, lj.data(), sizeof(float4) * nt * nt, hipMemcpyHostToDevice));
+
+    // exclusions: per-site rows, flag of the last xint entry naming the pair
+    std::vector<std::map<int, int>> ex(n);
+    int span = 0;
+    for (int i = 0; i < nx; i++) {
+        int a = xa[i], b = xb[i];
+        CHECK(a >= 0 && a < n && b >= 0 && b < n && a != b, TTC_ERR_ARG, "ttc_set_topology: bad exclusion");
+        ex[a][b] = xflag[i];
+        ex[b][a] = xflag[i];
+        span = std::max(span, std::abs(a - b));
+    }
+    std::vector<int> off(n + 1, 0), part, flg;
+    for (int a = 0; a < n; a++) {
+        off[a + 1] = off[a] + (int)ex[a].size();
+        for (auto &kv : ex[a]) { part.push_back(kv.first); flg.push_back(kv.second); }
+    }
+    c->ex_span = span;
+    c->nx = nx;
+    if (c->ex_off) { hipFree(c->ex_off); hipFree(c->ex_part); hipFree(c->ex_flag); }
+    ALLOC(c->ex_off, n + 1); ALLOC(c->ex_part, part.size()); ALLOC(c->ex_flag, flg.size());
+    TTC_HIP(hipMemcpy(c->ex_off, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    if (!part.empty()) {
+        TTC_HIP(hipMemcpy(c->ex_part, part.data(), sizeof(int) * part.size(), hipMemcpyHostToDevice));
+        TTC_HIP(hipMemcpy(c->ex_flag, flg.data(), sizeof(int) * flg.size(), hipMemcpyHostToDevice));
+    }
+
+    // bonds, both directions
+    std::vector<std::vector<int>> bl(n);
+    for (int i = 0; i < nbond; i++) {
+        CHECK(ba[i] >= 0 && ba[i] < n && bb[i] >= 0 && bb[i] < n && ba[i] != bb[i], TTC_ERR_ARG,
+              "ttc_set_topology: bad bond");
+        bl[ba[i]].push_back(i);
+        bl[bb[i]].push_back(i);
+    }
+    std::vector<int> boff(n + 1, 0), bpart;
+    std::vector<double> br0, bk;
+    for (int a = 0; a < n; a++) {
+        boff[a + 1] = boff[a] + (int)bl[a].size();
+        for (int i : bl[a]) {
+            bpart.push_back(ba[i] == a ? bb[i] : ba[i]);
+            br0.push_back(req[i]);
+            bk.push_back(kbond[i]);
+        }
+    }
+    c->nbond = nbond;
+    if (c->b_off) { hipFree(c->b_off); hipFree(c->b_part); hipFree(c->b_r0); hipFree(c->b_k); }
+    ALLOC(c->b_off, n + 1); ALLOC(c->b_part, bpart.size()); ALLOC(c->b_r0, br0.size()); ALLOC(c->b_k, bk.size());
+    TTC_HIP(hipMemcpy(c->b_off, boff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    if (!bpart.empty()) {
+        TTC_HIP(hipMemcpy(c->b_part, bpart.data(), sizeof(int) * bpart.size(), hipMemcpyHostToDevice));
+        TTC_HIP(hipMemcpy(c->b_r0, br0.data(), sizeof(double) * br0.size(), hipMemcpyHostToDevice));
+        TTC_HIP(hipMemcpy(c->b_k, bk.data(), sizeof(double) * bk.size(), hipMemcpyHostToDevice));
+    }
+    c->topo = true;
+    c->f4_valid = false;
+    destroy_graphs(c);
+    return TTC_OK;
+}
+
+static int put_xyz(ttc_ctx *c, const double *x, int64_t stride, bool velocity) {
+    CHECK(c && x && stride >= 3, TTC_ERR_ARG, "ttc_set_*: bad argument");
+    std::vector<int> orig;
+    int r = fetch_orig(c, orig);
+    if (r) return r;
+    int n = c->n;
+    double4 *dst = velocity ? c->vel4 : c->pos4;
+    std::vector<double4> h(n);
+    TTC_HIP(hipMemcpy(h.data(), dst, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    for (int s = 0; s < n; s++) {
+        const double *src = x + (size_t)orig[s] * stride;
+        h[s].x = src[0]; h[s].y = src[1]; h[s].z = src[2];
+    }
+    TTC_HIP(hipMemcpy(dst, h.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    if (!velocity) c->f4_valid = false;
+    return TTC_OK;
+}
+
+static int get_xyz(ttc_ctx *c, double *x, int64_t stride, const double4 *src, bool add) {
+    CHECK(c && x && stride >= 3, TTC_ERR_ARG, "ttc_get_*: bad argument");
+    std::vector<int> orig;
+    int r = fetch_orig(c, orig);
+    if (r) return r;
+    int n = c->n;
+    std::vector<double4> h(n);
+    TTC_HIP(hipMemcpyAsync(h.data(), src, sizeof(double4) * n, hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < n; s++) {
+        double *d = x + (size_t)orig[s] * stride;
+        if (add) { d[0] += h[s].x; d[1] += h[s].y; d[2] += h[s].z; }
+        else { d[0] = h[s].x; d[1] = h[s].y; d[2] = h[s].z; }
+    }
+    return TTC_OK;
+}
+
+extern "C" int ttc_set_positions(ttc_ctx *c, const double *x, int64_t stride) { return put_xyz(c, x, stride, false); }
+extern "C" int ttc_get_positions(ttc_ctx *c, double *x, int64_t stride) { return get_xyz(c, x, stride, c->pos4, false); }
+extern "C" int ttc_set_velocities(ttc_ctx *c, const double *v, int64_t stride) { return put_xyz(c, v, stride, true); }
+extern "C" int ttc_get_velocities(ttc_ctx *c, double *v, int64_t stride) { return get_xyz(c, v, stride, c->vel4, false); }
+
+static int read_status(ttc_ctx *c) {
+    TTC_HIP(hipMemcpyAsync(c->h_status, c->status, sizeof(int) * ST_NSTAT, hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    return TTC_OK;
+}
+
+static int grow_list(ttc_ctx *c, int need) {
+    hipFree(c->nbl);
+    c->cap = ((int)(need * 1.2) + 127) & ~63;
+    ALLOC(c->nbl, (size_t)c->G * c->cap * 64);
+    destroy_graphs(c);
+    return TTC_OK;
+}
+
+// nblist.c:32: unconditional rebuild; grows the device list if a row
+// did not fit (the reference would exit(5) instead)
+extern "C" int ttc_nblist(ttc_ctx *c) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "ttc_nblist: topology not set");
+    for (int attempt = 0; attempt < 3; attempt++) {
+        TTC_HIP(hipMemsetAsync(c->status, 0, sizeof(int) * 4, c->stream));
+        int r = ttc_launch_rebuild(c, false);
+        if (r) return r;
+        TTC_HIP(hipGetLastError());
+        r = read_status(c);
+        if (r) return r;
+        int ov = c->h_status[ST_OVERFLOW];
+        if (ov & 2) {
+            ttc_set_error("ttc_nblist: a neighbour cell holds more sites than its capacity");
+            return TTC_ERR_LIST_OVERFLOW;
+        }
+        if (ov & 1) {
+            r = grow_list(c, c->h_status[ST_MAXROW]);
+            if (r) return r;
+            continue;
+        }
+        c->have_list = true;
+        c->f4_valid = false;   // the sorted order changed
+        return TTC_OK;
+    }
+    ttc_set_error("ttc_nblist: list capacity could not be grown");
+    return TTC_ERR_LIST_OVERFLOW;
+}
+
+extern "C" int ttc_nl_check(ttc_ctx *c, int lb, int ub, int *nl_flag) {
+    CHECK(c && nl_flag && lb >= 0 && ub <= c->n && lb <= ub, TTC_ERR_ARG, "ttc_nl_check: bad argument");
+    CHECK(c->have_list, TTC_ERR_ARG, "ttc_nl_check: no list built");
+    return ttc_nl_check_range(c, lb, ub, nl_flag);
+}
+
+extern "C" int ttc_get_nblist(ttc_ctx *c, int which, int max_nb, int *count, int *list, int *xflag) {
+    CHECK(c && count && list && xflag && max_nb > 0, TTC_ERR_ARG, "ttc_get_nblist: bad argument");
+    double cutoff;
+    if (which == TTC_LIST_NLIST) cutoff = ttc_list_cutoff(c);
+    else if (which == TTC_LIST_NLIST_EW) {
+        CHECK(c->p.ewald_lists, TTC_ERR_ARG, "ttc_get_nblist: nlist_ew exists only with ewald_lists");
+        cutoff = c->p.rclist_ew;
+    } else {
+        ttc_set_error("ttc_get_nblist: which must be TTC_LIST_NLIST or TTC_LIST_NLIST_EW");
+        return TTC_ERR_ARG;
+    }
+    return ttc_export_list(c, cutoff, max_nb, count, list, xflag);
+}
+
+static int need_list(ttc_ctx *c) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "topology not set");
+    if (!c->have_list) return ttc_nblist(c);
+    return TTC_OK;
+}
+
+static int finish(ttc_ctx *c, double *ff, int64_t stride) {
+    TTC_HIP(hipGetLastError());
+    TTC_HIP(hipMemcpyAsync(c->h_ev, c->ev, sizeof(double) * EV_N, hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    if (ff) return get_xyz(c, ff, stride, c->f4, true);
+    return TTC_OK;
+}
+
+extern "C" int ttc_pair(ttc_ctx *c, double *ff, int64_t stride, double *e5, double *vir6) {
+    int r = need_list(c);
+    if (r) return r;
+    ttc_launch_pair(c, true, 0);
+    ttc_launch_reduce(c, false);
+    ttc_launch_combine(c, 1);
+    r = finish(c, ff, stride);
+    if (r) return r;
+    if (e5) for (int i = 0; i < 5; i++) e5[i] = c->h_ev[EV_LJ + i];
+    if (vir6) for (int i = 0; i < 6; i++) vir6[i] = c->h_ev[EV_PV0 + i];
+    c->f4_valid = false;
+    return TTC_OK;
+}
+
+extern "C" int ttc_pme_calc(ttc_ctx *c, double *ff, int64_t stride, double *energy3, double *vir6) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "ttc_pme_calc: topology not set");
+    ttc_launch_spread(c);
+    ttc_launch_pme(c, true, true, false);
+    ttc_launch_reduce(c, false);
+    ttc_launch_combine(c, 2 | 4);
+    int r = finish(c, ff, stride);
+    if (r) return r;
+    if (energy3) {
+        energy3[0] = c->h_ev[EV_REC];
+        energy3[1] = c->e_self;
+        energy3[2] = energy3[0] - energy3[1];
+    }
+    if (vir6) for (int i = 0; i < 6; i++) vir6[i] = c->h_ev[EV_KV0 + i];
+    c->f4_valid = false;
+    return TTC_OK;
+}
+
+static void slots8(const ttc_ctx *c, double *energy) {
+    const double *ev = c->h_ev;
+    double enk2 = ev[EV_REC] - c->e_self;
+    double enr0 = ev[EV_REAL], enr1 = ev[EV_REAL] - ev[EV_REALS], enr2 = ev[EV_INTRA];
+    energy[0] = ev[EV_LJ] + enr0 - enr2 + enk2;
+    energy[1] = energy[0] - ev[EV_LJS];
+    energy[2] = enr0 + enk2 - enr2;
+    energy[3] = enr0 - enr1;
+    energy[4] = ev[EV_REC];
+    energy[5] = c->e_self;
+    energy[6] = enr2;
+    energy[7] = enr0;
+}
+
+extern "C" int ttc_cnonbondnl(ttc_ctx *c, double *ff, int64_t stride, double *energy8, double *vir_nbond6,
+                              double *vir_kspace6) {
+    int r = need_list(c);
+    if (r) return r;
+    ttc_launch_pair(c, true, 0);
+    ttc_launch_spread(c);
+    ttc_launch_pme(c, true, true, false);
+    ttc_launch_reduce(c, false);
+    ttc_launch_combine(c, 1 | 2 | 4);
+    r = finish(c, ff, stride);
+    if (r) return r;
+    if (energy8) slots8(c, energy8);
+    if (vir_nbond6) for (int i = 0; i < 6; i++) vir_nbond6[i] = c->h_ev[EV_PV0 + i];
+    if (vir_kspace6) for (int i = 0; i < 6; i++) vir_kspace6[i] = c->h_ev[EV_KV0 + i];
+    c->f4_valid = false;
+    return TTC_OK;
+}
+
+extern "C" int ttc_cbond(ttc_ctx *c, double *ff, int64_t stride, double *ebond, double *vir6) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "ttc_cbond: topology not set");
+    ttc_launch_pme(c, true, false, true);
+    ttc_launch_reduce(c, false);
+    ttc_launch_combine(c, 2);
+    int r = finish(c, ff, stride);
+    if (r) return r;
+    if (ebond) *ebond = c->h_ev[EV_BOND];
+    if (vir6) for (int i = 0; i < 6; i++) vir6[i] = c->h_ev[EV_BV0 + i];
+    c->f4_valid = false;
+    return TTC_OK;
+}
+
+// one full force evaluation into f4 (+ energies in ev)
+static void launch_forces(ttc_ctx *c, bool energy) {
+    ttc_launch_pair(c, energy, 0);
+    ttc_launch_spread(c);
+    ttc_launch_pme(c, energy, true, true);
+    ttc_launch_combine(c, 7);
+    if (energy) ttc_launch_reduce(c, false);
+}
+
+extern "C" int ttc_forces(ttc_ctx *c, double *ff, int64_t stride, double *en9) {
+    int r = need_list(c);
+    if (r) return r;
+    launch_forces(c, true);
+    if (ff) {
+        // forces.c:126-130 zeroes ff first
+        for (int i = 0; i < c->n; i++) { double *d = ff + (size_t)i * stride; d[0] = d[1] = d[2] = 0.0; }
+    }
+    r = finish(c, ff, stride);
+    if (r) return r;
+    c->f4_valid = true;
+    if (en9) {
+        double e8[8];
+        slots8(c, e8);
+        en9[0] = c->h_ev[EV_BOND];
+        en9[1] = e8[0]; en9[2] = e8[1]; en9[3] = e8[2]; en9[4] = e8[7];
+        en9[5] = e8[3]; en9[6] = e8[4]; en9[7] = e8[5]; en9[8] = e8[6];
+    }
+    return TTC_OK;
+}
+
+extern "C" int ttc_kinet(ttc_ctx *c, double *kinet, double *ken6) {
+    CHECK(c, TTC_ERR_ARG, "ttc_kinet: null context");
+    ttc_launch_kinet(c);
+    ttc_launch_reduce(c, false);
+    int r = finish(c, nullptr, 0);
+    if (r) return r;
+    if (kinet) *kinet = c->h_ev[EV_KE0] + c->h_ev[EV_KE2] + c->h_ev[EV_KE5];
+    if (ken6) for (int i = 0; i < 6; i++) ken6[i] = c->h_ev[EV_KE0 + i];
+    return TTC_OK;
+}
+
+// One velocity-Verlet step as launched (and captured): conditional rebuild,
+// kick + drift + spread, pair, PME (+ bonds), kick + trigger.
+static void launch_step(ttc_ctx *c, double vscal, bool energy) {
+    ttc_launch_rebuild(c, true);
+    ttc_launch_drift(c, vscal);
+    ttc_launch_pair(c, energy, 0);
+    ttc_launch_pme(c, energy, true, true);
+    ttc_launch_kick(c, vscal, energy);
+    if (energy) ttc_launch_reduce(c, true);
+}
+
+static int capture(ttc_ctx *c, double vscal, bool energy, hipGraphExec_t *exec) {
+    hipGraph_t g;
+    TTC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    launch_step(c, vscal, energy);
+    TTC_HIP(hipStreamEndCapture(c->stream, &g));
+    TTC_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    hipGraphDestroy(g);
+    return TTC_OK;
+}
+
+extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_every, double *energies,
+                              int max_rows, int *nrows, int *rebuilds) {
+    CHECK(c && c->topo && nsteps >= 0, TTC_ERR_ARG, "ttc_md_vverlet: bad argument");
+    CHECK(energy_every == 0 || (energies && max_rows > 0), TTC_ERR_ARG, "ttc_md_vverlet: no row buffer");
+    int r = need_list(c);
+    if (r) return r;
+    if (energy_every > 0 && max_rows > c->rows_cap) {
+        if (c->rows) hipFree(c->rows);
+        ALLOC(c->rows, (size_t)max_rows * ROW_LEN);
+        c->rows_cap = max_rows;
+        destroy_graphs(c);
+    }
+    if (!c->f4_valid) {
+        // main.c:489-493 / nve_md.c:301-303: forces before the first step
+        launch_forces(c, false);
+        c->f4_valid = true;
+    }
+    if (!c->g_plain || c->g_vscal != vscal) {
+        destroy_graphs(c);
+        r = capture(c, vscal, false, &c->g_plain);
+        if (r) return r;
+        r = capture(c, vscal, true, &c->g_energy);
+        if (r) return r;
+        c->g_vscal = vscal;
+    }
+    // counters; the trigger for step 0 is checked on the current positions
+    TTC_HIP(hipMemsetAsync(c->status, 0, sizeof(int) * ST_NSTAT, c->stream));
+    ttc_launch_nlcheck(c);
+    hipEvent_t e0, e1;
+    TTC_HIP(hipEventCreate(&e0));
+    TTC_HIP(hipEventCreate(&e1));
+    TTC_HIP(hipEventRecord(e0, c->stream));
+    for (int step = 1; step <= nsteps; step++) {
+        bool en = energy_every > 0 && step % energy_every == 0;
+        TTC_HIP(hipGraphLaunch(en ? c->g_energy : c->g_plain, c->stream));
+    }
+    TTC_HIP(hipEventRecord(e1, c->stream));
+    r = read_status(c);
+    if (r) return r;
+    float ms = 0.0f;
+    TTC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->t_step = nsteps > 0 ? 1000.0 * ms / nsteps : 0.0;
+    int nr = std::min(c->h_status[ST_NROWS], max_rows);
+    if (nrows) *nrows = energy_every > 0 ? nr : 0;
+    if (rebuilds) *rebuilds = c->h_status[ST_NREBUILD];
+    if (energy_every > 0 && nr > 0)
+        TTC_HIP(hipMemcpy(energies, c->rows, sizeof(double) * ROW_LEN * nr, hipMemcpyDeviceToHost));
+    if (c->h_status[ST_OVERFLOW]) {
+        ttc_set_error("ttc_md_vverlet: neighbour list overflow during the run (nblist.c:185-196)");
+        return TTC_ERR_LIST_OVERFLOW;
+    }
+    return TTC_OK;
+}
+
+extern "C" int ttc_timing(ttc_ctx *c, int which, double *usec) {
+    CHECK(c && usec, TTC_ERR_ARG, "ttc_timing: bad argument");
+    if (which == 2) { *usec = c->t_step; return TTC_OK; }
+    CHECK(which == 0 || which == 1, TTC_ERR_ARG, "ttc_timing: which must be 0, 1 or 2");
+    int r = need_list(c);
+    if (r) return r;
+    const int reps = 20;
+    hipEvent_t e0, e1;
+    TTC_HIP(hipEventCreate(&e0));
+    TTC_HIP(hipEventCreate(&e1));
+    // warm-up launch, then timed back-to-back launches on the context stream
+    if (which == 0) ttc_launch_pair(c, false, 0); else ttc_launch_spread(c);
+    TTC_HIP(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; i++) {
+        if (which == 0) ttc_launch_pair(c, false, 0); else ttc_launch_spread(c);
+    }
+    TTC_HIP(hipEventRecord(e1, c->stream));
+    TTC_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    TTC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (which == 1) {
+        // the repeated spreads accumulated: clear the grid again
+        TTC_HIP(hipMemsetAsync(c->qfix, 0, sizeof(long long) * c->ngrid, c->stream));
+        TTC_HIP(hipStreamSynchronize(c->stream));
+    }
+    *usec = 1000.0 * ms / reps;
+    if (which == 0) c->t_pair = *usec; else c->t_spread = *usec;
+    return TTC_OK;
+}
+
+extern "C" int ttc_list_stats(ttc_ctx *c, int64_t *entries, int *max_row, int *capacity) {
+    CHECK(c, TTC_ERR_ARG, "ttc_list_stats: null context");
+    std::vector<int> gl(c->G);
+    TTC_HIP(hipMemcpy(gl.data(), c->glen, sizeof(int) * c->G, hipMemcpyDeviceToHost));
+    int64_t tot = 0;
+    int mx = 0;
+    for (int g = 0; g < c->G; g++) { tot += (int64_t)gl[g] * 64; mx = std::max(mx, gl[g]); }
+    if (entries) *entries = tot;   // padded ELL slots streamed by the pair kernel
+    if (max_row) *max_row = mx;
+    if (capacity) *capacity = c->cap;
+    return TTC_OK;
+}

@@ -1,0 +1,200 @@
+// ttc_common.h -- device state and shared helpers of libttc.so (gfx950).
+//
+// Data layout in HBM (one context = one periodic box):
+//   Per-site state is kept in "sorted order": sites ordered by neighbour
+//   cell, and by original index inside a cell.  The order is rebuilt with
+//   every neighbour-list build and all per-site arrays are permuted with it,
+//   so every per-step kernel reads and writes coalesced.
+//     pos4[s]  = {x, y, z, q}            FP64 (positions wrapped to [-L/2,L/2])
+//     vel4[s]  = {vx, vy, vz, 100/m}     FP64 (A/ps; 100/m turns kJ/mol/A into A/ps^2)
+//     f4[s]    = {fx, fy, fz, 0}         FP64 total force of the last evaluation
+//     type[s], orig[s] (original index), inv[orig] = s
+//     anchor[s] = position at the last list build (DLIST)
+//     fix[3][N] int64 fixed-point force accumulator (2^-32 kJ/mol/A units)
+//   Neighbour list: full list (each pair in both rows), ELL layout per group
+//   of 64 consecutive sorted sites: nbl[g][k][lane] (uint32), glen[g] rows.
+//   Entry = j (sorted index, 30 bits) | class << 30 (0 plain, 1 excluded
+//   xflag 1/2/3, 2 one-four xflag 4); 0xFFFFFFFF pads short rows.
+//   PME: charge grid as int64 fixed point (2^-40 e units) for deterministic
+//   atomic spreading, complex FP32 work grid, FP32 influence function.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ttc.h"
+
+#define TTC_WAVE 64
+#define TTC_PAD 0xFFFFFFFFu
+#define TTC_JMASK 0x3FFFFFFFu
+#define TTC_FIX_SCALE 4294967296.0          /* 2^32 */
+#define TTC_FIX_INV (1.0 / 4294967296.0)
+#define TTC_QFIX_SCALE 1099511627776.0      /* 2^40 */
+#define TTC_QFIX_INV (1.0 / 1099511627776.0)
+#define TTC_MAX_TYPES 64
+
+// defines.h:60-71, product in the same order as the reference macro
+#define TTC_PI 3.141592653589793238462643383
+#define TTC_KB 1.380662E-23
+#define TTC_NA 6.022045E23
+#define TTC_AU 1.6605655E-27
+#define TTC_QFACTOR (8.987551789e19 * 1.602176462e-19 * 1.602176462e-19 * 6.022045E23 * 1.0E-3)
+
+enum { ST_NLFLAG = 0, ST_OVERFLOW = 1, ST_MAXROW = 2, ST_GRIDERR = 3, ST_NSTAT = 8 };
+
+// energy/virial slots reduced on the device (FP64)
+enum {
+    EV_LJ = 0, EV_LJS, EV_REAL, EV_REALS, EV_INTRA,   // pair loop
+    EV_PV0, EV_PV1, EV_PV2, EV_PV3, EV_PV4, EV_PV5,   // pair virial
+    EV_REC, EV_KV0, EV_KV1, EV_KV2, EV_KV3, EV_KV4, EV_KV5,  // reciprocal
+    EV_BOND, EV_BV0, EV_BV1, EV_BV2, EV_BV3, EV_BV4, EV_BV5, // bonds
+    EV_KIN, EV_KE0, EV_KE1, EV_KE2, EV_KE3, EV_KE4, EV_KE5,  // kinetic tensor
+    EV_FSX, EV_FSY, EV_FSZ,                           // sum of PME forces
+    EV_N
+};
+
+struct FftPlan {
+    int n = 0;
+    int nrad = 0;
+    int rad[16];
+    float2 *tw = nullptr;   // e^{+2 pi i m / n}, m = 0..n-1
+};
+
+struct ttc_ctx {
+    ttc_params p{};
+    int n = 0, G = 0, cap = 0, ntypes = 0;
+    int order = 4;
+    double e_self = 0.0;
+    bool topo = false;
+    bool have_list = false;
+    hipStream_t stream = nullptr;
+
+    // sorted-order state
+    double4 *pos4 = nullptr, *vel4 = nullptr, *f4 = nullptr, *anchor = nullptr;
+    double4 *pos4b = nullptr, *vel4b = nullptr, *f4b = nullptr, *anchorb = nullptr;
+    int *type = nullptr, *typeb = nullptr, *orig = nullptr, *origb = nullptr, *inv = nullptr;
+    long long *fix = nullptr;
+
+    // cells
+    int ncx = 0, ncy = 0, ncz = 0, ncell = 0;
+    int *cell_of = nullptr, *cell_cnt = nullptr, *cell_start = nullptr, *cell_acc = nullptr;
+    int *cell_members = nullptr, *perm = nullptr;
+
+    // list
+    uint32_t *nbl = nullptr;
+    int *glen = nullptr;
+    int *status = nullptr;          // device ST_* flags
+    int *h_status = nullptr;        // pinned host mirror
+
+    // exclusions, original numbering, CSR both directions
+    int *ex_off = nullptr, *ex_part = nullptr, *ex_flag = nullptr;
+    int ex_span = 0;                // max |a-b| over exclusions
+    int nx = 0;
+
+    // LJ tables: {eps, sig^2, eps14, sig14^2} per type pair
+    float4 *ljtab = nullptr;
+
+    // bonds, per site in original numbering (CSR, both directions)
+    int nbond = 0;
+    int *b_off = nullptr, *b_part = nullptr;
+    double *b_r0 = nullptr, *b_k = nullptr;
+
+    // force contributions of one evaluation (sorted order)
+    double4 *fpair = nullptr;       // pair kernel
+    double4 *g4 = nullptr;          // PME gather + bonds
+    bool f4_valid = false;          // f4 holds the force at the current positions
+
+    // PME
+    int K[3] = {0, 0, 0};
+    size_t ngrid = 0;
+    long long *qfix = nullptr;
+    float2 *qc = nullptr;
+    float *qr = nullptr;
+    float *eterm = nullptr;
+    FftPlan plan[3];
+
+    // reductions: partials[slot * nblk_max + block]
+    double *partials = nullptr;
+    int nblk_max = 0;
+    double *ev = nullptr;           // [EV_N] reduced
+    double *h_ev = nullptr;         // pinned
+    double *rows = nullptr;         // energy rows of ttc_md_vverlet
+    int rows_cap = 0;
+
+    // graph
+    hipGraphExec_t g_plain = nullptr, g_energy = nullptr;
+    double g_vscal = 1.0;
+
+    // timing (microseconds)
+    double t_pair = 0, t_spread = 0, t_step = 0;
+};
+
+enum { ST_NREBUILD = 4, ST_STEP = 5, ST_NROWS = 6 };
+
+#define ROW_LEN 9
+
+// ---- launchers (defined in the .hip files) ----
+// nblist
+int ttc_launch_rebuild(ttc_ctx *c, bool conditional);   // cells, sort, permute, list
+void ttc_launch_nlcheck(ttc_ctx *c);
+int ttc_export_list(ttc_ctx *c, double cutoff, int max_nb, int *count, int *list, int *xflag);
+int ttc_nl_check_range(ttc_ctx *c, int lb, int ub, int *flag);
+double ttc_list_cutoff(const ttc_ctx *c);
+int ttc_cellcap(const ttc_ctx *c);
+// pair
+void ttc_launch_pair(ttc_ctx *c, bool energy, int blk_off);
+int ttc_pair_blocks(ttc_ctx *c);
+// pme (+ bonds in the gather)
+int ttc_pme_setup(ttc_ctx *c);
+void ttc_pme_free(ttc_ctx *c);
+// gather: pme -> PME forces, bonds -> bond forces (either may be off)
+void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, bool bonds);
+int ttc_pme_conv_blocks(ttc_ctx *c);
+int ttc_gather_blocks(ttc_ctx *c);
+void ttc_launch_spread(ttc_ctx *c);
+void ttc_launch_drift(ttc_ctx *c, double vscal);
+// md / combine / reductions
+// mask bits: 1 pair, 2 gather (PME + bonds), 4 remove mean PME force
+void ttc_launch_combine(ttc_ctx *c, int mask);
+void ttc_launch_kick(ttc_ctx *c, double vscal, bool energy);
+void ttc_launch_kinet(ttc_ctx *c);
+void ttc_launch_reduce(ttc_ctx *c, bool append_row);
+int ttc_kick_blocks(ttc_ctx *c);
+double ttc_nr_erfc_host(double x);
+
+void ttc_set_error(const std::string &msg);
+
+#define TTC_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            ttc_set_error(std::string(#call) + ": " + hipGetErrorString(e_));      \
+            return TTC_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+// ---- device helpers ----
+__device__ __forceinline__ long long ttc_to_fix(double v) {
+    return __double2ll_rn(v * TTC_FIX_SCALE);
+}
+
+// Minimum image exactly as cnonbond_nblist.c:166-173 (one-sided wrap).
+__device__ __forceinline__ double ttc_minimg(double d, double h, double l) {
+    if (d > h) d -= l;
+    else if (d < -h) d += l;
+    return d;
+}
+
+// FP64 sum of squares without contraction: (dx*dx + dy*dy) + dz*dz rounded
+// op by op, as the reference's C computes it.
+__device__ __forceinline__ double ttc_r2_exact(double dx, double dy, double dz) {
+    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+__device__ __forceinline__ double ttc_wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}

@@ -1,0 +1,373 @@
+// ttc_nblist.hip -- neighbour list on the device (replaces nblist.c).
+//
+// Rebuild = four kernels, each a no-op unless the displacement trigger is
+// set (so the whole sequence can sit in a captured HIP graph every step):
+//   R1 cell_assign  : cell of every site, counted into fixed-capacity cells
+//   R2 cell_scan    : exclusive scan of the counts (one workgroup)
+//   R3 reorder      : rank inside the cell by original index (deterministic
+//                     order without a sort), scatter the site state into the
+//                     B buffers at its new sorted position
+//   R5 build        : copy B -> A, anchors, inverse map, and the full Verlet
+//                     list of each site with the reference's FP64 test
+// Cells are at least rclist/2 wide, so a 5x5x5 stencil covers rclist.
+#include "ttc_common.h"
+
+#include <algorithm>
+
+namespace {
+
+struct CellGeom {
+    int ncx, ncy, ncz;
+    double lx, ly, lz, hx, hy, hz;
+};
+
+__device__ __forceinline__ int cell_coord(double x, double h, double l, int nc) {
+    int c = (int)((x + h) * (double)nc / l);
+    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__device__ __forceinline__ bool rebuild_off(const int *status, int conditional) {
+    return conditional && status[ST_NLFLAG] == 0;
+}
+
+__global__ void k_cell_assign(int n, const double4 *__restrict__ pos4, CellGeom g, int cellcap,
+                              int *cell_acc, int *cell_of, int *members, int *status, int conditional) {
+    if (rebuild_off(status, conditional)) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 p = pos4[i];
+    int cx = cell_coord(p.x, g.hx, g.lx, g.ncx);
+    int cy = cell_coord(p.y, g.hy, g.ly, g.ncy);
+    int cz = cell_coord(p.z, g.hz, g.lz, g.ncz);
+    int c = cx + g.ncx * (cy + g.ncy * cz);
+    int slot = atomicAdd(&cell_acc[c], 1);
+    cell_of[i] = c;
+    if (slot < cellcap) members[(size_t)c * cellcap + slot] = i;
+    else atomicOr(&status[ST_OVERFLOW], 2);
+}
+
+// one workgroup of 1024 threads; counts -> start, cell_acc reset to zero
+__global__ void k_cell_scan(int ncell, int *cell_acc, int *cell_cnt, int *cell_start, int *status,
+                            int conditional) {
+    if (rebuild_off(status, conditional)) return;
+    __shared__ int sh[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) {
+        carry = 0;
+        status[ST_NREBUILD] += 1;
+    }
+    __syncthreads();
+    for (int base = 0; base < ncell; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < ncell ? cell_acc[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < ncell) {
+            cell_start[i] = carry + sh[threadIdx.x] - v;
+            cell_cnt[i] = v;
+            cell_acc[i] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += sh[1023];
+        __syncthreads();
+    }
+}
+
+__global__ void k_reorder(int n, int cellcap, const int *cell_of, const int *cell_cnt,
+                          const int *cell_start, const int *members, const int *orig,
+                          const double4 *pos4, const double4 *vel4, const double4 *f4, const int *type,
+                          double4 *pos4b, double4 *vel4b, double4 *f4b, int *typeb, int *origb,
+                          int *status, int conditional) {
+    if (rebuild_off(status, conditional)) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = cell_of[i];
+    int m = min(cell_cnt[c], cellcap);
+    int o = orig[i];
+    int r = 0;
+    const int *mem = members + (size_t)c * cellcap;
+    for (int k = 0; k < m; k++) r += (orig[mem[k]] < o);
+    int s = cell_start[c] + r;
+    pos4b[s] = pos4[i];
+    vel4b[s] = vel4[i];
+    f4b[s] = f4[i];   // the last force feeds the next half kick
+    typeb[s] = type[i];
+    origb[s] = o;
+}
+
+// Full Verlet list, one lane per sorted site.  The pair test is the
+// reference's (nblist.c:116-145): |d| wrapped once, per-component reject,
+// dr2 < rcut^2 in IEEE double rounded op by op.  It is symmetric in the two
+// sites, so the full list holds every reference pair in both rows.
+__global__ void __launch_bounds__(256)
+k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__restrict__ pos4b,
+        const double4 *__restrict__ vel4b, const double4 *__restrict__ f4b, const int *__restrict__ typeb,
+        const int *__restrict__ origb, double4 *pos4, double4 *vel4, double4 *f4, int *type, int *orig,
+        double4 *anchor, int *inv,
+        const int *__restrict__ cell_cnt, const int *__restrict__ cell_start,
+        const int *__restrict__ ex_off, const int *__restrict__ ex_part, const int *__restrict__ ex_flag,
+        int ex_span, uint32_t *nbl, int *glen, int *status, int conditional, int copy_state) {
+    if (rebuild_off(status, conditional)) return;
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int lane = threadIdx.x & 63;
+    int grp = s >> 6;
+    int cnt = 0;
+    if (s < n) {
+        double4 pi = pos4b[s];
+        int oi = origb[s];
+        if (copy_state) {
+            pos4[s] = pi;
+            vel4[s] = vel4b[s];
+            f4[s] = f4b[s];
+            type[s] = typeb[s];
+            orig[s] = oi;
+            anchor[s] = pi;
+            inv[oi] = s;
+        }
+        int cx = cell_coord(pi.x, g.hx, g.lx, g.ncx);
+        int cy = cell_coord(pi.y, g.hy, g.ly, g.ncy);
+        int cz = cell_coord(pi.z, g.hz, g.lz, g.ncz);
+        int lox = g.ncx >= 5 ? -2 : 0, hix = g.ncx >= 5 ? 2 : g.ncx - 1;
+        int loy = g.ncy >= 5 ? -2 : 0, hiy = g.ncy >= 5 ? 2 : g.ncy - 1;
+        int loz = g.ncz >= 5 ? -2 : 0, hiz = g.ncz >= 5 ? 2 : g.ncz - 1;
+        int xb = ex_off[oi], xe = ex_off[oi + 1];
+        uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
+        for (int oz = loz; oz <= hiz; oz++) {
+            int z = g.ncz >= 5 ? (cz + oz + g.ncz) % g.ncz : oz;
+            for (int oy = loy; oy <= hiy; oy++) {
+                int y = g.ncy >= 5 ? (cy + oy + g.ncy) % g.ncy : oy;
+                for (int ox = lox; ox <= hix; ox++) {
+                    int x = g.ncx >= 5 ? (cx + ox + g.ncx) % g.ncx : ox;
+                    int c = x + g.ncx * (y + g.ncy * z);
+                    int j0 = cell_start[c], j1 = j0 + cell_cnt[c];
+                    for (int j = j0; j < j1; j++) {
+                        if (j == s) continue;
+                        double4 pj = pos4b[j];
+                        double dx = fabs(pi.x - pj.x);
+                        if (dx > g.hx) dx -= g.lx;
+                        if (dx > rcut) continue;
+                        double dy = fabs(pi.y - pj.y);
+                        if (dy > g.hy) dy -= g.ly;
+                        if (dy > rcut) continue;
+                        double dz = fabs(pi.z - pj.z);
+                        if (dz > g.hz) dz -= g.lz;
+                        if (dz > rcut) continue;
+                        double dr2 = ttc_r2_exact(dx, dy, dz);
+                        if (!(dr2 < rcut2)) continue;
+                        uint32_t cls = 0;
+                        int oj = origb[j];
+                        int d = oj - oi;
+                        if (d <= ex_span && -d <= ex_span) {
+                            for (int k = xb; k < xe; k++)
+                                if (ex_part[k] == oj) { int f = ex_flag[k]; cls = f == 4 ? 2u : (f ? 1u : 0u); }
+                        }
+                        if (cnt < cap) row[(size_t)cnt * 64] = (uint32_t)j | (cls << 30);
+                        cnt++;
+                    }
+                }
+            }
+        }
+    }
+    if (grp * 64 >= n) return;   // whole wave past the end (tail block)
+    // row length of the group = longest row; pad the shorter ones
+    int full = cnt;
+    for (int o = 32; o > 0; o >>= 1) full = max(full, __shfl_xor(full, o, 64));
+    int m = min(full, cap);
+    if (lane == 0) {
+        glen[grp] = m;
+        if (full > cap) atomicOr(&status[ST_OVERFLOW], 1);
+        atomicMax(&status[ST_MAXROW], full);
+    }
+    uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
+    for (int k = min(cnt, cap); k < m; k++) row[(size_t)k * 64] = TTC_PAD;
+}
+
+// nblist.c:607-647 on every site; sets the trigger flag.  Also the
+// device-side nl_check(lb,ub) of the C ABI (original numbering range).
+__global__ void k_nlcheck(int n, const double4 *pos4, const double4 *anchor, const int *orig, int lb,
+                          int ub, double lx, double ly, double lz, double rcut, int *status) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int o = orig[s];
+    if (o < lb || o >= ub) return;
+    double4 p = pos4[s], a = anchor[s];
+    double hx = 0.5 * lx, hy = 0.5 * ly, hz = 0.5 * lz;
+    double dx = fabs(p.x - a.x);
+    if (dx > hx) dx -= lx;
+    bool f = dx > rcut;
+    double dy = fabs(p.y - a.y);
+    if (dy > hy) dy -= ly;
+    f = f || dy > rcut;
+    double dz = fabs(p.z - a.z);
+    if (dz > hz) dz -= lz;
+    f = f || dz > rcut;
+    f = f || (ttc_r2_exact(dx, dy, dz) > rcut * rcut);
+    if (f) atomicOr(&status[ST_NLFLAG], 1);
+}
+
+// Reference-layout export: row orig[s] keeps entries with orig[j] > orig[s].
+__global__ void k_export(int n, int cap, const uint32_t *nbl, const int *glen, const int *orig,
+                         const int *ex_off, const int *ex_part, const int *ex_flag, int max_nb,
+                         int *count, int *list, int *xflag, int *status) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int lane = s & 63, grp = s >> 6;
+    int a = orig[s];
+    int len = glen[grp];
+    int c = 0;
+    const uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
+    for (int k = 0; k < len; k++) {
+        uint32_t e = row[(size_t)k * 64];
+        if (e == TTC_PAD) continue;
+        int b = orig[e & TTC_JMASK];
+        if (b <= a) continue;
+        int f = 0;
+        for (int q = ex_off[a]; q < ex_off[a + 1]; q++)
+            if (ex_part[q] == b) f = ex_flag[q];
+        if (c < max_nb) {
+            list[(size_t)a * max_nb + c] = b;
+            xflag[(size_t)a * max_nb + c] = f;
+        }
+        c++;
+    }
+    if (c > max_nb) atomicOr(&status[ST_OVERFLOW], 4);
+    count[a] = c;
+}
+
+CellGeom geom(const ttc_ctx *c) {
+    CellGeom g;
+    g.ncx = c->ncx; g.ncy = c->ncy; g.ncz = c->ncz;
+    g.lx = c->p.lx; g.ly = c->p.ly; g.lz = c->p.lz;
+    g.hx = 0.5 * c->p.lx; g.hy = 0.5 * c->p.ly; g.hz = 0.5 * c->p.lz;
+    return g;
+}
+
+}  // namespace
+
+double ttc_list_cutoff(const ttc_ctx *c) {
+    return c->p.ewald_lists ? std::min(c->p.rclist, c->p.rclist_ew) : c->p.rclist;
+}
+
+// capacity of one cell's member slots
+int ttc_cellcap(const ttc_ctx *c) {
+    double vcell = (c->p.lx / c->ncx) * (c->p.ly / c->ncy) * (c->p.lz / c->ncz);
+    double avg = c->n * vcell / (c->p.lx * c->p.ly * c->p.lz);
+    int cap = (int)(avg * 3.0) + 64;
+    return cap;
+}
+
+// The per-component early rejects use sim.rclist whichever list is built
+// (nblist.c:116-127 test against rcut = sim.rclist also while filling
+// nlist_ew), so a wider nlist_ew keeps only pairs with every |d| <= rclist.
+static void launch_build(ttc_ctx *c, bool conditional, double cutoff, uint32_t *nbl, int *glen,
+                         int cap, bool copy_state) {
+    CellGeom g = geom(c);
+    int blocks = (c->n + 255) / 256;
+    k_build<<<blocks, 256, 0, c->stream>>>(
+        c->n, cap, g, c->p.rclist, cutoff * cutoff, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4,
+        c->vel4, c->f4, c->type, c->orig, c->anchor, c->inv, c->cell_cnt, c->cell_start, c->ex_off, c->ex_part, c->ex_flag,
+        c->ex_span, nbl, glen, c->status, conditional ? 1 : 0, copy_state ? 1 : 0);
+}
+
+int ttc_launch_rebuild(ttc_ctx *c, bool conditional) {
+    CellGeom g = geom(c);
+    int cellcap = ttc_cellcap(c);
+    int blocks = (c->n + 255) / 256;
+    int cond = conditional ? 1 : 0;
+    k_cell_assign<<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, g, cellcap, c->cell_acc, c->cell_of,
+                                                 c->cell_members, c->status, cond);
+    k_cell_scan<<<1, 1024, 0, c->stream>>>(c->ncell, c->cell_acc, c->cell_cnt, c->cell_start, c->status,
+                                          cond);
+    k_reorder<<<blocks, 256, 0, c->stream>>>(c->n, cellcap, c->cell_of, c->cell_cnt, c->cell_start,
+                                             c->cell_members, c->orig, c->pos4, c->vel4, c->f4, c->type,
+                                             c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->status, cond);
+    launch_build(c, conditional, ttc_list_cutoff(c), c->nbl, c->glen, c->cap, true);
+    return TTC_OK;
+}
+
+void ttc_launch_nlcheck(ttc_ctx *c) {
+    int blocks = (c->n + 255) / 256;
+    k_nlcheck<<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->anchor, c->orig, 0, c->n, c->p.lx, c->p.ly,
+                                             c->p.lz, (c->p.rclist - c->p.rc) / 4.0, c->status);
+}
+
+int ttc_nl_check_range(ttc_ctx *c, int lb, int ub, int *flag) {
+    TTC_HIP(hipMemsetAsync(c->status + ST_NLFLAG, 0, sizeof(int), c->stream));
+    int blocks = (c->n + 255) / 256;
+    k_nlcheck<<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->anchor, c->orig, lb, ub, c->p.lx, c->p.ly,
+                                             c->p.lz, (c->p.rclist - c->p.rc) / 4.0, c->status);
+    TTC_HIP(hipMemcpyAsync(c->h_status, c->status, sizeof(int) * ST_NSTAT, hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    *flag = c->h_status[ST_NLFLAG];
+    return TTC_OK;
+}
+
+int ttc_export_list(ttc_ctx *c, double cutoff, int max_nb, int *count, int *list, int *xflag) {
+    if (!c->have_list) { ttc_set_error("ttc_get_nblist: no list built (call ttc_nblist first)"); return TTC_ERR_ARG; }
+    uint32_t *nbl = c->nbl;
+    int *glen = c->glen;
+    int cap = c->cap;
+    uint32_t *tmp_nbl = nullptr;
+    int *tmp_glen = nullptr;
+    if (cutoff != ttc_list_cutoff(c)) {
+        // Another cutoff (nlist_ew wider than nlist): build a scratch list at
+        // the anchor positions, i.e. the positions of the last ttc_nblist, for
+        // which the current sorted order and cell ranges were made.  Cells
+        // are sized for the wider cutoff at creation.
+        double ratio = cutoff / ttc_list_cutoff(c);
+        cap = ((int)(c->cap * ratio * ratio * ratio) + 127) & ~63;
+        TTC_HIP(hipMalloc(&tmp_nbl, (size_t)c->G * cap * 64 * sizeof(uint32_t)));
+        TTC_HIP(hipMalloc(&tmp_glen, (size_t)c->G * sizeof(int)));
+        TTC_HIP(hipMemcpyAsync(c->pos4b, c->anchor, sizeof(double4) * c->n, hipMemcpyDeviceToDevice, c->stream));
+        TTC_HIP(hipMemcpyAsync(c->origb, c->orig, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
+        launch_build(c, false, cutoff, tmp_nbl, tmp_glen, cap, false);
+        nbl = tmp_nbl;
+        glen = tmp_glen;
+    }
+    int *d_count = nullptr, *d_list = nullptr, *d_xf = nullptr;
+    size_t tot = (size_t)c->n * max_nb;
+    TTC_HIP(hipMalloc(&d_count, sizeof(int) * c->n));
+    TTC_HIP(hipMalloc(&d_list, sizeof(int) * tot));
+    TTC_HIP(hipMalloc(&d_xf, sizeof(int) * tot));
+    TTC_HIP(hipMemsetAsync(c->status + ST_OVERFLOW, 0, sizeof(int), c->stream));
+    int blocks = (c->n + 255) / 256;
+    k_export<<<blocks, 256, 0, c->stream>>>(c->n, cap, nbl, glen, c->orig, c->ex_off, c->ex_part, c->ex_flag,
+                                            max_nb, d_count, d_list, d_xf, c->status);
+    TTC_HIP(hipMemcpyAsync(c->h_status, c->status, sizeof(int) * ST_NSTAT, hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipMemcpyAsync(count, d_count, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipMemcpyAsync(list, d_list, sizeof(int) * tot, hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipMemcpyAsync(xflag, d_xf, sizeof(int) * tot, hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    hipFree(d_count); hipFree(d_list); hipFree(d_xf);
+    if (tmp_nbl) hipFree(tmp_nbl);
+    if (tmp_glen) hipFree(tmp_glen);
+    int overflow = c->h_status[ST_OVERFLOW];
+    c->h_status[ST_OVERFLOW] = 0;
+    if (overflow & 4) {
+        ttc_set_error("ttc_get_nblist: a row exceeds max_nb (nblist.c:185-196)");
+        return TTC_ERR_LIST_OVERFLOW;
+    }
+    if (overflow & 3) {
+        ttc_set_error("ttc_get_nblist: device list overflow");
+        return TTC_ERR_LIST_OVERFLOW;
+    }
+    // reference row order: ascending b (nblist.c:107, b = a+1 .. ns-1)
+    std::vector<std::pair<int, int>> tmp;
+    for (int a = 0; a < c->n; a++) {
+        int k = count[a];
+        tmp.resize(k);
+        for (int j = 0; j < k; j++) tmp[j] = {list[(size_t)a * max_nb + j], xflag[(size_t)a * max_nb + j]};
+        std::sort(tmp.begin(), tmp.end());
+        for (int j = 0; j < k; j++) {
+            list[(size_t)a * max_nb + j] = tmp[j].first;
+            xflag[(size_t)a * max_nb + j] = tmp[j].second;
+        }
+    }
+    return TTC_OK;
+}

@@ -1,0 +1,158 @@
+// ttc_pair.hip -- LJ + SPME real-space pair kernel (replaces the pair loop
+// of cnonbond_nblist.c:134-523, STYPE + S14 + SPME branch).
+//
+// One lane per sorted site, one wavefront per ELL group of 64 sites.  Each
+// step of the row loop reads one coalesced 256-byte slice of the list.  The
+// cutoff test and minimum image run in FP64 on FP64 positions (so exactly
+// the reference's pairs are evaluated); the LJ, erfc and exp arithmetic runs
+// in FP32; each site's force and the energies/virial accumulate in FP64.
+// The list is full (both directions), so no scatter to j and no atomics:
+// every site's force is a fixed-order sum, deterministic by construction.
+#include "ttc_common.h"
+
+namespace {
+
+struct PairConst {
+    double lx, ly, lz, hx, hy, hz, rc2;
+    double erfc_rc_over_rc;   // erfc(kappa rc)/rc with the reference's erfc, FP64
+    float rc2f, kappa, kpi2;  // kpi2 = 2 kappa / sqrt(pi)
+    float qfac;               // QFACTOR
+    float inv_epsrf;
+    float e14, one_m_e14;
+    int ntypes;
+};
+
+// Numerical Recipes erfcc (ewald.c:817-835) in FP32, argument z >= 0.
+// Returns erfc(z); *ez2 receives exp(-z^2).
+__device__ __forceinline__ float nr_erfc(float z, float *ez2) {
+    float t = __frcp_rn(1.0f + 0.5f * z);
+    float p = -1.26551223f + t * (1.00002368f + t * (0.37409196f + t * (0.09678418f +
+              t * (-0.18628806f + t * (0.27886807f + t * (-1.13520398f +
+              t * (1.48851587f + t * (-0.82215223f + t * 0.17087277f))))))));
+    float e = __expf(-z * z);
+    *ez2 = e;
+    return t * e * __expf(p);
+}
+
+template <bool ENERGY>
+__global__ void __launch_bounds__(256)
+k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__ glen,
+       const double4 *__restrict__ pos4, const int *__restrict__ type, const float4 *__restrict__ ljtab,
+       PairConst pc, double4 *__restrict__ fpair, double *__restrict__ partials, int maxb) {
+    extern __shared__ float4 slj[];
+    int nt2 = pc.ntypes * pc.ntypes;
+    for (int k = threadIdx.x; k < nt2; k += blockDim.x) slj[k] = ljtab[k];
+    __syncthreads();
+
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int lane = threadIdx.x & 63;
+    int grp = s >> 6;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double elj = 0.0, eljs = 0.0, er = 0.0, ers = 0.0, ei = 0.0;
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
+    if (grp * 64 < n) {
+        int len = glen[grp];
+        double4 pi = s < n ? pos4[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+        int ti = s < n ? type[s] : 0;
+        const float4 *ljrow = slj + ti * pc.ntypes;
+        float qi = (float)pi.w;
+        const uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
+        for (int k = 0; k < len; k++) {
+            uint32_t e = row[(size_t)k * 64];
+            if (e == TTC_PAD) continue;
+            int j = (int)(e & TTC_JMASK);
+            uint32_t cls = e >> 30;
+            double4 pj = pos4[j];
+            double dx = ttc_minimg(pi.x - pj.x, pc.hx, pc.lx);
+            double dy = ttc_minimg(pi.y - pj.y, pc.hy, pc.ly);
+            double dz = ttc_minimg(pi.z - pj.z, pc.hz, pc.lz);
+            double dr2 = ttc_r2_exact(dx, dy, dz);
+            if (dr2 > pc.rc2) continue;
+            float r2 = (float)dr2;
+            float rinv = __builtin_amdgcn_rsqf(r2);
+            float rinv2 = rinv * rinv;
+            float r = r2 * rinv;
+            float4 lj = ljrow[type[j]];
+            float eps = cls == 0 ? lj.x : (cls == 2 ? lj.z : 0.0f);
+            float sg2 = cls == 2 ? lj.w : lj.y;
+            float sr2 = sg2 * rinv2;
+            float sr6 = sr2 * sr2 * sr2;
+            float sr12 = sr6 * sr6;
+            float force = 12.0f * eps * rinv2 * (sr12 - sr6);
+            float qqab = qi * (float)pj.w * pc.qfac;
+            float qq = (cls == 0 ? qqab : (cls == 2 ? pc.e14 * qqab : 0.0f)) * pc.inv_epsrf;
+            float ez2;
+            float erfckr = nr_erfc(pc.kappa * r, &ez2);
+            float kpid = pc.kpi2 * r * ez2;
+            float rinv3 = rinv * rinv2;
+            force += qq * rinv3 * (erfckr + kpid);
+            float qi2 = cls == 2 ? pc.one_m_e14 * qqab : qqab;
+            if (cls) force -= qi2 * rinv3 * ((1.0f - erfckr) - kpid);
+            float fxf = force * (float)dx, fyf = force * (float)dy, fzf = force * (float)dz;
+            fx += (double)fxf;
+            fy += (double)fyf;
+            fz += (double)fzf;
+            if (ENERGY) {
+                float sr2s = sg2 / pc.rc2f;
+                float sr6s = sr2s * sr2s * sr2s;
+                elj += (double)(eps * (sr12 - 2.0f * sr6));
+                eljs += (double)(eps * (sr6s * sr6s - 2.0f * sr6s));
+                er += (double)(qq * erfckr * rinv);
+                ers += (double)qq * pc.erfc_rc_over_rc;
+                if (cls) ei += (double)(qi2 * (1.0f - erfckr) * rinv);
+                w0 += (double)(fxf * (float)dx);
+                w1 += (double)(fyf * (float)dx);
+                w2 += (double)(fyf * (float)dy);
+                w3 += (double)(fzf * (float)dx);
+                w4 += (double)(fzf * (float)dy);
+                w5 += (double)(fzf * (float)dz);
+            }
+        }
+        if (s < n) fpair[s] = make_double4(fx, fy, fz, 0.0);
+    }
+    if (ENERGY) {
+        // each pair was visited from both of its sites: halve
+        double v[11] = {elj, eljs, er, ers, ei, w0, w1, w2, w3, w4, w5};
+        __shared__ double red[4][11];
+        int wid = threadIdx.x >> 6;
+        for (int q = 0; q < 11; q++) {
+            double t = ttc_wave_sum(v[q]);
+            if (lane == 0) red[wid][q] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x < 11) {
+            double t = 0.5 * (red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
+            partials[(size_t)(EV_LJ + threadIdx.x) * maxb + blockIdx.x] = t;
+        }
+    }
+}
+
+}  // namespace
+
+extern double ttc_nr_erfc_host(double x);
+
+int ttc_pair_blocks(ttc_ctx *c) { return (c->n + 255) / 256; }
+
+void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
+    PairConst pc;
+    pc.lx = c->p.lx; pc.ly = c->p.ly; pc.lz = c->p.lz;
+    pc.hx = 0.5 * c->p.lx; pc.hy = 0.5 * c->p.ly; pc.hz = 0.5 * c->p.lz;
+    pc.rc2 = c->p.rc * c->p.rc;
+    pc.erfc_rc_over_rc = ttc_nr_erfc_host(c->p.kappa * c->p.rc) / c->p.rc;
+    pc.rc2f = (float)pc.rc2;
+    pc.kappa = (float)c->p.kappa;
+    pc.kpi2 = (float)(2.0 * c->p.kappa / sqrt(TTC_PI));
+    pc.qfac = (float)TTC_QFACTOR;
+    pc.inv_epsrf = (float)(1.0 / c->p.eps_rf);
+    pc.e14 = (float)c->p.e14fac;
+    pc.one_m_e14 = (float)(1.0 - c->p.e14fac);
+    pc.ntypes = c->ntypes;
+    int blocks = ttc_pair_blocks(c);
+    size_t shm = sizeof(float4) * c->ntypes * c->ntypes;
+    if (energy)
+        k_pair<true><<<blocks, 256, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, c->ljtab,
+                                                      pc, c->fpair, c->partials, c->nblk_max);
+    else
+        k_pair<false><<<blocks, 256, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, c->ljtab,
+                                                       pc, c->fpair, c->partials, c->nblk_max);
+}

@@ -1,0 +1,626 @@
+// ttc_pme.hip -- smooth PME on the device (replaces pme_qgrid.c,
+// pme_calc.c, pme_spline.c) plus the fused kick/drift that feeds it.
+//
+//   k_drift_spread  first half of integrate_vverlet.c:52-83 (kick, drift,
+//                   pbc_all.c:23-88) fused with charge_grid (pme_qgrid.c:
+//                   26-113): B-spline weights (pme_spline.c:10-49) in FP32,
+//                   q*Mx*My*Mz spread with int64 fixed-point atomics, so the
+//                   grid is the same whatever order the atomics land in.
+//   k_fft<MODE>     3-D complex FFT as three line passes in LDS (Stockham,
+//                   mixed radix 2/3/4 + direct DFT for other primes).  The
+//                   z pass reads the fixed-point grid and clears it for the
+//                   next step; the x pass does forward FFT, the influence
+//                   function of pme_calc.c:52-93 (energy + virial on demand)
+//                   and the inverse FFT without leaving LDS.
+//   k_gather        pme_force (pme_calc.c:122-265) + bonds (cbond.c:65-120)
+//                   per site; per-block sums of the PME force for the mean
+//                   force removal of pme_calc.c:268-277.
+#include "ttc_common.h"
+
+#include <math.h>
+
+#define FFT_LB 16
+#define FFT_THREADS 256
+
+namespace {
+
+struct Spread {
+    double lx, ly, lz;
+    int K0, K1, K2;
+};
+
+// pme_spline.c:10-49 in FP32: M[1..P] weights, D[1..P] derivatives.
+template <int P>
+__device__ __forceinline__ void bspline(float w, float *M, float *D) {
+    float a[P + 1];
+#pragma unroll
+    for (int i = 0; i <= P; i++) a[i] = 0.0f;
+    a[2] = w;
+    a[1] = 1.0f - w;
+#pragma unroll
+    for (int i = 3; i <= P - 1; i++) {
+        float div = 1.0f / (float)(i - 1);
+        a[i] = div * w * a[i - 1];
+#pragma unroll
+        for (int j = 1; j <= i - 2; j++)
+            a[i - j] = div * ((w + (float)j) * a[i - j - 1] + ((float)i - (float)j - w) * a[i - j]);
+        a[1] *= div * (1.0f - w);
+    }
+    D[1] = -a[1];
+#pragma unroll
+    for (int i = 2; i <= P; i++) D[i] = a[i - 1] - a[i];
+    float div = 1.0f / (float)(P - 1);
+    a[P] = div * w * a[P - 1];
+#pragma unroll
+    for (int i = 1; i <= P - 2; i++)
+        a[P - i] = div * ((w + (float)i) * a[P - i - 1] + ((float)P - (float)i - w) * a[P - i]);
+    a[1] *= div * (1.0f - w);
+#pragma unroll
+    for (int i = 1; i <= P; i++) M[i] = a[i];
+}
+
+// grid coordinate u = K (x/L + 0.5) (pme_qgrid.c:26-34); fl = (int)u.
+// Wrapped positions give 0 <= u <= K, where floor equals the reference's
+// truncation; the base index is reduced modulo K so that a site exactly at
+// +L/2 (u = K), or an unwrapped position, still addresses the grid.
+__device__ __forceinline__ void grid_coord(double x, double l, int k, int *fl, float *w) {
+    double u = (double)k * (x / l + 0.5);
+    double fu = floor(u);
+    int f = (int)fu;
+    f %= k;
+    if (f < 0) f += k;
+    *fl = f;
+    *w = (float)(u - fu);
+}
+
+__device__ __forceinline__ int wrapk(int g, int k) {
+    if (g < 0) g += k;
+    if (g >= k) g -= k;
+    return g;
+}
+
+template <int P>
+__device__ __forceinline__ void spread_site(double4 p, const Spread &sp, long long *qfix) {
+    int fx, fy, fz;
+    float wx, wy, wz;
+    grid_coord(p.x, sp.lx, sp.K0, &fx, &wx);
+    grid_coord(p.y, sp.ly, sp.K1, &fy, &wy);
+    grid_coord(p.z, sp.lz, sp.K2, &fz, &wz);
+    float Mx[P + 1], My[P + 1], Mz[P + 1], D[P + 1];
+    bspline<P>(wx, Mx, D);
+    bspline<P>(wy, My, D);
+    bspline<P>(wz, Mz, D);
+    float q = (float)p.w;
+#pragma unroll
+    for (int xi = 1; xi <= P; xi++) {
+        int gx = wrapk(fx - P + xi, sp.K0);
+        float qx = q * Mx[xi];
+#pragma unroll
+        for (int yi = 1; yi <= P; yi++) {
+            int gy = wrapk(fy - P + yi, sp.K1);
+            float qxy = qx * My[yi];
+            size_t rowb = ((size_t)gx * sp.K1 + gy) * sp.K2;
+#pragma unroll
+            for (int zi = 1; zi <= P; zi++) {
+                int gz = wrapk(fz - P + zi, sp.K2);
+                long long v = __double2ll_rn((double)(qxy * Mz[zi]) * TTC_QFIX_SCALE);
+                atomicAdd((unsigned long long *)&qfix[rowb + gz], (unsigned long long)v);
+            }
+        }
+    }
+}
+
+// pbc_all.c:23-88: shift by whole box lengths until inside [-h, h]
+__device__ __forceinline__ double pbc1(double x, double l) {
+    double h = 0.5 * l;
+    for (int it = 0; it < 64 && (x < -h || x > h); it++) {
+        double sh = 0.0;
+        if (x < -h) sh = +l;
+        if (x > h) sh = -l;
+        x += sh;
+    }
+    return x;
+}
+
+template <int P, bool DRIFT>
+__global__ void __launch_bounds__(256)
+k_drift_spread(int n, double4 *pos4, double4 *vel4, const double4 *f4, double dt, double vscal, Spread sp,
+               long long *qfix, int *status) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (DRIFT && s == 0) status[ST_NLFLAG] = 0;   // the rebuild before this kernel consumed it
+    if (s >= n) return;
+    double4 p = pos4[s];
+    if (DRIFT) {
+        double4 v = vel4[s];
+        double4 f = f4[s];
+        double dth = 0.5 * dt;
+        double im = v.w;   // 100/m
+        v.x = vscal * (v.x + dth * im * f.x);
+        v.y = vscal * (v.y + dth * im * f.y);
+        v.z = vscal * (v.z + dth * im * f.z);
+        p.x = pbc1(p.x + dt * v.x, sp.lx);
+        p.y = pbc1(p.y + dt * v.y, sp.ly);
+        p.z = pbc1(p.z + dt * v.z, sp.lz);
+        vel4[s] = v;
+        pos4[s] = p;
+    }
+    spread_site<P>(p, sp, qfix);
+}
+
+struct FftArgs {
+    int K0, K1, K2;
+    int n;          // line length
+    int nrad;
+    int rad[16];
+    const float2 *tw;
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 twid(const float2 *tw, int m, int n, int sign) {
+    float2 t = tw[m % n];
+    return sign > 0 ? t : make_float2(t.x, -t.y);
+}
+
+// Stockham passes over `nlines` lines held in LDS; element e of line l sits
+// at buf[l*ls + e*es].  Returns the buffer that holds the result.
+__device__ float2 *fft_lds(float2 *A, float2 *B, int nlines, int ls, int es, const FftArgs &f,
+                           const float2 *tw, int sign) {
+    int n = f.n;
+    int Ns = 1;
+    float2 *src = A, *dst = B;
+    for (int st = 0; st < f.nrad; st++) {
+        int R = f.rad[st];
+        int nR = n / R;
+        int tstride = n / (Ns * R);
+        for (int t = threadIdx.x; t < nlines * nR; t += blockDim.x) {
+            int line = t / nR, j = t - line * nR;
+            int k = j % Ns;
+            int tstep = k * tstride;
+            const float2 *s0 = src + line * ls;
+            float2 *d0 = dst + line * ls;
+            int idxD = (j / Ns) * Ns * R + k;
+            if (R == 2) {
+                float2 a = s0[j * es], b = cmul(s0[(j + nR) * es], twid(tw, tstep, n, sign));
+                d0[idxD * es] = make_float2(a.x + b.x, a.y + b.y);
+                d0[(idxD + Ns) * es] = make_float2(a.x - b.x, a.y - b.y);
+            } else if (R == 4) {
+                float2 a = s0[j * es];
+                float2 b = cmul(s0[(j + nR) * es], twid(tw, tstep, n, sign));
+                float2 c = cmul(s0[(j + 2 * nR) * es], twid(tw, 2 * tstep, n, sign));
+                float2 d = cmul(s0[(j + 3 * nR) * es], twid(tw, 3 * tstep, n, sign));
+                float2 apc = make_float2(a.x + c.x, a.y + c.y), amc = make_float2(a.x - c.x, a.y - c.y);
+                float2 bpd = make_float2(b.x + d.x, b.y + d.y), bmd = make_float2(b.x - d.x, b.y - d.y);
+                // (sign i) * bmd
+                float2 ib = sign > 0 ? make_float2(-bmd.y, bmd.x) : make_float2(bmd.y, -bmd.x);
+                d0[idxD * es] = make_float2(apc.x + bpd.x, apc.y + bpd.y);
+                d0[(idxD + Ns) * es] = make_float2(amc.x + ib.x, amc.y + ib.y);
+                d0[(idxD + 2 * Ns) * es] = make_float2(apc.x - bpd.x, apc.y - bpd.y);
+                d0[(idxD + 3 * Ns) * es] = make_float2(amc.x - ib.x, amc.y - ib.y);
+            } else if (R == 3) {
+                float2 a = s0[j * es];
+                float2 b = cmul(s0[(j + nR) * es], twid(tw, tstep, n, sign));
+                float2 c = cmul(s0[(j + 2 * nR) * es], twid(tw, 2 * tstep, n, sign));
+                const float h = 0.86602540378443864676f * (float)sign;
+                float2 bpc = make_float2(b.x + c.x, b.y + c.y), bmc = make_float2(b.x - c.x, b.y - c.y);
+                float2 m = make_float2(a.x - 0.5f * bpc.x, a.y - 0.5f * bpc.y);
+                d0[idxD * es] = make_float2(a.x + bpc.x, a.y + bpc.y);
+                d0[(idxD + Ns) * es] = make_float2(m.x - h * bmc.y, m.y + h * bmc.x);
+                d0[(idxD + 2 * Ns) * es] = make_float2(m.x + h * bmc.y, m.y - h * bmc.x);
+            } else {
+                // direct DFT of length R: X_q = sum_r src_r w^{r (tstep + q n/R)}
+                for (int q = 0; q < R; q++) {
+                    float2 acc = make_float2(0.0f, 0.0f);
+                    int base = tstep + q * nR;
+                    for (int r = 0; r < R; r++) {
+                        float2 v = cmul(s0[(j + r * nR) * es], twid(tw, (r * base) % n, n, sign));
+                        acc.x += v.x;
+                        acc.y += v.y;
+                    }
+                    d0[(idxD + q * Ns) * es] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        float2 *t = src;
+        src = dst;
+        dst = t;
+        Ns *= R;
+    }
+    return src;
+}
+
+// MODE 0: z lines, forward, input = fixed-point grid (cleared after reading)
+// MODE 1: y lines, forward (sign +1) / inverse (sign -1), in place
+// MODE 2: x lines, forward + influence function + inverse, in place
+// MODE 3: z lines, inverse, output = real part
+template <int MODE, bool ENERGY>
+__global__ void __launch_bounds__(FFT_THREADS)
+k_fft(FftArgs f, int sign, long long *qfix, float2 *qc, float *qr, const float *eterm, double fac,
+      double lx, double ly, double lz, double *partials, int maxb) {
+    extern __shared__ float2 lds[];
+    int n = f.n;
+    float2 *A = lds, *B = lds + FFT_LB * n;
+    float2 *tw = B + FFT_LB * n;
+    for (int m = threadIdx.x; m < n; m += blockDim.x) tw[m] = f.tw[m];
+    int nlines, ls, es;
+    size_t gbase = 0;
+    int x0 = 0, y0 = 0, z0 = 0, nl = 0;
+    if (MODE == 0 || MODE == 3) {
+        nlines = f.K0 * f.K1;
+        int l0 = blockIdx.x * FFT_LB;
+        nl = min(FFT_LB, nlines - l0);
+        ls = n; es = 1;
+        gbase = (size_t)l0 * n;
+        for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
+            float2 v;
+            if (MODE == 0) {
+                long long q = qfix[gbase + idx];
+                qfix[gbase + idx] = 0;
+                v = make_float2((float)((double)q * TTC_QFIX_INV), 0.0f);
+            } else {
+                v = qc[gbase + idx];
+            }
+            A[idx] = v;
+        }
+    } else {
+        // lines along y (MODE 1: fixed x) or x (MODE 2: fixed y), LB z values
+        int nzb = (f.K2 + FFT_LB - 1) / FFT_LB;
+        int fixed = blockIdx.x / nzb;
+        z0 = (blockIdx.x - fixed * nzb) * FFT_LB;
+        nl = min(FFT_LB, f.K2 - z0);
+        ls = 1; es = FFT_LB;
+        if (MODE == 1) x0 = fixed; else y0 = fixed;
+        for (int idx = threadIdx.x; idx < n * FFT_LB; idx += blockDim.x) {
+            int e = idx / FFT_LB, l = idx - e * FFT_LB;
+            float2 v = make_float2(0.0f, 0.0f);
+            if (l < nl) {
+                size_t g = MODE == 1 ? ((size_t)x0 * f.K1 + e) * f.K2 + z0 + l
+                                     : ((size_t)e * f.K1 + y0) * f.K2 + z0 + l;
+                v = qc[g];
+            }
+            A[idx] = v;
+        }
+        nl = FFT_LB;
+    }
+    __syncthreads();
+    float2 *res = fft_lds(A, B, nl, ls, es, f, tw, MODE == 3 ? -1 : (MODE == 1 ? sign : +1));
+
+    if (MODE == 2) {
+        // influence function (pme_calc.c:52-93); element e = k1, line l -> (k2=y0, k3=z0+l)
+        double en = 0.0, w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
+        int nzl = min(FFT_LB, f.K2 - z0);
+        for (int idx = threadIdx.x; idx < n * FFT_LB; idx += blockDim.x) {
+            int e = idx / FFT_LB, l = idx - e * FFT_LB;
+            if (l >= nzl) continue;
+            size_t g = ((size_t)e * f.K1 + y0) * f.K2 + z0 + l;
+            float et = eterm[g];
+            float2 v = res[idx];
+            if (ENERGY && g != 0) {
+                int k1 = e, k2 = y0, k3 = z0 + l;
+                int mx = k1 - (k1 > f.K0 / 2 ? f.K0 : 0);
+                int my = k2 - (k2 > f.K1 / 2 ? f.K1 : 0);
+                int mz = k3 - (k3 > f.K2 / 2 ? f.K2 : 0);
+                double hx = mx / lx, hy = my / ly, hz = mz / lz;
+                double m2 = hx * hx + hy * hy + hz * hz;
+                double st2 = (double)v.x * v.x + (double)v.y * v.y;
+                double es2 = (double)et * st2;
+                double wt = -2.0 * (fac * m2 + 1.0) / m2;
+                en += es2;
+                w0 += es2 * (wt * hx * hx + 1.0);
+                w1 += es2 * (wt * hy * hx);
+                w2 += es2 * (wt * hy * hy + 1.0);
+                w3 += es2 * (wt * hz * hx);
+                w4 += es2 * (wt * hz * hy);
+                w5 += es2 * (wt * hz * hz + 1.0);
+            }
+            res[idx] = make_float2(v.x * et, v.y * et);
+        }
+        if (ENERGY) {
+            double vals[7] = {en, w0, w1, w2, w3, w4, w5};
+            __shared__ double red[FFT_THREADS / 64][7];
+            int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            for (int q = 0; q < 7; q++) {
+                double t = ttc_wave_sum(vals[q]);
+                if (lane == 0) red[wid][q] = t;
+            }
+            __syncthreads();
+            if (threadIdx.x < 7) {
+                double t = 0.0;
+                for (int w = 0; w < FFT_THREADS / 64; w++) t += red[w][threadIdx.x];
+                partials[(size_t)(EV_REC + threadIdx.x) * maxb + blockIdx.x] = t;
+            }
+        }
+        __syncthreads();
+        float2 *other = res == A ? B : A;
+        res = fft_lds(res, other, FFT_LB, ls, es, f, tw, -1);
+    }
+
+    // store
+    if (MODE == 0 || MODE == 3) {
+        for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
+            if (MODE == 0) qc[gbase + idx] = res[idx];
+            else qr[gbase + idx] = res[idx].x;
+        }
+    } else {
+        int nzl = min(FFT_LB, f.K2 - z0);
+        for (int idx = threadIdx.x; idx < n * FFT_LB; idx += blockDim.x) {
+            int e = idx / FFT_LB, l = idx - e * FFT_LB;
+            if (l >= nzl) continue;
+            size_t g = MODE == 1 ? ((size_t)x0 * f.K1 + e) * f.K2 + z0 + l
+                                 : ((size_t)e * f.K1 + y0) * f.K2 + z0 + l;
+            qc[g] = res[idx];
+        }
+    }
+}
+
+struct GatherConst {
+    double lx, ly, lz, hx, hy, hz;
+    int K0, K1, K2;
+    double qscale;   // QFACTOR / epsRF
+};
+
+template <int P, bool ENERGY>
+__global__ void __launch_bounds__(256)
+k_gather(int n, const double4 *__restrict__ pos4, const int *__restrict__ orig, const int *__restrict__ inv,
+         const float *__restrict__ qr, GatherConst gc, const int *__restrict__ b_off,
+         const int *__restrict__ b_part, const double *__restrict__ b_r0, const double *__restrict__ b_k,
+         double4 *__restrict__ g4, double *partials, int maxb, int do_pme, int do_bonds) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    double eb = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
+    if (s < n) {
+        double4 p = pos4[s];
+        if (do_pme) {
+        int fx, fy, fz;
+        float wx, wy, wz;
+        grid_coord(p.x, gc.lx, gc.K0, &fx, &wx);
+        grid_coord(p.y, gc.ly, gc.K1, &fy, &wy);
+        grid_coord(p.z, gc.lz, gc.K2, &fz, &wz);
+        float Mx[P + 1], My[P + 1], Mz[P + 1], Dx[P + 1], Dy[P + 1], Dz[P + 1];
+        bspline<P>(wx, Mx, Dx);
+        bspline<P>(wy, My, Dy);
+        bspline<P>(wz, Mz, Dz);
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll
+        for (int xi = 1; xi <= P; xi++) {
+            int gx = wrapk(fx - P + xi, gc.K0);
+#pragma unroll
+            for (int yi = 1; yi <= P; yi++) {
+                int gy = wrapk(fy - P + yi, gc.K1);
+                const float *rowp = qr + ((size_t)gx * gc.K1 + gy) * gc.K2;
+                float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll
+                for (int zi = 1; zi <= P; zi++) {
+                    int gz = wrapk(fz - P + zi, gc.K2);
+                    float t = rowp[gz];
+                    sx += Mz[zi] * t;
+                    sz += Dz[zi] * t;
+                }
+                ax -= Dx[xi] * My[yi] * sx;
+                ay -= Mx[xi] * Dy[yi] * sx;
+                az -= Mx[xi] * My[yi] * sz;
+                (void)sy;
+            }
+        }
+        double f = p.w * gc.qscale;
+        px = (double)ax * (f * gc.K0 / gc.lx);
+        py = (double)ay * (f * gc.K1 / gc.ly);
+        pz = (double)az * (f * gc.K2 / gc.lz);
+        }
+        // bonds of this site (cbond.c:65-120), each bond seen from both ends
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        int o = orig[s];
+        int kb0 = do_bonds ? b_off[o] : 0, kb1 = do_bonds ? b_off[o + 1] : 0;
+        for (int k = kb0; k < kb1; k++) {
+            double4 q = pos4[inv[b_part[k]]];
+            double dx = ttc_minimg(q.x - p.x, gc.hx, gc.lx);
+            double dy = ttc_minimg(q.y - p.y, gc.hy, gc.ly);
+            double dz = ttc_minimg(q.z - p.z, gc.hz, gc.lz);
+            double dr = sqrt(dx * dx + dy * dy + dz * dz);
+            double rr = dr - b_r0[k];
+            double fr = -b_k[k] * rr * 2.0;
+            double fxb = fr / dr * dx, fyb = fr / dr * dy, fzb = fr / dr * dz;
+            bx -= fxb; by -= fyb; bz -= fzb;
+            if (ENERGY) {
+                eb += -(fr * rr * 0.5) * 0.5;
+                v0 += 0.5 * fxb * dx; v1 += 0.5 * fyb * dx; v2 += 0.5 * fyb * dy;
+                v3 += 0.5 * fzb * dx; v4 += 0.5 * fzb * dy; v5 += 0.5 * fzb * dz;
+            }
+        }
+        g4[s] = make_double4(px + bx, py + by, pz + bz, 0.0);
+    }
+    // per-block sums (PME force sum always; bond energy/virial on demand)
+    __shared__ double red[4][10];
+    int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double vals[10] = {px, py, pz, eb, v0, v1, v2, v3, v4, v5};
+    int nv = ENERGY ? 10 : 3;
+    for (int q = 0; q < nv; q++) {
+        double t = ttc_wave_sum(vals[q]);
+        if (lane == 0) red[wid][q] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < nv) {
+        double t = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+        int slot = threadIdx.x < 3 ? EV_FSX + threadIdx.x : EV_BOND + (threadIdx.x - 3);
+        partials[(size_t)slot * maxb + blockIdx.x] = t;
+    }
+}
+
+FftArgs fft_args(const ttc_ctx *c, int axis) {
+    FftArgs f;
+    f.K0 = c->K[0]; f.K1 = c->K[1]; f.K2 = c->K[2];
+    const FftPlan &pl = c->plan[axis];
+    f.n = pl.n;
+    f.nrad = pl.nrad;
+    for (int i = 0; i < 16; i++) f.rad[i] = i < pl.nrad ? pl.rad[i] : 1;
+    f.tw = pl.tw;
+    return f;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+
+// pme_setup.c:86-130 (bsp_mod) and the influence function of pme_calc.c:
+// 52-72, evaluated once per box in FP64 and stored as FP32.
+static void influence_host(const ttc_ctx *c, std::vector<float> &et) {
+    int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2], P = c->order;
+    // fillspline at w = 0 in FP64 (pme_spline.c:10-49)
+    std::vector<double> a(P + 2, 0.0);
+    {
+        double w = 0.0;
+        a[2] = w; a[1] = 1.0 - w; a[0] = 0.0;
+        for (int i = 3; i <= P - 1; i++) {
+            double div = 1.0 / (i - 1);
+            a[i] = div * w * a[i - 1];
+            for (int j = 1; j <= i - 2; j++) a[i - j] = div * ((w + j) * a[i - j - 1] + (i - j - w) * a[i - j]);
+            a[1] *= div * (1.0 - w);
+        }
+        double div = 1.0 / (P - 1);
+        a[P] = div * w * a[P - 1];
+        for (int i = 1; i <= P - 2; i++) a[P - i] = div * ((w + i) * a[P - i - 1] + (P - i - w) * a[P - i]);
+        a[1] *= div * (1.0 - w);
+    }
+    auto dftmod = [&](int K, std::vector<double> &bm) {
+        std::vector<double> barr(K + P + 3, 0.0);
+        for (int i = 2; i <= P + 1; i++) barr[i] = a[i - 1];
+        bm.assign(K + 2, 0.0);
+        for (int i = 1; i <= K; i++) {
+            double s1 = 0.0, s2 = 0.0;
+            for (int j = 1; j <= K; j++) {
+                double arg = 2.0 * TTC_PI * (i - 1) * (j - 1) / K;
+                s1 += barr[j] * cos(arg);
+                s2 += barr[j] * sin(arg);
+            }
+            bm[i] = s1 * s1 + s2 * s2;
+        }
+    };
+    std::vector<double> bx, by, bz;
+    dftmod(K0, bx); dftmod(K1, by); dftmod(K2, bz);
+    double lx = c->p.lx, ly = c->p.ly, lz = c->p.lz, vol = lx * ly * lz;
+    double fac = TTC_PI * TTC_PI / c->p.kappa / c->p.kappa;
+    et.assign((size_t)K0 * K1 * K2, 0.0f);
+    for (int k1 = 0; k1 < K0; k1++)
+        for (int k2 = 0; k2 < K1; k2++)
+            for (int k3 = 0; k3 < K2; k3++) {
+                size_t ind = ((size_t)k1 * K1 + k2) * K2 + k3;
+                if (ind == 0) continue;   // m = 0 carries no energy; its value drops out of the force
+                int mx = k1 - (k1 > K0 / 2 ? K0 : 0);
+                int my = k2 - (k2 > K1 / 2 ? K1 : 0);
+                int mz = k3 - (k3 > K2 / 2 ? K2 : 0);
+                double hx = mx / lx, hy = my / ly, hz = mz / lz;
+                double m2 = hx * hx + hy * hy + hz * hz;
+                double bb = bx[k1 + 1] * by[k2 + 1] * bz[k3 + 1];
+                et[ind] = (float)(exp(-fac * m2) / (TTC_PI * vol * m2 * bb));
+            }
+}
+
+static int make_plan(FftPlan &pl, int n) {
+    pl.n = n;
+    pl.nrad = 0;
+    int m = n;
+    while (m % 4 == 0 && m > 4) { pl.rad[pl.nrad++] = 4; m /= 4; }
+    while (m % 2 == 0) { pl.rad[pl.nrad++] = 2; m /= 2; }
+    while (m % 3 == 0) { pl.rad[pl.nrad++] = 3; m /= 3; }
+    for (int p = 5; m > 1; p++)
+        while (m % p == 0) { pl.rad[pl.nrad++] = p; m /= p; }
+    if (pl.nrad > 16) return TTC_ERR_ARG;
+    std::vector<float2> tw(n);
+    for (int i = 0; i < n; i++) tw[i] = make_float2((float)cos(2.0 * TTC_PI * i / n), (float)sin(2.0 * TTC_PI * i / n));
+    if (hipMalloc(&pl.tw, sizeof(float2) * n) != hipSuccess) return TTC_ERR_ALLOC;
+    if (hipMemcpy(pl.tw, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice) != hipSuccess) return TTC_ERR_HIP;
+    return TTC_OK;
+}
+
+int ttc_pme_setup(ttc_ctx *c) {
+    c->K[0] = c->p.gx; c->K[1] = c->p.gy; c->K[2] = c->p.gz;
+    c->ngrid = (size_t)c->K[0] * c->K[1] * c->K[2];
+    for (int a = 0; a < 3; a++) {
+        int r = make_plan(c->plan[a], c->K[a]);
+        if (r) { ttc_set_error("PME grid size cannot be planned"); return r; }
+    }
+    TTC_HIP(hipMalloc(&c->qfix, sizeof(long long) * c->ngrid));
+    TTC_HIP(hipMemset(c->qfix, 0, sizeof(long long) * c->ngrid));
+    TTC_HIP(hipMalloc(&c->qc, sizeof(float2) * c->ngrid));
+    TTC_HIP(hipMalloc(&c->qr, sizeof(float) * c->ngrid));
+    TTC_HIP(hipMalloc(&c->eterm, sizeof(float) * c->ngrid));
+    std::vector<float> et;
+    influence_host(c, et);
+    TTC_HIP(hipMemcpy(c->eterm, et.data(), sizeof(float) * c->ngrid, hipMemcpyHostToDevice));
+    return TTC_OK;
+}
+
+void ttc_pme_free(ttc_ctx *c) {
+    hipFree(c->qfix); hipFree(c->qc); hipFree(c->qr); hipFree(c->eterm);
+    for (int a = 0; a < 3; a++) hipFree(c->plan[a].tw);
+}
+
+int ttc_pme_conv_blocks(ttc_ctx *c) { return c->K[1] * ((c->K[2] + FFT_LB - 1) / FFT_LB); }
+int ttc_gather_blocks(ttc_ctx *c) { return (c->n + 255) / 256; }
+
+static Spread spread_args(const ttc_ctx *c) {
+    Spread sp;
+    sp.lx = c->p.lx; sp.ly = c->p.ly; sp.lz = c->p.lz;
+    sp.K0 = c->K[0]; sp.K1 = c->K[1]; sp.K2 = c->K[2];
+    return sp;
+}
+
+// spread only (ttc_pme_calc / ttc_cnonbondnl): positions unchanged
+void ttc_launch_spread(ttc_ctx *c) {
+    int blocks = (c->n + 255) / 256;
+    Spread sp = spread_args(c);
+    if (c->order == 4)
+        k_drift_spread<4, false><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->vel4, c->f4, 0.0, 1.0, sp, c->qfix, c->status);
+    else
+        k_drift_spread<6, false><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->vel4, c->f4, 0.0, 1.0, sp, c->qfix, c->status);
+}
+
+void ttc_launch_drift(ttc_ctx *c, double vscal) {
+    int blocks = (c->n + 255) / 256;
+    Spread sp = spread_args(c);
+    if (c->order == 4)
+        k_drift_spread<4, true><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->vel4, c->f4, c->p.dt, vscal, sp, c->qfix, c->status);
+    else
+        k_drift_spread<6, true><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->vel4, c->f4, c->p.dt, vscal, sp, c->qfix, c->status);
+}
+
+static size_t fft_shm(const ttc_ctx *c, int axis) {
+    return sizeof(float2) * (2 * FFT_LB * c->plan[axis].n + c->plan[axis].n);
+}
+
+// FFT passes, influence function, gather (+ bonds).  Expects the spread
+// grid in qfix; leaves qfix zeroed.
+void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, bool bonds) {
+    double fac = TTC_PI * TTC_PI / c->p.kappa / c->p.kappa;
+    double lx = c->p.lx, ly = c->p.ly, lz = c->p.lz;
+    int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2];
+    int nzb = (K2 + FFT_LB - 1) / FFT_LB;
+    FftArgs fz = fft_args(c, 2), fy = fft_args(c, 1), fx = fft_args(c, 0);
+    int bz = (K0 * K1 + FFT_LB - 1) / FFT_LB;
+    if (pme) {
+    k_fft<0, false><<<bz, FFT_THREADS, fft_shm(c, 2), c->stream>>>(fz, +1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
+    k_fft<1, false><<<K0 * nzb, FFT_THREADS, fft_shm(c, 1), c->stream>>>(fy, +1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
+    if (energy)
+        k_fft<2, true><<<K1 * nzb, FFT_THREADS, fft_shm(c, 0), c->stream>>>(fx, +1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
+    else
+        k_fft<2, false><<<K1 * nzb, FFT_THREADS, fft_shm(c, 0), c->stream>>>(fx, +1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
+    k_fft<1, false><<<K0 * nzb, FFT_THREADS, fft_shm(c, 1), c->stream>>>(fy, -1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
+    k_fft<3, false><<<bz, FFT_THREADS, fft_shm(c, 2), c->stream>>>(fz, -1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
+    }
+
+    GatherConst gc;
+    gc.lx = lx; gc.ly = ly; gc.lz = lz;
+    gc.hx = 0.5 * lx; gc.hy = 0.5 * ly; gc.hz = 0.5 * lz;
+    gc.K0 = K0; gc.K1 = K1; gc.K2 = K2;
+    gc.qscale = TTC_QFACTOR / c->p.eps_rf;
+    int blocks = ttc_gather_blocks(c);
+#define GATHER(P, E)                                                                                    \
+    k_gather<P, E><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->orig, c->inv, c->qr, gc, c->b_off,   \
+                                                  c->b_part, c->b_r0, c->b_k, c->g4, c->partials, c->nblk_max, \
+                                                  pme ? 1 : 0, bonds ? 1 : 0)
+    if (c->order == 4) { if (energy) GATHER(4, true); else GATHER(4, false); }
+    else { if (energy) GATHER(6, true); else GATHER(6, false); }
+#undef GATHER
+}

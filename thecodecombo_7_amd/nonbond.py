@@ -1,0 +1,166 @@
+"""Host mirror of the reference's nonbonded entry points over libttc.so.
+
+`Box` holds one device context and exposes the reference's operations by
+their reference names, with the reference's argument meaning and error
+behaviour (errors carry the reference's exit codes, see include/ttc.h):
+
+    nblist()                 nblist.c:32
+    nl_check(lb, ub)         nblist.c:607
+    get_nblist(which)        the nlist[k] / nlist_ew[k] arrays
+    cnonbondnl()             cnonbond_nblist.c:51   (pair loop + pme_calc)
+    pme_calc()               pme_calc.c:18
+    cbond()                  cbond.c:65
+    forces()                 forces.c:118
+    integrate_vverlet(n)     integrate_vverlet.c:36, n steps with the
+                             nl_check/nblist trigger of nve_md.c:369-370
+    kinet()                  kinet.c:18
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as T
+
+
+class Box:
+    def __init__(self, sysm, ewald_lists: bool = True):
+        L = T.lib()
+        self.L = L
+        self.sys = sysm
+        self.n = sysm.n
+        p = T.TtcParams(
+            n=sysm.n, lx=sysm.box[0], ly=sysm.box[1], lz=sysm.box[2], rc=sysm.rc,
+            rclist=sysm.rclist, rclist_ew=sysm.rclist_ew, ewald_lists=int(ewald_lists),
+            kappa=sysm.kappa, eps_rf=sysm.eps_rf, e14fac=sysm.e14fac, order=sysm.order,
+            gx=sysm.grid[0], gy=sysm.grid[1], gz=sysm.grid[2], ntypes=sysm.ntypes, dt=sysm.dt)
+        h = C.c_void_p()
+        T.check(L.ttc_create(C.byref(p), C.byref(h)))
+        self.h = h
+        try:
+            self._topology()
+            self.set_positions(sysm.pos)
+            self.set_velocities(sysm.vel)
+        except Exception:
+            self.close()
+            raise
+
+    def _topology(self):
+        s = self.sys
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        self._keep = [i32(s.type), f64(s.q), f64(s.mass), f64(s.lj_eps), f64(s.lj_sig),
+                      f64(s.lj14_eps), f64(s.lj14_sig), i32(s.xint[:, 0]), i32(s.xint[:, 1]),
+                      i32(s.xint[:, 2]), i32(s.bonds[:, 0]), i32(s.bonds[:, 1]), f64(s.bond_r0),
+                      f64(s.bond_k)]
+        k = self._keep
+        T.check(self.L.ttc_set_topology(
+            self.h, T.iptr(k[0]), T.dptr(k[1]), T.dptr(k[2]), T.dptr(k[3]), T.dptr(k[4]),
+            T.dptr(k[5]), T.dptr(k[6]), len(s.xint), T.iptr(k[7]), T.iptr(k[8]), T.iptr(k[9]),
+            len(s.bonds), T.iptr(k[10]), T.iptr(k[11]), T.dptr(k[12]), T.dptr(k[13])))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ttc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state ----
+    def set_positions(self, pos):
+        a = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.n, 3)
+        T.check(self.L.ttc_set_positions(self.h, T.dptr(a), 3))
+
+    def positions(self):
+        a = np.zeros((self.n, 3))
+        T.check(self.L.ttc_get_positions(self.h, T.dptr(a), 3))
+        return a
+
+    def set_velocities(self, vel):
+        a = np.ascontiguousarray(vel, dtype=np.float64).reshape(self.n, 3)
+        T.check(self.L.ttc_set_velocities(self.h, T.dptr(a), 3))
+
+    def velocities(self):
+        a = np.zeros((self.n, 3))
+        T.check(self.L.ttc_get_velocities(self.h, T.dptr(a), 3))
+        return a
+
+    @property
+    def e_self(self) -> float:
+        return self.L.ttc_self_energy(self.h)
+
+    # ---- nblist.c ----
+    def nblist(self):
+        T.check(self.L.ttc_nblist(self.h))
+
+    def nl_check(self, lb: int = 0, ub: int | None = None) -> int:
+        f = C.c_int(0)
+        T.check(self.L.ttc_nl_check(self.h, lb, self.n if ub is None else ub, C.byref(f)))
+        return f.value
+
+    def get_nblist(self, which: int = T.TTC_LIST_NLIST, max_nb: int = 1500):
+        cnt = np.zeros(self.n, np.int32)
+        lst = np.zeros(self.n * max_nb, np.int32)
+        xf = np.zeros(self.n * max_nb, np.int32)
+        T.check(self.L.ttc_get_nblist(self.h, which, max_nb, T.iptr(cnt), T.iptr(lst), T.iptr(xf)))
+        return cnt, lst.reshape(self.n, max_nb), xf.reshape(self.n, max_nb)
+
+    # ---- forces ----
+    def cnonbondnl(self):
+        ff = np.zeros((self.n, 3)); en = np.zeros(8); vn = np.zeros(6); vk = np.zeros(6)
+        T.check(self.L.ttc_cnonbondnl(self.h, T.dptr(ff), 3, T.dptr(en), T.dptr(vn), T.dptr(vk)))
+        return ff, en, vn, vk
+
+    def pme_calc(self):
+        ff = np.zeros((self.n, 3)); en = np.zeros(3); vir = np.zeros(6)
+        T.check(self.L.ttc_pme_calc(self.h, T.dptr(ff), 3, T.dptr(en), T.dptr(vir)))
+        return ff, en, vir
+
+    def pair(self):
+        ff = np.zeros((self.n, 3)); e5 = np.zeros(5); vir = np.zeros(6)
+        T.check(self.L.ttc_pair(self.h, T.dptr(ff), 3, T.dptr(e5), T.dptr(vir)))
+        return ff, e5, vir
+
+    def cbond(self):
+        ff = np.zeros((self.n, 3)); eb = np.zeros(1); vir = np.zeros(6)
+        T.check(self.L.ttc_cbond(self.h, T.dptr(ff), 3, T.dptr(eb), T.dptr(vir)))
+        return ff, float(eb[0]), vir
+
+    def forces(self):
+        ff = np.zeros((self.n, 3)); en = np.zeros(9)
+        T.check(self.L.ttc_forces(self.h, T.dptr(ff), 3, T.dptr(en)))
+        keys = ["bond", "nbond", "nbonds", "coulomb", "rewald", "rewalds", "kewald", "sewald", "iewald"]
+        return ff, dict(zip(keys, en))
+
+    def kinet(self):
+        k = np.zeros(1); ken = np.zeros(6)
+        T.check(self.L.ttc_kinet(self.h, T.dptr(k), T.dptr(ken)))
+        return float(k[0]), ken
+
+    # ---- integrate_vverlet.c ----
+    def integrate_vverlet(self, nsteps: int, vscal: float = 1.0, energy_every: int = 0):
+        rows = max(1, nsteps // energy_every) if energy_every else 1
+        buf = np.zeros((rows, 9))
+        nr = C.c_int(0)
+        nreb = C.c_int(0)
+        T.check(self.L.ttc_md_vverlet(self.h, nsteps, vscal, energy_every, T.dptr(buf), rows,
+                                      C.byref(nr), C.byref(nreb)))
+        return buf[:nr.value], nreb.value
+
+    def timing(self, which: int) -> float:
+        t = np.zeros(1)
+        T.check(self.L.ttc_timing(self.h, which, T.dptr(t)))
+        return float(t[0])
+
+    def list_stats(self):
+        e = C.c_int64(0); m = C.c_int(0); cap = C.c_int(0)
+        T.check(self.L.ttc_list_stats(self.h, C.byref(e), C.byref(m), C.byref(cap)))
+        return e.value, m.value, cap.value
+
+
+ROW_KEYS = ["step", "kinet", "bond", "nbond", "nbonds", "coulomb", "kewald", "rewald", "iewald"]
