@@ -120,6 +120,8 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
 
     int r = ttc_pme_setup(c);
     if (r) return r;
+    r = ttc_pme_attrs(c);
+    if (r) return r;
     c->nblk_max = std::max(std::max(ttc_pair_blocks(c), ttc_pme_conv_blocks(c)),
                            std::max(ttc_gather_blocks(c), ttc_kick_blocks(c)));
     ALLOC(c->partials, (size_t)EV_N * c->nblk_max);
@@ -406,6 +408,8 @@ extern "C" int ttc_pair(ttc_ctx *c, double *ff, int64_t stride, double *e5, doub
 
 extern "C" int ttc_pme_calc(ttc_ctx *c, double *ff, int64_t stride, double *energy3, double *vir6) {
     CHECK(c && c->topo, TTC_ERR_ARG, "ttc_pme_calc: topology not set");
+    int r0 = need_list(c);   // the spread walks the sites cell by cell
+    if (r0) return r0;
     ttc_launch_spread(c);
     ttc_launch_pme(c, true, true, false);
     ttc_launch_reduce(c, false);

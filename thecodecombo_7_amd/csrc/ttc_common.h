@@ -149,6 +149,7 @@ void ttc_launch_pair(ttc_ctx *c, bool energy, int blk_off);
 int ttc_pair_blocks(ttc_ctx *c);
 // pme (+ bonds in the gather)
 int ttc_pme_setup(ttc_ctx *c);
+int ttc_pme_attrs(ttc_ctx *c);   // LDS limits of the spread kernel (needs the cell geometry)
 void ttc_pme_free(ttc_ctx *c);
 // gather: pme -> PME forces, bonds -> bond forces (either may be off)
 void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, bool bonds);
@@ -182,16 +183,23 @@ __device__ __forceinline__ long long ttc_to_fix(double v) {
 }
 
 // Minimum image exactly as cnonbond_nblist.c:166-173 (one-sided wrap).
+// Kept as branches: almost every pair takes neither, so they are cheaper
+// than computing both shifted values and selecting.
 __device__ __forceinline__ double ttc_minimg(double d, double h, double l) {
     if (d > h) d -= l;
     else if (d < -h) d += l;
     return d;
 }
 
-// FP64 sum of squares without contraction: (dx*dx + dy*dy) + dz*dz rounded
-// op by op, as the reference's C computes it.
+// FP64 sum of squares rounded op by op, (dx*dx + dy*dy) + dz*dz, as the
+// reference's C computes it (x86-64 without FMA).  Contraction into FMAs is
+// switched off here: it would change which boundary pairs pass the test.
 __device__ __forceinline__ double ttc_r2_exact(double dx, double dy, double dz) {
-    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+#pragma clang fp contract(off)
+    double x2 = dx * dx;
+    double y2 = dy * dy;
+    double z2 = dz * dz;
+    return (x2 + y2) + z2;
 }
 
 __device__ __forceinline__ double ttc_wave_sum(double v) {

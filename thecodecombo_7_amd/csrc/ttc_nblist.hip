@@ -101,11 +101,38 @@ __global__ void k_reorder(int n, int cellcap, const int *cell_of, const int *cel
     origb[s] = o;
 }
 
-// Full Verlet list, one lane per sorted site.  The pair test is the
-// reference's (nblist.c:116-145): |d| wrapped once, per-component reject,
-// dr2 < rcut^2 in IEEE double rounded op by op.  It is symmetric in the two
-// sites, so the full list holds every reference pair in both rows.
-__global__ void __launch_bounds__(256)
+#define BUILD_WAVES 8
+
+// The reference's pair test (nblist.c:116-145): |d| wrapped once, every
+// component <= rcut, dr2 < rcut2 in IEEE double rounded op by op.  It is
+// symmetric in the two sites, so the full list holds every reference pair in
+// both rows.  Evaluated without branches.
+__device__ __forceinline__ bool ref_pair_test(const double4 &pi, const double4 &pj, const CellGeom &g,
+                                              double rcut, double rcut2) {
+    double dx = fabs(pi.x - pj.x);
+    dx = dx > g.hx ? dx - g.lx : dx;
+    double dy = fabs(pi.y - pj.y);
+    dy = dy > g.hy ? dy - g.ly : dy;
+    double dz = fabs(pi.z - pj.z);
+    dz = dz > g.hz ? dz - g.lz : dz;
+    double dr2 = ttc_r2_exact(dx, dy, dz);
+    return (dx <= rcut) & (dy <= rcut) & (dz <= rcut) & (dr2 < rcut2);
+}
+
+// smallest periodic distance from coordinate x to the slab [lo, lo + w)
+__device__ __forceinline__ double slab_gap(double x, double lo, double w, double l) {
+    double d = x - lo;
+    d -= l * floor(d / l);           // 0 <= d < l
+    if (d < w) return 0.0;
+    return fmin(d - w, l - d);
+}
+
+// Full Verlet list.  One workgroup per ELL group of 64 sorted sites; its
+// BUILD_WAVES waves split each site's 5x5x5 cell stencil (stencil cell t
+// goes to wave t % BUILD_WAVES) and skip cells farther than rcut.  Pass 1
+// counts each site's hits per wave, pass 2 writes them at prefix offsets,
+// so the row order is fixed (stencil order within a wave, waves in order).
+__global__ void __launch_bounds__(64 * BUILD_WAVES)
 k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__restrict__ pos4b,
         const double4 *__restrict__ vel4b, const double4 *__restrict__ f4b, const int *__restrict__ typeb,
         const int *__restrict__ origb, double4 *pos4, double4 *vel4, double4 *f4, int *type, int *orig,
@@ -114,78 +141,84 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__
         const int *__restrict__ ex_off, const int *__restrict__ ex_part, const int *__restrict__ ex_flag,
         int ex_span, uint32_t *nbl, int *glen, int *status, int conditional, int copy_state) {
     if (rebuild_off(status, conditional)) return;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    int lane = threadIdx.x & 63;
-    int grp = s >> 6;
-    int cnt = 0;
-    if (s < n) {
-        double4 pi = pos4b[s];
-        int oi = origb[s];
-        if (copy_state) {
-            pos4[s] = pi;
-            vel4[s] = vel4b[s];
-            f4[s] = f4b[s];
-            type[s] = typeb[s];
-            orig[s] = oi;
-            anchor[s] = pi;
-            inv[oi] = s;
-        }
-        int cx = cell_coord(pi.x, g.hx, g.lx, g.ncx);
-        int cy = cell_coord(pi.y, g.hy, g.ly, g.ncy);
-        int cz = cell_coord(pi.z, g.hz, g.lz, g.ncz);
-        int lox = g.ncx >= 5 ? -2 : 0, hix = g.ncx >= 5 ? 2 : g.ncx - 1;
-        int loy = g.ncy >= 5 ? -2 : 0, hiy = g.ncy >= 5 ? 2 : g.ncy - 1;
-        int loz = g.ncz >= 5 ? -2 : 0, hiz = g.ncz >= 5 ? 2 : g.ncz - 1;
-        int xb = ex_off[oi], xe = ex_off[oi + 1];
-        uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
-        for (int oz = loz; oz <= hiz; oz++) {
-            int z = g.ncz >= 5 ? (cz + oz + g.ncz) % g.ncz : oz;
-            for (int oy = loy; oy <= hiy; oy++) {
-                int y = g.ncy >= 5 ? (cy + oy + g.ncy) % g.ncy : oy;
-                for (int ox = lox; ox <= hix; ox++) {
-                    int x = g.ncx >= 5 ? (cx + ox + g.ncx) % g.ncx : ox;
-                    int c = x + g.ncx * (y + g.ncy * z);
-                    int j0 = cell_start[c], j1 = j0 + cell_cnt[c];
-                    for (int j = j0; j < j1; j++) {
-                        if (j == s) continue;
-                        double4 pj = pos4b[j];
-                        double dx = fabs(pi.x - pj.x);
-                        if (dx > g.hx) dx -= g.lx;
-                        if (dx > rcut) continue;
-                        double dy = fabs(pi.y - pj.y);
-                        if (dy > g.hy) dy -= g.ly;
-                        if (dy > rcut) continue;
-                        double dz = fabs(pi.z - pj.z);
-                        if (dz > g.hz) dz -= g.lz;
-                        if (dz > rcut) continue;
-                        double dr2 = ttc_r2_exact(dx, dy, dz);
-                        if (!(dr2 < rcut2)) continue;
-                        uint32_t cls = 0;
-                        int oj = origb[j];
-                        int d = oj - oi;
-                        if (d <= ex_span && -d <= ex_span) {
-                            for (int k = xb; k < xe; k++)
-                                if (ex_part[k] == oj) { int f = ex_flag[k]; cls = f == 4 ? 2u : (f ? 1u : 0u); }
-                        }
-                        if (cnt < cap) row[(size_t)cnt * 64] = (uint32_t)j | (cls << 30);
-                        cnt++;
-                    }
+    __shared__ int scnt[BUILD_WAVES][64];
+    __shared__ int smax;
+    int grp = blockIdx.x;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int s = grp * 64 + lane;
+    bool valid = s < n;
+    double4 pi = valid ? pos4b[s] : make_double4(0, 0, 0, 0);
+    int oi = valid ? origb[s] : 0;
+    if (valid && copy_state && wave == 0) {
+        pos4[s] = pi;
+        vel4[s] = vel4b[s];
+        f4[s] = f4b[s];
+        type[s] = typeb[s];
+        orig[s] = oi;
+        anchor[s] = pi;
+        inv[oi] = s;
+    }
+    int cx = cell_coord(pi.x, g.hx, g.lx, g.ncx);
+    int cy = cell_coord(pi.y, g.hy, g.ly, g.ncy);
+    int cz = cell_coord(pi.z, g.hz, g.lz, g.ncz);
+    int nsx = g.ncx >= 5 ? 5 : g.ncx, nsy = g.ncy >= 5 ? 5 : g.ncy, nsz = g.ncz >= 5 ? 5 : g.ncz;
+    int nst = nsx * nsy * nsz;
+    double cwx = g.lx / g.ncx, cwy = g.ly / g.ncy, cwz = g.lz / g.ncz;
+    int xb = valid ? ex_off[oi] : 0, xe = valid ? ex_off[oi + 1] : 0;
+    uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
+
+    int mine = 0, off = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        int k = off;
+        for (int t = wave; t < nst && valid; t += BUILD_WAVES) {
+            int ox = t % nsx, oy = (t / nsx) % nsy, oz = t / (nsx * nsy);
+            int x = g.ncx >= 5 ? (cx + ox - 2 + g.ncx) % g.ncx : ox;
+            int y = g.ncy >= 5 ? (cy + oy - 2 + g.ncy) % g.ncy : oy;
+            int z = g.ncz >= 5 ? (cz + oz - 2 + g.ncz) % g.ncz : oz;
+            double gx = slab_gap(pi.x, -g.hx + x * cwx, cwx, g.lx);
+            double gy = slab_gap(pi.y, -g.hy + y * cwy, cwy, g.ly);
+            double gz = slab_gap(pi.z, -g.hz + z * cwz, cwz, g.lz);
+            // margin: the test is exact, the pruning only needs to be safe
+            if (gx * gx + gy * gy + gz * gz > rcut2 * 1.0001 + 1e-9) continue;
+            int c = x + g.ncx * (y + g.ncy * z);
+            int j0 = cell_start[c], j1 = j0 + cell_cnt[c];
+            for (int j = j0; j < j1; j++) {
+                double4 pj = pos4b[j];
+                if (j == s || !ref_pair_test(pi, pj, g, rcut, rcut2)) continue;
+                if (pass == 0) { mine++; continue; }
+                uint32_t cls = 0;
+                int oj = origb[j];
+                int d = oj - oi;
+                if (d <= ex_span && -d <= ex_span) {
+                    for (int q = xb; q < xe; q++)
+                        if (ex_part[q] == oj) { int f = ex_flag[q]; cls = f == 4 ? 2u : (f ? 1u : 0u); }
                 }
+                if (k < cap) row[(size_t)k * 64] = (uint32_t)j | (cls << 30);
+                k++;
             }
         }
+        if (pass == 0) {
+            scnt[wave][lane] = mine;
+            __syncthreads();
+            for (int w = 0; w < wave; w++) off += scnt[w][lane];
+        }
     }
-    if (grp * 64 >= n) return;   // whole wave past the end (tail block)
     // row length of the group = longest row; pad the shorter ones
-    int full = cnt;
-    for (int o = 32; o > 0; o >>= 1) full = max(full, __shfl_xor(full, o, 64));
-    int m = min(full, cap);
-    if (lane == 0) {
-        glen[grp] = m;
-        if (full > cap) atomicOr(&status[ST_OVERFLOW], 1);
-        atomicMax(&status[ST_MAXROW], full);
+    int total = 0;
+    for (int w = 0; w < BUILD_WAVES; w++) total += scnt[w][lane];
+    if (wave == 0) {
+        int full = total;
+        for (int o = 32; o > 0; o >>= 1) full = max(full, __shfl_xor(full, o, 64));
+        if (lane == 0) {
+            smax = full;
+            glen[grp] = min(full, cap);
+            if (full > cap) atomicOr(&status[ST_OVERFLOW], 1);
+            atomicMax(&status[ST_MAXROW], full);
+        }
     }
-    uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
-    for (int k = min(cnt, cap); k < m; k++) row[(size_t)k * 64] = TTC_PAD;
+    __syncthreads();
+    int m = min(smax, cap);
+    for (int k = min(total, cap) + wave; k < m; k += BUILD_WAVES) row[(size_t)k * 64] = TTC_PAD;
 }
 
 // nblist.c:607-647 on every site; sets the trigger flag.  Also the
@@ -268,8 +301,7 @@ int ttc_cellcap(const ttc_ctx *c) {
 static void launch_build(ttc_ctx *c, bool conditional, double cutoff, uint32_t *nbl, int *glen,
                          int cap, bool copy_state) {
     CellGeom g = geom(c);
-    int blocks = (c->n + 255) / 256;
-    k_build<<<blocks, 256, 0, c->stream>>>(
+    k_build<<<c->G, 64 * BUILD_WAVES, 0, c->stream>>>(
         c->n, cap, g, c->p.rclist, cutoff * cutoff, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4,
         c->vel4, c->f4, c->type, c->orig, c->anchor, c->inv, c->cell_cnt, c->cell_start, c->ex_off, c->ex_part, c->ex_flag,
         c->ex_span, nbl, glen, c->status, conditional ? 1 : 0, copy_state ? 1 : 0);

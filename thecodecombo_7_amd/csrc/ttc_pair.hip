@@ -25,7 +25,7 @@ struct PairConst {
 // Numerical Recipes erfcc (ewald.c:817-835) in FP32, argument z >= 0.
 // Returns erfc(z); *ez2 receives exp(-z^2).
 __device__ __forceinline__ float nr_erfc(float z, float *ez2) {
-    float t = __frcp_rn(1.0f + 0.5f * z);
+    float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * z);
     float p = -1.26551223f + t * (1.00002368f + t * (0.37409196f + t * (0.09678418f +
               t * (-0.18628806f + t * (0.27886807f + t * (-1.13520398f +
               t * (1.48851587f + t * (-0.82215223f + t * 0.17087277f))))))));
@@ -34,96 +34,130 @@ __device__ __forceinline__ float nr_erfc(float z, float *ez2) {
     return t * e * __expf(p);
 }
 
+#define PAIR_WAVES 8
+
+struct PairAcc {
+    double fx, fy, fz;
+    double elj, eljs, er, ers, ei, w0, w1, w2, w3, w4, w5;
+};
+
+// One list entry of site i (cnonbond_nblist.c:150-519 for one pair).
 template <bool ENERGY>
-__global__ void __launch_bounds__(256)
+__device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pi, float qi, const float4 *ljrow,
+                                           const double4 *__restrict__ pos4, const int *__restrict__ type,
+                                           const PairConst &pc, PairAcc &a) {
+    if (e == TTC_PAD) return;
+    int j = (int)(e & TTC_JMASK);
+    uint32_t cls = e >> 30;
+    double4 pj = pos4[j];
+    int tj = type[j];
+    double dx = ttc_minimg(pi.x - pj.x, pc.hx, pc.lx);
+    double dy = ttc_minimg(pi.y - pj.y, pc.hy, pc.ly);
+    double dz = ttc_minimg(pi.z - pj.z, pc.hz, pc.lz);
+    double dr2 = ttc_r2_exact(dx, dy, dz);
+    if (dr2 > pc.rc2) return;
+    float r2 = (float)dr2;
+    float rinv = __builtin_amdgcn_rsqf(r2);
+    float rinv2 = rinv * rinv;
+    float r = r2 * rinv;
+    float4 lj = ljrow[tj];
+    bool one4 = cls == 2;
+    float eps = one4 ? lj.z : lj.x;
+    eps = cls == 1 ? 0.0f : eps;
+    float sg2 = one4 ? lj.w : lj.y;
+    float sr2 = sg2 * rinv2;
+    float sr6 = sr2 * sr2 * sr2;
+    float sr12 = sr6 * sr6;
+    float force = 12.0f * eps * rinv2 * (sr12 - sr6);
+    float qqab = qi * (float)pj.w * pc.qfac;
+    float qq = (cls == 0 ? qqab : (cls == 2 ? pc.e14 * qqab : 0.0f)) * pc.inv_epsrf;
+    float ez2;
+    float erfckr = nr_erfc(pc.kappa * r, &ez2);
+    float kpid = pc.kpi2 * r * ez2;
+    float rinv3 = rinv * rinv2;
+    force += qq * rinv3 * (erfckr + kpid);
+    float qi2 = cls == 2 ? pc.one_m_e14 * qqab : qqab;
+    if (cls) force -= qi2 * rinv3 * ((1.0f - erfckr) - kpid);
+    float fxf = force * (float)dx, fyf = force * (float)dy, fzf = force * (float)dz;
+    a.fx += (double)fxf;
+    a.fy += (double)fyf;
+    a.fz += (double)fzf;
+    if (ENERGY) {
+        float sr2s = sg2 / pc.rc2f;
+        float sr6s = sr2s * sr2s * sr2s;
+        a.elj += (double)(eps * (sr12 - 2.0f * sr6));
+        a.eljs += (double)(eps * (sr6s * sr6s - 2.0f * sr6s));
+        a.er += (double)(qq * erfckr * rinv);
+        a.ers += (double)qq * pc.erfc_rc_over_rc;
+        if (cls) a.ei += (double)(qi2 * (1.0f - erfckr) * rinv);
+        a.w0 += (double)(fxf * (float)dx);
+        a.w1 += (double)(fyf * (float)dx);
+        a.w2 += (double)(fyf * (float)dy);
+        a.w3 += (double)(fzf * (float)dx);
+        a.w4 += (double)(fzf * (float)dy);
+        a.w5 += (double)(fzf * (float)dz);
+    }
+}
+
+// One workgroup per ELL group of 64 sites; its PAIR_WAVES waves take the
+// row slices k = wave, wave + PAIR_WAVES, ... (each slice one coalesced
+// 256-byte read) and their partial forces are summed in a fixed wave order.
+template <bool ENERGY>
+__global__ void __launch_bounds__(64 * PAIR_WAVES)
 k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__ glen,
        const double4 *__restrict__ pos4, const int *__restrict__ type, const float4 *__restrict__ ljtab,
        PairConst pc, double4 *__restrict__ fpair, double *__restrict__ partials, int maxb) {
     extern __shared__ float4 slj[];
+    __shared__ double sred[PAIR_WAVES][64][3];
+    __shared__ double ered[PAIR_WAVES][11];
     int nt2 = pc.ntypes * pc.ntypes;
     for (int k = threadIdx.x; k < nt2; k += blockDim.x) slj[k] = ljtab[k];
     __syncthreads();
 
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    int lane = threadIdx.x & 63;
-    int grp = s >> 6;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    double elj = 0.0, eljs = 0.0, er = 0.0, ers = 0.0, ei = 0.0;
-    double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
-    if (grp * 64 < n) {
-        int len = glen[grp];
-        double4 pi = s < n ? pos4[s] : make_double4(0.0, 0.0, 0.0, 0.0);
-        int ti = s < n ? type[s] : 0;
-        const float4 *ljrow = slj + ti * pc.ntypes;
-        float qi = (float)pi.w;
-        const uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
-        for (int k = 0; k < len; k++) {
-            uint32_t e = row[(size_t)k * 64];
-            if (e == TTC_PAD) continue;
-            int j = (int)(e & TTC_JMASK);
-            uint32_t cls = e >> 30;
-            double4 pj = pos4[j];
-            double dx = ttc_minimg(pi.x - pj.x, pc.hx, pc.lx);
-            double dy = ttc_minimg(pi.y - pj.y, pc.hy, pc.ly);
-            double dz = ttc_minimg(pi.z - pj.z, pc.hz, pc.lz);
-            double dr2 = ttc_r2_exact(dx, dy, dz);
-            if (dr2 > pc.rc2) continue;
-            float r2 = (float)dr2;
-            float rinv = __builtin_amdgcn_rsqf(r2);
-            float rinv2 = rinv * rinv;
-            float r = r2 * rinv;
-            float4 lj = ljrow[type[j]];
-            float eps = cls == 0 ? lj.x : (cls == 2 ? lj.z : 0.0f);
-            float sg2 = cls == 2 ? lj.w : lj.y;
-            float sr2 = sg2 * rinv2;
-            float sr6 = sr2 * sr2 * sr2;
-            float sr12 = sr6 * sr6;
-            float force = 12.0f * eps * rinv2 * (sr12 - sr6);
-            float qqab = qi * (float)pj.w * pc.qfac;
-            float qq = (cls == 0 ? qqab : (cls == 2 ? pc.e14 * qqab : 0.0f)) * pc.inv_epsrf;
-            float ez2;
-            float erfckr = nr_erfc(pc.kappa * r, &ez2);
-            float kpid = pc.kpi2 * r * ez2;
-            float rinv3 = rinv * rinv2;
-            force += qq * rinv3 * (erfckr + kpid);
-            float qi2 = cls == 2 ? pc.one_m_e14 * qqab : qqab;
-            if (cls) force -= qi2 * rinv3 * ((1.0f - erfckr) - kpid);
-            float fxf = force * (float)dx, fyf = force * (float)dy, fzf = force * (float)dz;
-            fx += (double)fxf;
-            fy += (double)fyf;
-            fz += (double)fzf;
-            if (ENERGY) {
-                float sr2s = sg2 / pc.rc2f;
-                float sr6s = sr2s * sr2s * sr2s;
-                elj += (double)(eps * (sr12 - 2.0f * sr6));
-                eljs += (double)(eps * (sr6s * sr6s - 2.0f * sr6s));
-                er += (double)(qq * erfckr * rinv);
-                ers += (double)qq * pc.erfc_rc_over_rc;
-                if (cls) ei += (double)(qi2 * (1.0f - erfckr) * rinv);
-                w0 += (double)(fxf * (float)dx);
-                w1 += (double)(fyf * (float)dx);
-                w2 += (double)(fyf * (float)dy);
-                w3 += (double)(fzf * (float)dx);
-                w4 += (double)(fzf * (float)dy);
-                w5 += (double)(fzf * (float)dz);
-            }
-        }
-        if (s < n) fpair[s] = make_double4(fx, fy, fz, 0.0);
+    int grp = blockIdx.x;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int s = grp * 64 + lane;
+    PairAcc a = {};
+    int len = glen[grp];
+    double4 pi = s < n ? pos4[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+    int ti = s < n ? type[s] : 0;
+    const float4 *ljrow = slj + ti * pc.ntypes;
+    float qi = (float)pi.w;
+    const uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
+    int k = wave;
+    for (; k + PAIR_WAVES < len; k += 2 * PAIR_WAVES) {
+        uint32_t e0 = row[(size_t)k * 64];
+        uint32_t e1 = row[(size_t)(k + PAIR_WAVES) * 64];
+        pair_entry<ENERGY>(e0, pi, qi, ljrow, pos4, type, pc, a);
+        pair_entry<ENERGY>(e1, pi, qi, ljrow, pos4, type, pc, a);
     }
+    if (k < len) pair_entry<ENERGY>(row[(size_t)k * 64], pi, qi, ljrow, pos4, type, pc, a);
+
+    sred[wave][lane][0] = a.fx;
+    sred[wave][lane][1] = a.fy;
+    sred[wave][lane][2] = a.fz;
     if (ENERGY) {
-        // each pair was visited from both of its sites: halve
-        double v[11] = {elj, eljs, er, ers, ei, w0, w1, w2, w3, w4, w5};
-        __shared__ double red[4][11];
-        int wid = threadIdx.x >> 6;
+        double v[11] = {a.elj, a.eljs, a.er, a.ers, a.ei, a.w0, a.w1, a.w2, a.w3, a.w4, a.w5};
         for (int q = 0; q < 11; q++) {
             double t = ttc_wave_sum(v[q]);
-            if (lane == 0) red[wid][q] = t;
+            if (lane == 0) ered[wave][q] = t;
         }
-        __syncthreads();
-        if (threadIdx.x < 11) {
-            double t = 0.5 * (red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
-            partials[(size_t)(EV_LJ + threadIdx.x) * maxb + blockIdx.x] = t;
+    }
+    __syncthreads();
+    if (wave == 0 && s < n) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int w = 0; w < PAIR_WAVES; w++) {
+            fx += sred[w][lane][0];
+            fy += sred[w][lane][1];
+            fz += sred[w][lane][2];
         }
+        fpair[s] = make_double4(fx, fy, fz, 0.0);
+    }
+    if (ENERGY && threadIdx.x < 11) {
+        // each pair was visited from both of its sites: halve
+        double t = 0.0;
+        for (int w = 0; w < PAIR_WAVES; w++) t += ered[w][threadIdx.x];
+        partials[(size_t)(EV_LJ + threadIdx.x) * maxb + blockIdx.x] = 0.5 * t;
     }
 }
 
@@ -131,7 +165,7 @@ k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__
 
 extern double ttc_nr_erfc_host(double x);
 
-int ttc_pair_blocks(ttc_ctx *c) { return (c->n + 255) / 256; }
+int ttc_pair_blocks(ttc_ctx *c) { return c->G; }
 
 void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
     PairConst pc;
@@ -150,9 +184,9 @@ void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
     int blocks = ttc_pair_blocks(c);
     size_t shm = sizeof(float4) * c->ntypes * c->ntypes;
     if (energy)
-        k_pair<true><<<blocks, 256, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, c->ljtab,
+        k_pair<true><<<blocks, 64 * PAIR_WAVES, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, c->ljtab,
                                                       pc, c->fpair, c->partials, c->nblk_max);
     else
-        k_pair<false><<<blocks, 256, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, c->ljtab,
+        k_pair<false><<<blocks, 64 * PAIR_WAVES, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, c->ljtab,
                                                        pc, c->fpair, c->partials, c->nblk_max);
 }

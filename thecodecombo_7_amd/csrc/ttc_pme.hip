@@ -19,6 +19,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 #define FFT_LB 16
 #define FFT_THREADS 256
 
@@ -122,29 +124,124 @@ __device__ __forceinline__ double pbc1(double x, double l) {
     return x;
 }
 
+// Bricks of neighbour cells for the LDS-privatised spread.  A brick's
+// tile covers every grid point its sites can touch: the brick's cells
+// widened by `delta` (sites drift at most (rclist-rc)/4 plus one step from
+// the cell they were sorted into before the trigger rebuilds the list).
+struct Brick {
+    int ncx, ncy, ncz;
+    int bx, by, bz;
+    int nbx, nby, nbz;
+    double cwx, cwy, cwz;
+    double delta;
+};
+
+__device__ __forceinline__ void tile_range(int c0, int c1, double cw, double l, int k, double delta, int P,
+                                           int *org, int *ext) {
+    double xlo = -0.5 * l + c0 * cw - delta, xhi = -0.5 * l + c1 * cw + delta;
+    int glo = (int)floor((double)k * (xlo / l + 0.5));
+    int ghi = (int)floor((double)k * (xhi / l + 0.5));
+    *org = glo - P + 1;
+    *ext = ghi - glo + P;
+}
+
+// local stencil base of a site inside [org, org+ext), trying the periodic
+// images of its grid coordinate; -1 if it lies outside the tile
+__device__ __forceinline__ int tile_base(int fl, int org, int ext, int k, int P) {
+    int b = fl - P + 1 - org;
+    if (b >= 0 && b + P <= ext) return b;
+    b += k;
+    if (b >= 0 && b + P <= ext) return b;
+    b -= 2 * k;
+    if (b >= 0 && b + P <= ext) return b;
+    return -1;
+}
+
+// kick + drift + pbc (integrate_vverlet.c:52-83) of every site of a brick,
+// then charge_grid (pme_qgrid.c:77-113) into an LDS tile of int64 fixed
+// point, flushed to the global fixed-point grid with one atomic per
+// touched grid point.  Integer adds make the grid independent of order.
 template <int P, bool DRIFT>
 __global__ void __launch_bounds__(256)
-k_drift_spread(int n, double4 *pos4, double4 *vel4, const double4 *f4, double dt, double vscal, Spread sp,
-               long long *qfix, int *status) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (DRIFT && s == 0) status[ST_NLFLAG] = 0;   // the rebuild before this kernel consumed it
-    if (s >= n) return;
-    double4 p = pos4[s];
-    if (DRIFT) {
-        double4 v = vel4[s];
-        double4 f = f4[s];
-        double dth = 0.5 * dt;
-        double im = v.w;   // 100/m
-        v.x = vscal * (v.x + dth * im * f.x);
-        v.y = vscal * (v.y + dth * im * f.y);
-        v.z = vscal * (v.z + dth * im * f.z);
-        p.x = pbc1(p.x + dt * v.x, sp.lx);
-        p.y = pbc1(p.y + dt * v.y, sp.ly);
-        p.z = pbc1(p.z + dt * v.z, sp.lz);
-        vel4[s] = v;
-        pos4[s] = p;
+k_drift_spread(double4 *pos4, double4 *vel4, const double4 *f4, double dt, double vscal, Spread sp, Brick br,
+               const int *__restrict__ cell_start, const int *__restrict__ cell_cnt, long long *qfix,
+               int *status) {
+    extern __shared__ unsigned long long tile[];
+    int b = blockIdx.x;
+    int ibx = b % br.nbx, iby = (b / br.nbx) % br.nby, ibz = b / (br.nbx * br.nby);
+    int cx0 = ibx * br.bx, cy0 = iby * br.by, cz0 = ibz * br.bz;
+    int cx1 = min(cx0 + br.bx, br.ncx), cy1 = min(cy0 + br.by, br.ncy), cz1 = min(cz0 + br.bz, br.ncz);
+    int ox, oy, oz, tx, ty, tz;
+    tile_range(cx0, cx1, br.cwx, sp.lx, sp.K0, br.delta, P, &ox, &tx);
+    tile_range(cy0, cy1, br.cwy, sp.ly, sp.K1, br.delta, P, &oy, &ty);
+    tile_range(cz0, cz1, br.cwz, sp.lz, sp.K2, br.delta, P, &oz, &tz);
+    int tsz = tx * ty * tz;
+    for (int t = threadIdx.x; t < tsz; t += blockDim.x) tile[t] = 0ull;
+    if (DRIFT && b == 0 && threadIdx.x == 0) status[ST_NLFLAG] = 0;   // consumed by the rebuild
+    __syncthreads();
+    for (int cz = cz0; cz < cz1; cz++)
+        for (int cy = cy0; cy < cy1; cy++) {
+            int c0 = cx0 + br.ncx * (cy + br.ncy * cz);
+            int cl = (cx1 - 1) + br.ncx * (cy + br.ncy * cz);
+            int s0 = cell_start[c0], s1 = cell_start[cl] + cell_cnt[cl];
+            for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+                double4 p = pos4[s];
+                if (DRIFT) {
+                    double4 v = vel4[s];
+                    double4 f = f4[s];
+                    double dth = 0.5 * dt;
+                    double im = v.w;   // 100/m
+                    v.x = vscal * (v.x + dth * im * f.x);
+                    v.y = vscal * (v.y + dth * im * f.y);
+                    v.z = vscal * (v.z + dth * im * f.z);
+                    p.x = pbc1(p.x + dt * v.x, sp.lx);
+                    p.y = pbc1(p.y + dt * v.y, sp.ly);
+                    p.z = pbc1(p.z + dt * v.z, sp.lz);
+                    vel4[s] = v;
+                    pos4[s] = p;
+                }
+                double ux = (double)sp.K0 * (p.x / sp.lx + 0.5), uy = (double)sp.K1 * (p.y / sp.ly + 0.5),
+                       uz = (double)sp.K2 * (p.z / sp.lz + 0.5);
+                double fxd = floor(ux), fyd = floor(uy), fzd = floor(uz);
+                int bxl = tile_base((int)fxd, ox, tx, sp.K0, P);
+                int byl = tile_base((int)fyd, oy, ty, sp.K1, P);
+                int bzl = tile_base((int)fzd, oz, tz, sp.K2, P);
+                if (bxl < 0 || byl < 0 || bzl < 0) {
+                    spread_site<P>(p, sp, qfix);   // outside the tile: straight to the grid
+                    continue;
+                }
+                float Mx[P + 1], My[P + 1], Mz[P + 1], D[P + 1];
+                bspline<P>((float)(ux - fxd), Mx, D);
+                bspline<P>((float)(uy - fyd), My, D);
+                bspline<P>((float)(uz - fzd), Mz, D);
+                float q = (float)p.w;
+#pragma unroll
+                for (int xi = 0; xi < P; xi++) {
+                    float qx = q * Mx[xi + 1];
+#pragma unroll
+                    for (int yi = 0; yi < P; yi++) {
+                        float qxy = qx * My[yi + 1];
+                        int rowb = ((bxl + xi) * ty + (byl + yi)) * tz + bzl;
+#pragma unroll
+                        for (int zi = 0; zi < P; zi++) {
+                            long long v = __double2ll_rn((double)(qxy * Mz[zi + 1]) * TTC_QFIX_SCALE);
+                            atomicAdd(&tile[rowb + zi], (unsigned long long)v);
+                        }
+                    }
+                }
+            }
+        }
+    __syncthreads();
+    for (int t = threadIdx.x; t < tsz; t += blockDim.x) {
+        unsigned long long v = tile[t];
+        if (!v) continue;
+        int lz = t % tz, ly = (t / tz) % ty, lx = t / (tz * ty);
+        int gx = (ox + lx) % sp.K0, gy = (oy + ly) % sp.K1, gz = (oz + lz) % sp.K2;
+        if (gx < 0) gx += sp.K0;
+        if (gy < 0) gy += sp.K1;
+        if (gz < 0) gz += sp.K2;
+        atomicAdd((unsigned long long *)&qfix[((size_t)gx * sp.K1 + gy) * sp.K2 + gz], v);
     }
-    spread_site<P>(p, sp, qfix);
 }
 
 struct FftArgs {
@@ -534,6 +631,8 @@ static int make_plan(FftPlan &pl, int n) {
     return TTC_OK;
 }
 
+static int set_lds_attrs(ttc_ctx *c);
+
 int ttc_pme_setup(ttc_ctx *c) {
     c->K[0] = c->p.gx; c->K[1] = c->p.gy; c->K[2] = c->p.gz;
     c->ngrid = (size_t)c->K[0] * c->K[1] * c->K[2];
@@ -552,6 +651,9 @@ int ttc_pme_setup(ttc_ctx *c) {
     return TTC_OK;
 }
 
+int ttc_pme_attrs(ttc_ctx *c) { return set_lds_attrs(c);
+}
+
 void ttc_pme_free(ttc_ctx *c) {
     hipFree(c->qfix); hipFree(c->qc); hipFree(c->qr); hipFree(c->eterm);
     for (int a = 0; a < 3; a++) hipFree(c->plan[a].tw);
@@ -567,23 +669,67 @@ static Spread spread_args(const ttc_ctx *c) {
     return sp;
 }
 
-// spread only (ttc_pme_calc / ttc_cnonbondnl): positions unchanged
-void ttc_launch_spread(ttc_ctx *c) {
-    int blocks = (c->n + 255) / 256;
-    Spread sp = spread_args(c);
-    if (c->order == 4)
-        k_drift_spread<4, false><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->vel4, c->f4, 0.0, 1.0, sp, c->qfix, c->status);
-    else
-        k_drift_spread<6, false><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->vel4, c->f4, 0.0, 1.0, sp, c->qfix, c->status);
+static Brick brick_args(const ttc_ctx *c) {
+    Brick br;
+    br.ncx = c->ncx; br.ncy = c->ncy; br.ncz = c->ncz;
+    br.bx = c->ncx >= 2 ? 2 : 1; br.by = c->ncy >= 2 ? 2 : 1; br.bz = c->ncz >= 2 ? 2 : 1;
+    br.nbx = (c->ncx + br.bx - 1) / br.bx; br.nby = (c->ncy + br.by - 1) / br.by; br.nbz = (c->ncz + br.bz - 1) / br.bz;
+    br.cwx = c->p.lx / c->ncx; br.cwy = c->p.ly / c->ncy; br.cwz = c->p.lz / c->ncz;
+    // the trigger fires at (rclist-rc)/4; one more step moves a site far less
+    br.delta = 0.25 * (c->p.rclist - c->p.rc) + 0.25;
+    return br;
 }
 
-void ttc_launch_drift(ttc_ctx *c, double vscal) {
-    int blocks = (c->n + 255) / 256;
+// exact largest tile over all bricks, by the kernel's own formula
+static size_t brick_shm(const ttc_ctx *c, const Brick &br) {
+    auto ext = [&](int nc, int bc, double cw, double l, int k) {
+        int mx = 0;
+        for (int c0 = 0; c0 < nc; c0 += bc) {
+            int c1 = std::min(c0 + bc, nc);
+            double xlo = -0.5 * l + c0 * cw - br.delta, xhi = -0.5 * l + c1 * cw + br.delta;
+            int glo = (int)floor((double)k * (xlo / l + 0.5));
+            int ghi = (int)floor((double)k * (xhi / l + 0.5));
+            mx = std::max(mx, ghi - glo + c->order);
+        }
+        return mx;
+    };
+    size_t t = (size_t)ext(br.ncx, br.bx, br.cwx, c->p.lx, c->K[0]) * ext(br.ncy, br.by, br.cwy, c->p.ly, c->K[1]) *
+               ext(br.ncz, br.bz, br.cwz, c->p.lz, c->K[2]);
+    return t * sizeof(unsigned long long);
+}
+
+static void launch_drift_spread(ttc_ctx *c, bool drift, double vscal) {
     Spread sp = spread_args(c);
-    if (c->order == 4)
-        k_drift_spread<4, true><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->vel4, c->f4, c->p.dt, vscal, sp, c->qfix, c->status);
-    else
-        k_drift_spread<6, true><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->vel4, c->f4, c->p.dt, vscal, sp, c->qfix, c->status);
+    Brick br = brick_args(c);
+    int blocks = br.nbx * br.nby * br.nbz;
+    size_t shm = brick_shm(c, br);
+    double dt = drift ? c->p.dt : 0.0;
+#define DS(P, D)                                                                                         \
+    k_drift_spread<P, D><<<blocks, 256, shm, c->stream>>>(c->pos4, c->vel4, c->f4, dt, vscal, sp, br,   \
+                                                          c->cell_start, c->cell_cnt, c->qfix, c->status)
+    if (c->order == 4) { if (drift) DS(4, true); else DS(4, false); }
+    else { if (drift) DS(6, true); else DS(6, false); }
+#undef DS
+}
+
+// spread only (ttc_pme_calc / ttc_cnonbondnl): positions unchanged
+void ttc_launch_spread(ttc_ctx *c) { launch_drift_spread(c, false, 1.0); }
+
+void ttc_launch_drift(ttc_ctx *c, double vscal) { launch_drift_spread(c, true, vscal); }
+
+int ttc_spread_shm(const ttc_ctx *c) { return (int)brick_shm(c, brick_args(c)); }
+
+// allow the spread tile and the FFT tiles above 64 KiB of LDS (gfx950 has 160 KiB per CU)
+static int set_lds_attrs(ttc_ctx *c) {
+    int shm = (int)brick_shm(c, brick_args(c));
+    if (shm > 160 * 1024) {
+        ttc_set_error("PME spread tile exceeds the LDS of one CU");
+        return TTC_ERR_ARG;
+    }
+    const void *fns[] = {(const void *)k_drift_spread<4, true>, (const void *)k_drift_spread<4, false>,
+                         (const void *)k_drift_spread<6, true>, (const void *)k_drift_spread<6, false>};
+    for (const void *f : fns) TTC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    return TTC_OK;
 }
 
 static size_t fft_shm(const ttc_ctx *c, int axis) {
