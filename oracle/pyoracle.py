@@ -239,4 +239,5 @@ def ref_lib():
     L.ref_ewald_kspace.argtypes = [_dp, _dp, _dp]
     L.ref_erfc.restype = C.c_double; L.ref_erfc.argtypes = [C.c_double]
     L.ref_self_energy.restype = C.c_double
+    L.ref_ran2_seq.argtypes = [C.c_long, C.c_int, _dp]
     return L

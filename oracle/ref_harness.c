@@ -217,3 +217,12 @@ void ref_ewald_kspace(const double *pos, double *ff_out, double *energy3)
 double ref_erfc(double x) { return erfc(x); }
 
 double ref_self_energy(void) { return en[0].sewald; }
+
+/* random.c:37-75: n successive ran2() values after setting idum = seed
+ * (negative seeds initialise, as main.c:278-287 does) */
+double ran2(void);
+void ref_ran2_seq(long seed, int n, double *out)
+{
+    idum = seed;
+    for (int i = 0; i < n; i++) out[i] = ran2();
+}
