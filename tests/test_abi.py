@@ -45,12 +45,10 @@ def test_exports_are_c_linkage():
     assert set(_declared()) <= exported
 
 
-def test_library_is_gfx950_code_object():
-    out = subprocess.run(["/opt/rocm/llvm/bin/clang-offload-bundler", "--list", "--type=o", f"--input={LIB}"],
-                         capture_output=True, text=True)
-    if out.returncode != 0:
-        pytest.skip("offload bundler cannot list a shared library here")
-    assert "gfx950" in out.stdout
+def test_library_carries_gfx950_code_object():
+    # the embedded offload bundle names its target triple
+    with open(LIB, "rb") as fh:
+        assert b"amdgcn-amd-amdhsa--gfx950" in fh.read()
 
 
 def test_python_mirror_fails_loudly_without_library(tmp_path, monkeypatch):
