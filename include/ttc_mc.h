@@ -1,0 +1,111 @@
+/*
+ * ttc_mc.h -- C ABI of the batched Monte Carlo / density-of-states path
+ * (libttc.so): many C-alpha Go-model walkers of uneven length sampled at
+ * once on the device, Wang-Landau acceptance, exact FP64 energy changes,
+ * and an optional learned first stage (delayed acceptance).
+ *
+ * Reference counterparts (file:line in the reference tree):
+ *   pivot and displacement moves   mcmove_pivot.c:32-283, mcmove_displ.c:33-139
+ *   range energy of a moved part   forces_mc.c:90-300 (o_potens / n_potens),
+ *                                  enbnd_mc.c:26-504 (pairs touching [lb,ub))
+ *   Go energy terms                cgolik.c:644-948, 12-10 form 1202-1229
+ *   Wang-Landau acceptance         pivot.c:236-253, ctdos_pivot.c:246-314
+ *   random numbers                 random.c:37-75 (ran2; here a per-walker
+ *                                  counter-free generator, see ttc_mc_run)
+ * The learned first stage and its training have no reference counterpart.
+ * Error codes are those of ttc.h.
+ */
+#ifndef TTC_MC_H
+#define TTC_MC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ttc_mc ttc_mc;
+
+typedef struct {
+    int    nwalk;        /* walkers                                         */
+    int    maxn;         /* longest chain (beads)                           */
+    int    nbins;        /* Wang-Landau bins per walker                     */
+    double ebin;         /* bin width, kJ/mol                               */
+    double kbond;        /* bond energy kbond (r - r0bond)^2 (cbond.c:103)  */
+    double r0bond;
+    double eps;          /* native 12-10 depth and non-native repulsion     */
+    double sig_rep;      /* non-native repulsion eps (sig_rep/r)^12         */
+    double pivot_frac;   /* probability of a pivot move (else displacement) */
+    double theta_max;    /* pivot angle = u * pi * theta_max                */
+    double disp_max;     /* displacement = (u - 0.5) * disp_max per axis     */
+    uint64_t seed;
+} ttc_mc_params;
+
+#define TTC_MC_NFEAT 36  /* surrogate input features per move */
+#define TTC_MC_HID   64  /* surrogate hidden width            */
+
+/* stats per walker (ttc_mc_get_stats), int64 each */
+#define TTC_MC_STAT_TRIALS   0
+#define TTC_MC_STAT_ACCEPT   1
+#define TTC_MC_STAT_STAGE1_REJECT 2
+#define TTC_MC_STAT_EXACT    3   /* exact energy evaluations */
+#define TTC_MC_STAT_OUTSIDE  4   /* proposals outside the WL window */
+#define TTC_MC_NSTAT 5
+
+int  ttc_mc_create(const ttc_mc_params *p, ttc_mc **out);
+int  ttc_mc_destroy(ttc_mc *m);
+
+/* Walker w: n beads, positions (n*3), native contacts (a<b, b-a>=3) with
+ * native distances, WL window [emin, emin + nbins*ebin).                  */
+int  ttc_mc_set_walker(ttc_mc *m, int w, int n, const double *pos, int nnat, const int *na,
+                       const int *nb, const double *nr0, double emin);
+int  ttc_mc_get_walker(ttc_mc *m, int w, double *pos, double *energy);
+
+/* Exact energy of every walker (bonds + native 12-10 + non-native
+ * repulsion, pairs |i-j| >= 3), FP64: e[nwalk].                          */
+int  ttc_mc_energy(ttc_mc *m, double *e);
+
+/* Range energy change of walker w when beads [lb,ub) move to newpos
+ * ((ub-lb)*3): sum over bonds and pairs touching [lb,ub) of
+ * E(new) - E(old), as forces_mc(lb,ub,1) - forces_mc(lb,ub,0).          */
+int  ttc_mc_delta_energy(ttc_mc *m, int w, int lb, int ub, const double *newpos, double *de);
+
+/* ntrials Monte Carlo trials per walker.  mode 0: plain Wang-Landau
+ * acceptance on the exact energy; mode 1: delayed acceptance, a surrogate
+ * energy change from the network screens each proposal and the exact
+ * energy makes the second-stage decision, so the chain keeps the same
+ * stationary distribution.  lnf is added to ln g of the visited bin after
+ * every trial (0 samples the fixed current g).  log != 0 appends
+ * (features, exact dE) of every exactly evaluated move to the log buffer. */
+int  ttc_mc_run(ttc_mc *m, int ntrials, int mode, double lnf, int log);
+
+int  ttc_mc_get_stats(ttc_mc *m, int64_t *stats /* nwalk * TTC_MC_NSTAT */);
+int  ttc_mc_reset_stats(ttc_mc *m);
+int  ttc_mc_get_dos(ttc_mc *m, int w, double *lng, int64_t *hist);
+int  ttc_mc_set_dos(ttc_mc *m, int w, const double *lng);
+int  ttc_mc_reset_hist(ttc_mc *m);
+/* copies up to max logged moves: features (max*NFEAT), exact dE (max) */
+int  ttc_mc_get_log(ttc_mc *m, int max, float *features, double *de, int *count);
+int  ttc_mc_clear_log(ttc_mc *m);
+/* Device time of the last ttc_mc_run, microseconds. */
+int  ttc_mc_last_time(ttc_mc *m, double *usec);
+
+/* ---- surrogate network: in=NFEAT -> HID tanh -> HID tanh -> 1 ----------
+ * Parameters packed W1[HID*in], b1[HID], W2[HID*HID], b2[HID], W3[HID], b3.
+ * Inputs are rounded to bfloat16, all arithmetic accumulates in FP32.     */
+int  ttc_mlp_nparams(void);
+int  ttc_mc_set_mlp(ttc_mc *m, const float *params);
+int  ttc_mc_get_mlp(ttc_mc *m, float *params);
+/* forward pass of the current network on nsamp feature rows -> out */
+int  ttc_mlp_forward(ttc_mc *m, int nsamp, const float *x, float *out);
+/* loss 0.5*mean((f(x)-y)^2) and its gradient (FP32 accumulation) */
+int  ttc_mlp_loss_grad(ttc_mc *m, int nsamp, const float *x, const float *y, double *loss, float *grad);
+/* Adam training on the logged moves: targets y = asinh(dE / escale) of the
+ * antisymmetric output g(old,new) - g(new,old); nepoch passes, minibatch
+ * batch, learning rate lr.  Returns the final mean loss.                  */
+int  ttc_mlp_train(ttc_mc *m, int nepoch, int batch, float lr, double escale, double *loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,161 @@
+"""GPU parity of the batched Go-model Monte Carlo path against the oracle.
+
+  * full energy and range energy changes: FP64 on both sides, agreement to
+    1e-9 of the magnitude of the terms summed;
+  * bookkeeping: the running energy equals a fresh full evaluation after a
+    run, pivots keep bond lengths, Wang-Landau counters add up;
+  * delayed acceptance keeps the sampled distribution: with a fixed ln g
+    (Boltzmann weights) plain and delayed-acceptance chains give the same
+    mean energy within their statistical error, even with an untrained
+    surrogate;
+  * the surrogate network's FP32 gradient matches the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_energy(ch, pos):
+    import pyoracle
+    from thecodecombo_7_amd import system
+    L = pyoracle.lib()
+    return L.orc_go_energy(ch.n, np.ascontiguousarray(pos, dtype=np.float64).ravel(), system.GO_KBOND,
+                           system.GO_BOND, len(ch.r0), np.ascontiguousarray(ch.contacts[:, 0]),
+                           np.ascontiguousarray(ch.contacts[:, 1]), np.ascontiguousarray(ch.r0),
+                           system.GO_EPS, system.GO_SIG_REP, 1e30)
+
+
+@pytest.fixture(scope="module")
+def walkers():
+    from thecodecombo_7_amd import system
+    from thecodecombo_7_amd.mc import Walkers
+    rng = np.random.default_rng(11)
+    chains = [system.make_go_chain(int(n), rng) for n in rng.integers(20, 90, 16)]
+    print("case: 16 Go walkers 20-90 beads", flush=True)
+    w = Walkers(chains, seed=5)
+    yield w
+    w.close()
+
+
+def test_full_energy_matches_oracle(walkers):
+    e = walkers.energy()
+    for w, ch in enumerate(walkers.chains):
+        ref = _oracle_energy(ch, ch.pos)
+        assert abs(e[w] - ref) <= 1e-10 * max(1.0, abs(ref))
+
+
+def test_delta_energy_matches_oracle(walkers):
+    import pyoracle
+    from thecodecombo_7_amd import system
+    L = pyoracle.lib()
+    rng = np.random.default_rng(3)
+    for w in (0, 5, 11):
+        ch = walkers.chains[w]
+        pos, _ = walkers.walker(w)
+        n = ch.n
+        nat = np.zeros((n, n), np.uint8); r0 = np.zeros((n, n))
+        for (a, b), r in zip(ch.contacts, ch.r0):
+            nat[a, b] = nat[b, a] = 1
+            r0[a, b] = r0[b, a] = r
+        for lb, ub in ((3, 4), (0, n // 2), (n // 3, n)):
+            newp = pos.copy()
+            newp[lb:ub] += rng.uniform(-0.4, 0.4, (ub - lb, 3))
+            rng_e = lambda p: L.orc_go_range_energy(n, np.ascontiguousarray(p).ravel(), lb, ub, system.GO_KBOND,
+                                                    system.GO_BOND, nat.ravel(), r0.ravel(), system.GO_EPS,
+                                                    system.GO_SIG_REP, 1e30)
+            ref = rng_e(newp) - rng_e(pos)
+            got = walkers.delta_energy(w, lb, ub, newp[lb:ub])
+            assert abs(got - ref) <= 1e-9 * max(1.0, abs(rng_e(pos)) + abs(ref)), (w, lb, ub, got, ref)
+
+
+def test_plain_run_bookkeeping(walkers):
+    walkers.reset_stats()
+    walkers.reset_hist()
+    ntr = 400
+    walkers.run(ntr, mode=0, lnf=0.01)
+    st = walkers.stats()
+    assert (st[:, 0] == ntr).all()
+    assert (st[:, 1] > 0).all() and (st[:, 1] <= ntr).all()
+    e_dev = walkers.energy()
+    for w, ch in enumerate(walkers.chains):
+        pos, ecur = walkers.walker(w)
+        ref = _oracle_energy(ch, pos)
+        assert abs(ecur - ref) <= 1e-8 * max(1.0, abs(ref))
+        assert abs(e_dev[w] - ref) <= 1e-10 * max(1.0, abs(ref))
+        bl = np.linalg.norm(np.diff(pos, axis=0), axis=1)
+        # pivots keep bonds rigid; displacements stretch them only by the step
+        assert bl.min() > 2.5 and bl.max() < 5.2
+        lng, hist = walkers.dos(w)
+        assert hist.sum() + st[w, 4] * 0 <= ntr
+        assert abs(lng.sum() - 0.01 * hist.sum()) < 1e-9
+
+
+def _mean_energy_fixed_g(mode, seed):
+    from thecodecombo_7_amd import system
+    from thecodecombo_7_amd.mc import Walkers
+    rng = np.random.default_rng(21)
+    chains = [system.make_go_chain(14, rng) for _ in range(128)]
+    w = Walkers(chains, seed=seed, nbins=256, window_below=5.0, disp_max=0.8, theta_max=0.3)
+    kT = 2.0 * 2.494   # 600 K in kJ/mol
+    for k in range(w.nwalk):
+        centres = w.emin[k] + (np.arange(w.nbins) + 0.5) * w.ebin
+        w.set_dos(k, centres / kT)      # pi(x) ~ exp(-E/kT), binned
+    if mode == 1:
+        rs = np.random.default_rng(seed)
+        w.set_mlp(rs.uniform(-0.4, 0.4, w.nparams()).astype(np.float32))
+    w.run(3000, mode=mode, lnf=0.0)     # equilibrate
+    w.reset_hist()
+    w.run(20000, mode=mode, lnf=0.0)
+    e = []
+    for k in range(w.nwalk):
+        _, h = w.dos(k)
+        centres = w.emin[k] + (np.arange(w.nbins) + 0.5) * w.ebin
+        e.append((h * (centres - w.emin[k])).sum() / h.sum())
+    st = w.stats()
+    w.close()
+    e = np.array(e)
+    return e.mean(), e.std() / np.sqrt(len(e)), st
+
+
+def test_delayed_acceptance_keeps_distribution():
+    m0, s0, st0 = _mean_energy_fixed_g(0, 101)
+    m1, s1, st1 = _mean_energy_fixed_g(1, 202)
+    print("plain", m0, s0, "DA", m1, s1, "stage-1 rejects", st1[:, 2].sum() / st1[:, 0].sum())
+    assert st1[:, 2].sum() > 0          # the first stage did screen moves
+    assert abs(m0 - m1) <= 4.0 * np.hypot(s0, s1) + 1e-3
+
+
+def test_mlp_gradient_matches_oracle(walkers):
+    import pyoracle
+    L = pyoracle.lib()
+    nin, H = 36, 64
+    npar = L.orc_mlp_nparams(nin, H)
+    assert npar == walkers.nparams()
+    rng = np.random.default_rng(9)
+    p = (rng.standard_normal(npar) * 0.2).astype(np.float32)
+    # bfloat16-exact inputs so both sides see the same x
+    x = rng.standard_normal((70, nin)).astype(np.float32)
+    x = (x.view(np.uint32) & 0xFFFF0000).view(np.float32)
+    y = rng.standard_normal(70).astype(np.float32)
+    walkers.set_mlp(p)
+    loss, g = walkers.mlp_loss_grad(x, y)
+    gref = np.zeros(npar)
+    lref = L.orc_mlp_loss_grad(nin, H, p.astype(np.float64), 70, x.astype(np.float64).ravel(),
+                               y.astype(np.float64), gref)
+    assert abs(loss - lref) <= 1e-5 * abs(lref)
+    assert np.sqrt(((g - gref) ** 2).sum() / (gref ** 2).sum()) < 1e-4
+    out = walkers.mlp_forward(x[:5])
+    ref = [L.orc_mlp_forward(nin, H, p.astype(np.float64), x[i].astype(np.float64)) for i in range(5)]
+    assert np.allclose(out, ref, rtol=1e-5, atol=1e-5)
+
+
+def test_surrogate_training_reduces_loss(walkers):
+    walkers.clear_log()
+    walkers.run(1500, mode=0, lnf=0.0, log=True)
+    x, de = walkers.log()
+    assert len(x) > 1000 and np.isfinite(de).all()
+    l0 = walkers.train_surrogate(nepoch=1, lr=1e-3)
+    l1 = walkers.train_surrogate(nepoch=4, lr=1e-3)
+    print("surrogate loss", l0, "->", l1)
+    assert l1 < l0

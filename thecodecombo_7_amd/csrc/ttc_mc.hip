@@ -1,0 +1,946 @@
+// ttc_mc.hip -- batched Go-model Monte Carlo with Wang-Landau acceptance and
+// a learned delayed-acceptance first stage (include/ttc_mc.h).
+//
+// One workgroup per walker runs every trial of its walker: the chain lives
+// in LDS, proposals are drawn by one lane, the exact FP64 energy change is a
+// block-wide sum over the bonds and pairs touching the moved beads (the
+// range energy of forces_mc.c:90-300 / enbnd_mc.c:26-504), reduced in a fixed
+// order, and the Wang-Landau bookkeeping (pivot.c:236-253) stays in LDS.
+// The surrogate network (in -> 64 tanh -> 64 tanh -> 1) runs in the same
+// workgroup with its weights in LDS; its training is a single-workgroup
+// Adam loop over the logged moves.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ttc.h"
+#include "../../include/ttc_mc.h"
+
+void ttc_set_error(const std::string &msg);
+
+#define MC_THREADS 512
+#define MC_WAVES (MC_THREADS / 64)
+#define NF TTC_MC_NFEAT
+#define HID TTC_MC_HID
+#define NPAR (HID * NF + HID + HID * HID + HID + HID + 1)
+#define MC_PI 3.141592653589793238462643383
+
+#define HIPCHK(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            ttc_set_error(std::string(#call) + ": " + hipGetErrorString(e_));        \
+            return TTC_ERR_HIP;                                                      \
+        }                                                                            \
+    } while (0)
+
+struct ttc_mc {
+    ttc_mc_params p;
+    double *pos = nullptr;      // [nwalk][maxn][3]
+    int *nbead = nullptr;       // [nwalk]
+    double *r0 = nullptr;       // [nwalk][maxn][maxn] native distance, 0 = non-native
+    double *emin = nullptr, *ecur = nullptr;
+    double *lng = nullptr;      // [nwalk][nbins]
+    long long *hist = nullptr;  // [nwalk][nbins]
+    uint64_t *rng = nullptr;    // [nwalk][4]
+    long long *stats = nullptr; // [nwalk][TTC_MC_NSTAT]
+    float *mlp = nullptr;       // [NPAR]
+    float *adam = nullptr;      // [2*NPAR]
+    int adam_t = 0;
+    float *logx = nullptr;      // [logcap][NF]
+    double *logde = nullptr;    // [logcap]
+    int *logn = nullptr;
+    int logcap = 0;
+    hipStream_t stream = nullptr;
+    std::vector<int> hn;        // host copy of bead counts
+    double t_last = 0.0;
+};
+
+// ---------------------------------------------------------------- device
+
+// xoshiro256** (Blackman and Vigna), one state per walker
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+__device__ __forceinline__ uint64_t xo_next(uint64_t *s) {
+    uint64_t r = rotl64(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl64(s[3], 45);
+    return r;
+}
+__device__ __forceinline__ double xo_uniform(uint64_t *s) { return (double)(xo_next(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+// Go pair energy (cgolik.c:1209-1229 form for natives, eps (sig/r)^12 else)
+__device__ __forceinline__ double go_pair(double r2, double r0, double eps, double sig2) {
+    double s2 = (r0 > 0.0 ? r0 * r0 : sig2) / r2;
+    double s6 = s2 * s2 * s2;
+    double s12 = s6 * s6;
+    if (r0 > 0.0) return eps * (5.0 * s12 - 6.0 * s12 / s2);
+    return eps * s12;
+}
+
+__device__ __forceinline__ float bf16_round(float x) {
+    uint32_t u = __float_as_uint(x);
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7FFFu + lsb;
+    return __uint_as_float(u & 0xFFFF0000u);
+}
+
+// pairs term of the moved-feature transform
+__device__ __forceinline__ float feat_term(double r2, double sig2) {
+    double s = sig2 / r2;
+    double v = s * s * s;
+    return (float)(v < 16.0 ? v : 16.0);
+}
+
+struct McArgs {
+    int nwalk, maxn, nbins;
+    double ebin, kbond, r0bond, eps, sig2, pivot_frac, theta_max, disp_max;
+};
+
+// block-wide FP64 sum in a fixed order
+__device__ double block_sum(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < MC_WAVES; i++) t += red[i];
+    return t;
+}
+
+// Energy change of walker when beads [s,e) take newp[], all other beads fixed:
+// sum over bonds (i,i+1) and pairs |a-b|>=3 touching [s,e) of E(new)-E(old).
+__device__ double range_delta(const double *pos, const double *newp, int n, int s, int e, const double *r0row,
+                              int maxn, const McArgs &a, double *red) {
+    double acc = 0.0;
+    int m = e - s;
+    long long npair = (long long)m * n;
+    for (long long t = threadIdx.x; t < npair; t += blockDim.x) {
+        int ia = s + (int)(t / n), b = (int)(t % n);
+        int d = ia - b;
+        if (d < 0) d = -d;
+        bool bin = (b >= s && b < e);
+        if (d < 3 || (bin && b < ia)) continue;   // bonded neighbours; intra pairs once
+        double ax = pos[3 * ia], ay = pos[3 * ia + 1], az = pos[3 * ia + 2];
+        double nx = newp[3 * ia], ny = newp[3 * ia + 1], nz = newp[3 * ia + 2];
+        double bx = pos[3 * b], by = pos[3 * b + 1], bz = pos[3 * b + 2];
+        double cx = bin ? newp[3 * b] : bx, cy = bin ? newp[3 * b + 1] : by, cz = bin ? newp[3 * b + 2] : bz;
+        double dx = ax - bx, dy = ay - by, dz = az - bz;
+        double ex = nx - cx, ey = ny - cy, ez = nz - cz;
+        double r0 = r0row[(size_t)ia * maxn + b];
+        acc += go_pair(ex * ex + ey * ey + ez * ez, r0, a.eps, a.sig2) - go_pair(dx * dx + dy * dy + dz * dz, r0, a.eps, a.sig2);
+    }
+    // bonds (cbond.c:103-104 form) with at least one end in [s,e)
+    int blo = s > 0 ? s - 1 : 0, bhi = e < n ? e : n - 1;
+    for (int i = blo + (int)threadIdx.x; i < bhi; i += blockDim.x) {
+        int j = i + 1;
+        bool im = i >= s && i < e, jm = j >= s && j < e;
+        if (!im && !jm) continue;
+        double ox = pos[3 * j] - pos[3 * i], oy = pos[3 * j + 1] - pos[3 * i + 1], oz = pos[3 * j + 2] - pos[3 * i + 2];
+        const double *pi = im ? newp + 3 * i : pos + 3 * i;
+        const double *pj = jm ? newp + 3 * j : pos + 3 * j;
+        double qx = pj[0] - pi[0], qy = pj[1] - pi[1], qz = pj[2] - pi[2];
+        double ro = sqrt(ox * ox + oy * oy + oz * oz) - a.r0bond;
+        double rn = sqrt(qx * qx + qy * qy + qz * qz) - a.r0bond;
+        acc += a.kbond * rn * rn - a.kbond * ro * ro;
+    }
+    return block_sum(acc, red);
+}
+
+__device__ double full_energy(const double *pos, int n, const double *r0row, int maxn, const McArgs &a, double *red) {
+    double acc = 0.0;
+    long long npair = (long long)n * n;
+    for (long long t = threadIdx.x; t < npair; t += blockDim.x) {
+        int ia = (int)(t / n), b = (int)(t % n);
+        if (b < ia + 3) continue;
+        double dx = pos[3 * ia] - pos[3 * b], dy = pos[3 * ia + 1] - pos[3 * b + 1], dz = pos[3 * ia + 2] - pos[3 * b + 2];
+        acc += go_pair(dx * dx + dy * dy + dz * dz, r0row[(size_t)ia * maxn + b], a.eps, a.sig2);
+    }
+    for (int i = threadIdx.x; i < n - 1; i += blockDim.x) {
+        double dx = pos[3 * (i + 1)] - pos[3 * i], dy = pos[3 * (i + 1) + 1] - pos[3 * i + 1], dz = pos[3 * (i + 1) + 2] - pos[3 * i + 2];
+        double r = sqrt(dx * dx + dy * dy + dz * dz) - a.r0bond;
+        acc += a.kbond * r * r;
+    }
+    return block_sum(acc, red);
+}
+
+// network forward of two feature rows (x, swapped x) -> out[2]; weights in LDS
+__device__ void mlp2(const float *W, const float *x0, const float *x1, float *h1, float *h2, float *out, float *fred) {
+    const float *W1 = W, *b1 = W1 + HID * NF, *W2 = b1 + HID, *b2 = W2 + HID * HID, *W3 = b2 + HID, *b3 = W3 + HID;
+    int t = threadIdx.x;
+    if (t < 2 * HID) {
+        int r = t / HID, j = t % HID;
+        const float *x = r ? x1 : x0;
+        float s = b1[j];
+        for (int i = 0; i < NF; i++) s += W1[j * NF + i] * x[i];
+        h1[r * HID + j] = tanhf(s);
+    }
+    __syncthreads();
+    if (t < 2 * HID) {
+        int r = t / HID, j = t % HID;
+        float s = b2[j];
+        for (int i = 0; i < HID; i++) s += W2[j * HID + i] * h1[r * HID + i];
+        h2[r * HID + j] = tanhf(s);
+    }
+    __syncthreads();
+    if (t < 2 * HID) {
+        int r = t / HID, j = t % HID;
+        float v = W3[j] * h2[r * HID + j];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (j == 0) fred[r] = v + b3[0];
+    }
+    __syncthreads();
+    out[0] = fred[0];
+    out[1] = fred[1];
+}
+
+template <bool DA>
+__global__ void __launch_bounds__(MC_THREADS)
+k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int *gn, const double *gr0,
+         const double *gemin, double *gecur, double *glng, long long *ghist, uint64_t *grng, long long *gstats,
+         const float *gmlp, float *logx, double *logde, int *logn, int logcap, double escale) {
+    extern __shared__ double smem[];
+    int w = blockIdx.x;
+    int n = gn[w];
+    if (n <= 0) return;
+    double *pos = smem;                       // 3*maxn
+    double *newp = pos + 3 * a.maxn;          // 3*maxn
+    double *lng = newp + 3 * a.maxn;          // nbins
+    long long *hist = (long long *)(lng + a.nbins);
+    double *red = (double *)(hist + a.nbins); // MC_WAVES
+    float *W = (float *)(red + MC_WAVES);     // NPAR
+    float *feat = W + (DA ? NPAR : 0);        // 2*NF
+    float *h1 = feat + 2 * NF;                // 2*HID
+    float *h2 = h1 + 2 * HID;
+    float *fred = h2 + 2 * HID;               // 2
+    __shared__ int mv_s, mv_e, mv_type, mv_decide;
+    __shared__ double mv_de, mv_dee;
+    __shared__ uint64_t rs[4];
+
+    const double *r0row = gr0 + (size_t)w * a.maxn * a.maxn;
+    for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) pos[i] = gpos[(size_t)w * 3 * a.maxn + i];
+    for (int i = threadIdx.x; i < a.nbins; i += blockDim.x) {
+        lng[i] = glng[(size_t)w * a.nbins + i];
+        hist[i] = ghist[(size_t)w * a.nbins + i];
+    }
+    if (DA)
+        for (int i = threadIdx.x; i < NPAR; i += blockDim.x) W[i] = gmlp[i];
+    if (threadIdx.x < 4) rs[threadIdx.x] = grng[4 * w + threadIdx.x];
+    __syncthreads();
+    double emin = gemin[w];
+    double ecur = gecur[w];
+    long long st_tr = 0, st_acc = 0, st_s1 = 0, st_ex = 0, st_out = 0;
+
+    for (int trial = 0; trial < ntrials; trial++) {
+        // ---- proposal (mcmove_pivot.c:54-140, mcmove_displ.c:45-99)
+        double rot[9];
+        int piv = 0;
+        double dvec[3] = {0.0, 0.0, 0.0};
+        if (threadIdx.x == 0) {
+            uint64_t s4[4] = {rs[0], rs[1], rs[2], rs[3]};
+            double u = xo_uniform(s4);
+            if (u < a.pivot_frac && n > 2) {
+                int p = (int)(xo_uniform(s4) * n);
+                int axis = (int)(xo_uniform(s4) * 3.0);
+                int side = (int)(xo_uniform(s4) * 2.0);
+                int dir = (int)(xo_uniform(s4) * 2.0);
+                double th = xo_uniform(s4) * MC_PI * a.theta_max;
+                mv_type = 1;
+                mv_s = side == 0 ? 0 : p + 1;
+                mv_e = side == 0 ? p : n;
+                mv_de = dir ? -th : th;   // signed angle
+                mv_dee = (double)(p * 3 + axis);
+            } else {
+                int k = (int)(xo_uniform(s4) * n);
+                mv_type = 0;
+                mv_s = k;
+                mv_e = k + 1;
+                dvec[0] = (xo_uniform(s4) - 0.5) * a.disp_max;
+                dvec[1] = (xo_uniform(s4) - 0.5) * a.disp_max;
+                dvec[2] = (xo_uniform(s4) - 0.5) * a.disp_max;
+                newp[3 * k] = pos[3 * k] + dvec[0];
+                newp[3 * k + 1] = pos[3 * k + 1] + dvec[1];
+                newp[3 * k + 2] = pos[3 * k + 2] + dvec[2];
+                mv_de = sqrt(dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2]) / a.disp_max;
+            }
+            rs[0] = s4[0]; rs[1] = s4[1]; rs[2] = s4[2]; rs[3] = s4[3];
+        }
+        __syncthreads();
+        int s = mv_s, e = mv_e, type = mv_type;
+        if (type == 1) {
+            // rotation about x, y or z through the pivot bead (PIVOT matrices)
+            int code = (int)mv_dee;
+            piv = code / 3;
+            int axis = code % 3;
+            double th = mv_de, c = cos(th), sn = sin(th);
+            for (int q = 0; q < 9; q++) rot[q] = 0.0;
+            if (axis == 0) { rot[0] = 1; rot[4] = c; rot[5] = -sn; rot[7] = sn; rot[8] = c; }
+            else if (axis == 1) { rot[0] = c; rot[2] = sn; rot[4] = 1; rot[6] = -sn; rot[8] = c; }
+            else { rot[0] = c; rot[1] = sn; rot[3] = -sn; rot[4] = c; rot[8] = 1; }
+            double px = pos[3 * piv], py = pos[3 * piv + 1], pz = pos[3 * piv + 2];
+            for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
+                double x = pos[3 * i] - px, y = pos[3 * i + 1] - py, z = pos[3 * i + 2] - pz;
+                newp[3 * i] = rot[0] * x + rot[1] * y + rot[2] * z + px;
+                newp[3 * i + 1] = rot[3] * x + rot[4] * y + rot[5] * z + py;
+                newp[3 * i + 2] = rot[6] * x + rot[7] * y + rot[8] * z + pz;
+            }
+        }
+        __syncthreads();
+        st_tr++;
+        if (e <= s) {   // empty moved part: nothing changes, counts as a rejected trial
+            if (threadIdx.x == 0) {
+                int b = (int)floor((ecur - emin) / a.ebin);
+                if (b >= 0 && b < a.nbins) { lng[b] += lnf; hist[b]++; }
+            }
+            __syncthreads();
+            continue;
+        }
+        // ---- features of pivot moves (local distances around the pivot)
+        bool need_feat = type == 1 && (DA || do_log);
+        if (need_feat) {
+            int sg = (s == 0) ? -1 : 1;     // moved side direction from the pivot
+            if (threadIdx.x < 16) {
+                int i = threadIdx.x / 4 + 1, j = threadIdx.x % 4 + 1;
+                int mb = piv + sg * i, fb = piv - sg * j;
+                float fo = 0.0f, fn = 0.0f;
+                if (mb >= 0 && mb < n && fb >= 0 && fb < n) {
+                    double ox = pos[3 * mb] - pos[3 * fb], oy = pos[3 * mb + 1] - pos[3 * fb + 1], oz = pos[3 * mb + 2] - pos[3 * fb + 2];
+                    double nx = newp[3 * mb] - pos[3 * fb], ny = newp[3 * mb + 1] - pos[3 * fb + 1], nz = newp[3 * mb + 2] - pos[3 * fb + 2];
+                    fo = feat_term(ox * ox + oy * oy + oz * oz, a.sig2);
+                    fn = feat_term(nx * nx + ny * ny + nz * nz, a.sig2);
+                }
+                feat[threadIdx.x] = bf16_round(fo);
+                feat[16 + threadIdx.x] = bf16_round(fn);
+                feat[NF + threadIdx.x] = bf16_round(fn);        // reverse move: old <-> new
+                feat[NF + 16 + threadIdx.x] = bf16_round(fo);
+            } else if (threadIdx.x < 20) {
+                int q = threadIdx.x - 16;
+                float v = q == 0 ? 1.0f : q == 1 ? (float)(fabs(mv_de) / MC_PI) : q == 2 ? (float)(e - s) / (float)n : 0.0f;
+                v = bf16_round(v);
+                feat[32 + q] = v;
+                feat[NF + 32 + q] = v;
+            }
+            __syncthreads();
+        }
+        int bx = (int)floor((ecur - emin) / a.ebin);
+        double sur = 0.0, a1f = 1.0;
+        if (DA && type == 1) {
+            float o2[2];
+            mlp2(W, feat, feat + NF, h1, h2, o2, fred);
+            sur = escale * sinh((double)(o2[0] - o2[1]));   // antisymmetric surrogate dE
+            if (threadIdx.x == 0) {
+                int by = (int)floor((ecur + sur - emin) / a.ebin);
+                double a1 = 0.0;
+                if (bx >= 0 && bx < a.nbins && by >= 0 && by < a.nbins) a1 = fmin(1.0, exp(lng[bx] - lng[by]));
+                uint64_t s4[4] = {rs[0], rs[1], rs[2], rs[3]};
+                double u1 = xo_uniform(s4);
+                rs[0] = s4[0]; rs[1] = s4[1]; rs[2] = s4[2]; rs[3] = s4[3];
+                mv_decide = u1 < a1 ? 1 : 0;
+                mv_dee = a1;
+            }
+            __syncthreads();
+            a1f = mv_dee;
+            if (!mv_decide) {
+                st_s1++;
+                if (threadIdx.x == 0 && bx >= 0 && bx < a.nbins) { lng[bx] += lnf; hist[bx]++; }
+                __syncthreads();
+                continue;
+            }
+        }
+        // ---- exact range energy change (FP64)
+        double de = range_delta(pos, newp, n, s, e, r0row, a.maxn, a, red);
+        st_ex++;
+        if (threadIdx.x == 0) {
+            int by = (int)floor((ecur + de - emin) / a.ebin);
+            int acc = 0;
+            if (by < 0 || by >= a.nbins || bx < 0 || bx >= a.nbins) {
+                acc = 0;
+                st_out++;
+            } else {
+                double ratio = exp(lng[bx] - lng[by]);
+                if (DA && type == 1) {
+                    // second stage: pi(y) a1(y,x) / (pi(x) a1(x,y)), a1(y,x) from the
+                    // surrogate of the reverse move, -sur
+                    int bz = (int)floor((ecur + de - sur - emin) / a.ebin);
+                    double a1r = 0.0;
+                    if (bz >= 0 && bz < a.nbins) a1r = fmin(1.0, exp(lng[by] - lng[bz]));
+                    ratio = ratio * a1r / a1f;
+                }
+                uint64_t s4[4] = {rs[0], rs[1], rs[2], rs[3]};
+                double u = xo_uniform(s4);
+                rs[0] = s4[0]; rs[1] = s4[1]; rs[2] = s4[2]; rs[3] = s4[3];
+                acc = (ratio >= 1.0 || u < ratio) ? 1 : 0;
+            }
+            mv_decide = acc;
+            if (do_log && type == 1) {
+                int slot = atomicAdd(logn, 1);
+                if (slot < logcap) {
+                    for (int q = 0; q < NF; q++) logx[(size_t)slot * NF + q] = feat[q];
+                    logde[slot] = de;
+                }
+            }
+        }
+        __syncthreads();
+        if (mv_decide) {
+            for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
+                pos[3 * i] = newp[3 * i]; pos[3 * i + 1] = newp[3 * i + 1]; pos[3 * i + 2] = newp[3 * i + 2];
+            }
+            ecur += de;
+            st_acc++;
+        }
+        if (threadIdx.x == 0) {
+            int b = (int)floor((ecur - emin) / a.ebin);
+            if (b >= 0 && b < a.nbins) { lng[b] += lnf; hist[b]++; }
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) gpos[(size_t)w * 3 * a.maxn + i] = pos[i];
+    for (int i = threadIdx.x; i < a.nbins; i += blockDim.x) {
+        glng[(size_t)w * a.nbins + i] = lng[i];
+        ghist[(size_t)w * a.nbins + i] = hist[i];
+    }
+    if (threadIdx.x < 4) grng[4 * w + threadIdx.x] = rs[threadIdx.x];
+    if (threadIdx.x == 0) {
+        gecur[w] = ecur;
+        long long *st = gstats + (size_t)w * TTC_MC_NSTAT;
+        st[0] += st_tr; st[1] += st_acc; st[2] += st_s1; st[3] += st_ex; st[4] += st_out;
+    }
+}
+
+__global__ void __launch_bounds__(MC_THREADS)
+k_mc_energy(McArgs a, const double *gpos, const int *gn, const double *gr0, double *eout) {
+    extern __shared__ double smem[];
+    int w = blockIdx.x;
+    int n = gn[w];
+    double *pos = smem, *red = pos + 3 * a.maxn;
+    for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) pos[i] = gpos[(size_t)w * 3 * a.maxn + i];
+    __syncthreads();
+    double e = n > 0 ? full_energy(pos, n, gr0 + (size_t)w * a.maxn * a.maxn, a.maxn, a, red) : 0.0;
+    if (threadIdx.x == 0) eout[w] = e;
+}
+
+__global__ void __launch_bounds__(MC_THREADS)
+k_mc_delta(McArgs a, int w, int s, int e, const double *gpos, const int *gn, const double *gr0,
+           const double *newrange, double *out) {
+    extern __shared__ double smem[];
+    int n = gn[w];
+    double *pos = smem, *newp = pos + 3 * a.maxn, *red = newp + 3 * a.maxn;
+    for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) pos[i] = gpos[(size_t)w * 3 * a.maxn + i];
+    for (int i = threadIdx.x; i < 3 * (e - s); i += blockDim.x) newp[3 * s + i] = newrange[i];
+    __syncthreads();
+    double d = range_delta(pos, newp, n, s, e, gr0 + (size_t)w * a.maxn * a.maxn, a.maxn, a, red);
+    if (threadIdx.x == 0) *out = d;
+}
+
+// ---- network training / evaluation (one workgroup, 256 threads) -----------
+
+#define TR_THREADS 256
+#define TR_B 32   // samples per minibatch step (each evaluated twice: x and swapped x)
+
+// forward of R rows of features X (R x NF, already bf16) -> H1, H2 (R x HID), out (R)
+__device__ void mlp_fwd_rows(const float *P, const float *X, int R, float *H1, float *H2, float *out) {
+    const float *W1 = P, *b1 = W1 + HID * NF, *W2 = b1 + HID, *b2 = W2 + HID * HID, *W3 = b2 + HID, *b3 = W3 + HID;
+    for (int t = threadIdx.x; t < R * HID; t += blockDim.x) {
+        int r = t / HID, j = t % HID;
+        float s = b1[j];
+        for (int i = 0; i < NF; i++) s += W1[j * NF + i] * X[r * NF + i];
+        H1[t] = tanhf(s);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * HID; t += blockDim.x) {
+        int r = t / HID, j = t % HID;
+        float s = b2[j];
+        for (int i = 0; i < HID; i++) s += W2[j * HID + i] * H1[r * HID + i];
+        H2[t] = tanhf(s);
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
+        float s = b3[0];
+        for (int j = 0; j < HID; j++) s += W3[j] * H2[r * HID + j];
+        out[r] = s;
+    }
+    __syncthreads();
+}
+
+// gradient of sum_r dout[r] * f(X_r) w.r.t. P, written (not added) to G
+__device__ void mlp_bwd_rows(const float *P, const float *X, int R, const float *H1, const float *H2,
+                             const float *dout, float *D2, float *D1, float *G) {
+    const float *W2 = P + HID * NF + HID, *W3 = W2 + HID * HID + HID;
+    float *gW1 = G, *gb1 = gW1 + HID * NF, *gW2 = gb1 + HID, *gb2 = gW2 + HID * HID, *gW3 = gb2 + HID, *gb3 = gW3 + HID;
+    for (int t = threadIdx.x; t < R * HID; t += blockDim.x) {
+        int r = t / HID, j = t % HID;
+        float h = H2[t];
+        D2[t] = dout[r] * W3[j] * (1.0f - h * h);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * HID; t += blockDim.x) {
+        int r = t / HID, i = t % HID;
+        float s = 0.0f;
+        for (int j = 0; j < HID; j++) s += W2[j * HID + i] * D2[r * HID + j];
+        float h = H1[t];
+        D1[t] = s * (1.0f - h * h);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < HID; t += blockDim.x) {
+        float s3 = 0.0f, sb2 = 0.0f, sb1 = 0.0f;
+        for (int r = 0; r < R; r++) {
+            s3 += dout[r] * H2[r * HID + t];
+            sb2 += D2[r * HID + t];
+            sb1 += D1[r * HID + t];
+        }
+        gW3[t] = s3;
+        gb2[t] = sb2;
+        gb1[t] = sb1;
+    }
+    if (threadIdx.x == 0) {
+        float s = 0.0f;
+        for (int r = 0; r < R; r++) s += dout[r];
+        gb3[0] = s;
+    }
+    for (int t = threadIdx.x; t < HID * HID; t += blockDim.x) {
+        int j = t / HID, i = t % HID;
+        float s = 0.0f;
+        for (int r = 0; r < R; r++) s += D2[r * HID + j] * H1[r * HID + i];
+        gW2[t] = s;
+    }
+    for (int t = threadIdx.x; t < HID * NF; t += blockDim.x) {
+        int j = t / NF, i = t % NF;
+        float s = 0.0f;
+        for (int r = 0; r < R; r++) s += D1[r * HID + j] * X[r * NF + i];
+        gW1[t] = s;
+    }
+    __syncthreads();
+}
+
+// plain (non-antisymmetric) loss and gradient over nsamp rows, chunked
+__global__ void __launch_bounds__(TR_THREADS)
+k_mlp_loss_grad(const float *P, int nsamp, const float *X, const float *Y, double *loss, float *grad) {
+    extern __shared__ float sm[];
+    float *Xs = sm;                       // TR_B*NF
+    float *H1 = Xs + TR_B * NF, *H2 = H1 + TR_B * HID, *D2 = H2 + TR_B * HID, *D1 = D2 + TR_B * HID;
+    float *out = D1 + TR_B * HID, *dout = out + TR_B, *G = dout + TR_B;
+    __shared__ double lsum;
+    if (threadIdx.x == 0) lsum = 0.0;
+    for (int t = threadIdx.x; t < NPAR; t += blockDim.x) grad[t] = 0.0f;
+    __syncthreads();
+    for (int c0 = 0; c0 < nsamp; c0 += TR_B) {
+        int R = min(TR_B, nsamp - c0);
+        for (int t = threadIdx.x; t < R * NF; t += blockDim.x) Xs[t] = bf16_round(X[(size_t)c0 * NF + t]);
+        __syncthreads();
+        mlp_fwd_rows(P, Xs, R, H1, H2, out);
+        if (threadIdx.x == 0) {
+            for (int r = 0; r < R; r++) {
+                float d = out[r] - Y[c0 + r];
+                lsum += 0.5 * (double)d * d;
+                dout[r] = d / (float)nsamp;
+            }
+        }
+        __syncthreads();
+        mlp_bwd_rows(P, Xs, R, H1, H2, dout, D2, D1, G);
+        for (int t = threadIdx.x; t < NPAR; t += blockDim.x) grad[t] += G[t];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = lsum / nsamp;
+}
+
+__global__ void __launch_bounds__(TR_THREADS)
+k_mlp_forward(const float *P, int nsamp, const float *X, float *Y) {
+    extern __shared__ float sm[];
+    float *Xs = sm, *H1 = Xs + TR_B * NF, *H2 = H1 + TR_B * HID, *out = H2 + TR_B * HID;
+    for (int c0 = 0; c0 < nsamp; c0 += TR_B) {
+        int R = min(TR_B, nsamp - c0);
+        for (int t = threadIdx.x; t < R * NF; t += blockDim.x) Xs[t] = bf16_round(X[(size_t)c0 * NF + t]);
+        __syncthreads();
+        mlp_fwd_rows(P, Xs, R, H1, H2, out);
+        for (int r = threadIdx.x; r < R; r += blockDim.x) Y[c0 + r] = out[r];
+        __syncthreads();
+    }
+}
+
+// Adam over the logged moves; antisymmetric output g(x) - g(swap x)
+__global__ void __launch_bounds__(TR_THREADS)
+k_mlp_train(float *P, float *adam, int t0, int nsamp, const float *X, const double *DE, int nepoch, float lr,
+            double escale, uint64_t seed, double *loss_out) {
+    extern __shared__ float sm[];
+    const int R = 2 * TR_B;
+    float *Xs = sm;                        // R*NF
+    float *H1 = Xs + R * NF, *H2 = H1 + R * HID, *D2 = H2 + R * HID, *D1 = D2 + R * HID;
+    float *out = D1 + R * HID, *dout = out + R, *G = dout + R;
+    __shared__ double lsum;
+    __shared__ int idx[TR_B];
+    uint64_t s4[4] = {seed ^ 0x9E3779B97F4A7C15ull, seed * 3 + 1, seed * 7 + 5, seed * 11 + 13};
+    int t = t0;
+    double last = 0.0;
+    for (int ep = 0; ep < nepoch; ep++) {
+        if (threadIdx.x == 0) lsum = 0.0;
+        int nstep = nsamp / TR_B;
+        for (int st = 0; st < nstep; st++) {
+            if (threadIdx.x == 0)
+                for (int r = 0; r < TR_B; r++) idx[r] = (int)(xo_uniform(s4) * nsamp) % nsamp;
+            __syncthreads();
+            for (int q = threadIdx.x; q < TR_B * NF; q += blockDim.x) {
+                int r = q / NF, i = q % NF;
+                const float *x = X + (size_t)idx[r] * NF;
+                // swapped row: old <-> new local terms
+                int si = i < 16 ? i + 16 : (i < 32 ? i - 16 : i);
+                Xs[r * NF + i] = bf16_round(x[i]);
+                Xs[(TR_B + r) * NF + i] = bf16_round(x[si]);
+            }
+            __syncthreads();
+            mlp_fwd_rows(P, Xs, R, H1, H2, out);
+            if (threadIdx.x == 0) {
+                for (int r = 0; r < TR_B; r++) {
+                    double y = asinh(DE[idx[r]] / escale);
+                    float d = (out[r] - out[TR_B + r]) - (float)y;
+                    lsum += 0.5 * (double)d * d;
+                    dout[r] = d / (float)TR_B;
+                    dout[TR_B + r] = -d / (float)TR_B;
+                }
+            }
+            __syncthreads();
+            mlp_bwd_rows(P, Xs, R, H1, H2, dout, D2, D1, G);
+            t++;
+            float b1c = 1.0f - powf(0.9f, (float)t), b2c = 1.0f - powf(0.999f, (float)t);
+            for (int q = threadIdx.x; q < NPAR; q += blockDim.x) {
+                float g = G[q];
+                float m = adam[q] = 0.9f * adam[q] + 0.1f * g;
+                float v = adam[NPAR + q] = 0.999f * adam[NPAR + q] + 0.001f * g * g;
+                P[q] -= lr * (m / b1c) / (sqrtf(v / b2c) + 1e-8f);
+            }
+            __syncthreads();
+        }
+        last = nstep > 0 ? lsum / (nstep * TR_B) : 0.0;
+    }
+    if (threadIdx.x == 0) *loss_out = last;
+}
+
+// ---------------------------------------------------------------- host
+
+static McArgs mc_args(const ttc_mc *m) {
+    McArgs a;
+    a.nwalk = m->p.nwalk; a.maxn = m->p.maxn; a.nbins = m->p.nbins;
+    a.ebin = m->p.ebin; a.kbond = m->p.kbond; a.r0bond = m->p.r0bond; a.eps = m->p.eps;
+    a.sig2 = m->p.sig_rep * m->p.sig_rep; a.pivot_frac = m->p.pivot_frac; a.theta_max = m->p.theta_max;
+    a.disp_max = m->p.disp_max;
+    return a;
+}
+
+static size_t run_shm(const ttc_mc *m, bool da) {
+    size_t d = (size_t)(6 * m->p.maxn + m->p.nbins) * sizeof(double) + (size_t)m->p.nbins * sizeof(long long) +
+               MC_WAVES * sizeof(double);
+    size_t f = (size_t)((da ? NPAR : 0) + 2 * NF + 4 * HID + 2) * sizeof(float);
+    return d + f;
+}
+
+static size_t train_shm() {
+    const int R = 2 * TR_B;
+    return (size_t)(R * NF + 4 * R * HID + 2 * R + NPAR) * sizeof(float);
+}
+
+static uint64_t splitmix(uint64_t &x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+#define MCHECK(cond, code, msg) \
+    do {                        \
+        if (!(cond)) {          \
+            ttc_set_error(msg); \
+            return code;        \
+        }                       \
+    } while (0)
+
+extern "C" int ttc_mlp_nparams(void) { return NPAR; }
+
+extern "C" int ttc_mc_create(const ttc_mc_params *p, ttc_mc **out) {
+    MCHECK(p && out, TTC_ERR_ARG, "ttc_mc_create: null argument");
+    MCHECK(p->nwalk > 0 && p->maxn >= 4 && p->maxn <= 1024 && p->nbins > 0 && p->nbins <= 4096 && p->ebin > 0,
+           TTC_ERR_ARG, "ttc_mc_create: bad sizes");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        ttc_set_error("ttc_mc_create: no HIP device");
+        return TTC_ERR_HIP;
+    }
+    ttc_mc *m = new ttc_mc();
+    m->p = *p;
+    size_t W = p->nwalk, N = p->maxn;
+    HIPCHK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    HIPCHK(hipMalloc(&m->pos, sizeof(double) * W * N * 3));
+    HIPCHK(hipMalloc(&m->nbead, sizeof(int) * W));
+    HIPCHK(hipMalloc(&m->r0, sizeof(double) * W * N * N));
+    HIPCHK(hipMalloc(&m->emin, sizeof(double) * W));
+    HIPCHK(hipMalloc(&m->ecur, sizeof(double) * W));
+    HIPCHK(hipMalloc(&m->lng, sizeof(double) * W * p->nbins));
+    HIPCHK(hipMalloc(&m->hist, sizeof(long long) * W * p->nbins));
+    HIPCHK(hipMalloc(&m->rng, sizeof(uint64_t) * W * 4));
+    HIPCHK(hipMalloc(&m->stats, sizeof(long long) * W * TTC_MC_NSTAT));
+    HIPCHK(hipMalloc(&m->mlp, sizeof(float) * NPAR));
+    HIPCHK(hipMalloc(&m->adam, sizeof(float) * 2 * NPAR));
+    HIPCHK(hipMemset(m->nbead, 0, sizeof(int) * W));
+    HIPCHK(hipMemset(m->r0, 0, sizeof(double) * W * N * N));
+    HIPCHK(hipMemset(m->lng, 0, sizeof(double) * W * p->nbins));
+    HIPCHK(hipMemset(m->hist, 0, sizeof(long long) * W * p->nbins));
+    HIPCHK(hipMemset(m->stats, 0, sizeof(long long) * W * TTC_MC_NSTAT));
+    HIPCHK(hipMemset(m->adam, 0, sizeof(float) * 2 * NPAR));
+    m->hn.assign(W, 0);
+    // network: small uniform init (seeded)
+    std::vector<float> w0(NPAR);
+    uint64_t sx = p->seed ^ 0x5DEECE66Dull;
+    for (int i = 0; i < NPAR; i++) w0[i] = (float)(((splitmix(sx) >> 11) * (1.0 / 9007199254740992.0) - 0.5) * 0.3);
+    HIPCHK(hipMemcpy(m->mlp, w0.data(), sizeof(float) * NPAR, hipMemcpyHostToDevice));
+    std::vector<uint64_t> rs(W * 4);
+    uint64_t sd = p->seed;
+    for (auto &v : rs) v = splitmix(sd);
+    HIPCHK(hipMemcpy(m->rng, rs.data(), sizeof(uint64_t) * W * 4, hipMemcpyHostToDevice));
+    m->logcap = 1 << 20;
+    HIPCHK(hipMalloc(&m->logx, sizeof(float) * (size_t)m->logcap * NF));
+    HIPCHK(hipMalloc(&m->logde, sizeof(double) * m->logcap));
+    HIPCHK(hipMalloc(&m->logn, sizeof(int)));
+    HIPCHK(hipMemset(m->logn, 0, sizeof(int)));
+    const void *fns[] = {(const void *)k_mc_run<true>, (const void *)k_mc_run<false>};
+    for (const void *f : fns)
+        HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)run_shm(m, true)));
+    const void *tfns[] = {(const void *)k_mlp_train, (const void *)k_mlp_loss_grad, (const void *)k_mlp_forward};
+    for (const void *f : tfns)
+        HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)train_shm()));
+    *out = m;
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_destroy(ttc_mc *m) {
+    if (!m) return TTC_OK;
+    hipStreamSynchronize(m->stream);
+    void *b[] = {m->pos, m->nbead, m->r0, m->emin, m->ecur, m->lng, m->hist, m->rng, m->stats, m->mlp, m->adam,
+                 m->logx, m->logde, m->logn};
+    for (void *x : b)
+        if (x) hipFree(x);
+    hipStreamDestroy(m->stream);
+    delete m;
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_set_walker(ttc_mc *m, int w, int n, const double *pos, int nnat, const int *na, const int *nb,
+                                 const double *nr0, double emin) {
+    MCHECK(m && pos && w >= 0 && w < m->p.nwalk && n >= 4 && n <= m->p.maxn, TTC_ERR_ARG,
+           "ttc_mc_set_walker: bad argument");
+    size_t N = m->p.maxn;
+    std::vector<double> r0(N * N, 0.0);
+    for (int c = 0; c < nnat; c++) {
+        int a = na[c], b = nb[c];
+        MCHECK(a >= 0 && a < n && b >= 0 && b < n && abs(a - b) >= 3 && nr0[c] > 0, TTC_ERR_ARG,
+               "ttc_mc_set_walker: bad native contact");
+        r0[(size_t)a * N + b] = r0[(size_t)b * N + a] = nr0[c];
+    }
+    HIPCHK(hipMemcpy(m->pos + (size_t)w * N * 3, pos, sizeof(double) * n * 3, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m->r0 + (size_t)w * N * N, r0.data(), sizeof(double) * N * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m->nbead + w, &n, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m->emin + w, &emin, sizeof(double), hipMemcpyHostToDevice));
+    m->hn[w] = n;
+    // current energy of the walker
+    std::vector<double> e(m->p.nwalk);
+    int r = ttc_mc_energy(m, e.data());
+    if (r) return r;
+    HIPCHK(hipMemcpy(m->ecur + w, &e[w], sizeof(double), hipMemcpyHostToDevice));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_get_walker(ttc_mc *m, int w, double *pos, double *energy) {
+    MCHECK(m && w >= 0 && w < m->p.nwalk, TTC_ERR_ARG, "ttc_mc_get_walker: bad argument");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (pos) HIPCHK(hipMemcpy(pos, m->pos + (size_t)w * m->p.maxn * 3, sizeof(double) * m->hn[w] * 3, hipMemcpyDeviceToHost));
+    if (energy) HIPCHK(hipMemcpy(energy, m->ecur + w, sizeof(double), hipMemcpyDeviceToHost));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_energy(ttc_mc *m, double *e) {
+    MCHECK(m && e, TTC_ERR_ARG, "ttc_mc_energy: bad argument");
+    double *d;
+    HIPCHK(hipMalloc(&d, sizeof(double) * m->p.nwalk));
+    size_t shm = (size_t)(3 * m->p.maxn + MC_WAVES) * sizeof(double);
+    k_mc_energy<<<m->p.nwalk, MC_THREADS, shm, m->stream>>>(mc_args(m), m->pos, m->nbead, m->r0, d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(e, d, sizeof(double) * m->p.nwalk, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    hipFree(d);
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_delta_energy(ttc_mc *m, int w, int lb, int ub, const double *newpos, double *de) {
+    MCHECK(m && newpos && de && w >= 0 && w < m->p.nwalk && lb >= 0 && ub <= m->hn[w] && lb < ub, TTC_ERR_ARG,
+           "ttc_mc_delta_energy: bad argument");
+    double *d, *np;
+    HIPCHK(hipMalloc(&d, sizeof(double)));
+    HIPCHK(hipMalloc(&np, sizeof(double) * 3 * (ub - lb)));
+    HIPCHK(hipMemcpy(np, newpos, sizeof(double) * 3 * (ub - lb), hipMemcpyHostToDevice));
+    size_t shm = (size_t)(6 * m->p.maxn + MC_WAVES) * sizeof(double);
+    k_mc_delta<<<1, MC_THREADS, shm, m->stream>>>(mc_args(m), w, lb, ub, m->pos, m->nbead, m->r0, np, d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(de, d, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    hipFree(d);
+    hipFree(np);
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_run(ttc_mc *m, int ntrials, int mode, double lnf, int log) {
+    MCHECK(m && ntrials >= 0 && (mode == 0 || mode == 1), TTC_ERR_ARG, "ttc_mc_run: bad argument");
+    double escale = 2.5;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, m->stream));
+    if (mode == 1)
+        k_mc_run<true><<<m->p.nwalk, MC_THREADS, run_shm(m, true), m->stream>>>(
+            mc_args(m), ntrials, lnf, log, m->pos, m->nbead, m->r0, m->emin, m->ecur, m->lng, m->hist, m->rng,
+            m->stats, m->mlp, m->logx, m->logde, m->logn, m->logcap, escale);
+    else
+        k_mc_run<false><<<m->p.nwalk, MC_THREADS, run_shm(m, false), m->stream>>>(
+            mc_args(m), ntrials, lnf, log, m->pos, m->nbead, m->r0, m->emin, m->ecur, m->lng, m->hist, m->rng,
+            m->stats, m->mlp, m->logx, m->logde, m->logn, m->logcap, escale);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, m->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    m->t_last = 1000.0 * ms;
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_last_time(ttc_mc *m, double *usec) {
+    MCHECK(m && usec, TTC_ERR_ARG, "ttc_mc_last_time: bad argument");
+    *usec = m->t_last;
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_get_stats(ttc_mc *m, int64_t *stats) {
+    MCHECK(m && stats, TTC_ERR_ARG, "ttc_mc_get_stats: bad argument");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy(stats, m->stats, sizeof(long long) * m->p.nwalk * TTC_MC_NSTAT, hipMemcpyDeviceToHost));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_reset_stats(ttc_mc *m) {
+    MCHECK(m, TTC_ERR_ARG, "ttc_mc_reset_stats: null");
+    HIPCHK(hipMemsetAsync(m->stats, 0, sizeof(long long) * m->p.nwalk * TTC_MC_NSTAT, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_get_dos(ttc_mc *m, int w, double *lng, int64_t *hist) {
+    MCHECK(m && w >= 0 && w < m->p.nwalk, TTC_ERR_ARG, "ttc_mc_get_dos: bad argument");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (lng) HIPCHK(hipMemcpy(lng, m->lng + (size_t)w * m->p.nbins, sizeof(double) * m->p.nbins, hipMemcpyDeviceToHost));
+    if (hist) HIPCHK(hipMemcpy(hist, m->hist + (size_t)w * m->p.nbins, sizeof(long long) * m->p.nbins, hipMemcpyDeviceToHost));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_set_dos(ttc_mc *m, int w, const double *lng) {
+    MCHECK(m && lng && w >= 0 && w < m->p.nwalk, TTC_ERR_ARG, "ttc_mc_set_dos: bad argument");
+    HIPCHK(hipMemcpy(m->lng + (size_t)w * m->p.nbins, lng, sizeof(double) * m->p.nbins, hipMemcpyHostToDevice));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_reset_hist(ttc_mc *m) {
+    MCHECK(m, TTC_ERR_ARG, "ttc_mc_reset_hist: null");
+    HIPCHK(hipMemsetAsync(m->hist, 0, sizeof(long long) * m->p.nwalk * m->p.nbins, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_get_log(ttc_mc *m, int max, float *features, double *de, int *count) {
+    MCHECK(m && count, TTC_ERR_ARG, "ttc_mc_get_log: bad argument");
+    int n = 0;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy(&n, m->logn, sizeof(int), hipMemcpyDeviceToHost));
+    n = std::min(n, m->logcap);
+    *count = n;
+    int k = std::min(n, max);
+    if (k > 0 && features) HIPCHK(hipMemcpy(features, m->logx, sizeof(float) * (size_t)k * NF, hipMemcpyDeviceToHost));
+    if (k > 0 && de) HIPCHK(hipMemcpy(de, m->logde, sizeof(double) * k, hipMemcpyDeviceToHost));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_clear_log(ttc_mc *m) {
+    MCHECK(m, TTC_ERR_ARG, "ttc_mc_clear_log: null");
+    HIPCHK(hipMemsetAsync(m->logn, 0, sizeof(int), m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_set_mlp(ttc_mc *m, const float *params) {
+    MCHECK(m && params, TTC_ERR_ARG, "ttc_mc_set_mlp: bad argument");
+    HIPCHK(hipMemcpy(m->mlp, params, sizeof(float) * NPAR, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(m->adam, 0, sizeof(float) * 2 * NPAR));
+    m->adam_t = 0;
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_get_mlp(ttc_mc *m, float *params) {
+    MCHECK(m && params, TTC_ERR_ARG, "ttc_mc_get_mlp: bad argument");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy(params, m->mlp, sizeof(float) * NPAR, hipMemcpyDeviceToHost));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mlp_forward(ttc_mc *m, int nsamp, const float *x, float *out) {
+    MCHECK(m && x && out && nsamp > 0, TTC_ERR_ARG, "ttc_mlp_forward: bad argument");
+    float *dx, *dy;
+    HIPCHK(hipMalloc(&dx, sizeof(float) * (size_t)nsamp * NF));
+    HIPCHK(hipMalloc(&dy, sizeof(float) * nsamp));
+    HIPCHK(hipMemcpy(dx, x, sizeof(float) * (size_t)nsamp * NF, hipMemcpyHostToDevice));
+    k_mlp_forward<<<1, TR_THREADS, train_shm(), m->stream>>>(m->mlp, nsamp, dx, dy);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dy, sizeof(float) * nsamp, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    hipFree(dx);
+    hipFree(dy);
+    return TTC_OK;
+}
+
+extern "C" int ttc_mlp_loss_grad(ttc_mc *m, int nsamp, const float *x, const float *y, double *loss, float *grad) {
+    MCHECK(m && x && y && loss && grad && nsamp > 0, TTC_ERR_ARG, "ttc_mlp_loss_grad: bad argument");
+    float *dx, *dy, *dg;
+    double *dl;
+    HIPCHK(hipMalloc(&dx, sizeof(float) * (size_t)nsamp * NF));
+    HIPCHK(hipMalloc(&dy, sizeof(float) * nsamp));
+    HIPCHK(hipMalloc(&dg, sizeof(float) * NPAR));
+    HIPCHK(hipMalloc(&dl, sizeof(double)));
+    HIPCHK(hipMemcpy(dx, x, sizeof(float) * (size_t)nsamp * NF, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dy, y, sizeof(float) * nsamp, hipMemcpyHostToDevice));
+    k_mlp_loss_grad<<<1, TR_THREADS, train_shm(), m->stream>>>(m->mlp, nsamp, dx, dy, dl, dg);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(grad, dg, sizeof(float) * NPAR, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipMemcpyAsync(loss, dl, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    hipFree(dx); hipFree(dy); hipFree(dg); hipFree(dl);
+    return TTC_OK;
+}
+
+extern "C" int ttc_mlp_train(ttc_mc *m, int nepoch, int batch, float lr, double escale, double *loss) {
+    MCHECK(m && loss && nepoch >= 0, TTC_ERR_ARG, "ttc_mlp_train: bad argument");
+    (void)batch;   // the minibatch is fixed at 32 moves (64 network rows)
+    int n = 0;
+    HIPCHK(hipMemcpy(&n, m->logn, sizeof(int), hipMemcpyDeviceToHost));
+    n = std::min(n, m->logcap);
+    MCHECK(n >= TR_B, TTC_ERR_ARG, "ttc_mlp_train: fewer logged moves than one minibatch");
+    double *dl;
+    HIPCHK(hipMalloc(&dl, sizeof(double)));
+    k_mlp_train<<<1, TR_THREADS, train_shm(), m->stream>>>(m->mlp, m->adam, m->adam_t, n, m->logx, m->logde, nepoch,
+                                                           lr, escale, m->p.seed + (uint64_t)m->adam_t, dl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(loss, dl, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    m->adam_t += nepoch * (n / TR_B);
+    hipFree(dl);
+    return TTC_OK;
+}
