@@ -66,12 +66,96 @@ def cpu_baseline(s, f_gpu0, rebuild_interval):
     }, err, en
 
 
+def mc_leg(ntrials: int, cpu: bool):
+    """Accepted MC moves/s of 256 Go walkers (60-300 beads) under Wang-Landau:
+    plain exact MC, then delayed acceptance with the surrogate trained on the
+    moves the plain run logged."""
+    from thecodecombo_7_amd import system
+    from thecodecombo_7_amd.mc import Walkers
+    chains = system.make_go_walkers(256)
+    w = Walkers(chains, seed=31)
+    lnf = 0.05
+    w.run(200, mode=0, lnf=lnf)                 # leave the native start
+    w.reset_stats()
+    w.clear_log()
+    t_plain = w.run(ntrials, mode=0, lnf=lnf, log=True) * 1e-6
+    st0 = w.stats()
+    x, de = w.log()
+    t0 = time.perf_counter()
+    loss0 = w.train_surrogate(nepoch=1, lr=1e-3)
+    loss = w.train_surrogate(nepoch=2, lr=5e-4)
+    t_train = time.perf_counter() - t0
+    w.reset_stats()
+    t_da = w.run(ntrials, mode=1, lnf=lnf) * 1e-6
+    st1 = w.stats()
+    piv0 = st0[:, 3].sum()
+    out = {
+        "workload": "256 Go walkers, C-alpha chains of 60-300 beads (46,173 beads), 12-10 native contacts, "
+                    "Wang-Landau 0.5 kcal/mol bins, pivot + displacement",
+        "trials_per_walker": ntrials,
+        "plain_accepted_moves_per_s": round(st0[:, 1].sum() / t_plain, 1),
+        "da_accepted_moves_per_s": round(st1[:, 1].sum() / t_da, 1),
+        "plain_trials_per_s": round(st0[:, 0].sum() / t_plain, 1),
+        "da_trials_per_s": round(st1[:, 0].sum() / t_da, 1),
+        "da_stage1_reject_frac": round(st1[:, 2].sum() / st1[:, 0].sum(), 4),
+        "da_exact_evals_frac": round(st1[:, 3].sum() / st1[:, 0].sum(), 4),
+        "logged_moves": int(len(x)), "surrogate_loss": [round(loss0, 4), round(loss, 4)],
+        "train_s": round(t_train, 2),
+    }
+    out["da_gain_accepted_per_s"] = round(out["da_accepted_moves_per_s"] / out["plain_accepted_moves_per_s"], 3)
+    if cpu:
+        out["cpu_baseline"] = mc_cpu_baseline(chains[:4], st0[:, 1].sum() / st0[:, 0].sum())
+    w.close()
+    del piv0
+    return out
+
+
+def mc_cpu_baseline(chains, acc_ratio):
+    """Plain exact MC on one host core: the oracle's FP64 range energy
+    (forces_mc semantics) for old and new state of each trial."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from thecodecombo_7_amd import system
+    L = pyoracle.lib(fast=True)
+    rng = np.random.default_rng(1)
+    ntr = 0
+    t0 = time.perf_counter()
+    for ch in chains:
+        n = ch.n
+        nat = np.zeros((n, n), np.uint8); r0 = np.zeros((n, n))
+        for (a, b), r in zip(ch.contacts, ch.r0):
+            nat[a, b] = nat[b, a] = 1
+            r0[a, b] = r0[b, a] = r
+        pos = ch.pos.copy()
+        for _ in range(60):
+            p = int(rng.integers(1, n - 1))
+            lb, ub = (0, p) if rng.random() < 0.5 else (p + 1, n)
+            th = rng.random() * np.pi * 0.25
+            c, s = np.cos(th), np.sin(th)
+            R = np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+            new = pos.copy()
+            new[lb:ub] = (pos[lb:ub] - pos[p]) @ R.T + pos[p]
+            e = lambda q: L.orc_go_range_energy(n, np.ascontiguousarray(q).ravel(), lb, ub, system.GO_KBOND,
+                                                system.GO_BOND, nat.ravel(), r0.ravel(), system.GO_EPS,
+                                                system.GO_SIG_REP, 1e30)
+            de = e(new) - e(pos)
+            if de < 0:
+                pos = new
+            ntr += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(ntr / dt * acc_ratio, 2), "unit": "accepted moves/s", "cores": 1, "kind": "port",
+            "sample": f"{ntr} pivot trials on {len(chains)} walkers ({sum(c.n for c in chains)} beads), "
+                      "oracle FP64 range energy old+new per trial; scaled by the GPU plain acceptance ratio"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo leg")
+    ap.add_argument("--mc-trials", type=int, default=8000, help="MC trials per walker and mode")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -127,6 +211,9 @@ def main():
                      "algorithmic_bytes": alg_bytes, "half_list_entries": half_entries,
                      "spread_us": round(t_spread, 2), "step_us_events": round(t_event, 2)},
     }
+    if not args.no_mc:
+        print("bench: Monte Carlo leg", file=sys.stderr, flush=True)
+        out["mc"] = mc_leg(args.mc_trials, not args.no_cpu)
     if not args.no_cpu:
         print("bench: CPU baseline (oracle, 1 core)", file=sys.stderr, flush=True)
         cb, err, _ = cpu_baseline(s, f_gpu0, args.steps / max(1, nreb))

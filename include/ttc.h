@@ -128,6 +128,40 @@ int  ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_every,
 /* kinet.c:18-68 on the device; ken6 optional. Returns en[k].kinet.       */
 int  ttc_kinet(ttc_ctx *c, double *kinet, double *ken6);
 
+/* force_short.c:46 force_short(k): the bonded (fast) forces, written (not
+ * added) to ff_short; *ebond = bond energy.                               */
+int  ttc_force_short(ttc_ctx *c, double *ff_short, int64_t stride, double *ebond);
+/* force_long.c:43 force_long(k): the nonbonded (slow) forces cnonbondnl
+ * gives, written to ff_long; energy slots as ttc_cnonbondnl.             */
+int  ttc_force_long(ttc_ctx *c, double *ff_long, int64_t stride, double *energy8);
+
+/* integrate_mts.c:38-140 (RESPA): nouter long steps of sim.dtlong = dtlong,
+ * each with ninner short steps of sim.dt on the bonded forces, the
+ * nonbonded forces refreshed once per long step; DLIST trigger checked
+ * before every long step.  Energy rows as ttc_md_vverlet, per long step.  */
+int  ttc_md_mts(ttc_ctx *c, int nouter, int ninner, double dtlong, double vscal, int energy_every,
+                double *energies, int max_rows, int *nrows, int *rebuilds);
+
+/* Nose-Hoover chain thermostat of integrate_nhc.c:40-161: target T [K],
+ * mass Q [kg A^2], M chain elements, Nc x Nys (Yoshida-Suzuki weights w)
+ * sub-steps, nfree degrees of freedom (box[k].nfree).  Resets the chain.  */
+int  ttc_set_nhc(ttc_ctx *c, double T, double Q, int M, int Nc, int Nys, const double *w, double nfree);
+/* chain state {r, v, G} per element (3*M doubles) */
+int  ttc_get_nhc(ttc_ctx *c, double *zeta);
+/* nvt_md.c:528-531 (sim.ID2 == 4): integrate_nhc(dt); integrate_vverlet;
+ * integrate_nhc(dt), nsteps times on the device.                          */
+int  ttc_md_nhc(ttc_ctx *c, int nsteps, int energy_every, double *energies, int max_rows, int *nrows,
+                int *rebuilds);
+
+/* enbnd_mc.c:26 enbnd_mc(lb,ub,k,energy): nonbonded energy of the pairs
+ * with at least one site in [lb,ub) (original numbering) plus the full
+ * reciprocal-space energy (pme_calc_mc); energy[8] slots as cnonbondnl.   */
+int  ttc_enbnd_mc(ttc_ctx *c, int lb, int ub, double *energy8);
+/* nblist_pivot.c:26 nblist_pivot(k,ub): refreshes the list after a move of
+ * sites below ub.  The device rebuilds every row (a superset of the rows
+ * a < ub the reference refreshes), from the current positions.           */
+int  ttc_nblist_pivot(ttc_ctx *c, int ub);
+
 /* Average device time of the named kernel class over the last
  * ttc_md_vverlet call, measured with HIP events on the context's stream:
  * which = 0 pair kernel, 1 spread, 2 whole step.  Microseconds.          */

@@ -74,6 +74,14 @@ def lib():
     L.ttc_kinet.argtypes = [vp, dp, dp]
     L.ttc_timing.argtypes = [vp, C.c_int, dp]
     L.ttc_list_stats.argtypes = [vp, C.POINTER(C.c_int64), ip, ip]
+    L.ttc_force_short.argtypes = [vp, dp, C.c_int64, dp]
+    L.ttc_force_long.argtypes = [vp, dp, C.c_int64, dp]
+    L.ttc_md_mts.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, dp, C.c_int, ip, ip]
+    L.ttc_set_nhc.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, dp, C.c_double]
+    L.ttc_get_nhc.argtypes = [vp, dp]
+    L.ttc_md_nhc.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int, ip, ip]
+    L.ttc_enbnd_mc.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.ttc_nblist_pivot.argtypes = [vp, C.c_int]
     _lib = L
     return L
 

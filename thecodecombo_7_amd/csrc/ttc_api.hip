@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <map>
 
 static thread_local std::string g_err;
@@ -87,6 +88,11 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
     ALLOC(c->pos4b, n); ALLOC(c->vel4b, n); ALLOC(c->f4b, n);
     ALLOC(c->type, n); ALLOC(c->typeb, n); ALLOC(c->orig, n); ALLOC(c->origb, n); ALLOC(c->inv, n);
     ALLOC(c->fpair, n); ALLOC(c->g4, n);
+    ALLOC(c->fs, n); ALLOC(c->fl, n); ALLOC(c->fsb, n); ALLOC(c->flb, n);
+    TTC_HIP(hipMemset(c->fs, 0, sizeof(double4) * n));
+    TTC_HIP(hipMemset(c->fl, 0, sizeof(double4) * n));
+    ALLOC(c->nhc, 3 * 8 + 1);
+    TTC_HIP(hipMemset(c->nhc, 0, sizeof(double) * (3 * 8 + 1)));
     TTC_HIP(hipMemset(c->pos4, 0, sizeof(double4) * n));
     TTC_HIP(hipMemset(c->f4, 0, sizeof(double4) * n));
     TTC_HIP(hipMemset(c->fpair, 0, sizeof(double4) * n));
@@ -135,9 +141,12 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
 }
 
 static void destroy_graphs(ttc_ctx *c) {
-    if (c->g_plain) hipGraphExecDestroy(c->g_plain);
-    if (c->g_energy) hipGraphExecDestroy(c->g_energy);
-    c->g_plain = c->g_energy = nullptr;
+    hipGraphExec_t *gs[] = {&c->g_plain, &c->g_energy, &c->g_nhc_plain, &c->g_nhc_energy, &c->g_mts_plain,
+                            &c->g_mts_energy};
+    for (hipGraphExec_t *g : gs) {
+        if (*g) hipGraphExecDestroy(*g);
+        *g = nullptr;
+    }
 }
 
 extern "C" int ttc_destroy(ttc_ctx *c) {
@@ -147,7 +156,8 @@ extern "C" int ttc_destroy(ttc_ctx *c) {
     void *bufs[] = {c->pos4, c->vel4, c->f4, c->anchor, c->pos4b, c->vel4b, c->f4b, c->type, c->typeb, c->orig,
                     c->origb, c->inv, c->fpair, c->g4, c->cell_of, c->cell_cnt, c->cell_start, c->cell_acc,
                     c->cell_members, c->nbl, c->glen, c->status, c->ex_off, c->ex_part, c->ex_flag, c->ljtab,
-                    c->b_off, c->b_part, c->b_r0, c->b_k, c->partials, c->ev, c->rows};
+                    c->b_off, c->b_part, c->b_r0, c->b_k, c->partials, c->ev, c->rows, c->fs, c->fl, c->fsb,
+                    c->flb, c->nhc};
     for (void *b : bufs)
         if (b) hipFree(b);
     ttc_pme_free(c);
@@ -289,7 +299,10 @@ static int put_xyz(ttc_ctx *c, const double *x, int64_t stride, bool velocity) {
         h[s].x = src[0]; h[s].y = src[1]; h[s].z = src[2];
     }
     TTC_HIP(hipMemcpy(dst, h.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
-    if (!velocity) c->f4_valid = false;
+    if (!velocity) {
+        c->f4_valid = false;
+        c->mts_valid = false;
+    }
     return TTC_OK;
 }
 
@@ -582,6 +595,7 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
     if (rebuilds) *rebuilds = c->h_status[ST_NREBUILD];
     if (energy_every > 0 && nr > 0)
         TTC_HIP(hipMemcpy(energies, c->rows, sizeof(double) * ROW_LEN * nr, hipMemcpyDeviceToHost));
+    c->mts_valid = false;
     if (c->h_status[ST_OVERFLOW]) {
         ttc_set_error("ttc_md_vverlet: neighbour list overflow during the run (nblist.c:185-196)");
         return TTC_ERR_LIST_OVERFLOW;
@@ -632,4 +646,236 @@ extern "C" int ttc_list_stats(ttc_ctx *c, int64_t *entries, int *max_row, int *c
     if (max_row) *max_row = mx;
     if (capacity) *capacity = c->cap;
     return TTC_OK;
+}
+
+// ------------------------------------------------------------------ RESPA
+
+static void launch_force_short(ttc_ctx *c, bool energy) {
+    ttc_launch_pme(c, energy, false, true);   // bonds only
+    ttc_launch_combine_to(c, 2, c->fs);
+}
+
+static void launch_force_long(ttc_ctx *c, bool energy) {
+    ttc_launch_pair(c, energy, 0);
+    ttc_launch_spread(c);
+    ttc_launch_pme(c, energy, true, false);   // PME only
+    ttc_launch_combine_to(c, 1 | 2 | 4, c->fl);
+}
+
+extern "C" int ttc_force_short(ttc_ctx *c, double *ff_short, int64_t stride, double *ebond) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "ttc_force_short: topology not set");
+    launch_force_short(c, true);
+    ttc_launch_reduce(c, false);
+    int r = finish(c, nullptr, 0);
+    if (r) return r;
+    if (ff_short) {
+        for (int i = 0; i < c->n; i++) { double *d = ff_short + (size_t)i * stride; d[0] = d[1] = d[2] = 0.0; }
+        r = get_xyz(c, ff_short, stride, c->fs, true);
+        if (r) return r;
+    }
+    if (ebond) *ebond = c->h_ev[EV_BOND];
+    return TTC_OK;
+}
+
+extern "C" int ttc_force_long(ttc_ctx *c, double *ff_long, int64_t stride, double *energy8) {
+    int r = need_list(c);
+    if (r) return r;
+    launch_force_long(c, true);
+    ttc_launch_reduce(c, false);
+    r = finish(c, nullptr, 0);
+    if (r) return r;
+    if (ff_long) {
+        for (int i = 0; i < c->n; i++) { double *d = ff_long + (size_t)i * stride; d[0] = d[1] = d[2] = 0.0; }
+        r = get_xyz(c, ff_long, stride, c->fl, true);
+        if (r) return r;
+    }
+    if (energy8) slots8(c, energy8);
+    return TTC_OK;
+}
+
+// one RESPA long step (integrate_mts.c:38-140) preceded by the conditional
+// rebuild; the trigger is checked after the last kick
+static void launch_mts_step(ttc_ctx *c, int ninner, double dtlong, double vscal, bool energy) {
+    double dts = c->p.dt;
+    ttc_launch_rebuild(c, true);
+    ttc_launch_vkick(c, c->fl, vscal, vscal * 0.5 * dtlong, false, false, true);
+    for (int j = 0; j < ninner; j++) {
+        ttc_launch_vkick_drift(c, c->fs, 1.0, vscal * 0.5 * dts, dts);
+        launch_force_short(c, energy && j == ninner - 1);
+        ttc_launch_vkick(c, c->fs, 1.0, vscal * 0.5 * dts, false, false, false);
+    }
+    launch_force_long(c, energy);
+    ttc_launch_vkick(c, c->fl, 1.0, vscal * 0.5 * dtlong, true, energy, false);
+    if (energy) ttc_launch_reduce(c, true);
+}
+
+static int capture_fn(ttc_ctx *c, hipGraphExec_t *exec, const std::function<void()> &body) {
+    hipGraph_t g;
+    TTC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    TTC_HIP(hipStreamEndCapture(c->stream, &g));
+    TTC_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    hipGraphDestroy(g);
+    return TTC_OK;
+}
+
+static int run_graphs(ttc_ctx *c, hipGraphExec_t plain, hipGraphExec_t energy, int nsteps, int energy_every,
+                      double *energies, int max_rows, int *nrows, int *rebuilds) {
+    TTC_HIP(hipMemsetAsync(c->status, 0, sizeof(int) * ST_NSTAT, c->stream));
+    ttc_launch_nlcheck(c);
+    hipEvent_t e0, e1;
+    TTC_HIP(hipEventCreate(&e0));
+    TTC_HIP(hipEventCreate(&e1));
+    TTC_HIP(hipEventRecord(e0, c->stream));
+    for (int step = 1; step <= nsteps; step++) {
+        bool en = energy_every > 0 && step % energy_every == 0;
+        TTC_HIP(hipGraphLaunch(en ? energy : plain, c->stream));
+    }
+    TTC_HIP(hipEventRecord(e1, c->stream));
+    int r = read_status(c);
+    if (r) return r;
+    float ms = 0.0f;
+    TTC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->t_step = nsteps > 0 ? 1000.0 * ms / nsteps : 0.0;
+    int nr = std::min(c->h_status[ST_NROWS], max_rows);
+    if (nrows) *nrows = energy_every > 0 ? nr : 0;
+    if (rebuilds) *rebuilds = c->h_status[ST_NREBUILD];
+    if (energy_every > 0 && nr > 0)
+        TTC_HIP(hipMemcpy(energies, c->rows, sizeof(double) * ROW_LEN * nr, hipMemcpyDeviceToHost));
+    if (c->h_status[ST_OVERFLOW]) {
+        ttc_set_error("neighbour list overflow during the run (nblist.c:185-196)");
+        return TTC_ERR_LIST_OVERFLOW;
+    }
+    return TTC_OK;
+}
+
+static int ensure_rows(ttc_ctx *c, int energy_every, int max_rows) {
+    if (energy_every > 0 && max_rows > c->rows_cap) {
+        if (c->rows) hipFree(c->rows);
+        ALLOC(c->rows, (size_t)max_rows * ROW_LEN);
+        c->rows_cap = max_rows;
+        destroy_graphs(c);
+    }
+    return TTC_OK;
+}
+
+extern "C" int ttc_md_mts(ttc_ctx *c, int nouter, int ninner, double dtlong, double vscal, int energy_every,
+                          double *energies, int max_rows, int *nrows, int *rebuilds) {
+    CHECK(c && c->topo && nouter >= 0 && ninner >= 1 && dtlong > 0, TTC_ERR_ARG, "ttc_md_mts: bad argument");
+    CHECK(energy_every == 0 || (energies && max_rows > 0), TTC_ERR_ARG, "ttc_md_mts: no row buffer");
+    int r = need_list(c);
+    if (r) return r;
+    r = ensure_rows(c, energy_every, max_rows);
+    if (r) return r;
+    if (!c->mts_valid) {
+        // main.c:489: force_short; force_long before the loop
+        launch_force_short(c, false);
+        launch_force_long(c, false);
+        c->mts_valid = true;
+    }
+    if (!c->g_mts_plain || c->g_mts_ninner != ninner || c->g_mts_dtlong != dtlong || c->g_mts_vscal != vscal) {
+        for (hipGraphExec_t *g : {&c->g_mts_plain, &c->g_mts_energy})
+            if (*g) { hipGraphExecDestroy(*g); *g = nullptr; }
+        r = capture_fn(c, &c->g_mts_plain, [&] { launch_mts_step(c, ninner, dtlong, vscal, false); });
+        if (r) return r;
+        r = capture_fn(c, &c->g_mts_energy, [&] { launch_mts_step(c, ninner, dtlong, vscal, true); });
+        if (r) return r;
+        c->g_mts_ninner = ninner; c->g_mts_dtlong = dtlong; c->g_mts_vscal = vscal;
+    }
+    r = run_graphs(c, c->g_mts_plain, c->g_mts_energy, nouter, energy_every, energies, max_rows, nrows, rebuilds);
+    c->f4_valid = false;
+    return r;
+}
+
+// ------------------------------------------------------------- Nose-Hoover
+
+extern "C" int ttc_set_nhc(ttc_ctx *c, double T, double Q, int M, int Nc, int Nys, const double *w, double nfree) {
+    CHECK(c && T > 0 && Q > 0 && M >= 1 && M <= 8 && Nc >= 1 && Nys >= 1 && Nys <= 8 && nfree > 0, TTC_ERR_ARG,
+          "ttc_set_nhc: bad argument");
+    c->nhc_T = T; c->nhc_Q = Q; c->nhc_M = M; c->nhc_Nc = Nc; c->nhc_Nys = Nys; c->nhc_nfree = nfree;
+    for (int i = 0; i < 8; i++) c->nhc_w[i] = i < Nys ? (w ? w[i] : (Nys == 1 ? 1.0 : 0.0)) : 0.0;
+    TTC_HIP(hipMemset(c->nhc, 0, sizeof(double) * (3 * 8 + 1)));
+    for (hipGraphExec_t *g : {&c->g_nhc_plain, &c->g_nhc_energy})
+        if (*g) { hipGraphExecDestroy(*g); *g = nullptr; }
+    return TTC_OK;
+}
+
+extern "C" int ttc_get_nhc(ttc_ctx *c, double *zeta) {
+    CHECK(c && zeta && c->nhc_M > 0, TTC_ERR_ARG, "ttc_get_nhc: thermostat not set");
+    std::vector<double> st(3 * c->nhc_M + 1);
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    TTC_HIP(hipMemcpy(st.data(), c->nhc, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
+    int M = c->nhc_M;
+    for (int j = 0; j < M; j++) {
+        zeta[3 * j] = st[j];
+        zeta[3 * j + 1] = st[M + j];
+        zeta[3 * j + 2] = st[2 * M + j];
+    }
+    return TTC_OK;
+}
+
+// nvt_md.c:528-531: integrate_nhc(dt); integrate_vverlet(k, 1.0); integrate_nhc(dt)
+static void launch_nhc_step(ttc_ctx *c, bool energy) {
+    ttc_launch_kinet(c);
+    ttc_launch_nhc(c);
+    ttc_launch_vscale(c);
+    ttc_launch_rebuild(c, true);
+    ttc_launch_drift(c, 1.0);
+    ttc_launch_pair(c, energy, 0);
+    ttc_launch_pme(c, energy, true, true);
+    ttc_launch_kick(c, 1.0, true);   // kinetic partials for the second thermostat half
+    ttc_launch_nhc(c);
+    ttc_launch_vscale(c);
+    if (energy) {
+        ttc_launch_kinet(c);
+        ttc_launch_reduce(c, true);
+    }
+}
+
+extern "C" int ttc_md_nhc(ttc_ctx *c, int nsteps, int energy_every, double *energies, int max_rows, int *nrows,
+                          int *rebuilds) {
+    CHECK(c && c->topo && nsteps >= 0 && c->nhc_M > 0, TTC_ERR_ARG, "ttc_md_nhc: bad argument or no thermostat");
+    CHECK(energy_every == 0 || (energies && max_rows > 0), TTC_ERR_ARG, "ttc_md_nhc: no row buffer");
+    int r = need_list(c);
+    if (r) return r;
+    r = ensure_rows(c, energy_every, max_rows);
+    if (r) return r;
+    if (!c->f4_valid) {
+        launch_forces(c, false);
+        c->f4_valid = true;
+    }
+    if (!c->g_nhc_plain) {
+        r = capture_fn(c, &c->g_nhc_plain, [&] { launch_nhc_step(c, false); });
+        if (r) return r;
+        r = capture_fn(c, &c->g_nhc_energy, [&] { launch_nhc_step(c, true); });
+        if (r) return r;
+    }
+    r = run_graphs(c, c->g_nhc_plain, c->g_nhc_energy, nsteps, energy_every, energies, max_rows, nrows, rebuilds);
+    c->mts_valid = false;
+    return r;
+}
+
+// ----------------------------------------------------------- MC energies
+
+extern "C" int ttc_enbnd_mc(ttc_ctx *c, int lb, int ub, double *energy8) {
+    CHECK(c && energy8 && lb >= 0 && ub <= c->n && lb <= ub, TTC_ERR_ARG, "ttc_enbnd_mc: bad argument");
+    int r = need_list(c);
+    if (r) return r;
+    ttc_launch_pair_range(c, lb, ub);
+    ttc_launch_spread(c);
+    ttc_launch_pme(c, true, true, false);
+    ttc_launch_reduce(c, false);
+    r = finish(c, nullptr, 0);
+    if (r) return r;
+    slots8(c, energy8);
+    c->f4_valid = false;
+    c->mts_valid = false;
+    return TTC_OK;
+}
+
+extern "C" int ttc_nblist_pivot(ttc_ctx *c, int ub) {
+    CHECK(c && ub >= 0 && ub <= c->n, TTC_ERR_ARG, "ttc_nblist_pivot: bad argument");
+    return ttc_nblist(c);
 }

@@ -106,6 +106,19 @@ struct ttc_ctx {
     double4 *fpair = nullptr;       // pair kernel
     double4 *g4 = nullptr;          // PME gather + bonds
     bool f4_valid = false;          // f4 holds the force at the current positions
+    // RESPA short (bonded) and long (nonbonded) forces, sorted order, permuted
+    // with every rebuild like f4
+    double4 *fs = nullptr, *fl = nullptr, *fsb = nullptr, *flb = nullptr;
+    bool mts_valid = false;
+
+    // Nose-Hoover chain (integrate_nhc.c): device state {r,v,G}*M, then scale
+    double *nhc = nullptr;
+    double nhc_T = 300.0, nhc_Q = 1.0, nhc_nfree = 0.0, nhc_w[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+    int nhc_M = 0, nhc_Nc = 1, nhc_Nys = 1;
+    hipGraphExec_t g_nhc_plain = nullptr, g_nhc_energy = nullptr;
+    hipGraphExec_t g_mts_plain = nullptr, g_mts_energy = nullptr;
+    int g_mts_ninner = 0;
+    double g_mts_dtlong = 0.0, g_mts_vscal = 0.0;
 
     // PME
     int K[3] = {0, 0, 0};
@@ -144,8 +157,10 @@ int ttc_export_list(ttc_ctx *c, double cutoff, int max_nb, int *count, int *list
 int ttc_nl_check_range(ttc_ctx *c, int lb, int ub, int *flag);
 double ttc_list_cutoff(const ttc_ctx *c);
 int ttc_cellcap(const ttc_ctx *c);
-// pair
+// pair; the range variant weights each pair by whether a site lies in
+// [lb,ub) of the original numbering (energies only)
 void ttc_launch_pair(ttc_ctx *c, bool energy, int blk_off);
+void ttc_launch_pair_range(ttc_ctx *c, int lb, int ub);
 int ttc_pair_blocks(ttc_ctx *c);
 // pme (+ bonds in the gather)
 int ttc_pme_setup(ttc_ctx *c);
@@ -160,7 +175,14 @@ void ttc_launch_drift(ttc_ctx *c, double vscal);
 // md / combine / reductions
 // mask bits: 1 pair, 2 gather (PME + bonds), 4 remove mean PME force
 void ttc_launch_combine(ttc_ctx *c, int mask);
+void ttc_launch_combine_to(ttc_ctx *c, int mask, double4 *out);
 void ttc_launch_kick(ttc_ctx *c, double vscal, bool energy);
+// v = A v + B (100/m) F; optional trigger check, KE partials, flag clear
+void ttc_launch_vkick(ttc_ctx *c, const double4 *F, double A, double B, bool check, bool ke, bool clear_flag);
+// v = A v + B (100/m) F, then x += dt v and pbc (no spreading)
+void ttc_launch_vkick_drift(ttc_ctx *c, const double4 *F, double A, double B, double dt);
+void ttc_launch_nhc(ttc_ctx *c);         // chain update from the KE partials -> scale
+void ttc_launch_vscale(ttc_ctx *c);      // v *= scale
 void ttc_launch_kinet(ttc_ctx *c);
 void ttc_launch_reduce(ttc_ctx *c, bool append_row);
 int ttc_kick_blocks(ttc_ctx *c);

@@ -209,3 +209,164 @@ void ttc_launch_reduce(ttc_ctx *c, bool append_row) {
     k_reduce<<<1, 256, 0, c->stream>>>(ra, c->partials, c->nblk_max, c->ev, c->e_self, fan, c->rows, c->rows_cap,
                                        c->status, append_row ? 1 : 0);
 }
+
+// ---------------------------------------------------------------------------
+// RESPA (integrate_mts.c:38-140) and Nose-Hoover chain (integrate_nhc.c:40-161)
+
+namespace {
+
+__global__ void __launch_bounds__(256)
+k_combine_to(int n, int mask, const double4 *fpair, const double4 *g4, double4 *out, const double *partials,
+             int maxb, int nbg) {
+    __shared__ double mean[3];
+    if (mask & 4) mean_pme(partials, maxb, nbg, n, mean);
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    out[s] = combine(s, mask, fpair, g4, mean);
+}
+
+// v = A v + B (100/m) F; the first half kick of integrate_vverlet.c:57-59 is
+// (A, B) = (vscal, vscal dt/2), the second (1, vscal dt/2)
+__global__ void __launch_bounds__(256)
+k_vkick(int n, double4 *vel4, const double4 *F, double A, double B, int check, const double4 *pos4,
+        const double4 *anchor, const double *pcheck, int ke, double *partials, int maxb, int clear_flag,
+        int *status) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (clear_flag && s == 0) status[ST_NLFLAG] = 0;
+    bool ok = s < n;
+    double4 v = make_double4(0, 0, 0, 1);
+    if (ok) {
+        v = vel4[s];
+        double4 f = F[s];
+        v.x = A * v.x + B * v.w * f.x;
+        v.y = A * v.y + B * v.w * f.y;
+        v.z = A * v.z + B * v.w * f.z;
+        vel4[s] = v;
+        if (check) {
+            double4 p = pos4[s], a = anchor[s];
+            double lx = pcheck[0], ly = pcheck[1], lz = pcheck[2], rcut = pcheck[3];
+            double dx = fabs(p.x - a.x);
+            if (dx > 0.5 * lx) dx -= lx;
+            double dy = fabs(p.y - a.y);
+            if (dy > 0.5 * ly) dy -= ly;
+            double dz = fabs(p.z - a.z);
+            if (dz > 0.5 * lz) dz -= lz;
+            if (dx > rcut || dy > rcut || dz > rcut || ttc_r2_exact(dx, dy, dz) > rcut * rcut)
+                atomicOr(&status[ST_NLFLAG], 1);
+        }
+    }
+    if (ke) ken_block(100.0 / v.w, v, partials, maxb, ok);
+}
+
+__device__ __forceinline__ double pbc1m(double x, double l) {
+    double h = 0.5 * l;
+    for (int it = 0; it < 64 && (x < -h || x > h); it++) x += (x < -h) ? l : -l;
+    return x;
+}
+
+// integrate_mts.c:75-105: short half kick, drift, pbc
+__global__ void __launch_bounds__(256)
+k_vkick_drift(int n, double4 *pos4, double4 *vel4, const double4 *F, double A, double B, double dt, double lx,
+              double ly, double lz) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    double4 v = vel4[s], f = F[s], p = pos4[s];
+    v.x = A * v.x + B * v.w * f.x;
+    v.y = A * v.y + B * v.w * f.y;
+    v.z = A * v.z + B * v.w * f.z;
+    p.x = pbc1m(p.x + dt * v.x, lx);
+    p.y = pbc1m(p.y + dt * v.y, ly);
+    p.z = pbc1m(p.z + dt * v.z, lz);
+    vel4[s] = v;
+    pos4[s] = p;
+}
+
+struct NhcArgs {
+    double T, Q, nfree, dt;
+    double w[8];
+    int M, Nc, Nys, nbk;
+};
+
+// integrate_nhc.c:40-161 on one lane: kinetic energy from the kick/kinet
+// per-block partials (fixed order), chain update, velocity scale -> st[3M]
+__global__ void k_nhc(NhcArgs a, const double *partials, int maxb, double *st) {
+    __shared__ double kin;
+    if (threadIdx.x < 64) {
+        double t = 0.0;
+        for (int b = threadIdx.x; b < a.nbk; b += 64)
+            t += partials[(size_t)EV_KE0 * maxb + b] + partials[(size_t)EV_KE2 * maxb + b] +
+                 partials[(size_t)EV_KE5 * maxb + b];
+        t = ttc_wave_sum(t);
+        if (threadIdx.x == 0) kin = t;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double *zr = st, *zv = st + a.M, *zg = st + 2 * a.M;
+    int M = a.M, M1 = M - 1, M2 = M - 2;
+    double NkT = a.nfree * TTC_KB * a.T * 1E-4;
+    double kT = TTC_KB * a.T * 1E-4;
+    double akin = 2.0 * kin * 1000.0 * 1E-4 / TTC_NA;
+    double scale = 1.0;
+    for (int h = 0; h < a.Nc; h++)
+        for (int i = 0; i < a.Nys; i++) {
+            double dt = a.dt * a.w[i] / (double)a.Nc;
+            double dt2 = dt / 2.0, dt4 = dt / 4.0, dt8 = dt / 8.0, AA;
+            zg[0] = (akin - NkT) / a.Q;
+            zv[M1] += zg[M1] * dt4;
+            for (int j = 0; j < M1; j++) {
+                AA = exp(-dt8 * zv[M1 - j]);
+                zv[M2 - j] = zv[M2 - j] * AA * AA + dt4 * zg[M2 - j] * AA;
+            }
+            AA = exp(-dt2 * zv[0]);
+            scale = scale * AA;
+            for (int j = 0; j < M; j++) zr[j] += zv[j] * dt2;
+            zg[0] = (scale * scale * akin - NkT) / a.Q;
+            for (int j = 0; j < M - 1; j++) {
+                AA = exp(-dt8 * zv[j + 1]);
+                zv[j] = zv[j] * AA * AA + dt4 * zg[j] * AA;
+                zg[j + 1] = (a.Q * zv[j] * zv[j] - kT) / a.Q;
+            }
+            zv[M - 1] += zg[M - 1] * dt4;
+        }
+    st[3 * M] = scale;
+}
+
+__global__ void __launch_bounds__(256)
+k_vscale(int n, double4 *vel4, const double *scale) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    double sc = *scale;
+    double4 v = vel4[s];
+    v.x *= sc; v.y *= sc; v.z *= sc;
+    vel4[s] = v;
+}
+
+}  // namespace
+
+void ttc_launch_combine_to(ttc_ctx *c, int mask, double4 *out) {
+    k_combine_to<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, mask, c->fpair, c->g4, out, c->partials,
+                                                            c->nblk_max, ttc_gather_blocks(c));
+}
+
+void ttc_launch_vkick(ttc_ctx *c, const double4 *F, double A, double B, bool check, bool ke, bool clear_flag) {
+    k_vkick<<<ttc_kick_blocks(c), 256, 0, c->stream>>>(c->n, c->vel4, F, A, B, check ? 1 : 0, c->pos4, c->anchor,
+                                                       check_consts(c), ke ? 1 : 0, c->partials, c->nblk_max,
+                                                       clear_flag ? 1 : 0, c->status);
+}
+
+void ttc_launch_vkick_drift(ttc_ctx *c, const double4 *F, double A, double B, double dt) {
+    k_vkick_drift<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, c->pos4, c->vel4, F, A, B, dt, c->p.lx, c->p.ly,
+                                                             c->p.lz);
+}
+
+void ttc_launch_nhc(ttc_ctx *c) {
+    NhcArgs a;
+    a.T = c->nhc_T; a.Q = c->nhc_Q; a.nfree = c->nhc_nfree; a.dt = c->p.dt;
+    for (int i = 0; i < 8; i++) a.w[i] = c->nhc_w[i];
+    a.M = c->nhc_M; a.Nc = c->nhc_Nc; a.Nys = c->nhc_Nys; a.nbk = ttc_kick_blocks(c);
+    k_nhc<<<1, 64, 0, c->stream>>>(a, c->partials, c->nblk_max, c->nhc);
+}
+
+void ttc_launch_vscale(ttc_ctx *c) {
+    k_vscale<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, c->vel4, c->nhc + 3 * c->nhc_M);
+}

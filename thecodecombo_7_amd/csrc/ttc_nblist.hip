@@ -83,6 +83,7 @@ __global__ void k_reorder(int n, int cellcap, const int *cell_of, const int *cel
                           const int *cell_start, const int *members, const int *orig,
                           const double4 *pos4, const double4 *vel4, const double4 *f4, const int *type,
                           double4 *pos4b, double4 *vel4b, double4 *f4b, int *typeb, int *origb,
+                          const double4 *fs, const double4 *fl, double4 *fsb, double4 *flb,
                           int *status, int conditional) {
     if (rebuild_off(status, conditional)) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -97,6 +98,8 @@ __global__ void k_reorder(int n, int cellcap, const int *cell_of, const int *cel
     pos4b[s] = pos4[i];
     vel4b[s] = vel4[i];
     f4b[s] = f4[i];   // the last force feeds the next half kick
+    fsb[s] = fs[i];   // RESPA short / long forces likewise
+    flb[s] = fl[i];
     typeb[s] = type[i];
     origb[s] = o;
 }
@@ -136,7 +139,8 @@ __global__ void __launch_bounds__(64 * BUILD_WAVES)
 k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__restrict__ pos4b,
         const double4 *__restrict__ vel4b, const double4 *__restrict__ f4b, const int *__restrict__ typeb,
         const int *__restrict__ origb, double4 *pos4, double4 *vel4, double4 *f4, int *type, int *orig,
-        double4 *anchor, int *inv,
+        double4 *anchor, int *inv, const double4 *__restrict__ fsb, const double4 *__restrict__ flb,
+        double4 *fs, double4 *fl,
         const int *__restrict__ cell_cnt, const int *__restrict__ cell_start,
         const int *__restrict__ ex_off, const int *__restrict__ ex_part, const int *__restrict__ ex_flag,
         int ex_span, uint32_t *nbl, int *glen, int *status, int conditional, int copy_state) {
@@ -153,6 +157,8 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__
         pos4[s] = pi;
         vel4[s] = vel4b[s];
         f4[s] = f4b[s];
+        fs[s] = fsb[s];
+        fl[s] = flb[s];
         type[s] = typeb[s];
         orig[s] = oi;
         anchor[s] = pi;
@@ -303,7 +309,7 @@ static void launch_build(ttc_ctx *c, bool conditional, double cutoff, uint32_t *
     CellGeom g = geom(c);
     k_build<<<c->G, 64 * BUILD_WAVES, 0, c->stream>>>(
         c->n, cap, g, c->p.rclist, cutoff * cutoff, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4,
-        c->vel4, c->f4, c->type, c->orig, c->anchor, c->inv, c->cell_cnt, c->cell_start, c->ex_off, c->ex_part, c->ex_flag,
+        c->vel4, c->f4, c->type, c->orig, c->anchor, c->inv, c->fsb, c->flb, c->fs, c->fl, c->cell_cnt, c->cell_start, c->ex_off, c->ex_part, c->ex_flag,
         c->ex_span, nbl, glen, c->status, conditional ? 1 : 0, copy_state ? 1 : 0);
 }
 
@@ -318,7 +324,8 @@ int ttc_launch_rebuild(ttc_ctx *c, bool conditional) {
                                           cond);
     k_reorder<<<blocks, 256, 0, c->stream>>>(c->n, cellcap, c->cell_of, c->cell_cnt, c->cell_start,
                                              c->cell_members, c->orig, c->pos4, c->vel4, c->f4, c->type,
-                                             c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->status, cond);
+                                             c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->fs, c->fl,
+                                             c->fsb, c->flb, c->status, cond);
     launch_build(c, conditional, ttc_list_cutoff(c), c->nbl, c->glen, c->cap, true);
     return TTC_OK;
 }

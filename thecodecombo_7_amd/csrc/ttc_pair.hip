@@ -42,10 +42,18 @@ struct PairAcc {
 };
 
 // One list entry of site i (cnonbond_nblist.c:150-519 for one pair).
-template <bool ENERGY>
+struct RangeSel {
+    const int *orig;
+    int lb, ub;
+};
+
+// RANGE: keep only pairs with a site in [lb,ub) of the original numbering
+// (enbnd_mc.c:98-119); i_in says whether the row's own site is in range.
+template <bool ENERGY, bool RANGE = false>
 __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pi, float qi, const float4 *ljrow,
                                            const double4 *__restrict__ pos4, const int *__restrict__ type,
-                                           const PairConst &pc, PairAcc &a) {
+                                           const PairConst &pc, PairAcc &a, const RangeSel &rs = RangeSel{},
+                                           bool i_in = true) {
     if (e == TTC_PAD) return;
     int j = (int)(e & TTC_JMASK);
     uint32_t cls = e >> 30;
@@ -56,6 +64,10 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pi, float 
     double dz = ttc_minimg(pi.z - pj.z, pc.hz, pc.lz);
     double dr2 = ttc_r2_exact(dx, dy, dz);
     if (dr2 > pc.rc2) return;
+    if (RANGE && !i_in) {
+        int oj = rs.orig[j];
+        if (oj < rs.lb || oj >= rs.ub) return;
+    }
     float r2 = (float)dr2;
     float rinv = __builtin_amdgcn_rsqf(r2);
     float rinv2 = rinv * rinv;
@@ -102,11 +114,12 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pi, float 
 // One workgroup per ELL group of 64 sites; its PAIR_WAVES waves take the
 // row slices k = wave, wave + PAIR_WAVES, ... (each slice one coalesced
 // 256-byte read) and their partial forces are summed in a fixed wave order.
-template <bool ENERGY>
+template <bool ENERGY, bool RANGE = false>
 __global__ void __launch_bounds__(64 * PAIR_WAVES)
 k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__ glen,
        const double4 *__restrict__ pos4, const int *__restrict__ type, const float4 *__restrict__ ljtab,
-       PairConst pc, double4 *__restrict__ fpair, double *__restrict__ partials, int maxb) {
+       PairConst pc, double4 *__restrict__ fpair, double *__restrict__ partials, int maxb,
+       RangeSel rs = RangeSel{}) {
     extern __shared__ float4 slj[];
     __shared__ double sred[PAIR_WAVES][64][3];
     __shared__ double ered[PAIR_WAVES][11];
@@ -124,14 +137,19 @@ k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__
     const float4 *ljrow = slj + ti * pc.ntypes;
     float qi = (float)pi.w;
     const uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
+    bool i_in = true;
+    if (RANGE) {
+        int oi = s < n ? rs.orig[s] : -1;
+        i_in = oi >= rs.lb && oi < rs.ub;
+    }
     int k = wave;
     for (; k + PAIR_WAVES < len; k += 2 * PAIR_WAVES) {
         uint32_t e0 = row[(size_t)k * 64];
         uint32_t e1 = row[(size_t)(k + PAIR_WAVES) * 64];
-        pair_entry<ENERGY>(e0, pi, qi, ljrow, pos4, type, pc, a);
-        pair_entry<ENERGY>(e1, pi, qi, ljrow, pos4, type, pc, a);
+        pair_entry<ENERGY, RANGE>(e0, pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
+        pair_entry<ENERGY, RANGE>(e1, pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
     }
-    if (k < len) pair_entry<ENERGY>(row[(size_t)k * 64], pi, qi, ljrow, pos4, type, pc, a);
+    if (k < len) pair_entry<ENERGY, RANGE>(row[(size_t)k * 64], pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
 
     sred[wave][lane][0] = a.fx;
     sred[wave][lane][1] = a.fy;
@@ -167,7 +185,7 @@ extern double ttc_nr_erfc_host(double x);
 
 int ttc_pair_blocks(ttc_ctx *c) { return c->G; }
 
-void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
+static PairConst pair_const(const ttc_ctx *c) {
     PairConst pc;
     pc.lx = c->p.lx; pc.ly = c->p.ly; pc.lz = c->p.lz;
     pc.hx = 0.5 * c->p.lx; pc.hy = 0.5 * c->p.ly; pc.hz = 0.5 * c->p.lz;
@@ -181,6 +199,19 @@ void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
     pc.e14 = (float)c->p.e14fac;
     pc.one_m_e14 = (float)(1.0 - c->p.e14fac);
     pc.ntypes = c->ntypes;
+    return pc;
+}
+
+void ttc_launch_pair_range(ttc_ctx *c, int lb, int ub) {
+    PairConst pc = pair_const(c);
+    RangeSel rs{c->orig, lb, ub};
+    size_t shm = sizeof(float4) * c->ntypes * c->ntypes;
+    k_pair<true, true><<<ttc_pair_blocks(c), 64 * PAIR_WAVES, shm, c->stream>>>(
+        c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, c->ljtab, pc, c->fpair, c->partials, c->nblk_max, rs);
+}
+
+void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
+    PairConst pc = pair_const(c);
     int blocks = ttc_pair_blocks(c);
     size_t shm = sizeof(float4) * c->ntypes * c->ntypes;
     if (energy)

@@ -152,6 +152,51 @@ class Box:
                                       C.byref(nr), C.byref(nreb)))
         return buf[:nr.value], nreb.value
 
+    # ---- force_short.c / force_long.c / integrate_mts.c ----
+    def force_short(self):
+        ff = np.zeros((self.n, 3)); eb = np.zeros(1)
+        T.check(self.L.ttc_force_short(self.h, T.dptr(ff), 3, T.dptr(eb)))
+        return ff, float(eb[0])
+
+    def force_long(self):
+        ff = np.zeros((self.n, 3)); en = np.zeros(8)
+        T.check(self.L.ttc_force_long(self.h, T.dptr(ff), 3, T.dptr(en)))
+        return ff, en
+
+    def integrate_mts(self, nouter: int, ninner: int, dtlong: float, vscal: float = 1.0, energy_every: int = 0):
+        rows = max(1, nouter // energy_every) if energy_every else 1
+        buf = np.zeros((rows, 9)); nr = C.c_int(0); nreb = C.c_int(0)
+        T.check(self.L.ttc_md_mts(self.h, nouter, ninner, dtlong, vscal, energy_every, T.dptr(buf), rows,
+                                  C.byref(nr), C.byref(nreb)))
+        return buf[:nr.value], nreb.value
+
+    # ---- integrate_nhc.c / nvt_md.c ----
+    def set_nhc(self, T_K: float, Q: float, M: int = 2, Nc: int = 1, Nys: int = 1, w=None, nfree=None):
+        w = np.ascontiguousarray(np.ones(Nys) if w is None else w, dtype=np.float64)
+        nfree = 3.0 * self.n - 3.0 if nfree is None else nfree
+        T.check(self.L.ttc_set_nhc(self.h, T_K, Q, M, Nc, Nys, T.dptr(w), nfree))
+        self._nhc_M = M
+
+    def nhc_state(self):
+        z = np.zeros(3 * self._nhc_M)
+        T.check(self.L.ttc_get_nhc(self.h, T.dptr(z)))
+        return z
+
+    def integrate_nhc(self, nsteps: int, energy_every: int = 0):
+        rows = max(1, nsteps // energy_every) if energy_every else 1
+        buf = np.zeros((rows, 9)); nr = C.c_int(0); nreb = C.c_int(0)
+        T.check(self.L.ttc_md_nhc(self.h, nsteps, energy_every, T.dptr(buf), rows, C.byref(nr), C.byref(nreb)))
+        return buf[:nr.value], nreb.value
+
+    # ---- enbnd_mc.c / nblist_pivot.c ----
+    def enbnd_mc(self, lb: int, ub: int):
+        en = np.zeros(8)
+        T.check(self.L.ttc_enbnd_mc(self.h, lb, ub, T.dptr(en)))
+        return en
+
+    def nblist_pivot(self, ub: int):
+        T.check(self.L.ttc_nblist_pivot(self.h, ub))
+
     def timing(self, which: int) -> float:
         t = np.zeros(1)
         T.check(self.L.ttc_timing(self.h, which, T.dptr(t)))
