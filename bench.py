@@ -148,8 +148,75 @@ def mc_cpu_baseline(chains, acc_ratio):
                       "oracle FP64 range energy old+new per trial; scaled by the GPU plain acceptance ratio"}
 
 
+COPY_MIB = 512
+
+
+def traffic_probe():
+    """Workload for the HBM-traffic counter runs: 20 pair-kernel launches on
+    the chains-100k step-0 state, then one plain FP32 copy kernel that reads
+    and writes COPY_MIB MiB (the calibration of FETCH_SIZE / WRITE_SIZE)."""
+    import torch
+    from thecodecombo_7_amd import system
+    from thecodecombo_7_amd.nonbond import Box
+    s = system.chains_100k()
+    box = Box(s)
+    box.nblist()
+    box.timing(0)
+    src = torch.ones(COPY_MIB << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    cnt, _, _ = box.get_nblist(0, 1500)
+    box.close()
+    print(json.dumps({"half_list_entries": int(cnt.sum()), "sites": s.n, "copy_mib": COPY_MIB}), flush=True)
+
+
+def _pmc_per_kernel(path, counter):
+    """Mean per-dispatch value of one counter for each kernel name in a
+    rocprofv3 counter_collection.csv."""
+    import csv
+    acc = {}
+    with open(path) as fh:
+        for row in csv.DictReader(fh):
+            if row.get("Counter_Name") != counter:
+                continue
+            name = row["Kernel_Name"]
+            key = (name, row.get("Dispatch_Id", row.get("Correlation_Id")))
+            acc.setdefault(name, {}).setdefault(key, 0.0)
+            acc[name][key] += float(row["Counter_Value"])
+    return {k: sum(v.values()) / len(v) for k, v in acc.items()}
+
+
+def traffic_summary(fetch_csv, write_csv, out_path):
+    """HBM bytes per pair-kernel launch from the two counter runs, each
+    scaled by the copy kernel's reading against its known COPY_MIB bytes."""
+    f = _pmc_per_kernel(fetch_csv, "FETCH_SIZE")
+    w = _pmc_per_kernel(write_csv, "WRITE_SIZE")
+    pick = lambda d, key: next(v for k, v in d.items() if key(k))
+    is_pair = lambda k: "k_pair" in k
+    is_copy = lambda k: "vectorized_elementwise_kernel" in k and "FillFunctor" not in k
+    known_kib = COPY_MIB * 1024.0
+    sf = known_kib / pick(f, is_copy)
+    sw = known_kib / pick(w, is_copy)
+    fetch = pick(f, is_pair) * sf * 1024.0
+    write = pick(w, is_pair) * sw * 1024.0
+    res = {"kernel": "k_pair<false,false>", "workload": "chains-100k step-0 state",
+           "fetch_kib_raw": pick(f, is_pair), "write_kib_raw": pick(w, is_pair),
+           "copy_fetch_kib_raw": pick(f, is_copy), "copy_write_kib_raw": pick(w, is_copy),
+           "fetch_scale": sf, "write_scale": sw,
+           "hbm_bytes_per_launch": round(fetch + write)}
+    with open(out_path, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run only the workload of the FETCH_SIZE / WRITE_SIZE counter runs")
+    ap.add_argument("--traffic-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
+                    help="turn the two counter runs into bytes per pair-kernel launch")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
@@ -157,6 +224,12 @@ def main():
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo leg")
     ap.add_argument("--mc-trials", type=int, default=8000, help="MC trials per walker and mode")
     args = ap.parse_args()
+    if args.traffic_summary:
+        traffic_summary(*args.traffic_summary)
+        return
+    if args.traffic_probe:
+        traffic_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is described, not built).", file=sys.stderr)

@@ -57,3 +57,33 @@ def test_python_mirror_fails_loudly_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(ImportError):
         _lib.lib()
+
+
+REFDIR = "/root/reference"
+SHIM = os.path.join(ROOT, "thecodecombo_7_amd", "host", "ttc_refshim.c")
+# entry points the shim stands in for, with the reference file that defines them
+SHIM_SYMBOLS = {
+    "nblist": "nblist.c", "nl_check": "nblist.c", "nblist_pivot": "nblist_pivot.c",
+    "cnonbondnl": "cnonbond_nblist.c", "pme_calc": "pme_calc.c", "pme_force": "pme_calc.c",
+    "charge_grid": "pme_qgrid.c", "pme_init": "pme_setup.c", "pme_clean": "pme_setup.c",
+    "ewald_setup": "ewald.c", "enbnd_mc": "enbnd_mc.c", "erfc": "ewald.c",
+}
+
+
+@pytest.mark.skipif(not os.path.isdir(REFDIR), reason="reference tree absent")
+def test_shim_compiles_against_reference_headers(tmp_path):
+    """The reference-side binding compiles against the reference's own
+    defines.h (flags NLIST DLIST STYPE S14 EWALD PRESSURE) and defines the
+    replaced entry points, each of which the reference defines by that name."""
+    obj = str(tmp_path / "shim.o")
+    r = subprocess.run(["gcc", "-std=c99", "-O2", "-c", "-DNLIST", "-DDLIST", "-DSTYPE", "-DS14", "-DEWALD",
+                        "-DPRESSURE", f"-I{REFDIR}", f"-I{ROOT}/include", SHIM, "-o", obj],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run(["nm", "--defined-only", obj], capture_output=True, text=True).stdout
+    defined = set(re.findall(r" T (\w+)$", out, flags=re.M))
+    assert set(SHIM_SYMBOLS) <= defined
+    # each one is really a reference entry point of that name
+    for name, src in SHIM_SYMBOLS.items():
+        text = open(os.path.join(REFDIR, src), errors="replace").read()
+        assert re.search(rf"^\s*(void|double)\s+{name}\s*\(", text, flags=re.M), (name, src)
