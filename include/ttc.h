@@ -148,6 +148,11 @@ int  ttc_md_mts(ttc_ctx *c, int nouter, int ninner, double dtlong, double vscal,
 int  ttc_set_nhc(ttc_ctx *c, double T, double Q, int M, int Nc, int Nys, const double *w, double nfree);
 /* chain state {r, v, G} per element (3*M doubles) */
 int  ttc_get_nhc(ttc_ctx *c, double *zeta);
+/* New target temperature T and mass Q of the chain, keeping its state
+ * (the temperature a replica exchange hands over, swap_box.c:4-86).      */
+int  ttc_set_nhc_temperature(ttc_ctx *c, double T, double Q);
+/* v *= factor for every site (swap_box.c:33-43 velocity rescaling).      */
+int  ttc_scale_velocities(ttc_ctx *c, double factor);
 /* nvt_md.c:528-531 (sim.ID2 == 4): integrate_nhc(dt); integrate_vverlet;
  * integrate_nhc(dt), nsteps times on the device.                          */
 int  ttc_md_nhc(ttc_ctx *c, int nsteps, int energy_every, double *energies, int max_rows, int *nrows,
@@ -161,6 +166,32 @@ int  ttc_enbnd_mc(ttc_ctx *c, int lb, int ub, double *energy8);
  * sites below ub.  The device rebuilds every row (a superset of the rows
  * a < ub the reference refreshes), from the current positions.           */
 int  ttc_nblist_pivot(ttc_ctx *c, int ub);
+
+/* ---- Batched replicas (nvt_md.c + swap_box.c, one MPI process per
+ * replica in the reference) ------------------------------------------------
+ * A batch advances many contexts of any sizes together: their NHC step
+ * graphs are captured into one graph (launch mode 0, one branch per
+ * replica) or one graph per replica on its own stream (mode 1), with
+ * steps_per_launch steps per launch.  Every context needs ttc_set_nhc.    */
+typedef struct ttc_batch ttc_batch;
+int  ttc_batch_create(ttc_ctx *const *ctx, int nrep, uint64_t seed, ttc_batch **out);
+int  ttc_batch_destroy(ttc_batch *b);
+int  ttc_batch_set_launch(ttc_batch *b, int mode, int steps_per_launch);
+/* nvt_md.c:528-531 nsteps times on every replica (nsteps a multiple of the
+ * steps per launch); potens[r] = en.potens (calcvalue.c:40, bond + nbonds)
+ * of replica r after the last step, rebuilds[r] its list rebuilds.       */
+int  ttc_batch_md_nhc(ttc_batch *b, int nsteps, double *potens, int *rebuilds);
+/* swap_box.c:7-11 for the replicas at temperature ranks (t, t+1), t = parity
+ * mod 2: accepted swaps exchange the two thermostat temperatures and
+ * rescale velocities by sqrt(T_new/T_old).  *accepted = swaps made.      */
+int  ttc_batch_exchange(ttc_batch *b, const double *potens, int parity, int *accepted);
+/* order[t] = replica at temperature rank t, temps[r] = current temperature
+ * of replica r, wall ms per step of the last ttc_batch_md_nhc, totals.    */
+int  ttc_batch_state(ttc_batch *b, int *order, double *temps, double *ms_per_step,
+                     long long *attempts, long long *accepts);
+/* Tries launch modes {0,1} x steps per launch {1,5,10,25} for trial_steps
+ * steps each (real dynamics) and keeps the fastest; timings[8] optional. */
+int  ttc_batch_autotune(ttc_batch *b, int trial_steps, int *best_mode, int *best_spl, double *timings);
 
 /* Average device time of the named kernel class over the last
  * ttc_md_vverlet call, measured with HIP events on the context's stream:

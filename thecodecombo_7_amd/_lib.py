@@ -82,6 +82,15 @@ def lib():
     L.ttc_md_nhc.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int, ip, ip]
     L.ttc_enbnd_mc.argtypes = [vp, C.c_int, C.c_int, dp]
     L.ttc_nblist_pivot.argtypes = [vp, C.c_int]
+    L.ttc_set_nhc_temperature.argtypes = [vp, C.c_double, C.c_double]
+    L.ttc_scale_velocities.argtypes = [vp, C.c_double]
+    L.ttc_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64, C.POINTER(vp)]
+    L.ttc_batch_destroy.argtypes = [vp]
+    L.ttc_batch_set_launch.argtypes = [vp, C.c_int, C.c_int]
+    L.ttc_batch_md_nhc.argtypes = [vp, C.c_int, dp, ip]
+    L.ttc_batch_exchange.argtypes = [vp, dp, C.c_int, ip]
+    L.ttc_batch_state.argtypes = [vp, ip, dp, dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.ttc_batch_autotune.argtypes = [vp, C.c_int, ip, ip, dp]
     _lib = L
     return L
 

@@ -91,8 +91,8 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
     ALLOC(c->fs, n); ALLOC(c->fl, n); ALLOC(c->fsb, n); ALLOC(c->flb, n);
     TTC_HIP(hipMemset(c->fs, 0, sizeof(double4) * n));
     TTC_HIP(hipMemset(c->fl, 0, sizeof(double4) * n));
-    ALLOC(c->nhc, 3 * 8 + 1);
-    TTC_HIP(hipMemset(c->nhc, 0, sizeof(double) * (3 * 8 + 1)));
+    ALLOC(c->nhc, TTC_NHC_LEN);
+    TTC_HIP(hipMemset(c->nhc, 0, sizeof(double) * TTC_NHC_LEN));
     TTC_HIP(hipMemset(c->pos4, 0, sizeof(double4) * n));
     TTC_HIP(hipMemset(c->f4, 0, sizeof(double4) * n));
     TTC_HIP(hipMemset(c->fpair, 0, sizeof(double4) * n));
@@ -339,6 +339,7 @@ static int grow_list(ttc_ctx *c, int need) {
     c->cap = ((int)(need * 1.2) + 127) & ~63;
     ALLOC(c->nbl, (size_t)c->G * c->cap * 64);
     destroy_graphs(c);
+    c->gen++;
     return TTC_OK;
 }
 
@@ -757,6 +758,7 @@ static int ensure_rows(ttc_ctx *c, int energy_every, int max_rows) {
         ALLOC(c->rows, (size_t)max_rows * ROW_LEN);
         c->rows_cap = max_rows;
         destroy_graphs(c);
+        c->gen++;
     }
     return TTC_OK;
 }
@@ -796,15 +798,40 @@ extern "C" int ttc_set_nhc(ttc_ctx *c, double T, double Q, int M, int Nc, int Ny
           "ttc_set_nhc: bad argument");
     c->nhc_T = T; c->nhc_Q = Q; c->nhc_M = M; c->nhc_Nc = Nc; c->nhc_Nys = Nys; c->nhc_nfree = nfree;
     for (int i = 0; i < 8; i++) c->nhc_w[i] = i < Nys ? (w ? w[i] : (Nys == 1 ? 1.0 : 0.0)) : 0.0;
-    TTC_HIP(hipMemset(c->nhc, 0, sizeof(double) * (3 * 8 + 1)));
+    double st[TTC_NHC_LEN] = {0};
+    st[TTC_NHC_SCALE] = 1.0;
+    st[TTC_NHC_T] = T;
+    st[TTC_NHC_Q] = Q;
+    TTC_HIP(hipMemcpyAsync(c->nhc, st, sizeof st, hipMemcpyHostToDevice, c->stream));
+    TTC_HIP(hipStreamSynchronize(c->stream));
     for (hipGraphExec_t *g : {&c->g_nhc_plain, &c->g_nhc_energy})
         if (*g) { hipGraphExecDestroy(*g); *g = nullptr; }
+    c->gen++;
+    return TTC_OK;
+}
+
+extern "C" int ttc_set_nhc_temperature(ttc_ctx *c, double T, double Q) {
+    CHECK(c && c->nhc_M > 0 && T > 0 && Q > 0, TTC_ERR_ARG, "ttc_set_nhc_temperature: thermostat not set");
+    double tq[2] = {T, Q};
+    TTC_HIP(hipMemcpyAsync(c->nhc + TTC_NHC_T, tq, sizeof tq, hipMemcpyHostToDevice, c->stream));
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    c->nhc_T = T;
+    c->nhc_Q = Q;
+    return TTC_OK;
+}
+
+extern "C" int ttc_scale_velocities(ttc_ctx *c, double factor) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "ttc_scale_velocities: topology not set");
+    TTC_HIP(hipMemcpyAsync(c->nhc + TTC_NHC_VMUL, &factor, sizeof factor, hipMemcpyHostToDevice, c->stream));
+    ttc_launch_vmul(c);
+    TTC_HIP(hipGetLastError());
+    TTC_HIP(hipStreamSynchronize(c->stream));
     return TTC_OK;
 }
 
 extern "C" int ttc_get_nhc(ttc_ctx *c, double *zeta) {
     CHECK(c && zeta && c->nhc_M > 0, TTC_ERR_ARG, "ttc_get_nhc: thermostat not set");
-    std::vector<double> st(3 * c->nhc_M + 1);
+    std::vector<double> st(3 * c->nhc_M);
     TTC_HIP(hipStreamSynchronize(c->stream));
     TTC_HIP(hipMemcpy(st.data(), c->nhc, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
     int M = c->nhc_M;
@@ -817,7 +844,7 @@ extern "C" int ttc_get_nhc(ttc_ctx *c, double *zeta) {
 }
 
 // nvt_md.c:528-531: integrate_nhc(dt); integrate_vverlet(k, 1.0); integrate_nhc(dt)
-static void launch_nhc_step(ttc_ctx *c, bool energy) {
+void ttc_launch_nhc_step(ttc_ctx *c, bool energy) {
     ttc_launch_kinet(c);
     ttc_launch_nhc(c);
     ttc_launch_vscale(c);
@@ -834,6 +861,22 @@ static void launch_nhc_step(ttc_ctx *c, bool energy) {
     }
 }
 
+// state a captured NHC step needs: list, row buffer, forces at the
+// current positions
+int ttc_prepare_nhc(ttc_ctx *c, int max_rows) {
+    CHECK(c && c->topo && c->nhc_M > 0, TTC_ERR_ARG, "ttc_md_nhc: no topology or no thermostat");
+    int r = need_list(c);
+    if (r) return r;
+    r = ensure_rows(c, 1, max_rows);
+    if (r) return r;
+    if (!c->f4_valid) {
+        launch_forces(c, false);
+        c->f4_valid = true;
+    }
+    c->mts_valid = false;
+    return TTC_OK;
+}
+
 extern "C" int ttc_md_nhc(ttc_ctx *c, int nsteps, int energy_every, double *energies, int max_rows, int *nrows,
                           int *rebuilds) {
     CHECK(c && c->topo && nsteps >= 0 && c->nhc_M > 0, TTC_ERR_ARG, "ttc_md_nhc: bad argument or no thermostat");
@@ -847,9 +890,9 @@ extern "C" int ttc_md_nhc(ttc_ctx *c, int nsteps, int energy_every, double *ener
         c->f4_valid = true;
     }
     if (!c->g_nhc_plain) {
-        r = capture_fn(c, &c->g_nhc_plain, [&] { launch_nhc_step(c, false); });
+        r = capture_fn(c, &c->g_nhc_plain, [&] { ttc_launch_nhc_step(c, false); });
         if (r) return r;
-        r = capture_fn(c, &c->g_nhc_energy, [&] { launch_nhc_step(c, true); });
+        r = capture_fn(c, &c->g_nhc_energy, [&] { ttc_launch_nhc_step(c, true); });
         if (r) return r;
     }
     r = run_graphs(c, c->g_nhc_plain, c->g_nhc_energy, nsteps, energy_every, energies, max_rows, nrows, rebuilds);

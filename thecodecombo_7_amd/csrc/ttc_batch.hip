@@ -1,0 +1,329 @@
+// ttc_batch.hip -- many uneven-size replicas advanced together on one GPU,
+// with temperature replica exchange between them.
+//
+// The reference runs each replica as its own MPI process (nvt_md.c with the
+// Nose-Hoover chain of integrate_nhc.c, exchanges by swap_box.c:4-86 /
+// swap_mpi.c).  Here every replica is a device context (its own list, PME
+// grid sized for its box, thermostat) and one call advances all of them:
+// the NHC step graphs of all replicas are captured together, either as one
+// graph whose branches are the replicas (launch mode 0: one launch per
+// `spl` steps for the whole batch, independent branches free to overlap on
+// the device) or as one graph per replica launched on the replica's own
+// stream (mode 1).  The launch mode and the steps per launch are chosen by
+// ttc_batch_autotune from measured step times.
+//
+// Exchanges: replicas holding neighbouring temperatures T_t < T_t+1 are
+// swapped with probability min(1, exp[(1/kT_t - 1/kT_t+1)(E_i - E_j)])
+// (swap_box.c:7-11, E = en[k].potens).  The replicas differ in size, so the
+// temperatures (thermostat targets) move between replicas instead of the
+// configurations, and each replica's velocities are rescaled by
+// sqrt(T_new/T_old) -- the same kinetic-energy matching swap_box.c:33-43
+// applies to the swapped configurations.
+#include "ttc_common.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+int ttc_prepare_nhc(ttc_ctx *c, int max_rows);
+void ttc_launch_nhc_step(ttc_ctx *c, bool energy);
+extern "C" int ttc_set_nhc_temperature(ttc_ctx *c, double T, double Q);
+extern "C" int ttc_scale_velocities(ttc_ctx *c, double factor);
+
+struct ttc_batch {
+    std::vector<ttc_ctx *> r;
+    hipStream_t main = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    int mode = 0;
+    int spl = 1;
+    hipGraphExec_t g_plain = nullptr, g_energy = nullptr;   // mode 0
+    std::vector<hipGraphExec_t> rp, re;                      // mode 1, per replica
+    std::vector<unsigned> gen;                               // context generations at capture
+    int cap_mode = -1, cap_spl = 0;
+    std::vector<int> order;                                  // order[t]: replica at temperature rank t
+    uint64_t rng[4] = {0, 0, 0, 0};
+    long long attempts = 0, accepts = 0;
+    double ms_per_step = 0.0;
+};
+
+namespace {
+
+#define BCHECK(cond, code, msg)          \
+    do {                                 \
+        if (!(cond)) {                   \
+            ttc_set_error(msg);          \
+            return code;                 \
+        }                                \
+    } while (0)
+
+uint64_t splitmix(uint64_t &x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// xoshiro256** (Blackman & Vigna), uniform double in [0, 1)
+double uniform(uint64_t *s) {
+    auto rotl = [](uint64_t x, int k) { return (x << k) | (x >> (64 - k)); };
+    uint64_t out = rotl(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return (double)(out >> 11) * (1.0 / 9007199254740992.0);
+}
+
+void drop_graphs(ttc_batch *b) {
+    if (b->g_plain) hipGraphExecDestroy(b->g_plain);
+    if (b->g_energy) hipGraphExecDestroy(b->g_energy);
+    b->g_plain = b->g_energy = nullptr;
+    for (auto *v : {&b->rp, &b->re})
+        for (auto &g : *v)
+            if (g) { hipGraphExecDestroy(g); g = nullptr; }
+    b->cap_mode = -1;
+}
+
+// one graph of `spl` NHC steps of every replica, one branch per replica
+int capture_all(ttc_batch *b, bool energy, hipGraphExec_t *out) {
+    hipGraph_t g = nullptr;
+    TTC_HIP(hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipEventRecord(b->fork, b->main);
+    for (size_t i = 0; i < b->r.size() && e == hipSuccess; i++) {
+        ttc_ctx *c = b->r[i];
+        e = hipStreamWaitEvent(c->stream, b->fork, 0);
+        for (int s = 0; s < b->spl && e == hipSuccess; s++) {
+            ttc_launch_nhc_step(c, energy && s == b->spl - 1);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(b->join[i], c->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->main, b->join[i], 0);
+    }
+    hipError_t e2 = hipStreamEndCapture(b->main, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        ttc_set_error(std::string("ttc_batch: capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        return TTC_ERR_HIP;
+    }
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    TTC_HIP(e);
+    return TTC_OK;
+}
+
+// `spl` NHC steps of one replica on its own stream
+int capture_one(ttc_batch *b, ttc_ctx *c, bool energy, hipGraphExec_t *out) {
+    hipGraph_t g = nullptr;
+    TTC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < b->spl && e == hipSuccess; s++) {
+        ttc_launch_nhc_step(c, energy && s == b->spl - 1);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        ttc_set_error(std::string("ttc_batch: capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        return TTC_ERR_HIP;
+    }
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    TTC_HIP(e);
+    return TTC_OK;
+}
+
+int sync_all(ttc_batch *b) {
+    TTC_HIP(hipStreamSynchronize(b->main));
+    for (ttc_ctx *c : b->r) TTC_HIP(hipStreamSynchronize(c->stream));
+    return TTC_OK;
+}
+
+int ensure_graphs(ttc_batch *b) {
+    bool stale = b->cap_mode != b->mode || b->cap_spl != b->spl;
+    for (size_t i = 0; i < b->r.size(); i++) stale = stale || b->gen[i] != b->r[i]->gen;
+    if (!stale) return TTC_OK;
+    drop_graphs(b);
+    int r = sync_all(b);   // capture starts from idle streams
+    if (r) return r;
+    if (b->mode == 0) {
+        r = capture_all(b, false, &b->g_plain);
+        if (!r) r = capture_all(b, true, &b->g_energy);
+    } else {
+        b->rp.assign(b->r.size(), nullptr);
+        b->re.assign(b->r.size(), nullptr);
+        for (size_t i = 0; i < b->r.size() && !r; i++) {
+            r = capture_one(b, b->r[i], false, &b->rp[i]);
+            if (!r) r = capture_one(b, b->r[i], true, &b->re[i]);
+        }
+    }
+    if (r) { drop_graphs(b); return r; }
+    for (size_t i = 0; i < b->r.size(); i++) b->gen[i] = b->r[i]->gen;
+    b->cap_mode = b->mode;
+    b->cap_spl = b->spl;
+    return TTC_OK;
+}
+
+}  // namespace
+
+extern "C" int ttc_batch_create(ttc_ctx *const *ctx, int nrep, uint64_t seed, ttc_batch **out) {
+    BCHECK(ctx && out && nrep >= 1, TTC_ERR_ARG, "ttc_batch_create: bad argument");
+    for (int i = 0; i < nrep; i++)
+        BCHECK(ctx[i] && ctx[i]->topo && ctx[i]->nhc_M > 0, TTC_ERR_ARG,
+               "ttc_batch_create: every replica needs a topology and a thermostat (ttc_set_nhc)");
+    ttc_batch *b = new ttc_batch();
+    b->r.assign(ctx, ctx + nrep);
+    b->gen.assign(nrep, ~0u);
+    b->join.assign(nrep, nullptr);
+    hipError_t e = hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (int i = 0; i < nrep && e == hipSuccess; i++) e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        ttc_set_error(std::string("ttc_batch_create: ") + hipGetErrorString(e));
+        delete b;
+        return TTC_ERR_HIP;
+    }
+    b->order.resize(nrep);
+    for (int i = 0; i < nrep; i++) b->order[i] = i;
+    std::stable_sort(b->order.begin(), b->order.end(),
+                     [&](int x, int y) { return b->r[x]->nhc_T < b->r[y]->nhc_T; });
+    uint64_t sm = seed;
+    for (int k = 0; k < 4; k++) b->rng[k] = splitmix(sm);
+    *out = b;
+    return TTC_OK;
+}
+
+extern "C" int ttc_batch_destroy(ttc_batch *b) {
+    if (!b) return TTC_OK;
+    drop_graphs(b);
+    for (hipEvent_t e : b->join)
+        if (e) hipEventDestroy(e);
+    if (b->fork) hipEventDestroy(b->fork);
+    if (b->main) hipStreamDestroy(b->main);
+    delete b;
+    return TTC_OK;
+}
+
+extern "C" int ttc_batch_set_launch(ttc_batch *b, int mode, int steps_per_launch) {
+    BCHECK(b && (mode == 0 || mode == 1) && steps_per_launch >= 1 && steps_per_launch <= 1000, TTC_ERR_ARG,
+           "ttc_batch_set_launch: mode 0/1, 1 <= steps_per_launch <= 1000");
+    b->mode = mode;
+    b->spl = steps_per_launch;
+    return TTC_OK;
+}
+
+extern "C" int ttc_batch_md_nhc(ttc_batch *b, int nsteps, double *potens, int *rebuilds) {
+    BCHECK(b && nsteps >= 0 && nsteps % b->spl == 0, TTC_ERR_ARG,
+           "ttc_batch_md_nhc: nsteps must be a multiple of the steps per launch");
+    size_t R = b->r.size();
+    for (ttc_ctx *c : b->r) {
+        int r = ttc_prepare_nhc(c, 1);
+        if (r) return r;
+    }
+    int r = ensure_graphs(b);
+    if (r) return r;
+    for (ttc_ctx *c : b->r) {
+        TTC_HIP(hipMemsetAsync(c->status, 0, sizeof(int) * ST_NSTAT, c->stream));
+        ttc_launch_nlcheck(c);
+    }
+    r = sync_all(b);
+    if (r) return r;
+    auto t0 = std::chrono::steady_clock::now();
+    int nl = nsteps / b->spl;
+    for (int l = 0; l < nl; l++) {
+        bool en = l == nl - 1;
+        if (b->mode == 0) {
+            TTC_HIP(hipGraphLaunch(en ? b->g_energy : b->g_plain, b->main));
+        } else {
+            for (size_t i = 0; i < R; i++) TTC_HIP(hipGraphLaunch(en ? b->re[i] : b->rp[i], b->r[i]->stream));
+        }
+    }
+    r = sync_all(b);
+    if (r) return r;
+    double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    b->ms_per_step = nsteps > 0 ? ms / nsteps : 0.0;
+    for (size_t i = 0; i < R; i++) {
+        ttc_ctx *c = b->r[i];
+        TTC_HIP(hipMemcpy(c->h_status, c->status, sizeof(int) * ST_NSTAT, hipMemcpyDeviceToHost));
+        if (c->h_status[ST_OVERFLOW]) {
+            ttc_set_error("ttc_batch_md_nhc: neighbour list overflow in replica " + std::to_string(i) +
+                          " (nblist.c:185-196); call ttc_nblist on it and run again");
+            return TTC_ERR_LIST_OVERFLOW;
+        }
+        if (rebuilds) rebuilds[i] = c->h_status[ST_NREBUILD];
+        if (potens && nsteps > 0) {
+            double row[ROW_LEN];
+            TTC_HIP(hipMemcpy(row, c->rows, sizeof row, hipMemcpyDeviceToHost));
+            potens[i] = row[2] + row[4];   // calcvalue.c:40: bond + nbonds
+        }
+        c->f4_valid = true;
+    }
+    return TTC_OK;
+}
+
+extern "C" int ttc_batch_exchange(ttc_batch *b, const double *potens, int parity, int *accepted) {
+    BCHECK(b && potens && (parity == 0 || parity == 1), TTC_ERR_ARG, "ttc_batch_exchange: bad argument");
+    int acc = 0;
+    const double rg = TTC_KB * TTC_NA * 0.001;   // sim.kT = RG T .001 (read_sim.c:563), kJ/mol
+    for (size_t t = parity; t + 1 < b->order.size(); t += 2) {
+        int i = b->order[t], j = b->order[t + 1];
+        ttc_ctx *ci = b->r[i], *cj = b->r[j];
+        double Ti = ci->nhc_T, Tj = cj->nhc_T;
+        double delta_beta = 1.0 / (rg * Ti) - 1.0 / (rg * Tj);
+        double crit = exp(delta_beta * (potens[i] - potens[j]));
+        b->attempts++;
+        if (crit > uniform(b->rng)) {
+            int r = ttc_set_nhc_temperature(ci, Tj, ci->nhc_Q * Tj / Ti);
+            if (!r) r = ttc_set_nhc_temperature(cj, Ti, cj->nhc_Q * Ti / Tj);
+            if (!r) r = ttc_scale_velocities(ci, sqrt(Tj / Ti));
+            if (!r) r = ttc_scale_velocities(cj, sqrt(Ti / Tj));
+            if (r) return r;
+            std::swap(b->order[t], b->order[t + 1]);
+            b->accepts++;
+            acc++;
+        }
+    }
+    if (accepted) *accepted = acc;
+    return TTC_OK;
+}
+
+extern "C" int ttc_batch_state(ttc_batch *b, int *order, double *temps, double *ms_per_step, long long *attempts,
+                               long long *accepts) {
+    BCHECK(b, TTC_ERR_ARG, "ttc_batch_state: bad argument");
+    for (size_t t = 0; t < b->r.size(); t++) {
+        if (order) order[t] = b->order[t];
+        if (temps) temps[t] = b->r[t]->nhc_T;
+    }
+    if (ms_per_step) *ms_per_step = b->ms_per_step;
+    if (attempts) *attempts = b->attempts;
+    if (accepts) *accepts = b->accepts;
+    return TTC_OK;
+}
+
+// Tries every (launch mode, steps per launch) candidate for `trial_steps`
+// steps (the steps are real dynamics: they count as equilibration) and
+// keeps the fastest; the measured ms/step of each candidate is returned
+// in timings[2 * ncand] if given.
+extern "C" int ttc_batch_autotune(ttc_batch *b, int trial_steps, int *best_mode, int *best_spl, double *timings) {
+    BCHECK(b && trial_steps >= 1, TTC_ERR_ARG, "ttc_batch_autotune: bad argument");
+    const int spls[] = {1, 5, 10, 25};
+    double best = 1e300;
+    int bm = 0, bs = 1, q = 0;
+    for (int mode = 0; mode < 2; mode++)
+        for (int spl : spls) {
+            b->mode = mode;
+            b->spl = spl;
+            int steps = std::max(spl, (trial_steps / spl) * spl);
+            int r = ttc_batch_md_nhc(b, spl, nullptr, nullptr);   // capture + warm
+            if (!r) r = ttc_batch_md_nhc(b, steps, nullptr, nullptr);
+            if (r) return r;
+            if (timings) timings[q] = b->ms_per_step;
+            q++;
+            if (b->ms_per_step < best) { best = b->ms_per_step; bm = mode; bs = spl; }
+        }
+    b->mode = bm;
+    b->spl = bs;
+    if (best_mode) *best_mode = bm;
+    if (best_spl) *best_spl = bs;
+    return TTC_OK;
+}

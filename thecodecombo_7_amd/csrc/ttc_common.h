@@ -111,7 +111,10 @@ struct ttc_ctx {
     double4 *fs = nullptr, *fl = nullptr, *fsb = nullptr, *flb = nullptr;
     bool mts_valid = false;
 
-    // Nose-Hoover chain (integrate_nhc.c): device state {r,v,G}*M, then scale
+    // Nose-Hoover chain (integrate_nhc.c): device state {r,v,G}*M (M <= 8),
+    // then the velocity scale at [TTC_NHC_SCALE], target T and mass Q at
+    // [TTC_NHC_T], [TTC_NHC_Q] (device-resident so captured graphs follow
+    // temperature changes), and a scratch scale factor at [TTC_NHC_VMUL]
     double *nhc = nullptr;
     double nhc_T = 300.0, nhc_Q = 1.0, nhc_nfree = 0.0, nhc_w[8] = {1, 0, 0, 0, 0, 0, 0, 0};
     int nhc_M = 0, nhc_Nc = 1, nhc_Nys = 1;
@@ -137,8 +140,10 @@ struct ttc_ctx {
     double *rows = nullptr;         // energy rows of ttc_md_vverlet
     int rows_cap = 0;
 
-    // graph
+    // graph; gen counts reallocations of buffers that captured graphs
+    // reference (list, energy rows), so holders of graphs can re-capture
     hipGraphExec_t g_plain = nullptr, g_energy = nullptr;
+    unsigned gen = 0;
     double g_vscal = 1.0;
 
     // timing (microseconds)
@@ -146,6 +151,7 @@ struct ttc_ctx {
 };
 
 enum { ST_NREBUILD = 4, ST_STEP = 5, ST_NROWS = 6 };
+enum { TTC_NHC_SCALE = 24, TTC_NHC_T = 25, TTC_NHC_Q = 26, TTC_NHC_VMUL = 27, TTC_NHC_LEN = 28 };
 
 #define ROW_LEN 9
 
@@ -183,6 +189,7 @@ void ttc_launch_vkick(ttc_ctx *c, const double4 *F, double A, double B, bool che
 void ttc_launch_vkick_drift(ttc_ctx *c, const double4 *F, double A, double B, double dt);
 void ttc_launch_nhc(ttc_ctx *c);         // chain update from the KE partials -> scale
 void ttc_launch_vscale(ttc_ctx *c);      // v *= scale
+void ttc_launch_vmul(ttc_ctx *c);        // v *= nhc[TTC_NHC_VMUL]
 void ttc_launch_kinet(ttc_ctx *c);
 void ttc_launch_reduce(ttc_ctx *c, bool append_row);
 int ttc_kick_blocks(ttc_ctx *c);

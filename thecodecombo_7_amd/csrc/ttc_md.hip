@@ -282,13 +282,14 @@ k_vkick_drift(int n, double4 *pos4, double4 *vel4, const double4 *F, double A, d
 }
 
 struct NhcArgs {
-    double T, Q, nfree, dt;
+    double nfree, dt;
     double w[8];
     int M, Nc, Nys, nbk;
 };
 
 // integrate_nhc.c:40-161 on one lane: kinetic energy from the kick/kinet
-// per-block partials (fixed order), chain update, velocity scale -> st[3M]
+// per-block partials (fixed order), chain update, velocity scale ->
+// st[TTC_NHC_SCALE]; target T and mass Q are read from st[TTC_NHC_T/Q]
 __global__ void k_nhc(NhcArgs a, const double *partials, int maxb, double *st) {
     __shared__ double kin;
     if (threadIdx.x < 64) {
@@ -303,15 +304,16 @@ __global__ void k_nhc(NhcArgs a, const double *partials, int maxb, double *st) {
     if (threadIdx.x != 0) return;
     double *zr = st, *zv = st + a.M, *zg = st + 2 * a.M;
     int M = a.M, M1 = M - 1, M2 = M - 2;
-    double NkT = a.nfree * TTC_KB * a.T * 1E-4;
-    double kT = TTC_KB * a.T * 1E-4;
+    double T = st[TTC_NHC_T], Q = st[TTC_NHC_Q];
+    double NkT = a.nfree * TTC_KB * T * 1E-4;
+    double kT = TTC_KB * T * 1E-4;
     double akin = 2.0 * kin * 1000.0 * 1E-4 / TTC_NA;
     double scale = 1.0;
     for (int h = 0; h < a.Nc; h++)
         for (int i = 0; i < a.Nys; i++) {
             double dt = a.dt * a.w[i] / (double)a.Nc;
             double dt2 = dt / 2.0, dt4 = dt / 4.0, dt8 = dt / 8.0, AA;
-            zg[0] = (akin - NkT) / a.Q;
+            zg[0] = (akin - NkT) / Q;
             zv[M1] += zg[M1] * dt4;
             for (int j = 0; j < M1; j++) {
                 AA = exp(-dt8 * zv[M1 - j]);
@@ -320,15 +322,15 @@ __global__ void k_nhc(NhcArgs a, const double *partials, int maxb, double *st) {
             AA = exp(-dt2 * zv[0]);
             scale = scale * AA;
             for (int j = 0; j < M; j++) zr[j] += zv[j] * dt2;
-            zg[0] = (scale * scale * akin - NkT) / a.Q;
+            zg[0] = (scale * scale * akin - NkT) / Q;
             for (int j = 0; j < M - 1; j++) {
                 AA = exp(-dt8 * zv[j + 1]);
                 zv[j] = zv[j] * AA * AA + dt4 * zg[j] * AA;
-                zg[j + 1] = (a.Q * zv[j] * zv[j] - kT) / a.Q;
+                zg[j + 1] = (Q * zv[j] * zv[j] - kT) / Q;
             }
             zv[M - 1] += zg[M - 1] * dt4;
         }
-    st[3 * M] = scale;
+    st[TTC_NHC_SCALE] = scale;
 }
 
 __global__ void __launch_bounds__(256)
@@ -361,12 +363,16 @@ void ttc_launch_vkick_drift(ttc_ctx *c, const double4 *F, double A, double B, do
 
 void ttc_launch_nhc(ttc_ctx *c) {
     NhcArgs a;
-    a.T = c->nhc_T; a.Q = c->nhc_Q; a.nfree = c->nhc_nfree; a.dt = c->p.dt;
+    a.nfree = c->nhc_nfree; a.dt = c->p.dt;
     for (int i = 0; i < 8; i++) a.w[i] = c->nhc_w[i];
     a.M = c->nhc_M; a.Nc = c->nhc_Nc; a.Nys = c->nhc_Nys; a.nbk = ttc_kick_blocks(c);
     k_nhc<<<1, 64, 0, c->stream>>>(a, c->partials, c->nblk_max, c->nhc);
 }
 
 void ttc_launch_vscale(ttc_ctx *c) {
-    k_vscale<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, c->vel4, c->nhc + 3 * c->nhc_M);
+    k_vscale<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, c->vel4, c->nhc + TTC_NHC_SCALE);
+}
+
+void ttc_launch_vmul(ttc_ctx *c) {
+    k_vscale<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, c->vel4, c->nhc + TTC_NHC_VMUL);
 }
