@@ -243,6 +243,9 @@ def main():
     box = Box(s)
     box.nblist()
     f_gpu0, _ = box.forces()
+    waves, t_waves = box.tune()
+    print(f"bench: pair kernel waves per group {waves} (4/8/16: {np.round(t_waves, 1)} us)",
+          file=sys.stderr, flush=True)
     print("bench: warmup", args.warmup, file=sys.stderr, flush=True)
     box.integrate_vverlet(args.warmup)
     t0 = time.perf_counter()
@@ -280,7 +283,8 @@ def main():
                    "steps_per_rebuild": round(args.steps / max(1, nreb), 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_pair (LJ + real-space Ewald)", "kernel_us": round(t_pair, 2),
+                     "kernel": f"k_pair (LJ + real-space Ewald), {waves} waves per 64-site group",
+                     "kernel_us": round(t_pair, 2),
                      "algorithmic_bytes": alg_bytes, "half_list_entries": half_entries,
                      "spread_us": round(t_spread, 2), "step_us_events": round(t_event, 2)},
     }

@@ -198,6 +198,11 @@ int  ttc_batch_autotune(ttc_batch *b, int trial_steps, int *best_mode, int *best
  * which = 0 pair kernel, 1 spread, 2 whole step.  Microseconds.          */
 int  ttc_timing(ttc_ctx *c, int which, double *usec);
 
+/* Autotuner of the launch shape: times the pair kernel at 4, 8 and 16
+ * waves per 64-site group on the current list and keeps the fastest
+ * (usec3[3] optional: the three timings).  No reference counterpart.    */
+int  ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3);
+
 /* Counters of the last list build: total entries of the full device list
  * (each pair twice), longest row, rows capacity.                         */
 int  ttc_list_stats(ttc_ctx *c, int64_t *entries, int *max_row, int *capacity);

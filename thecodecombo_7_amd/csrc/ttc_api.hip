@@ -636,6 +636,32 @@ extern "C" int ttc_timing(ttc_ctx *c, int which, double *usec) {
     return TTC_OK;
 }
 
+// Times the pair kernel at 4, 8 and 16 waves per 64-site group on the
+// current list (ttc_timing(0): HIP events around 20 launches) and keeps the
+// fastest; captured graphs are re-made if the choice changes.
+extern "C" int ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "ttc_tune: topology not set");
+    const int cand[3] = {4, 8, 16};
+    int old = c->pair_waves, best = old;
+    double tbest = 1e300;
+    for (int i = 0; i < 3; i++) {
+        c->pair_waves = cand[i];
+        double t = 0.0;
+        int r = ttc_timing(c, 0, &t);
+        if (r) { c->pair_waves = old; return r; }
+        if (usec3) usec3[i] = t;
+        if (t < tbest) { tbest = t; best = cand[i]; }
+    }
+    c->pair_waves = best;
+    c->t_pair = tbest;
+    if (best != old) {
+        destroy_graphs(c);
+        c->gen++;
+    }
+    if (pair_waves) *pair_waves = best;
+    return TTC_OK;
+}
+
 extern "C" int ttc_list_stats(ttc_ctx *c, int64_t *entries, int *max_row, int *capacity) {
     CHECK(c, TTC_ERR_ARG, "ttc_list_stats: null context");
     std::vector<int> gl(c->G);

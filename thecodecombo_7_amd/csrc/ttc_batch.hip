@@ -28,6 +28,7 @@
 int ttc_prepare_nhc(ttc_ctx *c, int max_rows);
 void ttc_launch_nhc_step(ttc_ctx *c, bool energy);
 extern "C" int ttc_set_nhc_temperature(ttc_ctx *c, double T, double Q);
+extern "C" int ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3);
 extern "C" int ttc_scale_velocities(ttc_ctx *c, double factor);
 
 struct ttc_batch {
@@ -300,12 +301,18 @@ extern "C" int ttc_batch_state(ttc_batch *b, int *order, double *temps, double *
     return TTC_OK;
 }
 
-// Tries every (launch mode, steps per launch) candidate for `trial_steps`
-// steps (the steps are real dynamics: they count as equilibration) and
-// keeps the fastest; the measured ms/step of each candidate is returned
-// in timings[2 * ncand] if given.
+// First tunes each replica's pair-kernel shape (ttc_tune), then tries
+// every (launch mode, steps per launch) candidate for `trial_steps` steps
+// (the steps are real dynamics: they count as equilibration) and keeps the
+// fastest; the measured ms/step of each candidate is returned in
+// timings[8] if given.
 extern "C" int ttc_batch_autotune(ttc_batch *b, int trial_steps, int *best_mode, int *best_spl, double *timings) {
     BCHECK(b && trial_steps >= 1, TTC_ERR_ARG, "ttc_batch_autotune: bad argument");
+    for (ttc_ctx *c : b->r) {
+        int r = ttc_prepare_nhc(c, 1);
+        if (!r) r = ttc_tune(c, nullptr, nullptr);
+        if (r) return r;
+    }
     const int spls[] = {1, 5, 10, 25};
     double best = 1e300;
     int bm = 0, bs = 1, q = 0;

@@ -146,6 +146,9 @@ struct ttc_ctx {
     unsigned gen = 0;
     double g_vscal = 1.0;
 
+    // tuned launch shape: waves per 64-site group of the pair kernel (4, 8, 16)
+    int pair_waves = 8;
+
     // timing (microseconds)
     double t_pair = 0, t_spread = 0, t_step = 0;
 };

@@ -34,7 +34,6 @@ __device__ __forceinline__ float nr_erfc(float z, float *ez2) {
     return t * e * __expf(p);
 }
 
-#define PAIR_WAVES 8
 
 struct PairAcc {
     double fx, fy, fz;
@@ -111,18 +110,20 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pi, float 
     }
 }
 
-// One workgroup per ELL group of 64 sites; its PAIR_WAVES waves take the
-// row slices k = wave, wave + PAIR_WAVES, ... (each slice one coalesced
-// 256-byte read) and their partial forces are summed in a fixed wave order.
-template <bool ENERGY, bool RANGE = false>
-__global__ void __launch_bounds__(64 * PAIR_WAVES)
+// One workgroup per ELL group of 64 sites; its WAVES waves take the row
+// slices k = wave, wave + WAVES, ... (each slice one coalesced 256-byte
+// read) and their partial forces are summed in a fixed wave order.  More
+// waves per group shorten each group's dependent load chain (small boxes,
+// few groups); fewer suit many groups.  ttc_tune picks WAVES per context.
+template <bool ENERGY, bool RANGE, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
 k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__ glen,
        const double4 *__restrict__ pos4, const int *__restrict__ type, const float4 *__restrict__ ljtab,
        PairConst pc, double4 *__restrict__ fpair, double *__restrict__ partials, int maxb,
        RangeSel rs = RangeSel{}) {
     extern __shared__ float4 slj[];
-    __shared__ double sred[PAIR_WAVES][64][3];
-    __shared__ double ered[PAIR_WAVES][11];
+    __shared__ double sred[WAVES][64][3];
+    __shared__ double ered[WAVES][11];
     int nt2 = pc.ntypes * pc.ntypes;
     for (int k = threadIdx.x; k < nt2; k += blockDim.x) slj[k] = ljtab[k];
     __syncthreads();
@@ -143,9 +144,9 @@ k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__
         i_in = oi >= rs.lb && oi < rs.ub;
     }
     int k = wave;
-    for (; k + PAIR_WAVES < len; k += 2 * PAIR_WAVES) {
+    for (; k + WAVES < len; k += 2 * WAVES) {
         uint32_t e0 = row[(size_t)k * 64];
-        uint32_t e1 = row[(size_t)(k + PAIR_WAVES) * 64];
+        uint32_t e1 = row[(size_t)(k + WAVES) * 64];
         pair_entry<ENERGY, RANGE>(e0, pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
         pair_entry<ENERGY, RANGE>(e1, pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
     }
@@ -164,7 +165,7 @@ k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__
     __syncthreads();
     if (wave == 0 && s < n) {
         double fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int w = 0; w < PAIR_WAVES; w++) {
+        for (int w = 0; w < WAVES; w++) {
             fx += sred[w][lane][0];
             fy += sred[w][lane][1];
             fz += sred[w][lane][2];
@@ -174,7 +175,7 @@ k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__
     if (ENERGY && threadIdx.x < 11) {
         // each pair was visited from both of its sites: halve
         double t = 0.0;
-        for (int w = 0; w < PAIR_WAVES; w++) t += ered[w][threadIdx.x];
+        for (int w = 0; w < WAVES; w++) t += ered[w][threadIdx.x];
         partials[(size_t)(EV_LJ + threadIdx.x) * maxb + blockIdx.x] = 0.5 * t;
     }
 }
@@ -202,22 +203,27 @@ static PairConst pair_const(const ttc_ctx *c) {
     return pc;
 }
 
-void ttc_launch_pair_range(ttc_ctx *c, int lb, int ub) {
-    PairConst pc = pair_const(c);
-    RangeSel rs{c->orig, lb, ub};
-    size_t shm = sizeof(float4) * c->ntypes * c->ntypes;
-    k_pair<true, true><<<ttc_pair_blocks(c), 64 * PAIR_WAVES, shm, c->stream>>>(
-        c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, c->ljtab, pc, c->fpair, c->partials, c->nblk_max, rs);
-}
-
-void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
+template <bool ENERGY, bool RANGE>
+static void launch_pair_w(ttc_ctx *c, RangeSel rs) {
     PairConst pc = pair_const(c);
     int blocks = ttc_pair_blocks(c);
     size_t shm = sizeof(float4) * c->ntypes * c->ntypes;
-    if (energy)
-        k_pair<true><<<blocks, 64 * PAIR_WAVES, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, c->ljtab,
-                                                      pc, c->fpair, c->partials, c->nblk_max);
-    else
-        k_pair<false><<<blocks, 64 * PAIR_WAVES, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, c->ljtab,
-                                                       pc, c->fpair, c->partials, c->nblk_max);
+#define PAIR_LAUNCH(W)                                                                                           \
+    k_pair<ENERGY, RANGE, W><<<blocks, 64 * W, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, \
+                                                                 c->ljtab, pc, c->fpair, c->partials, c->nblk_max, rs)
+    switch (c->pair_waves) {
+    case 4: PAIR_LAUNCH(4); break;
+    case 16: PAIR_LAUNCH(16); break;
+    default: PAIR_LAUNCH(8); break;
+    }
+#undef PAIR_LAUNCH
+}
+
+void ttc_launch_pair_range(ttc_ctx *c, int lb, int ub) {
+    launch_pair_w<true, true>(c, RangeSel{c->orig, lb, ub});
+}
+
+void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
+    if (energy) launch_pair_w<true, false>(c, RangeSel{});
+    else launch_pair_w<false, false>(c, RangeSel{});
 }

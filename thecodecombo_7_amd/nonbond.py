@@ -202,6 +202,12 @@ class Box:
         T.check(self.L.ttc_timing(self.h, which, T.dptr(t)))
         return float(t[0])
 
+    def tune(self):
+        """Pick the pair kernel's waves per group by timing (ttc_tune)."""
+        w = C.c_int(0); t = np.zeros(3)
+        T.check(self.L.ttc_tune(self.h, C.byref(w), T.dptr(t)))
+        return w.value, t
+
     def list_stats(self):
         e = C.c_int64(0); m = C.c_int(0); cap = C.c_int(0)
         T.check(self.L.ttc_list_stats(self.h, C.byref(e), C.byref(m), C.byref(cap)))
