@@ -110,6 +110,23 @@ int  ttc_pair(ttc_ctx *c, double *ff, int64_t stride, double *e5, double *vir6);
 /* cbond.c:65-120: returns bond energy through *ebond, forces added.       */
 int  ttc_cbond(ttc_ctx *c, double *ff, int64_t stride, double *ebond, double *vir6);
 
+/* ---- CONFIGT (configurational temperature; used by the CTDOS drivers) ---
+ * hes3 = {hesx, hesy, hesz}: sums over sites of d2U/dx_i^2, d2U/dy_i^2,
+ * d2U/dz_i^2 (config[k].hesx/y/z).                                       */
+/* cnonbond_nblist2.c:47 cnonbondnl2(ibox, energy): ttc_cnonbondnl plus the
+ * Hessian diagonal of the LJ, real-space and excluded-pair terms
+ * (cnonbond_nblist2.c:286-297, 403-410, 433-440) and of the SPME
+ * reciprocal energy.  The reference calls pme_calc2 for the latter
+ * (cnonbond_nblist2.c:586) but never defines it; here it is the exact
+ * second derivative of the reciprocal energy PME evaluates.              */
+int  ttc_cnonbondnl2(ttc_ctx *c, double *ff, int64_t stride, double *energy8, double *hes3);
+/* cbond2.c:65-130 cbond2(k): bond energy, forces added, Hessian diagonal. */
+int  ttc_cbond2(ttc_ctx *c, double *ff, int64_t stride, double *ebond, double *hes3);
+/* forces.c:228-256 (CONFIGT branch) + configtemp.c:17-45: one force and
+ * Hessian evaluation; num = sum |F_i|^2, den = hesr = hesx+hesy+hesz,
+ * T = num / den / RG / 0.001.  hes3 optional.                            */
+int  ttc_configtemp(ttc_ctx *c, double *num, double *den, double *T, double *hes3);
+
 /* forces.c:118 forces(ibox) for a system whose bonded terms are bonds:
  * zero, cbond, cnonbondnl; en[k] slots (forces.c:204-219) are returned as
  * en9 = {bond, nbond, nbonds, coulomb, rewald, rewalds, kewald, sewald, iewald}. */

@@ -34,6 +34,12 @@ class OrcExcl(C.Structure):
                 ("flag", C.POINTER(C.c_int))]
 
 
+class OrcPairOut(C.Structure):
+    _fields_ = [("enonbond", C.c_double), ("enonbonds", C.c_double), ("eewald_real", C.c_double),
+                ("eewald_reals", C.c_double), ("eewald_intra", C.c_double), ("vir", C.c_double * 6),
+                ("hes", C.c_double * 3)]
+
+
 class Ran2State(C.Structure):
     _fields_ = [("idum", C.c_long), ("idum2", C.c_long), ("iy", C.c_long), ("iv", C.c_long * 32)]
 
@@ -106,6 +112,12 @@ def _declare(L):
                                       _dp, C.c_double, C.c_double, C.c_double]
     L.orc_ran2_seed.argtypes = [P(Ran2State), C.c_long]
     L.orc_ran2.restype = C.c_double; L.orc_ran2.argtypes = [P(Ran2State)]
+    L.orc_fillspline2.argtypes = [C.c_double, C.c_int, _dp, _dp, _dp]
+    L.orc_pair.argtypes = [P(OrcSystem), _dp, _ip, _ip, _ip, C.c_int, _dp, P(OrcPairOut)]
+    L.orc_pme_hessian.argtypes = [P(OrcSystem), _dp, _dp, C.c_void_p]
+    L.orc_cbond2.restype = C.c_double
+    L.orc_cbond2.argtypes = [P(OrcSystem), _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
+    L.orc_cnonbondnl2.argtypes = [P(OrcSystem), _dp, _ip, _ip, _ip, C.c_int, C.c_double, _dp, _dp, _dp]
     L.orc_mlp_nparams.restype = C.c_int; L.orc_mlp_nparams.argtypes = [C.c_int, C.c_int]
     L.orc_mlp_forward.restype = C.c_double
     L.orc_mlp_forward.argtypes = [C.c_int, C.c_int, _dp, _dp]
@@ -212,6 +224,44 @@ class Oracle:
         return ff + fb, {"bond": eb, "nbond": en[0], "nbonds": en[1], "coulomb": en[2],
                          "rewalds": en[3], "kewald": en[4], "sewald": en[5], "iewald": en[6],
                          "rewald": en[7], "slots": en, "vir_nbond": vn, "vir_kspace": vk}
+
+
+    # ---- CONFIGT (cnonbond_nblist2.c, cbond2.c) ----
+    def pair(self, pos, cnt, lst, xf):
+        """The pair loop alone: forces and the orc_pair_out fields."""
+        n = self.s.n
+        ff = np.zeros(n * 3)
+        po = OrcPairOut()
+        self.L.orc_pair(C.byref(self.cs), np.ascontiguousarray(pos, dtype=np.float64).ravel(),
+                        np.ascontiguousarray(cnt, dtype=np.int32), np.ascontiguousarray(lst, dtype=np.int32).ravel(),
+                        np.ascontiguousarray(xf, dtype=np.int32).ravel(), lst.shape[1], ff, C.byref(po))
+        return ff.reshape(n, 3), po
+
+    def pme_hessian(self, pos, per_site=False):
+        h = np.zeros(3)
+        ps = np.zeros(3 * self.s.n) if per_site else None
+        self.L.orc_pme_hessian(C.byref(self.cs), np.ascontiguousarray(pos, dtype=np.float64).ravel(), h,
+                               ps.ctypes.data_as(C.c_void_p) if per_site else None)
+        return (h, ps.reshape(-1, 3)) if per_site else h
+
+    def bonds2(self, pos):
+        n = self.s.n
+        ff = np.zeros(n * 3); h = np.zeros(3)
+        b = np.ascontiguousarray(self.s.bonds, dtype=np.int32)
+        e = self.L.orc_cbond2(C.byref(self.cs), np.ascontiguousarray(pos, dtype=np.float64).ravel(), len(b),
+                              np.ascontiguousarray(b[:, 0]), np.ascontiguousarray(b[:, 1]),
+                              np.ascontiguousarray(self.s.bond_r0, dtype=np.float64),
+                              np.ascontiguousarray(self.s.bond_k, dtype=np.float64), ff, h)
+        return e, ff.reshape(n, 3), h
+
+    def cnonbondnl2(self, pos, cnt, lst, xf):
+        n = self.s.n
+        ff = np.zeros(n * 3); en = np.zeros(8); h = np.zeros(3)
+        self.L.orc_cnonbondnl2(C.byref(self.cs), np.ascontiguousarray(pos, dtype=np.float64).ravel(),
+                               np.ascontiguousarray(cnt, dtype=np.int32),
+                               np.ascontiguousarray(lst, dtype=np.int32).ravel(),
+                               np.ascontiguousarray(xf, dtype=np.int32).ravel(), lst.shape[1], self.e_self, ff, en, h)
+        return ff.reshape(n, 3), en, h
 
 
 # --------------------------------------------------------------------------

@@ -521,6 +521,7 @@ void orc_pair(const orc_system *s, const double *pos, const int *count,
     double e14 = s->e14fac;
     double enb = 0.0, enbs = 0.0, er = 0.0, ers = 0.0, ei = 0.0;
     double wxx = 0, wyx = 0, wyy = 0, wzx = 0, wzy = 0, wzz = 0;
+    double hx2 = 0.0, hy2 = 0.0, hz2 = 0.0;   /* cnonbond_nblist2.c config[k].hes* */
     int nt = s->ntypes;
     for (int a = 0; a < s->n - 1; a++) {
         double fxa = 0.0, fya = 0.0, fza = 0.0;
@@ -555,22 +556,41 @@ void orc_pair(const orc_system *s, const double *pos, const int *count,
                 force = (12.0 * eps / dr2) * (sr12 - sr6);
                 enb += eps * (sr12 - 2.0 * sr6);
                 enbs += eps * (sr12s - 2.0 * sr6s);
+                /* cnonbond_nblist2.c:286-297 */
+                double term2 = (12.0 * eps / dr2) / dr2 * (14.0 * sr12 - 8.0 * sr6);
+                hx2 += 2.0 * (term2 * dx * dx - force);
+                hy2 += 2.0 * (term2 * dy * dy - force);
+                hz2 += 2.0 * (term2 * dz * dz - force);
             }
             double dr = sqrt(dr2);
             double dri = 1.0 / dr;
             double kr = kappa * dr;
             double erfckr = orc_erfc(kr);
-            double kpidrexpkr2 = 2.0 * kpi * dr * exp(-kr * kr);
+            double expkr2 = exp(-kr * kr);
+            double kpidrexpkr2 = 2.0 * kpi * dr * expkr2;
+            double dri2 = dri * dri, dri3 = dri2 * dri, dri4 = dri2 * dri2, dri5 = dri3 * dri2;
             qq /= s->eps_rf;
             if (qq != 0.0) {
                 er += qq * erfckr * dri;
                 ers += qq * orc_erfc(kappa * rc) / rc;
                 force += (qq * dri / dr2) * (erfckr + kpidrexpkr2);
+                /* cnonbond_nblist2.c:403-410 */
+                double hes1 = erfckr * dri3 + 2.0 * kpi * dri2 * expkr2;
+                double hes2 = -6.0 * kpi * dri4 * expkr2 - 4.0 * kpi * kappa * kappa * dri2 * expkr2 - 3.0 * dri5 * erfckr;
+                hx2 += -2.0 * qq * (hes1 + dx * dx * hes2);
+                hy2 += -2.0 * qq * (hes1 + dy * dy * hes2);
+                hz2 += -2.0 * qq * (hes1 + dz * dz * hes2);
             }
             if (xf) {
                 if (xf == 4) qqab *= (1.0 - e14);
                 ei += qqab * (1.0 - erfckr) * dri;
                 force -= (qqab * dri / dr2) * ((1.0 - erfckr) - kpidrexpkr2);
+                /* cnonbond_nblist2.c:433-440 */
+                double hes1 = (1.0 - erfckr) * dri3 - 2.0 * kpi * dri2 * expkr2;
+                double hes2 = -6.0 * kpi * dri4 * expkr2 - 4.0 * kpi * kappa * kappa * dri2 * expkr2 + 3.0 * dri5 * (1.0 - erfckr);
+                hx2 += 2.0 * qqab * (hes1 - dx * dx * hes2);
+                hy2 += 2.0 * qqab * (hes1 - dy * dy * hes2);
+                hz2 += 2.0 * qqab * (hes1 - dz * dz * hes2);
             }
             if (force != 0.0) {
                 double fx = force * dx, fy = force * dy, fz = force * dz;
@@ -587,6 +607,7 @@ void orc_pair(const orc_system *s, const double *pos, const int *count,
     out->eewald_real = er; out->eewald_reals = ers; out->eewald_intra = ei;
     out->vir[0] = wxx; out->vir[1] = wyx; out->vir[2] = wyy;
     out->vir[3] = wzx; out->vir[4] = wzy; out->vir[5] = wzz;
+    out->hes[0] = hx2; out->hes[1] = hy2; out->hes[2] = hz2;
 }
 
 /* ======================================================================== */
@@ -764,6 +785,192 @@ void orc_cnonbondnl(const orc_system *s, const double *pos, const int *count,
     energy[6] = enr2;
     energy[7] = enr0;
     for (int i = 0; i < 6; i++) vir_nbond[i] = po.vir[i];
+}
+
+void orc_fillspline2(double w, int order, double *array, double *darray, double *d2array)
+{
+    double a2[64], da2[64];
+    orc_fillspline(w, order, array, darray);
+    for (int k = 0; k <= order; k++) d2array[k] = 0.0;
+    if (order < 3) return;
+    if (order - 2 == 2) { a2[1] = 1.0 - w; a2[2] = w; }   /* M2: the recursion's start */
+    else if (order - 2 == 1) { a2[1] = 1.0; }
+    else orc_fillspline(w, order - 2, a2, da2);
+    for (int k = 1; k <= order; k++) {
+        double am2 = (k - 2 >= 1 && k - 2 <= order - 2) ? a2[k - 2] : 0.0;
+        double am1 = (k - 1 >= 1 && k - 1 <= order - 2) ? a2[k - 1] : 0.0;
+        double a0 = (k <= order - 2) ? a2[k] : 0.0;
+        d2array[k] = am2 - 2.0 * am1 + a0;
+    }
+}
+
+/* theta(m) on the grid: the convolution kernel of the reciprocal energy,
+ * obtained by running the convolution of orc_pme_calc on a unit charge at
+ * grid point 0 (so normalisation and the excluded m = 0 term match). */
+static void pme_theta(const orc_system *s, double *theta)
+{
+    int gx = s->gx, gy = s->gy, gz = s->gz;
+    double blx = s->lx, bly = s->ly, blz = s->lz, vol = blx * bly * blz;
+    double fac = ORC_PI * ORC_PI / s->kappa / s->kappa;
+    size_t ngrid = (size_t)gx * gy * gz, ggyz = (size_t)gy * gz;
+    double *q = (double *)calloc(2 * ngrid, sizeof(double));
+    double *bx = (double *)calloc((size_t)gx + 2, sizeof(double));
+    double *by = (double *)calloc((size_t)gy + 2, sizeof(double));
+    double *bz = (double *)calloc((size_t)gz + 2, sizeof(double));
+    orc_pme_bspline(s->order, gx, gy, gz, bx, by, bz);
+    q[0] = 1.0;
+    orc_fft3d(q, gx, gy, gz, +1);
+    q[0] = q[1] = 0.0;
+    for (size_t ind = 1; ind < ngrid; ind++) {
+        int k1 = (int)(ind / ggyz);
+        size_t knd = ind - (size_t)k1 * ggyz;
+        int k2 = (int)(knd / gz);
+        int k3 = (int)(knd - (size_t)k2 * gz);
+        int mx = k1 - (k1 > gx / 2 ? gx : 0), my = k2 - (k2 > gy / 2 ? gy : 0), mz = k3 - (k3 > gz / 2 ? gz : 0);
+        double mhx = mx / blx, mhy = my / bly, mhz = mz / blz;
+        double m2 = mhx * mhx + mhy * mhy + mhz * mhz;
+        double eterm = exp(-fac * m2) / (ORC_PI * vol * m2 * bx[k1 + 1] * by[k2 + 1] * bz[k3 + 1]);
+        q[2 * ind] *= eterm;
+        q[2 * ind + 1] *= eterm;
+    }
+    orc_fft3d(q, gx, gy, gz, -1);
+    for (size_t i = 0; i < ngrid; i++) theta[i] = q[2 * i];
+    free(q); free(bx); free(by); free(bz);
+}
+
+void orc_pme_hessian(const orc_system *s, const double *pos, double hes3[3], double *per_site)
+{
+    int order = s->order, gx = s->gx, gy = s->gy, gz = s->gz, n = s->n;
+    double blx = s->lx, bly = s->ly, blz = s->lz;
+    size_t ngrid = (size_t)gx * gy * gz;
+    double C = ORC_QFACTOR / s->eps_rf;
+    double *theta = (double *)malloc(sizeof(double) * ngrid);
+    double *conv = (double *)malloc(sizeof(double) * ngrid);
+    pme_theta(s, theta);
+    /* (theta*Q)(m): the convolved grid of orc_pme_calc, taken from the
+     * forces' gather input by recomputing the spread + convolution */
+    {
+        double *q = (double *)calloc(2 * ngrid, sizeof(double));
+        double mn[64], dmn[64];
+        for (int i = 0; i < n; i++) {
+            double u[3] = {gx * (pos[3 * i] / blx + 0.5), gy * (pos[3 * i + 1] / bly + 0.5),
+                           gz * (pos[3 * i + 2] / blz + 0.5)};
+            double m[3][64];
+            for (int d = 0; d < 3; d++) {
+                orc_fillspline(u[d] - (int)u[d], order, mn, dmn);
+                for (int k = 1; k <= order; k++) m[d][k] = mn[k];
+            }
+            for (int a = 1; a <= order; a++)
+                for (int b = 1; b <= order; b++)
+                    for (int c = 1; c <= order; c++) {
+                        int x = (int)u[0] - order + a, y = (int)u[1] - order + b, z = (int)u[2] - order + c;
+                        x = ((x % gx) + gx) % gx; y = ((y % gy) + gy) % gy; z = ((z % gz) + gz) % gz;
+                        q[2 * IDX3(x, y, z)] += m[0][a] * m[1][b] * m[2][c] * s->q[i];
+                    }
+        }
+        /* conv = theta * Q by direct periodic convolution through the FFT
+         * of both (theta is real and even) */
+        double *t = (double *)calloc(2 * ngrid, sizeof(double));
+        for (size_t i = 0; i < ngrid; i++) t[2 * i] = theta[i];
+        orc_fft3d(q, gx, gy, gz, +1);
+        orc_fft3d(t, gx, gy, gz, +1);
+        for (size_t i = 0; i < ngrid; i++) {
+            double ar = q[2 * i], ai = q[2 * i + 1], br = t[2 * i], bi = t[2 * i + 1];
+            q[2 * i] = ar * br - ai * bi;
+            q[2 * i + 1] = ar * bi + ai * br;
+        }
+        orc_fft3d(q, gx, gy, gz, -1);
+        for (size_t i = 0; i < ngrid; i++) conv[i] = q[2 * i] / (double)ngrid;
+        free(q); free(t);
+    }
+    double kl[3] = {gx / blx, gy / bly, gz / blz};
+    int K[3] = {gx, gy, gz};
+    hes3[0] = hes3[1] = hes3[2] = 0.0;
+    for (int i = 0; i < n; i++) {
+        double u[3] = {gx * (pos[3 * i] / blx + 0.5), gy * (pos[3 * i + 1] / bly + 0.5),
+                       gz * (pos[3 * i + 2] / blz + 0.5)};
+        double M[3][64], D[3][64], D2[3][64];
+        int base[3];
+        for (int d = 0; d < 3; d++) {
+            orc_fillspline2(u[d] - (int)u[d], order, M[d], D[d], D2[d]);
+            base[d] = (int)u[d] - order;
+        }
+        double hA[3] = {0, 0, 0}, hB[3] = {0, 0, 0};
+        for (int a = 1; a <= order; a++)
+            for (int b = 1; b <= order; b++)
+                for (int c = 1; c <= order; c++) {
+                    int x = ((base[0] + a) % gx + gx) % gx, y = ((base[1] + b) % gy + gy) % gy,
+                        z = ((base[2] + c) % gz + gz) % gz;
+                    double g = conv[IDX3(x, y, z)];
+                    hA[0] += D2[0][a] * M[1][b] * M[2][c] * g;
+                    hA[1] += M[0][a] * D2[1][b] * M[2][c] * g;
+                    hA[2] += M[0][a] * M[1][b] * D2[2][c] * g;
+                    for (int a2 = 1; a2 <= order; a2++)
+                        for (int b2 = 1; b2 <= order; b2++)
+                            for (int c2 = 1; c2 <= order; c2++) {
+                                int dx = ((a - a2) % K[0] + K[0]) % K[0], dy = ((b - b2) % K[1] + K[1]) % K[1],
+                                    dz = ((c - c2) % K[2] + K[2]) % K[2];
+                                double th = theta[IDX3(dx, dy, dz)];
+                                hB[0] += D[0][a] * M[1][b] * M[2][c] * th * D[0][a2] * M[1][b2] * M[2][c2];
+                                hB[1] += M[0][a] * D[1][b] * M[2][c] * th * M[0][a2] * D[1][b2] * M[2][c2];
+                                hB[2] += M[0][a] * M[1][b] * D[2][c] * th * M[0][a2] * M[1][b2] * D[2][c2];
+                            }
+                }
+        for (int d = 0; d < 3; d++) {
+            double h = C * s->q[i] * kl[d] * kl[d] * (hA[d] + s->q[i] * hB[d]);
+            hes3[d] += h;
+            if (per_site) per_site[3 * i + d] = h;
+        }
+    }
+    free(theta); free(conv);
+}
+
+/* cnonbond_nblist2.c:47-622 */
+void orc_cnonbondnl2(const orc_system *s, const double *pos, const int *count,
+                     const int *list, const int *xflag, int max_nb, double e_self,
+                     double *ff, double energy[8], double hes3[3])
+{
+    orc_pair_out po;
+    orc_pair(s, pos, count, list, xflag, max_nb, ff, &po);
+    double enk[3], vk[6], hk[3];
+    orc_pme_calc(s, pos, e_self, ff, enk, vk, NULL);
+    orc_pme_hessian(s, pos, hk, NULL);
+    double enr0 = po.eewald_real, enr1 = po.eewald_real - po.eewald_reals, enr2 = po.eewald_intra;
+    energy[0] = po.enonbond + enr0 - enr2 + enk[2];
+    energy[1] = energy[0] - po.enonbonds;
+    energy[2] = enr0 + enk[2] - enr2;
+    energy[3] = enr0 - enr1;
+    energy[4] = enk[0];
+    energy[5] = enk[1];
+    energy[6] = enr2;
+    energy[7] = enr0;
+    for (int d = 0; d < 3; d++) hes3[d] = po.hes[d] + hk[d];
+}
+
+/* cbond2.c:65-130 */
+double orc_cbond2(const orc_system *s, const double *pos, int nb, const int *ia,
+                  const int *ib, const double *req, const double *kb, double *ff, double hes3[3])
+{
+    double vir[6];
+    double e = orc_cbond(s, pos, nb, ia, ib, req, kb, ff, vir);
+    double lx = s->lx, ly = s->ly, lz = s->lz;
+    double hx = 0.5 * lx, hy = 0.5 * ly, hz = 0.5 * lz;
+    hes3[0] = hes3[1] = hes3[2] = 0.0;
+    for (int i = 0; i < nb; i++) {
+        int a = ia[i], b = ib[i];
+        double dx = pos[3 * b] - pos[3 * a], dy = pos[3 * b + 1] - pos[3 * a + 1], dz = pos[3 * b + 2] - pos[3 * a + 2];
+        if (dx > hx) dx -= lx; else if (dx < -hx) dx += lx;
+        if (dy > hy) dy -= ly; else if (dy < -hy) dy += ly;
+        if (dz > hz) dz -= lz; else if (dz < -hz) dz += lz;
+        double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+        double dt = r - req[i];
+        double de = 2.0 * kb[i] * dt / r;
+        double term = (2.0 * kb[i] - de) / r2;
+        hes3[0] += 2.0 * (term * dx * dx + de);
+        hes3[1] += 2.0 * (term * dy * dy + de);
+        hes3[2] += 2.0 * (term * dz * dz + de);
+    }
+    return e;
 }
 
 /* ======================================================================== */

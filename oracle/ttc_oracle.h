@@ -102,6 +102,8 @@ typedef struct {
     double enonbond, enonbonds;               /* LJ, LJ cutoff shifts        */
     double eewald_real, eewald_reals, eewald_intra;
     double vir[6];                            /* wxx wyx wyy wzx wzy wzz     */
+    double hes[3];   /* CONFIGT: sum_i d2E/dx_i2, dy_i2, dz_i2 of the pair
+                      * terms (cnonbond_nblist2.c:293-302, 403-440)         */
 } orc_pair_out;
 
 void orc_pair(const orc_system *s, const double *pos, const int *count,
@@ -112,6 +114,27 @@ void orc_pair(const orc_system *s, const double *pos, const int *count,
  * qgrid_out (optional, 2*gx*gy*gz doubles) receives the spread charge grid. */
 void orc_pme_calc(const orc_system *s, const double *pos, double e_self,
                   double *ff, double energy[3], double vir[6], double *qgrid_out);
+
+/* Cardinal B-spline weights with first and second derivatives for the
+ * fractional coordinate w (index convention of pme_spline.c:10-49);
+ * d2array[k] = M''_p, from M''_p(x) = M_{p-2}(x) - 2 M_{p-2}(x-1) + M_{p-2}(x-2). */
+void orc_fillspline2(double w, int order, double *array, double *darray, double *d2array);
+
+/* Diagonal of the Hessian of the SPME reciprocal energy (the reciprocal
+ * part of cnonbondnl2, whose pme_calc2 the reference calls at
+ * cnonbond_nblist2.c:586 but never defines).  E = C/2 sum_{m,m'} Q(m)
+ * theta(m-m') Q(m') with C = QFACTOR/epsRF; for site i and direction d
+ *   d2E/dd_i2 = C q_i (K_d/L_d)^2 [ sum_m W''_i(m) (theta*Q)(m)
+ *              + q_i sum_{m,m'} W'_i(m) theta(m-m') W'_i(m') ],
+ * the exact second derivative of the energy PME evaluates (so the grid
+ * self-interaction of site i is included, as its forces include it).
+ * hes3 = sums over sites; per_site (optional, 3n) per-site values.       */
+void orc_pme_hessian(const orc_system *s, const double *pos, double hes3[3], double *per_site);
+
+/* cbond2.c:65-130: bond energy and forces as orc_cbond plus the Hessian
+ * diagonal sums hes3 (2 * d2e11 etc. per bond).                          */
+double orc_cbond2(const orc_system *s, const double *pos, int nb, const int *ia,
+                  const int *ib, const double *req, const double *kb, double *ff, double hes3[3]);
 
 /* cnonbond_nblist.c:51-622 (STYPE+S14+SPME): pair loop + pme_calc, energy[8]
  * slots as at cnonbond_nblist.c:613-620.                                     */
@@ -165,6 +188,12 @@ double orc_mlp_forward(int in, int H, const double *p, const double *x);
 /* loss = 0.5*mean((f(x_i)-y_i)^2) over m samples; grad += dloss/dp */
 double orc_mlp_loss_grad(int in, int H, const double *p, int m, const double *x,
                          const double *y, double *grad);
+
+/* cnonbond_nblist2.c:47-622 (CONFIGT): orc_cnonbondnl plus the Hessian
+ * diagonal sums of the pair terms and of the reciprocal energy.           */
+void orc_cnonbondnl2(const orc_system *s, const double *pos, const int *count,
+                     const int *list, const int *xflag, int max_nb, double e_self,
+                     double *ff, double energy[8], double hes3[3]);
 
 #ifdef __cplusplus
 }

@@ -87,3 +87,18 @@ def test_shim_compiles_against_reference_headers(tmp_path):
     for name, src in SHIM_SYMBOLS.items():
         text = open(os.path.join(REFDIR, src), errors="replace").read()
         assert re.search(rf"^\s*(void|double)\s+{name}\s*\(", text, flags=re.M), (name, src)
+
+
+@pytest.mark.skipif(not os.path.isdir(REFDIR), reason="reference tree absent")
+def test_shim_configt_entry_point(tmp_path):
+    """In a CONFIGT build the binding also defines cnonbondnl2
+    (cnonbond_nblist2.c:47) over the reference's config[k] struct."""
+    obj = str(tmp_path / "shim2.o")
+    r = subprocess.run(["gcc", "-std=c99", "-O2", "-c", "-DNLIST", "-DDLIST", "-DSTYPE", "-DS14", "-DEWALD",
+                        "-DPRESSURE", "-DCONFIGT", f"-I{REFDIR}", f"-I{ROOT}/include", SHIM, "-o", obj],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run(["nm", "--defined-only", obj], capture_output=True, text=True).stdout
+    assert re.search(r" T cnonbondnl2$", out, flags=re.M)
+    text = open(os.path.join(REFDIR, "cnonbond_nblist2.c"), errors="replace").read()
+    assert re.search(r"^\s*void\s+cnonbondnl2\s*\(", text, flags=re.M)

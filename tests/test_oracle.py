@@ -524,3 +524,82 @@ def test_chain_generator_invariants():
     assert np.allclose(np.linalg.norm(b, axis=1), 1.53, atol=1e-9)
     assert (np.abs(s.pos) <= 0.5 * s.box[0] + 1e-12).all()
     assert abs(s.n / np.prod(s.box) - 0.1007) < 0.003
+
+
+# ---------------------------------------------------------------- CONFIGT Hessian
+def test_fillspline2_closed_form_and_fd():
+    a = np.zeros(65); da = np.zeros(65); d2 = np.zeros(65)
+    for w in (0.0, 0.13, 0.5, 0.87):
+        L.orc_fillspline2(w, 4, a, da, d2)
+        # M4'' in the reference's index order (pme_spline.c:10-49)
+        assert np.allclose(d2[1:5], [1.0 - w, 3.0 * w - 2.0, 1.0 - 3.0 * w, w], atol=1e-14)
+        assert abs(d2[1:5].sum()) < 1e-14
+    h = 1e-4
+    for order in (4, 6):
+        for w in (0.21, 0.64):
+            p = np.zeros(65); m = np.zeros(65); dd = np.zeros(65)
+            L.orc_fillspline2(w, order, a, da, d2)
+            L.orc_fillspline(w + h, order, p, dd)
+            L.orc_fillspline(w - h, order, m, dd)
+            fd = (p - 2 * a + m) / h ** 2
+            assert np.allclose(d2[1:order + 1], fd[1:order + 1], atol=1e-6)
+
+
+def _sparse_system(seed):
+    """60 sites in a 21 A box, no pair within 1e-3 A of the 10 A cutoff
+    (so finite differences never cross it)."""
+    for k in range(50):
+        s = system.make_chain_system(6, 21.0, 20, seed=seed + k)
+        d = s.pos[:, None, :] - s.pos[None, :, :]
+        d -= s.box * np.round(d / s.box)
+        r = np.sqrt((d ** 2).sum(-1))
+        if not (np.abs(r - s.rc) < 1e-3).any():
+            return s
+    raise RuntimeError("no suitable seed")
+
+
+def _fd_laplacian(efun, pos, h):
+    e0 = efun(pos)
+    out = np.zeros((len(pos), 3))
+    for i in range(len(pos)):
+        for d in range(3):
+            p1 = pos.copy(); p1[i, d] += h
+            p2 = pos.copy(); p2[i, d] -= h
+            out[i, d] = (efun(p1) - 2.0 * e0 + efun(p2)) / h ** 2
+    return out
+
+
+def test_pair_hessian_is_energy_second_derivative():
+    s = _sparse_system(11)
+    o = pyoracle.Oracle(s)
+    rc, cnt, lst, xf, *_ = o.nblist(s.pos, 100)
+    assert rc == 0
+
+    def epair(p):
+        _, po = o.pair(p, cnt, lst, xf)
+        return po.enonbond + po.eewald_real - po.eewald_intra
+
+    _, po = o.pair(s.pos, cnt, lst, xf)
+    fd = _fd_laplacian(epair, s.pos, 1e-4).sum(0)
+    print("pair hessian", list(po.hes), fd)
+    assert np.allclose(list(po.hes), fd, rtol=1e-5, atol=1e-3)
+
+
+def test_pme_hessian_is_energy_second_derivative():
+    s = _sparse_system(31)
+    o = pyoracle.Oracle(s)
+    h3, per = o.pme_hessian(s.pos, per_site=True)
+    fd = _fd_laplacian(lambda p: o.pme(p)[1][0], s.pos, 1e-4)
+    print("pme hessian", h3, fd.sum(0))
+    assert np.allclose(per, fd, rtol=1e-4, atol=1e-2)
+    assert np.allclose(h3, per.sum(0), rtol=1e-12)
+
+
+def test_bond_hessian_is_energy_second_derivative():
+    s = _sparse_system(51)
+    o = pyoracle.Oracle(s)
+    p = s.pos + np.random.default_rng(4).uniform(-0.05, 0.05, s.pos.shape)
+    e, f, h3 = o.bonds2(p)
+    fd = _fd_laplacian(lambda q: o.bonds(q)[0], p, 1e-4).sum(0)
+    assert np.allclose(h3, fd, rtol=1e-6, atol=1e-4)
+    assert abs(e - o.bonds(p)[0]) == 0.0

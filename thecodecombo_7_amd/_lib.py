@@ -83,6 +83,9 @@ def lib():
     L.ttc_enbnd_mc.argtypes = [vp, C.c_int, C.c_int, dp]
     L.ttc_nblist_pivot.argtypes = [vp, C.c_int]
     L.ttc_tune.argtypes = [vp, ip, dp]
+    L.ttc_cnonbondnl2.argtypes = [vp, dp, C.c_int64, dp, dp]
+    L.ttc_cbond2.argtypes = [vp, dp, C.c_int64, dp, dp]
+    L.ttc_configtemp.argtypes = [vp, dp, dp, dp, dp]
     L.ttc_set_nhc_temperature.argtypes = [vp, C.c_double, C.c_double]
     L.ttc_scale_velocities.argtypes = [vp, C.c_double]
     L.ttc_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64, C.POINTER(vp)]

@@ -485,6 +485,69 @@ extern "C" int ttc_cbond(ttc_ctx *c, double *ff, int64_t stride, double *ebond, 
     return TTC_OK;
 }
 
+// ------------------------------------------------------------- CONFIGT
+
+extern "C" int ttc_cnonbondnl2(ttc_ctx *c, double *ff, int64_t stride, double *energy8, double *hes3) {
+    int r = need_list(c);
+    if (r) return r;
+    r = ttc_pme_theta(c);
+    if (r) return r;
+    ttc_launch_pair_hes(c);
+    ttc_launch_spread(c);
+    ttc_launch_pme_hes(c, true, false);
+    ttc_launch_reduce(c, false);
+    ttc_launch_combine(c, 1 | 2 | 4);
+    r = finish(c, ff, stride);
+    if (r) return r;
+    if (energy8) slots8(c, energy8);
+    if (hes3) for (int d = 0; d < 3; d++) hes3[d] = c->h_ev[EV_PHX + d] + c->h_ev[EV_KHX + d];
+    c->f4_valid = false;
+    return TTC_OK;
+}
+
+extern "C" int ttc_cbond2(ttc_ctx *c, double *ff, int64_t stride, double *ebond, double *hes3) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "ttc_cbond2: topology not set");
+    int r = ttc_pme_theta(c);   // the Hessian gather variant stages theta even without PME
+    if (r) return r;
+    ttc_launch_pme_hes(c, false, true);
+    ttc_launch_reduce(c, false);
+    ttc_launch_combine(c, 2);
+    r = finish(c, ff, stride);
+    if (r) return r;
+    if (ebond) *ebond = c->h_ev[EV_BOND];
+    if (hes3) for (int d = 0; d < 3; d++) hes3[d] = c->h_ev[EV_BHX + d];
+    c->f4_valid = false;
+    return TTC_OK;
+}
+
+extern "C" int ttc_configtemp(ttc_ctx *c, double *num, double *den, double *T, double *hes3) {
+    int r = need_list(c);
+    if (r) return r;
+    r = ttc_pme_theta(c);
+    if (r) return r;
+    ttc_launch_pair_hes(c);
+    ttc_launch_spread(c);
+    ttc_launch_pme_hes(c, true, true);
+    ttc_launch_reduce(c, false);
+    ttc_launch_combine(c, 7);
+    TTC_HIP(hipGetLastError());
+    std::vector<double4> f(c->n);
+    TTC_HIP(hipMemcpyAsync(c->h_ev, c->ev, sizeof(double) * EV_N, hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipMemcpyAsync(f.data(), c->f4, sizeof(double4) * c->n, hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    double f2 = 0.0;
+    for (const double4 &v : f) f2 += v.x * v.x + v.y * v.y + v.z * v.z;
+    double h[3];
+    for (int d = 0; d < 3; d++) h[d] = c->h_ev[EV_PHX + d] + c->h_ev[EV_KHX + d] + c->h_ev[EV_BHX + d];
+    double hr = h[0] + h[1] + h[2];
+    if (num) *num = f2;
+    if (den) *den = hr;
+    if (T) *T = f2 / hr / (TTC_KB * TTC_NA) / 0.001;   // configtemp.c:43-45
+    if (hes3) for (int d = 0; d < 3; d++) hes3[d] = h[d];
+    c->f4_valid = true;   // f4 = the full force at the current positions
+    return TTC_OK;
+}
+
 // one full force evaluation into f4 (+ energies in ev)
 static void launch_forces(ttc_ctx *c, bool energy) {
     ttc_launch_pair(c, energy, 0);

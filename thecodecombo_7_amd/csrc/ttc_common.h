@@ -53,6 +53,10 @@ enum {
     EV_BOND, EV_BV0, EV_BV1, EV_BV2, EV_BV3, EV_BV4, EV_BV5, // bonds
     EV_KIN, EV_KE0, EV_KE1, EV_KE2, EV_KE3, EV_KE4, EV_KE5,  // kinetic tensor
     EV_FSX, EV_FSY, EV_FSZ,                           // sum of PME forces
+    // CONFIGT: sums over sites of d2U/dx_i2, dy_i2, dz_i2 (Hessian diagonal)
+    EV_PHX, EV_PHY, EV_PHZ,                           // pair terms
+    EV_KHX, EV_KHY, EV_KHZ,                           // reciprocal energy
+    EV_BHX, EV_BHY, EV_BHZ,                           // bonds
     EV_N
 };
 
@@ -131,6 +135,9 @@ struct ttc_ctx {
     float *qr = nullptr;
     float *eterm = nullptr;
     FftPlan plan[3];
+    // theta(d) for grid offsets |d_x|,|d_y|,|d_z| < order: the real-space
+    // kernel of the reciprocal energy near the origin (CONFIGT Hessian)
+    float *theta_near = nullptr;
 
     // reductions: partials[slot * nblk_max + block]
     double *partials = nullptr;
@@ -170,6 +177,7 @@ int ttc_cellcap(const ttc_ctx *c);
 // [lb,ub) of the original numbering (energies only)
 void ttc_launch_pair(ttc_ctx *c, bool energy, int blk_off);
 void ttc_launch_pair_range(ttc_ctx *c, int lb, int ub);
+void ttc_launch_pair_hes(ttc_ctx *c);    // energies + Hessian diagonal (CONFIGT)
 int ttc_pair_blocks(ttc_ctx *c);
 // pme (+ bonds in the gather)
 int ttc_pme_setup(ttc_ctx *c);
@@ -177,6 +185,9 @@ int ttc_pme_attrs(ttc_ctx *c);   // LDS limits of the spread kernel (needs the c
 void ttc_pme_free(ttc_ctx *c);
 // gather: pme -> PME forces, bonds -> bond forces (either may be off)
 void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, bool bonds);
+// energies + forces + Hessian diagonal (CONFIGT); needs ttc_pme_theta first
+void ttc_launch_pme_hes(ttc_ctx *c, bool pme, bool bonds);
+int ttc_pme_theta(ttc_ctx *c);   // fills theta_near (once per context)
 int ttc_pme_conv_blocks(ttc_ctx *c);
 int ttc_gather_blocks(ttc_ctx *c);
 void ttc_launch_spread(ttc_ctx *c);

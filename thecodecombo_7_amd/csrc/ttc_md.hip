@@ -205,6 +205,8 @@ void ttc_launch_reduce(ttc_ctx *c, bool append_row) {
     for (int i = EV_BOND; i <= EV_BV5; i++) ra.nb[i] = ttc_gather_blocks(c);
     for (int i = EV_KE0; i <= EV_KE5; i++) ra.nb[i] = ttc_kick_blocks(c);
     for (int i = EV_FSX; i <= EV_FSZ; i++) ra.nb[i] = ttc_gather_blocks(c);
+    for (int i = EV_PHX; i <= EV_PHZ; i++) ra.nb[i] = ttc_pair_blocks(c);
+    for (int i = EV_KHX; i <= EV_BHZ; i++) ra.nb[i] = ttc_gather_blocks(c);
     double fan = 0.5 * TTC_QFACTOR / c->p.eps_rf;
     k_reduce<<<1, 256, 0, c->stream>>>(ra, c->partials, c->nblk_max, c->ev, c->e_self, fan, c->rows, c->rows_cap,
                                        c->status, append_row ? 1 : 0);

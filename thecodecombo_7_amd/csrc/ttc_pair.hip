@@ -38,6 +38,7 @@ __device__ __forceinline__ float nr_erfc(float z, float *ez2) {
 struct PairAcc {
     double fx, fy, fz;
     double elj, eljs, er, ers, ei, w0, w1, w2, w3, w4, w5;
+    double hx, hy, hz;   // CONFIGT: this site's d2U/dx2, dy2, dz2 (pair terms)
 };
 
 // One list entry of site i (cnonbond_nblist.c:150-519 for one pair).
@@ -48,7 +49,7 @@ struct RangeSel {
 
 // RANGE: keep only pairs with a site in [lb,ub) of the original numbering
 // (enbnd_mc.c:98-119); i_in says whether the row's own site is in range.
-template <bool ENERGY, bool RANGE = false>
+template <bool ENERGY, bool RANGE = false, bool HES = false>
 __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pi, float qi, const float4 *ljrow,
                                            const double4 *__restrict__ pos4, const int *__restrict__ type,
                                            const PairConst &pc, PairAcc &a, const RangeSel &rs = RangeSel{},
@@ -89,6 +90,33 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pi, float 
     force += qq * rinv3 * (erfckr + kpid);
     float qi2 = cls == 2 ? pc.one_m_e14 * qqab : qqab;
     if (cls) force -= qi2 * rinv3 * ((1.0f - erfckr) - kpid);
+    if (HES) {
+        // Hessian diagonal of this site, one visit of the pair (the full
+        // list visits it from both sites, giving the reference's factor 2):
+        // cnonbond_nblist2.c:286-297 (LJ), 403-410 (real), 433-440 (excluded)
+        float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
+        float flj = 12.0f * eps * rinv2 * (sr12 - sr6);
+        float term2 = 12.0f * eps * rinv2 * rinv2 * (14.0f * sr12 - 8.0f * sr6);
+        float kpi = 0.5f * pc.kpi2;
+        float rinv4 = rinv2 * rinv2, rinv5 = rinv4 * rinv;
+        float e4 = 4.0f * kpi * pc.kappa * pc.kappa * rinv2 * ez2 + 6.0f * kpi * rinv4 * ez2;
+        float h1 = erfckr * rinv3 + 2.0f * kpi * rinv2 * ez2;
+        float h2 = -e4 - 3.0f * rinv5 * erfckr;
+        float ax = term2 * fdx * fdx - flj - qq * (h1 + fdx * fdx * h2);
+        float ay = term2 * fdy * fdy - flj - qq * (h1 + fdy * fdy * h2);
+        float az = term2 * fdz * fdz - flj - qq * (h1 + fdz * fdz * h2);
+        if (cls) {
+            float om = 1.0f - erfckr;
+            float g1 = om * rinv3 - 2.0f * kpi * rinv2 * ez2;
+            float g2 = -e4 + 3.0f * rinv5 * om;
+            ax += qi2 * (g1 - fdx * fdx * g2);
+            ay += qi2 * (g1 - fdy * fdy * g2);
+            az += qi2 * (g1 - fdz * fdz * g2);
+        }
+        a.hx += (double)ax;
+        a.hy += (double)ay;
+        a.hz += (double)az;
+    }
     float fxf = force * (float)dx, fyf = force * (float)dy, fzf = force * (float)dz;
     a.fx += (double)fxf;
     a.fy += (double)fyf;
@@ -115,7 +143,7 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pi, float 
 // read) and their partial forces are summed in a fixed wave order.  More
 // waves per group shorten each group's dependent load chain (small boxes,
 // few groups); fewer suit many groups.  ttc_tune picks WAVES per context.
-template <bool ENERGY, bool RANGE, int WAVES>
+template <bool ENERGY, bool RANGE, bool HES, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
 k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__ glen,
        const double4 *__restrict__ pos4, const int *__restrict__ type, const float4 *__restrict__ ljtab,
@@ -123,7 +151,7 @@ k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__
        RangeSel rs = RangeSel{}) {
     extern __shared__ float4 slj[];
     __shared__ double sred[WAVES][64][3];
-    __shared__ double ered[WAVES][11];
+    __shared__ double ered[WAVES][14];
     int nt2 = pc.ntypes * pc.ntypes;
     for (int k = threadIdx.x; k < nt2; k += blockDim.x) slj[k] = ljtab[k];
     __syncthreads();
@@ -147,17 +175,18 @@ k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__
     for (; k + WAVES < len; k += 2 * WAVES) {
         uint32_t e0 = row[(size_t)k * 64];
         uint32_t e1 = row[(size_t)(k + WAVES) * 64];
-        pair_entry<ENERGY, RANGE>(e0, pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
-        pair_entry<ENERGY, RANGE>(e1, pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
+        pair_entry<ENERGY, RANGE, HES>(e0, pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
+        pair_entry<ENERGY, RANGE, HES>(e1, pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
     }
-    if (k < len) pair_entry<ENERGY, RANGE>(row[(size_t)k * 64], pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
+    if (k < len) pair_entry<ENERGY, RANGE, HES>(row[(size_t)k * 64], pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
 
     sred[wave][lane][0] = a.fx;
     sred[wave][lane][1] = a.fy;
     sred[wave][lane][2] = a.fz;
     if (ENERGY) {
-        double v[11] = {a.elj, a.eljs, a.er, a.ers, a.ei, a.w0, a.w1, a.w2, a.w3, a.w4, a.w5};
-        for (int q = 0; q < 11; q++) {
+        const int nq = HES ? 14 : 11;
+        double v[14] = {a.elj, a.eljs, a.er, a.ers, a.ei, a.w0, a.w1, a.w2, a.w3, a.w4, a.w5, a.hx, a.hy, a.hz};
+        for (int q = 0; q < nq; q++) {
             double t = ttc_wave_sum(v[q]);
             if (lane == 0) ered[wave][q] = t;
         }
@@ -177,6 +206,12 @@ k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__
         double t = 0.0;
         for (int w = 0; w < WAVES; w++) t += ered[w][threadIdx.x];
         partials[(size_t)(EV_LJ + threadIdx.x) * maxb + blockIdx.x] = 0.5 * t;
+    }
+    if (HES && threadIdx.x >= 11 && threadIdx.x < 14) {
+        // per-site diagonal terms: already one per visit, no halving
+        double t = 0.0;
+        for (int w = 0; w < WAVES; w++) t += ered[w][threadIdx.x];
+        partials[(size_t)(EV_PHX + threadIdx.x - 11) * maxb + blockIdx.x] = t;
     }
 }
 
@@ -203,13 +238,13 @@ static PairConst pair_const(const ttc_ctx *c) {
     return pc;
 }
 
-template <bool ENERGY, bool RANGE>
+template <bool ENERGY, bool RANGE, bool HES = false>
 static void launch_pair_w(ttc_ctx *c, RangeSel rs) {
     PairConst pc = pair_const(c);
     int blocks = ttc_pair_blocks(c);
     size_t shm = sizeof(float4) * c->ntypes * c->ntypes;
 #define PAIR_LAUNCH(W)                                                                                           \
-    k_pair<ENERGY, RANGE, W><<<blocks, 64 * W, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, \
+    k_pair<ENERGY, RANGE, HES, W><<<blocks, 64 * W, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, \
                                                                  c->ljtab, pc, c->fpair, c->partials, c->nblk_max, rs)
     switch (c->pair_waves) {
     case 4: PAIR_LAUNCH(4); break;
@@ -221,6 +256,10 @@ static void launch_pair_w(ttc_ctx *c, RangeSel rs) {
 
 void ttc_launch_pair_range(ttc_ctx *c, int lb, int ub) {
     launch_pair_w<true, true>(c, RangeSel{c->orig, lb, ub});
+}
+
+void ttc_launch_pair_hes(ttc_ctx *c) {
+    launch_pair_w<true, false, true>(c, RangeSel{});
 }
 
 void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {

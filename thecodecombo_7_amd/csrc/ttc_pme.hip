@@ -61,6 +61,40 @@ __device__ __forceinline__ void bspline(float w, float *M, float *D) {
     for (int i = 1; i <= P; i++) M[i] = a[i];
 }
 
+// a[1..Q] = order-Q cardinal B-spline weights at fractional coordinate w,
+// the recursion of pme_spline.c:10-49 run to order Q (Q >= 2)
+template <int Q>
+__device__ __forceinline__ void bspline_m(float w, float *a) {
+#pragma unroll
+    for (int i = 0; i <= Q; i++) a[i] = 0.0f;
+    a[2] = w;
+    a[1] = 1.0f - w;
+#pragma unroll
+    for (int i = 3; i <= Q; i++) {
+        float div = 1.0f / (float)(i - 1);
+        a[i] = div * w * a[i - 1];
+#pragma unroll
+        for (int j = 1; j <= i - 2; j++)
+            a[i - j] = div * ((w + (float)j) * a[i - j - 1] + ((float)i - (float)j - w) * a[i - j]);
+        a[1] *= div * (1.0f - w);
+    }
+}
+
+// second derivatives in the same index order: M''_P(x) = M_{P-2}(x)
+// - 2 M_{P-2}(x-1) + M_{P-2}(x-2)
+template <int P>
+__device__ __forceinline__ void bspline_d2(float w, float *D2) {
+    float a[P + 1];
+    bspline_m<P - 2>(w, a);
+#pragma unroll
+    for (int k = 1; k <= P; k++) {
+        float am2 = (k - 2 >= 1) ? a[k - 2] : 0.0f;
+        float am1 = (k - 1 >= 1 && k - 1 <= P - 2) ? a[k - 1] : 0.0f;
+        float a0 = (k <= P - 2) ? a[k] : 0.0f;
+        D2[k] = am2 - 2.0f * am1 + a0;
+    }
+}
+
 // grid coordinate u = K (x/L + 0.5) (pme_qgrid.c:26-34); fl = (int)u.
 // Wrapped positions give 0 <= u <= K, where floor equals the reference's
 // truncation; the base index is reduced modulo K so that a site exactly at
@@ -458,15 +492,23 @@ struct GatherConst {
     double qscale;   // QFACTOR / epsRF
 };
 
-template <int P, bool ENERGY>
+template <int P, bool ENERGY, bool HES = false>
 __global__ void __launch_bounds__(256)
 k_gather(int n, const double4 *__restrict__ pos4, const int *__restrict__ orig, const int *__restrict__ inv,
          const float *__restrict__ qr, GatherConst gc, const int *__restrict__ b_off,
          const int *__restrict__ b_part, const double *__restrict__ b_r0, const double *__restrict__ b_k,
-         double4 *__restrict__ g4, double *partials, int maxb, int do_pme, int do_bonds) {
+         double4 *__restrict__ g4, double *partials, int maxb, int do_pme, int do_bonds,
+         const float *__restrict__ theta_near = nullptr) {
+    constexpr int ND = 2 * P - 1;   // grid offsets -(P-1) .. P-1
+    __shared__ float sth[HES ? ND * ND * ND : 1];
+    if (HES) {
+        for (int i = threadIdx.x; i < ND * ND * ND; i += blockDim.x) sth[i] = theta_near[i];
+        __syncthreads();
+    }
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     double px = 0.0, py = 0.0, pz = 0.0;
     double eb = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
+    double kh[3] = {0.0, 0.0, 0.0}, bh[3] = {0.0, 0.0, 0.0};
     if (s < n) {
         double4 p = pos4[s];
         if (do_pme) {
@@ -505,6 +547,70 @@ k_gather(int n, const double4 *__restrict__ pos4, const int *__restrict__ orig, 
         px = (double)ax * (f * gc.K0 / gc.lx);
         py = (double)ay * (f * gc.K1 / gc.ly);
         pz = (double)az * (f * gc.K2 / gc.lz);
+        if (HES) {
+            // d2E/dd_i2 = C q_i (K_d/L_d)^2 [sum_m W''(m) (theta*Q)(m)
+            //             + q_i sum_{m,m'} W'(m) theta(m-m') W'(m')]
+            float D2x[P + 1], D2y[P + 1], D2z[P + 1];
+            bspline_d2<P>(wx, D2x);
+            bspline_d2<P>(wy, D2y);
+            bspline_d2<P>(wz, D2z);
+            float hA[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int xi = 1; xi <= P; xi++) {
+                int gx = wrapk(fx - P + xi, gc.K0);
+#pragma unroll
+                for (int yi = 1; yi <= P; yi++) {
+                    int gy = wrapk(fy - P + yi, gc.K1);
+                    const float *rowp = qr + ((size_t)gx * gc.K1 + gy) * gc.K2;
+                    float sm = 0.0f, sd2 = 0.0f;
+#pragma unroll
+                    for (int zi = 1; zi <= P; zi++) {
+                        float t = rowp[wrapk(fz - P + zi, gc.K2)];
+                        sm += Mz[zi] * t;
+                        sd2 += D2z[zi] * t;
+                    }
+                    hA[0] += D2x[xi] * My[yi] * sm;
+                    hA[1] += Mx[xi] * D2y[yi] * sm;
+                    hA[2] += Mx[xi] * My[yi] * sd2;
+                }
+            }
+            // autocorrelations of the weights and of their derivatives over
+            // the stencil, then the sum over grid offsets of theta
+            float cm[3][ND], cd[3][ND];
+            const float *Mw[3] = {Mx, My, Mz}, *Dw[3] = {Dx, Dy, Dz};
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int o = 0; o < ND; o++) {
+                    int delta = o - (P - 1);
+                    float am = 0.0f, ad = 0.0f;
+#pragma unroll
+                    for (int a = 1; a <= P; a++) {
+                        int b = a - delta;
+                        if (b >= 1 && b <= P) { am += Mw[d][a] * Mw[d][b]; ad += Dw[d][a] * Dw[d][b]; }
+                    }
+                    cm[d][o] = am;
+                    cd[d][o] = ad;
+                }
+            float hB[3] = {0.0f, 0.0f, 0.0f};
+            for (int ox = 0; ox < ND; ox++)
+                for (int oy = 0; oy < ND; oy++) {
+                    const float *th = sth + (ox * ND + oy) * ND;
+                    float tm = 0.0f, td = 0.0f;
+#pragma unroll
+                    for (int oz = 0; oz < ND; oz++) {
+                        tm += th[oz] * cm[2][oz];
+                        td += th[oz] * cd[2][oz];
+                    }
+                    hB[0] += cd[0][ox] * cm[1][oy] * tm;
+                    hB[1] += cm[0][ox] * cd[1][oy] * tm;
+                    hB[2] += cm[0][ox] * cm[1][oy] * td;
+                }
+            double kl[3] = {gc.K0 / gc.lx, gc.K1 / gc.ly, gc.K2 / gc.lz};
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                kh[d] = f * kl[d] * kl[d] * ((double)hA[d] + p.w * (double)hB[d]);
+        }
         }
         // bonds of this site (cbond.c:65-120), each bond seen from both ends
         double bx = 0.0, by = 0.0, bz = 0.0;
@@ -520,6 +626,15 @@ k_gather(int n, const double4 *__restrict__ pos4, const int *__restrict__ orig, 
             double fr = -b_k[k] * rr * 2.0;
             double fxb = fr / dr * dx, fyb = fr / dr * dy, fzb = fr / dr * dz;
             bx -= fxb; by -= fyb; bz -= fzb;
+            if (HES) {
+                // cbond2.c:105-122, one end of the bond (the other end adds
+                // the same amount: the reference's factor 2)
+                double de = -fr / dr;
+                double term = (2.0 * b_k[k] - de) / (dr * dr);
+                bh[0] += term * dx * dx + de;
+                bh[1] += term * dy * dy + de;
+                bh[2] += term * dz * dz + de;
+            }
             if (ENERGY) {
                 eb += -(fr * rr * 0.5) * 0.5;
                 v0 += 0.5 * fxb * dx; v1 += 0.5 * fyb * dx; v2 += 0.5 * fyb * dy;
@@ -529,10 +644,10 @@ k_gather(int n, const double4 *__restrict__ pos4, const int *__restrict__ orig, 
         g4[s] = make_double4(px + bx, py + by, pz + bz, 0.0);
     }
     // per-block sums (PME force sum always; bond energy/virial on demand)
-    __shared__ double red[4][10];
+    __shared__ double red[4][16];
     int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double vals[10] = {px, py, pz, eb, v0, v1, v2, v3, v4, v5};
-    int nv = ENERGY ? 10 : 3;
+    double vals[16] = {px, py, pz, eb, v0, v1, v2, v3, v4, v5, kh[0], kh[1], kh[2], bh[0], bh[1], bh[2]};
+    int nv = HES ? 16 : (ENERGY ? 10 : 3);
     for (int q = 0; q < nv; q++) {
         double t = ttc_wave_sum(vals[q]);
         if (lane == 0) red[wid][q] = t;
@@ -540,7 +655,8 @@ k_gather(int n, const double4 *__restrict__ pos4, const int *__restrict__ orig, 
     __syncthreads();
     if (threadIdx.x < nv) {
         double t = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-        int slot = threadIdx.x < 3 ? EV_FSX + threadIdx.x : EV_BOND + (threadIdx.x - 3);
+        int q = threadIdx.x;
+        int slot = q < 3 ? EV_FSX + q : (q < 10 ? EV_BOND + (q - 3) : (q < 13 ? EV_KHX + (q - 10) : EV_BHX + (q - 13)));
         partials[(size_t)slot * maxb + blockIdx.x] = t;
     }
 }
@@ -656,6 +772,8 @@ int ttc_pme_attrs(ttc_ctx *c) { return set_lds_attrs(c);
 
 void ttc_pme_free(ttc_ctx *c) {
     hipFree(c->qfix); hipFree(c->qc); hipFree(c->qr); hipFree(c->eterm);
+    if (c->theta_near) hipFree(c->theta_near);
+    c->theta_near = nullptr;
     for (int a = 0; a < 3; a++) hipFree(c->plan[a].tw);
 }
 
@@ -738,7 +856,7 @@ static size_t fft_shm(const ttc_ctx *c, int axis) {
 
 // FFT passes, influence function, gather (+ bonds).  Expects the spread
 // grid in qfix; leaves qfix zeroed.
-void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, bool bonds) {
+static void launch_pme_impl(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes) {
     double fac = TTC_PI * TTC_PI / c->p.kappa / c->p.kappa;
     double lx = c->p.lx, ly = c->p.ly, lz = c->p.lz;
     int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2];
@@ -762,11 +880,46 @@ void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, bool bonds) {
     gc.K0 = K0; gc.K1 = K1; gc.K2 = K2;
     gc.qscale = TTC_QFACTOR / c->p.eps_rf;
     int blocks = ttc_gather_blocks(c);
-#define GATHER(P, E)                                                                                    \
-    k_gather<P, E><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->orig, c->inv, c->qr, gc, c->b_off,   \
-                                                  c->b_part, c->b_r0, c->b_k, c->g4, c->partials, c->nblk_max, \
-                                                  pme ? 1 : 0, bonds ? 1 : 0)
-    if (c->order == 4) { if (energy) GATHER(4, true); else GATHER(4, false); }
-    else { if (energy) GATHER(6, true); else GATHER(6, false); }
+#define GATHER(P, E, H)                                                                                    \
+    k_gather<P, E, H><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->orig, c->inv, c->qr, gc, c->b_off,   \
+                                                     c->b_part, c->b_r0, c->b_k, c->g4, c->partials,         \
+                                                     c->nblk_max, pme ? 1 : 0, bonds ? 1 : 0, c->theta_near)
+    if (c->order == 4) {
+        if (hes) GATHER(4, true, true); else if (energy) GATHER(4, true, false); else GATHER(4, false, false);
+    } else {
+        if (hes) GATHER(6, true, true); else if (energy) GATHER(6, true, false); else GATHER(6, false, false);
+    }
 #undef GATHER
+}
+
+void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, bool bonds) { launch_pme_impl(c, energy, pme, bonds, false); }
+
+void ttc_launch_pme_hes(ttc_ctx *c, bool pme, bool bonds) { launch_pme_impl(c, true, pme, bonds, true); }
+
+// theta near the origin: the FFT/convolution pipeline run on a unit charge
+// at grid point 0 leaves theta(m) in qr (the kernel the gather convolves
+// with); offsets |d| < order per axis are kept.  Runs once per context.
+int ttc_pme_theta(ttc_ctx *c) {
+    if (c->theta_near) return TTC_OK;
+    const int P = c->order, ND = 2 * P - 1;
+    int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2];
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    long long one = (long long)TTC_QFIX_SCALE;
+    TTC_HIP(hipMemcpyAsync(c->qfix, &one, sizeof one, hipMemcpyHostToDevice, c->stream));
+    launch_pme_impl(c, false, true, false, false);   // FFTs; the gather output is not used
+    TTC_HIP(hipGetLastError());
+    std::vector<float> grid((size_t)K0 * K1 * K2);
+    TTC_HIP(hipMemcpyAsync(grid.data(), c->qr, sizeof(float) * grid.size(), hipMemcpyDeviceToHost, c->stream));
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    std::vector<float> near((size_t)ND * ND * ND);
+    for (int ox = 0; ox < ND; ox++)
+        for (int oy = 0; oy < ND; oy++)
+            for (int oz = 0; oz < ND; oz++) {
+                int x = ((ox - (P - 1)) % K0 + K0) % K0, y = ((oy - (P - 1)) % K1 + K1) % K1,
+                    z = ((oz - (P - 1)) % K2 + K2) % K2;
+                near[(ox * ND + oy) * ND + oz] = grid[((size_t)x * K1 + y) * K2 + z];
+            }
+    TTC_HIP(hipMalloc(&c->theta_near, sizeof(float) * near.size()));
+    TTC_HIP(hipMemcpy(c->theta_near, near.data(), sizeof(float) * near.size(), hipMemcpyHostToDevice));
+    return TTC_OK;
 }

@@ -2,7 +2,8 @@
  * ttc_refshim.c -- the reference-side binding of libttc.so.
  *
  * A maintainer of the reference C code links this file instead of
- * nblist.c, nblist_pivot.c, cnonbond_nblist.c, enbnd_mc.c, pme_qgrid.c,
+ * nblist.c, nblist_pivot.c, cnonbond_nblist.c (and cnonbond_nblist2.c in a
+ * CONFIGT build), enbnd_mc.c, pme_qgrid.c,
  * pme_calc.c, pme_setup.c, pme_spline.c and the SPME part of ewald.c.  It
  * defines the reference's own entry points with the reference's own
  * signatures, reads and writes the reference's global structs from
@@ -185,6 +186,22 @@ void cnonbondnl(int ibox, double *energy)
     for (int i = 0; i < 6; i++) { pvir[k].nbond[i] = vn[i]; pvir[k].ewald_kspace[i] = vk[i]; }
 #endif
 }
+
+#ifdef CONFIGT
+/* cnonbond_nblist2.c:47 -- as cnonbondnl, plus the Hessian diagonal of the
+ * nonbonded terms added into config[k].hesx/y/z (forces.c:232-235 zeroes
+ * them); the reciprocal part is the exact second derivative of the SPME
+ * energy (the reference's pme_calc2 does not exist) */
+void cnonbondnl2(int ibox, double *energy)
+{
+    int k = ibox;
+    double h[3];
+    ok(ttc_cnonbondnl2(upload(k), &ff[k][0].x, 3, energy, h), "cnonbondnl2");
+    config[k].hesx += h[0];
+    config[k].hesy += h[1];
+    config[k].hesz += h[2];
+}
+#endif
 
 /* pme_calc.c:18 -- charge_grid, both FFTs, convolution and pme_force */
 void pme_calc(int k, double *energy)

@@ -142,6 +142,22 @@ class Box:
         T.check(self.L.ttc_kinet(self.h, T.dptr(k), T.dptr(ken)))
         return float(k[0]), ken
 
+    # ---- CONFIGT: cnonbond_nblist2.c, cbond2.c, configtemp.c ----
+    def cnonbondnl2(self):
+        ff = np.zeros((self.n, 3)); en = np.zeros(8); h = np.zeros(3)
+        T.check(self.L.ttc_cnonbondnl2(self.h, T.dptr(ff), 3, T.dptr(en), T.dptr(h)))
+        return ff, en, h
+
+    def cbond2(self):
+        ff = np.zeros((self.n, 3)); eb = np.zeros(1); h = np.zeros(3)
+        T.check(self.L.ttc_cbond2(self.h, T.dptr(ff), 3, T.dptr(eb), T.dptr(h)))
+        return ff, float(eb[0]), h
+
+    def configtemp(self):
+        num = np.zeros(1); den = np.zeros(1); t = np.zeros(1); h = np.zeros(3)
+        T.check(self.L.ttc_configtemp(self.h, T.dptr(num), T.dptr(den), T.dptr(t), T.dptr(h)))
+        return float(t[0]), float(num[0]), float(den[0]), h
+
     # ---- integrate_vverlet.c ----
     def integrate_vverlet(self, nsteps: int, vscal: float = 1.0, energy_every: int = 0):
         rows = max(1, nsteps // energy_every) if energy_every else 1
