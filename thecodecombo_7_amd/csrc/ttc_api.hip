@@ -365,10 +365,29 @@ extern "C" int ttc_nblist(ttc_ctx *c) {
             continue;
         }
         c->have_list = true;
+        c->list_pruned = false;
         c->f4_valid = false;   // the sorted order changed
         return TTC_OK;
     }
     ttc_set_error("ttc_nblist: list capacity could not be grown");
+    return TTC_ERR_LIST_OVERFLOW;
+}
+
+int ttc_launch_build_at_anchors(ttc_ctx *c);
+
+int ttc_rebuild_full_at_anchors(ttc_ctx *c) {
+    for (int attempt = 0; attempt < 3; attempt++) {
+        TTC_HIP(hipMemsetAsync(c->status + ST_OVERFLOW, 0, sizeof(int) * 2, c->stream));
+        int r = ttc_launch_build_at_anchors(c);
+        if (r) return r;
+        TTC_HIP(hipGetLastError());
+        r = read_status(c);
+        if (r) return r;
+        if (!(c->h_status[ST_OVERFLOW] & 1)) return TTC_OK;
+        r = grow_list(c, c->h_status[ST_MAXROW]);
+        if (r) return r;
+    }
+    ttc_set_error("ttc: list capacity could not be grown");
     return TTC_ERR_LIST_OVERFLOW;
 }
 
@@ -392,9 +411,22 @@ extern "C" int ttc_get_nblist(ttc_ctx *c, int which, int max_nb, int *count, int
     return ttc_export_list(c, cutoff, max_nb, count, list, xflag);
 }
 
+// The reference evaluates with the list of its last nblist() whatever the
+// positions are now.  A list made inside an MD loop is pruned to
+// rc + 2 (rclist-rc)/4 (complete while no site moved more than the DLIST
+// distance); if a site moved further since, the list is rebuilt at the full
+// rclist from the anchors -- the reference's list -- before evaluating.
 static int need_list(ttc_ctx *c) {
     CHECK(c && c->topo, TTC_ERR_ARG, "topology not set");
     if (!c->have_list) return ttc_nblist(c);
+    if (!c->list_pruned) return TTC_OK;
+    int flag = 0;
+    int r = ttc_nl_check_range(c, 0, c->n, &flag);
+    if (r) return r;
+    if (!flag) return TTC_OK;
+    r = ttc_rebuild_full_at_anchors(c);
+    if (r) return r;
+    c->list_pruned = false;
     return TTC_OK;
 }
 
@@ -591,9 +623,13 @@ extern "C" int ttc_kinet(ttc_ctx *c, double *kinet, double *ken6) {
 
 // One velocity-Verlet step as launched (and captured): conditional rebuild,
 // kick + drift + spread, pair, PME (+ bonds), kick + trigger.
+// integrate_vverlet.c:36-103 with the DLIST trigger of nve_md.c:369-370:
+// the drift checks the displacement and the conditional rebuild runs
+// before this step's forces, with the same anchors (this step's positions)
+// the reference rebuilds at after the step.
 static void launch_step(ttc_ctx *c, double vscal, bool energy) {
-    ttc_launch_rebuild(c, true);
     ttc_launch_drift(c, vscal);
+    ttc_launch_rebuild(c, true);
     ttc_launch_pair(c, energy, 0);
     ttc_launch_pme(c, energy, true, true);
     ttc_launch_kick(c, vscal, energy);
@@ -657,6 +693,7 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
     int nr = std::min(c->h_status[ST_NROWS], max_rows);
     if (nrows) *nrows = energy_every > 0 ? nr : 0;
     if (rebuilds) *rebuilds = c->h_status[ST_NREBUILD];
+    if (c->h_status[ST_NREBUILD] > 0) c->list_pruned = true;
     if (energy_every > 0 && nr > 0)
         TTC_HIP(hipMemcpy(energies, c->rows, sizeof(double) * ROW_LEN * nr, hipMemcpyDeviceToHost));
     c->mts_valid = false;
@@ -783,19 +820,21 @@ extern "C" int ttc_force_long(ttc_ctx *c, double *ff_long, int64_t stride, doubl
     return TTC_OK;
 }
 
-// one RESPA long step (integrate_mts.c:38-140) preceded by the conditional
-// rebuild; the trigger is checked after the last kick
+// one RESPA long step (integrate_mts.c:38-140); the DLIST trigger is
+// checked after the short steps and the conditional rebuild runs before
+// the long forces of the new positions
 static void launch_mts_step(ttc_ctx *c, int ninner, double dtlong, double vscal, bool energy) {
     double dts = c->p.dt;
-    ttc_launch_rebuild(c, true);
-    ttc_launch_vkick(c, c->fl, vscal, vscal * 0.5 * dtlong, false, false, true);
+    ttc_launch_vkick(c, c->fl, vscal, vscal * 0.5 * dtlong, false, false, false);
     for (int j = 0; j < ninner; j++) {
         ttc_launch_vkick_drift(c, c->fs, 1.0, vscal * 0.5 * dts, dts);
         launch_force_short(c, energy && j == ninner - 1);
         ttc_launch_vkick(c, c->fs, 1.0, vscal * 0.5 * dts, false, false, false);
     }
+    ttc_launch_nlcheck(c);
+    ttc_launch_rebuild(c, true);
     launch_force_long(c, energy);
-    ttc_launch_vkick(c, c->fl, 1.0, vscal * 0.5 * dtlong, true, energy, false);
+    ttc_launch_vkick(c, c->fl, 1.0, vscal * 0.5 * dtlong, false, energy, true);
     if (energy) ttc_launch_reduce(c, true);
 }
 
@@ -832,6 +871,7 @@ static int run_graphs(ttc_ctx *c, hipGraphExec_t plain, hipGraphExec_t energy, i
     int nr = std::min(c->h_status[ST_NROWS], max_rows);
     if (nrows) *nrows = energy_every > 0 ? nr : 0;
     if (rebuilds) *rebuilds = c->h_status[ST_NREBUILD];
+    if (c->h_status[ST_NREBUILD] > 0) c->list_pruned = true;
     if (energy_every > 0 && nr > 0)
         TTC_HIP(hipMemcpy(energies, c->rows, sizeof(double) * ROW_LEN * nr, hipMemcpyDeviceToHost));
     if (c->h_status[ST_OVERFLOW]) {
@@ -937,8 +977,8 @@ void ttc_launch_nhc_step(ttc_ctx *c, bool energy) {
     ttc_launch_kinet(c);
     ttc_launch_nhc(c);
     ttc_launch_vscale(c);
-    ttc_launch_rebuild(c, true);
     ttc_launch_drift(c, 1.0);
+    ttc_launch_rebuild(c, true);
     ttc_launch_pair(c, energy, 0);
     ttc_launch_pme(c, energy, true, true);
     ttc_launch_kick(c, 1.0, true);   // kinetic partials for the second thermostat half

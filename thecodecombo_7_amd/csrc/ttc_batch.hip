@@ -252,6 +252,7 @@ extern "C" int ttc_batch_md_nhc(ttc_batch *b, int nsteps, double *potens, int *r
             return TTC_ERR_LIST_OVERFLOW;
         }
         if (rebuilds) rebuilds[i] = c->h_status[ST_NREBUILD];
+        if (c->h_status[ST_NREBUILD] > 0) c->list_pruned = true;
         if (potens && nsteps > 0) {
             double row[ROW_LEN];
             TTC_HIP(hipMemcpy(row, c->rows, sizeof row, hipMemcpyDeviceToHost));

@@ -74,6 +74,7 @@ struct ttc_ctx {
     double e_self = 0.0;
     bool topo = false;
     bool have_list = false;
+    bool list_pruned = false;   // list built in an MD loop at the pruned cutoff
     hipStream_t stream = nullptr;
 
     // sorted-order state
@@ -172,6 +173,11 @@ void ttc_launch_nlcheck(ttc_ctx *c);
 int ttc_export_list(ttc_ctx *c, double cutoff, int max_nb, int *count, int *list, int *xflag);
 int ttc_nl_check_range(ttc_ctx *c, int lb, int ub, int *flag);
 double ttc_list_cutoff(const ttc_ctx *c);
+// cutoff of lists built inside MD loops: rc + 2 (rclist-rc)/4, complete for
+// every pair within rc while the DLIST trigger has not fired (each site
+// within (rclist-rc)/4 of its anchor)
+double ttc_compute_cutoff(const ttc_ctx *c);
+int ttc_rebuild_full_at_anchors(ttc_ctx *c);   // list at the full cutoff from the anchors
 int ttc_cellcap(const ttc_ctx *c);
 // pair; the range variant weights each pair by whether a site lies in
 // [lb,ub) of the original numbering (energies only)

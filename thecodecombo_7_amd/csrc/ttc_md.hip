@@ -72,18 +72,20 @@ __device__ __forceinline__ void ken_block(double m, double4 v, double *partials,
     }
 }
 
-// integrate_vverlet.c:93-102 (second half kick) on the combined force, then
-// the DLIST check of nblist.c:607-647 on the new positions (it decides the
-// rebuild at the start of the next step, as nve_md.c:369-370 does).
+// integrate_vverlet.c:93-102 (second half kick) on the combined force.  The
+// DLIST flag this step's drift set has been consumed by this step's
+// conditional rebuild: it is cleared here.
 template <bool ENERGY>
 __global__ void __launch_bounds__(256)
-k_kick(int n, const double4 *pos4, double4 *vel4, const double4 *fpair, const double4 *g4, double4 *f4,
-       const double4 *anchor, double dt, double vscal, const double *pcheck, double *partials, int maxb,
-       int nbg, int *status) {
+k_kick(int n, double4 *vel4, const double4 *fpair, const double4 *g4, double4 *f4, double dt, double vscal,
+       double *partials, int maxb, int nbg, int *status) {
     __shared__ double mean[3];
     mean_pme(partials, maxb, nbg, n, mean);
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0) status[ST_STEP] += 1;
+    if (s == 0) {
+        status[ST_STEP] += 1;
+        status[ST_NLFLAG] = 0;
+    }
     double4 v = make_double4(0, 0, 0, 1);
     bool ok = s < n;
     if (ok) {
@@ -95,17 +97,6 @@ k_kick(int n, const double4 *pos4, double4 *vel4, const double4 *fpair, const do
         v.y = v.y + vscal * dth * v.w * f.y;
         v.z = v.z + vscal * dth * v.w * f.z;
         vel4[s] = v;
-        // displacement trigger
-        double4 p = pos4[s], a = anchor[s];
-        double lx = pcheck[0], ly = pcheck[1], lz = pcheck[2], rcut = pcheck[3];
-        double dx = fabs(p.x - a.x);
-        if (dx > 0.5 * lx) dx -= lx;
-        double dy = fabs(p.y - a.y);
-        if (dy > 0.5 * ly) dy -= ly;
-        double dz = fabs(p.z - a.z);
-        if (dz > 0.5 * lz) dz -= lz;
-        if (dx > rcut || dy > rcut || dz > rcut || ttc_r2_exact(dx, dy, dz) > rcut * rcut)
-            atomicOr(&status[ST_NLFLAG], 1);
     }
     if (ENERGY) ken_block(100.0 / v.w, v, partials, maxb, ok);
 }
@@ -184,13 +175,11 @@ static double *check_consts(ttc_ctx *c) {
 void ttc_launch_kick(ttc_ctx *c, double vscal, bool energy) {
     int blocks = ttc_kick_blocks(c);
     if (energy)
-        k_kick<true><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->vel4, c->fpair, c->g4, c->f4, c->anchor,
-                                                    c->p.dt, vscal, check_consts(c), c->partials, c->nblk_max,
-                                                    ttc_gather_blocks(c), c->status);
+        k_kick<true><<<blocks, 256, 0, c->stream>>>(c->n, c->vel4, c->fpair, c->g4, c->f4, c->p.dt, vscal,
+                                                    c->partials, c->nblk_max, ttc_gather_blocks(c), c->status);
     else
-        k_kick<false><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->vel4, c->fpair, c->g4, c->f4, c->anchor,
-                                                     c->p.dt, vscal, check_consts(c), c->partials, c->nblk_max,
-                                                     ttc_gather_blocks(c), c->status);
+        k_kick<false><<<blocks, 256, 0, c->stream>>>(c->n, c->vel4, c->fpair, c->g4, c->f4, c->p.dt, vscal,
+                                                     c->partials, c->nblk_max, ttc_gather_blocks(c), c->status);
 }
 
 void ttc_launch_kinet(ttc_ctx *c) {

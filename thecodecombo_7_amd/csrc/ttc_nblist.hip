@@ -293,6 +293,11 @@ double ttc_list_cutoff(const ttc_ctx *c) {
     return c->p.ewald_lists ? std::min(c->p.rclist, c->p.rclist_ew) : c->p.rclist;
 }
 
+double ttc_compute_cutoff(const ttc_ctx *c) {
+    double pruned = c->p.rc + 0.5 * (c->p.rclist - c->p.rc) + 1e-6;
+    return std::min(ttc_list_cutoff(c), pruned);
+}
+
 // capacity of one cell's member slots
 int ttc_cellcap(const ttc_ctx *c) {
     double vcell = (c->p.lx / c->ncx) * (c->p.ly / c->ncy) * (c->p.lz / c->ncz);
@@ -301,14 +306,15 @@ int ttc_cellcap(const ttc_ctx *c) {
     return cap;
 }
 
-// The per-component early rejects use sim.rclist whichever list is built
-// (nblist.c:116-127 test against rcut = sim.rclist also while filling
-// nlist_ew), so a wider nlist_ew keeps only pairs with every |d| <= rclist.
+// The per-component early rejects use sim.rclist whichever reference list
+// is built (nblist.c:116-127 test against rcut = sim.rclist also while
+// filling nlist_ew), so a wider nlist_ew keeps only pairs with every
+// |d| <= rclist; lists built inside MD loops reject at their own cutoff.
 static void launch_build(ttc_ctx *c, bool conditional, double cutoff, uint32_t *nbl, int *glen,
-                         int cap, bool copy_state) {
+                         int cap, bool copy_state, double component_cut) {
     CellGeom g = geom(c);
     k_build<<<c->G, 64 * BUILD_WAVES, 0, c->stream>>>(
-        c->n, cap, g, c->p.rclist, cutoff * cutoff, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4,
+        c->n, cap, g, component_cut, cutoff * cutoff, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4,
         c->vel4, c->f4, c->type, c->orig, c->anchor, c->inv, c->fsb, c->flb, c->fs, c->fl, c->cell_cnt, c->cell_start, c->ex_off, c->ex_part, c->ex_flag,
         c->ex_span, nbl, glen, c->status, conditional ? 1 : 0, copy_state ? 1 : 0);
 }
@@ -326,7 +332,21 @@ int ttc_launch_rebuild(ttc_ctx *c, bool conditional) {
                                              c->cell_members, c->orig, c->pos4, c->vel4, c->f4, c->type,
                                              c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->fs, c->fl,
                                              c->fsb, c->flb, c->status, cond);
-    launch_build(c, conditional, ttc_list_cutoff(c), c->nbl, c->glen, c->cap, true);
+    // explicit builds (nblist.c:32) make the reference's list; the
+    // conditional builds of MD loops make the pruned one
+    if (conditional)
+        launch_build(c, true, ttc_compute_cutoff(c), c->nbl, c->glen, c->cap, true, ttc_compute_cutoff(c));
+    else
+        launch_build(c, false, ttc_list_cutoff(c), c->nbl, c->glen, c->cap, true, c->p.rclist);
+    return TTC_OK;
+}
+
+// the reference's list (full cutoff) at the anchors, in the current sorted
+// order (made from the anchors), without touching the site state
+int ttc_launch_build_at_anchors(ttc_ctx *c) {
+    TTC_HIP(hipMemcpyAsync(c->pos4b, c->anchor, sizeof(double4) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    TTC_HIP(hipMemcpyAsync(c->origb, c->orig, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    launch_build(c, false, ttc_list_cutoff(c), c->nbl, c->glen, c->cap, false, c->p.rclist);
     return TTC_OK;
 }
 
@@ -354,18 +374,18 @@ int ttc_export_list(ttc_ctx *c, double cutoff, int max_nb, int *count, int *list
     int cap = c->cap;
     uint32_t *tmp_nbl = nullptr;
     int *tmp_glen = nullptr;
-    if (cutoff != ttc_list_cutoff(c)) {
+    if (cutoff != ttc_list_cutoff(c) || c->list_pruned) {
         // Another cutoff (nlist_ew wider than nlist): build a scratch list at
         // the anchor positions, i.e. the positions of the last ttc_nblist, for
         // which the current sorted order and cell ranges were made.  Cells
         // are sized for the wider cutoff at creation.
-        double ratio = cutoff / ttc_list_cutoff(c);
+        double ratio = std::max(1.0, cutoff / ttc_list_cutoff(c));
         cap = ((int)(c->cap * ratio * ratio * ratio) + 127) & ~63;
         TTC_HIP(hipMalloc(&tmp_nbl, (size_t)c->G * cap * 64 * sizeof(uint32_t)));
         TTC_HIP(hipMalloc(&tmp_glen, (size_t)c->G * sizeof(int)));
         TTC_HIP(hipMemcpyAsync(c->pos4b, c->anchor, sizeof(double4) * c->n, hipMemcpyDeviceToDevice, c->stream));
         TTC_HIP(hipMemcpyAsync(c->origb, c->orig, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
-        launch_build(c, false, cutoff, tmp_nbl, tmp_glen, cap, false);
+        launch_build(c, false, cutoff, tmp_nbl, tmp_glen, cap, false, c->p.rclist);
         nbl = tmp_nbl;
         glen = tmp_glen;
     }

@@ -192,14 +192,18 @@ __device__ __forceinline__ int tile_base(int fl, int org, int ext, int k, int P)
 }
 
 // kick + drift + pbc (integrate_vverlet.c:52-83) of every site of a brick,
-// then charge_grid (pme_qgrid.c:77-113) into an LDS tile of int64 fixed
-// point, flushed to the global fixed-point grid with one atomic per
-// touched grid point.  Integer adds make the grid independent of order.
+// the DLIST displacement check of nblist.c:607-647 on the new position
+// (pcheck = {lx, ly, lz, (rclist-rc)/4}; it sets the flag the conditional
+// rebuild that follows in the same step reads, so the forces of this step
+// are evaluated with a list valid for these positions), then charge_grid
+// (pme_qgrid.c:77-113) into an LDS tile of int64 fixed point, flushed to
+// the global fixed-point grid with one atomic per touched grid point.
+// Integer adds make the grid independent of order.
 template <int P, bool DRIFT>
 __global__ void __launch_bounds__(256)
 k_drift_spread(double4 *pos4, double4 *vel4, const double4 *f4, double dt, double vscal, Spread sp, Brick br,
                const int *__restrict__ cell_start, const int *__restrict__ cell_cnt, long long *qfix,
-               int *status) {
+               int *status, const double4 *__restrict__ anchor, const double *__restrict__ pcheck) {
     extern __shared__ unsigned long long tile[];
     int b = blockIdx.x;
     int ibx = b % br.nbx, iby = (b / br.nbx) % br.nby, ibz = b / (br.nbx * br.nby);
@@ -211,7 +215,6 @@ k_drift_spread(double4 *pos4, double4 *vel4, const double4 *f4, double dt, doubl
     tile_range(cz0, cz1, br.cwz, sp.lz, sp.K2, br.delta, P, &oz, &tz);
     int tsz = tx * ty * tz;
     for (int t = threadIdx.x; t < tsz; t += blockDim.x) tile[t] = 0ull;
-    if (DRIFT && b == 0 && threadIdx.x == 0) status[ST_NLFLAG] = 0;   // consumed by the rebuild
     __syncthreads();
     for (int cz = cz0; cz < cz1; cz++)
         for (int cy = cy0; cy < cy1; cy++) {
@@ -233,6 +236,16 @@ k_drift_spread(double4 *pos4, double4 *vel4, const double4 *f4, double dt, doubl
                     p.z = pbc1(p.z + dt * v.z, sp.lz);
                     vel4[s] = v;
                     pos4[s] = p;
+                    double4 a = anchor[s];
+                    double rcut = pcheck[3];
+                    double dx = fabs(p.x - a.x);
+                    if (dx > 0.5 * sp.lx) dx -= sp.lx;
+                    double dy = fabs(p.y - a.y);
+                    if (dy > 0.5 * sp.ly) dy -= sp.ly;
+                    double dz = fabs(p.z - a.z);
+                    if (dz > 0.5 * sp.lz) dz -= sp.lz;
+                    if (dx > rcut || dy > rcut || dz > rcut || ttc_r2_exact(dx, dy, dz) > rcut * rcut)
+                        atomicOr(&status[ST_NLFLAG], 1);
                 }
                 double ux = (double)sp.K0 * (p.x / sp.lx + 0.5), uy = (double)sp.K1 * (p.y / sp.ly + 0.5),
                        uz = (double)sp.K2 * (p.z / sp.lz + 0.5);
@@ -824,7 +837,8 @@ static void launch_drift_spread(ttc_ctx *c, bool drift, double vscal) {
     double dt = drift ? c->p.dt : 0.0;
 #define DS(P, D)                                                                                         \
     k_drift_spread<P, D><<<blocks, 256, shm, c->stream>>>(c->pos4, c->vel4, c->f4, dt, vscal, sp, br,   \
-                                                          c->cell_start, c->cell_cnt, c->qfix, c->status)
+                                                          c->cell_start, c->cell_cnt, c->qfix, c->status, \
+                                                          c->anchor, c->ev + EV_N)
     if (c->order == 4) { if (drift) DS(4, true); else DS(4, false); }
     else { if (drift) DS(6, true); else DS(6, false); }
 #undef DS
