@@ -85,7 +85,7 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
     TTC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     int n = c->n;
     ALLOC(c->pos4, n); ALLOC(c->vel4, n); ALLOC(c->f4, n); ALLOC(c->anchor, n);
-    ALLOC(c->pos4b, n); ALLOC(c->vel4b, n); ALLOC(c->f4b, n);
+    ALLOC(c->pos4b, n); ALLOC(c->vel4b, n); ALLOC(c->f4b, n); ALLOC(c->posfb, n);
     ALLOC(c->type, n); ALLOC(c->typeb, n); ALLOC(c->orig, n); ALLOC(c->origb, n); ALLOC(c->inv, n);
     ALLOC(c->fpair, n); ALLOC(c->g4, n);
     ALLOC(c->fs, n); ALLOC(c->fl, n); ALLOC(c->fsb, n); ALLOC(c->flb, n);
@@ -157,7 +157,7 @@ extern "C" int ttc_destroy(ttc_ctx *c) {
                     c->origb, c->inv, c->fpair, c->g4, c->cell_of, c->cell_cnt, c->cell_start, c->cell_acc,
                     c->cell_members, c->nbl, c->glen, c->status, c->ex_off, c->ex_part, c->ex_flag, c->ljtab,
                     c->b_off, c->b_part, c->b_r0, c->b_k, c->partials, c->ev, c->rows, c->fs, c->fl, c->fsb,
-                    c->flb, c->nhc};
+                    c->flb, c->nhc, c->posfb};
     for (void *b : bufs)
         if (b) hipFree(b);
     ttc_pme_free(c);

@@ -80,6 +80,7 @@ struct ttc_ctx {
     // sorted-order state
     double4 *pos4 = nullptr, *vel4 = nullptr, *f4 = nullptr, *anchor = nullptr;
     double4 *pos4b = nullptr, *vel4b = nullptr, *f4b = nullptr, *anchorb = nullptr;
+    float4 *posfb = nullptr;        // FP32 positions in the new order (fast list build)
     int *type = nullptr, *typeb = nullptr, *orig = nullptr, *origb = nullptr, *inv = nullptr;
     long long *fix = nullptr;
 

@@ -84,7 +84,7 @@ __global__ void k_reorder(int n, int cellcap, const int *cell_of, const int *cel
                           const double4 *pos4, const double4 *vel4, const double4 *f4, const int *type,
                           double4 *pos4b, double4 *vel4b, double4 *f4b, int *typeb, int *origb,
                           const double4 *fs, const double4 *fl, double4 *fsb, double4 *flb,
-                          int *status, int conditional) {
+                          float4 *posfb, int *status, int conditional) {
     if (rebuild_off(status, conditional)) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -95,7 +95,9 @@ __global__ void k_reorder(int n, int cellcap, const int *cell_of, const int *cel
     const int *mem = members + (size_t)c * cellcap;
     for (int k = 0; k < m; k++) r += (orig[mem[k]] < o);
     int s = cell_start[c] + r;
-    pos4b[s] = pos4[i];
+    double4 p = pos4[i];
+    pos4b[s] = p;
+    posfb[s] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);   // candidates of the fast build
     vel4b[s] = vel4[i];
     f4b[s] = f4[i];   // the last force feeds the next half kick
     fsb[s] = fs[i];   // RESPA short / long forces likewise
@@ -122,6 +124,31 @@ __device__ __forceinline__ bool ref_pair_test(const double4 &pi, const double4 &
     return (dx <= rcut) & (dy <= rcut) & (dz <= rcut) & (dr2 < rcut2);
 }
 
+// The test of lists built inside MD loops: FP32 positions and a cutoff
+// widened by a margin far above the FP32 rounding of the distance, so the
+// list is a superset of the pairs within the exact cutoff (the pair kernel
+// applies the exact FP64 cutoff test to every entry).
+__device__ __forceinline__ bool fast_pair_test(const float4 &pi, const float4 &pj, float hx, float hy, float hz,
+                                               float lx, float ly, float lz, float cut2) {
+    float dx = fabsf(pi.x - pj.x);
+    dx = dx > hx ? dx - lx : dx;
+    float dy = fabsf(pi.y - pj.y);
+    dy = dy > hy ? dy - ly : dy;
+    float dz = fabsf(pi.z - pj.z);
+    dz = dz > hz ? dz - lz : dz;
+    return dx * dx + dy * dy + dz * dz < cut2;
+}
+
+template <bool FAST>
+__device__ __forceinline__ bool cand_hit(int j, const float4 &pif, const double4 &pi, const float4 *posfb,
+                                         const double4 *pos4b, const CellGeom &g, double rcut, double rcut2,
+                                         float cutf2) {
+    if (FAST)
+        return fast_pair_test(pif, posfb[j], (float)g.hx, (float)g.hy, (float)g.hz, (float)g.lx, (float)g.ly,
+                              (float)g.lz, cutf2);
+    return ref_pair_test(pi, pos4b[j], g, rcut, rcut2);
+}
+
 // smallest periodic distance from coordinate x to the slab [lo, lo + w)
 __device__ __forceinline__ double slab_gap(double x, double lo, double w, double l) {
     double d = x - lo;
@@ -135,8 +162,12 @@ __device__ __forceinline__ double slab_gap(double x, double lo, double w, double
 // goes to wave t % BUILD_WAVES) and skip cells farther than rcut.  Pass 1
 // counts each site's hits per wave, pass 2 writes them at prefix offsets,
 // so the row order is fixed (stencil order within a wave, waves in order).
+// FAST (lists of MD loops): candidates from the FP32 copy with
+// fast_pair_test at cutf2; otherwise the reference's exact test.
+template <bool FAST>
 __global__ void __launch_bounds__(64 * BUILD_WAVES)
-k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__restrict__ pos4b,
+k_build(int n, int cap, CellGeom g, double rcut, double rcut2, float cutf2, const float4 *__restrict__ posfb,
+        const double4 *__restrict__ pos4b,
         const double4 *__restrict__ vel4b, const double4 *__restrict__ f4b, const int *__restrict__ typeb,
         const int *__restrict__ origb, double4 *pos4, double4 *vel4, double4 *f4, int *type, int *orig,
         double4 *anchor, int *inv, const double4 *__restrict__ fsb, const double4 *__restrict__ flb,
@@ -147,6 +178,9 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__
     if (rebuild_off(status, conditional)) return;
     __shared__ int scnt[BUILD_WAVES][64];
     __shared__ int smax;
+    // hit masks of the (at most 125) stencil cells: cell t of a site goes to
+    // wave t % BUILD_WAVES, slot t / BUILD_WAVES
+    __shared__ unsigned long long hm[BUILD_WAVES][(125 + BUILD_WAVES - 1) / BUILD_WAVES][64];
     int grp = blockIdx.x;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int s = grp * 64 + lane;
@@ -171,12 +205,17 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__
     int nst = nsx * nsy * nsz;
     double cwx = g.lx / g.ncx, cwy = g.ly / g.ncy, cwz = g.lz / g.ncz;
     int xb = valid ? ex_off[oi] : 0, xe = valid ? ex_off[oi + 1] : 0;
+    float4 pif = make_float4((float)pi.x, (float)pi.y, (float)pi.z, 0.0f);
     uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
 
+    // pass 0 tests every candidate once and keeps, per stencil cell, the hits
+    // among its first 64 sites as a bit mask in LDS; pass 1 writes the
+    // entries from the masks (re-testing only sites beyond the 64th of a
+    // cell), at the per-wave prefix offsets of the pass-0 counts
     int mine = 0, off = 0;
     for (int pass = 0; pass < 2; pass++) {
         int k = off;
-        for (int t = wave; t < nst && valid; t += BUILD_WAVES) {
+        for (int t = wave, slot = 0; t < nst && valid; t += BUILD_WAVES, slot++) {
             int ox = t % nsx, oy = (t / nsx) % nsy, oz = t / (nsx * nsy);
             int x = g.ncx >= 5 ? (cx + ox - 2 + g.ncx) % g.ncx : ox;
             int y = g.ncy >= 5 ? (cy + oy - 2 + g.ncy) % g.ncy : oy;
@@ -188,10 +227,31 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__
             if (gx * gx + gy * gy + gz * gz > rcut2 * 1.0001 + 1e-9) continue;
             int c = x + g.ncx * (y + g.ncy * z);
             int j0 = cell_start[c], j1 = j0 + cell_cnt[c];
-            for (int j = j0; j < j1; j++) {
-                double4 pj = pos4b[j];
-                if (j == s || !ref_pair_test(pi, pj, g, rcut, rcut2)) continue;
-                if (pass == 0) { mine++; continue; }
+            unsigned long long mask = 0ull;
+            int jt = j0;   // first site tested again in pass 1
+            if (pass == 0) {
+                for (int j = j0; j < j1; j++) {
+                    if (j == s || !cand_hit<FAST>(j, pif, pi, posfb, pos4b, g, rcut, rcut2, cutf2)) continue;
+                    mine++;
+                    if (j - j0 < 64) mask |= 1ull << (j - j0);
+                }
+                hm[wave][slot][lane] = mask;
+                continue;
+            }
+            mask = hm[wave][slot][lane];
+            jt = j0 + 64;
+            while (true) {
+                int j;
+                if (mask) {
+                    j = j0 + __builtin_ctzll(mask);
+                    mask &= mask - 1;
+                } else {
+                    // sites past the 64th of the cell (dense cells only)
+                    while (jt < j1 && (jt == s || !cand_hit<FAST>(jt, pif, pi, posfb, pos4b, g, rcut, rcut2, cutf2)))
+                        jt++;
+                    if (jt >= j1) break;
+                    j = jt++;
+                }
                 uint32_t cls = 0;
                 int oj = origb[j];
                 int d = oj - oi;
@@ -310,11 +370,17 @@ int ttc_cellcap(const ttc_ctx *c) {
 // is built (nblist.c:116-127 test against rcut = sim.rclist also while
 // filling nlist_ew), so a wider nlist_ew keeps only pairs with every
 // |d| <= rclist; lists built inside MD loops reject at their own cutoff.
+// margin of the fast (FP32) test, A: far above FP32 rounding of distances
+// in boxes up to 1000 A, negligible in list length
+#define FAST_MARGIN 0.01
+
 static void launch_build(ttc_ctx *c, bool conditional, double cutoff, uint32_t *nbl, int *glen,
-                         int cap, bool copy_state, double component_cut) {
+                         int cap, bool copy_state, double component_cut, bool fast = false) {
     CellGeom g = geom(c);
-    k_build<<<c->G, 64 * BUILD_WAVES, 0, c->stream>>>(
-        c->n, cap, g, component_cut, cutoff * cutoff, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4,
+    double cf = cutoff + FAST_MARGIN;
+    auto kern = fast ? k_build<true> : k_build<false>;
+    kern<<<c->G, 64 * BUILD_WAVES, 0, c->stream>>>(
+        c->n, cap, g, component_cut, fast ? cf * cf : cutoff * cutoff, (float)(cf * cf), c->posfb, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4,
         c->vel4, c->f4, c->type, c->orig, c->anchor, c->inv, c->fsb, c->flb, c->fs, c->fl, c->cell_cnt, c->cell_start, c->ex_off, c->ex_part, c->ex_flag,
         c->ex_span, nbl, glen, c->status, conditional ? 1 : 0, copy_state ? 1 : 0);
 }
@@ -331,11 +397,11 @@ int ttc_launch_rebuild(ttc_ctx *c, bool conditional) {
     k_reorder<<<blocks, 256, 0, c->stream>>>(c->n, cellcap, c->cell_of, c->cell_cnt, c->cell_start,
                                              c->cell_members, c->orig, c->pos4, c->vel4, c->f4, c->type,
                                              c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->fs, c->fl,
-                                             c->fsb, c->flb, c->status, cond);
+                                             c->fsb, c->flb, c->posfb, c->status, cond);
     // explicit builds (nblist.c:32) make the reference's list; the
     // conditional builds of MD loops make the pruned one
     if (conditional)
-        launch_build(c, true, ttc_compute_cutoff(c), c->nbl, c->glen, c->cap, true, ttc_compute_cutoff(c));
+        launch_build(c, true, ttc_compute_cutoff(c), c->nbl, c->glen, c->cap, true, ttc_compute_cutoff(c), true);
     else
         launch_build(c, false, ttc_list_cutoff(c), c->nbl, c->glen, c->cap, true, c->p.rclist);
     return TTC_OK;
