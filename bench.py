@@ -148,6 +148,61 @@ def mc_cpu_baseline(chains, acc_ratio):
                       "oracle FP64 range energy old+new per trial; scaled by the GPU plain acceptance ratio"}
 
 
+def large_leg(steps: int, cpu: bool):
+    """chains-1M (10^6 sites, 215 A box, PME 224^3, rc 12 A, list 14 A): the
+    pair kernel's and the spreading step's algorithmic GB/s and roofline
+    fraction, step time, and the step-0 force error against the FP64 oracle
+    (cell-list build of the same pair set, nblist.c CLIST semantics)."""
+    from thecodecombo_7_amd import system
+    from thecodecombo_7_amd.nonbond import Box
+    s = system.chains_1m()
+    box = Box(s)
+    box.nblist()
+    entries, _, _ = box.list_stats()
+    half = entries // 2
+    f0, _ = box.forces()
+    waves, _ = box.tune()
+    box.integrate_vverlet(20)
+    t0 = time.perf_counter()
+    _, nreb = box.integrate_vverlet(steps)
+    wall = (time.perf_counter() - t0) / steps
+    t_pair = box.timing(0)
+    t_spread = box.timing(1)
+    pair_bytes = 4.0 * half + 48.0 * s.n
+    spread_bytes = 16.0 * s.n + 4.0 * s.grid[0] * s.grid[1] * s.grid[2]
+    out = {
+        "workload": "chains-1M: 100,000 chains x 10 sites, 215 A box, PME 224^3 order 4, rc 12 A, list 14 A, "
+                    "NVE velocity Verlet 1 fs",
+        "steps": steps, "ms_per_step": round(1e3 * wall, 4), "ns_per_day": round(0.0864 / wall, 3),
+        "rebuilds": nreb, "half_list_entries": half, "pair_waves": waves,
+        "pair_kernel": {"us": round(t_pair, 2), "algorithmic_bytes": pair_bytes,
+                        "achieved_GBs": round(pair_bytes / (t_pair * 1e-6) / 1e9, 1),
+                        "frac_of_8TBs": round(pair_bytes / (t_pair * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
+        "spread_kernel": {"us": round(t_spread, 2), "algorithmic_bytes": spread_bytes,
+                          "achieved_GBs": round(spread_bytes / (t_spread * 1e-6) / 1e9, 1),
+                          "frac_of_8TBs": round(spread_bytes / (t_spread * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+    box.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.Oracle(s, fast=True)
+        t0 = time.perf_counter()
+        rc, cnt, lst, xf = o.nblist(s.pos, 1300, cells=True)
+        t_list = time.perf_counter() - t0
+        if rc:
+            raise RuntimeError("oracle list overflow at chains-1M")
+        t0 = time.perf_counter()
+        ff, _ = o.forces(s.pos, lists=(cnt, lst, xf))
+        t_force = time.perf_counter() - t0
+        del lst, xf
+        out["force_rel_rms_vs_fp64"] = float(np.sqrt(((f0 - ff) ** 2).sum() / (ff ** 2).sum()))
+        out["cpu_step0"] = {"list_s": round(t_list, 2), "forces_s": round(t_force, 2), "cores": 1,
+                            "kind": "port", "sample": "oracle cell-list build + one force evaluation "
+                                                      "(pair loop, PME 224^3, bonds) on the step-0 state"}
+    return out
+
+
 COPY_MIB = 512
 
 
@@ -161,20 +216,22 @@ def traffic_probe():
     s = system.chains_100k()
     box = Box(s)
     box.nblist()
+    entries, _, _ = box.list_stats()
+    box.integrate_vverlet(50)          # the MD loop's (pruned) list, as in the timed run
+    box.tune()
     box.timing(0)
     src = torch.ones(COPY_MIB << 18, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
     torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
-    cnt, _, _ = box.get_nblist(0, 1500)
     box.close()
-    print(json.dumps({"half_list_entries": int(cnt.sum()), "sites": s.n, "copy_mib": COPY_MIB}), flush=True)
+    print(json.dumps({"half_list_entries": int(entries // 2), "sites": s.n, "copy_mib": COPY_MIB}), flush=True)
 
 
 def _pmc_per_kernel(path, counter):
-    """Mean per-dispatch value of one counter for each kernel name in a
-    rocprofv3 counter_collection.csv."""
+    """Mean per-dispatch value of one counter and the dispatch count for
+    each kernel name in a rocprofv3 counter_collection.csv."""
     import csv
     acc = {}
     with open(path) as fh:
@@ -185,7 +242,7 @@ def _pmc_per_kernel(path, counter):
             key = (name, row.get("Dispatch_Id", row.get("Correlation_Id")))
             acc.setdefault(name, {}).setdefault(key, 0.0)
             acc[name][key] += float(row["Counter_Value"])
-    return {k: sum(v.values()) / len(v) for k, v in acc.items()}
+    return {k: (sum(v.values()) / len(v), len(v)) for k, v in acc.items()}
 
 
 def traffic_summary(fetch_csv, write_csv, out_path):
@@ -193,15 +250,16 @@ def traffic_summary(fetch_csv, write_csv, out_path):
     scaled by the copy kernel's reading against its known COPY_MIB bytes."""
     f = _pmc_per_kernel(fetch_csv, "FETCH_SIZE")
     w = _pmc_per_kernel(write_csv, "WRITE_SIZE")
-    pick = lambda d, key: next(v for k, v in d.items() if key(k))
-    is_pair = lambda k: "k_pair" in k
+    # the pair kernel variant timed most often (the tuned one), the copy kernel
+    pick = lambda d, key: max(((v[1], v[0]) for k, v in d.items() if key(k)))[1]
+    is_pair = lambda k: "k_pair<false, false, false" in k
     is_copy = lambda k: "vectorized_elementwise_kernel" in k and "FillFunctor" not in k
     known_kib = COPY_MIB * 1024.0
     sf = known_kib / pick(f, is_copy)
     sw = known_kib / pick(w, is_copy)
     fetch = pick(f, is_pair) * sf * 1024.0
     write = pick(w, is_pair) * sw * 1024.0
-    res = {"kernel": "k_pair<false,false>", "workload": "chains-100k step-0 state",
+    res = {"kernel": "k_pair (plain, tuned waves per group)", "workload": "chains-100k after 50 MD steps",
            "fetch_kib_raw": pick(f, is_pair), "write_kib_raw": pick(w, is_pair),
            "copy_fetch_kib_raw": pick(f, is_copy), "copy_write_kib_raw": pick(w, is_copy),
            "fetch_scale": sf, "write_scale": sw,
@@ -223,6 +281,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo leg")
     ap.add_argument("--mc-trials", type=int, default=8000, help="MC trials per walker and mode")
+    ap.add_argument("--no-large", action="store_true", help="skip the chains-1M leg")
+    ap.add_argument("--large-steps", type=int, default=200, help="timed steps of the chains-1M leg")
     args = ap.parse_args()
     if args.traffic_summary:
         traffic_summary(*args.traffic_summary)
@@ -242,6 +302,8 @@ def main():
     s = system.chains_100k()
     box = Box(s)
     box.nblist()
+    entries, _, _ = box.list_stats()
+    half_entries = entries // 2          # the reference's half list at rclist (nblist.c:32)
     f_gpu0, _ = box.forces()
     waves, t_waves = box.tune()
     print(f"bench: pair kernel waves per group {waves} (4/8/16: {np.round(t_waves, 1)} us)",
@@ -260,8 +322,6 @@ def main():
     # dominant kernel: the pair kernel; HIP events on the context stream
     t_pair = box.timing(0)
     t_spread = box.timing(1)
-    cnt, _, _ = box.get_nblist(0, 1500)
-    half_entries = int(cnt.sum())
     alg_bytes = 4.0 * half_entries + 48.0 * s.n
     achieved = alg_bytes / (t_pair * 1e-6) / 1e9
     traffic = None
@@ -288,6 +348,9 @@ def main():
                      "algorithmic_bytes": alg_bytes, "half_list_entries": half_entries,
                      "spread_us": round(t_spread, 2), "step_us_events": round(t_event, 2)},
     }
+    if not args.no_large:
+        print("bench: chains-1M leg", file=sys.stderr, flush=True)
+        out["large"] = large_leg(args.large_steps, not args.no_cpu)
     if not args.no_mc:
         print("bench: Monte Carlo leg", file=sys.stderr, flush=True)
         out["mc"] = mc_leg(args.mc_trials, not args.no_cpu)

@@ -220,8 +220,9 @@ int  ttc_timing(ttc_ctx *c, int which, double *usec);
  * (usec3[3] optional: the three timings).  No reference counterpart.    */
 int  ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3);
 
-/* Counters of the last list build: total entries of the full device list
- * (each pair twice), longest row, rows capacity.                         */
+/* Counters of the last list build (an export's scratch build included):
+ * entries of the list (each pair in both rows), longest row, row capacity.
+ * After ttc_nblist, entries / 2 is the size of the reference's half list. */
 int  ttc_list_stats(ttc_ctx *c, int64_t *entries, int *max_row, int *capacity);
 
 #ifdef __cplusplus

@@ -118,6 +118,7 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
     c->cap = list_capacity(c, ttc_list_cutoff(c));
     ALLOC(c->nbl, (size_t)c->G * c->cap * 64);
     ALLOC(c->glen, c->G);
+    ALLOC(c->gcnt, c->G);
     TTC_HIP(hipMemset(c->glen, 0, sizeof(int) * c->G));
     ALLOC(c->status, ST_NSTAT);
     TTC_HIP(hipMemset(c->status, 0, sizeof(int) * ST_NSTAT));
@@ -157,7 +158,7 @@ extern "C" int ttc_destroy(ttc_ctx *c) {
                     c->origb, c->inv, c->fpair, c->g4, c->cell_of, c->cell_cnt, c->cell_start, c->cell_acc,
                     c->cell_members, c->nbl, c->glen, c->status, c->ex_off, c->ex_part, c->ex_flag, c->ljtab,
                     c->b_off, c->b_part, c->b_r0, c->b_k, c->partials, c->ev, c->rows, c->fs, c->fl, c->fsb,
-                    c->flb, c->nhc, c->posfb};
+                    c->flb, c->nhc, c->posfb, c->gcnt};
     for (void *b : bufs)
         if (b) hipFree(b);
     ttc_pme_free(c);
@@ -764,12 +765,14 @@ extern "C" int ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3) {
 
 extern "C" int ttc_list_stats(ttc_ctx *c, int64_t *entries, int *max_row, int *capacity) {
     CHECK(c, TTC_ERR_ARG, "ttc_list_stats: null context");
-    std::vector<int> gl(c->G);
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int> gl(c->G), gc(c->G);
     TTC_HIP(hipMemcpy(gl.data(), c->glen, sizeof(int) * c->G, hipMemcpyDeviceToHost));
+    TTC_HIP(hipMemcpy(gc.data(), c->gcnt, sizeof(int) * c->G, hipMemcpyDeviceToHost));
     int64_t tot = 0;
     int mx = 0;
-    for (int g = 0; g < c->G; g++) { tot += (int64_t)gl[g] * 64; mx = std::max(mx, gl[g]); }
-    if (entries) *entries = tot;   // padded ELL slots streamed by the pair kernel
+    for (int g = 0; g < c->G; g++) { tot += gc[g]; mx = std::max(mx, gl[g]); }
+    if (entries) *entries = tot;   // entries of the device list (each pair in both rows)
     if (max_row) *max_row = mx;
     if (capacity) *capacity = c->cap;
     return TTC_OK;

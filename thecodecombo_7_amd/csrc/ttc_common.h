@@ -92,6 +92,7 @@ struct ttc_ctx {
     // list
     uint32_t *nbl = nullptr;
     int *glen = nullptr;
+    int *gcnt = nullptr;            // entries (non-pad) per group of the last build
     int *status = nullptr;          // device ST_* flags
     int *h_status = nullptr;        // pinned host mirror
 

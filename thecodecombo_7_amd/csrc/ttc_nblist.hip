@@ -174,7 +174,7 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, float cutf2, cons
         double4 *fs, double4 *fl,
         const int *__restrict__ cell_cnt, const int *__restrict__ cell_start,
         const int *__restrict__ ex_off, const int *__restrict__ ex_part, const int *__restrict__ ex_flag,
-        int ex_span, uint32_t *nbl, int *glen, int *status, int conditional, int copy_state) {
+        int ex_span, uint32_t *nbl, int *glen, int *gcnt, int *status, int conditional, int copy_state) {
     if (rebuild_off(status, conditional)) return;
     __shared__ int scnt[BUILD_WAVES][64];
     __shared__ int smax;
@@ -273,10 +273,14 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, float cutf2, cons
     int total = 0;
     for (int w = 0; w < BUILD_WAVES; w++) total += scnt[w][lane];
     if (wave == 0) {
-        int full = total;
-        for (int o = 32; o > 0; o >>= 1) full = max(full, __shfl_xor(full, o, 64));
+        int full = total, sum = total;
+        for (int o = 32; o > 0; o >>= 1) {
+            full = max(full, __shfl_xor(full, o, 64));
+            sum += __shfl_xor(sum, o, 64);
+        }
         if (lane == 0) {
             smax = full;
+            gcnt[grp] = sum;
             glen[grp] = min(full, cap);
             if (full > cap) atomicOr(&status[ST_OVERFLOW], 1);
             atomicMax(&status[ST_MAXROW], full);
@@ -382,7 +386,7 @@ static void launch_build(ttc_ctx *c, bool conditional, double cutoff, uint32_t *
     kern<<<c->G, 64 * BUILD_WAVES, 0, c->stream>>>(
         c->n, cap, g, component_cut, fast ? cf * cf : cutoff * cutoff, (float)(cf * cf), c->posfb, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4,
         c->vel4, c->f4, c->type, c->orig, c->anchor, c->inv, c->fsb, c->flb, c->fs, c->fl, c->cell_cnt, c->cell_start, c->ex_off, c->ex_part, c->ex_flag,
-        c->ex_span, nbl, glen, c->status, conditional ? 1 : 0, copy_state ? 1 : 0);
+        c->ex_span, nbl, glen, c->gcnt, c->status, conditional ? 1 : 0, copy_state ? 1 : 0);
 }
 
 int ttc_launch_rebuild(ttc_ctx *c, bool conditional) {
