@@ -174,3 +174,32 @@ def test_md_vverlet_matches_oracle(sys3k):
     assert int(last[0]) == nsteps
     assert abs(last[1] - kin) <= 1e-6 * kin
     assert abs(last[3] - en["nbond"]) <= 2e-6 * abs(en["nbond"])
+
+
+@pytest.mark.parametrize("nchains,box,grid", [(300, 31.0, 32), (2000, 58.3, 60)])
+def test_md_loop_list_complete(nchains, box, grid):
+    """The list an MD loop rebuilds (pruned to rc + half the skin, FP32 test
+    with a margin, group-cooperative build) holds every pair within rc once
+    and with its exclusion class: forces and energies evaluated from it
+    after the loop match the oracle's from a fresh reference list at the
+    same positions.  The 31 A box takes the minimum-image path of the build
+    (candidate ranges wrap onto themselves), the 58 A box the periodic-shift
+    path."""
+    import pyoracle
+    from thecodecombo_7_amd import system
+    from thecodecombo_7_amd.nonbond import Box
+    s = system.make_chain_system(nchains, box, grid, seed=17)
+    print(f"case: MD-loop list, {s.n} sites, box {box}", flush=True)
+    b = Box(s)
+    b.nblist()
+    _, nreb = b.integrate_vverlet(30)
+    assert nreb >= 2, "the run should rebuild the list"
+    pos = b.positions()
+    ff, e9 = b.forces()          # within the trigger: evaluated from the MD loop's list
+    b.close()
+    o = pyoracle.Oracle(s)
+    fref, eref = o.forces(pos)
+    err = rel_rms(ff, fref)
+    print("MD-loop list: rebuilds", nreb, "force rel rms", err)
+    assert err < 5e-6
+    assert abs(e9["nbond"] - eref["nbond"]) <= 2e-6 * abs(eref["nbond"])

@@ -83,6 +83,8 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
     c->ntypes = p->ntypes;
     c->order = p->order;
     TTC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    TTC_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    for (int k = 0; k < 3; k++) TTC_HIP(hipEventCreateWithFlags(&c->sev[k], hipEventDisableTiming));
     int n = c->n;
     ALLOC(c->pos4, n); ALLOC(c->vel4, n); ALLOC(c->f4, n); ALLOC(c->anchor, n);
     ALLOC(c->pos4b, n); ALLOC(c->vel4b, n); ALLOC(c->f4b, n); ALLOC(c->posfb, n);
@@ -164,6 +166,10 @@ extern "C" int ttc_destroy(ttc_ctx *c) {
     ttc_pme_free(c);
     if (c->h_status) hipHostFree(c->h_status);
     if (c->h_ev) hipHostFree(c->h_ev);
+    hipStreamSynchronize(c->side);
+    for (int k = 0; k < 3; k++)
+        if (c->sev[k]) hipEventDestroy(c->sev[k]);
+    hipStreamDestroy(c->side);
     hipStreamDestroy(c->stream);
     delete c;
     return TTC_OK;
@@ -628,11 +634,34 @@ extern "C" int ttc_kinet(ttc_ctx *c, double *kinet, double *ken6) {
 // the drift checks the displacement and the conditional rebuild runs
 // before this step's forces, with the same anchors (this step's positions)
 // the reference rebuilds at after the step.
+// The FFTs need only the charge grid the drift spread, so they run on the
+// side stream while the main stream rebuilds the list; the gather (needing
+// the sorted order of the rebuild) runs there while the pair kernel runs.
+void ttc_side_fork(ttc_ctx *c, int k) {
+    hipEventRecord(c->sev[k], c->stream);
+    hipStreamWaitEvent(c->side, c->sev[k], 0);
+}
+
+void ttc_side_join(ttc_ctx *c) {
+    hipEventRecord(c->sev[2], c->side);
+    hipStreamWaitEvent(c->stream, c->sev[2], 0);
+}
+
 static void launch_step(ttc_ctx *c, double vscal, bool energy) {
     ttc_launch_drift(c, vscal);
+    ttc_side_fork(c, 0);
+    {
+        TtcOnSide side(c);
+        ttc_launch_fft(c, energy);
+    }
     ttc_launch_rebuild(c, true);
+    ttc_side_fork(c, 1);
+    {
+        TtcOnSide side(c);
+        ttc_launch_gather(c, energy, true, true, false);
+    }
     ttc_launch_pair(c, energy, 0);
-    ttc_launch_pme(c, energy, true, true);
+    ttc_side_join(c);
     ttc_launch_kick(c, vscal, energy);
     if (energy) ttc_launch_reduce(c, true);
 }
@@ -981,9 +1010,19 @@ void ttc_launch_nhc_step(ttc_ctx *c, bool energy) {
     ttc_launch_nhc(c);
     ttc_launch_vscale(c);
     ttc_launch_drift(c, 1.0);
+    ttc_side_fork(c, 0);
+    {
+        TtcOnSide side(c);
+        ttc_launch_fft(c, energy);
+    }
     ttc_launch_rebuild(c, true);
+    ttc_side_fork(c, 1);
+    {
+        TtcOnSide side(c);
+        ttc_launch_gather(c, energy, true, true, false);
+    }
     ttc_launch_pair(c, energy, 0);
-    ttc_launch_pme(c, energy, true, true);
+    ttc_side_join(c);
     ttc_launch_kick(c, 1.0, true);   // kinetic partials for the second thermostat half
     ttc_launch_nhc(c);
     ttc_launch_vscale(c);

@@ -12,7 +12,11 @@
 //     anchor[s] = position at the last list build (DLIST)
 //     fix[3][N] int64 fixed-point force accumulator (2^-32 kJ/mol/A units)
 //   Neighbour list: full list (each pair in both rows), ELL layout per group
-//   of 64 consecutive sorted sites: nbl[g][k][lane] (uint32), glen[g] rows.
+//   of 64 consecutive sorted sites, tiled by 16 entries:
+//   nbl[g][k/16][lane][k%16] (uint32), glen[g] entries per row (cap, the
+//   row capacity, is a multiple of 64).  A row's consecutive entries share a
+//   64-byte segment, so appends by the list build coalesce, and the 16 k of
+//   one 4 KB tile are what the pair kernel's waves read together.
 //   Entry = j (sorted index, 30 bits) | class << 30 (0 plain, 1 excluded
 //   xflag 1/2/3, 2 one-four xflag 4); 0xFFFFFFFF pads short rows.
 //   PME: charge grid as int64 fixed point (2^-40 e units) for deterministic
@@ -76,6 +80,11 @@ struct ttc_ctx {
     bool have_list = false;
     bool list_pruned = false;   // list built in an MD loop at the pruned cutoff
     hipStream_t stream = nullptr;
+    // second stream of the captured MD steps: the FFTs and the gather run on
+    // it, concurrently with the list rebuild and the pair kernel (fork after
+    // the drift, join before the kick); sev[0..2] mark the fork points
+    hipStream_t side = nullptr;
+    hipEvent_t sev[3] = {nullptr, nullptr, nullptr};
 
     // sorted-order state
     double4 *pos4 = nullptr, *vel4 = nullptr, *f4 = nullptr, *anchor = nullptr;
@@ -193,6 +202,22 @@ int ttc_pme_attrs(ttc_ctx *c);   // LDS limits of the spread kernel (needs the c
 void ttc_pme_free(ttc_ctx *c);
 // gather: pme -> PME forces, bonds -> bond forces (either may be off)
 void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, bool bonds);
+// the two halves of ttc_launch_pme: the FFT convolution of the charge grid
+// (charge grid -> qr) and the gather (qr + positions -> g4)
+void ttc_launch_fft(ttc_ctx *c, bool energy);
+void ttc_launch_gather(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes);
+// captured MD steps: side stream waits for the work queued on the main
+// stream so far (ttc_side_fork, event k), main waits for the side stream
+// (ttc_side_join)
+void ttc_side_fork(ttc_ctx *c, int k);
+void ttc_side_join(ttc_ctx *c);
+// run launches on the side stream for the lifetime of the object
+struct TtcOnSide {
+    ttc_ctx *c;
+    hipStream_t saved;
+    explicit TtcOnSide(ttc_ctx *cc) : c(cc), saved(cc->stream) { cc->stream = cc->side; }
+    ~TtcOnSide() { c->stream = saved; }
+};
 // energies + forces + Hessian diagonal (CONFIGT); needs ttc_pme_theta first
 void ttc_launch_pme_hes(ttc_ctx *c, bool pme, bool bonds);
 int ttc_pme_theta(ttc_ctx *c);   // fills theta_near (once per context)
@@ -251,6 +276,11 @@ __device__ __forceinline__ double ttc_r2_exact(double dx, double dy, double dz) 
     double y2 = dy * dy;
     double z2 = dz * dz;
     return (x2 + y2) + z2;
+}
+
+// index of entry k of row (lane) l inside a group's list block (cap*64 words)
+__device__ __forceinline__ size_t ttc_nbl_idx(int k, int l) {
+    return (size_t)(k >> 4) * 1024 + (size_t)l * 16 + (k & 15);
 }
 
 __device__ __forceinline__ double ttc_wave_sum(double v) {

@@ -7,8 +7,10 @@
 //   R3 reorder      : rank inside the cell by original index (deterministic
 //                     order without a sort), scatter the site state into the
 //                     B buffers at its new sorted position
-//   R5 build        : copy B -> A, anchors, inverse map, and the full Verlet
-//                     list of each site with the reference's FP64 test
+//   R5 build        : copy B -> A, anchors, and the full Verlet list: the
+//                     reference's FP64 test for explicit builds (k_build),
+//                     the group-cooperative FP32 build of the pruned list
+//                     inside MD loops (k_build_grp)
 // Cells are at least rclist/2 wide, so a 5x5x5 stencil covers rclist.
 #include "ttc_common.h"
 
@@ -84,7 +86,7 @@ __global__ void k_reorder(int n, int cellcap, const int *cell_of, const int *cel
                           const double4 *pos4, const double4 *vel4, const double4 *f4, const int *type,
                           double4 *pos4b, double4 *vel4b, double4 *f4b, int *typeb, int *origb,
                           const double4 *fs, const double4 *fl, double4 *fsb, double4 *flb,
-                          float4 *posfb, int *status, int conditional) {
+                          float4 *posfb, int *inv, int *status, int conditional) {
     if (rebuild_off(status, conditional)) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -104,6 +106,7 @@ __global__ void k_reorder(int n, int cellcap, const int *cell_of, const int *cel
     flb[s] = fl[i];
     typeb[s] = type[i];
     origb[s] = o;
+    inv[o] = s;
 }
 
 #define BUILD_WAVES 8
@@ -124,31 +127,6 @@ __device__ __forceinline__ bool ref_pair_test(const double4 &pi, const double4 &
     return (dx <= rcut) & (dy <= rcut) & (dz <= rcut) & (dr2 < rcut2);
 }
 
-// The test of lists built inside MD loops: FP32 positions and a cutoff
-// widened by a margin far above the FP32 rounding of the distance, so the
-// list is a superset of the pairs within the exact cutoff (the pair kernel
-// applies the exact FP64 cutoff test to every entry).
-__device__ __forceinline__ bool fast_pair_test(const float4 &pi, const float4 &pj, float hx, float hy, float hz,
-                                               float lx, float ly, float lz, float cut2) {
-    float dx = fabsf(pi.x - pj.x);
-    dx = dx > hx ? dx - lx : dx;
-    float dy = fabsf(pi.y - pj.y);
-    dy = dy > hy ? dy - ly : dy;
-    float dz = fabsf(pi.z - pj.z);
-    dz = dz > hz ? dz - lz : dz;
-    return dx * dx + dy * dy + dz * dz < cut2;
-}
-
-template <bool FAST>
-__device__ __forceinline__ bool cand_hit(int j, const float4 &pif, const double4 &pi, const float4 *posfb,
-                                         const double4 *pos4b, const CellGeom &g, double rcut, double rcut2,
-                                         float cutf2) {
-    if (FAST)
-        return fast_pair_test(pif, posfb[j], (float)g.hx, (float)g.hy, (float)g.hz, (float)g.lx, (float)g.ly,
-                              (float)g.lz, cutf2);
-    return ref_pair_test(pi, pos4b[j], g, rcut, rcut2);
-}
-
 // smallest periodic distance from coordinate x to the slab [lo, lo + w)
 __device__ __forceinline__ double slab_gap(double x, double lo, double w, double l) {
     double d = x - lo;
@@ -162,12 +140,10 @@ __device__ __forceinline__ double slab_gap(double x, double lo, double w, double
 // goes to wave t % BUILD_WAVES) and skip cells farther than rcut.  Pass 1
 // counts each site's hits per wave, pass 2 writes them at prefix offsets,
 // so the row order is fixed (stencil order within a wave, waves in order).
-// FAST (lists of MD loops): candidates from the FP32 copy with
-// fast_pair_test at cutf2; otherwise the reference's exact test.
-template <bool FAST>
+// This is the reference's list (nblist() of the C ABI, exports, lists at
+// the anchors); lists of MD loops come from k_build_grp.
 __global__ void __launch_bounds__(64 * BUILD_WAVES)
-k_build(int n, int cap, CellGeom g, double rcut, double rcut2, float cutf2, const float4 *__restrict__ posfb,
-        const double4 *__restrict__ pos4b,
+k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__restrict__ pos4b,
         const double4 *__restrict__ vel4b, const double4 *__restrict__ f4b, const int *__restrict__ typeb,
         const int *__restrict__ origb, double4 *pos4, double4 *vel4, double4 *f4, int *type, int *orig,
         double4 *anchor, int *inv, const double4 *__restrict__ fsb, const double4 *__restrict__ flb,
@@ -205,8 +181,7 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, float cutf2, cons
     int nst = nsx * nsy * nsz;
     double cwx = g.lx / g.ncx, cwy = g.ly / g.ncy, cwz = g.lz / g.ncz;
     int xb = valid ? ex_off[oi] : 0, xe = valid ? ex_off[oi + 1] : 0;
-    float4 pif = make_float4((float)pi.x, (float)pi.y, (float)pi.z, 0.0f);
-    uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
+    uint32_t *row = nbl + (size_t)grp * cap * 64;
 
     // pass 0 tests every candidate once and keeps, per stencil cell, the hits
     // among its first 64 sites as a bit mask in LDS; pass 1 writes the
@@ -231,7 +206,7 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, float cutf2, cons
             int jt = j0;   // first site tested again in pass 1
             if (pass == 0) {
                 for (int j = j0; j < j1; j++) {
-                    if (j == s || !cand_hit<FAST>(j, pif, pi, posfb, pos4b, g, rcut, rcut2, cutf2)) continue;
+                    if (j == s || !ref_pair_test(pi, pos4b[j], g, rcut, rcut2)) continue;
                     mine++;
                     if (j - j0 < 64) mask |= 1ull << (j - j0);
                 }
@@ -247,7 +222,7 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, float cutf2, cons
                     mask &= mask - 1;
                 } else {
                     // sites past the 64th of the cell (dense cells only)
-                    while (jt < j1 && (jt == s || !cand_hit<FAST>(jt, pif, pi, posfb, pos4b, g, rcut, rcut2, cutf2)))
+                    while (jt < j1 && (jt == s || !ref_pair_test(pi, pos4b[jt], g, rcut, rcut2)))
                         jt++;
                     if (jt >= j1) break;
                     j = jt++;
@@ -259,7 +234,7 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, float cutf2, cons
                     for (int q = xb; q < xe; q++)
                         if (ex_part[q] == oj) { int f = ex_flag[q]; cls = f == 4 ? 2u : (f ? 1u : 0u); }
                 }
-                if (k < cap) row[(size_t)k * 64] = (uint32_t)j | (cls << 30);
+                if (k < cap) row[ttc_nbl_idx(k, lane)] = (uint32_t)j | (cls << 30);
                 k++;
             }
         }
@@ -287,8 +262,333 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, float cutf2, cons
         }
     }
     __syncthreads();
-    int m = min(smax, cap);
-    for (int k = min(total, cap) + wave; k < m; k += BUILD_WAVES) row[(size_t)k * 64] = TTC_PAD;
+    int m = min((smax + 15) & ~15, cap);   // whole 16-entry tiles
+    for (int k = min(total, cap) + wave; k < m; k += BUILD_WAVES) row[ttc_nbl_idx(k, lane)] = TTC_PAD;
+}
+
+// ---- list of MD loops: group-cooperative build ------------------------------
+//
+// The list rebuilt inside MD loops only has to be a superset of the pairs
+// within the exact cutoff (the pair kernel applies the FP64 test to every
+// entry), so it is built from the FP32 positions at a cutoff widened by a
+// margin.  One workgroup per ELL group of 64 sorted sites:
+//   1. wave 0 takes the group's bounding box in the minimum-image frame of
+//      the group's first site (compact even where the group straddles the
+//      periodic boundary) and enumerates the rows of cells (fixed y, z cell)
+//      that come within the cutoff of it.  Sites are sorted x-fastest by
+//      cell, so each row contributes one or two contiguous ranges of sorted
+//      sites, each with one periodic shift into the frame (or, in boxes
+//      where the range wraps onto itself, the minimum-image flag).  Ranges
+//      are cut into chunks of GB_C sites.
+//   2. every wave owns GB_ROWS rows (sites) of the group and streams all
+//      chunks.  One test instruction covers GB_RPI rows x GB_C chunk sites
+//      (lane = row segment x chunk site); one ballot gives the hits, and each
+//      hit lane writes its entry at its row's running count (popcount of
+//      the hits below it in its row segment), in chunk order.
+// The row order is therefore fixed (chunk order) and no cross-wave prefix
+// is needed.  Short chunks keep the sites one row reads at nearby positions
+// in a small window of memory, which the pair kernel's gathers rely on.
+#define GB_WAVES 4
+#define GB_ROWS (64 / GB_WAVES)
+#define GB_C 16
+#define GB_RPI (64 / GB_C)
+#define GB_NT (GB_ROWS / GB_RPI)
+#define GB_MAXCH 2048
+#define GB_BATCH 64   // chunks whose sites are staged in LDS at once
+#define GB_MI (1 << 13)
+#define GB_EXMAX 8    // exclusion partners per site held in LDS (more: read from the CSR)
+
+struct GbGeom {
+    int ncx, ncy, ncz;
+    float lx, ly, lz, hx, hy, hz, cwx, cwy, cwz, ilx, ily, ilz;
+    float cut2;   // FP32 test radius squared
+    float cutr;   // radius of the cell-range enumeration (test radius + rounding margin)
+};
+
+__device__ __forceinline__ float wave_minf(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_maxf(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+// coordinate x in the minimum-image frame of ref
+__device__ __forceinline__ float frame1(float x, float ref, float l, float il) {
+    float d = x - ref;
+    return ref + (d - l * rintf(d * il));
+}
+
+struct AxisRange {
+    int a, b;   // cell range in the frame's (unwrapped) cell numbering
+    bool mi;    // covers the whole ring: visit every cell once, minimum image
+};
+
+__device__ __forceinline__ AxisRange axis_range(float mn, float mx, float r, float h, float cw, int nc) {
+    int a = (int)floorf((mn - r + h) / cw);
+    int b = (int)floorf((mx + r + h) / cw);
+    if (b - a + 1 >= nc || a < -nc || b >= 2 * nc) return {0, nc - 1, true};
+    return {a, b, false};
+}
+
+// the sorted-site ranges (at most two, three with the whole ring) of
+// candidate row r; returns the number of chunks, fills lo/len/meta
+__device__ int row_ranges(int r, const AxisRange &ay, const AxisRange &az, float xmn, float xmx, float ymn,
+                          float ymx, float zmn, float zmx, const GbGeom &g, const int *__restrict__ cell_start,
+                          const int *__restrict__ cell_cnt, int *lo, int *len, int *meta, int *nr) {
+    int nyr = ay.b - ay.a + 1;
+    int cyu = ay.a + r % nyr, czu = az.a + r / nyr;
+    float gy = 0.0f, gz = 0.0f;
+    if (!ay.mi) {
+        float s0 = -g.hy + cyu * g.cwy;
+        gy = fmaxf(0.0f, fmaxf(s0 - ymx, ymn - (s0 + g.cwy)));
+    }
+    if (!az.mi) {
+        float s0 = -g.hz + czu * g.cwz;
+        gz = fmaxf(0.0f, fmaxf(s0 - zmx, zmn - (s0 + g.cwz)));
+    }
+    float g2 = gy * gy + gz * gz;
+    *nr = 0;
+    if (g2 > g.cutr * g.cutr) return 0;
+    float xr = sqrtf(g.cutr * g.cutr - g2);
+    AxisRange ax = axis_range(xmn, xmx, xr, g.hx, g.cwx, g.ncx);
+    int cyw = cyu - floordiv(cyu, g.ncy) * g.ncy, czw = czu - floordiv(czu, g.ncz) * g.ncz;
+    int sy = ay.mi ? 0 : floordiv(cyu, g.ncy), sz = az.mi ? 0 : floordiv(czu, g.ncz);
+    int rowc = g.ncx * (cyw + g.ncy * czw);
+    bool mi = ay.mi || az.mi || ax.mi;
+    int chunks = 0;
+    int p0 = ax.a;
+    while (p0 <= ax.b && *nr < 3) {
+        int sx = ax.mi ? 0 : floordiv(p0, g.ncx);
+        int w0 = p0 - sx * g.ncx;
+        int p1 = ax.mi ? ax.b : min(ax.b, (sx + 1) * g.ncx - 1);
+        int w1 = p1 - sx * g.ncx;
+        int jl = cell_start[rowc + w0];
+        int jh = cell_start[rowc + w1] + cell_cnt[rowc + w1];
+        if (jh > jl) {
+            lo[*nr] = jl;
+            len[*nr] = jh - jl;
+            meta[*nr] = ((sx + 1) << 7) | ((sy + 1) << 9) | ((sz + 1) << 11) | (mi ? GB_MI : 0);
+            chunks += (jh - jl + GB_C - 1) / GB_C;
+            (*nr)++;
+        }
+        p0 = p1 + 1;
+    }
+    return chunks;
+}
+
+__global__ void __launch_bounds__(64 * GB_WAVES)
+k_build_grp(int n, int cap, GbGeom g, const float4 *__restrict__ posfb, const double4 *__restrict__ pos4b,
+            const double4 *__restrict__ vel4b, const double4 *__restrict__ f4b, const int *__restrict__ typeb,
+            const int *__restrict__ origb, double4 *pos4, double4 *vel4, double4 *f4, int *type, int *orig,
+            double4 *anchor, const double4 *__restrict__ fsb, const double4 *__restrict__ flb, double4 *fs,
+            double4 *fl, const int *__restrict__ cell_cnt, const int *__restrict__ cell_start,
+            const int *__restrict__ ex_off, const int *__restrict__ ex_part, const int *__restrict__ ex_flag,
+            int ex_span, uint32_t *nbl, int *glen, int *gcnt, int *status, int conditional) {
+    if (rebuild_off(status, conditional)) return;
+    __shared__ int2 sch[GB_MAXCH];
+    __shared__ float4 sst[GB_BATCH * GB_C];   // chunk sites: x, y, z in the frame, original index
+    __shared__ int snch;
+    __shared__ int srow[64];
+    __shared__ int smax;
+    // exclusion partners of the group's sites (original index) and their
+    // class; sxn = count, or -1 where the site has more than GB_EXMAX
+    __shared__ int sxo[64][GB_EXMAX];
+    __shared__ int sxc[64][GB_EXMAX];
+    __shared__ int sxn[64];
+    int grp = blockIdx.x;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int sb = grp * 64;
+    if (threadIdx.x < 64) {
+        int s = sb + threadIdx.x, cnt = 0;
+        if (s < n) {
+            int oi = origb[s];
+            int x0 = ex_off[oi], x1 = ex_off[oi + 1];
+            cnt = x1 - x0 > GB_EXMAX ? -1 : x1 - x0;
+            for (int q = 0; q < cnt; q++) {
+                int f = ex_flag[x0 + q];
+                sxo[threadIdx.x][q] = ex_part[x0 + q];
+                sxc[threadIdx.x][q] = f == 4 ? 2 : (f ? 1 : 0);
+            }
+        }
+        sxn[threadIdx.x] = cnt;
+    }
+    // the frame: minimum image around the group's first site
+    float4 pref = posfb[sb];
+
+    // the new sorted order becomes the current state
+    for (int s = sb + (int)threadIdx.x; s < min(sb + 64, n); s += 64 * GB_WAVES) {
+        double4 pi = pos4b[s];
+        pos4[s] = pi;
+        vel4[s] = vel4b[s];
+        f4[s] = f4b[s];
+        fs[s] = fsb[s];
+        fl[s] = flb[s];
+        type[s] = typeb[s];
+        orig[s] = origb[s];
+        anchor[s] = pi;
+    }
+    if (wave == 0) {
+        float4 p = posfb[min(sb + lane, n - 1)];
+        float ux = frame1(p.x, pref.x, g.lx, g.ilx), uy = frame1(p.y, pref.y, g.ly, g.ily),
+              uz = frame1(p.z, pref.z, g.lz, g.ilz);
+        float xmn = wave_minf(ux), xmx = wave_maxf(ux);
+        float ymn = wave_minf(uy), ymx = wave_maxf(uy);
+        float zmn = wave_minf(uz), zmx = wave_maxf(uz);
+        AxisRange ay = axis_range(ymn, ymx, g.cutr, g.hy, g.cwy, g.ncy);
+        AxisRange az = axis_range(zmn, zmx, g.cutr, g.hz, g.cwz, g.ncz);
+        int nrows = (ay.b - ay.a + 1) * (az.b - az.a + 1);
+        int base = 0;
+        for (int r0 = 0; r0 < nrows; r0 += 64) {
+            int r = r0 + lane;
+            int lo[3], len[3], meta[3], nr = 0, cnt = 0;
+            if (r < nrows)
+                cnt = row_ranges(r, ay, az, xmn, xmx, ymn, ymx, zmn, zmx, g, cell_start, cell_cnt, lo, len, meta, &nr);
+            int incl = cnt;
+            for (int o = 1; o < 64; o <<= 1) {
+                int t = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += t;
+            }
+            int k = base + incl - cnt;
+            for (int q = 0; q < nr; q++)
+                for (int c0 = 0; c0 < len[q]; c0 += GB_C, k++)
+                    if (k < GB_MAXCH) sch[k] = make_int2(lo[q] + c0, min(GB_C, len[q] - c0) | meta[q]);
+            base += __shfl(incl, 63, 64);
+        }
+        if (lane == 0) {
+            snch = min(base, GB_MAXCH);
+            if (base > GB_MAXCH) atomicOr(&status[ST_OVERFLOW], 8);
+        }
+    }
+    __syncthreads();
+
+    int nch = snch;
+    int seg = lane / GB_C, cl = lane % GB_C;
+    // rows of this lane in each of the wave's GB_NT row sets: row
+    // wave*GB_ROWS + t*GB_RPI + seg
+    float rx[GB_NT], ry[GB_NT], rz[GB_NT];
+    int ro[GB_NT], rk[GB_NT], rrow[GB_NT];
+    bool rv[GB_NT];
+#pragma unroll
+    for (int t = 0; t < GB_NT; t++) {
+        int r = wave * GB_ROWS + t * GB_RPI + seg;
+        int s = sb + r;
+        rv[t] = s < n;
+        float4 p = posfb[min(s, n - 1)];
+        rx[t] = frame1(p.x, pref.x, g.lx, g.ilx);
+        ry[t] = frame1(p.y, pref.y, g.ly, g.ily);
+        rz[t] = frame1(p.z, pref.z, g.lz, g.ilz);
+        ro[t] = rv[t] ? origb[s] : -1;
+        rk[t] = 0;
+        rrow[t] = r;
+    }
+    const int nset = min(GB_NT, (n - sb - wave * GB_ROWS + GB_RPI - 1) / GB_RPI);   // row sets with a site
+    unsigned long long segmask = ((GB_C == 64) ? ~0ull : ((1ull << GB_C) - 1ull)) << (seg * GB_C);
+    uint32_t *grow = nbl + (size_t)grp * cap * 64;
+    for (int cb = 0; cb < nch; cb += GB_BATCH) {
+        // stage the sites of the next GB_BATCH chunks (shifted into the frame)
+        int nb = min(GB_BATCH, nch - cb);
+        __syncthreads();
+        for (int slot = threadIdx.x; slot < nb * GB_C; slot += 64 * GB_WAVES) {
+            int2 d = sch[cb + slot / GB_C];
+            int q = slot % GB_C;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (q < (d.y & 127)) {
+                int jj = d.x + q;
+                float4 p = posfb[jj];
+                if (!(d.y & GB_MI)) {
+                    p.x += (float)(((d.y >> 7) & 3) - 1) * g.lx;
+                    p.y += (float)(((d.y >> 9) & 3) - 1) * g.ly;
+                    p.z += (float)(((d.y >> 11) & 3) - 1) * g.lz;
+                }
+                v = make_float4(p.x, p.y, p.z, __int_as_float(origb[jj]));
+            }
+            sst[slot] = v;
+        }
+        __syncthreads();
+        for (int c = cb; c < cb + nb; c++) {
+            int2 d = sch[c];
+            int j0 = __builtin_amdgcn_readfirstlane(d.x);
+            int meta = __builtin_amdgcn_readfirstlane(d.y);
+            int nn = meta & 127;
+            bool ok = cl < nn;
+            int j = j0 + cl;
+            float4 pj = sst[(c - cb) * GB_C + cl];
+            int oj = __float_as_int(pj.w);
+            bool mi = (meta & GB_MI) != 0;
+            uint32_t jw = (uint32_t)j;
+#pragma unroll
+            for (int t = 0; t < GB_NT; t++) {
+                if (t < nset) {
+                    float dx = rx[t] - pj.x, dy = ry[t] - pj.y, dz = rz[t] - pj.z;
+                    if (mi) {
+                        dx -= g.lx * rintf(dx * g.ilx);
+                        dy -= g.ly * rintf(dy * g.ily);
+                        dz -= g.lz * rintf(dz * g.ilz);
+                    }
+                    bool hit = ok & rv[t] & (dx * dx + dy * dy + dz * dz < g.cut2) & (j != sb + rrow[t]);
+                    unsigned long long b = __ballot(hit);
+                    if (b != 0ull) {
+                        unsigned long long mb = b & segmask;
+                        int pos = rk[t] + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u));
+                        rk[t] += (int)__popcll(mb);
+                        // exclusion class: only pairs close in the original numbering
+                        // can be excluded or 1-4 pairs (max |a-b| over the exclusion list)
+                        uint32_t cls = 0;
+                        int dd = oj - ro[t];
+                        bool nearo = hit & ((unsigned)(dd + ex_span) <= (unsigned)(2 * ex_span));
+                        if (__ballot(nearo) != 0ull) {
+                            if (nearo) {
+                                int r = rrow[t];
+                                int cnt = sxn[r];
+                                for (int q = 0; q < cnt; q++)
+                                    if (sxo[r][q] == oj) cls = (uint32_t)sxc[r][q];
+                                if (cnt < 0) {
+                                    int oi = ro[t];
+                                    for (int q = ex_off[oi]; q < ex_off[oi + 1]; q++)
+                                        if (ex_part[q] == oj) {
+                                            int f = ex_flag[q];
+                                            cls = f == 4 ? 2u : (f ? 1u : 0u);
+                                        }
+                                }
+                            }
+                        }
+                        if (hit && pos < cap) grow[ttc_nbl_idx(pos, rrow[t])] = jw | (cls << 30);
+                    }
+                }
+            }
+        }
+    }
+
+    // row length of the group = longest row; pad the shorter ones
+#pragma unroll
+    for (int t = 0; t < GB_NT; t++)
+        if (cl == 0) srow[rrow[t]] = rk[t];
+    __syncthreads();
+    if (wave == 0) {
+        int tot = sb + lane < n ? srow[lane] : 0;
+        int full = tot, sum = tot;
+        for (int o = 32; o > 0; o >>= 1) {
+            full = max(full, __shfl_xor(full, o, 64));
+            sum += __shfl_xor(sum, o, 64);
+        }
+        if (lane == 0) {
+            smax = full;
+            gcnt[grp] = sum;
+            glen[grp] = min(full, cap);
+            if (full > cap) atomicOr(&status[ST_OVERFLOW], 1);
+            atomicMax(&status[ST_MAXROW], full);
+        }
+    }
+    __syncthreads();
+    int m = min((smax + 15) & ~15, cap);   // whole 16-entry tiles
+    for (int r = wave * GB_ROWS; r < (wave + 1) * GB_ROWS; r++) {
+        int k0 = min(srow[r], cap);
+        for (int k = k0 + lane; k < m; k += 64) grow[ttc_nbl_idx(k, r)] = TTC_PAD;
+    }
 }
 
 // nblist.c:607-647 on every site; sets the trigger flag.  Also the
@@ -324,9 +624,9 @@ __global__ void k_export(int n, int cap, const uint32_t *nbl, const int *glen, c
     int a = orig[s];
     int len = glen[grp];
     int c = 0;
-    const uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
+    const uint32_t *row = nbl + (size_t)grp * cap * 64;
     for (int k = 0; k < len; k++) {
-        uint32_t e = row[(size_t)k * 64];
+        uint32_t e = row[ttc_nbl_idx(k, lane)];
         if (e == TTC_PAD) continue;
         int b = orig[e & TTC_JMASK];
         if (b <= a) continue;
@@ -374,19 +674,36 @@ int ttc_cellcap(const ttc_ctx *c) {
 // is built (nblist.c:116-127 test against rcut = sim.rclist also while
 // filling nlist_ew), so a wider nlist_ew keeps only pairs with every
 // |d| <= rclist; lists built inside MD loops reject at their own cutoff.
-// margin of the fast (FP32) test, A: far above FP32 rounding of distances
+// margin of the MD-loop (FP32) test, A: far above FP32 rounding of distances
 // in boxes up to 1000 A, negligible in list length
 #define FAST_MARGIN 0.01
 
 static void launch_build(ttc_ctx *c, bool conditional, double cutoff, uint32_t *nbl, int *glen,
-                         int cap, bool copy_state, double component_cut, bool fast = false) {
+                         int cap, bool copy_state, double component_cut) {
     CellGeom g = geom(c);
-    double cf = cutoff + FAST_MARGIN;
-    auto kern = fast ? k_build<true> : k_build<false>;
-    kern<<<c->G, 64 * BUILD_WAVES, 0, c->stream>>>(
-        c->n, cap, g, component_cut, fast ? cf * cf : cutoff * cutoff, (float)(cf * cf), c->posfb, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4,
-        c->vel4, c->f4, c->type, c->orig, c->anchor, c->inv, c->fsb, c->flb, c->fs, c->fl, c->cell_cnt, c->cell_start, c->ex_off, c->ex_part, c->ex_flag,
-        c->ex_span, nbl, glen, c->gcnt, c->status, conditional ? 1 : 0, copy_state ? 1 : 0);
+    k_build<<<c->G, 64 * BUILD_WAVES, 0, c->stream>>>(
+        c->n, cap, g, component_cut, cutoff * cutoff, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4,
+        c->vel4, c->f4, c->type, c->orig, c->anchor, c->inv, c->fsb, c->flb, c->fs, c->fl, c->cell_cnt,
+        c->cell_start, c->ex_off, c->ex_part, c->ex_flag, c->ex_span, nbl, glen, c->gcnt, c->status,
+        conditional ? 1 : 0, copy_state ? 1 : 0);
+}
+
+// the pruned list of MD loops (cutoff ttc_compute_cutoff), FP32 test with
+// FAST_MARGIN, group-cooperative build
+static void launch_build_grp(ttc_ctx *c) {
+    GbGeom g;
+    g.ncx = c->ncx; g.ncy = c->ncy; g.ncz = c->ncz;
+    g.lx = (float)c->p.lx; g.ly = (float)c->p.ly; g.lz = (float)c->p.lz;
+    g.hx = 0.5f * g.lx; g.hy = 0.5f * g.ly; g.hz = 0.5f * g.lz;
+    g.cwx = (float)(c->p.lx / c->ncx); g.cwy = (float)(c->p.ly / c->ncy); g.cwz = (float)(c->p.lz / c->ncz);
+    g.ilx = (float)(1.0 / c->p.lx); g.ily = (float)(1.0 / c->p.ly); g.ilz = (float)(1.0 / c->p.lz);
+    double cf = ttc_compute_cutoff(c) + FAST_MARGIN;
+    g.cut2 = (float)(cf * cf);
+    g.cutr = (float)(cf + FAST_MARGIN);
+    k_build_grp<<<c->G, 64 * GB_WAVES, 0, c->stream>>>(
+        c->n, c->cap, g, c->posfb, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4, c->vel4, c->f4,
+        c->type, c->orig, c->anchor, c->fsb, c->flb, c->fs, c->fl, c->cell_cnt, c->cell_start, c->ex_off,
+        c->ex_part, c->ex_flag, c->ex_span, c->nbl, c->glen, c->gcnt, c->status, 1);
 }
 
 int ttc_launch_rebuild(ttc_ctx *c, bool conditional) {
@@ -401,11 +718,11 @@ int ttc_launch_rebuild(ttc_ctx *c, bool conditional) {
     k_reorder<<<blocks, 256, 0, c->stream>>>(c->n, cellcap, c->cell_of, c->cell_cnt, c->cell_start,
                                              c->cell_members, c->orig, c->pos4, c->vel4, c->f4, c->type,
                                              c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->fs, c->fl,
-                                             c->fsb, c->flb, c->posfb, c->status, cond);
+                                             c->fsb, c->flb, c->posfb, c->inv, c->status, cond);
     // explicit builds (nblist.c:32) make the reference's list; the
     // conditional builds of MD loops make the pruned one
     if (conditional)
-        launch_build(c, true, ttc_compute_cutoff(c), c->nbl, c->glen, c->cap, true, ttc_compute_cutoff(c), true);
+        launch_build_grp(c);
     else
         launch_build(c, false, ttc_list_cutoff(c), c->nbl, c->glen, c->cap, true, c->p.rclist);
     return TTC_OK;

@@ -1,8 +1,9 @@
 // ttc_pair.hip -- LJ + SPME real-space pair kernel (replaces the pair loop
 // of cnonbond_nblist.c:134-523, STYPE + S14 + SPME branch).
 //
-// One lane per sorted site, one wavefront per ELL group of 64 sites.  Each
-// step of the row loop reads one coalesced 256-byte slice of the list.  The
+// One lane per sorted site, one workgroup per ELL group of 64 sites.  Each
+// step of the row loop reads one entry per lane from the group's current
+// 16-entry list tile (4 KB, shared by the group's waves).  The
 // cutoff test and minimum image run in FP64 on FP64 positions (so exactly
 // the reference's pairs are evaluated); the LJ, erfc and exp arithmetic runs
 // in FP32; each site's force and the energies/virial accumulate in FP64.
@@ -11,6 +12,8 @@
 #include "ttc_common.h"
 
 namespace {
+
+#define PAIR_AHEAD 2   // entries whose site gathers are in flight ahead of the arithmetic
 
 struct PairConst {
     double lx, ly, lz, hx, hy, hz, rc2;
@@ -49,16 +52,18 @@ struct RangeSel {
 
 // RANGE: keep only pairs with a site in [lb,ub) of the original numbering
 // (enbnd_mc.c:98-119); i_in says whether the row's own site is in range.
+// site gathered for entry e (site 0 for padding, which is skipped)
+__device__ __forceinline__ int entry_site(uint32_t e) { return e == TTC_PAD ? 0 : (int)(e & TTC_JMASK); }
+
+// pj, tj: position and type of the entry's site, gathered by the caller
+// (ahead of time, so the gathers of later entries overlap this one)
 template <bool ENERGY, bool RANGE = false, bool HES = false>
-__device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pi, float qi, const float4 *ljrow,
-                                           const double4 *__restrict__ pos4, const int *__restrict__ type,
-                                           const PairConst &pc, PairAcc &a, const RangeSel &rs = RangeSel{},
-                                           bool i_in = true) {
+__device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, int tj, const double4 &pi, float qi,
+                                           const float4 *ljrow, const PairConst &pc, PairAcc &a,
+                                           const RangeSel &rs = RangeSel{}, bool i_in = true) {
     if (e == TTC_PAD) return;
     int j = (int)(e & TTC_JMASK);
     uint32_t cls = e >> 30;
-    double4 pj = pos4[j];
-    int tj = type[j];
     double dx = ttc_minimg(pi.x - pj.x, pc.hx, pc.lx);
     double dy = ttc_minimg(pi.y - pj.y, pc.hy, pc.ly);
     double dz = ttc_minimg(pi.z - pj.z, pc.hz, pc.lz);
@@ -139,8 +144,9 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pi, float 
 }
 
 // One workgroup per ELL group of 64 sites; its WAVES waves take the row
-// slices k = wave, wave + WAVES, ... (each slice one coalesced 256-byte
-// read) and their partial forces are summed in a fixed wave order.  More
+// slices k = wave, wave + WAVES, ... (the waves together sweep one 16-entry
+// tile of every row at a time) and their partial forces are summed in a
+// fixed wave order.  More
 // waves per group shorten each group's dependent load chain (small boxes,
 // few groups); fewer suit many groups.  ttc_tune picks WAVES per context.
 template <bool ENERGY, bool RANGE, bool HES, int WAVES>
@@ -165,20 +171,49 @@ k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__
     int ti = s < n ? type[s] : 0;
     const float4 *ljrow = slj + ti * pc.ntypes;
     float qi = (float)pi.w;
-    const uint32_t *row = nbl + (size_t)grp * cap * 64 + lane;
+    // this lane's 64-byte segment of tile 0; tile t is 1024 words further
+    const uint4 *seg = (const uint4 *)(nbl + (size_t)grp * cap * 64 + (size_t)lane * 16);
     bool i_in = true;
     if (RANGE) {
         int oi = s < n ? rs.orig[s] : -1;
         i_in = oi >= rs.lb && oi < rs.ub;
     }
-    int k = wave;
-    for (; k + WAVES < len; k += 2 * WAVES) {
-        uint32_t e0 = row[(size_t)k * 64];
-        uint32_t e1 = row[(size_t)(k + WAVES) * 64];
-        pair_entry<ENERGY, RANGE, HES>(e0, pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
-        pair_entry<ENERGY, RANGE, HES>(e1, pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
+    // wave w takes the whole tiles w, w + WAVES, ...: each lane reads its
+    // 16 entries of a tile as four 16-byte loads, the next tile's while the
+    // current one is evaluated (rows are padded to a multiple of 16)
+    int ntile = (len + 15) >> 4;
+    int t = wave;
+    uint4 q0, q1, q2, q3;
+    if (t < ntile) {
+        const uint4 *p = seg + (size_t)t * 256;
+        q0 = p[0]; q1 = p[1]; q2 = p[2]; q3 = p[3];
     }
-    if (k < len) pair_entry<ENERGY, RANGE, HES>(row[(size_t)k * 64], pi, qi, ljrow, pos4, type, pc, a, rs, i_in);
+    for (; t < ntile; t += WAVES) {
+        uint32_t e[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+        // gathers run PAIR_AHEAD entries ahead of the arithmetic
+        double4 pj[16];
+        int tj[16];
+#pragma unroll
+        for (int q = 0; q < PAIR_AHEAD; q++) {
+            int j = entry_site(e[q]);
+            pj[q] = pos4[j];
+            tj[q] = type[j];
+        }
+        if (t + WAVES < ntile) {
+            const uint4 *p = seg + (size_t)(t + WAVES) * 256;
+            q0 = p[0]; q1 = p[1]; q2 = p[2]; q3 = p[3];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            if (q + PAIR_AHEAD < 16) {
+                int j = entry_site(e[q + PAIR_AHEAD]);
+                pj[q + PAIR_AHEAD] = pos4[j];
+                tj[q + PAIR_AHEAD] = type[j];
+            }
+            pair_entry<ENERGY, RANGE, HES>(e[q], pj[q], tj[q], pi, qi, ljrow, pc, a, rs, i_in);
+        }
+    }
 
     sred[wave][lane][0] = a.fx;
     sred[wave][lane][1] = a.fy;

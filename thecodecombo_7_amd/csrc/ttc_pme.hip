@@ -870,14 +870,14 @@ static size_t fft_shm(const ttc_ctx *c, int axis) {
 
 // FFT passes, influence function, gather (+ bonds).  Expects the spread
 // grid in qfix; leaves qfix zeroed.
-static void launch_pme_impl(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes) {
+void ttc_launch_fft(ttc_ctx *c, bool energy) {
     double fac = TTC_PI * TTC_PI / c->p.kappa / c->p.kappa;
     double lx = c->p.lx, ly = c->p.ly, lz = c->p.lz;
     int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2];
     int nzb = (K2 + FFT_LB - 1) / FFT_LB;
     FftArgs fz = fft_args(c, 2), fy = fft_args(c, 1), fx = fft_args(c, 0);
     int bz = (K0 * K1 + FFT_LB - 1) / FFT_LB;
-    if (pme) {
+    {
     k_fft<0, false><<<bz, FFT_THREADS, fft_shm(c, 2), c->stream>>>(fz, +1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
     k_fft<1, false><<<K0 * nzb, FFT_THREADS, fft_shm(c, 1), c->stream>>>(fy, +1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
     if (energy)
@@ -887,7 +887,11 @@ static void launch_pme_impl(ttc_ctx *c, bool energy, bool pme, bool bonds, bool 
     k_fft<1, false><<<K0 * nzb, FFT_THREADS, fft_shm(c, 1), c->stream>>>(fy, -1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
     k_fft<3, false><<<bz, FFT_THREADS, fft_shm(c, 2), c->stream>>>(fz, -1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
     }
+}
 
+void ttc_launch_gather(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes) {
+    double lx = c->p.lx, ly = c->p.ly, lz = c->p.lz;
+    int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2];
     GatherConst gc;
     gc.lx = lx; gc.ly = ly; gc.lz = lz;
     gc.hx = 0.5 * lx; gc.hy = 0.5 * ly; gc.hz = 0.5 * lz;
@@ -904,6 +908,11 @@ static void launch_pme_impl(ttc_ctx *c, bool energy, bool pme, bool bonds, bool 
         if (hes) GATHER(6, true, true); else if (energy) GATHER(6, true, false); else GATHER(6, false, false);
     }
 #undef GATHER
+}
+
+static void launch_pme_impl(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes) {
+    if (pme) ttc_launch_fft(c, energy);
+    ttc_launch_gather(c, energy, pme, bonds, hes);
 }
 
 void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, bool bonds) { launch_pme_impl(c, energy, pme, bonds, false); }
