@@ -97,6 +97,9 @@ def mc_leg(ntrials: int, cpu: bool):
         "da_accepted_moves_per_s": round(st1[:, 1].sum() / t_da, 1),
         "plain_trials_per_s": round(st0[:, 0].sum() / t_plain, 1),
         "da_trials_per_s": round(st1[:, 0].sum() / t_da, 1),
+        "plain_pivot_accept_frac": round(st0[:, 6].sum() / max(1, st0[:, 5].sum()), 4),
+        "plain_displacement_accept_frac": round((st0[:, 1] - st0[:, 6]).sum() / max(1, (st0[:, 0] - st0[:, 5]).sum()), 4),
+        "da_pivot_accept_frac": round(st1[:, 6].sum() / max(1, st1[:, 5].sum()), 4),
         "da_stage1_reject_frac": round(st1[:, 2].sum() / st1[:, 0].sum(), 4),
         "da_exact_evals_frac": round(st1[:, 3].sum() / st1[:, 0].sum(), 4),
         "logged_moves": int(len(x)), "surrogate_loss": [round(loss0, 4), round(loss, 4)],

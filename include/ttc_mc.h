@@ -50,13 +50,17 @@ typedef struct {
 #define TTC_MC_STAT_STAGE1_REJECT 2
 #define TTC_MC_STAT_EXACT    3   /* exact energy evaluations */
 #define TTC_MC_STAT_OUTSIDE  4   /* proposals outside the WL window */
-#define TTC_MC_NSTAT 5
+#define TTC_MC_STAT_PIVOT    5   /* pivot trials (the rest are displacements) */
+#define TTC_MC_STAT_PIVOT_ACCEPT 6
+#define TTC_MC_NSTAT 7
 
 int  ttc_mc_create(const ttc_mc_params *p, ttc_mc **out);
 int  ttc_mc_destroy(ttc_mc *m);
 
 /* Walker w: n beads, positions (n*3), native contacts (a<b, b-a>=3) with
- * native distances, WL window [emin, emin + nbins*ebin).                  */
+ * native distances, WL window [emin, emin + nbins*ebin).  At most
+ * TTC_MC_MAXNAT native contacts per bead.                                 */
+#define TTC_MC_MAXNAT 32
 int  ttc_mc_set_walker(ttc_mc *m, int w, int n, const double *pos, int nnat, const int *na,
                        const int *nb, const double *nr0, double emin);
 int  ttc_mc_get_walker(ttc_mc *m, int w, double *pos, double *energy);

@@ -14,6 +14,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -22,7 +23,7 @@
 
 void ttc_set_error(const std::string &msg);
 
-#define MC_THREADS 512
+#define MC_THREADS 1024
 #define MC_WAVES (MC_THREADS / 64)
 #define NF TTC_MC_NFEAT
 #define HID TTC_MC_HID
@@ -42,7 +43,12 @@ struct ttc_mc {
     ttc_mc_params p;
     double *pos = nullptr;      // [nwalk][maxn][3]
     int *nbead = nullptr;       // [nwalk]
-    double *r0 = nullptr;       // [nwalk][maxn][maxn] native distance, 0 = non-native
+    // native contacts per walker, CSR over beads, both directions, partners
+    // ascending: noff[w][maxn+1], npart[w][ncap], nr0[w][ncap]
+    int *noff = nullptr, *npart = nullptr;
+    double *nr0 = nullptr;
+    int ncap = 0;               // CSR capacity per walker (maxn * TTC_MC_MAXNAT)
+    int max_nnz = 0;            // largest CSR of the walkers set so far
     double *emin = nullptr, *ecur = nullptr;
     double *lng = nullptr;      // [nwalk][nbins]
     long long *hist = nullptr;  // [nwalk][nbins]
@@ -73,15 +79,6 @@ __device__ __forceinline__ uint64_t xo_next(uint64_t *s) {
     return r;
 }
 __device__ __forceinline__ double xo_uniform(uint64_t *s) { return (double)(xo_next(s) >> 11) * (1.0 / 9007199254740992.0); }
-
-// Go pair energy (cgolik.c:1209-1229 form for natives, eps (sig/r)^12 else)
-__device__ __forceinline__ double go_pair(double r2, double r0, double eps, double sig2) {
-    double s2 = (r0 > 0.0 ? r0 * r0 : sig2) / r2;
-    double s6 = s2 * s2 * s2;
-    double s12 = s6 * s6;
-    if (r0 > 0.0) return eps * (5.0 * s12 - 6.0 * s12 / s2);
-    return eps * s12;
-}
 
 __device__ __forceinline__ float bf16_round(float x) {
     uint32_t u = __float_as_uint(x);
@@ -114,34 +111,82 @@ __device__ double block_sum(double v, double *red) {
     return t;
 }
 
+// 1/x to within about an ulp: hardware reciprocal refined by two Newton steps
+__device__ __forceinline__ double recip64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
+// Go pair energy from 1/r^2 (cgolik.c:1209-1229 form for natives,
+// eps (sig/r)^12 for the rest)
+__device__ __forceinline__ double go_pair_inv(double inv_r2, double r0, double eps, double sig2) {
+    double s2 = (r0 > 0.0 ? r0 * r0 : sig2) * inv_r2;
+    double s4 = s2 * s2, s6 = s4 * s2, s12 = s6 * s6;
+    return r0 > 0.0 ? eps * (5.0 * s12 - 6.0 * (s6 * s4)) : eps * s12;
+}
+
+// A walker's native contacts: CSR over beads (both directions), partners in
+// ascending order, native distance r0 per entry.
+struct Natives {
+    const int *off, *part;
+    const double *r0;
+};
+
+// Native distance of (b, ia) for an ascending sequence of ia: *p walks
+// b's partner list, *nextp is the partner at *p (INT_MAX past the end).
+__device__ __forceinline__ double native_r0(const Natives &nat, int b, int ia, int *p, int *nextp) {
+    int pe = nat.off[b + 1];
+    while (*nextp < ia) {
+        (*p)++;
+        *nextp = *p < pe ? nat.part[*p] : 0x7FFFFFFF;
+    }
+    return *nextp == ia ? nat.r0[*p] : 0.0;
+}
+
+__device__ __forceinline__ void native_start(const Natives &nat, int b, int ia0, int *p, int *nextp) {
+    int q = nat.off[b], pe = nat.off[b + 1];
+    while (q < pe && nat.part[q] < ia0) q++;
+    *p = q;
+    *nextp = q < pe ? nat.part[q] : 0x7FFFFFFF;
+}
+
 // Energy change of walker when beads [s,e) take newp[], all other beads fixed:
-// sum over bonds (i,i+1) and pairs |a-b|>=3 touching [s,e) of E(new)-E(old).
-__device__ double range_delta(const double *pos, const double *newp, int n, int s, int e, const double *r0row,
-                              int maxn, const McArgs &a, double *red) {
+// sum over bonds (i,i+1) and pairs |a-b|>=3 touching [s,e) of E(new)-E(old)
+// (forces_mc(lb,ub,1) - forces_mc(lb,ub,0)).  Each thread owns a bead b and
+// walks the moved beads a (the workgroup split into groups of n threads,
+// group g taking a = s+g, s+g+S, ...); pairs with both beads moved are
+// counted once, from the smaller index.
+// rigid: [s,e) moved as a rigid body (pivot rotation), so pairs inside it
+// keep their distances and their energy change is zero: they are skipped.
+__device__ double range_delta(const double *pos, const double *newp, int n, int s, int e, const Natives &nat,
+                              const McArgs &a, double *red, bool rigid = false) {
     double acc = 0.0;
-    int m = e - s;
-    long long npair = (long long)m * n;
-    for (long long t = threadIdx.x; t < npair; t += blockDim.x) {
-        int ia = s + (int)(t / n), b = (int)(t % n);
-        int d = ia - b;
-        if (d < 0) d = -d;
-        bool bin = (b >= s && b < e);
-        if (d < 3 || (bin && b < ia)) continue;   // bonded neighbours; intra pairs once
-        double ax = pos[3 * ia], ay = pos[3 * ia + 1], az = pos[3 * ia + 2];
-        double nx = newp[3 * ia], ny = newp[3 * ia + 1], nz = newp[3 * ia + 2];
+    const int S = max(1, (int)blockDim.x / n);
+    for (int idx = threadIdx.x; idx < n * S; idx += blockDim.x) {
+        int g = idx / n, b = idx - g * n;
+        bool bin = b >= s && b < e;
         double bx = pos[3 * b], by = pos[3 * b + 1], bz = pos[3 * b + 2];
         double cx = bin ? newp[3 * b] : bx, cy = bin ? newp[3 * b + 1] : by, cz = bin ? newp[3 * b + 2] : bz;
-        double dx = ax - bx, dy = ay - by, dz = az - bz;
-        double ex = nx - cx, ey = ny - cy, ez = nz - cz;
-        double r0 = r0row[(size_t)ia * maxn + b];
-        acc += go_pair(ex * ex + ey * ey + ez * ez, r0, a.eps, a.sig2) - go_pair(dx * dx + dy * dy + dz * dz, r0, a.eps, a.sig2);
+        int p, nextp;
+        native_start(nat, b, s + g, &p, &nextp);
+        for (int ia = s + g; ia < e; ia += S) {
+            double r0 = native_r0(nat, b, ia, &p, &nextp);
+            int d = ia - b;
+            if (d < 3 && d > -3) continue;            // bonded neighbours and itself
+            if (bin && (rigid || b < ia)) continue;   // pairs inside the moved range once, or not at all
+            double dx = pos[3 * ia] - bx, dy = pos[3 * ia + 1] - by, dz = pos[3 * ia + 2] - bz;
+            double ex = newp[3 * ia] - cx, ey = newp[3 * ia + 1] - cy, ez = newp[3 * ia + 2] - cz;
+            acc += go_pair_inv(recip64(ex * ex + ey * ey + ez * ez), r0, a.eps, a.sig2) -
+                   go_pair_inv(recip64(dx * dx + dy * dy + dz * dz), r0, a.eps, a.sig2);
+        }
     }
     // bonds (cbond.c:103-104 form) with at least one end in [s,e)
     int blo = s > 0 ? s - 1 : 0, bhi = e < n ? e : n - 1;
     for (int i = blo + (int)threadIdx.x; i < bhi; i += blockDim.x) {
         int j = i + 1;
         bool im = i >= s && i < e, jm = j >= s && j < e;
-        if (!im && !jm) continue;
+        if ((!im && !jm) || (rigid && im && jm)) continue;
         double ox = pos[3 * j] - pos[3 * i], oy = pos[3 * j + 1] - pos[3 * i + 1], oz = pos[3 * j + 2] - pos[3 * i + 2];
         const double *pi = im ? newp + 3 * i : pos + 3 * i;
         const double *pj = jm ? newp + 3 * j : pos + 3 * j;
@@ -153,14 +198,19 @@ __device__ double range_delta(const double *pos, const double *newp, int n, int 
     return block_sum(acc, red);
 }
 
-__device__ double full_energy(const double *pos, int n, const double *r0row, int maxn, const McArgs &a, double *red) {
+__device__ double full_energy(const double *pos, int n, const Natives &nat, const McArgs &a, double *red) {
     double acc = 0.0;
-    long long npair = (long long)n * n;
-    for (long long t = threadIdx.x; t < npair; t += blockDim.x) {
-        int ia = (int)(t / n), b = (int)(t % n);
-        if (b < ia + 3) continue;
-        double dx = pos[3 * ia] - pos[3 * b], dy = pos[3 * ia + 1] - pos[3 * b + 1], dz = pos[3 * ia + 2] - pos[3 * b + 2];
-        acc += go_pair(dx * dx + dy * dy + dz * dz, r0row[(size_t)ia * maxn + b], a.eps, a.sig2);
+    const int S = max(1, (int)blockDim.x / n);
+    for (int idx = threadIdx.x; idx < n * S; idx += blockDim.x) {
+        int g = idx / n, b = idx - g * n;
+        double bx = pos[3 * b], by = pos[3 * b + 1], bz = pos[3 * b + 2];
+        int p, nextp;
+        native_start(nat, b, g, &p, &nextp);
+        for (int ia = g; ia + 3 <= b; ia += S) {
+            double r0 = native_r0(nat, b, ia, &p, &nextp);
+            double dx = pos[3 * ia] - bx, dy = pos[3 * ia + 1] - by, dz = pos[3 * ia + 2] - bz;
+            acc += go_pair_inv(recip64(dx * dx + dy * dy + dz * dz), r0, a.eps, a.sig2);
+        }
     }
     for (int i = threadIdx.x; i < n - 1; i += blockDim.x) {
         double dx = pos[3 * (i + 1)] - pos[3 * i], dy = pos[3 * (i + 1) + 1] - pos[3 * i + 1], dz = pos[3 * (i + 1) + 2] - pos[3 * i + 2];
@@ -170,28 +220,55 @@ __device__ double full_energy(const double *pos, int n, const double *r0row, int
     return block_sum(acc, red);
 }
 
-// network forward of two feature rows (x, swapped x) -> out[2]; weights in LDS
+// Network weights as held in LDS by the sampling kernel: the layers'
+// matrices transposed (input-major), so that the 64 lanes computing the 64
+// outputs of a layer read consecutive words.
+//   W1T[NF][HID], b1[HID], W2T[HID][HID], b2[HID], W3[HID], b3
+__device__ void mlp_load_lds(const float *g, float *W) {
+    const float *W1 = g, *b1 = W1 + HID * NF, *W2 = b1 + HID, *b2 = W2 + HID * HID, *W3 = b2 + HID, *b3 = W3 + HID;
+    float *W1T = W, *sb1 = W1T + NF * HID, *W2T = sb1 + HID, *sb2 = W2T + HID * HID, *sW3 = sb2 + HID, *sb3 = sW3 + HID;
+    for (int t = threadIdx.x; t < NF * HID; t += blockDim.x) {
+        int i = t / HID, j = t - i * HID;
+        W1T[t] = W1[j * NF + i];
+    }
+    for (int t = threadIdx.x; t < HID * HID; t += blockDim.x) {
+        int i = t / HID, j = t - i * HID;
+        W2T[t] = W2[j * HID + i];
+    }
+    for (int t = threadIdx.x; t < HID; t += blockDim.x) {
+        sb1[t] = b1[t];
+        sb2[t] = b2[t];
+        sW3[t] = W3[t];
+    }
+    if (threadIdx.x == 0) sb3[0] = b3[0];
+}
+
+// network forward of two feature rows (x, swapped x) -> out[2]; weights in
+// LDS as laid out by mlp_load_lds; threads 0..127 = (row, output unit)
 __device__ void mlp2(const float *W, const float *x0, const float *x1, float *h1, float *h2, float *out, float *fred) {
-    const float *W1 = W, *b1 = W1 + HID * NF, *W2 = b1 + HID, *b2 = W2 + HID * HID, *W3 = b2 + HID, *b3 = W3 + HID;
+    const float *W1T = W, *b1 = W1T + NF * HID, *W2T = b1 + HID, *b2 = W2T + HID * HID, *W3 = b2 + HID, *b3 = W3 + HID;
     int t = threadIdx.x;
+    int r = t >> 6, j = t & 63;
     if (t < 2 * HID) {
-        int r = t / HID, j = t % HID;
         const float *x = r ? x1 : x0;
-        float s = b1[j];
-        for (int i = 0; i < NF; i++) s += W1[j * NF + i] * x[i];
-        h1[r * HID + j] = tanhf(s);
+        float s0 = b1[j], s1 = 0.0f;
+#pragma unroll
+        for (int i = 0; i < NF; i += 2) {
+            s0 += W1T[i * HID + j] * x[i];
+            s1 += W1T[(i + 1) * HID + j] * x[i + 1];
+        }
+        h1[r * HID + j] = tanhf(s0 + s1);
     }
     __syncthreads();
     if (t < 2 * HID) {
-        int r = t / HID, j = t % HID;
-        float s = b2[j];
-        for (int i = 0; i < HID; i++) s += W2[j * HID + i] * h1[r * HID + i];
-        h2[r * HID + j] = tanhf(s);
-    }
-    __syncthreads();
-    if (t < 2 * HID) {
-        int r = t / HID, j = t % HID;
-        float v = W3[j] * h2[r * HID + j];
+        const float *h = h1 + r * HID;
+        float s0 = b2[j], s1 = 0.0f;
+#pragma unroll 8
+        for (int i = 0; i < HID; i += 2) {
+            s0 += W2T[i * HID + j] * h[i];
+            s1 += W2T[(i + 1) * HID + j] * h[i + 1];
+        }
+        float v = W3[j] * tanhf(s0 + s1);
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
         if (j == 0) fred[r] = v + b3[0];
     }
@@ -200,10 +277,26 @@ __device__ void mlp2(const float *W, const float *x0, const float *x1, float *h1
     out[1] = fred[1];
 }
 
+// u < exp(lr) for u in [0,1): decided in FP32 unless u lies within 1e-5
+// (relative) of the threshold, then in FP64.  Every thread gets the same
+// answer from the same inputs.
+__device__ __forceinline__ bool accept_log(double lr, double u) {
+    if (lr >= 0.0) return true;
+    if (lr < -80.0) return u < exp(lr);   // below the FP32 range of exp
+    float t = __expf((float)lr), uf = (float)u;
+    if (uf < t * (1.0f - 1e-5f)) return true;
+    if (uf > t * (1.0f + 1e-5f)) return false;
+    return u < exp(lr);
+}
+
+// One workgroup per walker runs all its trials.  Every thread keeps an
+// identical copy of the walker's random state and draws the same proposal
+// and acceptance numbers, so no broadcast is needed; the moved coordinates
+// and the exact energy change are computed cooperatively.
 template <bool DA>
 __global__ void __launch_bounds__(MC_THREADS)
-k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int *gn, const double *gr0,
-         const double *gemin, double *gecur, double *glng, long long *ghist, uint64_t *grng, long long *gstats,
+k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int *gn, const int *gnoff,
+         const int *gnpart, const double *gnr0, int ncap, int max_nnz, const double *gemin, double *gecur, double *glng, long long *ghist, uint64_t *grng, long long *gstats,
          const float *gmlp, float *logx, double *logde, int *logn, int logcap, double escale) {
     extern __shared__ double smem[];
     int w = blockIdx.x;
@@ -219,86 +312,93 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     float *h1 = feat + 2 * NF;                // 2*HID
     float *h2 = h1 + 2 * HID;
     float *fred = h2 + 2 * HID;               // 2
-    __shared__ int mv_s, mv_e, mv_type, mv_decide;
-    __shared__ double mv_de, mv_dee;
-    __shared__ uint64_t rs[4];
+    double *sr0 = (double *)(((uintptr_t)(fred + 2) + 7) & ~(uintptr_t)7);   // max_nnz
+    int *soff = (int *)(sr0 + max_nnz);       // maxn + 1
+    int *spart = soff + a.maxn + 1;           // max_nnz
 
-    const double *r0row = gr0 + (size_t)w * a.maxn * a.maxn;
     for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) pos[i] = gpos[(size_t)w * 3 * a.maxn + i];
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) soff[i] = gnoff[(size_t)w * (a.maxn + 1) + i];
+    {
+        int nnz = gnoff[(size_t)w * (a.maxn + 1) + n];
+        for (int i = threadIdx.x; i < nnz; i += blockDim.x) {
+            spart[i] = gnpart[(size_t)w * ncap + i];
+            sr0[i] = gnr0[(size_t)w * ncap + i];
+        }
+    }
+    const Natives nat = {soff, spart, sr0};
     for (int i = threadIdx.x; i < a.nbins; i += blockDim.x) {
         lng[i] = glng[(size_t)w * a.nbins + i];
         hist[i] = ghist[(size_t)w * a.nbins + i];
     }
-    if (DA)
-        for (int i = threadIdx.x; i < NPAR; i += blockDim.x) W[i] = gmlp[i];
-    if (threadIdx.x < 4) rs[threadIdx.x] = grng[4 * w + threadIdx.x];
+    if (DA) mlp_load_lds(gmlp, W);
+    uint64_t s4[4] = {grng[4 * w], grng[4 * w + 1], grng[4 * w + 2], grng[4 * w + 3]};
     __syncthreads();
     double emin = gemin[w];
     double ecur = gecur[w];
-    long long st_tr = 0, st_acc = 0, st_s1 = 0, st_ex = 0, st_out = 0;
+    long long st_tr = 0, st_acc = 0, st_s1 = 0, st_ex = 0, st_out = 0, st_piv = 0, st_pacc = 0;
+    // Wang-Landau update of the bin a trial ended in (pivot.c:236-253):
+    // every thread reads ln g for its own (identical) acceptance decision, so
+    // thread 0 applies the update only after the next barrier, when no wave
+    // can still be reading the old value
+    int pend = -1;
 
     for (int trial = 0; trial < ntrials; trial++) {
         // ---- proposal (mcmove_pivot.c:54-140, mcmove_displ.c:45-99)
-        double rot[9];
-        int piv = 0;
-        double dvec[3] = {0.0, 0.0, 0.0};
-        if (threadIdx.x == 0) {
-            uint64_t s4[4] = {rs[0], rs[1], rs[2], rs[3]};
-            double u = xo_uniform(s4);
-            if (u < a.pivot_frac && n > 2) {
-                int p = (int)(xo_uniform(s4) * n);
-                int axis = (int)(xo_uniform(s4) * 3.0);
-                int side = (int)(xo_uniform(s4) * 2.0);
-                int dir = (int)(xo_uniform(s4) * 2.0);
-                double th = xo_uniform(s4) * MC_PI * a.theta_max;
-                mv_type = 1;
-                mv_s = side == 0 ? 0 : p + 1;
-                mv_e = side == 0 ? p : n;
-                mv_de = dir ? -th : th;   // signed angle
-                mv_dee = (double)(p * 3 + axis);
-            } else {
-                int k = (int)(xo_uniform(s4) * n);
-                mv_type = 0;
-                mv_s = k;
-                mv_e = k + 1;
-                dvec[0] = (xo_uniform(s4) - 0.5) * a.disp_max;
-                dvec[1] = (xo_uniform(s4) - 0.5) * a.disp_max;
-                dvec[2] = (xo_uniform(s4) - 0.5) * a.disp_max;
-                newp[3 * k] = pos[3 * k] + dvec[0];
-                newp[3 * k + 1] = pos[3 * k + 1] + dvec[1];
-                newp[3 * k + 2] = pos[3 * k + 2] + dvec[2];
-                mv_de = sqrt(dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2]) / a.disp_max;
+        int s, e, type, piv = 0;
+        double th = 0.0;
+        double u = xo_uniform(s4);
+        if (u < a.pivot_frac && n > 2) {
+            int p = (int)(xo_uniform(s4) * n);
+            int axis = (int)(xo_uniform(s4) * 3.0);
+            int side = (int)(xo_uniform(s4) * 2.0);
+            int dir = (int)(xo_uniform(s4) * 2.0);
+            double ang = xo_uniform(s4) * MC_PI * a.theta_max;
+            type = 1;
+            s = side == 0 ? 0 : p + 1;
+            e = side == 0 ? p : n;
+            th = dir ? -ang : ang;   // signed angle
+            piv = p;
+            // rotation about x, y or z through the pivot bead (PIVOT matrices),
+            // evaluated only by the threads that move a bead; written out per
+            // axis (hipcc 7.2 at -O3 mis-assembled the z matrix when it was
+            // built in a zero-initialised 3x3 array and applied generically)
+            if ((int)threadIdx.x < e - s) {
+                double c = cos(th), sn = sin(th);
+                double px = pos[3 * piv], py = pos[3 * piv + 1], pz = pos[3 * piv + 2];
+                for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
+                    double x = pos[3 * i] - px, y = pos[3 * i + 1] - py, z = pos[3 * i + 2] - pz;
+                    double nx, ny, nz;
+                    if (axis == 0) { nx = x; ny = c * y - sn * z; nz = sn * y + c * z; }
+                    else if (axis == 1) { nx = c * x + sn * z; ny = y; nz = c * z - sn * x; }
+                    else { nx = c * x + sn * y; ny = c * y - sn * x; nz = z; }
+                    newp[3 * i] = nx + px;
+                    newp[3 * i + 1] = ny + py;
+                    newp[3 * i + 2] = nz + pz;
+                }
             }
-            rs[0] = s4[0]; rs[1] = s4[1]; rs[2] = s4[2]; rs[3] = s4[3];
-        }
-        __syncthreads();
-        int s = mv_s, e = mv_e, type = mv_type;
-        if (type == 1) {
-            // rotation about x, y or z through the pivot bead (PIVOT matrices)
-            int code = (int)mv_dee;
-            piv = code / 3;
-            int axis = code % 3;
-            double th = mv_de, c = cos(th), sn = sin(th);
-            for (int q = 0; q < 9; q++) rot[q] = 0.0;
-            if (axis == 0) { rot[0] = 1; rot[4] = c; rot[5] = -sn; rot[7] = sn; rot[8] = c; }
-            else if (axis == 1) { rot[0] = c; rot[2] = sn; rot[4] = 1; rot[6] = -sn; rot[8] = c; }
-            else { rot[0] = c; rot[1] = sn; rot[3] = -sn; rot[4] = c; rot[8] = 1; }
-            double px = pos[3 * piv], py = pos[3 * piv + 1], pz = pos[3 * piv + 2];
-            for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
-                double x = pos[3 * i] - px, y = pos[3 * i + 1] - py, z = pos[3 * i + 2] - pz;
-                newp[3 * i] = rot[0] * x + rot[1] * y + rot[2] * z + px;
-                newp[3 * i + 1] = rot[3] * x + rot[4] * y + rot[5] * z + py;
-                newp[3 * i + 2] = rot[6] * x + rot[7] * y + rot[8] * z + pz;
-            }
-        }
-        __syncthreads();
-        st_tr++;
-        if (e <= s) {   // empty moved part: nothing changes, counts as a rejected trial
+        } else {
+            int k = (int)(xo_uniform(s4) * n);
+            type = 0;
+            s = k;
+            e = k + 1;
+            double d0 = (xo_uniform(s4) - 0.5) * a.disp_max;
+            double d1 = (xo_uniform(s4) - 0.5) * a.disp_max;
+            double d2 = (xo_uniform(s4) - 0.5) * a.disp_max;
             if (threadIdx.x == 0) {
-                int b = (int)floor((ecur - emin) / a.ebin);
-                if (b >= 0 && b < a.nbins) { lng[b] += lnf; hist[b]++; }
+                newp[3 * k] = pos[3 * k] + d0;
+                newp[3 * k + 1] = pos[3 * k + 1] + d1;
+                newp[3 * k + 2] = pos[3 * k + 2] + d2;
             }
-            __syncthreads();
+        }
+        __syncthreads();
+        // the previous trial's bin; every read of ln g below follows another barrier
+        if (threadIdx.x == 0 && pend >= 0) { lng[pend] += lnf; hist[pend]++; }
+        pend = -1;
+        st_tr++;
+        st_piv += type;
+        int bx = (int)floor((ecur - emin) / a.ebin);
+        if (e <= s) {   // empty moved part: nothing changes, counts as a rejected trial
+            if (bx >= 0 && bx < a.nbins) pend = bx;
             continue;
         }
         // ---- features of pivot moves (local distances around the pivot)
@@ -321,120 +421,111 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                 feat[NF + 16 + threadIdx.x] = bf16_round(fo);
             } else if (threadIdx.x < 20) {
                 int q = threadIdx.x - 16;
-                float v = q == 0 ? 1.0f : q == 1 ? (float)(fabs(mv_de) / MC_PI) : q == 2 ? (float)(e - s) / (float)n : 0.0f;
+                float v = q == 0 ? 1.0f : q == 1 ? (float)(fabs(th) / MC_PI) : q == 2 ? (float)(e - s) / (float)n : 0.0f;
                 v = bf16_round(v);
                 feat[32 + q] = v;
                 feat[NF + 32 + q] = v;
             }
             __syncthreads();
         }
-        int bx = (int)floor((ecur - emin) / a.ebin);
-        double sur = 0.0, a1f = 1.0;
+        double sur = 0.0, a1f = 0.0;   // a1f: log of the first-stage probability
         if (DA && type == 1) {
             float o2[2];
             mlp2(W, feat, feat + NF, h1, h2, o2, fred);
-            sur = escale * sinh((double)(o2[0] - o2[1]));   // antisymmetric surrogate dE
-            if (threadIdx.x == 0) {
-                int by = (int)floor((ecur + sur - emin) / a.ebin);
-                double a1 = 0.0;
-                if (bx >= 0 && bx < a.nbins && by >= 0 && by < a.nbins) a1 = fmin(1.0, exp(lng[bx] - lng[by]));
-                uint64_t s4[4] = {rs[0], rs[1], rs[2], rs[3]};
-                double u1 = xo_uniform(s4);
-                rs[0] = s4[0]; rs[1] = s4[1]; rs[2] = s4[2]; rs[3] = s4[3];
-                mv_decide = u1 < a1 ? 1 : 0;
-                mv_dee = a1;
-            }
-            __syncthreads();
-            a1f = mv_dee;
-            if (!mv_decide) {
+            sur = escale * (double)sinhf(o2[0] - o2[1]);   // antisymmetric surrogate dE
+            int by = (int)floor((ecur + sur - emin) / a.ebin);
+            // first stage a1 = min(1, g(E_x)/g(E_x + sur)), as log a1
+            double la1 = -INFINITY;
+            if (bx >= 0 && bx < a.nbins && by >= 0 && by < a.nbins) la1 = fmin(0.0, lng[bx] - lng[by]);
+            double u1 = xo_uniform(s4);
+            a1f = la1;
+            if (!accept_log(la1, u1)) {
                 st_s1++;
-                if (threadIdx.x == 0 && bx >= 0 && bx < a.nbins) { lng[bx] += lnf; hist[bx]++; }
-                __syncthreads();
+                if (bx >= 0 && bx < a.nbins) pend = bx;
                 continue;
             }
         }
         // ---- exact range energy change (FP64)
-        double de = range_delta(pos, newp, n, s, e, r0row, a.maxn, a, red);
+        double de = range_delta(pos, newp, n, s, e, nat, a, red, type == 1);
         st_ex++;
-        if (threadIdx.x == 0) {
-            int by = (int)floor((ecur + de - emin) / a.ebin);
-            int acc = 0;
-            if (by < 0 || by >= a.nbins || bx < 0 || bx >= a.nbins) {
-                acc = 0;
-                st_out++;
-            } else {
-                double ratio = exp(lng[bx] - lng[by]);
-                if (DA && type == 1) {
-                    // second stage: pi(y) a1(y,x) / (pi(x) a1(x,y)), a1(y,x) from the
-                    // surrogate of the reverse move, -sur
-                    int bz = (int)floor((ecur + de - sur - emin) / a.ebin);
-                    double a1r = 0.0;
-                    if (bz >= 0 && bz < a.nbins) a1r = fmin(1.0, exp(lng[by] - lng[bz]));
-                    ratio = ratio * a1r / a1f;
-                }
-                uint64_t s4[4] = {rs[0], rs[1], rs[2], rs[3]};
-                double u = xo_uniform(s4);
-                rs[0] = s4[0]; rs[1] = s4[1]; rs[2] = s4[2]; rs[3] = s4[3];
-                acc = (ratio >= 1.0 || u < ratio) ? 1 : 0;
+        int by = (int)floor((ecur + de - emin) / a.ebin);
+        bool acc = false;
+        if (by < 0 || by >= a.nbins || bx < 0 || bx >= a.nbins) {
+            st_out++;
+        } else {
+            double lratio = lng[bx] - lng[by];
+            if (DA && type == 1) {
+                // second stage: pi(y) a1(y,x) / (pi(x) a1(x,y)), a1(y,x) from the
+                // surrogate of the reverse move, -sur (all as logarithms)
+                int bz = (int)floor((ecur + de - sur - emin) / a.ebin);
+                double la1r = -INFINITY;
+                if (bz >= 0 && bz < a.nbins) la1r = fmin(0.0, lng[by] - lng[bz]);
+                lratio = lratio + la1r - a1f;
             }
-            mv_decide = acc;
-            if (do_log && type == 1) {
-                int slot = atomicAdd(logn, 1);
-                if (slot < logcap) {
-                    for (int q = 0; q < NF; q++) logx[(size_t)slot * NF + q] = feat[q];
-                    logde[slot] = de;
-                }
+            double u2 = xo_uniform(s4);
+            acc = accept_log(lratio, u2);
+        }
+        if (threadIdx.x == 0 && do_log && type == 1) {
+            int slot = atomicAdd(logn, 1);
+            if (slot < logcap) {
+                for (int q = 0; q < NF; q++) logx[(size_t)slot * NF + q] = feat[q];
+                logde[slot] = de;
             }
         }
-        __syncthreads();
-        if (mv_decide) {
+        if (acc) {
             for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
                 pos[3 * i] = newp[3 * i]; pos[3 * i + 1] = newp[3 * i + 1]; pos[3 * i + 2] = newp[3 * i + 2];
             }
             ecur += de;
             st_acc++;
+            st_pacc += type;
+            __syncthreads();   // the next proposal reads the moved beads
         }
-        if (threadIdx.x == 0) {
-            int b = (int)floor((ecur - emin) / a.ebin);
-            if (b >= 0 && b < a.nbins) { lng[b] += lnf; hist[b]++; }
-        }
-        __syncthreads();
+        int b = (int)floor((ecur - emin) / a.ebin);
+        if (b >= 0 && b < a.nbins) pend = b;
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && pend >= 0) { lng[pend] += lnf; hist[pend]++; }
+    __syncthreads();
     for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) gpos[(size_t)w * 3 * a.maxn + i] = pos[i];
     for (int i = threadIdx.x; i < a.nbins; i += blockDim.x) {
         glng[(size_t)w * a.nbins + i] = lng[i];
         ghist[(size_t)w * a.nbins + i] = hist[i];
     }
-    if (threadIdx.x < 4) grng[4 * w + threadIdx.x] = rs[threadIdx.x];
     if (threadIdx.x == 0) {
+        for (int q = 0; q < 4; q++) grng[4 * w + q] = s4[q];
         gecur[w] = ecur;
         long long *st = gstats + (size_t)w * TTC_MC_NSTAT;
         st[0] += st_tr; st[1] += st_acc; st[2] += st_s1; st[3] += st_ex; st[4] += st_out;
+        st[5] += st_piv; st[6] += st_pacc;
     }
 }
 
 __global__ void __launch_bounds__(MC_THREADS)
-k_mc_energy(McArgs a, const double *gpos, const int *gn, const double *gr0, double *eout) {
+k_mc_energy(McArgs a, const double *gpos, const int *gn, const int *gnoff, const int *gnpart, const double *gnr0,
+            int ncap, double *eout) {
     extern __shared__ double smem[];
     int w = blockIdx.x;
     int n = gn[w];
     double *pos = smem, *red = pos + 3 * a.maxn;
     for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) pos[i] = gpos[(size_t)w * 3 * a.maxn + i];
     __syncthreads();
-    double e = n > 0 ? full_energy(pos, n, gr0 + (size_t)w * a.maxn * a.maxn, a.maxn, a, red) : 0.0;
+    const Natives nat = {gnoff + (size_t)w * (a.maxn + 1), gnpart + (size_t)w * ncap, gnr0 + (size_t)w * ncap};
+    double e = n > 0 ? full_energy(pos, n, nat, a, red) : 0.0;
     if (threadIdx.x == 0) eout[w] = e;
 }
 
 __global__ void __launch_bounds__(MC_THREADS)
-k_mc_delta(McArgs a, int w, int s, int e, const double *gpos, const int *gn, const double *gr0,
-           const double *newrange, double *out) {
+k_mc_delta(McArgs a, int w, int s, int e, const double *gpos, const int *gn, const int *gnoff, const int *gnpart,
+           const double *gnr0, int ncap, const double *newrange, double *out) {
     extern __shared__ double smem[];
     int n = gn[w];
     double *pos = smem, *newp = pos + 3 * a.maxn, *red = newp + 3 * a.maxn;
     for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) pos[i] = gpos[(size_t)w * 3 * a.maxn + i];
     for (int i = threadIdx.x; i < 3 * (e - s); i += blockDim.x) newp[3 * s + i] = newrange[i];
     __syncthreads();
-    double d = range_delta(pos, newp, n, s, e, gr0 + (size_t)w * a.maxn * a.maxn, a.maxn, a, red);
+    const Natives nat = {gnoff + (size_t)w * (a.maxn + 1), gnpart + (size_t)w * ncap, gnr0 + (size_t)w * ncap};
+    double d = range_delta(pos, newp, n, s, e, nat, a, red);
     if (threadIdx.x == 0) *out = d;
 }
 
@@ -635,8 +726,13 @@ static size_t run_shm(const ttc_mc *m, bool da) {
     size_t d = (size_t)(6 * m->p.maxn + m->p.nbins) * sizeof(double) + (size_t)m->p.nbins * sizeof(long long) +
                MC_WAVES * sizeof(double);
     size_t f = (size_t)((da ? NPAR : 0) + 2 * NF + 4 * HID + 2) * sizeof(float);
-    return d + f;
+    // native contacts: r0 (aligned up to 8 bytes), offsets, partners
+    size_t nat = 8 + (size_t)m->max_nnz * (sizeof(double) + sizeof(int)) + (size_t)(m->p.maxn + 1) * sizeof(int);
+    return d + f + nat;
 }
+
+// LDS of the sampling kernel on gfx950 (160 KB per workgroup)
+#define MC_MAX_LDS (160 * 1024)
 
 static size_t train_shm() {
     const int R = 2 * TR_B;
@@ -675,7 +771,10 @@ extern "C" int ttc_mc_create(const ttc_mc_params *p, ttc_mc **out) {
     HIPCHK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
     HIPCHK(hipMalloc(&m->pos, sizeof(double) * W * N * 3));
     HIPCHK(hipMalloc(&m->nbead, sizeof(int) * W));
-    HIPCHK(hipMalloc(&m->r0, sizeof(double) * W * N * N));
+    m->ncap = (int)N * TTC_MC_MAXNAT;
+    HIPCHK(hipMalloc(&m->noff, sizeof(int) * W * (N + 1)));
+    HIPCHK(hipMalloc(&m->npart, sizeof(int) * W * m->ncap));
+    HIPCHK(hipMalloc(&m->nr0, sizeof(double) * W * m->ncap));
     HIPCHK(hipMalloc(&m->emin, sizeof(double) * W));
     HIPCHK(hipMalloc(&m->ecur, sizeof(double) * W));
     HIPCHK(hipMalloc(&m->lng, sizeof(double) * W * p->nbins));
@@ -685,7 +784,7 @@ extern "C" int ttc_mc_create(const ttc_mc_params *p, ttc_mc **out) {
     HIPCHK(hipMalloc(&m->mlp, sizeof(float) * NPAR));
     HIPCHK(hipMalloc(&m->adam, sizeof(float) * 2 * NPAR));
     HIPCHK(hipMemset(m->nbead, 0, sizeof(int) * W));
-    HIPCHK(hipMemset(m->r0, 0, sizeof(double) * W * N * N));
+    HIPCHK(hipMemset(m->noff, 0, sizeof(int) * W * (N + 1)));
     HIPCHK(hipMemset(m->lng, 0, sizeof(double) * W * p->nbins));
     HIPCHK(hipMemset(m->hist, 0, sizeof(long long) * W * p->nbins));
     HIPCHK(hipMemset(m->stats, 0, sizeof(long long) * W * TTC_MC_NSTAT));
@@ -705,9 +804,6 @@ extern "C" int ttc_mc_create(const ttc_mc_params *p, ttc_mc **out) {
     HIPCHK(hipMalloc(&m->logde, sizeof(double) * m->logcap));
     HIPCHK(hipMalloc(&m->logn, sizeof(int)));
     HIPCHK(hipMemset(m->logn, 0, sizeof(int)));
-    const void *fns[] = {(const void *)k_mc_run<true>, (const void *)k_mc_run<false>};
-    for (const void *f : fns)
-        HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)run_shm(m, true)));
     const void *tfns[] = {(const void *)k_mlp_train, (const void *)k_mlp_loss_grad, (const void *)k_mlp_forward};
     for (const void *f : tfns)
         HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)train_shm()));
@@ -718,7 +814,7 @@ extern "C" int ttc_mc_create(const ttc_mc_params *p, ttc_mc **out) {
 extern "C" int ttc_mc_destroy(ttc_mc *m) {
     if (!m) return TTC_OK;
     hipStreamSynchronize(m->stream);
-    void *b[] = {m->pos, m->nbead, m->r0, m->emin, m->ecur, m->lng, m->hist, m->rng, m->stats, m->mlp, m->adam,
+    void *b[] = {m->pos, m->nbead, m->noff, m->npart, m->nr0, m->emin, m->ecur, m->lng, m->hist, m->rng, m->stats, m->mlp, m->adam,
                  m->logx, m->logde, m->logn};
     for (void *x : b)
         if (x) hipFree(x);
@@ -732,15 +828,33 @@ extern "C" int ttc_mc_set_walker(ttc_mc *m, int w, int n, const double *pos, int
     MCHECK(m && pos && w >= 0 && w < m->p.nwalk && n >= 4 && n <= m->p.maxn, TTC_ERR_ARG,
            "ttc_mc_set_walker: bad argument");
     size_t N = m->p.maxn;
-    std::vector<double> r0(N * N, 0.0);
+    // native contacts as CSR over beads, both directions, partners ascending
+    std::vector<std::vector<std::pair<int, double>>> adj(n);
     for (int c = 0; c < nnat; c++) {
         int a = na[c], b = nb[c];
         MCHECK(a >= 0 && a < n && b >= 0 && b < n && abs(a - b) >= 3 && nr0[c] > 0, TTC_ERR_ARG,
                "ttc_mc_set_walker: bad native contact");
-        r0[(size_t)a * N + b] = r0[(size_t)b * N + a] = nr0[c];
+        adj[a].push_back({b, nr0[c]});
+        adj[b].push_back({a, nr0[c]});
     }
+    std::vector<int> off(n + 1, 0), part;
+    std::vector<double> rr;
+    for (int i = 0; i < n; i++) {
+        std::sort(adj[i].begin(), adj[i].end());
+        for (size_t k = 1; k < adj[i].size(); k++)
+            MCHECK(adj[i][k].first != adj[i][k - 1].first, TTC_ERR_ARG, "ttc_mc_set_walker: repeated native contact");
+        MCHECK((int)adj[i].size() <= TTC_MC_MAXNAT, TTC_ERR_ARG,
+               "ttc_mc_set_walker: more than TTC_MC_MAXNAT native contacts on one bead");
+        for (auto &q : adj[i]) { part.push_back(q.first); rr.push_back(q.second); }
+        off[i + 1] = (int)part.size();
+    }
+    HIPCHK(hipMemcpy(m->noff + (size_t)w * (N + 1), off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    if (!part.empty()) {
+        HIPCHK(hipMemcpy(m->npart + (size_t)w * m->ncap, part.data(), sizeof(int) * part.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(m->nr0 + (size_t)w * m->ncap, rr.data(), sizeof(double) * rr.size(), hipMemcpyHostToDevice));
+    }
+    m->max_nnz = std::max(m->max_nnz, (int)part.size());
     HIPCHK(hipMemcpy(m->pos + (size_t)w * N * 3, pos, sizeof(double) * n * 3, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(m->r0 + (size_t)w * N * N, r0.data(), sizeof(double) * N * N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m->nbead + w, &n, sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m->emin + w, &emin, sizeof(double), hipMemcpyHostToDevice));
     m->hn[w] = n;
@@ -765,7 +879,8 @@ extern "C" int ttc_mc_energy(ttc_mc *m, double *e) {
     double *d;
     HIPCHK(hipMalloc(&d, sizeof(double) * m->p.nwalk));
     size_t shm = (size_t)(3 * m->p.maxn + MC_WAVES) * sizeof(double);
-    k_mc_energy<<<m->p.nwalk, MC_THREADS, shm, m->stream>>>(mc_args(m), m->pos, m->nbead, m->r0, d);
+    k_mc_energy<<<m->p.nwalk, MC_THREADS, shm, m->stream>>>(mc_args(m), m->pos, m->nbead, m->noff, m->npart,
+                                                            m->nr0, m->ncap, d);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(e, d, sizeof(double) * m->p.nwalk, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
@@ -781,7 +896,8 @@ extern "C" int ttc_mc_delta_energy(ttc_mc *m, int w, int lb, int ub, const doubl
     HIPCHK(hipMalloc(&np, sizeof(double) * 3 * (ub - lb)));
     HIPCHK(hipMemcpy(np, newpos, sizeof(double) * 3 * (ub - lb), hipMemcpyHostToDevice));
     size_t shm = (size_t)(6 * m->p.maxn + MC_WAVES) * sizeof(double);
-    k_mc_delta<<<1, MC_THREADS, shm, m->stream>>>(mc_args(m), w, lb, ub, m->pos, m->nbead, m->r0, np, d);
+    k_mc_delta<<<1, MC_THREADS, shm, m->stream>>>(mc_args(m), w, lb, ub, m->pos, m->nbead, m->noff, m->npart,
+                                                  m->nr0, m->ncap, np, d);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(de, d, sizeof(double), hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
@@ -797,14 +913,21 @@ extern "C" int ttc_mc_run(ttc_mc *m, int ntrials, int mode, double lnf, int log)
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, m->stream));
-    if (mode == 1)
-        k_mc_run<true><<<m->p.nwalk, MC_THREADS, run_shm(m, true), m->stream>>>(
-            mc_args(m), ntrials, lnf, log, m->pos, m->nbead, m->r0, m->emin, m->ecur, m->lng, m->hist, m->rng,
-            m->stats, m->mlp, m->logx, m->logde, m->logn, m->logcap, escale);
-    else
-        k_mc_run<false><<<m->p.nwalk, MC_THREADS, run_shm(m, false), m->stream>>>(
-            mc_args(m), ntrials, lnf, log, m->pos, m->nbead, m->r0, m->emin, m->ecur, m->lng, m->hist, m->rng,
-            m->stats, m->mlp, m->logx, m->logde, m->logn, m->logcap, escale);
+    // at least half the CU's LDS per workgroup: one walker per CU, so no CU
+    // holds two walkers while another holds none
+    size_t shm = std::max(run_shm(m, mode == 1), (size_t)MC_MAX_LDS / 2 + 1024);
+    MCHECK(run_shm(m, mode == 1) <= MC_MAX_LDS, TTC_ERR_ARG, "ttc_mc_run: walkers too large for the workgroup's LDS");
+    if (mode == 1) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_mc_run<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        k_mc_run<true><<<m->p.nwalk, MC_THREADS, shm, m->stream>>>(
+            mc_args(m), ntrials, lnf, log, m->pos, m->nbead, m->noff, m->npart, m->nr0, m->ncap, m->max_nnz, m->emin,
+            m->ecur, m->lng, m->hist, m->rng, m->stats, m->mlp, m->logx, m->logde, m->logn, m->logcap, escale);
+    } else {
+        HIPCHK(hipFuncSetAttribute((const void *)k_mc_run<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        k_mc_run<false><<<m->p.nwalk, MC_THREADS, shm, m->stream>>>(
+            mc_args(m), ntrials, lnf, log, m->pos, m->nbead, m->noff, m->npart, m->nr0, m->ncap, m->max_nnz, m->emin,
+            m->ecur, m->lng, m->hist, m->rng, m->stats, m->mlp, m->logx, m->logde, m->logn, m->logcap, escale);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, m->stream));
     HIPCHK(hipEventSynchronize(e1));

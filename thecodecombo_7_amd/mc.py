@@ -23,6 +23,7 @@ from . import system
 
 NF = 36
 HID = 64
+NSTAT = 7   # trials, accepted, stage-1 rejects, exact evaluations, outside window, pivots, accepted pivots
 
 
 class McParams(C.Structure):
@@ -134,9 +135,9 @@ class Walkers:
         return float(t[0])
 
     def stats(self):
-        s = np.zeros(self.nwalk * 5, np.int64)
+        s = np.zeros(self.nwalk * NSTAT, np.int64)
         T.check(self.L.ttc_mc_get_stats(self.h, s.ctypes.data_as(C.POINTER(C.c_int64))))
-        return s.reshape(self.nwalk, 5)
+        return s.reshape(self.nwalk, NSTAT)
 
     def reset_stats(self):
         T.check(self.L.ttc_mc_reset_stats(self.h))
