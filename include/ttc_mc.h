@@ -41,7 +41,15 @@ typedef struct {
     uint64_t seed;
 } ttc_mc_params;
 
-#define TTC_MC_NFEAT 36  /* surrogate input features per move */
+/* Surrogate input features of a pivot move (all rounded to bfloat16):
+ *   [0,16)  min((sig/r)^6, 16) for the 4 moved x 4 fixed beads nearest the
+ *           pivot, old positions;  [16,32) the same, new positions
+ *   [32,40) the same term for the closest fixed bead (|i-j| >= 3) of the
+ *           moved beads at chain offsets 2, 4, ..., 256 from the pivot, old;
+ *           [40,48) new
+ *   [48] 1, [49] |angle|/pi, [50] moved fraction, [51] 0
+ * The reverse move swaps old and new.                                       */
+#define TTC_MC_NFEAT 52  /* surrogate input features per move */
 #define TTC_MC_HID   64  /* surrogate hidden width            */
 
 /* stats per walker (ttc_mc_get_stats), int64 each */

@@ -129,7 +129,8 @@ def test_delayed_acceptance_keeps_distribution():
 def test_mlp_gradient_matches_oracle(walkers):
     import pyoracle
     L = pyoracle.lib()
-    nin, H = 36, 64
+    from thecodecombo_7_amd.mc import NF, HID
+    nin, H = NF, HID
     npar = L.orc_mlp_nparams(nin, H)
     assert npar == walkers.nparams()
     rng = np.random.default_rng(9)

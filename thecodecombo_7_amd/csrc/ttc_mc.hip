@@ -158,27 +158,49 @@ __device__ __forceinline__ void native_start(const Natives &nat, int b, int ia0,
 // group g taking a = s+g, s+g+S, ...); pairs with both beads moved are
 // counted once, from the smaller index.
 // rigid: [s,e) moved as a rigid body (pivot rotation), so pairs inside it
-// keep their distances and their energy change is zero: they are skipped.
+// keep their distances and their energy change is zero: only the fixed beads
+// own pairs, and the workgroup is split over them alone.
 __device__ double range_delta(const double *pos, const double *newp, int n, int s, int e, const Natives &nat,
                               const McArgs &a, double *red, bool rigid = false) {
     double acc = 0.0;
-    const int S = max(1, (int)blockDim.x / n);
-    for (int idx = threadIdx.x; idx < n * S; idx += blockDim.x) {
-        int g = idx / n, b = idx - g * n;
-        bool bin = b >= s && b < e;
-        double bx = pos[3 * b], by = pos[3 * b + 1], bz = pos[3 * b + 2];
-        double cx = bin ? newp[3 * b] : bx, cy = bin ? newp[3 * b + 1] : by, cz = bin ? newp[3 * b + 2] : bz;
-        int p, nextp;
-        native_start(nat, b, s + g, &p, &nextp);
-        for (int ia = s + g; ia < e; ia += S) {
-            double r0 = native_r0(nat, b, ia, &p, &nextp);
-            int d = ia - b;
-            if (d < 3 && d > -3) continue;            // bonded neighbours and itself
-            if (bin && (rigid || b < ia)) continue;   // pairs inside the moved range once, or not at all
-            double dx = pos[3 * ia] - bx, dy = pos[3 * ia + 1] - by, dz = pos[3 * ia + 2] - bz;
-            double ex = newp[3 * ia] - cx, ey = newp[3 * ia + 1] - cy, ez = newp[3 * ia + 2] - cz;
-            acc += go_pair_inv(recip64(ex * ex + ey * ey + ez * ez), r0, a.eps, a.sig2) -
-                   go_pair_inv(recip64(dx * dx + dy * dy + dz * dz), r0, a.eps, a.sig2);
+    if (rigid) {
+        const int m = e - s, nf = n - m;
+        const int S = max(1, (int)blockDim.x / max(nf, 1));
+        for (int idx = threadIdx.x; idx < nf * S; idx += blockDim.x) {
+            int g = idx / nf, r = idx - g * nf;
+            int b = r < s ? r : r + m;   // r-th bead outside [s,e)
+            double bx = pos[3 * b], by = pos[3 * b + 1], bz = pos[3 * b + 2];
+            int p, nextp;
+            native_start(nat, b, s + g, &p, &nextp);
+            for (int ia = s + g; ia < e; ia += S) {
+                double r0 = native_r0(nat, b, ia, &p, &nextp);
+                int d = ia - b;
+                if (d < 3 && d > -3) continue;   // bonded neighbours
+                double dx = pos[3 * ia] - bx, dy = pos[3 * ia + 1] - by, dz = pos[3 * ia + 2] - bz;
+                double ex = newp[3 * ia] - bx, ey = newp[3 * ia + 1] - by, ez = newp[3 * ia + 2] - bz;
+                acc += go_pair_inv(recip64(ex * ex + ey * ey + ez * ez), r0, a.eps, a.sig2) -
+                       go_pair_inv(recip64(dx * dx + dy * dy + dz * dz), r0, a.eps, a.sig2);
+            }
+        }
+    } else {
+        const int S = max(1, (int)blockDim.x / n);
+        for (int idx = threadIdx.x; idx < n * S; idx += blockDim.x) {
+            int g = idx / n, b = idx - g * n;
+            bool bin = b >= s && b < e;
+            double bx = pos[3 * b], by = pos[3 * b + 1], bz = pos[3 * b + 2];
+            double cx = bin ? newp[3 * b] : bx, cy = bin ? newp[3 * b + 1] : by, cz = bin ? newp[3 * b + 2] : bz;
+            int p, nextp;
+            native_start(nat, b, s + g, &p, &nextp);
+            for (int ia = s + g; ia < e; ia += S) {
+                double r0 = native_r0(nat, b, ia, &p, &nextp);
+                int d = ia - b;
+                if (d < 3 && d > -3) continue;            // bonded neighbours and itself
+                if (bin && b < ia) continue;              // pairs inside the moved range once
+                double dx = pos[3 * ia] - bx, dy = pos[3 * ia + 1] - by, dz = pos[3 * ia + 2] - bz;
+                double ex = newp[3 * ia] - cx, ey = newp[3 * ia + 1] - cy, ez = newp[3 * ia + 2] - cz;
+                acc += go_pair_inv(recip64(ex * ex + ey * ey + ez * ez), r0, a.eps, a.sig2) -
+                       go_pair_inv(recip64(dx * dx + dy * dy + dz * dz), r0, a.eps, a.sig2);
+            }
         }
     }
     // bonds (cbond.c:103-104 form) with at least one end in [s,e)
@@ -220,61 +242,78 @@ __device__ double full_energy(const double *pos, int n, const Natives &nat, cons
     return block_sum(acc, red);
 }
 
-// Network weights as held in LDS by the sampling kernel: the layers'
-// matrices transposed (input-major), so that the 64 lanes computing the 64
-// outputs of a layer read consecutive words.
-//   W1T[NF][HID], b1[HID], W2T[HID][HID], b2[HID], W3[HID], b3
+// Network weights as held in LDS by the sampling kernel.  The forward pass
+// runs on all 16 waves: wave q evaluates the move (q < 8) or the reverse move
+// (q >= 8), and thread t = (q % 8) * 64 + lane takes output j = t / 8 of a
+// layer over the inputs i = k, k + 8, ..., k + 56 with k = t % 8.  Each
+// layer's matrix is stored so that thread t finds its 8 weights as two
+// float4 at R[t] and R[512 + t] (words (h * 512 + t) * 4 + m hold input
+// k + 8 * (4 h + m)); inputs past NF carry zero weight.
+//   R1[4096], R2[4096], b1[HID], b2[HID], W3[HID]   (b3 cancels in g(x) - g(swap x))
+static_assert(HID == 64 && NF <= 64 && MC_THREADS == 1024, "network layout assumes 64 hidden units and 16 waves");
+#define NRUN (2 * 4096 + 3 * HID)
+#define FEAT_SWAP 64   // offset of the reverse move's features in the feature buffer
+
 __device__ void mlp_load_lds(const float *g, float *W) {
-    const float *W1 = g, *b1 = W1 + HID * NF, *W2 = b1 + HID, *b2 = W2 + HID * HID, *W3 = b2 + HID, *b3 = W3 + HID;
-    float *W1T = W, *sb1 = W1T + NF * HID, *W2T = sb1 + HID, *sb2 = W2T + HID * HID, *sW3 = sb2 + HID, *sb3 = sW3 + HID;
-    for (int t = threadIdx.x; t < NF * HID; t += blockDim.x) {
-        int i = t / HID, j = t - i * HID;
-        W1T[t] = W1[j * NF + i];
-    }
-    for (int t = threadIdx.x; t < HID * HID; t += blockDim.x) {
-        int i = t / HID, j = t - i * HID;
-        W2T[t] = W2[j * HID + i];
+    const float *W1 = g, *b1 = W1 + HID * NF, *W2 = b1 + HID, *b2 = W2 + HID * HID, *W3 = b2 + HID;
+    float *R1 = W, *R2 = W + 4096, *sb1 = W + 8192, *sb2 = sb1 + HID, *sW3 = sb2 + HID;
+    for (int u = threadIdx.x; u < 4096; u += blockDim.x) {
+        int h = u >> 11, t = (u >> 2) & 511, m = u & 3;
+        int j = t >> 3, i = (t & 7) + 8 * (4 * h + m);
+        R1[u] = i < NF ? W1[j * NF + i] : 0.0f;
+        R2[u] = W2[j * HID + i];
     }
     for (int t = threadIdx.x; t < HID; t += blockDim.x) {
         sb1[t] = b1[t];
         sb2[t] = b2[t];
         sW3[t] = W3[t];
     }
-    if (threadIdx.x == 0) sb3[0] = b3[0];
 }
 
-// network forward of two feature rows (x, swapped x) -> out[2]; weights in
-// LDS as laid out by mlp_load_lds; threads 0..127 = (row, output unit)
-__device__ void mlp2(const float *W, const float *x0, const float *x1, float *h1, float *h2, float *out, float *fred) {
-    const float *W1T = W, *b1 = W1T + NF * HID, *W2T = b1 + HID, *b2 = W2T + HID * HID, *W3 = b2 + HID, *b3 = W3 + HID;
-    int t = threadIdx.x;
-    int r = t >> 6, j = t & 63;
-    if (t < 2 * HID) {
-        const float *x = r ? x1 : x0;
-        float s0 = b1[j], s1 = 0.0f;
+// feature index of the reverse move (old <-> new), as the sampling kernel
+// writes it at FEAT_SWAP
+__device__ __forceinline__ int feat_swap(int i) {
+    return i < 16 ? i + 16 : (i < 32 ? i - 16 : (i < 40 ? i + 8 : (i < 48 ? i - 8 : i)));
+}
+
+__device__ __forceinline__ float dot8(float4 a, float4 b, const float *x, int k) {
+    return a.x * x[k] + a.y * x[k + 8] + a.z * x[k + 16] + a.w * x[k + 24] +
+           b.x * x[k + 32] + b.y * x[k + 40] + b.z * x[k + 48] + b.w * x[k + 56];
+}
+
+// sum over the 8 lanes k = 0..7 of a lane group; every lane gets the same value
+__device__ __forceinline__ float group8_sum(float v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    return v;
+}
+
+// Surrogate g(x) - g(swap x) for the features x = xs[0..NF) and the reverse
+// move's xs[FEAT_SWAP..FEAT_SWAP+NF) (slots NF..63 of both hold zero), with
+// the weights laid out by mlp_load_lds.  Called by all 1024 threads (two
+// barriers inside); every thread returns the same value, summed in a fixed
+// order.  hs: 2*HID floats, pv: MC_WAVES floats.
+__device__ float mlp_block(const float *W, const float *xs, float *hs, float *pv) {
+    const float4 *R1 = (const float4 *)W, *R2 = R1 + 1024;
+    const float *b1 = W + 8192, *b2 = b1 + HID, *W3 = b2 + HID;
+    int q = threadIdx.x >> 6, l = threadIdx.x & 63;
+    int sm = q >> 3, t = ((q & 7) << 6) | l, j = t >> 3, k = t & 7;
+    float acc = group8_sum(dot8(R1[t], R1[512 + t], xs + FEAT_SWAP * sm, k));
+    if (k == 0) hs[HID * sm + j] = tanhf(acc + b1[j]);
+    __syncthreads();
+    acc = group8_sum(dot8(R2[t], R2[512 + t], hs + HID * sm, k));
+    float c = W3[j] * tanhf(acc + b2[j]);
+    // the 8 lanes of a group hold the same c: add one copy of each group
+    c += __shfl_xor(c, 8, 64);
+    c += __shfl_xor(c, 16, 64);
+    c += __shfl_xor(c, 32, 64);
+    if (l == 0) pv[q] = sm ? -c : c;
+    __syncthreads();
+    float g0 = 0.0f, g1 = 0.0f;
 #pragma unroll
-        for (int i = 0; i < NF; i += 2) {
-            s0 += W1T[i * HID + j] * x[i];
-            s1 += W1T[(i + 1) * HID + j] * x[i + 1];
-        }
-        h1[r * HID + j] = tanhf(s0 + s1);
-    }
-    __syncthreads();
-    if (t < 2 * HID) {
-        const float *h = h1 + r * HID;
-        float s0 = b2[j], s1 = 0.0f;
-#pragma unroll 8
-        for (int i = 0; i < HID; i += 2) {
-            s0 += W2T[i * HID + j] * h[i];
-            s1 += W2T[(i + 1) * HID + j] * h[i + 1];
-        }
-        float v = W3[j] * tanhf(s0 + s1);
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if (j == 0) fred[r] = v + b3[0];
-    }
-    __syncthreads();
-    out[0] = fred[0];
-    out[1] = fred[1];
+    for (int r = 0; r < 8; r++) { g0 += pv[r]; g1 += pv[8 + r]; }
+    return g0 + g1;
 }
 
 // u < exp(lr) for u in [0,1): decided in FP32 unless u lies within 1e-5
@@ -307,12 +346,11 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     double *lng = newp + 3 * a.maxn;          // nbins
     long long *hist = (long long *)(lng + a.nbins);
     double *red = (double *)(hist + a.nbins); // MC_WAVES
-    float *W = (float *)(red + MC_WAVES);     // NPAR
-    float *feat = W + (DA ? NPAR : 0);        // 2*NF
-    float *h1 = feat + 2 * NF;                // 2*HID
-    float *h2 = h1 + 2 * HID;
-    float *fred = h2 + 2 * HID;               // 2
-    double *sr0 = (double *)(((uintptr_t)(fred + 2) + 7) & ~(uintptr_t)7);   // max_nnz
+    float *W = (float *)(((uintptr_t)(red + MC_WAVES) + 15) & ~(uintptr_t)15);   // NRUN (16-byte aligned)
+    float *feat = W + (DA ? NRUN : 0);        // 2*FEAT_SWAP: move, reverse move
+    float *hs = feat + 2 * FEAT_SWAP;         // 2*HID
+    float *pv = hs + 2 * HID;                 // MC_WAVES
+    double *sr0 = (double *)(((uintptr_t)(pv + MC_WAVES) + 7) & ~(uintptr_t)7);   // max_nnz
     int *soff = (int *)(sr0 + max_nnz);       // maxn + 1
     int *spart = soff + a.maxn + 1;           // max_nnz
 
@@ -331,6 +369,7 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
         hist[i] = ghist[(size_t)w * a.nbins + i];
     }
     if (DA) mlp_load_lds(gmlp, W);
+    for (int i = threadIdx.x; i < 2 * FEAT_SWAP; i += blockDim.x) feat[i] = 0.0f;   // slots past NF stay zero
     uint64_t s4[4] = {grng[4 * w], grng[4 * w + 1], grng[4 * w + 2], grng[4 * w + 3]};
     __syncthreads();
     double emin = gemin[w];
@@ -401,45 +440,79 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
             if (bx >= 0 && bx < a.nbins) pend = bx;
             continue;
         }
-        // ---- features of pivot moves (local distances around the pivot)
+        // ---- features of pivot moves: local distances around the pivot, and
+        // the closest approach of sampled moved beads to the fixed part
         bool need_feat = type == 1 && (DA || do_log);
         if (need_feat) {
             int sg = (s == 0) ? -1 : 1;     // moved side direction from the pivot
-            if (threadIdx.x < 16) {
-                int i = threadIdx.x / 4 + 1, j = threadIdx.x % 4 + 1;
-                int mb = piv + sg * i, fb = piv - sg * j;
-                float fo = 0.0f, fn = 0.0f;
-                if (mb >= 0 && mb < n && fb >= 0 && fb < n) {
-                    double ox = pos[3 * mb] - pos[3 * fb], oy = pos[3 * mb + 1] - pos[3 * fb + 1], oz = pos[3 * mb + 2] - pos[3 * fb + 2];
-                    double nx = newp[3 * mb] - pos[3 * fb], ny = newp[3 * mb + 1] - pos[3 * fb + 1], nz = newp[3 * mb + 2] - pos[3 * fb + 2];
-                    fo = feat_term(ox * ox + oy * oy + oz * oz, a.sig2);
-                    fn = feat_term(nx * nx + ny * ny + nz * nz, a.sig2);
+            int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+            if (wv < 8) {
+                // wave k < 8 takes sampled bead k: lanes scan the fixed beads,
+                // the minima are reduced in the wave
+                int bk = piv + sg * (2 << wv);
+                float mo = 3.0e38f, mn = 3.0e38f;
+                if (bk >= s && bk < e) {
+                    float bx0 = (float)pos[3 * bk], by0 = (float)pos[3 * bk + 1], bz0 = (float)pos[3 * bk + 2];
+                    float bx1 = (float)newp[3 * bk], by1 = (float)newp[3 * bk + 1], bz1 = (float)newp[3 * bk + 2];
+                    for (int f = ln; f < n; f += 64) {
+                        int d = f - bk;
+                        if ((f >= s && f < e) || (d < 3 && d > -3)) continue;
+                        float fx = (float)pos[3 * f], fy = (float)pos[3 * f + 1], fz = (float)pos[3 * f + 2];
+                        float ox = bx0 - fx, oy = by0 - fy, oz = bz0 - fz;
+                        float nx = bx1 - fx, ny = by1 - fy, nz = bz1 - fz;
+                        mo = fminf(mo, ox * ox + oy * oy + oz * oz);
+                        mn = fminf(mn, nx * nx + ny * ny + nz * nz);
+                    }
                 }
-                feat[threadIdx.x] = bf16_round(fo);
-                feat[16 + threadIdx.x] = bf16_round(fn);
-                feat[NF + threadIdx.x] = bf16_round(fn);        // reverse move: old <-> new
-                feat[NF + 16 + threadIdx.x] = bf16_round(fo);
-            } else if (threadIdx.x < 20) {
-                int q = threadIdx.x - 16;
-                float v = q == 0 ? 1.0f : q == 1 ? (float)(fabs(th) / MC_PI) : q == 2 ? (float)(e - s) / (float)n : 0.0f;
-                v = bf16_round(v);
-                feat[32 + q] = v;
-                feat[NF + 32 + q] = v;
+                for (int o = 32; o > 0; o >>= 1) {
+                    mo = fminf(mo, __shfl_xor(mo, o, 64));
+                    mn = fminf(mn, __shfl_xor(mn, o, 64));
+                }
+                if (ln == 0) {
+                    float fo = bf16_round(mo < 1e37f ? feat_term((double)mo, a.sig2) : 0.0f);
+                    float fnw = bf16_round(mn < 1e37f ? feat_term((double)mn, a.sig2) : 0.0f);
+                    feat[32 + wv] = fo;
+                    feat[40 + wv] = fnw;
+                    feat[FEAT_SWAP + 32 + wv] = fnw;
+                    feat[FEAT_SWAP + 40 + wv] = fo;
+                }
+            } else if (wv == 8) {
+                // local distances of the 4 moved x 4 fixed beads nearest the
+                // pivot, and the move's scalars
+                if (ln < 16) {
+                    int i = ln / 4 + 1, j = ln % 4 + 1;
+                    int mb = piv + sg * i, fb = piv - sg * j;
+                    float v = 0.0f, v2 = 0.0f;
+                    if (mb >= 0 && mb < n && fb >= 0 && fb < n) {
+                        double ox = pos[3 * mb] - pos[3 * fb], oy = pos[3 * mb + 1] - pos[3 * fb + 1], oz = pos[3 * mb + 2] - pos[3 * fb + 2];
+                        double nx = newp[3 * mb] - pos[3 * fb], ny = newp[3 * mb + 1] - pos[3 * fb + 1], nz = newp[3 * mb + 2] - pos[3 * fb + 2];
+                        v = bf16_round(feat_term(ox * ox + oy * oy + oz * oz, a.sig2));
+                        v2 = bf16_round(feat_term(nx * nx + ny * ny + nz * nz, a.sig2));
+                    }
+                    feat[ln] = v;
+                    feat[16 + ln] = v2;
+                    feat[FEAT_SWAP + ln] = v2;
+                    feat[FEAT_SWAP + 16 + ln] = v;
+                } else if (ln < 20) {
+                    int q = ln - 16;
+                    float v = bf16_round(q == 0 ? 1.0f : q == 1 ? (float)(fabs(th) / MC_PI) : q == 2 ? (float)(e - s) / (float)n : 0.0f);
+                    feat[48 + q] = v;
+                    feat[FEAT_SWAP + 48 + q] = v;
+                }
             }
             __syncthreads();
         }
         double sur = 0.0, a1f = 0.0;   // a1f: log of the first-stage probability
         if (DA && type == 1) {
-            float o2[2];
-            mlp2(W, feat, feat + NF, h1, h2, o2, fred);
-            sur = escale * (double)sinhf(o2[0] - o2[1]);   // antisymmetric surrogate dE
-            int by = (int)floor((ecur + sur - emin) / a.ebin);
-            // first stage a1 = min(1, g(E_x)/g(E_x + sur)), as log a1
-            double la1 = -INFINITY;
-            if (bx >= 0 && bx < a.nbins && by >= 0 && by < a.nbins) la1 = fmin(0.0, lng[bx] - lng[by]);
+            // first stage: surrogate dE of the move (antisymmetric network),
+            // a1 = min(1, g(E_x) / g(E_x + sur)) as log a1, decided by every
+            // thread from the same values
             double u1 = xo_uniform(s4);
-            a1f = la1;
-            if (!accept_log(la1, u1)) {
+            float so = sinhf(mlp_block(W, feat, hs, pv));
+            sur = escale * (double)so;
+            int by = (int)floor((ecur + sur - emin) / a.ebin);
+            a1f = (bx >= 0 && bx < a.nbins && by >= 0 && by < a.nbins) ? fmin(0.0, lng[bx] - lng[by]) : -INFINITY;
+            if (!accept_log(a1f, u1)) {
                 st_s1++;
                 if (bx >= 0 && bx < a.nbins) pend = bx;
                 continue;
@@ -656,8 +729,11 @@ k_mlp_forward(const float *P, int nsamp, const float *X, float *Y) {
 
 // Adam over the logged moves; antisymmetric output g(x) - g(swap x)
 __global__ void __launch_bounds__(TR_THREADS)
+// targets asinh(clip(dE)/escale): energy changes beyond declip (twice the
+// Wang-Landau window) are all rejected alike and are clipped so that they do
+// not dominate the loss
 k_mlp_train(float *P, float *adam, int t0, int nsamp, const float *X, const double *DE, int nepoch, float lr,
-            double escale, uint64_t seed, double *loss_out) {
+            double escale, double declip, uint64_t seed, double *loss_out) {
     extern __shared__ float sm[];
     const int R = 2 * TR_B;
     float *Xs = sm;                        // R*NF
@@ -679,7 +755,7 @@ k_mlp_train(float *P, float *adam, int t0, int nsamp, const float *X, const doub
                 int r = q / NF, i = q % NF;
                 const float *x = X + (size_t)idx[r] * NF;
                 // swapped row: old <-> new local terms
-                int si = i < 16 ? i + 16 : (i < 32 ? i - 16 : i);
+                int si = feat_swap(i);
                 Xs[r * NF + i] = bf16_round(x[i]);
                 Xs[(TR_B + r) * NF + i] = bf16_round(x[si]);
             }
@@ -687,7 +763,7 @@ k_mlp_train(float *P, float *adam, int t0, int nsamp, const float *X, const doub
             mlp_fwd_rows(P, Xs, R, H1, H2, out);
             if (threadIdx.x == 0) {
                 for (int r = 0; r < TR_B; r++) {
-                    double y = asinh(DE[idx[r]] / escale);
+                    double y = asinh(fmax(-declip, fmin(declip, DE[idx[r]])) / escale);
                     float d = (out[r] - out[TR_B + r]) - (float)y;
                     lsum += 0.5 * (double)d * d;
                     dout[r] = d / (float)TR_B;
@@ -725,7 +801,7 @@ static McArgs mc_args(const ttc_mc *m) {
 static size_t run_shm(const ttc_mc *m, bool da) {
     size_t d = (size_t)(6 * m->p.maxn + m->p.nbins) * sizeof(double) + (size_t)m->p.nbins * sizeof(long long) +
                MC_WAVES * sizeof(double);
-    size_t f = (size_t)((da ? NPAR : 0) + 2 * NF + 4 * HID + 2) * sizeof(float);
+    size_t f = 16 + (size_t)((da ? NRUN : 0) + 2 * FEAT_SWAP + 2 * HID + MC_WAVES) * sizeof(float);
     // native contacts: r0 (aligned up to 8 bytes), offsets, partners
     size_t nat = 8 + (size_t)m->max_nnz * (sizeof(double) + sizeof(int)) + (size_t)(m->p.maxn + 1) * sizeof(int);
     return d + f + nat;
@@ -1059,7 +1135,8 @@ extern "C" int ttc_mlp_train(ttc_mc *m, int nepoch, int batch, float lr, double 
     double *dl;
     HIPCHK(hipMalloc(&dl, sizeof(double)));
     k_mlp_train<<<1, TR_THREADS, train_shm(), m->stream>>>(m->mlp, m->adam, m->adam_t, n, m->logx, m->logde, nepoch,
-                                                           lr, escale, m->p.seed + (uint64_t)m->adam_t, dl);
+                                                           lr, escale, 2.0 * m->p.nbins * m->p.ebin,
+                                                           m->p.seed + (uint64_t)m->adam_t, dl);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(loss, dl, sizeof(double), hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));

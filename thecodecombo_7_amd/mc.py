@@ -21,7 +21,7 @@ import numpy as np
 from . import _lib as T
 from . import system
 
-NF = 36
+NF = 52
 HID = 64
 NSTAT = 7   # trials, accepted, stage-1 rejects, exact evaluations, outside window, pivots, accepted pivots
 
