@@ -87,11 +87,12 @@ __device__ __forceinline__ float bf16_round(float x) {
     return __uint_as_float(u & 0xFFFF0000u);
 }
 
-// pairs term of the moved-feature transform
-__device__ __forceinline__ float feat_term(double r2, double sig2) {
-    double s = sig2 / r2;
-    double v = s * s * s;
-    return (float)(v < 16.0 ? v : 16.0);
+// pair term of the move features, min((sig/r)^6, 16), in FP32 (the result
+// is rounded to bfloat16)
+__device__ __forceinline__ float feat_term(float r2, float sig2) {
+    float s = sig2 * __builtin_amdgcn_rcpf(r2);
+    float v = s * s * s;
+    return v < 16.0f ? v : 16.0f;
 }
 
 struct McArgs {
@@ -99,12 +100,62 @@ struct McArgs {
     double ebin, kbond, r0bond, eps, sig2, pivot_frac, theta_max, disp_max;
 };
 
-// block-wide FP64 sum in a fixed order
+// Lane exchanges inside rows of 16 lanes by DPP (no LDS round trip): partner
+// lane i^1, i^2 (quad permutations), 7-i within each 8 (half mirror) and
+// 15-i within each 16 (mirror).  All 64 lanes must be active.
+#define DPP_XOR1 0xB1
+#define DPP_XOR2 0x4E
+#define DPP_HALF_MIRROR 0x141
+#define DPP_MIRROR 0x140
+
+template <int CTRL> __device__ __forceinline__ float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float readlane_f32(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Wave-wide sum (and minimum): the partners' values are combined in the
+// same order in every lane (a + b = b + a), rows of 16 first, then the four
+// row totals as ((r0 + r1) + (r2 + r3)); the result is wave-uniform.
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    v += dpp_f64<DPP_XOR1>(v);
+    v += dpp_f64<DPP_XOR2>(v);
+    v += dpp_f64<DPP_HALF_MIRROR>(v);
+    v += dpp_f64<DPP_MIRROR>(v);
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+__device__ __forceinline__ float wave_sum_f32(float v) {
+    v += dpp_f32<DPP_XOR1>(v);
+    v += dpp_f32<DPP_XOR2>(v);
+    v += dpp_f32<DPP_HALF_MIRROR>(v);
+    v += dpp_f32<DPP_MIRROR>(v);
+    return (readlane_f32(v, 0) + readlane_f32(v, 16)) + (readlane_f32(v, 32) + readlane_f32(v, 48));
+}
+__device__ __forceinline__ float wave_min_f32(float v) {
+    v = fminf(v, dpp_f32<DPP_XOR1>(v));
+    v = fminf(v, dpp_f32<DPP_XOR2>(v));
+    v = fminf(v, dpp_f32<DPP_HALF_MIRROR>(v));
+    v = fminf(v, dpp_f32<DPP_MIRROR>(v));
+    return fminf(fminf(readlane_f32(v, 0), readlane_f32(v, 16)), fminf(readlane_f32(v, 32), readlane_f32(v, 48)));
+}
+
+// Block-wide FP64 sum in a fixed order.  red[] is written before the one
+// barrier and read after it: callers separate consecutive calls by a barrier.
 __device__ double block_sum(double v, double *red) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) red[w] = v;
+    v = wave_sum_f64(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     double t = 0.0;
     for (int i = 0; i < MC_WAVES; i++) t += red[i];
@@ -116,6 +167,34 @@ __device__ __forceinline__ double recip64(double x) {
     double r = __builtin_amdgcn_rcp(x);
     r = fma(fma(-x, r, 1.0), r, r);
     return fma(fma(-x, r, 1.0), r, r);
+}
+
+// sin and cos of |x| <= pi/4 from their Taylor series, Horner in x^2 with
+// the coefficients (-1)^k/(2k+1)! and (-1)^k/(2k)! up to x^17 and x^18 (the
+// first omitted terms are below 1e-19 there); larger angles take the library
+// routines.  A few dozen FP64 operations instead of two range-reduced calls.
+__device__ __forceinline__ void sincos_pivot(double x, double *sn, double *c) {
+    if (!(fabs(x) <= 0.78539816339744831)) { *sn = sin(x); *c = cos(x); return; }
+    double x2 = x * x;
+    double ps = -1.0 / 355687428096000.0;
+    ps = fma(ps, x2, 1.0 / 1307674368000.0);
+    ps = fma(ps, x2, -1.0 / 6227020800.0);
+    ps = fma(ps, x2, 1.0 / 39916800.0);
+    ps = fma(ps, x2, -1.0 / 362880.0);
+    ps = fma(ps, x2, 1.0 / 5040.0);
+    ps = fma(ps, x2, -1.0 / 120.0);
+    ps = fma(ps, x2, 1.0 / 6.0);
+    *sn = fma(-x * x2, ps, x);
+    double pc = 1.0 / 6402373705728000.0;
+    pc = fma(pc, x2, -1.0 / 20922789888000.0);
+    pc = fma(pc, x2, 1.0 / 87178291200.0);
+    pc = fma(pc, x2, -1.0 / 479001600.0);
+    pc = fma(pc, x2, 1.0 / 3628800.0);
+    pc = fma(pc, x2, -1.0 / 40320.0);
+    pc = fma(pc, x2, 1.0 / 720.0);
+    pc = fma(pc, x2, -1.0 / 24.0);
+    pc = fma(pc, x2, 0.5);
+    *c = fma(-x2, pc, 1.0);
 }
 
 // Go pair energy from 1/r^2 (cgolik.c:1209-1229 form for natives,
@@ -144,11 +223,16 @@ __device__ __forceinline__ double native_r0(const Natives &nat, int b, int ia, i
     return *nextp == ia ? nat.r0[*p] : 0.0;
 }
 
+// first entry of b's partner list with partner >= ia0 (binary search)
 __device__ __forceinline__ void native_start(const Natives &nat, int b, int ia0, int *p, int *nextp) {
-    int q = nat.off[b], pe = nat.off[b + 1];
-    while (q < pe && nat.part[q] < ia0) q++;
-    *p = q;
-    *nextp = q < pe ? nat.part[q] : 0x7FFFFFFF;
+    int lo = nat.off[b], pe = nat.off[b + 1], hi = pe;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (nat.part[mid] < ia0) lo = mid + 1;
+        else hi = mid;
+    }
+    *p = lo;
+    *nextp = lo < pe ? nat.part[lo] : 0x7FFFFFFF;
 }
 
 // Energy change of walker when beads [s,e) take newp[], all other beads fixed:
@@ -243,12 +327,12 @@ __device__ double full_energy(const double *pos, int n, const Natives &nat, cons
 }
 
 // Network weights as held in LDS by the sampling kernel.  The forward pass
-// runs on all 16 waves: wave q evaluates the move (q < 8) or the reverse move
-// (q >= 8), and thread t = (q % 8) * 64 + lane takes output j = t / 8 of a
-// layer over the inputs i = k, k + 8, ..., k + 56 with k = t % 8.  Each
-// layer's matrix is stored so that thread t finds its 8 weights as two
-// float4 at R[t] and R[512 + t] (words (h * 512 + t) * 4 + m hold input
-// k + 8 * (4 h + m)); inputs past NF carry zero weight.
+// runs on waves 0-3 (one per SIMD): wave q evaluates the move (q < 2) or the
+// reverse move (q >= 2), and its thread t = (q % 2) * 64 + lane takes output
+// j = t / 2 of a layer over the inputs i = 32 k .. 32 k + 31, k = t % 2.
+// Each layer's matrix is stored so that thread t finds its 32 weights as
+// eight float4 at R[m4 * 128 + t] (words (m4 * 128 + t) * 4 + m hold input
+// 32 k + 4 m4 + m); inputs past NF carry zero weight.
 //   R1[4096], R2[4096], b1[HID], b2[HID], W3[HID]   (b3 cancels in g(x) - g(swap x))
 static_assert(HID == 64 && NF <= 64 && MC_THREADS == 1024, "network layout assumes 64 hidden units and 16 waves");
 #define NRUN (2 * 4096 + 3 * HID)
@@ -258,8 +342,8 @@ __device__ void mlp_load_lds(const float *g, float *W) {
     const float *W1 = g, *b1 = W1 + HID * NF, *W2 = b1 + HID, *b2 = W2 + HID * HID, *W3 = b2 + HID;
     float *R1 = W, *R2 = W + 4096, *sb1 = W + 8192, *sb2 = sb1 + HID, *sW3 = sb2 + HID;
     for (int u = threadIdx.x; u < 4096; u += blockDim.x) {
-        int h = u >> 11, t = (u >> 2) & 511, m = u & 3;
-        int j = t >> 3, i = (t & 7) + 8 * (4 * h + m);
+        int m4 = u >> 9, t = (u >> 2) & 127, m = u & 3;
+        int j = t >> 1, i = 32 * (t & 1) + 4 * m4 + m;
         R1[u] = i < NF ? W1[j * NF + i] : 0.0f;
         R2[u] = W2[j * HID + i];
     }
@@ -276,44 +360,46 @@ __device__ __forceinline__ int feat_swap(int i) {
     return i < 16 ? i + 16 : (i < 32 ? i - 16 : (i < 40 ? i + 8 : (i < 48 ? i - 8 : i)));
 }
 
-__device__ __forceinline__ float dot8(float4 a, float4 b, const float *x, int k) {
-    return a.x * x[k] + a.y * x[k + 8] + a.z * x[k + 16] + a.w * x[k + 24] +
-           b.x * x[k + 32] + b.y * x[k + 40] + b.z * x[k + 48] + b.w * x[k + 56];
-}
-
-// sum over the 8 lanes k = 0..7 of a lane group; every lane gets the same value
-__device__ __forceinline__ float group8_sum(float v) {
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    v += __shfl_xor(v, 4, 64);
-    return v;
+// 32-term dot product of thread t's weights with inputs x[0..32)
+__device__ __forceinline__ float dot32(const float4 *R, int t, const float4 *x) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int m4 = 0; m4 < 8; m4++) {
+        float4 w = R[m4 * 128 + t], v = x[m4];
+        acc = fmaf(w.x, v.x, acc);
+        acc = fmaf(w.y, v.y, acc);
+        acc = fmaf(w.z, v.z, acc);
+        acc = fmaf(w.w, v.w, acc);
+    }
+    return acc;
 }
 
 // Surrogate g(x) - g(swap x) for the features x = xs[0..NF) and the reverse
 // move's xs[FEAT_SWAP..FEAT_SWAP+NF) (slots NF..63 of both hold zero), with
 // the weights laid out by mlp_load_lds.  Called by all 1024 threads (two
-// barriers inside); every thread returns the same value, summed in a fixed
-// order.  hs: 2*HID floats, pv: MC_WAVES floats.
+// barriers inside; waves 4-15 only wait); every thread returns the same
+// value, summed in a fixed order.  xs and hs 16-byte aligned; hs: 2*HID
+// floats, pv: 4 floats.
 __device__ float mlp_block(const float *W, const float *xs, float *hs, float *pv) {
     const float4 *R1 = (const float4 *)W, *R2 = R1 + 1024;
     const float *b1 = W + 8192, *b2 = b1 + HID, *W3 = b2 + HID;
     int q = threadIdx.x >> 6, l = threadIdx.x & 63;
-    int sm = q >> 3, t = ((q & 7) << 6) | l, j = t >> 3, k = t & 7;
-    float acc = group8_sum(dot8(R1[t], R1[512 + t], xs + FEAT_SWAP * sm, k));
-    if (k == 0) hs[HID * sm + j] = tanhf(acc + b1[j]);
+    int sm = q >> 1, t = ((q & 1) << 6) | l, j = t >> 1, k = t & 1;
+    if (q < 4) {
+        float acc = dot32(R1, t, (const float4 *)(xs + FEAT_SWAP * sm) + 8 * k);
+        acc += dpp_f32<DPP_XOR1>(acc);
+        if (k == 0) hs[HID * sm + j] = tanhf(acc + b1[j]);
+    }
     __syncthreads();
-    acc = group8_sum(dot8(R2[t], R2[512 + t], hs + HID * sm, k));
-    float c = W3[j] * tanhf(acc + b2[j]);
-    // the 8 lanes of a group hold the same c: add one copy of each group
-    c += __shfl_xor(c, 8, 64);
-    c += __shfl_xor(c, 16, 64);
-    c += __shfl_xor(c, 32, 64);
-    if (l == 0) pv[q] = sm ? -c : c;
+    if (q < 4) {
+        float acc = dot32(R2, t, (const float4 *)(hs + HID * sm) + 8 * k);
+        acc += dpp_f32<DPP_XOR1>(acc);
+        float c = k == 0 ? W3[j] * tanhf(acc + b2[j]) : 0.0f;
+        c = wave_sum_f32(c);
+        if (l == 0) pv[q] = sm ? -c : c;
+    }
     __syncthreads();
-    float g0 = 0.0f, g1 = 0.0f;
-#pragma unroll
-    for (int r = 0; r < 8; r++) { g0 += pv[r]; g1 += pv[8 + r]; }
-    return g0 + g1;
+    return (pv[0] + pv[1]) + (pv[2] + pv[3]);
 }
 
 // u < exp(lr) for u in [0,1): decided in FP32 unless u lies within 1e-5
@@ -328,10 +414,22 @@ __device__ __forceinline__ bool accept_log(double lr, double u) {
     return u < exp(lr);
 }
 
-// One workgroup per walker runs all its trials.  Every thread keeps an
-// identical copy of the walker's random state and draws the same proposal
-// and acceptance numbers, so no broadcast is needed; the moved coordinates
-// and the exact energy change are computed cooperatively.
+// Random numbers of the sampling kernel: MC_RPT uniforms per trial (move
+// type, four or five move parameters, first- and second-stage acceptance),
+// drawn for MC_RB trials at a time by wave 0, lane l from its own xoshiro256**
+// stream l of the walker, into LDS as u[k * MC_RB + trial % MC_RB].
+#define MC_STREAMS 64
+#define MC_RB 64
+#define MC_RPT 8
+__device__ __forceinline__ void draw_batch(uint64_t *st, double *dst) {
+    int l = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < MC_RPT; k++) dst[k * MC_RB + l] = xo_uniform(st);
+}
+
+// One workgroup per walker runs all its trials.  Every thread reads the same
+// proposal and acceptance numbers from LDS and takes the same decisions; the
+// moved coordinates and the energy changes are computed cooperatively.
 template <bool DA>
 __global__ void __launch_bounds__(MC_THREADS)
 k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int *gn, const int *gnoff,
@@ -346,15 +444,26 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     double *lng = newp + 3 * a.maxn;          // nbins
     long long *hist = (long long *)(lng + a.nbins);
     double *red = (double *)(hist + a.nbins); // MC_WAVES
-    float *W = (float *)(((uintptr_t)(red + MC_WAVES) + 15) & ~(uintptr_t)15);   // NRUN (16-byte aligned)
+    // (offsets are aligned by arithmetic on the LDS base pointer: a round
+    // trip through an integer would make every access a flat one)
+    size_t wofs = (((char *)(red + MC_WAVES) - (char *)smem) + 15) & ~(size_t)15;
+    float *W = (float *)((char *)smem + wofs);   // NRUN (16-byte aligned)
     float *feat = W + (DA ? NRUN : 0);        // 2*FEAT_SWAP: move, reverse move
     float *hs = feat + 2 * FEAT_SWAP;         // 2*HID
     float *pv = hs + 2 * HID;                 // MC_WAVES
-    double *sr0 = (double *)(((uintptr_t)(pv + MC_WAVES) + 7) & ~(uintptr_t)7);   // max_nnz
+    size_t rofs = (((char *)(pv + MC_WAVES) - (char *)smem) + 7) & ~(size_t)7;
+    double *sr0 = (double *)((char *)smem + rofs);   // max_nnz
     int *soff = (int *)(sr0 + max_nnz);       // maxn + 1
     int *spart = soff + a.maxn + 1;           // max_nnz
+    size_t uofs = (((char *)(spart + max_nnz) - (char *)smem) + 15) & ~(size_t)15;
+    double *rnd = (double *)((char *)smem + uofs);   // 2 * MC_RPT * MC_RB (two batches)
+    float4 *posf = (float4 *)(rnd + 2 * MC_RPT * MC_RB);   // maxn: FP32 copy of pos for the features
 
     for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) pos[i] = gpos[(size_t)w * 3 * a.maxn + i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double *g = gpos + (size_t)w * 3 * a.maxn + 3 * i;
+        posf[i] = make_float4((float)g[0], (float)g[1], (float)g[2], 0.0f);
+    }
     for (int i = threadIdx.x; i <= n; i += blockDim.x) soff[i] = gnoff[(size_t)w * (a.maxn + 1) + i];
     {
         int nnz = gnoff[(size_t)w * (a.maxn + 1) + n];
@@ -370,10 +479,16 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     }
     if (DA) mlp_load_lds(gmlp, W);
     for (int i = threadIdx.x; i < 2 * FEAT_SWAP; i += blockDim.x) feat[i] = 0.0f;   // slots past NF stay zero
-    uint64_t s4[4] = {grng[4 * w], grng[4 * w + 1], grng[4 * w + 2], grng[4 * w + 3]};
+    uint64_t s4[4] = {0, 0, 0, 0};   // wave 0: lane's random stream
+    if (threadIdx.x < MC_STREAMS) {
+        const uint64_t *g = grng + ((size_t)w * MC_STREAMS + threadIdx.x) * 4;
+        s4[0] = g[0]; s4[1] = g[1]; s4[2] = g[2]; s4[3] = g[3];
+        draw_batch(s4, rnd);
+    }
     __syncthreads();
     double emin = gemin[w];
     double ecur = gecur[w];
+    int bx = (int)floor((ecur - emin) / a.ebin);   // bin of the current energy
     long long st_tr = 0, st_acc = 0, st_s1 = 0, st_ex = 0, st_out = 0, st_piv = 0, st_pacc = 0;
     // Wang-Landau update of the bin a trial ended in (pivot.c:236-253):
     // every thread reads ln g for its own (identical) acceptance decision, so
@@ -382,16 +497,16 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     int pend = -1;
 
     for (int trial = 0; trial < ntrials; trial++) {
+        const double *ur = rnd + ((trial / MC_RB) & 1) * (MC_RPT * MC_RB) + trial % MC_RB;   // ur[k * MC_RB]
         // ---- proposal (mcmove_pivot.c:54-140, mcmove_displ.c:45-99)
         int s, e, type, piv = 0;
         double th = 0.0;
-        double u = xo_uniform(s4);
-        if (u < a.pivot_frac && n > 2) {
-            int p = (int)(xo_uniform(s4) * n);
-            int axis = (int)(xo_uniform(s4) * 3.0);
-            int side = (int)(xo_uniform(s4) * 2.0);
-            int dir = (int)(xo_uniform(s4) * 2.0);
-            double ang = xo_uniform(s4) * MC_PI * a.theta_max;
+        if (ur[0] < a.pivot_frac && n > 2) {
+            int p = __builtin_amdgcn_readfirstlane((int)(ur[MC_RB] * n));
+            int axis = __builtin_amdgcn_readfirstlane((int)(ur[2 * MC_RB] * 3.0));
+            int side = __builtin_amdgcn_readfirstlane((int)(ur[3 * MC_RB] * 2.0));
+            int dir = __builtin_amdgcn_readfirstlane((int)(ur[4 * MC_RB] * 2.0));
+            double ang = ur[5 * MC_RB] * MC_PI * a.theta_max;
             type = 1;
             s = side == 0 ? 0 : p + 1;
             e = side == 0 ? p : n;
@@ -402,7 +517,8 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
             // axis (hipcc 7.2 at -O3 mis-assembled the z matrix when it was
             // built in a zero-initialised 3x3 array and applied generically)
             if ((int)threadIdx.x < e - s) {
-                double c = cos(th), sn = sin(th);
+                double c, sn;
+                sincos_pivot(th, &sn, &c);
                 double px = pos[3 * piv], py = pos[3 * piv + 1], pz = pos[3 * piv + 2];
                 for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
                     double x = pos[3 * i] - px, y = pos[3 * i + 1] - py, z = pos[3 * i + 2] - pz;
@@ -416,26 +532,28 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                 }
             }
         } else {
-            int k = (int)(xo_uniform(s4) * n);
+            int k = __builtin_amdgcn_readfirstlane((int)(ur[MC_RB] * n));
             type = 0;
             s = k;
             e = k + 1;
-            double d0 = (xo_uniform(s4) - 0.5) * a.disp_max;
-            double d1 = (xo_uniform(s4) - 0.5) * a.disp_max;
-            double d2 = (xo_uniform(s4) - 0.5) * a.disp_max;
             if (threadIdx.x == 0) {
+                double d0 = (ur[2 * MC_RB] - 0.5) * a.disp_max;
+                double d1 = (ur[3 * MC_RB] - 0.5) * a.disp_max;
+                double d2 = (ur[4 * MC_RB] - 0.5) * a.disp_max;
                 newp[3 * k] = pos[3 * k] + d0;
                 newp[3 * k + 1] = pos[3 * k + 1] + d1;
                 newp[3 * k + 2] = pos[3 * k + 2] + d2;
             }
         }
         __syncthreads();
+        // every thread has read the previous batch's numbers: draw the next one
+        if (trial % MC_RB == 0 && threadIdx.x < MC_STREAMS)
+            draw_batch(s4, rnd + ((trial / MC_RB + 1) & 1) * (MC_RPT * MC_RB));
         // the previous trial's bin; every read of ln g below follows another barrier
         if (threadIdx.x == 0 && pend >= 0) { lng[pend] += lnf; hist[pend]++; }
         pend = -1;
         st_tr++;
         st_piv += type;
-        int bx = (int)floor((ecur - emin) / a.ebin);
         if (e <= s) {   // empty moved part: nothing changes, counts as a rejected trial
             if (bx >= 0 && bx < a.nbins) pend = bx;
             continue;
@@ -452,25 +570,23 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                 int bk = piv + sg * (2 << wv);
                 float mo = 3.0e38f, mn = 3.0e38f;
                 if (bk >= s && bk < e) {
-                    float bx0 = (float)pos[3 * bk], by0 = (float)pos[3 * bk + 1], bz0 = (float)pos[3 * bk + 2];
+                    float4 b0 = posf[bk];
                     float bx1 = (float)newp[3 * bk], by1 = (float)newp[3 * bk + 1], bz1 = (float)newp[3 * bk + 2];
                     for (int f = ln; f < n; f += 64) {
                         int d = f - bk;
                         if ((f >= s && f < e) || (d < 3 && d > -3)) continue;
-                        float fx = (float)pos[3 * f], fy = (float)pos[3 * f + 1], fz = (float)pos[3 * f + 2];
-                        float ox = bx0 - fx, oy = by0 - fy, oz = bz0 - fz;
-                        float nx = bx1 - fx, ny = by1 - fy, nz = bz1 - fz;
+                        float4 pf = posf[f];
+                        float ox = b0.x - pf.x, oy = b0.y - pf.y, oz = b0.z - pf.z;
+                        float nx = bx1 - pf.x, ny = by1 - pf.y, nz = bz1 - pf.z;
                         mo = fminf(mo, ox * ox + oy * oy + oz * oz);
                         mn = fminf(mn, nx * nx + ny * ny + nz * nz);
                     }
                 }
-                for (int o = 32; o > 0; o >>= 1) {
-                    mo = fminf(mo, __shfl_xor(mo, o, 64));
-                    mn = fminf(mn, __shfl_xor(mn, o, 64));
-                }
+                mo = wave_min_f32(mo);
+                mn = wave_min_f32(mn);
                 if (ln == 0) {
-                    float fo = bf16_round(mo < 1e37f ? feat_term((double)mo, a.sig2) : 0.0f);
-                    float fnw = bf16_round(mn < 1e37f ? feat_term((double)mn, a.sig2) : 0.0f);
+                    float fo = bf16_round(mo < 1e37f ? feat_term(mo, (float)a.sig2) : 0.0f);
+                    float fnw = bf16_round(mn < 1e37f ? feat_term(mn, (float)a.sig2) : 0.0f);
                     feat[32 + wv] = fo;
                     feat[40 + wv] = fnw;
                     feat[FEAT_SWAP + 32 + wv] = fnw;
@@ -484,10 +600,11 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                     int mb = piv + sg * i, fb = piv - sg * j;
                     float v = 0.0f, v2 = 0.0f;
                     if (mb >= 0 && mb < n && fb >= 0 && fb < n) {
-                        double ox = pos[3 * mb] - pos[3 * fb], oy = pos[3 * mb + 1] - pos[3 * fb + 1], oz = pos[3 * mb + 2] - pos[3 * fb + 2];
-                        double nx = newp[3 * mb] - pos[3 * fb], ny = newp[3 * mb + 1] - pos[3 * fb + 1], nz = newp[3 * mb + 2] - pos[3 * fb + 2];
-                        v = bf16_round(feat_term(ox * ox + oy * oy + oz * oz, a.sig2));
-                        v2 = bf16_round(feat_term(nx * nx + ny * ny + nz * nz, a.sig2));
+                        float4 pm = posf[mb], pf = posf[fb];
+                        float ox = pm.x - pf.x, oy = pm.y - pf.y, oz = pm.z - pf.z;
+                        float nx = (float)newp[3 * mb] - pf.x, ny = (float)newp[3 * mb + 1] - pf.y, nz = (float)newp[3 * mb + 2] - pf.z;
+                        v = bf16_round(feat_term(ox * ox + oy * oy + oz * oz, (float)a.sig2));
+                        v2 = bf16_round(feat_term(nx * nx + ny * ny + nz * nz, (float)a.sig2));
                     }
                     feat[ln] = v;
                     feat[16 + ln] = v2;
@@ -507,7 +624,7 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
             // first stage: surrogate dE of the move (antisymmetric network),
             // a1 = min(1, g(E_x) / g(E_x + sur)) as log a1, decided by every
             // thread from the same values
-            double u1 = xo_uniform(s4);
+            double u1 = ur[6 * MC_RB];
             float so = sinhf(mlp_block(W, feat, hs, pv));
             sur = escale * (double)so;
             int by = (int)floor((ecur + sur - emin) / a.ebin);
@@ -535,8 +652,7 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                 if (bz >= 0 && bz < a.nbins) la1r = fmin(0.0, lng[by] - lng[bz]);
                 lratio = lratio + la1r - a1f;
             }
-            double u2 = xo_uniform(s4);
-            acc = accept_log(lratio, u2);
+            acc = accept_log(lratio, ur[7 * MC_RB]);
         }
         if (threadIdx.x == 0 && do_log && type == 1) {
             int slot = atomicAdd(logn, 1);
@@ -547,15 +663,17 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
         }
         if (acc) {
             for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
-                pos[3 * i] = newp[3 * i]; pos[3 * i + 1] = newp[3 * i + 1]; pos[3 * i + 2] = newp[3 * i + 2];
+                double x = newp[3 * i], y = newp[3 * i + 1], z = newp[3 * i + 2];
+                pos[3 * i] = x; pos[3 * i + 1] = y; pos[3 * i + 2] = z;
+                posf[i] = make_float4((float)x, (float)y, (float)z, 0.0f);
             }
             ecur += de;
+            bx = by;   // floor((ecur - emin) / ebin) of the new ecur
             st_acc++;
             st_pacc += type;
             __syncthreads();   // the next proposal reads the moved beads
         }
-        int b = (int)floor((ecur - emin) / a.ebin);
-        if (b >= 0 && b < a.nbins) pend = b;
+        if (bx >= 0 && bx < a.nbins) pend = bx;
     }
     __syncthreads();
     if (threadIdx.x == 0 && pend >= 0) { lng[pend] += lnf; hist[pend]++; }
@@ -565,8 +683,11 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
         glng[(size_t)w * a.nbins + i] = lng[i];
         ghist[(size_t)w * a.nbins + i] = hist[i];
     }
+    if (threadIdx.x < MC_STREAMS) {
+        uint64_t *g = grng + ((size_t)w * MC_STREAMS + threadIdx.x) * 4;
+        g[0] = s4[0]; g[1] = s4[1]; g[2] = s4[2]; g[3] = s4[3];
+    }
     if (threadIdx.x == 0) {
-        for (int q = 0; q < 4; q++) grng[4 * w + q] = s4[q];
         gecur[w] = ecur;
         long long *st = gstats + (size_t)w * TTC_MC_NSTAT;
         st[0] += st_tr; st[1] += st_acc; st[2] += st_s1; st[3] += st_ex; st[4] += st_out;
@@ -804,7 +925,9 @@ static size_t run_shm(const ttc_mc *m, bool da) {
     size_t f = 16 + (size_t)((da ? NRUN : 0) + 2 * FEAT_SWAP + 2 * HID + MC_WAVES) * sizeof(float);
     // native contacts: r0 (aligned up to 8 bytes), offsets, partners
     size_t nat = 8 + (size_t)m->max_nnz * (sizeof(double) + sizeof(int)) + (size_t)(m->p.maxn + 1) * sizeof(int);
-    return d + f + nat;
+    // random-number batches (aligned up to 16 bytes) and the FP32 bead copy
+    size_t rn = 16 + (size_t)2 * MC_RPT * MC_RB * sizeof(double) + (size_t)m->p.maxn * 4 * sizeof(float);
+    return d + f + nat + rn;
 }
 
 // LDS of the sampling kernel on gfx950 (160 KB per workgroup)
@@ -855,7 +978,7 @@ extern "C" int ttc_mc_create(const ttc_mc_params *p, ttc_mc **out) {
     HIPCHK(hipMalloc(&m->ecur, sizeof(double) * W));
     HIPCHK(hipMalloc(&m->lng, sizeof(double) * W * p->nbins));
     HIPCHK(hipMalloc(&m->hist, sizeof(long long) * W * p->nbins));
-    HIPCHK(hipMalloc(&m->rng, sizeof(uint64_t) * W * 4));
+    HIPCHK(hipMalloc(&m->rng, sizeof(uint64_t) * W * MC_STREAMS * 4));
     HIPCHK(hipMalloc(&m->stats, sizeof(long long) * W * TTC_MC_NSTAT));
     HIPCHK(hipMalloc(&m->mlp, sizeof(float) * NPAR));
     HIPCHK(hipMalloc(&m->adam, sizeof(float) * 2 * NPAR));
@@ -871,10 +994,10 @@ extern "C" int ttc_mc_create(const ttc_mc_params *p, ttc_mc **out) {
     uint64_t sx = p->seed ^ 0x5DEECE66Dull;
     for (int i = 0; i < NPAR; i++) w0[i] = (float)(((splitmix(sx) >> 11) * (1.0 / 9007199254740992.0) - 0.5) * 0.3);
     HIPCHK(hipMemcpy(m->mlp, w0.data(), sizeof(float) * NPAR, hipMemcpyHostToDevice));
-    std::vector<uint64_t> rs(W * 4);
+    std::vector<uint64_t> rs((size_t)W * MC_STREAMS * 4);
     uint64_t sd = p->seed;
     for (auto &v : rs) v = splitmix(sd);
-    HIPCHK(hipMemcpy(m->rng, rs.data(), sizeof(uint64_t) * W * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m->rng, rs.data(), sizeof(uint64_t) * rs.size(), hipMemcpyHostToDevice));
     m->logcap = 1 << 20;
     HIPCHK(hipMalloc(&m->logx, sizeof(float) * (size_t)m->logcap * NF));
     HIPCHK(hipMalloc(&m->logde, sizeof(double) * m->logcap));
