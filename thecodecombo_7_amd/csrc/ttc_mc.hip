@@ -170,30 +170,26 @@ __device__ __forceinline__ double recip64(double x) {
 }
 
 // sin and cos of |x| <= pi/4 from their Taylor series, Horner in x^2 with
-// the coefficients (-1)^k/(2k+1)! and (-1)^k/(2k)! up to x^17 and x^18 (the
-// first omitted terms are below 1e-19 there); larger angles take the library
-// routines.  A few dozen FP64 operations instead of two range-reduced calls.
-__device__ __forceinline__ void sincos_pivot(double x, double *sn, double *c) {
+// the coefficients (-1)^k/(2k+1)! (sine, k = 8..1) and (-1)^k/(2k)! (cosine,
+// k = 9..1) up to x^17 and x^18 (the first omitted terms are below 1e-19
+// there); larger angles take the library routines.  The table is read from
+// memory (the sampling kernel keeps it in LDS) so that the compiler does not
+// pin 17 FP64 constants in registers across the trial loop.
+__constant__ double kSinCosTaylor[17] = {
+    -1.0 / 355687428096000.0, 1.0 / 1307674368000.0, -1.0 / 6227020800.0, 1.0 / 39916800.0,
+    -1.0 / 362880.0, 1.0 / 5040.0, -1.0 / 120.0, 1.0 / 6.0,
+    1.0 / 6402373705728000.0, -1.0 / 20922789888000.0, 1.0 / 87178291200.0, -1.0 / 479001600.0,
+    1.0 / 3628800.0, -1.0 / 40320.0, 1.0 / 720.0, -1.0 / 24.0, 0.5};
+__device__ __forceinline__ void sincos_pivot(double x, const double *tc, double *sn, double *c) {
     if (!(fabs(x) <= 0.78539816339744831)) { *sn = sin(x); *c = cos(x); return; }
     double x2 = x * x;
-    double ps = -1.0 / 355687428096000.0;
-    ps = fma(ps, x2, 1.0 / 1307674368000.0);
-    ps = fma(ps, x2, -1.0 / 6227020800.0);
-    ps = fma(ps, x2, 1.0 / 39916800.0);
-    ps = fma(ps, x2, -1.0 / 362880.0);
-    ps = fma(ps, x2, 1.0 / 5040.0);
-    ps = fma(ps, x2, -1.0 / 120.0);
-    ps = fma(ps, x2, 1.0 / 6.0);
+    double ps = tc[0];
+#pragma unroll
+    for (int k = 1; k < 8; k++) ps = fma(ps, x2, tc[k]);
     *sn = fma(-x * x2, ps, x);
-    double pc = 1.0 / 6402373705728000.0;
-    pc = fma(pc, x2, -1.0 / 20922789888000.0);
-    pc = fma(pc, x2, 1.0 / 87178291200.0);
-    pc = fma(pc, x2, -1.0 / 479001600.0);
-    pc = fma(pc, x2, 1.0 / 3628800.0);
-    pc = fma(pc, x2, -1.0 / 40320.0);
-    pc = fma(pc, x2, 1.0 / 720.0);
-    pc = fma(pc, x2, -1.0 / 24.0);
-    pc = fma(pc, x2, 0.5);
+    double pc = tc[8];
+#pragma unroll
+    for (int k = 9; k < 17; k++) pc = fma(pc, x2, tc[k]);
     *c = fma(-x2, pc, 1.0);
 }
 
@@ -244,8 +240,8 @@ __device__ __forceinline__ void native_start(const Natives &nat, int b, int ia0,
 // rigid: [s,e) moved as a rigid body (pivot rotation), so pairs inside it
 // keep their distances and their energy change is zero: only the fixed beads
 // own pairs, and the workgroup is split over them alone.
-__device__ double range_delta(const double *pos, const double *newp, int n, int s, int e, const Natives &nat,
-                              const McArgs &a, double *red, bool rigid = false) {
+__device__ double range_delta_part(const double *pos, const double *newp, int n, int s, int e, const Natives &nat,
+                                   const McArgs &a, bool rigid) {
     double acc = 0.0;
     if (rigid) {
         const int m = e - s, nf = n - m;
@@ -301,7 +297,12 @@ __device__ double range_delta(const double *pos, const double *newp, int n, int 
         double rn = sqrt(qx * qx + qy * qy + qz * qz) - a.r0bond;
         acc += a.kbond * rn * rn - a.kbond * ro * ro;
     }
-    return block_sum(acc, red);
+    return acc;   // this thread's share
+}
+
+__device__ double range_delta(const double *pos, const double *newp, int n, int s, int e, const Natives &nat,
+                              const McArgs &a, double *red, bool rigid = false) {
+    return block_sum(range_delta_part(pos, newp, n, s, e, nat, a, rigid), red);
 }
 
 __device__ double full_energy(const double *pos, int n, const Natives &nat, const McArgs &a, double *red) {
@@ -360,16 +361,29 @@ __device__ __forceinline__ int feat_swap(int i) {
     return i < 16 ? i + 16 : (i < 32 ? i - 16 : (i < 40 ? i + 8 : (i < 48 ? i - 8 : i)));
 }
 
-// 32-term dot product of thread t's weights with inputs x[0..32)
+// 32-term dot product of thread t's weights with inputs x[0..32).  The LDS
+// reads of each half are issued together (one round trip per half instead of
+// one per float4: the scheduler otherwise serialises them at this kernel's
+// register budget).
 __device__ __forceinline__ float dot32(const float4 *R, int t, const float4 *x) {
     float acc = 0.0f;
 #pragma unroll
-    for (int m4 = 0; m4 < 8; m4++) {
-        float4 w = R[m4 * 128 + t], v = x[m4];
-        acc = fmaf(w.x, v.x, acc);
-        acc = fmaf(w.y, v.y, acc);
-        acc = fmaf(w.z, v.z, acc);
-        acc = fmaf(w.w, v.w, acc);
+    for (int h = 0; h < 2; h++) {
+        float4 w[4], v[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            w[m] = R[(4 * h + m) * 128 + t];
+            v[m] = x[4 * h + m];
+        }
+        __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);   // 8 DS reads first
+        __builtin_amdgcn_sched_group_barrier(0x002, 16, 0);  // then the 16 FMAs
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            acc = fmaf(w[m].x, v[m].x, acc);
+            acc = fmaf(w[m].y, v[m].y, acc);
+            acc = fmaf(w[m].z, v[m].z, acc);
+            acc = fmaf(w[m].w, v[m].w, acc);
+        }
     }
     return acc;
 }
@@ -377,10 +391,10 @@ __device__ __forceinline__ float dot32(const float4 *R, int t, const float4 *x) 
 // Surrogate g(x) - g(swap x) for the features x = xs[0..NF) and the reverse
 // move's xs[FEAT_SWAP..FEAT_SWAP+NF) (slots NF..63 of both hold zero), with
 // the weights laid out by mlp_load_lds.  Called by all 1024 threads (two
-// barriers inside; waves 4-15 only wait); every thread returns the same
-// value, summed in a fixed order.  xs and hs 16-byte aligned; hs: 2*HID
-// floats, pv: 4 floats.
-__device__ float mlp_block(const float *W, const float *xs, float *hs, float *pv) {
+// barriers inside; waves 4-15 only wait); leaves four partial sums in pv,
+// g = (pv[0] + pv[1]) + (pv[2] + pv[3]).  xs and hs 16-byte aligned; hs:
+// 2*HID floats.
+__device__ void mlp_block(const float *W, const float *xs, float *hs, float *pv) {
     const float4 *R1 = (const float4 *)W, *R2 = R1 + 1024;
     const float *b1 = W + 8192, *b2 = b1 + HID, *W3 = b2 + HID;
     int q = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -399,7 +413,6 @@ __device__ float mlp_block(const float *W, const float *xs, float *hs, float *pv
         if (l == 0) pv[q] = sm ? -c : c;
     }
     __syncthreads();
-    return (pv[0] + pv[1]) + (pv[2] + pv[3]);
 }
 
 // u < exp(lr) for u in [0,1): decided in FP32 unless u lies within 1e-5
@@ -414,22 +427,87 @@ __device__ __forceinline__ bool accept_log(double lr, double u) {
     return u < exp(lr);
 }
 
-// Random numbers of the sampling kernel: MC_RPT uniforms per trial (move
-// type, four or five move parameters, first- and second-stage acceptance),
-// drawn for MC_RB trials at a time by wave 0, lane l from its own xoshiro256**
-// stream l of the walker, into LDS as u[k * MC_RB + trial % MC_RB].
+// Proposals of the sampling kernel, MC_RB trials at a time: wave 0, lane l
+// draws MC_RPT uniforms from its own xoshiro256** stream l of the walker
+// (state srng[q * 64 + l], q < 4, in LDS) and turns them into trial l's move
+// (mcmove_pivot.c:54-140: pivot about a random bead and axis, one side of the
+// chain, angle u * pi * theta_max of random sign; mcmove_displ.c:45-99: one
+// bead displaced by (u - 0.5) * disp_max per axis) in a table in LDS:
+//   doubles tb[k * MC_RB + l]: first- and second-stage acceptance numbers,
+//           signed angle, its cos and sin, displacement x, y, z
+//   ints    ti[k * MC_RB + l]: type (1 pivot, 0 displacement), moved beads
+//           [s, e), pivot (or displaced) bead, rotation axis
+// A proposal does not depend on the walker's state, so a whole batch can be
+// prepared ahead of the trials that use it.
 #define MC_STREAMS 64
 #define MC_RB 64
 #define MC_RPT 8
-__device__ __forceinline__ void draw_batch(uint64_t *st, double *dst) {
+#define MC_TD 8
+#define MC_TI 5
+__device__ __forceinline__ void draw_batch(uint64_t *srng, double *tb, int *ti, int n, const McArgs &a, const double *tc) {
     int l = threadIdx.x & 63;
+    uint64_t st[4] = {srng[l], srng[64 + l], srng[128 + l], srng[192 + l]};
+    double u[MC_RPT];
 #pragma unroll
-    for (int k = 0; k < MC_RPT; k++) dst[k * MC_RB + l] = xo_uniform(st);
+    for (int k = 0; k < MC_RPT; k++) u[k] = xo_uniform(st);
+    srng[l] = st[0]; srng[64 + l] = st[1]; srng[128 + l] = st[2]; srng[192 + l] = st[3];
+    int type, s, e, b, axis = 0;
+    double th = 0.0, c = 1.0, sn = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    if (u[0] < a.pivot_frac && n > 2) {
+        int p = (int)(u[1] * n);
+        int side = (int)(u[3] * 2.0);
+        double ang = u[5] * MC_PI * a.theta_max;
+        axis = (int)(u[2] * 3.0);
+        th = (int)(u[4] * 2.0) ? -ang : ang;
+        sincos_pivot(th, tc, &sn, &c);
+        type = 1;
+        s = side == 0 ? 0 : p + 1;
+        e = side == 0 ? p : n;
+        b = p;
+    } else {
+        int k = (int)(u[1] * n);
+        type = 0;
+        s = k;
+        e = k + 1;
+        b = k;
+        d0 = (u[2] - 0.5) * a.disp_max;
+        d1 = (u[3] - 0.5) * a.disp_max;
+        d2 = (u[4] - 0.5) * a.disp_max;
+    }
+    tb[l] = u[6]; tb[MC_RB + l] = u[7]; tb[2 * MC_RB + l] = th; tb[3 * MC_RB + l] = c; tb[4 * MC_RB + l] = sn;
+    tb[5 * MC_RB + l] = d0; tb[6 * MC_RB + l] = d1; tb[7 * MC_RB + l] = d2;
+    ti[l] = type; ti[MC_RB + l] = s; ti[2 * MC_RB + l] = e; ti[3 * MC_RB + l] = b; ti[4 * MC_RB + l] = axis;
 }
 
-// One workgroup per walker runs all its trials.  Every thread reads the same
-// proposal and acceptance numbers from LDS and takes the same decisions; the
-// moved coordinates and the energy changes are computed cooperatively.
+// New position of bead i under the rotation about `axis` through bead piv
+// (PIVOT matrices), written out per axis (hipcc 7.2 at -O3 mis-assembled the
+// z matrix when it was built in a zero-initialised 3x3 array and applied
+// generically).
+__device__ __forceinline__ void pivot_bead(const double *pos, int i, int piv, int axis, double c, double sn, double *o) {
+    double px = pos[3 * piv], py = pos[3 * piv + 1], pz = pos[3 * piv + 2];
+    double x = pos[3 * i] - px, y = pos[3 * i + 1] - py, z = pos[3 * i + 2] - pz;
+    double nx, ny, nz;
+    if (axis == 0) { nx = x; ny = c * y - sn * z; nz = sn * y + c * z; }
+    else if (axis == 1) { nx = c * x + sn * z; ny = y; nz = c * z - sn * x; }
+    else { nx = c * x + sn * y; ny = c * y - sn * x; nz = z; }
+    o[0] = nx + px; o[1] = ny + py; o[2] = nz + pz;
+}
+
+// sum of the 16 waves' partial sums red[0..16), computed by one wave in a
+// fixed order; wave-uniform result
+__device__ __forceinline__ double sum_waves(const double *red) {
+    int l = threadIdx.x & 63;
+    return wave_sum_f64(l < MC_WAVES ? red[l] : 0.0);
+}
+
+// One workgroup per walker runs all its trials.  The work that is the same
+// for the whole walker (drawing proposals, Wang-Landau bookkeeping,
+// acceptance, copying accepted coordinates, logging) runs once, in wave 0;
+// all 16 waves read the trial's move from the proposal table and compute the
+// moved coordinates, the move features, the network and the exact energy
+// change.  Barriers per trial: moved coordinates and features ready; (delayed
+// acceptance: between the layers, after the network, after the first-stage
+// decision); exact partial sums ready; bookkeeping done.
 template <bool DA>
 __global__ void __launch_bounds__(MC_THREADS)
 k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int *gn, const int *gnoff,
@@ -456,8 +534,12 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     int *soff = (int *)(sr0 + max_nnz);       // maxn + 1
     int *spart = soff + a.maxn + 1;           // max_nnz
     size_t uofs = (((char *)(spart + max_nnz) - (char *)smem) + 15) & ~(size_t)15;
-    double *rnd = (double *)((char *)smem + uofs);   // 2 * MC_RPT * MC_RB (two batches)
-    float4 *posf = (float4 *)(rnd + 2 * MC_RPT * MC_RB);   // maxn: FP32 copy of pos for the features
+    double *tb = (double *)((char *)smem + uofs);   // MC_TD * MC_RB: proposal table (doubles)
+    uint64_t *srng = (uint64_t *)(tb + MC_TD * MC_RB);   // 4 * MC_STREAMS: random streams
+    float4 *posf = (float4 *)(srng + 4 * MC_STREAMS);      // maxn: FP32 copy of pos for the features
+    double *tcoef = (double *)(posf + a.maxn);   // 17: sin/cos Taylor coefficients
+    int *ti = (int *)(tcoef + 17);            // MC_TI * MC_RB: proposal table (ints)
+    int *pass = ti + MC_TI * MC_RB;           // first-stage decision
 
     for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) pos[i] = gpos[(size_t)w * 3 * a.maxn + i];
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -479,91 +561,51 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     }
     if (DA) mlp_load_lds(gmlp, W);
     for (int i = threadIdx.x; i < 2 * FEAT_SWAP; i += blockDim.x) feat[i] = 0.0f;   // slots past NF stay zero
-    uint64_t s4[4] = {0, 0, 0, 0};   // wave 0: lane's random stream
-    if (threadIdx.x < MC_STREAMS) {
-        const uint64_t *g = grng + ((size_t)w * MC_STREAMS + threadIdx.x) * 4;
-        s4[0] = g[0]; s4[1] = g[1]; s4[2] = g[2]; s4[3] = g[3];
-        draw_batch(s4, rnd);
-    }
+    if (threadIdx.x < 17) tcoef[threadIdx.x] = kSinCosTaylor[threadIdx.x];
     __syncthreads();
-    double emin = gemin[w];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const bool w0 = wv == 0;
+    // wave 0's walker state
+    const double emin = gemin[w];
     double ecur = gecur[w];
     int bx = (int)floor((ecur - emin) / a.ebin);   // bin of the current energy
-    long long st_tr = 0, st_acc = 0, st_s1 = 0, st_ex = 0, st_out = 0, st_piv = 0, st_pacc = 0;
-    // Wang-Landau update of the bin a trial ended in (pivot.c:236-253):
-    // every thread reads ln g for its own (identical) acceptance decision, so
-    // thread 0 applies the update only after the next barrier, when no wave
-    // can still be reading the old value
-    int pend = -1;
+    double sur = 0.0, a1f = 0.0;   // surrogate dE, log first-stage probability of the trial
+    int st_acc = 0, st_s1 = 0, st_ex = 0, st_out = 0, st_piv = 0, st_pacc = 0;   // this launch
+    if (w0) {
+        const uint64_t *g = grng + ((size_t)w * MC_STREAMS + lane) * 4;
+        for (int q = 0; q < 4; q++) srng[q * 64 + lane] = g[q];
+        draw_batch(srng, tb, ti, n, a, tcoef);
+    }
+    __syncthreads();
+
+    // Wave 0: the Wang-Landau update of the bin the trial ended in
+    // (pivot.c:236-253), and the next batch of proposals when this one is used up.
+    auto finish = [&](int trial) {
+        if (bx >= 0 && bx < a.nbins && lane == 0) { lng[bx] += lnf; hist[bx]++; }
+        if ((trial + 1) % MC_RB == 0 && trial + 1 < ntrials) draw_batch(srng, tb, ti, n, a, tcoef);
+    };
 
     for (int trial = 0; trial < ntrials; trial++) {
-        const double *ur = rnd + ((trial / MC_RB) & 1) * (MC_RPT * MC_RB) + trial % MC_RB;   // ur[k * MC_RB]
-        // ---- proposal (mcmove_pivot.c:54-140, mcmove_displ.c:45-99)
-        int s, e, type, piv = 0;
-        double th = 0.0;
-        if (ur[0] < a.pivot_frac && n > 2) {
-            int p = __builtin_amdgcn_readfirstlane((int)(ur[MC_RB] * n));
-            int axis = __builtin_amdgcn_readfirstlane((int)(ur[2 * MC_RB] * 3.0));
-            int side = __builtin_amdgcn_readfirstlane((int)(ur[3 * MC_RB] * 2.0));
-            int dir = __builtin_amdgcn_readfirstlane((int)(ur[4 * MC_RB] * 2.0));
-            double ang = ur[5 * MC_RB] * MC_PI * a.theta_max;
-            type = 1;
-            s = side == 0 ? 0 : p + 1;
-            e = side == 0 ? p : n;
-            th = dir ? -ang : ang;   // signed angle
-            piv = p;
-            // rotation about x, y or z through the pivot bead (PIVOT matrices),
-            // evaluated only by the threads that move a bead; written out per
-            // axis (hipcc 7.2 at -O3 mis-assembled the z matrix when it was
-            // built in a zero-initialised 3x3 array and applied generically)
-            if ((int)threadIdx.x < e - s) {
-                double c, sn;
-                sincos_pivot(th, &sn, &c);
-                double px = pos[3 * piv], py = pos[3 * piv + 1], pz = pos[3 * piv + 2];
-                for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
-                    double x = pos[3 * i] - px, y = pos[3 * i + 1] - py, z = pos[3 * i + 2] - pz;
-                    double nx, ny, nz;
-                    if (axis == 0) { nx = x; ny = c * y - sn * z; nz = sn * y + c * z; }
-                    else if (axis == 1) { nx = c * x + sn * z; ny = y; nz = c * z - sn * x; }
-                    else { nx = c * x + sn * y; ny = c * y - sn * x; nz = z; }
-                    newp[3 * i] = nx + px;
-                    newp[3 * i + 1] = ny + py;
-                    newp[3 * i + 2] = nz + pz;
-                }
-            }
-        } else {
-            int k = __builtin_amdgcn_readfirstlane((int)(ur[MC_RB] * n));
-            type = 0;
-            s = k;
-            e = k + 1;
-            if (threadIdx.x == 0) {
-                double d0 = (ur[2 * MC_RB] - 0.5) * a.disp_max;
-                double d1 = (ur[3 * MC_RB] - 0.5) * a.disp_max;
-                double d2 = (ur[4 * MC_RB] - 0.5) * a.disp_max;
-                newp[3 * k] = pos[3 * k] + d0;
-                newp[3 * k + 1] = pos[3 * k + 1] + d1;
-                newp[3 * k + 2] = pos[3 * k + 2] + d2;
-            }
+        const int tl = trial % MC_RB;   // this trial's column of the proposal table
+        const int type = __builtin_amdgcn_readfirstlane(ti[tl]);
+        const int s = __builtin_amdgcn_readfirstlane(ti[MC_RB + tl]), e = __builtin_amdgcn_readfirstlane(ti[2 * MC_RB + tl]);
+        const int piv = __builtin_amdgcn_readfirstlane(ti[3 * MC_RB + tl]), axis = __builtin_amdgcn_readfirstlane(ti[4 * MC_RB + tl]);
+        const double c = tb[3 * MC_RB + tl], sn = tb[4 * MC_RB + tl];
+        // ---- moved coordinates
+        if (type == 1) {
+            for (int i = s + threadIdx.x; i < e; i += blockDim.x) pivot_bead(pos, i, piv, axis, c, sn, newp + 3 * i);
+        } else if (threadIdx.x == 0) {
+            newp[3 * s] = pos[3 * s] + tb[5 * MC_RB + tl];
+            newp[3 * s + 1] = pos[3 * s + 1] + tb[6 * MC_RB + tl];
+            newp[3 * s + 2] = pos[3 * s + 2] + tb[7 * MC_RB + tl];
         }
-        __syncthreads();
-        // every thread has read the previous batch's numbers: draw the next one
-        if (trial % MC_RB == 0 && threadIdx.x < MC_STREAMS)
-            draw_batch(s4, rnd + ((trial / MC_RB + 1) & 1) * (MC_RPT * MC_RB));
-        // the previous trial's bin; every read of ln g below follows another barrier
-        if (threadIdx.x == 0 && pend >= 0) { lng[pend] += lnf; hist[pend]++; }
-        pend = -1;
-        st_tr++;
-        st_piv += type;
-        if (e <= s) {   // empty moved part: nothing changes, counts as a rejected trial
-            if (bx >= 0 && bx < a.nbins) pend = bx;
-            continue;
-        }
+        const bool move = e > s;   // empty moved part: nothing changes, a rejected trial
         // ---- features of pivot moves: local distances around the pivot, and
-        // the closest approach of sampled moved beads to the fixed part
-        bool need_feat = type == 1 && (DA || do_log);
-        if (need_feat) {
+        // the closest approach of sampled moved beads to the fixed part (the
+        // waves rotate the beads they need themselves)
+        if (type == 1 && move && (DA || do_log)) {
             int sg = (s == 0) ? -1 : 1;     // moved side direction from the pivot
-            int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+            float sig2f = (float)a.sig2;
             if (wv < 8) {
                 // wave k < 8 takes sampled bead k: lanes scan the fixed beads,
                 // the minima are reduced in the wave
@@ -571,8 +613,10 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                 float mo = 3.0e38f, mn = 3.0e38f;
                 if (bk >= s && bk < e) {
                     float4 b0 = posf[bk];
-                    float bx1 = (float)newp[3 * bk], by1 = (float)newp[3 * bk + 1], bz1 = (float)newp[3 * bk + 2];
-                    for (int f = ln; f < n; f += 64) {
+                    double nb[3];
+                    pivot_bead(pos, bk, piv, axis, c, sn, nb);
+                    float bx1 = (float)nb[0], by1 = (float)nb[1], bz1 = (float)nb[2];
+                    for (int f = lane; f < n; f += 64) {
                         int d = f - bk;
                         if ((f >= s && f < e) || (d < 3 && d > -3)) continue;
                         float4 pf = posf[f];
@@ -584,9 +628,9 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                 }
                 mo = wave_min_f32(mo);
                 mn = wave_min_f32(mn);
-                if (ln == 0) {
-                    float fo = bf16_round(mo < 1e37f ? feat_term(mo, (float)a.sig2) : 0.0f);
-                    float fnw = bf16_round(mn < 1e37f ? feat_term(mn, (float)a.sig2) : 0.0f);
+                if (lane == 0) {
+                    float fo = bf16_round(mo < 1e37f ? feat_term(mo, sig2f) : 0.0f);
+                    float fnw = bf16_round(mn < 1e37f ? feat_term(mn, sig2f) : 0.0f);
                     feat[32 + wv] = fo;
                     feat[40 + wv] = fnw;
                     feat[FEAT_SWAP + 32 + wv] = fnw;
@@ -595,102 +639,117 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
             } else if (wv == 8) {
                 // local distances of the 4 moved x 4 fixed beads nearest the
                 // pivot, and the move's scalars
-                if (ln < 16) {
-                    int i = ln / 4 + 1, j = ln % 4 + 1;
+                if (lane < 16) {
+                    int i = lane / 4 + 1, j = lane % 4 + 1;
                     int mb = piv + sg * i, fb = piv - sg * j;
                     float v = 0.0f, v2 = 0.0f;
                     if (mb >= 0 && mb < n && fb >= 0 && fb < n) {
                         float4 pm = posf[mb], pf = posf[fb];
+                        double nb[3];
+                        pivot_bead(pos, mb, piv, axis, c, sn, nb);
                         float ox = pm.x - pf.x, oy = pm.y - pf.y, oz = pm.z - pf.z;
-                        float nx = (float)newp[3 * mb] - pf.x, ny = (float)newp[3 * mb + 1] - pf.y, nz = (float)newp[3 * mb + 2] - pf.z;
-                        v = bf16_round(feat_term(ox * ox + oy * oy + oz * oz, (float)a.sig2));
-                        v2 = bf16_round(feat_term(nx * nx + ny * ny + nz * nz, (float)a.sig2));
+                        float nx = (float)nb[0] - pf.x, ny = (float)nb[1] - pf.y, nz = (float)nb[2] - pf.z;
+                        v = bf16_round(feat_term(ox * ox + oy * oy + oz * oz, sig2f));
+                        v2 = bf16_round(feat_term(nx * nx + ny * ny + nz * nz, sig2f));
                     }
-                    feat[ln] = v;
-                    feat[16 + ln] = v2;
-                    feat[FEAT_SWAP + ln] = v2;
-                    feat[FEAT_SWAP + 16 + ln] = v;
-                } else if (ln < 20) {
-                    int q = ln - 16;
-                    float v = bf16_round(q == 0 ? 1.0f : q == 1 ? (float)(fabs(th) / MC_PI) : q == 2 ? (float)(e - s) / (float)n : 0.0f);
+                    feat[lane] = v;
+                    feat[16 + lane] = v2;
+                    feat[FEAT_SWAP + lane] = v2;
+                    feat[FEAT_SWAP + 16 + lane] = v;
+                } else if (lane < 20) {
+                    int q = lane - 16;
+                    float v = bf16_round(q == 0 ? 1.0f : q == 1 ? (float)(fabs(tb[2 * MC_RB + tl]) * (1.0 / MC_PI))
+                                                 : q == 2 ? (float)(e - s) / (float)n : 0.0f);
                     feat[48 + q] = v;
                     feat[FEAT_SWAP + 48 + q] = v;
                 }
             }
+        }
+        __syncthreads();   // moved coordinates and features are visible
+        if (w0) st_piv += type;
+        if (DA && type == 1 && move) {
+            // first stage: surrogate dE of the move (antisymmetric network),
+            // a1 = min(1, g(E_x) / g(E_x + sur)) as log a1
+            mlp_block(W, feat, hs, pv);
+            if (w0) {
+                float so = sinhf((pv[0] + pv[1]) + (pv[2] + pv[3]));
+                sur = escale * (double)so;
+                int by = (int)floor((ecur + sur - emin) / a.ebin);
+                a1f = (bx >= 0 && bx < a.nbins && by >= 0 && by < a.nbins) ? fmin(0.0, lng[bx] - lng[by]) : -INFINITY;
+                bool ok = accept_log(a1f, tb[tl]);
+                if (lane == 0) *pass = ok ? 1 : 0;
+                if (!ok) {
+                    st_s1++;
+                    finish(trial);
+                }
+            }
+            __syncthreads();   // first-stage decision (and, after a rejection, the bookkeeping) visible
+            if (!__builtin_amdgcn_readfirstlane(*pass)) continue;
+        }
+        // ---- exact range energy change (FP64), partial sums per wave
+        if (move) {
+            double part = wave_sum_f64(range_delta_part(pos, newp, n, s, e, nat, a, type == 1));
+            if (lane == 0) red[wv] = part;
             __syncthreads();
         }
-        double sur = 0.0, a1f = 0.0;   // a1f: log of the first-stage probability
-        if (DA && type == 1) {
-            // first stage: surrogate dE of the move (antisymmetric network),
-            // a1 = min(1, g(E_x) / g(E_x + sur)) as log a1, decided by every
-            // thread from the same values
-            double u1 = ur[6 * MC_RB];
-            float so = sinhf(mlp_block(W, feat, hs, pv));
-            sur = escale * (double)so;
-            int by = (int)floor((ecur + sur - emin) / a.ebin);
-            a1f = (bx >= 0 && bx < a.nbins && by >= 0 && by < a.nbins) ? fmin(0.0, lng[bx] - lng[by]) : -INFINITY;
-            if (!accept_log(a1f, u1)) {
-                st_s1++;
-                if (bx >= 0 && bx < a.nbins) pend = bx;
-                continue;
+        if (w0) {
+            bool acc = false;
+            if (move) {
+                double de = sum_waves(red);
+                st_ex++;
+                int by = (int)floor((ecur + de - emin) / a.ebin);
+                if (by < 0 || by >= a.nbins || bx < 0 || bx >= a.nbins) {
+                    st_out++;
+                } else {
+                    double lratio = lng[bx] - lng[by];
+                    if (DA && type == 1) {
+                        // second stage: pi(y) a1(y,x) / (pi(x) a1(x,y)), a1(y,x) from
+                        // the surrogate of the reverse move, -sur (all as logarithms)
+                        int bz = (int)floor((ecur + de - sur - emin) / a.ebin);
+                        double la1r = -INFINITY;
+                        if (bz >= 0 && bz < a.nbins) la1r = fmin(0.0, lng[by] - lng[bz]);
+                        lratio = lratio + la1r - a1f;
+                    }
+                    acc = accept_log(lratio, tb[MC_RB + tl]);
+                }
+                if (do_log && type == 1) {
+                    int slot = 0;
+                    if (lane == 0) slot = atomicAdd(logn, 1);
+                    slot = __builtin_amdgcn_readfirstlane(slot);
+                    if (slot < logcap) {
+                        for (int q = lane; q < NF; q += 64) logx[(size_t)slot * NF + q] = feat[q];
+                        if (lane == 0) logde[slot] = de;
+                    }
+                }
+                if (acc) {
+                    for (int i = s + lane; i < e; i += 64) {
+                        double x = newp[3 * i], y = newp[3 * i + 1], z = newp[3 * i + 2];
+                        pos[3 * i] = x; pos[3 * i + 1] = y; pos[3 * i + 2] = z;
+                        posf[i] = make_float4((float)x, (float)y, (float)z, 0.0f);
+                    }
+                    ecur += de;
+                    bx = by;   // floor((ecur - emin) / ebin) of the new ecur
+                    st_acc++;
+                    st_pacc += type;
+                }
             }
+            finish(trial);
         }
-        // ---- exact range energy change (FP64)
-        double de = range_delta(pos, newp, n, s, e, nat, a, red, type == 1);
-        st_ex++;
-        int by = (int)floor((ecur + de - emin) / a.ebin);
-        bool acc = false;
-        if (by < 0 || by >= a.nbins || bx < 0 || bx >= a.nbins) {
-            st_out++;
-        } else {
-            double lratio = lng[bx] - lng[by];
-            if (DA && type == 1) {
-                // second stage: pi(y) a1(y,x) / (pi(x) a1(x,y)), a1(y,x) from the
-                // surrogate of the reverse move, -sur (all as logarithms)
-                int bz = (int)floor((ecur + de - sur - emin) / a.ebin);
-                double la1r = -INFINITY;
-                if (bz >= 0 && bz < a.nbins) la1r = fmin(0.0, lng[by] - lng[bz]);
-                lratio = lratio + la1r - a1f;
-            }
-            acc = accept_log(lratio, ur[7 * MC_RB]);
-        }
-        if (threadIdx.x == 0 && do_log && type == 1) {
-            int slot = atomicAdd(logn, 1);
-            if (slot < logcap) {
-                for (int q = 0; q < NF; q++) logx[(size_t)slot * NF + q] = feat[q];
-                logde[slot] = de;
-            }
-        }
-        if (acc) {
-            for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
-                double x = newp[3 * i], y = newp[3 * i + 1], z = newp[3 * i + 2];
-                pos[3 * i] = x; pos[3 * i + 1] = y; pos[3 * i + 2] = z;
-                posf[i] = make_float4((float)x, (float)y, (float)z, 0.0f);
-            }
-            ecur += de;
-            bx = by;   // floor((ecur - emin) / ebin) of the new ecur
-            st_acc++;
-            st_pacc += type;
-            __syncthreads();   // the next proposal reads the moved beads
-        }
-        if (bx >= 0 && bx < a.nbins) pend = bx;
+        __syncthreads();   // bookkeeping (accepted coordinates, next proposals) visible
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && pend >= 0) { lng[pend] += lnf; hist[pend]++; }
-    __syncthreads();
     for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) gpos[(size_t)w * 3 * a.maxn + i] = pos[i];
     for (int i = threadIdx.x; i < a.nbins; i += blockDim.x) {
         glng[(size_t)w * a.nbins + i] = lng[i];
         ghist[(size_t)w * a.nbins + i] = hist[i];
     }
-    if (threadIdx.x < MC_STREAMS) {
-        uint64_t *g = grng + ((size_t)w * MC_STREAMS + threadIdx.x) * 4;
-        g[0] = s4[0]; g[1] = s4[1]; g[2] = s4[2]; g[3] = s4[3];
+    if (w0) {
+        uint64_t *g = grng + ((size_t)w * MC_STREAMS + lane) * 4;
+        for (int q = 0; q < 4; q++) g[q] = srng[q * 64 + lane];
     }
     if (threadIdx.x == 0) {
         gecur[w] = ecur;
         long long *st = gstats + (size_t)w * TTC_MC_NSTAT;
-        st[0] += st_tr; st[1] += st_acc; st[2] += st_s1; st[3] += st_ex; st[4] += st_out;
+        st[0] += ntrials; st[1] += st_acc; st[2] += st_s1; st[3] += st_ex; st[4] += st_out;
         st[5] += st_piv; st[6] += st_pacc;
     }
 }
@@ -926,7 +985,8 @@ static size_t run_shm(const ttc_mc *m, bool da) {
     // native contacts: r0 (aligned up to 8 bytes), offsets, partners
     size_t nat = 8 + (size_t)m->max_nnz * (sizeof(double) + sizeof(int)) + (size_t)(m->p.maxn + 1) * sizeof(int);
     // random-number batches (aligned up to 16 bytes) and the FP32 bead copy
-    size_t rn = 16 + (size_t)2 * MC_RPT * MC_RB * sizeof(double) + (size_t)m->p.maxn * 4 * sizeof(float);
+    size_t rn = 16 + (size_t)MC_TD * MC_RB * sizeof(double) + (size_t)4 * MC_STREAMS * sizeof(uint64_t) +
+                (size_t)m->p.maxn * 4 * sizeof(float) + 17 * sizeof(double) + (size_t)(MC_TI * MC_RB + 1) * sizeof(int);
     return d + f + nat + rn;
 }
 
