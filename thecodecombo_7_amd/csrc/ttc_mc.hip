@@ -305,6 +305,39 @@ __device__ double range_delta(const double *pos, const double *newp, int n, int 
     return block_sum(range_delta_part(pos, newp, n, s, e, nat, a, rigid), red);
 }
 
+// The same for a one-bead displacement of bead k (mcmove_displ.c): thread b
+// takes the pair (b, k), |b - k| >= 3, with the native distance nmark[b]
+// (set for k's native partners, zero for every other bead), and the bonds of
+// k.  This thread's share.
+__device__ double displ_delta_part(const double *pos, const double *newp, int n, int k, const double *nmark,
+                                   const McArgs &a) {
+    double acc = 0.0;
+    double ox = pos[3 * k], oy = pos[3 * k + 1], oz = pos[3 * k + 2];
+    double cx = newp[3 * k], cy = newp[3 * k + 1], cz = newp[3 * k + 2];
+    for (int b = threadIdx.x; b < n; b += blockDim.x) {
+        int d = b - k;
+        if (d < 3 && d > -3) continue;
+        double r0 = nmark[b];
+        double bx = pos[3 * b], by = pos[3 * b + 1], bz = pos[3 * b + 2];
+        double dx = ox - bx, dy = oy - by, dz = oz - bz;
+        double ex = cx - bx, ey = cy - by, ez = cz - bz;
+        acc += go_pair_inv(recip64(ex * ex + ey * ey + ez * ez), r0, a.eps, a.sig2) -
+               go_pair_inv(recip64(dx * dx + dy * dy + dz * dz), r0, a.eps, a.sig2);
+    }
+    int i = k - 1 + (int)threadIdx.x;   // bonds (k-1, k) and (k, k+1), cbond.c:103-104 form
+    if (threadIdx.x < 2 && i >= 0 && i + 1 < n) {
+        int j = i + 1;
+        const double *pi = i == k ? newp + 3 * i : pos + 3 * i;
+        const double *pj = j == k ? newp + 3 * j : pos + 3 * j;
+        double qx = pos[3 * j] - pos[3 * i], qy = pos[3 * j + 1] - pos[3 * i + 1], qz = pos[3 * j + 2] - pos[3 * i + 2];
+        double rx = pj[0] - pi[0], ry = pj[1] - pi[1], rz = pj[2] - pi[2];
+        double ro = sqrt(qx * qx + qy * qy + qz * qz) - a.r0bond;
+        double rn = sqrt(rx * rx + ry * ry + rz * rz) - a.r0bond;
+        acc += a.kbond * rn * rn - a.kbond * ro * ro;
+    }
+    return acc;
+}
+
 __device__ double full_energy(const double *pos, int n, const Natives &nat, const McArgs &a, double *red) {
     double acc = 0.0;
     const int S = max(1, (int)blockDim.x / n);
@@ -540,6 +573,8 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     double *tcoef = (double *)(posf + a.maxn);   // 17: sin/cos Taylor coefficients
     int *ti = (int *)(tcoef + 17);            // MC_TI * MC_RB: proposal table (ints)
     int *pass = ti + MC_TI * MC_RB;           // first-stage decision
+    size_t mofs = (((char *)(pass + 1) - (char *)smem) + 7) & ~(size_t)7;
+    double *nmark = (double *)((char *)smem + mofs);   // maxn: native distance to the displaced bead
 
     for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) pos[i] = gpos[(size_t)w * 3 * a.maxn + i];
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -562,6 +597,7 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     if (DA) mlp_load_lds(gmlp, W);
     for (int i = threadIdx.x; i < 2 * FEAT_SWAP; i += blockDim.x) feat[i] = 0.0f;   // slots past NF stay zero
     if (threadIdx.x < 17) tcoef[threadIdx.x] = kSinCosTaylor[threadIdx.x];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) nmark[i] = 0.0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const bool w0 = wv == 0;
@@ -594,10 +630,15 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
         // ---- moved coordinates
         if (type == 1) {
             for (int i = s + threadIdx.x; i < e; i += blockDim.x) pivot_bead(pos, i, piv, axis, c, sn, newp + 3 * i);
-        } else if (threadIdx.x == 0) {
-            newp[3 * s] = pos[3 * s] + tb[5 * MC_RB + tl];
-            newp[3 * s + 1] = pos[3 * s + 1] + tb[6 * MC_RB + tl];
-            newp[3 * s + 2] = pos[3 * s + 2] + tb[7 * MC_RB + tl];
+        } else {
+            if (threadIdx.x == 0) {
+                newp[3 * s] = pos[3 * s] + tb[5 * MC_RB + tl];
+                newp[3 * s + 1] = pos[3 * s + 1] + tb[6 * MC_RB + tl];
+                newp[3 * s + 2] = pos[3 * s + 2] + tb[7 * MC_RB + tl];
+            }
+            // native distances to the displaced bead (wave 0 clears them again)
+            int o = soff[s], dg = soff[s + 1] - o;
+            if ((int)threadIdx.x < dg) nmark[spart[o + threadIdx.x]] = sr0[o + threadIdx.x];
         }
         const bool move = e > s;   // empty moved part: nothing changes, a rejected trial
         // ---- features of pivot moves: local distances around the pivot, and
@@ -616,14 +657,17 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                     double nb[3];
                     pivot_bead(pos, bk, piv, axis, c, sn, nb);
                     float bx1 = (float)nb[0], by1 = (float)nb[1], bz1 = (float)nb[2];
-                    for (int f = lane; f < n; f += 64) {
+                    // the fixed part of a pivot move is one side, [e, n) or [0, s)
+                    int flo = s == 0 ? e : 0, fhi = s == 0 ? n : s;
+#pragma unroll 4
+                    for (int f = flo + lane; f < fhi; f += 64) {
                         int d = f - bk;
-                        if ((f >= s && f < e) || (d < 3 && d > -3)) continue;
                         float4 pf = posf[f];
                         float ox = b0.x - pf.x, oy = b0.y - pf.y, oz = b0.z - pf.z;
                         float nx = bx1 - pf.x, ny = by1 - pf.y, nz = bz1 - pf.z;
-                        mo = fminf(mo, ox * ox + oy * oy + oz * oz);
-                        mn = fminf(mn, nx * nx + ny * ny + nz * nz);
+                        bool far = d >= 3 || d <= -3;
+                        mo = far ? fminf(mo, ox * ox + oy * oy + oz * oz) : mo;
+                        mn = far ? fminf(mn, nx * nx + ny * ny + nz * nz) : mn;
                     }
                 }
                 mo = wave_min_f32(mo);
@@ -688,7 +732,9 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
         }
         // ---- exact range energy change (FP64), partial sums per wave
         if (move) {
-            double part = wave_sum_f64(range_delta_part(pos, newp, n, s, e, nat, a, type == 1));
+            double part = type == 1 ? range_delta_part(pos, newp, n, s, e, nat, a, true)
+                                    : displ_delta_part(pos, newp, n, s, nmark, a);
+            part = wave_sum_f64(part);
             if (lane == 0) red[wv] = part;
             __syncthreads();
         }
@@ -731,6 +777,10 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                     bx = by;   // floor((ecur - emin) / ebin) of the new ecur
                     st_acc++;
                     st_pacc += type;
+                }
+                if (type == 0) {
+                    int o = soff[s], dg = soff[s + 1] - o;
+                    for (int q = lane; q < dg; q += 64) nmark[spart[o + q]] = 0.0;
                 }
             }
             finish(trial);
@@ -986,7 +1036,8 @@ static size_t run_shm(const ttc_mc *m, bool da) {
     size_t nat = 8 + (size_t)m->max_nnz * (sizeof(double) + sizeof(int)) + (size_t)(m->p.maxn + 1) * sizeof(int);
     // random-number batches (aligned up to 16 bytes) and the FP32 bead copy
     size_t rn = 16 + (size_t)MC_TD * MC_RB * sizeof(double) + (size_t)4 * MC_STREAMS * sizeof(uint64_t) +
-                (size_t)m->p.maxn * 4 * sizeof(float) + 17 * sizeof(double) + (size_t)(MC_TI * MC_RB + 1) * sizeof(int);
+                (size_t)m->p.maxn * 4 * sizeof(float) + 17 * sizeof(double) + (size_t)(MC_TI * MC_RB + 1) * sizeof(int) +
+                8 + (size_t)m->p.maxn * sizeof(double);
     return d + f + nat + rn;
 }
 
