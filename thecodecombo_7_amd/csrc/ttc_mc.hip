@@ -604,7 +604,12 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     // wave 0's walker state
     const double emin = gemin[w];
     double ecur = gecur[w];
-    int bx = (int)floor((ecur - emin) / a.ebin);   // bin of the current energy
+    // Wang-Landau bin of an energy: floor((E - emin) / ebin), with the
+    // division done as a multiplication by the reciprocal (the bin edges move
+    // by at most an ulp of the quotient; every bin here uses the same form)
+    const double inv_ebin = 1.0 / a.ebin;
+    auto bin_of = [&](double en) { return (int)floor((en - emin) * inv_ebin); };
+    int bx = bin_of(ecur);   // bin of the current energy
     double sur = 0.0, a1f = 0.0;   // surrogate dE, log first-stage probability of the trial
     int st_acc = 0, st_s1 = 0, st_ex = 0, st_out = 0, st_piv = 0, st_pacc = 0;   // this launch
     if (w0) {
@@ -617,7 +622,10 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     // Wave 0: the Wang-Landau update of the bin the trial ended in
     // (pivot.c:236-253), and the next batch of proposals when this one is used up.
     auto finish = [&](int trial) {
-        if (bx >= 0 && bx < a.nbins && lane == 0) { lng[bx] += lnf; hist[bx]++; }
+        if (bx >= 0 && bx < a.nbins && lane == 0) {   // LDS atomics: nothing waits for them
+            atomicAdd(&lng[bx], lnf);
+            atomicAdd((unsigned long long *)&hist[bx], 1ull);
+        }
         if ((trial + 1) % MC_RB == 0 && trial + 1 < ntrials) draw_batch(srng, tb, ti, n, a, tcoef);
     };
 
@@ -716,9 +724,10 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
             // a1 = min(1, g(E_x) / g(E_x + sur)) as log a1
             mlp_block(W, feat, hs, pv);
             if (w0) {
-                float so = sinhf((pv[0] + pv[1]) + (pv[2] + pv[3]));
+                float ex = __expf((pv[0] + pv[1]) + (pv[2] + pv[3]));
+                float so = 0.5f * (ex - __builtin_amdgcn_rcpf(ex));   // sinh of the network output
                 sur = escale * (double)so;
-                int by = (int)floor((ecur + sur - emin) / a.ebin);
+                int by = bin_of(ecur + sur);
                 a1f = (bx >= 0 && bx < a.nbins && by >= 0 && by < a.nbins) ? fmin(0.0, lng[bx] - lng[by]) : -INFINITY;
                 bool ok = accept_log(a1f, tb[tl]);
                 if (lane == 0) *pass = ok ? 1 : 0;
@@ -743,7 +752,7 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
             if (move) {
                 double de = sum_waves(red);
                 st_ex++;
-                int by = (int)floor((ecur + de - emin) / a.ebin);
+                int by = bin_of(ecur + de);
                 if (by < 0 || by >= a.nbins || bx < 0 || bx >= a.nbins) {
                     st_out++;
                 } else {
@@ -751,7 +760,7 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                     if (DA && type == 1) {
                         // second stage: pi(y) a1(y,x) / (pi(x) a1(x,y)), a1(y,x) from
                         // the surrogate of the reverse move, -sur (all as logarithms)
-                        int bz = (int)floor((ecur + de - sur - emin) / a.ebin);
+                        int bz = bin_of(ecur + de - sur);
                         double la1r = -INFINITY;
                         if (bz >= 0 && bz < a.nbins) la1r = fmin(0.0, lng[by] - lng[bz]);
                         lratio = lratio + la1r - a1f;
@@ -768,13 +777,25 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                     }
                 }
                 if (acc) {
-                    for (int i = s + lane; i < e; i += 64) {
-                        double x = newp[3 * i], y = newp[3 * i + 1], z = newp[3 * i + 2];
-                        pos[3 * i] = x; pos[3 * i + 1] = y; pos[3 * i + 2] = z;
-                        posf[i] = make_float4((float)x, (float)y, (float)z, 0.0f);
+                    // moved beads into pos: two per lane per pass, loads first
+                    for (int i0 = s + lane; i0 < e; i0 += 128) {
+                        double v[2][3];
+#pragma unroll
+                        for (int u = 0; u < 2; u++) {
+                            int i = i0 + 64 * u;
+                            if (i < e) { v[u][0] = newp[3 * i]; v[u][1] = newp[3 * i + 1]; v[u][2] = newp[3 * i + 2]; }
+                        }
+#pragma unroll
+                        for (int u = 0; u < 2; u++) {
+                            int i = i0 + 64 * u;
+                            if (i < e) {
+                                pos[3 * i] = v[u][0]; pos[3 * i + 1] = v[u][1]; pos[3 * i + 2] = v[u][2];
+                                posf[i] = make_float4((float)v[u][0], (float)v[u][1], (float)v[u][2], 0.0f);
+                            }
+                        }
                     }
                     ecur += de;
-                    bx = by;   // floor((ecur - emin) / ebin) of the new ecur
+                    bx = by;   // bin_of(ecur) of the new ecur
                     st_acc++;
                     st_pacc += type;
                 }
