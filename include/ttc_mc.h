@@ -91,6 +91,12 @@ int  ttc_mc_delta_energy(ttc_mc *m, int w, int lb, int ub, const double *newpos,
  * (features, exact dE) of every exactly evaluated move to the log buffer. */
 int  ttc_mc_run(ttc_mc *m, int ntrials, int mode, double lnf, int log);
 
+/* Device copy of the sampling state (walker positions and energies, ln g,
+ * histograms, random streams): restore = 0 takes it, 1 puts it back.  The
+ * in-memory counterpart of the reference's restart files for MC runs
+ * (checkpoint.c:146-917); counters, network and log are not part of it.  */
+int  ttc_mc_checkpoint(ttc_mc *m, int restore);
+
 int  ttc_mc_get_stats(ttc_mc *m, int64_t *stats /* nwalk * TTC_MC_NSTAT */);
 int  ttc_mc_reset_stats(ttc_mc *m);
 int  ttc_mc_get_dos(ttc_mc *m, int w, double *lng, int64_t *hist);

@@ -63,6 +63,10 @@ struct ttc_mc {
     int logcap = 0;
     hipStream_t stream = nullptr;
     std::vector<int> hn;        // host copy of bead counts
+    // device copy of the sampling state taken by ttc_mc_checkpoint
+    double *ck_pos = nullptr, *ck_ecur = nullptr, *ck_lng = nullptr;
+    long long *ck_hist = nullptr;
+    uint64_t *ck_rng = nullptr;
     double t_last = 0.0;
 };
 
@@ -201,6 +205,23 @@ __device__ __forceinline__ double go_pair_inv(double inv_r2, double r0, double e
     return r0 > 0.0 ? eps * (5.0 * s12 - 6.0 * (s6 * s4)) : eps * s12;
 }
 
+// E(new) - E(old) of one pair from a = r_new^2 and b = r_old^2 with one
+// reciprocal: E(r) = eps (c12 (s/r^2)^6 - c10 (s/r^2)^5) with (c12, c10, s) =
+// (5, 6, r0^2) for a native pair (cgolik.c:1209-1229 form) and (1, 0, sig^2)
+// for the rest, so
+//   E(a) - E(b) = eps [c12 s^6 (b^6 - a^6) - c10 s^5 (a b^6 - b a^6)] / (a^6 b^6).
+// (a^6 b^6 stays inside the FP64 range for pair distances from 1e-3 to 1e5 A.)
+__device__ __forceinline__ double go_pair_delta(double a, double b, double r0, double eps, double sig2) {
+    const double a3 = a * a * a, b3 = b * b * b, a6 = a3 * a3, b6 = b3 * b3;
+    const double inv = recip64(a6 * b6);
+    if (r0 > 0.0) {
+        const double s = r0 * r0, s2 = s * s, s5 = s2 * s2 * s, s6 = s5 * s;
+        return eps * inv * (5.0 * s6 * (b6 - a6) - 6.0 * s5 * (a * b6 - b * a6));
+    }
+    const double s3 = sig2 * sig2 * sig2;
+    return eps * (s3 * s3) * (b6 - a6) * inv;
+}
+
 // A walker's native contacts: CSR over beads (both directions), partners in
 // ascending order, native distance r0 per entry.
 struct Natives {
@@ -258,8 +279,7 @@ __device__ double range_delta_part(const double *pos, const double *newp, int n,
                 if (d < 3 && d > -3) continue;   // bonded neighbours
                 double dx = pos[3 * ia] - bx, dy = pos[3 * ia + 1] - by, dz = pos[3 * ia + 2] - bz;
                 double ex = newp[3 * ia] - bx, ey = newp[3 * ia + 1] - by, ez = newp[3 * ia + 2] - bz;
-                acc += go_pair_inv(recip64(ex * ex + ey * ey + ez * ez), r0, a.eps, a.sig2) -
-                       go_pair_inv(recip64(dx * dx + dy * dy + dz * dz), r0, a.eps, a.sig2);
+                acc += go_pair_delta(ex * ex + ey * ey + ez * ez, dx * dx + dy * dy + dz * dz, r0, a.eps, a.sig2);
             }
         }
     } else {
@@ -321,8 +341,7 @@ __device__ double displ_delta_part(const double *pos, const double *newp, int n,
         double bx = pos[3 * b], by = pos[3 * b + 1], bz = pos[3 * b + 2];
         double dx = ox - bx, dy = oy - by, dz = oz - bz;
         double ex = cx - bx, ey = cy - by, ez = cz - bz;
-        acc += go_pair_inv(recip64(ex * ex + ey * ey + ez * ez), r0, a.eps, a.sig2) -
-               go_pair_inv(recip64(dx * dx + dy * dy + dz * dz), r0, a.eps, a.sig2);
+        acc += go_pair_delta(ex * ex + ey * ey + ez * ez, dx * dx + dy * dy + dz * dz, r0, a.eps, a.sig2);
     }
     int i = k - 1 + (int)threadIdx.x;   // bonds (k-1, k) and (k, k+1), cbond.c:103-104 form
     if (threadIdx.x < 2 && i >= 0 && i + 1 < n) {
@@ -1146,7 +1165,7 @@ extern "C" int ttc_mc_destroy(ttc_mc *m) {
     if (!m) return TTC_OK;
     hipStreamSynchronize(m->stream);
     void *b[] = {m->pos, m->nbead, m->noff, m->npart, m->nr0, m->emin, m->ecur, m->lng, m->hist, m->rng, m->stats, m->mlp, m->adam,
-                 m->logx, m->logde, m->logn};
+                 m->logx, m->logde, m->logn, m->ck_pos, m->ck_ecur, m->ck_lng, m->ck_hist, m->ck_rng};
     for (void *x : b)
         if (x) hipFree(x);
     hipStreamDestroy(m->stream);
@@ -1273,6 +1292,24 @@ extern "C" int ttc_mc_run(ttc_mc *m, int ntrials, int mode, double lnf, int log)
 extern "C" int ttc_mc_last_time(ttc_mc *m, double *usec) {
     MCHECK(m && usec, TTC_ERR_ARG, "ttc_mc_last_time: bad argument");
     *usec = m->t_last;
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_checkpoint(ttc_mc *m, int restore) {
+    MCHECK(m && (restore == 0 || restore == 1), TTC_ERR_ARG, "ttc_mc_checkpoint: bad argument");
+    const size_t W = m->p.nwalk, N = m->p.maxn, B = m->p.nbins;
+    const size_t sz[5] = {sizeof(double) * W * N * 3, sizeof(double) * W, sizeof(double) * W * B,
+                          sizeof(long long) * W * B, sizeof(uint64_t) * W * MC_STREAMS * 4};
+    void *live[5] = {m->pos, m->ecur, m->lng, m->hist, m->rng};
+    void **copy[5] = {(void **)&m->ck_pos, (void **)&m->ck_ecur, (void **)&m->ck_lng, (void **)&m->ck_hist,
+                      (void **)&m->ck_rng};
+    MCHECK(!restore || m->ck_pos, TTC_ERR_ARG, "ttc_mc_checkpoint: no checkpoint taken");
+    for (int i = 0; i < 5; i++) {
+        if (!*copy[i]) HIPCHK(hipMalloc(copy[i], sz[i]));
+        if (restore) HIPCHK(hipMemcpyAsync(live[i], *copy[i], sz[i], hipMemcpyDeviceToDevice, m->stream));
+        else HIPCHK(hipMemcpyAsync(*copy[i], live[i], sz[i], hipMemcpyDeviceToDevice, m->stream));
+    }
+    HIPCHK(hipStreamSynchronize(m->stream));
     return TTC_OK;
 }
 

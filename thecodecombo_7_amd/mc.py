@@ -8,6 +8,7 @@ reference's MC operations:
     run(ntrials, mode, lnf)       pivot / displacement trials with Wang-Landau
                                   acceptance (pivot.c:236-253); mode 1 adds the
                                   learned first stage (delayed acceptance)
+    snapshot() / restore()        device copy of the sampling state
     flat_histogram(w)             the flatness test of ctdos.c:268
     train_surrogate(...)          Adam on the logged (move, dE) pairs
 """
@@ -45,6 +46,7 @@ def _declare(L):
     L.ttc_mc_energy.argtypes = [vp, dp]
     L.ttc_mc_delta_energy.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
     L.ttc_mc_run.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int]
+    L.ttc_mc_checkpoint.argtypes = [vp, C.c_int]
     L.ttc_mc_get_stats.argtypes = [vp, i64p]
     L.ttc_mc_reset_stats.argtypes = [vp]
     L.ttc_mc_get_dos.argtypes = [vp, C.c_int, dp, i64p]
@@ -133,6 +135,15 @@ class Walkers:
         t = np.zeros(1)
         T.check(self.L.ttc_mc_last_time(self.h, T.dptr(t)))
         return float(t[0])
+
+    def snapshot(self):
+        """Take a device copy of the sampling state (one slot per batch)."""
+        T.check(self.L.ttc_mc_checkpoint(self.h, 0))
+        return True
+
+    def restore(self, _snap=True):
+        """Put the state of the last snapshot() back."""
+        T.check(self.L.ttc_mc_checkpoint(self.h, 1))
 
     def stats(self):
         s = np.zeros(self.nwalk * NSTAT, np.int64)
