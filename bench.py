@@ -208,6 +208,69 @@ def large_leg(steps: int, cpu: bool):
     return out
 
 
+def replica_leg(steps: int, cpu: bool):
+    """64 replicas of 2,000-20,000 sites (log-uniform, seeded; 1 A PME grids),
+    Nose-Hoover NVT at temperatures geometric from 300 to 600 K, all stepped
+    together (ttc_batch_*, launch mode autotuned), exchanges attempted every
+    500 steps.  Aggregate ns/day = sum over replicas of simulated time per
+    day; step-0 force error of every replica against the FP64 oracle."""
+    from thecodecombo_7_amd import system
+    from thecodecombo_7_amd.replicas import ReplicaBatch
+    systems = system.make_replicas(64)
+    nrep = len(systems)
+    temps = 300.0 * 2.0 ** (np.arange(nrep) / (nrep - 1))
+    rb = ReplicaBatch(systems, temps, seed=5)
+    out = {"workload": "64 replicas, 2,000-20,000 sites (log-uniform), chain chemistry at the same density, "
+                       "PME grids at 1 A spacing, NHC NVT 300-600 K (geometric), exchanges every 500 steps",
+           "replicas": nrep, "sites": int(sum(x.n for x in systems))}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        errs, t_cpu = [], 0.0
+        for b, x in zip(rb.boxes, systems):
+            ff, _ = b.forces()
+            o = pyoracle.Oracle(x, fast=True)
+            t0 = time.perf_counter()
+            rc, cnt, lst, xf = o.nblist(x.pos, 1500, cells=True)
+            fr, _ = o.forces(x.pos, lists=(cnt, lst, xf))
+            t_cpu += time.perf_counter() - t0
+            if rc:
+                raise RuntimeError("oracle list overflow in the replica leg")
+            errs.append(float(np.sqrt(((ff - fr) ** 2).sum() / (fr ** 2).sum())))
+        out["force_rel_rms_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs)), "min": min(errs)}
+        out["cpu_baseline"] = {"value": round(nrep * 0.0864 / t_cpu, 5), "unit": "aggregate ns/day", "cores": 1,
+                               "kind": "port",
+                               "sample": "oracle (reference flags, 1 thread): cell-list build + one force "
+                                         "evaluation of every replica at step 0, summed, as one step of all "
+                                         "replicas (integration and thermostat not counted)"}
+    mode, spl, tt = rb.autotune(50)
+    rb.run(500, exchange_every=500)
+    t0 = time.perf_counter()
+    r = rb.run(steps, exchange_every=500)
+    wall = (time.perf_counter() - t0) / steps
+    out.update({"steps": steps, "launch_mode": int(mode), "steps_per_launch": int(spl),
+                "ms_per_step": round(1e3 * wall, 4),
+                "aggregate_ns_per_day": round(nrep * 0.0864 / wall, 2),
+                "site_ns_per_day": round(out["sites"] * 0.0864 / wall, 1),
+                "exchange_attempts": int(r["attempts"]), "exchange_accepts": int(r["accepts"])})
+    rb.close()
+    return out
+
+
+def nve_drift(box, s, steps: int, every: int):
+    """Energy conservation of NVE velocity Verlet: the conserved total
+    kinet + bond + nbonds (calcvalue.c:40-64, shifted nonbonded energy) every
+    `every` steps; drift = least-squares slope per DOF."""
+    rows, _ = box.integrate_vverlet(steps, energy_every=every)
+    t_ps = rows[:, 0] * s.dt
+    tot = rows[:, 1] + rows[:, 2] + rows[:, 4]
+    slope = np.polyfit(t_ps, tot, 1)[0] * 1e3          # kJ/mol per ns
+    dof = 3 * s.n - 3
+    return {"steps": steps, "rows": int(len(rows)), "total_kJ_mol": [round(float(tot[0]), 3), round(float(tot[-1]), 3)],
+            "drift_kJ_mol_per_ns_per_dof": float(slope / dof),
+            "rms_fluct_rel": float(np.std(tot) / abs(np.mean(tot)))}
+
+
 COPY_MIB = 512
 
 
@@ -288,6 +351,9 @@ def main():
     ap.add_argument("--mc-trials", type=int, default=8000, help="MC trials per walker and mode")
     ap.add_argument("--no-large", action="store_true", help="skip the chains-1M leg")
     ap.add_argument("--large-steps", type=int, default=200, help="timed steps of the chains-1M leg")
+    ap.add_argument("--no-replicas", action="store_true", help="skip the batched replica leg")
+    ap.add_argument("--replica-steps", type=int, default=1000, help="timed steps of the replica leg")
+    ap.add_argument("--drift-steps", type=int, default=5000, help="NVE steps of the energy-drift run (0: skip)")
     args = ap.parse_args()
     if args.traffic_summary:
         traffic_summary(*args.traffic_summary)
@@ -353,9 +419,15 @@ def main():
                      "algorithmic_bytes": alg_bytes, "half_list_entries": half_entries,
                      "spread_us": round(t_spread, 2), "step_us_events": round(t_event, 2)},
     }
+    if args.drift_steps > 0:
+        print("bench: NVE energy drift", file=sys.stderr, flush=True)
+        out["nve_drift"] = nve_drift(box, s, args.drift_steps, 50)
     if not args.no_large:
         print("bench: chains-1M leg", file=sys.stderr, flush=True)
         out["large"] = large_leg(args.large_steps, not args.no_cpu)
+    if not args.no_replicas:
+        print("bench: batched replica leg", file=sys.stderr, flush=True)
+        out["replicas"] = replica_leg(args.replica_steps, not args.no_cpu)
     if not args.no_mc:
         print("bench: Monte Carlo leg", file=sys.stderr, flush=True)
         out["mc"] = mc_leg(args.mc_trials, not args.no_cpu)
