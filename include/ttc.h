@@ -186,10 +186,13 @@ int  ttc_nblist_pivot(ttc_ctx *c, int ub);
 
 /* ---- Batched replicas (nvt_md.c + swap_box.c, one MPI process per
  * replica in the reference) ------------------------------------------------
- * A batch advances many contexts of any sizes together: their NHC step
- * graphs are captured into one graph (launch mode 0, one branch per
- * replica) or one graph per replica on its own stream (mode 1), with
- * steps_per_launch steps per launch.  Every context needs ttc_set_nhc.    */
+ * A batch advances many contexts of any sizes together.  Launch mode 2:
+ * every kernel of the NHC step is launched once for the whole batch, its
+ * grid the concatenation of the replicas' own grids (about 20 launches per
+ * step for any number of replicas); mode 0: the replicas' step graphs
+ * captured into one graph, one branch per replica; mode 1: one graph per
+ * replica on its own stream; steps_per_launch steps per graph launch.
+ * Every context needs ttc_set_nhc.                                         */
 typedef struct ttc_batch ttc_batch;
 int  ttc_batch_create(ttc_ctx *const *ctx, int nrep, uint64_t seed, ttc_batch **out);
 int  ttc_batch_destroy(ttc_batch *b);
@@ -206,8 +209,9 @@ int  ttc_batch_exchange(ttc_batch *b, const double *potens, int parity, int *acc
  * of replica r, wall ms per step of the last ttc_batch_md_nhc, totals.    */
 int  ttc_batch_state(ttc_batch *b, int *order, double *temps, double *ms_per_step,
                      long long *attempts, long long *accepts);
-/* Tries launch modes {0,1} x steps per launch {1,5,10,25} for trial_steps
- * steps each (real dynamics) and keeps the fastest; timings[8] optional. */
+/* Tries launch modes {0,1,2} x steps per launch {1,5,10,25} for
+ * trial_steps steps each (real dynamics) and keeps the fastest;
+ * timings[12] optional.                                                   */
 int  ttc_batch_autotune(ttc_batch *b, int trial_steps, int *best_mode, int *best_spl, double *timings);
 
 /* Average device time of the named kernel class over the last

@@ -3,7 +3,8 @@ each with its own PME grid and Nose-Hoover chain (nvt_md.c:528-531), run
 together, and the exchange rule of swap_box.c:7-11.
 
 * A batch step is the same computation as each replica's own NHC step
-  (ttc_md_nhc): trajectories must agree bit for bit, in both launch modes.
+  (ttc_md_nhc): trajectories must agree bit for bit, in every launch mode
+  (2: batched launches of every kernel over all replicas).
 * Per-replica step-0 forces against the FP64 oracle: relative RMS <= 5e-6.
 * Exchanges: a certain swap exchanges the two temperatures and rescales
   velocities by sqrt(T_new/T_old) exactly; an impossible one changes nothing.
@@ -32,7 +33,7 @@ def systems():
     return _systems()
 
 
-@pytest.mark.parametrize("mode,spl", [(0, 5), (1, 1)])
+@pytest.mark.parametrize("mode,spl", [(0, 5), (1, 1), (2, 1), (2, 4)])
 def test_batch_equals_single_contexts(systems, mode, spl):
     from thecodecombo_7_amd.nonbond import Box
     from thecodecombo_7_amd.replicas import KB, ReplicaBatch

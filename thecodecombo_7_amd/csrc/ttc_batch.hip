@@ -5,12 +5,19 @@
 // Nose-Hoover chain of integrate_nhc.c, exchanges by swap_box.c:4-86 /
 // swap_mpi.c).  Here every replica is a device context (its own list, PME
 // grid sized for its box, thermostat) and one call advances all of them:
-// the NHC step graphs of all replicas are captured together, either as one
-// graph whose branches are the replicas (launch mode 0: one launch per
-// `spl` steps for the whole batch, independent branches free to overlap on
-// the device) or as one graph per replica launched on the replica's own
-// stream (mode 1).  The launch mode and the steps per launch are chosen by
-// ttc_batch_autotune from measured step times.
+// the NHC step of all replicas is captured as a HIP graph, in one of three
+// launch modes:
+//   mode 2: batched launches -- every kernel of the step is launched once for
+//           the whole batch, its grid the concatenation of every replica's
+//           own grid (a block finds its replica and its block index in a
+//           device map and the replica's arguments in a device array, so it
+//           computes exactly what the replica's own launch would); about 20
+//           launches per step for any number of replicas;
+//   mode 0: one graph whose branches are the replicas' own step graphs (one
+//           launch per `spl` steps, independent branches free to overlap);
+//   mode 1: one graph per replica launched on the replica's own stream.
+// The launch mode and the steps per launch are chosen by ttc_batch_autotune
+// from measured step times.
 //
 // Exchanges: replicas holding neighbouring temperatures T_t < T_t+1 are
 // swapped with probability min(1, exp[(1/kT_t - 1/kT_t+1)(E_i - E_j)])
@@ -31,6 +38,53 @@ extern "C" int ttc_set_nhc_temperature(ttc_ctx *c, double T, double Q);
 extern "C" int ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3);
 extern "C" int ttc_scale_velocities(ttc_ctx *c, double factor);
 
+// batched stages of the NHC step (kernel files)
+int ttc_bstage_kinet(const std::vector<ttc_ctx *> &r, TtcStage *st);
+int ttc_bstage_nhc(const std::vector<ttc_ctx *> &r, TtcStage *st);
+int ttc_bstage_vscale(const std::vector<ttc_ctx *> &r, TtcStage *st);
+int ttc_bstage_kick(const std::vector<ttc_ctx *> &r, TtcStage *st);
+int ttc_bstage_reduce(const std::vector<ttc_ctx *> &r, TtcStage *st);
+int ttc_bstage_drift(const std::vector<ttc_ctx *> &r, TtcStage *st);
+int ttc_bstage_fft(const std::vector<ttc_ctx *> &r, TtcStage *st /* [5] */);
+int ttc_bstage_gather(const std::vector<ttc_ctx *> &r, TtcStage *st);
+int ttc_bstage_rebuild(const std::vector<ttc_ctx *> &r, TtcStage *st /* [4] */);
+int ttc_bstage_pair(const std::vector<ttc_ctx *> &r, TtcStage *st);
+void ttc_blaunch_kinet(const TtcStage &st, hipStream_t s);
+void ttc_blaunch_nhc(const TtcStage &st, hipStream_t s);
+void ttc_blaunch_vscale(const TtcStage &st, hipStream_t s);
+void ttc_blaunch_kick(const TtcStage &st, bool energy, hipStream_t s);
+void ttc_blaunch_reduce(const TtcStage &st, hipStream_t s);
+void ttc_blaunch_drift(const TtcStage &st, hipStream_t s);
+void ttc_blaunch_fft(const TtcStage *st, bool energy, hipStream_t s);
+void ttc_blaunch_gather(const TtcStage &st, bool energy, hipStream_t s);
+void ttc_blaunch_rebuild(const TtcStage *st, hipStream_t s);
+void ttc_blaunch_pair(const TtcStage &st, bool energy, hipStream_t s);
+
+int ttc_stage_upload(TtcStage *st, const void *packs, size_t pack_bytes, int nrep, const std::vector<int> &blocks,
+                     size_t shm) {
+    ttc_stage_free(st);
+    std::vector<TtcBmap> map;
+    for (int i = 0; i < nrep; i++)
+        for (int k = 0; k < blocks[i]; k++) map.push_back(TtcBmap{i, k});
+    TTC_HIP(hipMalloc(&st->pk, pack_bytes * nrep));
+    TTC_HIP(hipMalloc(&st->map, sizeof(TtcBmap) * std::max<size_t>(1, map.size())));
+    TTC_HIP(hipMemcpy(st->pk, packs, pack_bytes * nrep, hipMemcpyHostToDevice));
+    if (!map.empty()) TTC_HIP(hipMemcpy(st->map, map.data(), sizeof(TtcBmap) * map.size(), hipMemcpyHostToDevice));
+    st->nblk = (int)map.size();
+    st->shm = shm;
+    return TTC_OK;
+}
+
+void ttc_stage_free(TtcStage *st) {
+    if (st->pk) hipFree(st->pk);
+    if (st->map) hipFree(st->map);
+    *st = TtcStage{};
+}
+
+// stages of the batched NHC step
+enum { BS_KINET = 0, BS_NHC, BS_VSCALE, BS_DRIFT, BS_FFT, BS_REBUILD = BS_FFT + 5, BS_GATHER = BS_REBUILD + 4,
+       BS_PAIR, BS_KICK, BS_REDUCE, BS_N };
+
 struct ttc_batch {
     std::vector<ttc_ctx *> r;
     hipStream_t main = nullptr;
@@ -40,6 +94,10 @@ struct ttc_batch {
     int spl = 1;
     hipGraphExec_t g_plain = nullptr, g_energy = nullptr;   // mode 0
     std::vector<hipGraphExec_t> rp, re;                      // mode 1, per replica
+    hipStream_t bside = nullptr;                             // mode 2: the batch's side stream
+    hipEvent_t bev[3] = {nullptr, nullptr, nullptr};
+    TtcStage st[BS_N];                                       // mode 2 stages
+    bool st_valid = false;
     std::vector<unsigned> gen;                               // context generations at capture
     int cap_mode = -1, cap_spl = 0;
     std::vector<int> order;                                  // order[t]: replica at temperature rank t
@@ -134,8 +192,78 @@ int capture_one(ttc_batch *b, ttc_ctx *c, bool energy, hipGraphExec_t *out) {
     return TTC_OK;
 }
 
+void drop_stages(ttc_batch *b) {
+    for (TtcStage &x : b->st) ttc_stage_free(&x);
+    b->st_valid = false;
+}
+
+int build_stages(ttc_batch *b) {
+    drop_stages(b);
+    int r = ttc_bstage_kinet(b->r, &b->st[BS_KINET]);
+    if (!r) r = ttc_bstage_nhc(b->r, &b->st[BS_NHC]);
+    if (!r) r = ttc_bstage_vscale(b->r, &b->st[BS_VSCALE]);
+    if (!r) r = ttc_bstage_drift(b->r, &b->st[BS_DRIFT]);
+    if (!r) r = ttc_bstage_fft(b->r, &b->st[BS_FFT]);
+    if (!r) r = ttc_bstage_rebuild(b->r, &b->st[BS_REBUILD]);
+    if (!r) r = ttc_bstage_gather(b->r, &b->st[BS_GATHER]);
+    if (!r) r = ttc_bstage_pair(b->r, &b->st[BS_PAIR]);
+    if (!r) r = ttc_bstage_kick(b->r, &b->st[BS_KICK]);
+    if (!r) r = ttc_bstage_reduce(b->r, &b->st[BS_REDUCE]);
+    if (r) { drop_stages(b); return r; }
+    b->st_valid = true;
+    return TTC_OK;
+}
+
+// one graph of `spl` NHC steps of the whole batch as batched launches, the
+// same sequence as ttc_launch_nhc_step: thermostat half, kick + drift +
+// spread, FFTs (side stream) beside the conditional rebuild, gather (side)
+// beside the pair kernel, kick, thermostat half (+ energies)
+int capture_batched(ttc_batch *b, bool energy, hipGraphExec_t *out) {
+    hipGraph_t g = nullptr;
+    const TtcStage *st = b->st;
+    hipStream_t m = b->main, sd = b->bside;
+    TTC_HIP(hipStreamBeginCapture(m, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < b->spl && e == hipSuccess; s++) {
+        bool en = energy && s == b->spl - 1;
+        ttc_blaunch_kinet(st[BS_KINET], m);
+        ttc_blaunch_nhc(st[BS_NHC], m);
+        ttc_blaunch_vscale(st[BS_VSCALE], m);
+        ttc_blaunch_drift(st[BS_DRIFT], m);
+        e = hipEventRecord(b->bev[0], m);
+        if (e == hipSuccess) e = hipStreamWaitEvent(sd, b->bev[0], 0);
+        ttc_blaunch_fft(&st[BS_FFT], en, sd);
+        ttc_blaunch_rebuild(&st[BS_REBUILD], m);
+        if (e == hipSuccess) e = hipEventRecord(b->bev[1], m);
+        if (e == hipSuccess) e = hipStreamWaitEvent(sd, b->bev[1], 0);
+        ttc_blaunch_gather(st[BS_GATHER], en, sd);
+        ttc_blaunch_pair(st[BS_PAIR], en, m);
+        if (e == hipSuccess) e = hipEventRecord(b->bev[2], sd);
+        if (e == hipSuccess) e = hipStreamWaitEvent(m, b->bev[2], 0);
+        ttc_blaunch_kick(st[BS_KICK], true, m);   // kinetic partials for the second thermostat half
+        ttc_blaunch_nhc(st[BS_NHC], m);
+        ttc_blaunch_vscale(st[BS_VSCALE], m);
+        if (en) {
+            ttc_blaunch_kinet(st[BS_KINET], m);
+            ttc_blaunch_reduce(st[BS_REDUCE], m);
+        }
+        if (e == hipSuccess) e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamEndCapture(m, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        ttc_set_error(std::string("ttc_batch: capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        return TTC_ERR_HIP;
+    }
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    TTC_HIP(e);
+    return TTC_OK;
+}
+
 int sync_all(ttc_batch *b) {
     TTC_HIP(hipStreamSynchronize(b->main));
+    if (b->bside) TTC_HIP(hipStreamSynchronize(b->bside));
     for (ttc_ctx *c : b->r) TTC_HIP(hipStreamSynchronize(c->stream));
     return TTC_OK;
 }
@@ -147,7 +275,11 @@ int ensure_graphs(ttc_batch *b) {
     drop_graphs(b);
     int r = sync_all(b);   // capture starts from idle streams
     if (r) return r;
-    if (b->mode == 0) {
+    if (b->mode == 2) {
+        r = build_stages(b);
+        if (!r) r = capture_batched(b, false, &b->g_plain);
+        if (!r) r = capture_batched(b, true, &b->g_energy);
+    } else if (b->mode == 0) {
         r = capture_all(b, false, &b->g_plain);
         if (!r) r = capture_all(b, true, &b->g_energy);
     } else {
@@ -177,6 +309,8 @@ extern "C" int ttc_batch_create(ttc_ctx *const *ctx, int nrep, uint64_t seed, tt
     b->gen.assign(nrep, ~0u);
     b->join.assign(nrep, nullptr);
     hipError_t e = hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->bside, hipStreamNonBlocking);
+    for (int k = 0; k < 3 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&b->bev[k], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
     for (int i = 0; i < nrep && e == hipSuccess; i++) e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming);
     if (e != hipSuccess) {
@@ -197,17 +331,21 @@ extern "C" int ttc_batch_create(ttc_ctx *const *ctx, int nrep, uint64_t seed, tt
 extern "C" int ttc_batch_destroy(ttc_batch *b) {
     if (!b) return TTC_OK;
     drop_graphs(b);
+    drop_stages(b);
     for (hipEvent_t e : b->join)
         if (e) hipEventDestroy(e);
+    for (hipEvent_t e : b->bev)
+        if (e) hipEventDestroy(e);
     if (b->fork) hipEventDestroy(b->fork);
+    if (b->bside) hipStreamDestroy(b->bside);
     if (b->main) hipStreamDestroy(b->main);
     delete b;
     return TTC_OK;
 }
 
 extern "C" int ttc_batch_set_launch(ttc_batch *b, int mode, int steps_per_launch) {
-    BCHECK(b && (mode == 0 || mode == 1) && steps_per_launch >= 1 && steps_per_launch <= 1000, TTC_ERR_ARG,
-           "ttc_batch_set_launch: mode 0/1, 1 <= steps_per_launch <= 1000");
+    BCHECK(b && mode >= 0 && mode <= 2 && steps_per_launch >= 1 && steps_per_launch <= 1000, TTC_ERR_ARG,
+           "ttc_batch_set_launch: mode 0/1/2, 1 <= steps_per_launch <= 1000");
     b->mode = mode;
     b->spl = steps_per_launch;
     return TTC_OK;
@@ -233,7 +371,7 @@ extern "C" int ttc_batch_md_nhc(ttc_batch *b, int nsteps, double *potens, int *r
     int nl = nsteps / b->spl;
     for (int l = 0; l < nl; l++) {
         bool en = l == nl - 1;
-        if (b->mode == 0) {
+        if (b->mode == 0 || b->mode == 2) {
             TTC_HIP(hipGraphLaunch(en ? b->g_energy : b->g_plain, b->main));
         } else {
             for (size_t i = 0; i < R; i++) TTC_HIP(hipGraphLaunch(en ? b->re[i] : b->rp[i], b->r[i]->stream));
@@ -302,11 +440,12 @@ extern "C" int ttc_batch_state(ttc_batch *b, int *order, double *temps, double *
     return TTC_OK;
 }
 
-// First tunes each replica's pair-kernel shape (ttc_tune), then tries
+// First tunes each replica's pair-kernel shape (ttc_tune; the batched
+// launches of mode 2 use 8 waves per group for every replica), then tries
 // every (launch mode, steps per launch) candidate for `trial_steps` steps
 // (the steps are real dynamics: they count as equilibration) and keeps the
 // fastest; the measured ms/step of each candidate is returned in
-// timings[8] if given.
+// timings[12] if given.
 extern "C" int ttc_batch_autotune(ttc_batch *b, int trial_steps, int *best_mode, int *best_spl, double *timings) {
     BCHECK(b && trial_steps >= 1, TTC_ERR_ARG, "ttc_batch_autotune: bad argument");
     for (ttc_ctx *c : b->r) {
@@ -317,7 +456,7 @@ extern "C" int ttc_batch_autotune(ttc_batch *b, int trial_steps, int *best_mode,
     const int spls[] = {1, 5, 10, 25};
     double best = 1e300;
     int bm = 0, bs = 1, q = 0;
-    for (int mode = 0; mode < 2; mode++)
+    for (int mode = 0; mode < 3; mode++)
         for (int spl : spls) {
             b->mode = mode;
             b->spl = spl;

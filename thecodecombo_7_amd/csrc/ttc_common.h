@@ -287,3 +287,27 @@ __device__ __forceinline__ double ttc_wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+
+// ---- batched launches ------------------------------------------------------
+// One grid covering the blocks of many contexts: block b runs block
+// map[b].blk of context map[b].rep's own grid with that context's argument
+// pack pk[map[b].rep], so a batched launch computes exactly what launching
+// every context's kernel on its own would.  Stages are built once per set of
+// contexts (ttc_stage_upload) and launched by the kernel files' ttc_blaunch_*.
+struct TtcBmap {
+    int rep, blk;
+};
+struct TtcStage {
+    void *pk = nullptr;        // device array of per-context argument packs
+    TtcBmap *map = nullptr;    // device array, one entry per block
+    int nblk = 0;
+    size_t shm = 0;            // dynamic LDS per block (largest over the contexts)
+};
+int ttc_stage_upload(TtcStage *st, const void *packs, size_t pack_bytes, int nrep, const std::vector<int> &blocks,
+                     size_t shm);
+void ttc_stage_free(TtcStage *st);
+template <typename Pack>
+int ttc_stage_make(TtcStage *st, const std::vector<Pack> &packs, const std::vector<int> &blocks, size_t shm) {
+    return ttc_stage_upload(st, packs.data(), sizeof(Pack), (int)packs.size(), blocks, shm);
+}
+

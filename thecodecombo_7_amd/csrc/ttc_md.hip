@@ -52,7 +52,7 @@ k_combine(int n, int mask, const double4 *fpair, const double4 *g4, double4 *f4,
 }
 
 // kinet.c:18-68 with uu = vv: ken = 0.125 m (v+u)^2 * NA*AU*10
-__device__ __forceinline__ void ken_block(double m, double4 v, double *partials, int maxb, bool valid) {
+__device__ __forceinline__ void ken_block(int bid, double m, double4 v, double *partials, int maxb, bool valid) {
     double k[6] = {0, 0, 0, 0, 0, 0};
     if (valid) {
         double sx = v.x + v.x, sy = v.y + v.y, sz = v.z + v.z;
@@ -68,7 +68,7 @@ __device__ __forceinline__ void ken_block(double m, double4 v, double *partials,
     __syncthreads();
     if (threadIdx.x < 6) {
         double t = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-        partials[(size_t)(EV_KE0 + threadIdx.x) * maxb + blockIdx.x] = t * (0.125 * (TTC_NA * TTC_AU * 10.0));
+        partials[(size_t)(EV_KE0 + threadIdx.x) * maxb + bid] = t * (0.125 * (TTC_NA * TTC_AU * 10.0));
     }
 }
 
@@ -76,12 +76,12 @@ __device__ __forceinline__ void ken_block(double m, double4 v, double *partials,
 // DLIST flag this step's drift set has been consumed by this step's
 // conditional rebuild: it is cleared here.
 template <bool ENERGY>
-__global__ void __launch_bounds__(256)
-k_kick(int n, double4 *vel4, const double4 *fpair, const double4 *g4, double4 *f4, double dt, double vscal,
-       double *partials, int maxb, int nbg, int *status) {
+__device__ __forceinline__ void kick_body(int bid, int n, double4 *vel4, const double4 *fpair, const double4 *g4,
+                                          double4 *f4, double dt, double vscal, double *partials, int maxb, int nbg,
+                                          int *status) {
     __shared__ double mean[3];
     mean_pme(partials, maxb, nbg, n, mean);
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = bid * blockDim.x + threadIdx.x;
     if (s == 0) {
         status[ST_STEP] += 1;
         status[ST_NLFLAG] = 0;
@@ -98,15 +98,50 @@ k_kick(int n, double4 *vel4, const double4 *fpair, const double4 *g4, double4 *f
         v.z = v.z + vscal * dth * v.w * f.z;
         vel4[s] = v;
     }
-    if (ENERGY) ken_block(100.0 / v.w, v, partials, maxb, ok);
+    if (ENERGY) ken_block(bid, 100.0 / v.w, v, partials, maxb, ok);
 }
 
-__global__ void __launch_bounds__(256)
-k_kinet(int n, const double4 *vel4, double *partials, int maxb) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+struct KickPack {
+    int n;
+    double4 *vel4;
+    const double4 *fpair, *g4;
+    double4 *f4;
+    double dt, vscal;
+    double *partials;
+    int maxb, nbg;
+    int *status;
+};
+
+template <bool ENERGY>
+__global__ void __launch_bounds__(256) k_kick(KickPack a) {
+    kick_body<ENERGY>(blockIdx.x, a.n, a.vel4, a.fpair, a.g4, a.f4, a.dt, a.vscal, a.partials, a.maxb, a.nbg, a.status);
+}
+template <bool ENERGY>
+__global__ void __launch_bounds__(256) kb_kick(const KickPack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    const KickPack a = pk[m.rep];
+    kick_body<ENERGY>(m.blk, a.n, a.vel4, a.fpair, a.g4, a.f4, a.dt, a.vscal, a.partials, a.maxb, a.nbg, a.status);
+}
+
+__device__ __forceinline__ void kinet_body(int bid, int n, const double4 *vel4, double *partials, int maxb) {
+    int s = bid * blockDim.x + threadIdx.x;
     bool ok = s < n;
     double4 v = ok ? vel4[s] : make_double4(0, 0, 0, 1);
-    ken_block(100.0 / v.w, v, partials, maxb, ok);
+    ken_block(bid, 100.0 / v.w, v, partials, maxb, ok);
+}
+
+struct KinetPack {
+    int n;
+    const double4 *vel4;
+    double *partials;
+    int maxb;
+};
+
+__global__ void __launch_bounds__(256) k_kinet(KinetPack a) { kinet_body(blockIdx.x, a.n, a.vel4, a.partials, a.maxb); }
+__global__ void __launch_bounds__(256) kb_kinet(const KinetPack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    const KinetPack a = pk[m.rep];
+    kinet_body(m.blk, a.n, a.vel4, a.partials, a.maxb);
 }
 
 struct ReduceArgs {
@@ -116,9 +151,9 @@ struct ReduceArgs {
 // one workgroup: every slot summed over its producer's blocks in a fixed
 // order; optionally appends an energy row
 // {step, kinet, bond, nbond, nbonds, coulomb, kewald, rewald, iewald}
-__global__ void __launch_bounds__(256)
-k_reduce(ReduceArgs ra, const double *partials, int maxb, double *ev, double e_self, double fan,
-         double *rows, int rows_cap, int *status, int append) {
+__device__ __forceinline__ void reduce_body(const ReduceArgs &ra, const double *partials, int maxb, double *ev,
+                                            double e_self, double fan, double *rows, int rows_cap, int *status,
+                                            int append) {
     __shared__ double red[4];
     for (int slot = 0; slot < EV_N; slot++) {
         double t = 0.0;
@@ -157,6 +192,26 @@ k_reduce(ReduceArgs ra, const double *partials, int maxb, double *ev, double e_s
     }
 }
 
+struct ReducePack {
+    ReduceArgs ra;
+    const double *partials;
+    int maxb;
+    double *ev;
+    double e_self, fan;
+    double *rows;
+    int rows_cap;
+    int *status;
+    int append;
+};
+
+__global__ void __launch_bounds__(256) k_reduce(ReducePack a) {
+    reduce_body(a.ra, a.partials, a.maxb, a.ev, a.e_self, a.fan, a.rows, a.rows_cap, a.status, a.append);
+}
+__global__ void __launch_bounds__(256) kb_reduce(const ReducePack *pk, const TtcBmap *map) {
+    const ReducePack &a = pk[map[blockIdx.x].rep];
+    reduce_body(a.ra, a.partials, a.maxb, a.ev, a.e_self, a.fan, a.rows, a.rows_cap, a.status, a.append);
+}
+
 }  // namespace
 
 int ttc_kick_blocks(ttc_ctx *c) { return (c->n + 255) / 256; }
@@ -172,21 +227,24 @@ static double *check_consts(ttc_ctx *c) {
     return c->ev + EV_N;   // four doubles stored after the reduced slots
 }
 
+static KickPack kick_pack(ttc_ctx *c, double vscal) {
+    return KickPack{c->n, c->vel4, c->fpair, c->g4, c->f4, c->p.dt, vscal, c->partials, c->nblk_max,
+                    ttc_gather_blocks(c), c->status};
+}
+
 void ttc_launch_kick(ttc_ctx *c, double vscal, bool energy) {
     int blocks = ttc_kick_blocks(c);
-    if (energy)
-        k_kick<true><<<blocks, 256, 0, c->stream>>>(c->n, c->vel4, c->fpair, c->g4, c->f4, c->p.dt, vscal,
-                                                    c->partials, c->nblk_max, ttc_gather_blocks(c), c->status);
-    else
-        k_kick<false><<<blocks, 256, 0, c->stream>>>(c->n, c->vel4, c->fpair, c->g4, c->f4, c->p.dt, vscal,
-                                                     c->partials, c->nblk_max, ttc_gather_blocks(c), c->status);
+    if (energy) k_kick<true><<<blocks, 256, 0, c->stream>>>(kick_pack(c, vscal));
+    else k_kick<false><<<blocks, 256, 0, c->stream>>>(kick_pack(c, vscal));
 }
+
+static KinetPack kinet_pack(ttc_ctx *c) { return KinetPack{c->n, c->vel4, c->partials, c->nblk_max}; }
 
 void ttc_launch_kinet(ttc_ctx *c) {
-    k_kinet<<<ttc_kick_blocks(c), 256, 0, c->stream>>>(c->n, c->vel4, c->partials, c->nblk_max);
+    k_kinet<<<ttc_kick_blocks(c), 256, 0, c->stream>>>(kinet_pack(c));
 }
 
-void ttc_launch_reduce(ttc_ctx *c, bool append_row) {
+static ReducePack reduce_pack(ttc_ctx *c, bool append_row) {
     ReduceArgs ra;
     for (int i = 0; i < EV_N; i++) ra.nb[i] = 0;
     for (int i = EV_LJ; i <= EV_PV5; i++) ra.nb[i] = ttc_pair_blocks(c);
@@ -197,8 +255,12 @@ void ttc_launch_reduce(ttc_ctx *c, bool append_row) {
     for (int i = EV_PHX; i <= EV_PHZ; i++) ra.nb[i] = ttc_pair_blocks(c);
     for (int i = EV_KHX; i <= EV_BHZ; i++) ra.nb[i] = ttc_gather_blocks(c);
     double fan = 0.5 * TTC_QFACTOR / c->p.eps_rf;
-    k_reduce<<<1, 256, 0, c->stream>>>(ra, c->partials, c->nblk_max, c->ev, c->e_self, fan, c->rows, c->rows_cap,
-                                       c->status, append_row ? 1 : 0);
+    return ReducePack{ra, c->partials, c->nblk_max, c->ev, c->e_self, fan, c->rows, c->rows_cap, c->status,
+                      append_row ? 1 : 0};
+}
+
+void ttc_launch_reduce(ttc_ctx *c, bool append_row) {
+    k_reduce<<<1, 256, 0, c->stream>>>(reduce_pack(c, append_row));
 }
 
 // ---------------------------------------------------------------------------
@@ -246,7 +308,7 @@ k_vkick(int n, double4 *vel4, const double4 *F, double A, double B, int check, c
                 atomicOr(&status[ST_NLFLAG], 1);
         }
     }
-    if (ke) ken_block(100.0 / v.w, v, partials, maxb, ok);
+    if (ke) ken_block(blockIdx.x, 100.0 / v.w, v, partials, maxb, ok);
 }
 
 __device__ __forceinline__ double pbc1m(double x, double l) {
@@ -281,7 +343,7 @@ struct NhcArgs {
 // integrate_nhc.c:40-161 on one lane: kinetic energy from the kick/kinet
 // per-block partials (fixed order), chain update, velocity scale ->
 // st[TTC_NHC_SCALE]; target T and mass Q are read from st[TTC_NHC_T/Q]
-__global__ void k_nhc(NhcArgs a, const double *partials, int maxb, double *st) {
+__device__ __forceinline__ void nhc_body(const NhcArgs &a, const double *partials, int maxb, double *st) {
     __shared__ double kin;
     if (threadIdx.x < 64) {
         double t = 0.0;
@@ -324,14 +386,39 @@ __global__ void k_nhc(NhcArgs a, const double *partials, int maxb, double *st) {
     st[TTC_NHC_SCALE] = scale;
 }
 
-__global__ void __launch_bounds__(256)
-k_vscale(int n, double4 *vel4, const double *scale) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+struct NhcPack {
+    NhcArgs a;
+    const double *partials;
+    int maxb;
+    double *st;
+};
+
+__global__ void k_nhc(NhcPack p) { nhc_body(p.a, p.partials, p.maxb, p.st); }
+__global__ void kb_nhc(const NhcPack *pk, const TtcBmap *map) {
+    const NhcPack &p = pk[map[blockIdx.x].rep];
+    nhc_body(p.a, p.partials, p.maxb, p.st);
+}
+
+__device__ __forceinline__ void vscale_body(int bid, int n, double4 *vel4, const double *scale) {
+    int s = bid * blockDim.x + threadIdx.x;
     if (s >= n) return;
     double sc = *scale;
     double4 v = vel4[s];
     v.x *= sc; v.y *= sc; v.z *= sc;
     vel4[s] = v;
+}
+
+struct VscalePack {
+    int n;
+    double4 *vel4;
+    const double *scale;
+};
+
+__global__ void __launch_bounds__(256) k_vscale(VscalePack a) { vscale_body(blockIdx.x, a.n, a.vel4, a.scale); }
+__global__ void __launch_bounds__(256) kb_vscale(const VscalePack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    const VscalePack a = pk[m.rep];
+    vscale_body(m.blk, a.n, a.vel4, a.scale);
 }
 
 }  // namespace
@@ -352,18 +439,81 @@ void ttc_launch_vkick_drift(ttc_ctx *c, const double4 *F, double A, double B, do
                                                              c->p.lz);
 }
 
+static NhcPack nhc_pack(ttc_ctx *c) {
+    NhcPack p;
+    p.a.nfree = c->nhc_nfree; p.a.dt = c->p.dt;
+    for (int i = 0; i < 8; i++) p.a.w[i] = c->nhc_w[i];
+    p.a.M = c->nhc_M; p.a.Nc = c->nhc_Nc; p.a.Nys = c->nhc_Nys; p.a.nbk = ttc_kick_blocks(c);
+    p.partials = c->partials;
+    p.maxb = c->nblk_max;
+    p.st = c->nhc;
+    return p;
+}
+
 void ttc_launch_nhc(ttc_ctx *c) {
-    NhcArgs a;
-    a.nfree = c->nhc_nfree; a.dt = c->p.dt;
-    for (int i = 0; i < 8; i++) a.w[i] = c->nhc_w[i];
-    a.M = c->nhc_M; a.Nc = c->nhc_Nc; a.Nys = c->nhc_Nys; a.nbk = ttc_kick_blocks(c);
-    k_nhc<<<1, 64, 0, c->stream>>>(a, c->partials, c->nblk_max, c->nhc);
+    k_nhc<<<1, 64, 0, c->stream>>>(nhc_pack(c));
 }
 
 void ttc_launch_vscale(ttc_ctx *c) {
-    k_vscale<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, c->vel4, c->nhc + TTC_NHC_SCALE);
+    k_vscale<<<(c->n + 255) / 256, 256, 0, c->stream>>>(VscalePack{c->n, c->vel4, c->nhc + TTC_NHC_SCALE});
 }
 
 void ttc_launch_vmul(ttc_ctx *c) {
-    k_vscale<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, c->vel4, c->nhc + TTC_NHC_VMUL);
+    k_vscale<<<(c->n + 255) / 256, 256, 0, c->stream>>>(VscalePack{c->n, c->vel4, c->nhc + TTC_NHC_VMUL});
+}
+
+// ---- batched stages of the NHC step (ttc_batch.hip) ------------------------
+
+int ttc_bstage_kinet(const std::vector<ttc_ctx *> &r, TtcStage *st) {
+    std::vector<KinetPack> pk;
+    std::vector<int> nb;
+    for (ttc_ctx *c : r) { pk.push_back(kinet_pack(c)); nb.push_back(ttc_kick_blocks(c)); }
+    return ttc_stage_make(st, pk, nb, 0);
+}
+void ttc_blaunch_kinet(const TtcStage &st, hipStream_t s) {
+    kb_kinet<<<st.nblk, 256, 0, s>>>((const KinetPack *)st.pk, st.map);
+}
+
+int ttc_bstage_kick(const std::vector<ttc_ctx *> &r, TtcStage *st) {
+    std::vector<KickPack> pk;
+    std::vector<int> nb;
+    for (ttc_ctx *c : r) { pk.push_back(kick_pack(c, 1.0)); nb.push_back(ttc_kick_blocks(c)); }
+    return ttc_stage_make(st, pk, nb, 0);
+}
+void ttc_blaunch_kick(const TtcStage &st, bool energy, hipStream_t s) {
+    if (energy) kb_kick<true><<<st.nblk, 256, 0, s>>>((const KickPack *)st.pk, st.map);
+    else kb_kick<false><<<st.nblk, 256, 0, s>>>((const KickPack *)st.pk, st.map);
+}
+
+int ttc_bstage_nhc(const std::vector<ttc_ctx *> &r, TtcStage *st) {
+    std::vector<NhcPack> pk;
+    std::vector<int> nb;
+    for (ttc_ctx *c : r) { pk.push_back(nhc_pack(c)); nb.push_back(1); }
+    return ttc_stage_make(st, pk, nb, 0);
+}
+void ttc_blaunch_nhc(const TtcStage &st, hipStream_t s) {
+    kb_nhc<<<st.nblk, 64, 0, s>>>((const NhcPack *)st.pk, st.map);
+}
+
+int ttc_bstage_vscale(const std::vector<ttc_ctx *> &r, TtcStage *st) {
+    std::vector<VscalePack> pk;
+    std::vector<int> nb;
+    for (ttc_ctx *c : r) {
+        pk.push_back(VscalePack{c->n, c->vel4, c->nhc + TTC_NHC_SCALE});
+        nb.push_back((c->n + 255) / 256);
+    }
+    return ttc_stage_make(st, pk, nb, 0);
+}
+void ttc_blaunch_vscale(const TtcStage &st, hipStream_t s) {
+    kb_vscale<<<st.nblk, 256, 0, s>>>((const VscalePack *)st.pk, st.map);
+}
+
+int ttc_bstage_reduce(const std::vector<ttc_ctx *> &r, TtcStage *st) {
+    std::vector<ReducePack> pk;
+    std::vector<int> nb;
+    for (ttc_ctx *c : r) { pk.push_back(reduce_pack(c, true)); nb.push_back(1); }
+    return ttc_stage_make(st, pk, nb, 0);
+}
+void ttc_blaunch_reduce(const TtcStage &st, hipStream_t s) {
+    kb_reduce<<<st.nblk, 256, 0, s>>>((const ReducePack *)st.pk, st.map);
 }

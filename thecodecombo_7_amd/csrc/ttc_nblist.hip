@@ -32,10 +32,11 @@ __device__ __forceinline__ bool rebuild_off(const int *status, int conditional) 
     return conditional && status[ST_NLFLAG] == 0;
 }
 
-__global__ void k_cell_assign(int n, const double4 *__restrict__ pos4, CellGeom g, int cellcap,
-                              int *cell_acc, int *cell_of, int *members, int *status, int conditional) {
+__device__ __forceinline__ void cell_assign_body(int bid, int n, const double4 *__restrict__ pos4,
+                                                 const CellGeom &g, int cellcap, int *cell_acc, int *cell_of,
+                                                 int *members, int *status, int conditional) {
     if (rebuild_off(status, conditional)) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 p = pos4[i];
     int cx = cell_coord(p.x, g.hx, g.lx, g.ncx);
@@ -48,9 +49,28 @@ __global__ void k_cell_assign(int n, const double4 *__restrict__ pos4, CellGeom 
     else atomicOr(&status[ST_OVERFLOW], 2);
 }
 
+struct CellAssignPack {
+    int n;
+    const double4 *pos4;
+    CellGeom g;
+    int cellcap;
+    int *cell_acc, *cell_of, *members, *status;
+    int conditional;
+};
+
+__global__ void k_cell_assign(CellAssignPack a) {
+    cell_assign_body(blockIdx.x, a.n, a.pos4, a.g, a.cellcap, a.cell_acc, a.cell_of, a.members, a.status,
+                     a.conditional);
+}
+__global__ void kb_cell_assign(const CellAssignPack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    const CellAssignPack a = pk[m.rep];
+    cell_assign_body(m.blk, a.n, a.pos4, a.g, a.cellcap, a.cell_acc, a.cell_of, a.members, a.status, a.conditional);
+}
+
 // one workgroup of 1024 threads; counts -> start, cell_acc reset to zero
-__global__ void k_cell_scan(int ncell, int *cell_acc, int *cell_cnt, int *cell_start, int *status,
-                            int conditional) {
+__device__ __forceinline__ void cell_scan_body(int ncell, int *cell_acc, int *cell_cnt, int *cell_start,
+                                               int *status, int conditional) {
     if (rebuild_off(status, conditional)) return;
     __shared__ int sh[1024];
     __shared__ int carry;
@@ -81,14 +101,44 @@ __global__ void k_cell_scan(int ncell, int *cell_acc, int *cell_cnt, int *cell_s
     }
 }
 
-__global__ void k_reorder(int n, int cellcap, const int *cell_of, const int *cell_cnt,
-                          const int *cell_start, const int *members, const int *orig,
-                          const double4 *pos4, const double4 *vel4, const double4 *f4, const int *type,
-                          double4 *pos4b, double4 *vel4b, double4 *f4b, int *typeb, int *origb,
-                          const double4 *fs, const double4 *fl, double4 *fsb, double4 *flb,
-                          float4 *posfb, int *inv, int *status, int conditional) {
-    if (rebuild_off(status, conditional)) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+struct CellScanPack {
+    int ncell;
+    int *cell_acc, *cell_cnt, *cell_start, *status;
+    int conditional;
+};
+
+__global__ void k_cell_scan(CellScanPack a) {
+    cell_scan_body(a.ncell, a.cell_acc, a.cell_cnt, a.cell_start, a.status, a.conditional);
+}
+__global__ void kb_cell_scan(const CellScanPack *pk, const TtcBmap *map) {
+    const CellScanPack a = pk[map[blockIdx.x].rep];
+    cell_scan_body(a.ncell, a.cell_acc, a.cell_cnt, a.cell_start, a.status, a.conditional);
+}
+
+struct ReorderPack {
+    int n, cellcap;
+    const int *cell_of, *cell_cnt, *cell_start, *members, *orig;
+    const double4 *pos4, *vel4, *f4;
+    const int *type;
+    double4 *pos4b, *vel4b, *f4b;
+    int *typeb, *origb;
+    const double4 *fs, *fl;
+    double4 *fsb, *flb;
+    float4 *posfb;
+    int *inv, *status;
+    int conditional;
+};
+
+__device__ __forceinline__ void reorder_body(int bid, const ReorderPack &a) {
+    const int n = a.n, cellcap = a.cellcap;
+    const int *cell_of = a.cell_of, *cell_cnt = a.cell_cnt, *cell_start = a.cell_start, *members = a.members,
+              *orig = a.orig, *type = a.type;
+    const double4 *pos4 = a.pos4, *vel4 = a.vel4, *f4 = a.f4, *fs = a.fs, *fl = a.fl;
+    double4 *pos4b = a.pos4b, *vel4b = a.vel4b, *f4b = a.f4b, *fsb = a.fsb, *flb = a.flb;
+    int *typeb = a.typeb, *origb = a.origb, *inv = a.inv;
+    float4 *posfb = a.posfb;
+    if (rebuild_off(a.status, a.conditional)) return;
+    int i = bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int c = cell_of[i];
     int m = min(cell_cnt[c], cellcap);
@@ -107,6 +157,13 @@ __global__ void k_reorder(int n, int cellcap, const int *cell_of, const int *cel
     typeb[s] = type[i];
     origb[s] = o;
     inv[o] = s;
+}
+
+__global__ void k_reorder(ReorderPack a) { reorder_body(blockIdx.x, a); }
+__global__ void kb_reorder(const ReorderPack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    const ReorderPack a = pk[m.rep];
+    reorder_body(m.blk, a);
 }
 
 #define BUILD_WAVES 8
@@ -379,14 +436,14 @@ __device__ int row_ranges(int r, const AxisRange &ay, const AxisRange &az, float
     return chunks;
 }
 
-__global__ void __launch_bounds__(64 * GB_WAVES)
-k_build_grp(int n, int cap, GbGeom g, const float4 *__restrict__ posfb, const double4 *__restrict__ pos4b,
-            const double4 *__restrict__ vel4b, const double4 *__restrict__ f4b, const int *__restrict__ typeb,
-            const int *__restrict__ origb, double4 *pos4, double4 *vel4, double4 *f4, int *type, int *orig,
-            double4 *anchor, const double4 *__restrict__ fsb, const double4 *__restrict__ flb, double4 *fs,
-            double4 *fl, const int *__restrict__ cell_cnt, const int *__restrict__ cell_start,
-            const int *__restrict__ ex_off, const int *__restrict__ ex_part, const int *__restrict__ ex_flag,
-            int ex_span, uint32_t *nbl, int *glen, int *gcnt, int *status, int conditional) {
+__device__ __forceinline__ void build_grp_body(
+    int grp, int n, int cap, const GbGeom &g, const float4 *__restrict__ posfb, const double4 *__restrict__ pos4b,
+    const double4 *__restrict__ vel4b, const double4 *__restrict__ f4b, const int *__restrict__ typeb,
+    const int *__restrict__ origb, double4 *pos4, double4 *vel4, double4 *f4, int *type, int *orig, double4 *anchor,
+    const double4 *__restrict__ fsb, const double4 *__restrict__ flb, double4 *fs, double4 *fl,
+    const int *__restrict__ cell_cnt, const int *__restrict__ cell_start, const int *__restrict__ ex_off,
+    const int *__restrict__ ex_part, const int *__restrict__ ex_flag, int ex_span, uint32_t *nbl, int *glen,
+    int *gcnt, int *status, int conditional) {
     if (rebuild_off(status, conditional)) return;
     __shared__ int2 sch[GB_MAXCH];
     __shared__ float4 sst[GB_BATCH * GB_C];   // chunk sites: x, y, z in the frame, original index
@@ -398,7 +455,6 @@ k_build_grp(int n, int cap, GbGeom g, const float4 *__restrict__ posfb, const do
     __shared__ int sxo[64][GB_EXMAX];
     __shared__ int sxc[64][GB_EXMAX];
     __shared__ int sxn[64];
-    int grp = blockIdx.x;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int sb = grp * 64;
     if (threadIdx.x < 64) {
@@ -591,6 +647,36 @@ k_build_grp(int n, int cap, GbGeom g, const float4 *__restrict__ posfb, const do
     }
 }
 
+struct BuildGrpPack {
+    int n, cap;
+    GbGeom g;
+    const float4 *posfb;
+    const double4 *pos4b, *vel4b, *f4b;
+    const int *typeb, *origb;
+    double4 *pos4, *vel4, *f4;
+    int *type, *orig;
+    double4 *anchor;
+    const double4 *fsb, *flb;
+    double4 *fs, *fl;
+    const int *cell_cnt, *cell_start, *ex_off, *ex_part, *ex_flag;
+    int ex_span;
+    uint32_t *nbl;
+    int *glen, *gcnt, *status;
+    int conditional;
+};
+
+__device__ __forceinline__ void build_grp_pack(int grp, const BuildGrpPack &a) {
+    build_grp_body(grp, a.n, a.cap, a.g, a.posfb, a.pos4b, a.vel4b, a.f4b, a.typeb, a.origb, a.pos4, a.vel4, a.f4,
+                   a.type, a.orig, a.anchor, a.fsb, a.flb, a.fs, a.fl, a.cell_cnt, a.cell_start, a.ex_off, a.ex_part,
+                   a.ex_flag, a.ex_span, a.nbl, a.glen, a.gcnt, a.status, a.conditional);
+}
+__global__ void __launch_bounds__(64 * GB_WAVES) k_build_grp(BuildGrpPack a) { build_grp_pack(blockIdx.x, a); }
+__global__ void __launch_bounds__(64 * GB_WAVES) kb_build_grp(const BuildGrpPack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    const BuildGrpPack a = pk[m.rep];
+    build_grp_pack(m.blk, a);
+}
+
 // nblist.c:607-647 on every site; sets the trigger flag.  Also the
 // device-side nl_check(lb,ub) of the C ABI (original numbering range).
 __global__ void k_nlcheck(int n, const double4 *pos4, const double4 *anchor, const int *orig, int lb,
@@ -690,7 +776,7 @@ static void launch_build(ttc_ctx *c, bool conditional, double cutoff, uint32_t *
 
 // the pruned list of MD loops (cutoff ttc_compute_cutoff), FP32 test with
 // FAST_MARGIN, group-cooperative build
-static void launch_build_grp(ttc_ctx *c) {
+static BuildGrpPack build_grp_pack_of(ttc_ctx *c) {
     GbGeom g;
     g.ncx = c->ncx; g.ncy = c->ncy; g.ncz = c->ncz;
     g.lx = (float)c->p.lx; g.ly = (float)c->p.ly; g.lz = (float)c->p.lz;
@@ -700,25 +786,36 @@ static void launch_build_grp(ttc_ctx *c) {
     double cf = ttc_compute_cutoff(c) + FAST_MARGIN;
     g.cut2 = (float)(cf * cf);
     g.cutr = (float)(cf + FAST_MARGIN);
-    k_build_grp<<<c->G, 64 * GB_WAVES, 0, c->stream>>>(
-        c->n, c->cap, g, c->posfb, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4, c->vel4, c->f4,
-        c->type, c->orig, c->anchor, c->fsb, c->flb, c->fs, c->fl, c->cell_cnt, c->cell_start, c->ex_off,
-        c->ex_part, c->ex_flag, c->ex_span, c->nbl, c->glen, c->gcnt, c->status, 1);
+    return BuildGrpPack{c->n, c->cap, g, c->posfb, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4, c->vel4,
+                        c->f4, c->type, c->orig, c->anchor, c->fsb, c->flb, c->fs, c->fl, c->cell_cnt, c->cell_start,
+                        c->ex_off, c->ex_part, c->ex_flag, c->ex_span, c->nbl, c->glen, c->gcnt, c->status, 1};
+}
+
+static void launch_build_grp(ttc_ctx *c) {
+    k_build_grp<<<c->G, 64 * GB_WAVES, 0, c->stream>>>(build_grp_pack_of(c));
+}
+
+static CellAssignPack cell_assign_pack(ttc_ctx *c, int cond) {
+    return CellAssignPack{c->n, c->pos4, geom(c), ttc_cellcap(c), c->cell_acc, c->cell_of, c->cell_members,
+                          c->status, cond};
+}
+
+static CellScanPack cell_scan_pack(ttc_ctx *c, int cond) {
+    return CellScanPack{c->ncell, c->cell_acc, c->cell_cnt, c->cell_start, c->status, cond};
+}
+
+static ReorderPack reorder_pack(ttc_ctx *c, int cond) {
+    return ReorderPack{c->n, ttc_cellcap(c), c->cell_of, c->cell_cnt, c->cell_start, c->cell_members, c->orig,
+                       c->pos4, c->vel4, c->f4, c->type, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->fs,
+                       c->fl, c->fsb, c->flb, c->posfb, c->inv, c->status, cond};
 }
 
 int ttc_launch_rebuild(ttc_ctx *c, bool conditional) {
-    CellGeom g = geom(c);
-    int cellcap = ttc_cellcap(c);
     int blocks = (c->n + 255) / 256;
     int cond = conditional ? 1 : 0;
-    k_cell_assign<<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, g, cellcap, c->cell_acc, c->cell_of,
-                                                 c->cell_members, c->status, cond);
-    k_cell_scan<<<1, 1024, 0, c->stream>>>(c->ncell, c->cell_acc, c->cell_cnt, c->cell_start, c->status,
-                                          cond);
-    k_reorder<<<blocks, 256, 0, c->stream>>>(c->n, cellcap, c->cell_of, c->cell_cnt, c->cell_start,
-                                             c->cell_members, c->orig, c->pos4, c->vel4, c->f4, c->type,
-                                             c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->fs, c->fl,
-                                             c->fsb, c->flb, c->posfb, c->inv, c->status, cond);
+    k_cell_assign<<<blocks, 256, 0, c->stream>>>(cell_assign_pack(c, cond));
+    k_cell_scan<<<1, 1024, 0, c->stream>>>(cell_scan_pack(c, cond));
+    k_reorder<<<blocks, 256, 0, c->stream>>>(reorder_pack(c, cond));
     // explicit builds (nblist.c:32) make the reference's list; the
     // conditional builds of MD loops make the pruned one
     if (conditional)
@@ -816,4 +913,36 @@ int ttc_export_list(ttc_ctx *c, double cutoff, int max_nb, int *count, int *list
         }
     }
     return TTC_OK;
+}
+
+// ---- batched stages of the NHC step (ttc_batch.hip): the conditional
+// rebuild of every context, st[0..3] = assign, scan, reorder, build
+
+int ttc_bstage_rebuild(const std::vector<ttc_ctx *> &r, TtcStage *st) {
+    std::vector<CellAssignPack> pa;
+    std::vector<CellScanPack> ps;
+    std::vector<ReorderPack> pr;
+    std::vector<BuildGrpPack> pb;
+    std::vector<int> nsite, one, ngrp;
+    for (ttc_ctx *c : r) {
+        pa.push_back(cell_assign_pack(c, 1));
+        ps.push_back(cell_scan_pack(c, 1));
+        pr.push_back(reorder_pack(c, 1));
+        pb.push_back(build_grp_pack_of(c));
+        nsite.push_back((c->n + 255) / 256);
+        one.push_back(1);
+        ngrp.push_back(c->G);
+    }
+    int e = ttc_stage_make(&st[0], pa, nsite, 0);
+    if (!e) e = ttc_stage_make(&st[1], ps, one, 0);
+    if (!e) e = ttc_stage_make(&st[2], pr, nsite, 0);
+    if (!e) e = ttc_stage_make(&st[3], pb, ngrp, 0);
+    return e;
+}
+
+void ttc_blaunch_rebuild(const TtcStage *st, hipStream_t s) {
+    kb_cell_assign<<<st[0].nblk, 256, 0, s>>>((const CellAssignPack *)st[0].pk, st[0].map);
+    kb_cell_scan<<<st[1].nblk, 1024, 0, s>>>((const CellScanPack *)st[1].pk, st[1].map);
+    kb_reorder<<<st[2].nblk, 256, 0, s>>>((const ReorderPack *)st[2].pk, st[2].map);
+    kb_build_grp<<<st[3].nblk, 64 * GB_WAVES, 0, s>>>((const BuildGrpPack *)st[3].pk, st[3].map);
 }

@@ -150,11 +150,11 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, int tj
 // waves per group shorten each group's dependent load chain (small boxes,
 // few groups); fewer suit many groups.  ttc_tune picks WAVES per context.
 template <bool ENERGY, bool RANGE, bool HES, int WAVES>
-__global__ void __launch_bounds__(64 * WAVES)
-k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__ glen,
-       const double4 *__restrict__ pos4, const int *__restrict__ type, const float4 *__restrict__ ljtab,
-       PairConst pc, double4 *__restrict__ fpair, double *__restrict__ partials, int maxb,
-       RangeSel rs = RangeSel{}) {
+__device__ __forceinline__ void pair_body(int grp, int n, int cap, const uint32_t *__restrict__ nbl,
+                                          const int *__restrict__ glen, const double4 *__restrict__ pos4,
+                                          const int *__restrict__ type, const float4 *__restrict__ ljtab,
+                                          const PairConst &pc, double4 *__restrict__ fpair,
+                                          double *__restrict__ partials, int maxb, const RangeSel &rs) {
     extern __shared__ float4 slj[];
     __shared__ double sred[WAVES][64][3];
     __shared__ double ered[WAVES][14];
@@ -162,7 +162,6 @@ k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__
     for (int k = threadIdx.x; k < nt2; k += blockDim.x) slj[k] = ljtab[k];
     __syncthreads();
 
-    int grp = blockIdx.x;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int s = grp * 64 + lane;
     PairAcc a = {};
@@ -240,14 +239,41 @@ k_pair(int n, int cap, const uint32_t *__restrict__ nbl, const int *__restrict__
         // each pair was visited from both of its sites: halve
         double t = 0.0;
         for (int w = 0; w < WAVES; w++) t += ered[w][threadIdx.x];
-        partials[(size_t)(EV_LJ + threadIdx.x) * maxb + blockIdx.x] = 0.5 * t;
+        partials[(size_t)(EV_LJ + threadIdx.x) * maxb + grp] = 0.5 * t;
     }
     if (HES && threadIdx.x >= 11 && threadIdx.x < 14) {
         // per-site diagonal terms: already one per visit, no halving
         double t = 0.0;
         for (int w = 0; w < WAVES; w++) t += ered[w][threadIdx.x];
-        partials[(size_t)(EV_PHX + threadIdx.x - 11) * maxb + blockIdx.x] = t;
+        partials[(size_t)(EV_PHX + threadIdx.x - 11) * maxb + grp] = t;
     }
+}
+
+struct PairPack {
+    int n, cap;
+    const uint32_t *nbl;
+    const int *glen;
+    const double4 *pos4;
+    const int *type;
+    const float4 *ljtab;
+    PairConst pc;
+    double4 *fpair;
+    double *partials;
+    int maxb;
+    RangeSel rs;
+};
+
+template <bool ENERGY, bool RANGE, bool HES, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_pair(PairPack a) {
+    pair_body<ENERGY, RANGE, HES, WAVES>(blockIdx.x, a.n, a.cap, a.nbl, a.glen, a.pos4, a.type, a.ljtab, a.pc, a.fpair,
+                                         a.partials, a.maxb, a.rs);
+}
+template <bool ENERGY, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) kb_pair(const PairPack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    const PairPack a = pk[m.rep];
+    pair_body<ENERGY, false, false, WAVES>(m.blk, a.n, a.cap, a.nbl, a.glen, a.pos4, a.type, a.ljtab, a.pc, a.fpair,
+                                           a.partials, a.maxb, a.rs);
 }
 
 }  // namespace
@@ -273,14 +299,17 @@ static PairConst pair_const(const ttc_ctx *c) {
     return pc;
 }
 
+static PairPack pair_pack(ttc_ctx *c, RangeSel rs) {
+    return PairPack{c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, c->ljtab, pair_const(c), c->fpair, c->partials,
+                    c->nblk_max, rs};
+}
+
 template <bool ENERGY, bool RANGE, bool HES = false>
 static void launch_pair_w(ttc_ctx *c, RangeSel rs) {
-    PairConst pc = pair_const(c);
     int blocks = ttc_pair_blocks(c);
     size_t shm = sizeof(float4) * c->ntypes * c->ntypes;
-#define PAIR_LAUNCH(W)                                                                                           \
-    k_pair<ENERGY, RANGE, HES, W><<<blocks, 64 * W, shm, c->stream>>>(c->n, c->cap, c->nbl, c->glen, c->pos4, c->type, \
-                                                                 c->ljtab, pc, c->fpair, c->partials, c->nblk_max, rs)
+    PairPack a = pair_pack(c, rs);
+#define PAIR_LAUNCH(W) k_pair<ENERGY, RANGE, HES, W><<<blocks, 64 * W, shm, c->stream>>>(a)
     switch (c->pair_waves) {
     case 4: PAIR_LAUNCH(4); break;
     case 16: PAIR_LAUNCH(16); break;
@@ -300,4 +329,23 @@ void ttc_launch_pair_hes(ttc_ctx *c) {
 void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
     if (energy) launch_pair_w<true, false>(c, RangeSel{});
     else launch_pair_w<false, false>(c, RangeSel{});
+}
+
+// ---- batched stage of the NHC step (ttc_batch.hip): every context's pair
+// kernel with 8 waves per group (the untuned default of a context)
+int ttc_bstage_pair(const std::vector<ttc_ctx *> &r, TtcStage *st) {
+    std::vector<PairPack> pk;
+    std::vector<int> nb;
+    size_t shm = 0;
+    for (ttc_ctx *c : r) {
+        pk.push_back(pair_pack(c, RangeSel{}));
+        nb.push_back(ttc_pair_blocks(c));
+        shm = std::max(shm, sizeof(float4) * c->ntypes * c->ntypes);
+    }
+    return ttc_stage_make(st, pk, nb, shm);
+}
+
+void ttc_blaunch_pair(const TtcStage &st, bool energy, hipStream_t s) {
+    if (energy) kb_pair<true, 8><<<st.nblk, 64 * 8, st.shm, s>>>((const PairPack *)st.pk, st.map);
+    else kb_pair<false, 8><<<st.nblk, 64 * 8, st.shm, s>>>((const PairPack *)st.pk, st.map);
 }

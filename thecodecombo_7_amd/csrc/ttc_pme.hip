@@ -200,12 +200,12 @@ __device__ __forceinline__ int tile_base(int fl, int org, int ext, int k, int P)
 // the global fixed-point grid with one atomic per touched grid point.
 // Integer adds make the grid independent of order.
 template <int P, bool DRIFT>
-__global__ void __launch_bounds__(256)
-k_drift_spread(double4 *pos4, double4 *vel4, const double4 *f4, double dt, double vscal, Spread sp, Brick br,
-               const int *__restrict__ cell_start, const int *__restrict__ cell_cnt, long long *qfix,
-               int *status, const double4 *__restrict__ anchor, const double *__restrict__ pcheck) {
+__device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 *vel4, const double4 *f4, double dt,
+                                                  double vscal, const Spread &sp, const Brick &br,
+                                                  const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
+                                                  long long *qfix, int *status, const double4 *__restrict__ anchor,
+                                                  const double *__restrict__ pcheck) {
     extern __shared__ unsigned long long tile[];
-    int b = blockIdx.x;
     int ibx = b % br.nbx, iby = (b / br.nbx) % br.nby, ibz = b / (br.nbx * br.nby);
     int cx0 = ibx * br.bx, cy0 = iby * br.by, cz0 = ibz * br.bz;
     int cx1 = min(cx0 + br.bx, br.ncx), cy1 = min(cy0 + br.by, br.ncy), cz1 = min(cz0 + br.bz, br.ncz);
@@ -289,6 +289,32 @@ k_drift_spread(double4 *pos4, double4 *vel4, const double4 *f4, double dt, doubl
         if (gz < 0) gz += sp.K2;
         atomicAdd((unsigned long long *)&qfix[((size_t)gx * sp.K1 + gy) * sp.K2 + gz], v);
     }
+}
+
+struct DsPack {
+    double4 *pos4, *vel4;
+    const double4 *f4;
+    double dt, vscal;
+    Spread sp;
+    Brick br;
+    const int *cell_start, *cell_cnt;
+    long long *qfix;
+    int *status;
+    const double4 *anchor;
+    const double *pcheck;
+};
+
+template <int P, bool DRIFT>
+__global__ void __launch_bounds__(256) k_drift_spread(DsPack a) {
+    drift_spread_body<P, DRIFT>(blockIdx.x, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start, a.cell_cnt,
+                                a.qfix, a.status, a.anchor, a.pcheck);
+}
+template <int P, bool DRIFT>
+__global__ void __launch_bounds__(256) kb_drift_spread(const DsPack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    const DsPack a = pk[m.rep];
+    drift_spread_body<P, DRIFT>(m.blk, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start, a.cell_cnt,
+                                a.qfix, a.status, a.anchor, a.pcheck);
 }
 
 struct FftArgs {
@@ -380,9 +406,9 @@ __device__ float2 *fft_lds(float2 *A, float2 *B, int nlines, int ls, int es, con
 // MODE 2: x lines, forward + influence function + inverse, in place
 // MODE 3: z lines, inverse, output = real part
 template <int MODE, bool ENERGY>
-__global__ void __launch_bounds__(FFT_THREADS)
-k_fft(FftArgs f, int sign, long long *qfix, float2 *qc, float *qr, const float *eterm, double fac,
-      double lx, double ly, double lz, double *partials, int maxb) {
+__device__ __forceinline__ void fft_body(int bid, const FftArgs &f, int sign, long long *qfix, float2 *qc, float *qr,
+                                         const float *eterm, double fac, double lx, double ly, double lz,
+                                         double *partials, int maxb) {
     extern __shared__ float2 lds[];
     int n = f.n;
     float2 *A = lds, *B = lds + FFT_LB * n;
@@ -393,7 +419,7 @@ k_fft(FftArgs f, int sign, long long *qfix, float2 *qc, float *qr, const float *
     int x0 = 0, y0 = 0, z0 = 0, nl = 0;
     if (MODE == 0 || MODE == 3) {
         nlines = f.K0 * f.K1;
-        int l0 = blockIdx.x * FFT_LB;
+        int l0 = bid * FFT_LB;
         nl = min(FFT_LB, nlines - l0);
         ls = n; es = 1;
         gbase = (size_t)l0 * n;
@@ -411,8 +437,8 @@ k_fft(FftArgs f, int sign, long long *qfix, float2 *qc, float *qr, const float *
     } else {
         // lines along y (MODE 1: fixed x) or x (MODE 2: fixed y), LB z values
         int nzb = (f.K2 + FFT_LB - 1) / FFT_LB;
-        int fixed = blockIdx.x / nzb;
-        z0 = (blockIdx.x - fixed * nzb) * FFT_LB;
+        int fixed = bid / nzb;
+        z0 = (bid - fixed * nzb) * FFT_LB;
         nl = min(FFT_LB, f.K2 - z0);
         ls = 1; es = FFT_LB;
         if (MODE == 1) x0 = fixed; else y0 = fixed;
@@ -473,7 +499,7 @@ k_fft(FftArgs f, int sign, long long *qfix, float2 *qc, float *qr, const float *
             if (threadIdx.x < 7) {
                 double t = 0.0;
                 for (int w = 0; w < FFT_THREADS / 64; w++) t += red[w][threadIdx.x];
-                partials[(size_t)(EV_REC + threadIdx.x) * maxb + blockIdx.x] = t;
+                partials[(size_t)(EV_REC + threadIdx.x) * maxb + bid] = t;
             }
         }
         __syncthreads();
@@ -499,6 +525,31 @@ k_fft(FftArgs f, int sign, long long *qfix, float2 *qc, float *qr, const float *
     }
 }
 
+struct FftPack {
+    FftArgs f;
+    int sign;
+    long long *qfix;
+    float2 *qc;
+    float *qr;
+    const float *eterm;
+    double fac, lx, ly, lz;
+    double *partials;
+    int maxb;
+};
+
+template <int MODE, bool ENERGY>
+__global__ void __launch_bounds__(FFT_THREADS) k_fft(FftPack a) {
+    fft_body<MODE, ENERGY>(blockIdx.x, a.f, a.sign, a.qfix, a.qc, a.qr, a.eterm, a.fac, a.lx, a.ly, a.lz, a.partials,
+                           a.maxb);
+}
+template <int MODE, bool ENERGY>
+__global__ void __launch_bounds__(FFT_THREADS) kb_fft(const FftPack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    const FftPack a = pk[m.rep];
+    fft_body<MODE, ENERGY>(m.blk, a.f, a.sign, a.qfix, a.qc, a.qr, a.eterm, a.fac, a.lx, a.ly, a.lz, a.partials,
+                           a.maxb);
+}
+
 struct GatherConst {
     double lx, ly, lz, hx, hy, hz;
     int K0, K1, K2;
@@ -506,19 +557,20 @@ struct GatherConst {
 };
 
 template <int P, bool ENERGY, bool HES = false>
-__global__ void __launch_bounds__(256)
-k_gather(int n, const double4 *__restrict__ pos4, const int *__restrict__ orig, const int *__restrict__ inv,
-         const float *__restrict__ qr, GatherConst gc, const int *__restrict__ b_off,
-         const int *__restrict__ b_part, const double *__restrict__ b_r0, const double *__restrict__ b_k,
-         double4 *__restrict__ g4, double *partials, int maxb, int do_pme, int do_bonds,
-         const float *__restrict__ theta_near = nullptr) {
+__device__ __forceinline__ void gather_body(int bid, int n, const double4 *__restrict__ pos4,
+                                            const int *__restrict__ orig, const int *__restrict__ inv,
+                                            const float *__restrict__ qr, const GatherConst &gc,
+                                            const int *__restrict__ b_off, const int *__restrict__ b_part,
+                                            const double *__restrict__ b_r0, const double *__restrict__ b_k,
+                                            double4 *__restrict__ g4, double *partials, int maxb, int do_pme,
+                                            int do_bonds, const float *__restrict__ theta_near) {
     constexpr int ND = 2 * P - 1;   // grid offsets -(P-1) .. P-1
     __shared__ float sth[HES ? ND * ND * ND : 1];
     if (HES) {
         for (int i = threadIdx.x; i < ND * ND * ND; i += blockDim.x) sth[i] = theta_near[i];
         __syncthreads();
     }
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = bid * blockDim.x + threadIdx.x;
     double px = 0.0, py = 0.0, pz = 0.0;
     double eb = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
     double kh[3] = {0.0, 0.0, 0.0}, bh[3] = {0.0, 0.0, 0.0};
@@ -670,8 +722,35 @@ k_gather(int n, const double4 *__restrict__ pos4, const int *__restrict__ orig, 
         double t = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
         int q = threadIdx.x;
         int slot = q < 3 ? EV_FSX + q : (q < 10 ? EV_BOND + (q - 3) : (q < 13 ? EV_KHX + (q - 10) : EV_BHX + (q - 13)));
-        partials[(size_t)slot * maxb + blockIdx.x] = t;
+        partials[(size_t)slot * maxb + bid] = t;
     }
+}
+
+struct GatherPack {
+    int n;
+    const double4 *pos4;
+    const int *orig, *inv;
+    const float *qr;
+    GatherConst gc;
+    const int *b_off, *b_part;
+    const double *b_r0, *b_k;
+    double4 *g4;
+    double *partials;
+    int maxb, do_pme, do_bonds;
+    const float *theta_near;
+};
+
+template <int P, bool ENERGY, bool HES = false>
+__global__ void __launch_bounds__(256) k_gather(GatherPack a) {
+    gather_body<P, ENERGY, HES>(blockIdx.x, a.n, a.pos4, a.orig, a.inv, a.qr, a.gc, a.b_off, a.b_part, a.b_r0, a.b_k,
+                                a.g4, a.partials, a.maxb, a.do_pme, a.do_bonds, a.theta_near);
+}
+template <int P, bool ENERGY>
+__global__ void __launch_bounds__(256) kb_gather(const GatherPack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    const GatherPack a = pk[m.rep];
+    gather_body<P, ENERGY, false>(m.blk, a.n, a.pos4, a.orig, a.inv, a.qr, a.gc, a.b_off, a.b_part, a.b_r0, a.b_k,
+                                  a.g4, a.partials, a.maxb, a.do_pme, a.do_bonds, a.theta_near);
 }
 
 FftArgs fft_args(const ttc_ctx *c, int axis) {
@@ -829,16 +908,21 @@ static size_t brick_shm(const ttc_ctx *c, const Brick &br) {
     return t * sizeof(unsigned long long);
 }
 
-static void launch_drift_spread(ttc_ctx *c, bool drift, double vscal) {
-    Spread sp = spread_args(c);
+static DsPack ds_pack(ttc_ctx *c, bool drift, double vscal) {
+    return DsPack{c->pos4, c->vel4, c->f4, drift ? c->p.dt : 0.0, vscal, spread_args(c), brick_args(c),
+                  c->cell_start, c->cell_cnt, c->qfix, c->status, c->anchor, c->ev + EV_N};
+}
+
+static int ds_blocks(const ttc_ctx *c) {
     Brick br = brick_args(c);
-    int blocks = br.nbx * br.nby * br.nbz;
-    size_t shm = brick_shm(c, br);
-    double dt = drift ? c->p.dt : 0.0;
-#define DS(P, D)                                                                                         \
-    k_drift_spread<P, D><<<blocks, 256, shm, c->stream>>>(c->pos4, c->vel4, c->f4, dt, vscal, sp, br,   \
-                                                          c->cell_start, c->cell_cnt, c->qfix, c->status, \
-                                                          c->anchor, c->ev + EV_N)
+    return br.nbx * br.nby * br.nbz;
+}
+
+static void launch_drift_spread(ttc_ctx *c, bool drift, double vscal) {
+    int blocks = ds_blocks(c);
+    size_t shm = brick_shm(c, brick_args(c));
+    DsPack a = ds_pack(c, drift, vscal);
+#define DS(P, D) k_drift_spread<P, D><<<blocks, 256, shm, c->stream>>>(a)
     if (c->order == 4) { if (drift) DS(4, true); else DS(4, false); }
     else { if (drift) DS(6, true); else DS(6, false); }
 #undef DS
@@ -868,25 +952,36 @@ static size_t fft_shm(const ttc_ctx *c, int axis) {
     return sizeof(float2) * (2 * FFT_LB * c->plan[axis].n + c->plan[axis].n);
 }
 
-// FFT passes, influence function, gather (+ bonds).  Expects the spread
-// grid in qfix; leaves qfix zeroed.
+// the five line passes of one step: z forward, y forward, x forward +
+// influence function + x inverse, y inverse, z inverse
+enum { FP_Z_FWD = 0, FP_Y_FWD, FP_X_CONV, FP_Y_INV, FP_Z_INV, FP_N };
+static const int kPassAxis[FP_N] = {2, 1, 0, 1, 2};
+static const int kPassSign[FP_N] = {+1, +1, +1, -1, -1};
+
+static FftPack fft_pack(const ttc_ctx *c, int pass) {
+    return FftPack{fft_args(c, kPassAxis[pass]), kPassSign[pass], c->qfix, c->qc, c->qr, c->eterm,
+                   TTC_PI * TTC_PI / c->p.kappa / c->p.kappa, c->p.lx, c->p.ly, c->p.lz, c->partials, c->nblk_max};
+}
+
+static int fft_blocks(const ttc_ctx *c, int pass) {
+    int nzb = (c->K[2] + FFT_LB - 1) / FFT_LB;
+    if (pass == FP_Z_FWD || pass == FP_Z_INV) return (c->K[0] * c->K[1] + FFT_LB - 1) / FFT_LB;
+    if (pass == FP_X_CONV) return c->K[1] * nzb;
+    return c->K[0] * nzb;
+}
+
+// FFT passes and influence function.  Expects the spread grid in qfix;
+// leaves qfix zeroed.
 void ttc_launch_fft(ttc_ctx *c, bool energy) {
-    double fac = TTC_PI * TTC_PI / c->p.kappa / c->p.kappa;
-    double lx = c->p.lx, ly = c->p.ly, lz = c->p.lz;
-    int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2];
-    int nzb = (K2 + FFT_LB - 1) / FFT_LB;
-    FftArgs fz = fft_args(c, 2), fy = fft_args(c, 1), fx = fft_args(c, 0);
-    int bz = (K0 * K1 + FFT_LB - 1) / FFT_LB;
-    {
-    k_fft<0, false><<<bz, FFT_THREADS, fft_shm(c, 2), c->stream>>>(fz, +1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
-    k_fft<1, false><<<K0 * nzb, FFT_THREADS, fft_shm(c, 1), c->stream>>>(fy, +1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
-    if (energy)
-        k_fft<2, true><<<K1 * nzb, FFT_THREADS, fft_shm(c, 0), c->stream>>>(fx, +1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
-    else
-        k_fft<2, false><<<K1 * nzb, FFT_THREADS, fft_shm(c, 0), c->stream>>>(fx, +1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
-    k_fft<1, false><<<K0 * nzb, FFT_THREADS, fft_shm(c, 1), c->stream>>>(fy, -1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
-    k_fft<3, false><<<bz, FFT_THREADS, fft_shm(c, 2), c->stream>>>(fz, -1, c->qfix, c->qc, c->qr, c->eterm, fac, lx, ly, lz, c->partials, c->nblk_max);
-    }
+#define FFTL(MODE, E, PASS) \
+    k_fft<MODE, E><<<fft_blocks(c, PASS), FFT_THREADS, fft_shm(c, kPassAxis[PASS]), c->stream>>>(fft_pack(c, PASS))
+    FFTL(0, false, FP_Z_FWD);
+    FFTL(1, false, FP_Y_FWD);
+    if (energy) FFTL(2, true, FP_X_CONV);
+    else FFTL(2, false, FP_X_CONV);
+    FFTL(1, false, FP_Y_INV);
+    FFTL(3, false, FP_Z_INV);
+#undef FFTL
 }
 
 void ttc_launch_gather(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes) {
@@ -898,10 +993,9 @@ void ttc_launch_gather(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes) 
     gc.K0 = K0; gc.K1 = K1; gc.K2 = K2;
     gc.qscale = TTC_QFACTOR / c->p.eps_rf;
     int blocks = ttc_gather_blocks(c);
-#define GATHER(P, E, H)                                                                                    \
-    k_gather<P, E, H><<<blocks, 256, 0, c->stream>>>(c->n, c->pos4, c->orig, c->inv, c->qr, gc, c->b_off,   \
-                                                     c->b_part, c->b_r0, c->b_k, c->g4, c->partials,         \
-                                                     c->nblk_max, pme ? 1 : 0, bonds ? 1 : 0, c->theta_near)
+    GatherPack a{c->n, c->pos4, c->orig, c->inv, c->qr, gc, c->b_off, c->b_part, c->b_r0, c->b_k, c->g4,
+                 c->partials, c->nblk_max, pme ? 1 : 0, bonds ? 1 : 0, c->theta_near};
+#define GATHER(P, E, H) k_gather<P, E, H><<<blocks, 256, 0, c->stream>>>(a)
     if (c->order == 4) {
         if (hes) GATHER(4, true, true); else if (energy) GATHER(4, true, false); else GATHER(4, false, false);
     } else {
@@ -945,4 +1039,79 @@ int ttc_pme_theta(ttc_ctx *c) {
     TTC_HIP(hipMalloc(&c->theta_near, sizeof(float) * near.size()));
     TTC_HIP(hipMemcpy(c->theta_near, near.data(), sizeof(float) * near.size(), hipMemcpyHostToDevice));
     return TTC_OK;
+}
+
+// ---- batched stages of the NHC step (ttc_batch.hip) ------------------------
+
+int ttc_bstage_drift(const std::vector<ttc_ctx *> &r, TtcStage *st) {
+    std::vector<DsPack> pk;
+    std::vector<int> nb;
+    size_t shm = 0;
+    for (ttc_ctx *c : r) {
+        if (c->order != 4) {
+            ttc_set_error("ttc_batch: batched launches need spline order 4 in every replica");
+            return TTC_ERR_ARG;
+        }
+        pk.push_back(ds_pack(c, true, 1.0));
+        nb.push_back(ds_blocks(c));
+        shm = std::max(shm, brick_shm(c, brick_args(c)));
+    }
+    if (shm > 160 * 1024) {
+        ttc_set_error("ttc_batch: PME spread tile exceeds the LDS of one CU");
+        return TTC_ERR_ARG;
+    }
+    TTC_HIP(hipFuncSetAttribute((const void *)kb_drift_spread<4, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shm));
+    return ttc_stage_make(st, pk, nb, shm);
+}
+void ttc_blaunch_drift(const TtcStage &st, hipStream_t s) {
+    kb_drift_spread<4, true><<<st.nblk, 256, st.shm, s>>>((const DsPack *)st.pk, st.map);
+}
+
+// st[FP_N]: the five FFT passes
+int ttc_bstage_fft(const std::vector<ttc_ctx *> &r, TtcStage *st) {
+    for (int pass = 0; pass < FP_N; pass++) {
+        std::vector<FftPack> pk;
+        std::vector<int> nb;
+        size_t shm = 0;
+        for (ttc_ctx *c : r) {
+            pk.push_back(fft_pack(c, pass));
+            nb.push_back(fft_blocks(c, pass));
+            shm = std::max(shm, fft_shm(c, kPassAxis[pass]));
+        }
+        int e = ttc_stage_make(&st[pass], pk, nb, shm);
+        if (e) return e;
+    }
+    return TTC_OK;
+}
+void ttc_blaunch_fft(const TtcStage *st, bool energy, hipStream_t s) {
+#define FFTB(MODE, E, PASS) \
+    kb_fft<MODE, E><<<st[PASS].nblk, FFT_THREADS, st[PASS].shm, s>>>((const FftPack *)st[PASS].pk, st[PASS].map)
+    FFTB(0, false, FP_Z_FWD);
+    FFTB(1, false, FP_Y_FWD);
+    if (energy) FFTB(2, true, FP_X_CONV);
+    else FFTB(2, false, FP_X_CONV);
+    FFTB(1, false, FP_Y_INV);
+    FFTB(3, false, FP_Z_INV);
+#undef FFTB
+}
+
+int ttc_bstage_gather(const std::vector<ttc_ctx *> &r, TtcStage *st) {
+    std::vector<GatherPack> pk;
+    std::vector<int> nb;
+    for (ttc_ctx *c : r) {
+        GatherConst gc;
+        gc.lx = c->p.lx; gc.ly = c->p.ly; gc.lz = c->p.lz;
+        gc.hx = 0.5 * c->p.lx; gc.hy = 0.5 * c->p.ly; gc.hz = 0.5 * c->p.lz;
+        gc.K0 = c->K[0]; gc.K1 = c->K[1]; gc.K2 = c->K[2];
+        gc.qscale = TTC_QFACTOR / c->p.eps_rf;
+        pk.push_back(GatherPack{c->n, c->pos4, c->orig, c->inv, c->qr, gc, c->b_off, c->b_part, c->b_r0, c->b_k, c->g4,
+                                c->partials, c->nblk_max, 1, 1, c->theta_near});
+        nb.push_back(ttc_gather_blocks(c));
+    }
+    return ttc_stage_make(st, pk, nb, 0);
+}
+void ttc_blaunch_gather(const TtcStage &st, bool energy, hipStream_t s) {
+    if (energy) kb_gather<4, true><<<st.nblk, 256, 0, s>>>((const GatherPack *)st.pk, st.map);
+    else kb_gather<4, false><<<st.nblk, 256, 0, s>>>((const GatherPack *)st.pk, st.map);
 }

@@ -58,7 +58,7 @@ class ReplicaBatch:
         self.spl = steps_per_launch
 
     def autotune(self, trial_steps: int = 50):
-        m = C.c_int(0); s = C.c_int(0); t = np.zeros(8)
+        m = C.c_int(0); s = C.c_int(0); t = np.zeros(12)
         T.check(self.L.ttc_batch_autotune(self.h, trial_steps, C.byref(m), C.byref(s), T.dptr(t)))
         self.spl = s.value
         return m.value, s.value, t
