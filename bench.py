@@ -264,10 +264,13 @@ def nve_drift(box, s, steps: int, every: int):
     rows, _ = box.integrate_vverlet(steps, energy_every=every)
     t_ps = rows[:, 0] * s.dt
     tot = rows[:, 1] + rows[:, 2] + rows[:, 4]
-    slope = np.polyfit(t_ps, tot, 1)[0] * 1e3          # kJ/mol per ns
+    (slope, icpt), cov = np.polyfit(t_ps, tot, 1, cov=True)
     dof = 3 * s.n - 3
+    # least-squares slope and its standard error (independent residuals),
+    # kJ/mol per ns per degree of freedom
     return {"steps": steps, "rows": int(len(rows)), "total_kJ_mol": [round(float(tot[0]), 3), round(float(tot[-1]), 3)],
-            "drift_kJ_mol_per_ns_per_dof": float(slope / dof),
+            "drift_kJ_mol_per_ns_per_dof": float(slope * 1e3 / dof),
+            "drift_stderr": float(np.sqrt(cov[0, 0]) * 1e3 / dof),
             "rms_fluct_rel": float(np.std(tot) / abs(np.mean(tot)))}
 
 
@@ -353,7 +356,7 @@ def main():
     ap.add_argument("--large-steps", type=int, default=200, help="timed steps of the chains-1M leg")
     ap.add_argument("--no-replicas", action="store_true", help="skip the batched replica leg")
     ap.add_argument("--replica-steps", type=int, default=1000, help="timed steps of the replica leg")
-    ap.add_argument("--drift-steps", type=int, default=5000, help="NVE steps of the energy-drift run (0: skip)")
+    ap.add_argument("--drift-steps", type=int, default=10000, help="NVE steps of the energy-drift run (0: skip)")
     args = ap.parse_args()
     if args.traffic_summary:
         traffic_summary(*args.traffic_summary)
