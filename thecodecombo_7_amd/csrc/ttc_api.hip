@@ -61,7 +61,7 @@ static int list_capacity(const ttc_ctx *c, double cutoff) {
 
 extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
     CHECK(p && out, TTC_ERR_ARG, "ttc_create: null argument");
-    CHECK(p->n > 0 && p->n < (1 << 30), TTC_ERR_ARG, "ttc_create: bad site count");
+    CHECK(p->n > 0 && p->n < TTC_MAX_SITES, TTC_ERR_ARG, "ttc_create: 1 .. 2^24 - 1 sites supported");
     CHECK(p->lx > 0 && p->ly > 0 && p->lz > 0, TTC_ERR_ARG, "ttc_create: bad box");
     CHECK(p->rc > 0 && p->rclist >= p->rc, TTC_ERR_ARG, "ttc_create: need rclist >= rc > 0");
     CHECK(p->order == 4 || p->order == 6, TTC_ERR_ARG, "ttc_create: spline order must be 4 or 6");

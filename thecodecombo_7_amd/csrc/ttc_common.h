@@ -17,8 +17,10 @@
 //   row capacity, is a multiple of 64).  A row's consecutive entries share a
 //   64-byte segment, so appends by the list build coalesce, and the 16 k of
 //   one 4 KB tile are what the pair kernel's waves read together.
-//   Entry = j (sorted index, 30 bits) | class << 30 (0 plain, 1 excluded
-//   xflag 1/2/3, 2 one-four xflag 4); 0xFFFFFFFF pads short rows.
+//   Entry = j (sorted index, 24 bits) | type of j << 24 (6 bits) |
+//   class << 30 (0 plain, 1 excluded xflag 1/2/3, 2 one-four xflag 4);
+//   0xFFFFFFFF pads short rows.  Carrying j's type in the entry spares the
+//   pair kernel a second gather per entry.
 //   PME: charge grid as int64 fixed point (2^-40 e units) for deterministic
 //   atomic spreading, complex FP32 work grid, FP32 influence function.
 #pragma once
@@ -33,7 +35,9 @@
 
 #define TTC_WAVE 64
 #define TTC_PAD 0xFFFFFFFFu
-#define TTC_JMASK 0x3FFFFFFFu
+#define TTC_JMASK 0x00FFFFFFu
+#define TTC_TSHIFT 24
+#define TTC_MAX_SITES (1 << 24)
 #define TTC_FIX_SCALE 4294967296.0          /* 2^32 */
 #define TTC_FIX_INV (1.0 / 4294967296.0)
 #define TTC_QFIX_SCALE 1099511627776.0      /* 2^40 */

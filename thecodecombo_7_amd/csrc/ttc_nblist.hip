@@ -291,7 +291,7 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__
                     for (int q = xb; q < xe; q++)
                         if (ex_part[q] == oj) { int f = ex_flag[q]; cls = f == 4 ? 2u : (f ? 1u : 0u); }
                 }
-                if (k < cap) row[ttc_nbl_idx(k, lane)] = (uint32_t)j | (cls << 30);
+                if (k < cap) row[ttc_nbl_idx(k, lane)] = (uint32_t)j | ((uint32_t)typeb[j] << TTC_TSHIFT) | (cls << 30);
                 k++;
             }
         }
@@ -446,7 +446,7 @@ __device__ __forceinline__ void build_grp_body(
     int *gcnt, int *status, int conditional) {
     if (rebuild_off(status, conditional)) return;
     __shared__ int2 sch[GB_MAXCH];
-    __shared__ float4 sst[GB_BATCH * GB_C];   // chunk sites: x, y, z in the frame, original index
+    __shared__ float4 sst[GB_BATCH * GB_C];   // chunk sites: x, y, z in the frame, original index | type << 24
     __shared__ int snch;
     __shared__ int srow[64];
     __shared__ int smax;
@@ -559,7 +559,7 @@ __device__ __forceinline__ void build_grp_body(
                     p.y += (float)(((d.y >> 9) & 3) - 1) * g.ly;
                     p.z += (float)(((d.y >> 11) & 3) - 1) * g.lz;
                 }
-                v = make_float4(p.x, p.y, p.z, __int_as_float(origb[jj]));
+                v = make_float4(p.x, p.y, p.z, __int_as_float(origb[jj] | (typeb[jj] << TTC_TSHIFT)));
             }
             sst[slot] = v;
         }
@@ -572,7 +572,9 @@ __device__ __forceinline__ void build_grp_body(
             bool ok = cl < nn;
             int j = j0 + cl;
             float4 pj = sst[(c - cb) * GB_C + cl];
-            int oj = __float_as_int(pj.w);
+            const int wj = __float_as_int(pj.w);
+            const int oj = wj & (int)TTC_JMASK;                 // original index
+            const uint32_t tjb = ((uint32_t)wj >> TTC_TSHIFT) << TTC_TSHIFT;   // type bits of the entry
             bool mi = (meta & GB_MI) != 0;
             uint32_t jw = (uint32_t)j;
 #pragma unroll
@@ -612,7 +614,7 @@ __device__ __forceinline__ void build_grp_body(
                                 }
                             }
                         }
-                        if (hit && pos < cap) grow[ttc_nbl_idx(pos, rrow[t])] = jw | (cls << 30);
+                        if (hit && pos < cap) grow[ttc_nbl_idx(pos, rrow[t])] = jw | tjb | (cls << 30);
                     }
                 }
             }

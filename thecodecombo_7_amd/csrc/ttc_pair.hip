@@ -13,10 +13,18 @@
 
 namespace {
 
+#ifndef PAIR_AHEAD
 #define PAIR_AHEAD 2   // entries whose site gathers are in flight ahead of the arithmetic
+#endif
+
+// relative band around rc^2 inside which the cutoff is decided by the
+// reference's op-by-op FP64 test; outside it r^2 summed with FMAs (a few ulp
+// from the op-by-op sum) decides the same way
+#define PAIR_RC2_BAND 1.0e-12
 
 struct PairConst {
     double lx, ly, lz, hx, hy, hz, rc2;
+    double ilx, ily, ilz;     // 1/L per axis
     double erfc_rc_over_rc;   // erfc(kappa rc)/rc with the reference's erfc, FP64
     float rc2f, kappa, kpi2;  // kpi2 = 2 kappa / sqrt(pi)
     float qfac;               // QFACTOR
@@ -55,25 +63,33 @@ struct RangeSel {
 // site gathered for entry e (site 0 for padding, which is skipped)
 __device__ __forceinline__ int entry_site(uint32_t e) { return e == TTC_PAD ? 0 : (int)(e & TTC_JMASK); }
 
-// pj, tj: position and type of the entry's site, gathered by the caller
-// (ahead of time, so the gathers of later entries overlap this one)
+// pj: position of the entry's site, gathered by the caller (ahead of time,
+// so the gathers of later entries overlap this one); its type is in the entry
 template <bool ENERGY, bool RANGE = false, bool HES = false>
-__device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, int tj, const double4 &pi, float qi,
+__device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const double4 &pi, float qi,
                                            const float4 *ljrow, const PairConst &pc, PairAcc &a,
                                            const RangeSel &rs = RangeSel{}, bool i_in = true) {
     if (e == TTC_PAD) return;
     int j = (int)(e & TTC_JMASK);
+    int tj = (int)((e >> TTC_TSHIFT) & 63u);
     uint32_t cls = e >> 30;
-    double dx = ttc_minimg(pi.x - pj.x, pc.hx, pc.lx);
-    double dy = ttc_minimg(pi.y - pj.y, pc.hy, pc.ly);
-    double dz = ttc_minimg(pi.z - pj.z, pc.hz, pc.lz);
-    double dr2 = ttc_r2_exact(dx, dy, dz);
-    if (dr2 > pc.rc2) return;
+    // minimum image in FP64 as d - L rint(d/L): for a list pair |d| is far
+    // from L/2, so this wraps exactly when the reference's d > L/2 test does,
+    // and the FMA gives the reference's d -= L rounded once
+    double ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+    ex = fma(-pc.lx, rint(ex * pc.ilx), ex);
+    ey = fma(-pc.ly, rint(ey * pc.ily), ey);
+    ez = fma(-pc.lz, rint(ez * pc.ilz), ez);
+    const double dr2 = fma(ex, ex, fma(ey, ey, ez * ez));
+    bool inside = dr2 <= pc.rc2;
+    if (fabs(dr2 - pc.rc2) < PAIR_RC2_BAND * pc.rc2) inside = ttc_r2_exact(ex, ey, ez) <= pc.rc2;
+    if (!inside) return;
+    const float dx = (float)ex, dy = (float)ey, dz = (float)ez;
+    const float r2 = (float)dr2;
     if (RANGE && !i_in) {
         int oj = rs.orig[j];
         if (oj < rs.lb || oj >= rs.ub) return;
     }
-    float r2 = (float)dr2;
     float rinv = __builtin_amdgcn_rsqf(r2);
     float rinv2 = rinv * rinv;
     float r = r2 * rinv;
@@ -99,7 +115,7 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, int tj
         // Hessian diagonal of this site, one visit of the pair (the full
         // list visits it from both sites, giving the reference's factor 2):
         // cnonbond_nblist2.c:286-297 (LJ), 403-410 (real), 433-440 (excluded)
-        float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
+        float fdx = dx, fdy = dy, fdz = dz;
         float flj = 12.0f * eps * rinv2 * (sr12 - sr6);
         float term2 = 12.0f * eps * rinv2 * rinv2 * (14.0f * sr12 - 8.0f * sr6);
         float kpi = 0.5f * pc.kpi2;
@@ -122,7 +138,7 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, int tj
         a.hy += (double)ay;
         a.hz += (double)az;
     }
-    float fxf = force * (float)dx, fyf = force * (float)dy, fzf = force * (float)dz;
+    float fxf = force * dx, fyf = force * dy, fzf = force * dz;
     a.fx += (double)fxf;
     a.fy += (double)fyf;
     a.fz += (double)fzf;
@@ -134,12 +150,12 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, int tj
         a.er += (double)(qq * erfckr * rinv);
         a.ers += (double)qq * pc.erfc_rc_over_rc;
         if (cls) a.ei += (double)(qi2 * (1.0f - erfckr) * rinv);
-        a.w0 += (double)(fxf * (float)dx);
-        a.w1 += (double)(fyf * (float)dx);
-        a.w2 += (double)(fyf * (float)dy);
-        a.w3 += (double)(fzf * (float)dx);
-        a.w4 += (double)(fzf * (float)dy);
-        a.w5 += (double)(fzf * (float)dz);
+        a.w0 += (double)(fxf * dx);
+        a.w1 += (double)(fyf * dx);
+        a.w2 += (double)(fyf * dy);
+        a.w3 += (double)(fzf * dx);
+        a.w4 += (double)(fzf * dy);
+        a.w5 += (double)(fzf * dz);
     }
 }
 
@@ -192,25 +208,16 @@ __device__ __forceinline__ void pair_body(int grp, int n, int cap, const uint32_
                           q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
         // gathers run PAIR_AHEAD entries ahead of the arithmetic
         double4 pj[16];
-        int tj[16];
 #pragma unroll
-        for (int q = 0; q < PAIR_AHEAD; q++) {
-            int j = entry_site(e[q]);
-            pj[q] = pos4[j];
-            tj[q] = type[j];
-        }
+        for (int q = 0; q < PAIR_AHEAD; q++) pj[q] = pos4[entry_site(e[q])];
         if (t + WAVES < ntile) {
             const uint4 *p = seg + (size_t)(t + WAVES) * 256;
             q0 = p[0]; q1 = p[1]; q2 = p[2]; q3 = p[3];
         }
 #pragma unroll
         for (int q = 0; q < 16; q++) {
-            if (q + PAIR_AHEAD < 16) {
-                int j = entry_site(e[q + PAIR_AHEAD]);
-                pj[q + PAIR_AHEAD] = pos4[j];
-                tj[q + PAIR_AHEAD] = type[j];
-            }
-            pair_entry<ENERGY, RANGE, HES>(e[q], pj[q], tj[q], pi, qi, ljrow, pc, a, rs, i_in);
+            if (q + PAIR_AHEAD < 16) pj[q + PAIR_AHEAD] = pos4[entry_site(e[q + PAIR_AHEAD])];
+            pair_entry<ENERGY, RANGE, HES>(e[q], pj[q], pi, qi, ljrow, pc, a, rs, i_in);
         }
     }
 
@@ -287,6 +294,7 @@ static PairConst pair_const(const ttc_ctx *c) {
     pc.lx = c->p.lx; pc.ly = c->p.ly; pc.lz = c->p.lz;
     pc.hx = 0.5 * c->p.lx; pc.hy = 0.5 * c->p.ly; pc.hz = 0.5 * c->p.lz;
     pc.rc2 = c->p.rc * c->p.rc;
+    pc.ilx = 1.0 / c->p.lx; pc.ily = 1.0 / c->p.ly; pc.ilz = 1.0 / c->p.lz;
     pc.erfc_rc_over_rc = ttc_nr_erfc_host(c->p.kappa * c->p.rc) / c->p.rc;
     pc.rc2f = (float)pc.rc2;
     pc.kappa = (float)c->p.kappa;
