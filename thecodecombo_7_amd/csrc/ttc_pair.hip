@@ -48,6 +48,7 @@ __device__ __forceinline__ float nr_erfc(float z, float *ez2) {
 
 struct PairAcc {
     double fx, fy, fz;
+    float tx, ty, tz;    // this tile's force, added to fx, fy, fz once per 16 entries
     double elj, eljs, er, ers, ei, w0, w1, w2, w3, w4, w5;
     double hx, hy, hz;   // CONFIGT: this site's d2U/dx2, dy2, dz2 (pair terms)
 };
@@ -67,7 +68,7 @@ __device__ __forceinline__ int entry_site(uint32_t e) { return e == TTC_PAD ? 0 
 // so the gathers of later entries overlap this one); its type is in the entry
 template <bool ENERGY, bool RANGE = false, bool HES = false>
 __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const double4 &pi, float qi,
-                                           const float4 *ljrow, const PairConst &pc, PairAcc &a,
+                                           const float2 *ljrow, const float2 *ljrow14, const PairConst &pc, PairAcc &a,
                                            const RangeSel &rs = RangeSel{}, bool i_in = true) {
     if (e == TTC_PAD) return;
     int j = (int)(e & TTC_JMASK);
@@ -93,11 +94,11 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const 
     float rinv = __builtin_amdgcn_rsqf(r2);
     float rinv2 = rinv * rinv;
     float r = r2 * rinv;
-    float4 lj = ljrow[tj];
-    bool one4 = cls == 2;
-    float eps = one4 ? lj.z : lj.x;
-    eps = cls == 1 ? 0.0f : eps;
-    float sg2 = one4 ? lj.w : lj.y;
+    const bool one4 = cls == 2;
+    float2 lj = ljrow[tj];
+    if (__builtin_expect(one4, 0)) lj = ljrow14[tj];   // 1-4 parameters only for 1-4 pairs
+    const float eps = cls == 1 ? 0.0f : lj.x;
+    const float sg2 = lj.y;
     float sr2 = sg2 * rinv2;
     float sr6 = sr2 * sr2 * sr2;
     float sr12 = sr6 * sr6;
@@ -139,9 +140,9 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const 
         a.hz += (double)az;
     }
     float fxf = force * dx, fyf = force * dy, fzf = force * dz;
-    a.fx += (double)fxf;
-    a.fy += (double)fyf;
-    a.fz += (double)fzf;
+    a.tx += fxf;
+    a.ty += fyf;
+    a.tz += fzf;
     if (ENERGY) {
         float sr2s = sg2 / pc.rc2f;
         float sr6s = sr2s * sr2s * sr2s;
@@ -171,11 +172,16 @@ __device__ __forceinline__ void pair_body(int grp, int n, int cap, const uint32_
                                           const int *__restrict__ type, const float4 *__restrict__ ljtab,
                                           const PairConst &pc, double4 *__restrict__ fpair,
                                           double *__restrict__ partials, int maxb, const RangeSel &rs) {
-    extern __shared__ float4 slj[];
+    // LJ pair table in two halves: {eps, sig^2} and the 1-4 {eps, sig^2}
+    extern __shared__ float2 slj[];
     __shared__ double sred[WAVES][64][3];
     __shared__ double ered[WAVES][14];
     int nt2 = pc.ntypes * pc.ntypes;
-    for (int k = threadIdx.x; k < nt2; k += blockDim.x) slj[k] = ljtab[k];
+    for (int k = threadIdx.x; k < nt2; k += blockDim.x) {
+        const float4 v = ljtab[k];
+        slj[k] = make_float2(v.x, v.y);
+        slj[nt2 + k] = make_float2(v.z, v.w);
+    }
     __syncthreads();
 
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -184,7 +190,7 @@ __device__ __forceinline__ void pair_body(int grp, int n, int cap, const uint32_
     int len = glen[grp];
     double4 pi = s < n ? pos4[s] : make_double4(0.0, 0.0, 0.0, 0.0);
     int ti = s < n ? type[s] : 0;
-    const float4 *ljrow = slj + ti * pc.ntypes;
+    const float2 *ljrow = slj + ti * pc.ntypes, *ljrow14 = slj + nt2 + ti * pc.ntypes;
     float qi = (float)pi.w;
     // this lane's 64-byte segment of tile 0; tile t is 1024 words further
     const uint4 *seg = (const uint4 *)(nbl + (size_t)grp * cap * 64 + (size_t)lane * 16);
@@ -217,8 +223,13 @@ __device__ __forceinline__ void pair_body(int grp, int n, int cap, const uint32_
 #pragma unroll
         for (int q = 0; q < 16; q++) {
             if (q + PAIR_AHEAD < 16) pj[q + PAIR_AHEAD] = pos4[entry_site(e[q + PAIR_AHEAD])];
-            pair_entry<ENERGY, RANGE, HES>(e[q], pj[q], pi, qi, ljrow, pc, a, rs, i_in);
+            pair_entry<ENERGY, RANGE, HES>(e[q], pj[q], pi, qi, ljrow, ljrow14, pc, a, rs, i_in);
         }
+        // the tile's FP32 force into the FP64 sum
+        a.fx += (double)a.tx;
+        a.fy += (double)a.ty;
+        a.fz += (double)a.tz;
+        a.tx = a.ty = a.tz = 0.0f;
     }
 
     sred[wave][lane][0] = a.fx;
