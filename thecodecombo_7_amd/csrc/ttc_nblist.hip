@@ -525,7 +525,7 @@ __device__ __forceinline__ void build_grp_body(
     // rows of this lane in each of the wave's GB_NT row sets: row
     // wave*GB_ROWS + t*GB_RPI + seg
     float rx[GB_NT], ry[GB_NT], rz[GB_NT];
-    int ro[GB_NT], rk[GB_NT], rrow[GB_NT];
+    int ro[GB_NT], rk[GB_NT], rrow[GB_NT], rself[GB_NT], roff[GB_NT];
     bool rv[GB_NT];
 #pragma unroll
     for (int t = 0; t < GB_NT; t++) {
@@ -539,6 +539,8 @@ __device__ __forceinline__ void build_grp_body(
         ro[t] = rv[t] ? origb[s] : -1;
         rk[t] = 0;
         rrow[t] = r;
+        rself[t] = rv[t] ? s : -1;   // never a candidate index when the row is empty
+        roff[t] = r * 16;            // the row's words inside each 16-entry tile
     }
     const int nset = min(GB_NT, (n - sb - wave * GB_ROWS + GB_RPI - 1) / GB_RPI);   // row sets with a site
     unsigned long long segmask = ((GB_C == 64) ? ~0ull : ((1ull << GB_C) - 1ull)) << (seg * GB_C);
@@ -586,7 +588,7 @@ __device__ __forceinline__ void build_grp_body(
                         dy -= g.ly * rintf(dy * g.ily);
                         dz -= g.lz * rintf(dz * g.ilz);
                     }
-                    bool hit = ok & rv[t] & (dx * dx + dy * dy + dz * dz < g.cut2) & (j != sb + rrow[t]);
+                    bool hit = ok & (dx * dx + dy * dy + dz * dz < g.cut2) & (j != rself[t]) & rv[t];
                     unsigned long long b = __ballot(hit);
                     if (b != 0ull) {
                         unsigned long long mb = b & segmask;
@@ -614,7 +616,8 @@ __device__ __forceinline__ void build_grp_body(
                                 }
                             }
                         }
-                        if (hit && pos < cap) grow[ttc_nbl_idx(pos, rrow[t])] = jw | tjb | (cls << 30);
+                        // ttc_nbl_idx in 32-bit arithmetic (a group's block is cap*64 < 2^31 words)
+                        if (hit && pos < cap) grow[((pos >> 4) << 10) + roff[t] + (pos & 15)] = jw | tjb | (cls << 30);
                     }
                 }
             }
