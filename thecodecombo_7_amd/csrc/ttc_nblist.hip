@@ -15,6 +15,7 @@
 #include "ttc_common.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace {
 
@@ -149,7 +150,8 @@ __device__ __forceinline__ void reorder_body(int bid, const ReorderPack &a) {
     int s = cell_start[c] + r;
     double4 p = pos4[i];
     pos4b[s] = p;
-    posfb[s] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);   // candidates of the fast build
+    // candidates of the fast build: FP32 position, original index | LJ type << TTC_TSHIFT
+    posfb[s] = make_float4((float)p.x, (float)p.y, (float)p.z, __int_as_float(o | (type[i] << TTC_TSHIFT)));
     vel4b[s] = vel4[i];
     f4b[s] = f4[i];   // the last force feeds the next half kick
     fsb[s] = fs[i];   // RESPA short / long forces likewise
@@ -328,31 +330,30 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__
 // The list rebuilt inside MD loops only has to be a superset of the pairs
 // within the exact cutoff (the pair kernel applies the FP64 test to every
 // entry), so it is built from the FP32 positions at a cutoff widened by a
-// margin.  One workgroup per ELL group of 64 sorted sites:
-//   1. wave 0 takes the group's bounding box in the minimum-image frame of
-//      the group's first site (compact even where the group straddles the
-//      periodic boundary) and enumerates the rows of cells (fixed y, z cell)
-//      that come within the cutoff of it.  Sites are sorted x-fastest by
-//      cell, so each row contributes one or two contiguous ranges of sorted
-//      sites, each with one periodic shift into the frame (or, in boxes
-//      where the range wraps onto itself, the minimum-image flag).  Ranges
-//      are cut into chunks of GB_C sites.
-//   2. every wave owns GB_ROWS rows (sites) of the group and streams all
-//      chunks.  One test instruction covers GB_RPI rows x GB_C chunk sites
-//      (lane = row segment x chunk site); one ballot gives the hits, and each
-//      hit lane writes its entry at its row's running count (popcount of
-//      the hits below it in its row segment), in chunk order.
-// The row order is therefore fixed (chunk order) and no cross-wave prefix
-// is needed.  Short chunks keep the sites one row reads at nearby positions
-// in a small window of memory, which the pair kernel's gathers rely on.
+// margin.  One workgroup per ELL group of 64 sorted sites, GB_WAVES waves of
+// GB_ROWS rows (sites) each.  Every wave works on its own rows alone:
+//   1. it takes its rows' bounding box in the minimum-image frame of its
+//      first row (compact even where the rows straddle the periodic
+//      boundary) and enumerates the rows of cells (fixed y, z cell) that come
+//      within the cutoff of it, one cell row per lane.  Sites are sorted
+//      x-fastest by cell, so each cell row contributes one or two contiguous
+//      ranges of sorted sites, each with one periodic shift into the frame
+//      (or, in boxes where the range wraps onto itself, the minimum-image
+//      flag).
+//   2. it walks the ranges in lane order, 64 consecutive candidate sites per
+//      load (lane = candidate, one coalesced 16-byte load each), and tests
+//      each of its rows against the 64 candidates with the row's coordinates
+//      wave-uniform; one ballot gives the row's hits, and each hit lane
+//      writes its entry at the row's running count plus the number of hits
+//      below it.
+// The row order is therefore fixed (range order, then sorted order), no
+// cross-wave exchange is needed and nothing limits the number of ranges.
+// Small per-wave boxes keep the candidate count near twice the sites inside
+// the cutoff sphere's bounding cells rather than the group's.
 #define GB_WAVES 4
 #define GB_ROWS (64 / GB_WAVES)
-#define GB_C 16
-#define GB_RPI (64 / GB_C)
-#define GB_NT (GB_ROWS / GB_RPI)
-#define GB_MAXCH 2048
-#define GB_BATCH 64   // chunks whose sites are staged in LDS at once
 #define GB_MI (1 << 13)
+#define GB_QCAP 128   // candidate ring per wave (two batches of 64)
 #define GB_EXMAX 8    // exclusion partners per site held in LDS (more: read from the CSR)
 
 struct GbGeom {
@@ -391,8 +392,8 @@ __device__ __forceinline__ AxisRange axis_range(float mn, float mx, float r, flo
 }
 
 // the sorted-site ranges (at most two, three with the whole ring) of
-// candidate row r; returns the number of chunks, fills lo/len/meta
-__device__ int row_ranges(int r, const AxisRange &ay, const AxisRange &az, float xmn, float xmx, float ymn,
+// candidate row r; fills lo/len/meta and their count nr
+__device__ void row_ranges(int r, const AxisRange &ay, const AxisRange &az, float xmn, float xmx, float ymn,
                           float ymx, float zmn, float zmx, const GbGeom &g, const int *__restrict__ cell_start,
                           const int *__restrict__ cell_cnt, int *lo, int *len, int *meta, int *nr) {
     int nyr = ay.b - ay.a + 1;
@@ -408,14 +409,13 @@ __device__ int row_ranges(int r, const AxisRange &ay, const AxisRange &az, float
     }
     float g2 = gy * gy + gz * gz;
     *nr = 0;
-    if (g2 > g.cutr * g.cutr) return 0;
+    if (g2 > g.cutr * g.cutr) return;
     float xr = sqrtf(g.cutr * g.cutr - g2);
     AxisRange ax = axis_range(xmn, xmx, xr, g.hx, g.cwx, g.ncx);
     int cyw = cyu - floordiv(cyu, g.ncy) * g.ncy, czw = czu - floordiv(czu, g.ncz) * g.ncz;
     int sy = ay.mi ? 0 : floordiv(cyu, g.ncy), sz = az.mi ? 0 : floordiv(czu, g.ncz);
     int rowc = g.ncx * (cyw + g.ncy * czw);
     bool mi = ay.mi || az.mi || ax.mi;
-    int chunks = 0;
     int p0 = ax.a;
     while (p0 <= ax.b && *nr < 3) {
         int sx = ax.mi ? 0 : floordiv(p0, g.ncx);
@@ -428,12 +428,10 @@ __device__ int row_ranges(int r, const AxisRange &ay, const AxisRange &az, float
             lo[*nr] = jl;
             len[*nr] = jh - jl;
             meta[*nr] = ((sx + 1) << 7) | ((sy + 1) << 9) | ((sz + 1) << 11) | (mi ? GB_MI : 0);
-            chunks += (jh - jl + GB_C - 1) / GB_C;
             (*nr)++;
         }
         p0 = p1 + 1;
     }
-    return chunks;
 }
 
 __device__ __forceinline__ void build_grp_body(
@@ -445,18 +443,19 @@ __device__ __forceinline__ void build_grp_body(
     const int *__restrict__ ex_part, const int *__restrict__ ex_flag, int ex_span, uint32_t *nbl, int *glen,
     int *gcnt, int *status, int conditional) {
     if (rebuild_off(status, conditional)) return;
-    __shared__ int2 sch[GB_MAXCH];
-    __shared__ float4 sst[GB_BATCH * GB_C];   // chunk sites: x, y, z in the frame, original index | type << 24
-    __shared__ int snch;
     __shared__ int srow[64];
     __shared__ int smax;
+    // per wave: ring of candidates that passed the box filter (frame
+    // position, original index | type bits; sorted index)
+    __shared__ float4 sq[GB_WAVES][GB_QCAP];
+    __shared__ int sqj[GB_WAVES][GB_QCAP];
     // exclusion partners of the group's sites (original index) and their
     // class; sxn = count, or -1 where the site has more than GB_EXMAX
     __shared__ int sxo[64][GB_EXMAX];
     __shared__ int sxc[64][GB_EXMAX];
     __shared__ int sxn[64];
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int sb = grp * 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int sb = grp * 64;
     if (threadIdx.x < 64) {
         int s = sb + threadIdx.x, cnt = 0;
         if (s < n) {
@@ -471,9 +470,6 @@ __device__ __forceinline__ void build_grp_body(
         }
         sxn[threadIdx.x] = cnt;
     }
-    // the frame: minimum image around the group's first site
-    float4 pref = posfb[sb];
-
     // the new sorted order becomes the current state
     for (int s = sb + (int)threadIdx.x; s < min(sb + 64, n); s += 64 * GB_WAVES) {
         double4 pi = pos4b[s];
@@ -486,148 +482,148 @@ __device__ __forceinline__ void build_grp_body(
         orig[s] = origb[s];
         anchor[s] = pi;
     }
-    if (wave == 0) {
-        float4 p = posfb[min(sb + lane, n - 1)];
-        float ux = frame1(p.x, pref.x, g.lx, g.ilx), uy = frame1(p.y, pref.y, g.ly, g.ily),
-              uz = frame1(p.z, pref.z, g.lz, g.ilz);
-        float xmn = wave_minf(ux), xmx = wave_maxf(ux);
-        float ymn = wave_minf(uy), ymx = wave_maxf(uy);
-        float zmn = wave_minf(uz), zmx = wave_maxf(uz);
-        AxisRange ay = axis_range(ymn, ymx, g.cutr, g.hy, g.cwy, g.ncy);
-        AxisRange az = axis_range(zmn, zmx, g.cutr, g.hz, g.cwz, g.ncz);
-        int nrows = (ay.b - ay.a + 1) * (az.b - az.a + 1);
-        int base = 0;
-        for (int r0 = 0; r0 < nrows; r0 += 64) {
-            int r = r0 + lane;
-            int lo[3], len[3], meta[3], nr = 0, cnt = 0;
-            if (r < nrows)
-                cnt = row_ranges(r, ay, az, xmn, xmx, ymn, ymx, zmn, zmx, g, cell_start, cell_cnt, lo, len, meta, &nr);
-            int incl = cnt;
-            for (int o = 1; o < 64; o <<= 1) {
-                int t = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += t;
-            }
-            int k = base + incl - cnt;
-            for (int q = 0; q < nr; q++)
-                for (int c0 = 0; c0 < len[q]; c0 += GB_C, k++)
-                    if (k < GB_MAXCH) sch[k] = make_int2(lo[q] + c0, min(GB_C, len[q] - c0) | meta[q]);
-            base += __shfl(incl, 63, 64);
-        }
-        if (lane == 0) {
-            snch = min(base, GB_MAXCH);
-            if (base > GB_MAXCH) atomicOr(&status[ST_OVERFLOW], 8);
-        }
-    }
     __syncthreads();
 
-    int nch = snch;
-    int seg = lane / GB_C, cl = lane % GB_C;
-    // rows of this lane in each of the wave's GB_NT row sets: row
-    // wave*GB_ROWS + t*GB_RPI + seg
-    float rx[GB_NT], ry[GB_NT], rz[GB_NT];
-    int ro[GB_NT], rk[GB_NT], rrow[GB_NT], rself[GB_NT], roff[GB_NT];
-    bool rv[GB_NT];
-#pragma unroll
-    for (int t = 0; t < GB_NT; t++) {
-        int r = wave * GB_ROWS + t * GB_RPI + seg;
-        int s = sb + r;
-        rv[t] = s < n;
-        float4 p = posfb[min(s, n - 1)];
-        rx[t] = frame1(p.x, pref.x, g.lx, g.ilx);
-        ry[t] = frame1(p.y, pref.y, g.ly, g.ily);
-        rz[t] = frame1(p.z, pref.z, g.lz, g.ilz);
-        ro[t] = rv[t] ? origb[s] : -1;
-        rk[t] = 0;
-        rrow[t] = r;
-        rself[t] = rv[t] ? s : -1;   // never a candidate index when the row is empty
-        roff[t] = r * 16;            // the row's words inside each 16-entry tile
-    }
-    const int nset = min(GB_NT, (n - sb - wave * GB_ROWS + GB_RPI - 1) / GB_RPI);   // row sets with a site
-    unsigned long long segmask = ((GB_C == 64) ? ~0ull : ((1ull << GB_C) - 1ull)) << (seg * GB_C);
+    const int row0 = __builtin_amdgcn_readfirstlane(wave) * GB_ROWS;   // this wave's first row in the group
+    const int wb = sb + row0;          // and its sorted index
+    const int nrw = max(0, min(GB_ROWS, n - wb));
+    int rkl = 0;   // entry count of row lane (lanes below GB_ROWS)
     uint32_t *grow = nbl + (size_t)grp * cap * 64;
-    for (int cb = 0; cb < nch; cb += GB_BATCH) {
-        // stage the sites of the next GB_BATCH chunks (shifted into the frame)
-        int nb = min(GB_BATCH, nch - cb);
-        __syncthreads();
-        for (int slot = threadIdx.x; slot < nb * GB_C; slot += 64 * GB_WAVES) {
-            int2 d = sch[cb + slot / GB_C];
-            int q = slot % GB_C;
-            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (q < (d.y & 127)) {
-                int jj = d.x + q;
-                float4 p = posfb[jj];
-                if (!(d.y & GB_MI)) {
-                    p.x += (float)(((d.y >> 7) & 3) - 1) * g.lx;
-                    p.y += (float)(((d.y >> 9) & 3) - 1) * g.ly;
-                    p.z += (float)(((d.y >> 11) & 3) - 1) * g.lz;
-                }
-                v = make_float4(p.x, p.y, p.z, __int_as_float(origb[jj] | (typeb[jj] << TTC_TSHIFT)));
-            }
-            sst[slot] = v;
-        }
-        __syncthreads();
-        for (int c = cb; c < cb + nb; c++) {
-            int2 d = sch[c];
-            int j0 = __builtin_amdgcn_readfirstlane(d.x);
-            int meta = __builtin_amdgcn_readfirstlane(d.y);
-            int nn = meta & 127;
-            bool ok = cl < nn;
-            int j = j0 + cl;
-            float4 pj = sst[(c - cb) * GB_C + cl];
+    if (nrw > 0) {
+        // the frame: minimum image around the wave's first row
+        const float4 pref = posfb[wb];
+        const float4 pw = posfb[wb + min(lane, nrw - 1)];
+        // lane r < nrw holds row r: frame coordinates (ux, uy, uz) and
+        // original index (pwo), read wave-uniform with readlane
+        const float ux = frame1(pw.x, pref.x, g.lx, g.ilx), uy = frame1(pw.y, pref.y, g.ly, g.ily),
+                    uz = frame1(pw.z, pref.z, g.lz, g.ilz);
+        const int pwo = __float_as_int(pw.w) & (int)TTC_JMASK;
+        const float xmn = wave_minf(ux), xmx = wave_maxf(ux);
+        const float ymn = wave_minf(uy), ymx = wave_maxf(uy);
+        const float zmn = wave_minf(uz), zmx = wave_maxf(uz);
+        const AxisRange ay = axis_range(ymn, ymx, g.cutr, g.hy, g.cwy, g.ncy);
+        const AxisRange az = axis_range(zmn, zmx, g.cutr, g.hz, g.cwz, g.ncz);
+        // whole-ring ranges (small boxes): candidates keep their own image
+        // and every pair takes the minimum image; no box filter then
+        const bool wmi = ay.mi || az.mi || axis_range(xmn, xmx, g.cutr, g.hx, g.cwx, g.ncx).mi;
+        const float bcx = 0.5f * (xmn + xmx), bcy = 0.5f * (ymn + ymx), bcz = 0.5f * (zmn + zmx);
+        const float bhx = 0.5f * (xmx - xmn), bhy = 0.5f * (ymx - ymn), bhz = 0.5f * (zmx - zmn);
+        const float cutr2 = g.cutr * g.cutr;   // box filter radius: the test radius plus a margin
+        float4 *qp = sq[row0 / GB_ROWS];
+        int *qj = sqj[row0 / GB_ROWS];
+        int qh = 0, qn = 0;   // ring head and length (wave-uniform)
+
+        // test every row against the cnt (<= 64) candidates at the head of the ring
+        auto consume = [&](int cnt, auto MI) {
+            const int slot = (qh + lane) & (GB_QCAP - 1);
+            const float4 pj = qp[slot];
+            const int j = qj[slot];
+            const bool ok = lane < cnt;
             const int wj = __float_as_int(pj.w);
-            const int oj = wj & (int)TTC_JMASK;                 // original index
-            const uint32_t tjb = ((uint32_t)wj >> TTC_TSHIFT) << TTC_TSHIFT;   // type bits of the entry
-            bool mi = (meta & GB_MI) != 0;
-            uint32_t jw = (uint32_t)j;
-#pragma unroll
-            for (int t = 0; t < GB_NT; t++) {
-                if (t < nset) {
-                    float dx = rx[t] - pj.x, dy = ry[t] - pj.y, dz = rz[t] - pj.z;
-                    if (mi) {
-                        dx -= g.lx * rintf(dx * g.ilx);
-                        dy -= g.ly * rintf(dy * g.ily);
-                        dz -= g.lz * rintf(dz * g.ilz);
-                    }
-                    bool hit = ok & (dx * dx + dy * dy + dz * dz < g.cut2) & (j != rself[t]) & rv[t];
-                    unsigned long long b = __ballot(hit);
-                    if (b != 0ull) {
-                        unsigned long long mb = b & segmask;
-                        int pos = rk[t] + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32),
-                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u));
-                        rk[t] += (int)__popcll(mb);
-                        // exclusion class: only pairs close in the original numbering
-                        // can be excluded or 1-4 pairs (max |a-b| over the exclusion list)
-                        uint32_t cls = 0;
-                        int dd = oj - ro[t];
-                        bool nearo = hit & ((unsigned)(dd + ex_span) <= (unsigned)(2 * ex_span));
-                        if (__ballot(nearo) != 0ull) {
-                            if (nearo) {
-                                int r = rrow[t];
-                                int cnt = sxn[r];
-                                for (int q = 0; q < cnt; q++)
-                                    if (sxo[r][q] == oj) cls = (uint32_t)sxc[r][q];
-                                if (cnt < 0) {
-                                    int oi = ro[t];
-                                    for (int q = ex_off[oi]; q < ex_off[oi + 1]; q++)
-                                        if (ex_part[q] == oj) {
-                                            int f = ex_flag[q];
-                                            cls = f == 4 ? 2u : (f ? 1u : 0u);
-                                        }
-                                }
+            const int oj = wj & (int)TTC_JMASK;                                    // original index
+            const uint32_t ej = (uint32_t)j | ((uint32_t)wj & ~TTC_JMASK);         // sorted index | type bits
+            for (int rr = 0; rr < nrw; rr++) {
+                const float rwx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ux), rr));
+                const float rwy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(uy), rr));
+                const float rwz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(uz), rr));
+                float dx = rwx - pj.x, dy = rwy - pj.y, dz = rwz - pj.z;
+                if (decltype(MI)::value) {
+                    dx -= g.lx * rintf(dx * g.ilx);
+                    dy -= g.ly * rintf(dy * g.ily);
+                    dz -= g.lz * rintf(dz * g.ilz);
+                }
+                const bool hit = ok & (dx * dx + dy * dy + dz * dz < g.cut2) & (j != wb + rr);
+                const unsigned long long b = __builtin_amdgcn_ballot_w64(hit);
+                if (b != 0ull) {
+                    const int base = __builtin_amdgcn_readlane(rkl, rr);
+                    const int pos = base + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+                    if (lane == rr) rkl = base + (int)__popcll(b);
+                    // exclusion class: only pairs close in the original numbering
+                    // can be excluded or 1-4 pairs (max |a-b| over the exclusion list)
+                    uint32_t cls = 0;
+                    const int oi = __builtin_amdgcn_readlane(pwo, rr);
+                    const bool nearo = hit & ((unsigned)(oj - oi + ex_span) <= (unsigned)(2 * ex_span));
+                    if (__builtin_amdgcn_ballot_w64(nearo) != 0ull) {
+                        if (nearo) {
+                            const int rw = row0 + rr;
+                            const int xc = sxn[rw];
+                            for (int q2 = 0; q2 < xc; q2++)
+                                if (sxo[rw][q2] == oj) cls = (uint32_t)sxc[rw][q2];
+                            if (xc < 0) {
+                                for (int q2 = ex_off[oi]; q2 < ex_off[oi + 1]; q2++)
+                                    if (ex_part[q2] == oj) {
+                                        const int f = ex_flag[q2];
+                                        cls = f == 4 ? 2u : (f ? 1u : 0u);
+                                    }
                             }
                         }
-                        // ttc_nbl_idx in 32-bit arithmetic (a group's block is cap*64 < 2^31 words)
-                        if (hit && pos < cap) grow[((pos >> 4) << 10) + roff[t] + (pos & 15)] = jw | tjb | (cls << 30);
+                    }
+                    // ttc_nbl_idx in 32-bit arithmetic (a group's block is cap*64 < 2^31 words)
+                    if (hit && pos < cap)
+                        grow[(uint32_t)(((pos >> 4) << 10) + (row0 + rr) * 16 + (pos & 15))] = ej | (cls << 30);
+                }
+            }
+            qh = (qh + cnt) & (GB_QCAP - 1);
+            qn -= cnt;
+        };
+        auto drain = [&](int cnt) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // ring writes before the reads
+            if (wmi) consume(cnt, std::true_type{});
+            else consume(cnt, std::false_type{});
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // reads before the next writes
+        };
+
+        const int nrows = (ay.b - ay.a + 1) * (az.b - az.a + 1);
+        for (int r0 = 0; r0 < nrows; r0 += 64) {
+            const int r = r0 + lane;
+            int lo[3], len[3], meta[3], nr = 0;
+            if (r < nrows) row_ranges(r, ay, az, xmn, xmx, ymn, ymx, zmn, zmx, g, cell_start, cell_cnt, lo, len, meta, &nr);
+            unsigned long long act = __ballot(nr > 0);
+            while (act != 0ull) {
+                const int src = __builtin_ctzll(act);
+                act &= act - 1ull;
+                const int nq = __builtin_amdgcn_readlane(nr, src);
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    if (q >= nq) break;
+                    const int j0 = __builtin_amdgcn_readlane(lo[q], src);
+                    const int jn = __builtin_amdgcn_readlane(len[q], src);
+                    const int mt = __builtin_amdgcn_readlane(meta[q], src);
+                    // shift into the frame (none for whole-ring ranges or whole-ring waves)
+                    const bool shift = !wmi && !(mt & GB_MI);
+                    const float shx = shift ? (float)(((mt >> 7) & 3) - 1) * g.lx : 0.0f;
+                    const float shy = shift ? (float)(((mt >> 9) & 3) - 1) * g.ly : 0.0f;
+                    const float shz = shift ? (float)(((mt >> 11) & 3) - 1) * g.lz : 0.0f;
+                    for (int c0 = 0; c0 < jn; c0 += 64) {
+                        const bool ok = c0 + lane < jn;
+                        const int j = j0 + min(c0 + lane, jn - 1);   // every lane loads inside the range
+                        const float4 pj = posfb[j];
+                        const float px = pj.x + shx, py = pj.y + shy, pz = pj.z + shz;
+                        // keep candidates within the filter radius of the rows' box
+                        const float gx = fmaxf(fabsf(px - bcx) - bhx, 0.0f);
+                        const float gy = fmaxf(fabsf(py - bcy) - bhy, 0.0f);
+                        const float gz = fmaxf(fabsf(pz - bcz) - bhz, 0.0f);
+                        const bool keep = ok & (wmi | (gx * gx + gy * gy + gz * gz < cutr2));
+                        const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
+                        if (keep) {
+                            const int at = (qh + qn + (int)__builtin_amdgcn_mbcnt_hi(
+                                                         (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))) &
+                                           (GB_QCAP - 1);
+                            qp[at] = make_float4(px, py, pz, pj.w);
+                            qj[at] = j;
+                        }
+                        qn += (int)__popcll(m);
+                        if (qn >= 64) drain(64);
                     }
                 }
             }
         }
+        if (qn > 0) drain(qn);
     }
 
     // row length of the group = longest row; pad the shorter ones
-#pragma unroll
-    for (int t = 0; t < GB_NT; t++)
-        if (cl == 0) srow[rrow[t]] = rk[t];
+    if (lane < GB_ROWS) srow[row0 + lane] = rkl;
     __syncthreads();
     if (wave == 0) {
         int tot = sb + lane < n ? srow[lane] : 0;
@@ -646,7 +642,7 @@ __device__ __forceinline__ void build_grp_body(
     }
     __syncthreads();
     int m = min((smax + 15) & ~15, cap);   // whole 16-entry tiles
-    for (int r = wave * GB_ROWS; r < (wave + 1) * GB_ROWS; r++) {
+    for (int r = row0; r < row0 + GB_ROWS; r++) {
         int k0 = min(srow[r], cap);
         for (int k = k0 + lane; k < m; k += 64) grow[ttc_nbl_idx(k, r)] = TTC_PAD;
     }
