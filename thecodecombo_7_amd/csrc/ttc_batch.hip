@@ -47,7 +47,7 @@ int ttc_bstage_reduce(const std::vector<ttc_ctx *> &r, TtcStage *st);
 int ttc_bstage_drift(const std::vector<ttc_ctx *> &r, TtcStage *st);
 int ttc_bstage_fft(const std::vector<ttc_ctx *> &r, TtcStage *st /* [5] */);
 int ttc_bstage_gather(const std::vector<ttc_ctx *> &r, TtcStage *st);
-int ttc_bstage_rebuild(const std::vector<ttc_ctx *> &r, TtcStage *st /* [4] */);
+int ttc_bstage_rebuild(const std::vector<ttc_ctx *> &r, TtcStage *st /* [3] */);
 int ttc_bstage_pair(const std::vector<ttc_ctx *> &r, TtcStage *st);
 void ttc_blaunch_kinet(const TtcStage &st, hipStream_t s);
 void ttc_blaunch_nhc(const TtcStage &st, hipStream_t s);
@@ -82,7 +82,7 @@ void ttc_stage_free(TtcStage *st) {
 }
 
 // stages of the batched NHC step
-enum { BS_KINET = 0, BS_NHC, BS_VSCALE, BS_DRIFT, BS_FFT, BS_REBUILD = BS_FFT + 5, BS_GATHER = BS_REBUILD + 4,
+enum { BS_KINET = 0, BS_NHC, BS_VSCALE, BS_DRIFT, BS_FFT, BS_REBUILD = BS_FFT + 5, BS_GATHER = BS_REBUILD + 3,
        BS_PAIR, BS_KICK, BS_REDUCE, BS_N };
 
 struct ttc_batch {

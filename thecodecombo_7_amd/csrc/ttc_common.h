@@ -176,7 +176,7 @@ struct ttc_ctx {
     double t_pair = 0, t_spread = 0, t_step = 0;
 };
 
-enum { ST_NREBUILD = 4, ST_STEP = 5, ST_NROWS = 6 };
+enum { ST_NREBUILD = 4, ST_STEP = 5, ST_NROWS = 6, ST_TICKET = 7 };
 enum { TTC_NHC_SCALE = 24, TTC_NHC_T = 25, TTC_NHC_Q = 26, TTC_NHC_VMUL = 27, TTC_NHC_LEN = 28 };
 
 #define ROW_LEN 9

@@ -1,13 +1,14 @@
 // ttc_nblist.hip -- neighbour list on the device (replaces nblist.c).
 //
-// Rebuild = four kernels, each a no-op unless the displacement trigger is
+// Rebuild = three kernels, each a no-op unless the displacement trigger is
 // set (so the whole sequence can sit in a captured HIP graph every step):
-//   R1 cell_assign  : cell of every site, counted into fixed-capacity cells
-//   R2 cell_scan    : exclusive scan of the counts (one workgroup)
-//   R3 reorder      : rank inside the cell by original index (deterministic
+//   R1 cell_assign  : cell of every site, counted into fixed-capacity cells;
+//                     the workgroup that finishes last scans the counts into
+//                     cell starts
+//   R2 reorder      : rank inside the cell by original index (deterministic
 //                     order without a sort), scatter the site state into the
 //                     B buffers at its new sorted position
-//   R5 build        : copy B -> A, anchors, and the full Verlet list: the
+//   R3 build        : copy B -> A, anchors, and the full Verlet list: the
 //                     reference's FP64 test for explicit builds (k_build),
 //                     the group-cooperative FP32 build of the pruned list
 //                     inside MD loops (k_build_grp)
@@ -33,87 +34,114 @@ __device__ __forceinline__ bool rebuild_off(const int *status, int conditional) 
     return conditional && status[ST_NLFLAG] == 0;
 }
 
-__device__ __forceinline__ void cell_assign_body(int bid, int n, const double4 *__restrict__ pos4,
-                                                 const CellGeom &g, int cellcap, int *cell_acc, int *cell_of,
-                                                 int *members, int *status, int conditional) {
+// Counts to cell starts in one workgroup of 256 threads: chunks of
+// 256 x 16 cells are read coalesced at device scope (the other workgroups'
+// atomics) into LDS, each thread scans a run of 16, and the starts are
+// written back coalesced; cell_acc is reset to zero on the way.
+#define SCAN_PER 16
+__device__ void cell_scan_block(int ncell, int *cell_acc, int *cell_cnt, int *cell_start, int *status) {
+    __shared__ int sc[256 * SCAN_PER];
+    __shared__ int wsum[4];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int carry = 0;
+    for (int base = 0; base < ncell; base += 256 * SCAN_PER) {
+        int vv[SCAN_PER];
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; k++) {   // all loads in flight at once
+            const int c = base + k * 256 + t;
+            vv[k] = c < ncell ? __hip_atomic_load(&cell_acc[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; k++) {
+            const int c = base + k * 256 + t;
+            if (c < ncell) {
+                cell_cnt[c] = vv[k];
+                cell_acc[c] = 0;
+            }
+            sc[k * 256 + t] = vv[k];
+        }
+        __syncthreads();
+        int sum = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; k++) sum += sc[t * SCAN_PER + k];
+        int incl = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+            int u = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += u;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int run = carry + incl - sum;
+        for (int w = 0; w < wave; w++) run += wsum[w];
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; k++) {
+            const int v = sc[t * SCAN_PER + k];
+            sc[t * SCAN_PER + k] = run;
+            run += v;
+        }
+        carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; k++) {
+            const int c = base + k * 256 + t;
+            if (c < ncell) cell_start[c] = sc[k * 256 + t];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        status[ST_NREBUILD] += 1;
+        status[ST_TICKET] = 0;   // ready for the next rebuild
+    }
+}
+
+// Cell binning of every site; the workgroup that finishes last (atomic
+// ticket, no waiting) turns the counts into cell starts.
+__device__ __forceinline__ void cell_assign_body(int bid, int nblk, int n, const double4 *__restrict__ pos4,
+                                                 const CellGeom &g, int cellcap, int ncell, int *cell_acc,
+                                                 int *cell_cnt, int *cell_start, int *cell_of, int *members,
+                                                 int *status, int conditional) {
     if (rebuild_off(status, conditional)) return;
     int i = bid * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double4 p = pos4[i];
-    int cx = cell_coord(p.x, g.hx, g.lx, g.ncx);
-    int cy = cell_coord(p.y, g.hy, g.ly, g.ncy);
-    int cz = cell_coord(p.z, g.hz, g.lz, g.ncz);
-    int c = cx + g.ncx * (cy + g.ncy * cz);
-    int slot = atomicAdd(&cell_acc[c], 1);
-    cell_of[i] = c;
-    if (slot < cellcap) members[(size_t)c * cellcap + slot] = i;
-    else atomicOr(&status[ST_OVERFLOW], 2);
+    if (i < n) {
+        double4 p = pos4[i];
+        int cx = cell_coord(p.x, g.hx, g.lx, g.ncx);
+        int cy = cell_coord(p.y, g.hy, g.ly, g.ncy);
+        int cz = cell_coord(p.z, g.hz, g.lz, g.ncz);
+        int c = cx + g.ncx * (cy + g.ncy * cz);
+        int slot = atomicAdd(&cell_acc[c], 1);
+        cell_of[i] = c;
+        if (slot < cellcap) members[(size_t)c * cellcap + slot] = i;
+        else atomicOr(&status[ST_OVERFLOW], 2);
+    }
+    // Every count is a device-scope atomic whose old value this workgroup
+    // has already consumed (the slot), so the counts are performed before
+    // the ticket; the last workgroup reads them with device-scope loads.  No
+    // fence is needed (an agent-scope release would write back the L2).
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&status[ST_TICKET], 1) == nblk - 1;
+    __syncthreads();
+    if (!last) return;
+    cell_scan_block(ncell, cell_acc, cell_cnt, cell_start, status);
 }
 
 struct CellAssignPack {
-    int n;
+    int n, nblk;
     const double4 *pos4;
     CellGeom g;
-    int cellcap;
-    int *cell_acc, *cell_of, *members, *status;
+    int cellcap, ncell;
+    int *cell_acc, *cell_cnt, *cell_start, *cell_of, *members, *status;
     int conditional;
 };
 
-__global__ void k_cell_assign(CellAssignPack a) {
-    cell_assign_body(blockIdx.x, a.n, a.pos4, a.g, a.cellcap, a.cell_acc, a.cell_of, a.members, a.status,
-                     a.conditional);
+__device__ __forceinline__ void cell_assign_pack_body(int bid, const CellAssignPack &a) {
+    cell_assign_body(bid, a.nblk, a.n, a.pos4, a.g, a.cellcap, a.ncell, a.cell_acc, a.cell_cnt, a.cell_start,
+                     a.cell_of, a.members, a.status, a.conditional);
 }
-__global__ void kb_cell_assign(const CellAssignPack *pk, const TtcBmap *map) {
+__global__ void __launch_bounds__(256) k_cell_assign(CellAssignPack a) { cell_assign_pack_body(blockIdx.x, a); }
+__global__ void __launch_bounds__(256) kb_cell_assign(const CellAssignPack *pk, const TtcBmap *map) {
     const TtcBmap m = map[blockIdx.x];
-    const CellAssignPack a = pk[m.rep];
-    cell_assign_body(m.blk, a.n, a.pos4, a.g, a.cellcap, a.cell_acc, a.cell_of, a.members, a.status, a.conditional);
-}
-
-// one workgroup of 1024 threads; counts -> start, cell_acc reset to zero
-__device__ __forceinline__ void cell_scan_body(int ncell, int *cell_acc, int *cell_cnt, int *cell_start,
-                                               int *status, int conditional) {
-    if (rebuild_off(status, conditional)) return;
-    __shared__ int sh[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) {
-        carry = 0;
-        status[ST_NREBUILD] += 1;
-    }
-    __syncthreads();
-    for (int base = 0; base < ncell; base += 1024) {
-        int i = base + threadIdx.x;
-        int v = i < ncell ? cell_acc[i] : 0;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            int t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < ncell) {
-            cell_start[i] = carry + sh[threadIdx.x] - v;
-            cell_cnt[i] = v;
-            cell_acc[i] = 0;
-        }
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += sh[1023];
-        __syncthreads();
-    }
-}
-
-struct CellScanPack {
-    int ncell;
-    int *cell_acc, *cell_cnt, *cell_start, *status;
-    int conditional;
-};
-
-__global__ void k_cell_scan(CellScanPack a) {
-    cell_scan_body(a.ncell, a.cell_acc, a.cell_cnt, a.cell_start, a.status, a.conditional);
-}
-__global__ void kb_cell_scan(const CellScanPack *pk, const TtcBmap *map) {
-    const CellScanPack a = pk[map[blockIdx.x].rep];
-    cell_scan_body(a.ncell, a.cell_acc, a.cell_cnt, a.cell_start, a.status, a.conditional);
+    cell_assign_pack_body(m.blk, pk[m.rep]);
 }
 
 struct ReorderPack {
@@ -797,12 +825,8 @@ static void launch_build_grp(ttc_ctx *c) {
 }
 
 static CellAssignPack cell_assign_pack(ttc_ctx *c, int cond) {
-    return CellAssignPack{c->n, c->pos4, geom(c), ttc_cellcap(c), c->cell_acc, c->cell_of, c->cell_members,
-                          c->status, cond};
-}
-
-static CellScanPack cell_scan_pack(ttc_ctx *c, int cond) {
-    return CellScanPack{c->ncell, c->cell_acc, c->cell_cnt, c->cell_start, c->status, cond};
+    return CellAssignPack{c->n, (c->n + 255) / 256, c->pos4, geom(c), ttc_cellcap(c), c->ncell, c->cell_acc,
+                          c->cell_cnt, c->cell_start, c->cell_of, c->cell_members, c->status, cond};
 }
 
 static ReorderPack reorder_pack(ttc_ctx *c, int cond) {
@@ -815,7 +839,6 @@ int ttc_launch_rebuild(ttc_ctx *c, bool conditional) {
     int blocks = (c->n + 255) / 256;
     int cond = conditional ? 1 : 0;
     k_cell_assign<<<blocks, 256, 0, c->stream>>>(cell_assign_pack(c, cond));
-    k_cell_scan<<<1, 1024, 0, c->stream>>>(cell_scan_pack(c, cond));
     k_reorder<<<blocks, 256, 0, c->stream>>>(reorder_pack(c, cond));
     // explicit builds (nblist.c:32) make the reference's list; the
     // conditional builds of MD loops make the pruned one
@@ -917,33 +940,28 @@ int ttc_export_list(ttc_ctx *c, double cutoff, int max_nb, int *count, int *list
 }
 
 // ---- batched stages of the NHC step (ttc_batch.hip): the conditional
-// rebuild of every context, st[0..3] = assign, scan, reorder, build
+// rebuild of every context, st[0..2] = assign (+ scan), reorder, build
 
 int ttc_bstage_rebuild(const std::vector<ttc_ctx *> &r, TtcStage *st) {
     std::vector<CellAssignPack> pa;
-    std::vector<CellScanPack> ps;
     std::vector<ReorderPack> pr;
     std::vector<BuildGrpPack> pb;
-    std::vector<int> nsite, one, ngrp;
+    std::vector<int> nsite, ngrp;
     for (ttc_ctx *c : r) {
         pa.push_back(cell_assign_pack(c, 1));
-        ps.push_back(cell_scan_pack(c, 1));
         pr.push_back(reorder_pack(c, 1));
         pb.push_back(build_grp_pack_of(c));
         nsite.push_back((c->n + 255) / 256);
-        one.push_back(1);
         ngrp.push_back(c->G);
     }
     int e = ttc_stage_make(&st[0], pa, nsite, 0);
-    if (!e) e = ttc_stage_make(&st[1], ps, one, 0);
-    if (!e) e = ttc_stage_make(&st[2], pr, nsite, 0);
-    if (!e) e = ttc_stage_make(&st[3], pb, ngrp, 0);
+    if (!e) e = ttc_stage_make(&st[1], pr, nsite, 0);
+    if (!e) e = ttc_stage_make(&st[2], pb, ngrp, 0);
     return e;
 }
 
 void ttc_blaunch_rebuild(const TtcStage *st, hipStream_t s) {
     kb_cell_assign<<<st[0].nblk, 256, 0, s>>>((const CellAssignPack *)st[0].pk, st[0].map);
-    kb_cell_scan<<<st[1].nblk, 1024, 0, s>>>((const CellScanPack *)st[1].pk, st[1].map);
-    kb_reorder<<<st[2].nblk, 256, 0, s>>>((const ReorderPack *)st[2].pk, st[2].map);
-    kb_build_grp<<<st[3].nblk, 64 * GB_WAVES, 0, s>>>((const BuildGrpPack *)st[3].pk, st[3].map);
+    kb_reorder<<<st[1].nblk, 256, 0, s>>>((const ReorderPack *)st[1].pk, st[1].map);
+    kb_build_grp<<<st[2].nblk, 64 * GB_WAVES, 0, s>>>((const BuildGrpPack *)st[2].pk, st[2].map);
 }
