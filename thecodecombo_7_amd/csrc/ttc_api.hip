@@ -89,7 +89,7 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
     ALLOC(c->pos4, n); ALLOC(c->vel4, n); ALLOC(c->f4, n); ALLOC(c->anchor, n);
     ALLOC(c->pos4b, n); ALLOC(c->vel4b, n); ALLOC(c->f4b, n); ALLOC(c->posfb, n);
     ALLOC(c->type, n); ALLOC(c->typeb, n); ALLOC(c->orig, n); ALLOC(c->origb, n); ALLOC(c->inv, n);
-    ALLOC(c->fpair, n); ALLOC(c->g4, n);
+    ALLOC(c->fpair, n); ALLOC(c->g4, n); ALLOC(c->snap, n); ALLOC(c->g4o, n);
     ALLOC(c->fs, n); ALLOC(c->fl, n); ALLOC(c->fsb, n); ALLOC(c->flb, n);
     TTC_HIP(hipMemset(c->fs, 0, sizeof(double4) * n));
     TTC_HIP(hipMemset(c->fl, 0, sizeof(double4) * n));
@@ -144,8 +144,8 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
 }
 
 static void destroy_graphs(ttc_ctx *c) {
-    hipGraphExec_t *gs[] = {&c->g_plain, &c->g_energy, &c->g_nhc_plain, &c->g_nhc_energy, &c->g_mts_plain,
-                            &c->g_mts_energy};
+    hipGraphExec_t *gs[] = {&c->g_vv[0][0], &c->g_vv[0][1], &c->g_vv[1][0], &c->g_vv[1][1], &c->g_nhc_plain,
+                            &c->g_nhc_energy, &c->g_mts_plain, &c->g_mts_energy};
     for (hipGraphExec_t *g : gs) {
         if (*g) hipGraphExecDestroy(*g);
         *g = nullptr;
@@ -160,7 +160,7 @@ extern "C" int ttc_destroy(ttc_ctx *c) {
                     c->origb, c->inv, c->fpair, c->g4, c->cell_of, c->cell_cnt, c->cell_start, c->cell_acc,
                     c->cell_members, c->nbl, c->glen, c->status, c->ex_off, c->ex_part, c->ex_flag, c->ljtab,
                     c->b_off, c->b_part, c->b_r0, c->b_k, c->partials, c->ev, c->rows, c->fs, c->fl, c->fsb,
-                    c->flb, c->nhc, c->posfb, c->gcnt};
+                    c->flb, c->nhc, c->posfb, c->gcnt, c->snap, c->g4o};
     for (void *b : bufs)
         if (b) hipFree(b);
     ttc_pme_free(c);
@@ -634,9 +634,10 @@ extern "C" int ttc_kinet(ttc_ctx *c, double *kinet, double *ken6) {
 // the drift checks the displacement and the conditional rebuild runs
 // before this step's forces, with the same anchors (this step's positions)
 // the reference rebuilds at after the step.
-// The FFTs need only the charge grid the drift spread, so they run on the
-// side stream while the main stream rebuilds the list; the gather (needing
-// the sorted order of the rebuild) runs there while the pair kernel runs.
+// The FFTs need only the charge grid the drift spread, and the gather works
+// in the original numbering on a snapshot the drift writes, so both run on
+// the side stream while the main stream rebuilds the list and runs the pair
+// kernel: one fork and one join per step.
 void ttc_side_fork(ttc_ctx *c, int k) {
     hipEventRecord(c->sev[k], c->stream);
     hipStreamWaitEvent(c->side, c->sev[k], 0);
@@ -647,29 +648,33 @@ void ttc_side_join(ttc_ctx *c) {
     hipStreamWaitEvent(c->stream, c->sev[2], 0);
 }
 
-static void launch_step(ttc_ctx *c, double vscal, bool energy) {
-    ttc_launch_drift(c, vscal);
+//
+// The second half kick of a plain step is deferred into the next step's
+// drift kernel (head = true), which saves a dependent launch per step; an
+// energy step ends with its own kick (the kinetic energy of the row) and the
+// run with one more kick, so every call leaves the state a full step ends
+// with.  The pair kernel counts the step and clears the trigger flag.
+static void launch_step(ttc_ctx *c, double vscal, bool energy, bool head) {
+    ttc_launch_md_drift(c, vscal, head);
     ttc_side_fork(c, 0);
     {
         TtcOnSide side(c);
         ttc_launch_fft(c, energy);
+        ttc_launch_gather_orig(c, energy);
     }
     ttc_launch_rebuild(c, true);
-    ttc_side_fork(c, 1);
-    {
-        TtcOnSide side(c);
-        ttc_launch_gather(c, energy, true, true, false);
-    }
-    ttc_launch_pair(c, energy, 0);
+    ttc_launch_pair_step(c, energy);
     ttc_side_join(c);
-    ttc_launch_kick(c, vscal, energy);
-    if (energy) ttc_launch_reduce(c, true);
+    if (energy) {
+        ttc_launch_kick_tail(c, vscal, true);
+        ttc_launch_reduce(c, true);
+    }
 }
 
-static int capture(ttc_ctx *c, double vscal, bool energy, hipGraphExec_t *exec) {
+static int capture(ttc_ctx *c, double vscal, bool energy, bool head, hipGraphExec_t *exec) {
     hipGraph_t g;
     TTC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    launch_step(c, vscal, energy);
+    launch_step(c, vscal, energy, head);
     TTC_HIP(hipStreamEndCapture(c->stream, &g));
     TTC_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
     hipGraphDestroy(g);
@@ -693,12 +698,13 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
         launch_forces(c, false);
         c->f4_valid = true;
     }
-    if (!c->g_plain || c->g_vscal != vscal) {
+    if (!c->g_vv[0][0] || c->g_vscal != vscal) {
         destroy_graphs(c);
-        r = capture(c, vscal, false, &c->g_plain);
-        if (r) return r;
-        r = capture(c, vscal, true, &c->g_energy);
-        if (r) return r;
+        for (int h = 0; h < 2; h++)
+            for (int e = 0; e < 2; e++) {
+                r = capture(c, vscal, e == 1, h == 1, &c->g_vv[h][e]);
+                if (r) return r;
+            }
         c->g_vscal = vscal;
     }
     // counters; the trigger for step 0 is checked on the current positions
@@ -708,10 +714,13 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
     TTC_HIP(hipEventCreate(&e0));
     TTC_HIP(hipEventCreate(&e1));
     TTC_HIP(hipEventRecord(e0, c->stream));
+    bool kicked = true;   // the state ends a full step: no kick pending
     for (int step = 1; step <= nsteps; step++) {
         bool en = energy_every > 0 && step % energy_every == 0;
-        TTC_HIP(hipGraphLaunch(en ? c->g_energy : c->g_plain, c->stream));
+        TTC_HIP(hipGraphLaunch(c->g_vv[kicked ? 0 : 1][en ? 1 : 0], c->stream));
+        kicked = en;
     }
+    if (!kicked) ttc_launch_kick_tail(c, vscal, false);
     TTC_HIP(hipEventRecord(e1, c->stream));
     r = read_status(c);
     if (r) return r;

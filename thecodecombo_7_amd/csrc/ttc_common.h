@@ -125,6 +125,10 @@ struct ttc_ctx {
     // force contributions of one evaluation (sorted order)
     double4 *fpair = nullptr;       // pair kernel
     double4 *g4 = nullptr;          // PME gather + bonds
+    // MD step: positions + charge by original index (written by the drift)
+    // and the gather's output by original index, so the gather does not
+    // wait for the step's rebuild
+    double4 *snap = nullptr, *g4o = nullptr;
     bool f4_valid = false;          // f4 holds the force at the current positions
     // RESPA short (bonded) and long (nonbonded) forces, sorted order, permuted
     // with every rebuild like f4
@@ -165,7 +169,9 @@ struct ttc_ctx {
 
     // graph; gen counts reallocations of buffers that captured graphs
     // reference (list, energy rows), so holders of graphs can re-capture
-    hipGraphExec_t g_plain = nullptr, g_energy = nullptr;
+    // velocity-Verlet step graphs [head][energy]: head = the previous
+    // step's second half kick fused into this step's drift
+    hipGraphExec_t g_vv[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     unsigned gen = 0;
     double g_vscal = 1.0;
 
@@ -197,6 +203,9 @@ int ttc_cellcap(const ttc_ctx *c);
 // pair; the range variant weights each pair by whether a site lies in
 // [lb,ub) of the original numbering (energies only)
 void ttc_launch_pair(ttc_ctx *c, bool energy, int blk_off);
+// pair kernel of an MD step: also counts the step and clears the trigger
+// flag the step's conditional rebuild has consumed
+void ttc_launch_pair_step(ttc_ctx *c, bool energy);
 void ttc_launch_pair_range(ttc_ctx *c, int lb, int ub);
 void ttc_launch_pair_hes(ttc_ctx *c);    // energies + Hessian diagonal (CONFIGT)
 int ttc_pair_blocks(ttc_ctx *c);
@@ -229,11 +238,19 @@ int ttc_pme_conv_blocks(ttc_ctx *c);
 int ttc_gather_blocks(ttc_ctx *c);
 void ttc_launch_spread(ttc_ctx *c);
 void ttc_launch_drift(ttc_ctx *c, double vscal);
+// drift of an MD step: writes the original-order snapshot for the gather;
+// kick = the previous step's second half kick fused ahead of it
+void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick);
+// the gather in the original numbering (snapshot in, g4o out)
+void ttc_launch_gather_orig(ttc_ctx *c, bool energy);
 // md / combine / reductions
 // mask bits: 1 pair, 2 gather (PME + bonds), 4 remove mean PME force
 void ttc_launch_combine(ttc_ctx *c, int mask);
 void ttc_launch_combine_to(ttc_ctx *c, int mask, double4 *out);
 void ttc_launch_kick(ttc_ctx *c, double vscal, bool energy);
+// the same kick leaving the step counter and the trigger flag alone (the
+// MD loop's pair kernel keeps them), on the original-order gather output
+void ttc_launch_kick_tail(ttc_ctx *c, double vscal, bool energy);
 // v = A v + B (100/m) F; optional trigger check, KE partials, flag clear
 void ttc_launch_vkick(ttc_ctx *c, const double4 *F, double A, double B, bool check, bool ke, bool clear_flag);
 // v = A v + B (100/m) F, then x += dt v and pbc (no spreading)
@@ -290,6 +307,43 @@ __device__ __forceinline__ size_t ttc_nbl_idx(int k, int l) {
 __device__ __forceinline__ double ttc_wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+// mean PME force over all sites from the gather's per-block sums, reduced
+// in a fixed order (lane-strided, then a shuffle tree), wave d for
+// component d; every thread of the workgroup (at least 192) must call it
+// (it ends with a barrier)
+__device__ __forceinline__ void ttc_mean_pme(const double *partials, int maxb, int nbg, int n, double *sh3) {
+    const int d = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (d < 3) {
+        double t = 0.0;
+        for (int b = lane; b < nbg; b += 64) t += partials[(size_t)(EV_FSX + d) * maxb + b];
+        t = ttc_wave_sum(t);
+        if (lane == 0) sh3[d] = t / (double)n;
+    }
+    __syncthreads();
+}
+
+// The per-site force of one evaluation: pair force (mask 1) + PME and bond
+// force of the gather (mask 2) - mean PME force (mask 4), added as int64
+// fixed point (2^-32 kJ/mol/A) so the total does not depend on the order.
+// The gather force is read at gi (the site's original index where the
+// gather ran in the original numbering), the pair force at s.
+__device__ __forceinline__ double4 ttc_combine(int s, int mask, const double4 *fpair, const double4 *g4,
+                                               const double *mean, int gi = -1) {
+    long long fx = 0, fy = 0, fz = 0;
+    if (mask & 1) {
+        double4 a = fpair[s];
+        fx += ttc_to_fix(a.x); fy += ttc_to_fix(a.y); fz += ttc_to_fix(a.z);
+    }
+    if (mask & 2) {
+        double4 b = g4[gi < 0 ? s : gi];
+        fx += ttc_to_fix(b.x); fy += ttc_to_fix(b.y); fz += ttc_to_fix(b.z);
+    }
+    if (mask & 4) {
+        fx -= ttc_to_fix(mean[0]); fy -= ttc_to_fix(mean[1]); fz -= ttc_to_fix(mean[2]);
+    }
+    return make_double4((double)fx * TTC_FIX_INV, (double)fy * TTC_FIX_INV, (double)fz * TTC_FIX_INV, 0.0);
 }
 
 // ---- batched launches ------------------------------------------------------

@@ -10,45 +10,14 @@
 
 namespace {
 
-// mean PME force over all sites from the gather's per-block sums, reduced
-// in a fixed order (lane-strided, then a shuffle tree) by wave 0
-__device__ __forceinline__ void mean_pme(const double *partials, int maxb, int nbg, int n, double *sh3) {
-    if (threadIdx.x < 64) {
-        for (int d = 0; d < 3; d++) {
-            double t = 0.0;
-            for (int b = threadIdx.x; b < nbg; b += 64) t += partials[(size_t)(EV_FSX + d) * maxb + b];
-            t = ttc_wave_sum(t);
-            if (threadIdx.x == 0) sh3[d] = t / (double)n;
-        }
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ double4 combine(int s, int mask, const double4 *fpair, const double4 *g4,
-                                           const double *mean) {
-    long long fx = 0, fy = 0, fz = 0;
-    if (mask & 1) {
-        double4 a = fpair[s];
-        fx += ttc_to_fix(a.x); fy += ttc_to_fix(a.y); fz += ttc_to_fix(a.z);
-    }
-    if (mask & 2) {
-        double4 b = g4[s];
-        fx += ttc_to_fix(b.x); fy += ttc_to_fix(b.y); fz += ttc_to_fix(b.z);
-    }
-    if (mask & 4) {
-        fx -= ttc_to_fix(mean[0]); fy -= ttc_to_fix(mean[1]); fz -= ttc_to_fix(mean[2]);
-    }
-    return make_double4((double)fx * TTC_FIX_INV, (double)fy * TTC_FIX_INV, (double)fz * TTC_FIX_INV, 0.0);
-}
-
 __global__ void __launch_bounds__(256)
 k_combine(int n, int mask, const double4 *fpair, const double4 *g4, double4 *f4, const double *partials,
           int maxb, int nbg) {
     __shared__ double mean[3];
-    if (mask & 4) mean_pme(partials, maxb, nbg, n, mean);
+    if (mask & 4) ttc_mean_pme(partials, maxb, nbg, n, mean);
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    f4[s] = combine(s, mask, fpair, g4, mean);
+    f4[s] = ttc_combine(s, mask, fpair, g4, mean);
 }
 
 // kinet.c:18-68 with uu = vv: ken = 0.125 m (v+u)^2 * NA*AU*10
@@ -78,18 +47,18 @@ __device__ __forceinline__ void ken_block(int bid, double m, double4 v, double *
 template <bool ENERGY>
 __device__ __forceinline__ void kick_body(int bid, int n, double4 *vel4, const double4 *fpair, const double4 *g4,
                                           double4 *f4, double dt, double vscal, double *partials, int maxb, int nbg,
-                                          int *status) {
+                                          int *status, const int *gorig = nullptr) {
     __shared__ double mean[3];
-    mean_pme(partials, maxb, nbg, n, mean);
+    ttc_mean_pme(partials, maxb, nbg, n, mean);
     int s = bid * blockDim.x + threadIdx.x;
-    if (s == 0) {
+    if (s == 0 && status) {
         status[ST_STEP] += 1;
         status[ST_NLFLAG] = 0;
     }
     double4 v = make_double4(0, 0, 0, 1);
     bool ok = s < n;
     if (ok) {
-        double4 f = combine(s, 7, fpair, g4, mean);
+        double4 f = ttc_combine(s, 7, fpair, g4, mean, gorig ? gorig[s] : s);
         f4[s] = f;
         v = vel4[s];
         double dth = 0.5 * dt;
@@ -110,17 +79,20 @@ struct KickPack {
     double *partials;
     int maxb, nbg;
     int *status;
+    const int *gorig = nullptr;   // g4 in the original numbering: read at gorig[s]
 };
 
 template <bool ENERGY>
 __global__ void __launch_bounds__(256) k_kick(KickPack a) {
-    kick_body<ENERGY>(blockIdx.x, a.n, a.vel4, a.fpair, a.g4, a.f4, a.dt, a.vscal, a.partials, a.maxb, a.nbg, a.status);
+    kick_body<ENERGY>(blockIdx.x, a.n, a.vel4, a.fpair, a.g4, a.f4, a.dt, a.vscal, a.partials, a.maxb, a.nbg, a.status,
+                      a.gorig);
 }
 template <bool ENERGY>
 __global__ void __launch_bounds__(256) kb_kick(const KickPack *pk, const TtcBmap *map) {
     const TtcBmap m = map[blockIdx.x];
     const KickPack a = pk[m.rep];
-    kick_body<ENERGY>(m.blk, a.n, a.vel4, a.fpair, a.g4, a.f4, a.dt, a.vscal, a.partials, a.maxb, a.nbg, a.status);
+    kick_body<ENERGY>(m.blk, a.n, a.vel4, a.fpair, a.g4, a.f4, a.dt, a.vscal, a.partials, a.maxb, a.nbg, a.status,
+                      a.gorig);
 }
 
 __device__ __forceinline__ void kinet_body(int bid, int n, const double4 *vel4, double *partials, int maxb) {
@@ -238,6 +210,16 @@ void ttc_launch_kick(ttc_ctx *c, double vscal, bool energy) {
     else k_kick<false><<<blocks, 256, 0, c->stream>>>(kick_pack(c, vscal));
 }
 
+void ttc_launch_kick_tail(ttc_ctx *c, double vscal, bool energy) {
+    int blocks = ttc_kick_blocks(c);
+    KickPack a = kick_pack(c, vscal);
+    a.status = nullptr;
+    a.g4 = c->g4o;
+    a.gorig = c->orig;
+    if (energy) k_kick<true><<<blocks, 256, 0, c->stream>>>(a);
+    else k_kick<false><<<blocks, 256, 0, c->stream>>>(a);
+}
+
 static KinetPack kinet_pack(ttc_ctx *c) { return KinetPack{c->n, c->vel4, c->partials, c->nblk_max}; }
 
 void ttc_launch_kinet(ttc_ctx *c) {
@@ -272,10 +254,10 @@ __global__ void __launch_bounds__(256)
 k_combine_to(int n, int mask, const double4 *fpair, const double4 *g4, double4 *out, const double *partials,
              int maxb, int nbg) {
     __shared__ double mean[3];
-    if (mask & 4) mean_pme(partials, maxb, nbg, n, mean);
+    if (mask & 4) ttc_mean_pme(partials, maxb, nbg, n, mean);
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    out[s] = combine(s, mask, fpair, g4, mean);
+    out[s] = ttc_combine(s, mask, fpair, g4, mean);
 }
 
 // v = A v + B (100/m) F; the first half kick of integrate_vverlet.c:57-59 is

@@ -279,10 +279,15 @@ struct PairPack {
     double *partials;
     int maxb;
     RangeSel rs;
+    int *stepst = nullptr;   // MD step: count the step, clear the consumed trigger flag
 };
 
 template <bool ENERGY, bool RANGE, bool HES, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES) k_pair(PairPack a) {
+    if (a.stepst && blockIdx.x == 0 && threadIdx.x == 0) {
+        a.stepst[ST_STEP] += 1;
+        a.stepst[ST_NLFLAG] = 0;
+    }
     pair_body<ENERGY, RANGE, HES, WAVES>(blockIdx.x, a.n, a.cap, a.nbl, a.glen, a.pos4, a.type, a.ljtab, a.pc, a.fpair,
                                          a.partials, a.maxb, a.rs);
 }
@@ -324,10 +329,11 @@ static PairPack pair_pack(ttc_ctx *c, RangeSel rs) {
 }
 
 template <bool ENERGY, bool RANGE, bool HES = false>
-static void launch_pair_w(ttc_ctx *c, RangeSel rs) {
+static void launch_pair_w(ttc_ctx *c, RangeSel rs, int *stepst = nullptr) {
     int blocks = ttc_pair_blocks(c);
     size_t shm = sizeof(float4) * c->ntypes * c->ntypes;
     PairPack a = pair_pack(c, rs);
+    a.stepst = stepst;
 #define PAIR_LAUNCH(W) k_pair<ENERGY, RANGE, HES, W><<<blocks, 64 * W, shm, c->stream>>>(a)
     switch (c->pair_waves) {
     case 4: PAIR_LAUNCH(4); break;
@@ -348,6 +354,11 @@ void ttc_launch_pair_hes(ttc_ctx *c) {
 void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
     if (energy) launch_pair_w<true, false>(c, RangeSel{});
     else launch_pair_w<false, false>(c, RangeSel{});
+}
+
+void ttc_launch_pair_step(ttc_ctx *c, bool energy) {
+    if (energy) launch_pair_w<true, false>(c, RangeSel{}, c->status);
+    else launch_pair_w<false, false>(c, RangeSel{}, c->status);
 }
 
 // ---- batched stage of the NHC step (ttc_batch.hip): every context's pair

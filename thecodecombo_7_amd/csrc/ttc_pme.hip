@@ -191,6 +191,19 @@ __device__ __forceinline__ int tile_base(int fl, int org, int ext, int k, int P)
     return -1;
 }
 
+// inputs of the fused second half kick: pair force, gather force, the
+// gather's per-block sums (mean PME force), site count, velocity scale
+struct KickIn {
+    const double4 *fpair = nullptr, *g4 = nullptr;
+    const double *partials = nullptr;
+    int maxb = 0, nbg = 0, n = 0;
+    double vscal = 1.0;
+    // original numbering: g4 is read at orig[s]; the new position + charge
+    // is written to snap[orig[s]] when snap is set
+    const int *orig = nullptr;
+    double4 *snap = nullptr;
+};
+
 // kick + drift + pbc (integrate_vverlet.c:52-83) of every site of a brick,
 // the DLIST displacement check of nblist.c:607-647 on the new position
 // (pcheck = {lx, ly, lz, (rclist-rc)/4}; it sets the flag the conditional
@@ -199,13 +212,19 @@ __device__ __forceinline__ int tile_base(int fl, int org, int ext, int k, int P)
 // (pme_qgrid.c:77-113) into an LDS tile of int64 fixed point, flushed to
 // the global fixed-point grid with one atomic per touched grid point.
 // Integer adds make the grid independent of order.
-template <int P, bool DRIFT>
-__device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 *vel4, const double4 *f4, double dt,
+//
+// KICK: the previous step's second half kick (integrate_vverlet.c:93-102)
+// first, on the force combined here from the pair and gather outputs (kept
+// in f4), exactly as the separate kick kernel computes it.
+template <int P, bool DRIFT, bool KICK = false>
+__device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 *vel4, double4 *f4, double dt,
                                                   double vscal, const Spread &sp, const Brick &br,
                                                   const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
                                                   long long *qfix, int *status, const double4 *__restrict__ anchor,
-                                                  const double *__restrict__ pcheck) {
+                                                  const double *__restrict__ pcheck, const KickIn &ki = KickIn{}) {
     extern __shared__ unsigned long long tile[];
+    __shared__ double mean[3];
+    if (KICK) ttc_mean_pme(ki.partials, ki.maxb, ki.nbg, ki.n, mean);
     int ibx = b % br.nbx, iby = (b / br.nbx) % br.nby, ibz = b / (br.nbx * br.nby);
     int cx0 = ibx * br.bx, cy0 = iby * br.by, cz0 = ibz * br.bz;
     int cx1 = min(cx0 + br.bx, br.ncx), cy1 = min(cy0 + br.by, br.ncy), cz1 = min(cz0 + br.bz, br.ncz);
@@ -225,9 +244,19 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
                 double4 p = pos4[s];
                 if (DRIFT) {
                     double4 v = vel4[s];
-                    double4 f = f4[s];
+                    double4 f;
                     double dth = 0.5 * dt;
                     double im = v.w;   // 100/m
+                    const int so = ki.orig ? ki.orig[s] : s;
+                    if (KICK) {
+                        f = ttc_combine(s, 7, ki.fpair, ki.g4, mean, so);
+                        f4[s] = f;
+                        v.x = v.x + ki.vscal * dth * v.w * f.x;
+                        v.y = v.y + ki.vscal * dth * v.w * f.y;
+                        v.z = v.z + ki.vscal * dth * v.w * f.z;
+                    } else {
+                        f = f4[s];
+                    }
                     v.x = vscal * (v.x + dth * im * f.x);
                     v.y = vscal * (v.y + dth * im * f.y);
                     v.z = vscal * (v.z + dth * im * f.z);
@@ -236,6 +265,7 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
                     p.z = pbc1(p.z + dt * v.z, sp.lz);
                     vel4[s] = v;
                     pos4[s] = p;
+                    if (ki.snap) ki.snap[so] = p;
                     double4 a = anchor[s];
                     double rcut = pcheck[3];
                     double dx = fabs(p.x - a.x);
@@ -293,7 +323,7 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
 
 struct DsPack {
     double4 *pos4, *vel4;
-    const double4 *f4;
+    double4 *f4;
     double dt, vscal;
     Spread sp;
     Brick br;
@@ -302,12 +332,13 @@ struct DsPack {
     int *status;
     const double4 *anchor;
     const double *pcheck;
+    KickIn ki;
 };
 
-template <int P, bool DRIFT>
+template <int P, bool DRIFT, bool KICK = false>
 __global__ void __launch_bounds__(256) k_drift_spread(DsPack a) {
-    drift_spread_body<P, DRIFT>(blockIdx.x, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start, a.cell_cnt,
-                                a.qfix, a.status, a.anchor, a.pcheck);
+    drift_spread_body<P, DRIFT, KICK>(blockIdx.x, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start,
+                                      a.cell_cnt, a.qfix, a.status, a.anchor, a.pcheck, a.ki);
 }
 template <int P, bool DRIFT>
 __global__ void __launch_bounds__(256) kb_drift_spread(const DsPack *pk, const TtcBmap *map) {
@@ -679,10 +710,10 @@ __device__ __forceinline__ void gather_body(int bid, int n, const double4 *__res
         }
         // bonds of this site (cbond.c:65-120), each bond seen from both ends
         double bx = 0.0, by = 0.0, bz = 0.0;
-        int o = orig[s];
+        int o = orig ? orig[s] : s;   // no map: the sites are in the original numbering
         int kb0 = do_bonds ? b_off[o] : 0, kb1 = do_bonds ? b_off[o + 1] : 0;
         for (int k = kb0; k < kb1; k++) {
-            double4 q = pos4[inv[b_part[k]]];
+            double4 q = pos4[inv ? inv[b_part[k]] : b_part[k]];
             double dx = ttc_minimg(q.x - p.x, gc.hx, gc.lx);
             double dy = ttc_minimg(q.y - p.y, gc.hy, gc.ly);
             double dz = ttc_minimg(q.z - p.z, gc.hz, gc.lz);
@@ -910,7 +941,8 @@ static size_t brick_shm(const ttc_ctx *c, const Brick &br) {
 
 static DsPack ds_pack(ttc_ctx *c, bool drift, double vscal) {
     return DsPack{c->pos4, c->vel4, c->f4, drift ? c->p.dt : 0.0, vscal, spread_args(c), brick_args(c),
-                  c->cell_start, c->cell_cnt, c->qfix, c->status, c->anchor, c->ev + EV_N};
+                  c->cell_start, c->cell_cnt, c->qfix, c->status, c->anchor, c->ev + EV_N,
+                  KickIn{c->fpair, c->g4, c->partials, c->nblk_max, ttc_gather_blocks(c), c->n, vscal}};
 }
 
 static int ds_blocks(const ttc_ctx *c) {
@@ -933,6 +965,19 @@ void ttc_launch_spread(ttc_ctx *c) { launch_drift_spread(c, false, 1.0); }
 
 void ttc_launch_drift(ttc_ctx *c, double vscal) { launch_drift_spread(c, true, vscal); }
 
+void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick) {
+    int blocks = ds_blocks(c);
+    size_t shm = brick_shm(c, brick_args(c));
+    DsPack a = ds_pack(c, true, vscal);
+    a.ki.g4 = c->g4o;
+    a.ki.orig = c->orig;
+    a.ki.snap = c->snap;
+#define DS(P, K) k_drift_spread<P, true, K><<<blocks, 256, shm, c->stream>>>(a)
+    if (c->order == 4) { if (kick) DS(4, true); else DS(4, false); }
+    else { if (kick) DS(6, true); else DS(6, false); }
+#undef DS
+}
+
 int ttc_spread_shm(const ttc_ctx *c) { return (int)brick_shm(c, brick_args(c)); }
 
 // allow the spread tile and the FFT tiles above 64 KiB of LDS (gfx950 has 160 KiB per CU)
@@ -943,7 +988,8 @@ static int set_lds_attrs(ttc_ctx *c) {
         return TTC_ERR_ARG;
     }
     const void *fns[] = {(const void *)k_drift_spread<4, true>, (const void *)k_drift_spread<4, false>,
-                         (const void *)k_drift_spread<6, true>, (const void *)k_drift_spread<6, false>};
+                         (const void *)k_drift_spread<6, true>, (const void *)k_drift_spread<6, false>,
+                         (const void *)k_drift_spread<4, true, true>, (const void *)k_drift_spread<6, true, true>};
     for (const void *f : fns) TTC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
     return TTC_OK;
 }
@@ -1001,6 +1047,22 @@ void ttc_launch_gather(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes) 
     } else {
         if (hes) GATHER(6, true, true); else if (energy) GATHER(6, true, false); else GATHER(6, false, false);
     }
+#undef GATHER
+}
+
+void ttc_launch_gather_orig(ttc_ctx *c, bool energy) {
+    double lx = c->p.lx, ly = c->p.ly, lz = c->p.lz;
+    GatherConst gc;
+    gc.lx = lx; gc.ly = ly; gc.lz = lz;
+    gc.hx = 0.5 * lx; gc.hy = 0.5 * ly; gc.hz = 0.5 * lz;
+    gc.K0 = c->K[0]; gc.K1 = c->K[1]; gc.K2 = c->K[2];
+    gc.qscale = TTC_QFACTOR / c->p.eps_rf;
+    int blocks = ttc_gather_blocks(c);
+    GatherPack a{c->n, c->snap, nullptr, nullptr, c->qr, gc, c->b_off, c->b_part, c->b_r0, c->b_k, c->g4o,
+                 c->partials, c->nblk_max, 1, 1, c->theta_near};
+#define GATHER(P, E) k_gather<P, E, false><<<blocks, 256, 0, c->stream>>>(a)
+    if (c->order == 4) { if (energy) GATHER(4, true); else GATHER(4, false); }
+    else { if (energy) GATHER(6, true); else GATHER(6, false); }
 #undef GATHER
 }
 
