@@ -144,7 +144,7 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
 }
 
 static void destroy_graphs(ttc_ctx *c) {
-    hipGraphExec_t *gs[] = {&c->g_vv[0][0], &c->g_vv[0][1], &c->g_vv[1][0], &c->g_vv[1][1], &c->g_nhc_plain,
+    hipGraphExec_t *gs[] = {&c->g_vv[0][0], &c->g_vv[0][1], &c->g_vv[1][0], &c->g_vv[1][1], &c->g_vv_run, &c->g_nhc_plain,
                             &c->g_nhc_energy, &c->g_mts_plain, &c->g_mts_energy};
     for (hipGraphExec_t *g : gs) {
         if (*g) hipGraphExecDestroy(*g);
@@ -656,14 +656,17 @@ void ttc_side_join(ttc_ctx *c) {
 // with.  The pair kernel counts the step and clears the trigger flag.
 static void launch_step(ttc_ctx *c, double vscal, bool energy, bool head) {
     ttc_launch_md_drift(c, vscal, head);
-    ttc_side_fork(c, 0);
+    // the main stream's rebuild and pair kernel are issued before the side
+    // stream's work, so the longer chain is launched first
+    hipEventRecord(c->sev[0], c->stream);
+    ttc_launch_rebuild(c, true);
+    ttc_launch_pair_step(c, energy);
+    hipStreamWaitEvent(c->side, c->sev[0], 0);
     {
         TtcOnSide side(c);
         ttc_launch_fft(c, energy);
         ttc_launch_gather_orig(c, energy);
     }
-    ttc_launch_rebuild(c, true);
-    ttc_launch_pair_step(c, energy);
     ttc_side_join(c);
     if (energy) {
         ttc_launch_kick_tail(c, vscal, true);
@@ -671,10 +674,10 @@ static void launch_step(ttc_ctx *c, double vscal, bool energy, bool head) {
     }
 }
 
-static int capture(ttc_ctx *c, double vscal, bool energy, bool head, hipGraphExec_t *exec) {
+static int capture(ttc_ctx *c, double vscal, bool energy, bool head, hipGraphExec_t *exec, int nsteps = 1) {
     hipGraph_t g;
     TTC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    launch_step(c, vscal, energy, head);
+    for (int k = 0; k < nsteps; k++) launch_step(c, vscal, energy, head);
     TTC_HIP(hipStreamEndCapture(c->stream, &g));
     TTC_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
     hipGraphDestroy(g);
@@ -705,6 +708,8 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
                 r = capture(c, vscal, e == 1, h == 1, &c->g_vv[h][e]);
                 if (r) return r;
             }
+        r = capture(c, vscal, false, true, &c->g_vv_run, TTC_VV_RUN);
+        if (r) return r;
         c->g_vscal = vscal;
     }
     // counters; the trigger for step 0 is checked on the current positions
@@ -715,10 +720,21 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
     TTC_HIP(hipEventCreate(&e1));
     TTC_HIP(hipEventRecord(e0, c->stream));
     bool kicked = true;   // the state ends a full step: no kick pending
-    for (int step = 1; step <= nsteps; step++) {
+    for (int step = 1; step <= nsteps;) {
+        // TTC_VV_RUN plain steps in one graph (no launch gap between them)
+        // when a kick is pending and none of them writes an energy row
+        int last = step + TTC_VV_RUN - 1;
+        bool run = !kicked && last <= nsteps &&
+                   (energy_every <= 0 || last / energy_every == (step - 1) / energy_every);
+        if (run) {
+            TTC_HIP(hipGraphLaunch(c->g_vv_run, c->stream));
+            step = last + 1;
+            continue;
+        }
         bool en = energy_every > 0 && step % energy_every == 0;
         TTC_HIP(hipGraphLaunch(c->g_vv[kicked ? 0 : 1][en ? 1 : 0], c->stream));
         kicked = en;
+        step++;
     }
     if (!kicked) ttc_launch_kick_tail(c, vscal, false);
     TTC_HIP(hipEventRecord(e1, c->stream));

@@ -43,6 +43,7 @@
 #define TTC_QFIX_SCALE 1099511627776.0      /* 2^40 */
 #define TTC_QFIX_INV (1.0 / 1099511627776.0)
 #define TTC_MAX_TYPES 64
+#define TTC_VV_RUN 10                       /* plain MD steps captured in one graph */
 
 // defines.h:60-71, product in the same order as the reference macro
 #define TTC_PI 3.141592653589793238462643383
@@ -172,6 +173,7 @@ struct ttc_ctx {
     // velocity-Verlet step graphs [head][energy]: head = the previous
     // step's second half kick fused into this step's drift
     hipGraphExec_t g_vv[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    hipGraphExec_t g_vv_run = nullptr;   // TTC_VV_RUN plain steps with the deferred kick, one graph
     unsigned gen = 0;
     double g_vscal = 1.0;
 

@@ -17,33 +17,36 @@ namespace {
 #define PAIR_AHEAD 2   // entries whose site gathers are in flight ahead of the arithmetic
 #endif
 
-// relative band around rc^2 inside which the cutoff is decided by the
-// reference's op-by-op FP64 test; outside it r^2 summed with FMAs (a few ulp
-// from the op-by-op sum) decides the same way
-#define PAIR_RC2_BAND 1.0e-12
-
 struct PairConst {
     double lx, ly, lz, hx, hy, hz, rc2;
     double ilx, ily, ilz;     // 1/L per axis
     double erfc_rc_over_rc;   // erfc(kappa rc)/rc with the reference's erfc, FP64
     float rc2f, kappa, kpi2;  // kpi2 = 2 kappa / sqrt(pi)
+    float half_kappa;         // kappa / 2
+    float mk2l2;              // -kappa^2 log2(e)
     float qfac;               // QFACTOR
     float inv_epsrf;
     float e14, one_m_e14;
     int ntypes;
 };
 
-// Numerical Recipes erfcc (ewald.c:817-835) in FP32, argument z >= 0.
+// Numerical Recipes erfcc (ewald.c:817-835) in FP32 at z = kappa r >= 0:
+// erfc(z) = t exp(p(t) - z^2), t = 1 / (1 + z/2), with p the reference's
+// degree-9 polynomial.  Evaluated with base-2 exponentials: the polynomial's
+// coefficients carry the factor log2(e), and -z^2 log2(e) = mk2l2 r^2.
 // Returns erfc(z); *ez2 receives exp(-z^2).
-__device__ __forceinline__ float nr_erfc(float z, float *ez2) {
-    float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * z);
-    float p = -1.26551223f + t * (1.00002368f + t * (0.37409196f + t * (0.09678418f +
-              t * (-0.18628806f + t * (0.27886807f + t * (-1.13520398f +
-              t * (1.48851587f + t * (-0.82215223f + t * 0.17087277f))))))));
-    float e = __expf(-z * z);
-    *ez2 = e;
-    return t * e * __expf(p);
+#define L2E 1.44269504088896341f
+__device__ __forceinline__ float nr_erfc(float r, float r2, const PairConst &pc, float *ez2) {
+    float t = __builtin_amdgcn_rcpf(fmaf(pc.half_kappa, r, 1.0f));
+    float p = (L2E * -1.26551223f) + t * ((L2E * 1.00002368f) + t * ((L2E * 0.37409196f) +
+              t * ((L2E * 0.09678418f) + t * ((L2E * -0.18628806f) + t * ((L2E * 0.27886807f) +
+              t * ((L2E * -1.13520398f) + t * ((L2E * 1.48851587f) + t * ((L2E * -0.82215223f) +
+              t * (L2E * 0.17087277f)))))))));
+    float a = pc.mk2l2 * r2;
+    *ez2 = __builtin_amdgcn_exp2f(a);
+    return t * __builtin_amdgcn_exp2f(p + a);
 }
+#undef L2E
 
 
 struct PairAcc {
@@ -81,10 +84,10 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const 
     ex = fma(-pc.lx, rint(ex * pc.ilx), ex);
     ey = fma(-pc.ly, rint(ey * pc.ily), ey);
     ez = fma(-pc.lz, rint(ez * pc.ilz), ez);
-    const double dr2 = fma(ex, ex, fma(ey, ey, ez * ez));
-    bool inside = dr2 <= pc.rc2;
-    if (fabs(dr2 - pc.rc2) < PAIR_RC2_BAND * pc.rc2) inside = ttc_r2_exact(ex, ey, ez) <= pc.rc2;
-    if (!inside) return;
+    // the reference's cutoff test on r^2 summed op by op (5 FP64 operations,
+    // no band or second form to select between)
+    const double dr2 = ttc_r2_exact(ex, ey, ez);
+    if (!(dr2 <= pc.rc2)) return;
     const float dx = (float)ex, dy = (float)ey, dz = (float)ez;
     const float r2 = (float)dr2;
     if (RANGE && !i_in) {
@@ -103,10 +106,10 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const 
     float sr6 = sr2 * sr2 * sr2;
     float sr12 = sr6 * sr6;
     float force = 12.0f * eps * rinv2 * (sr12 - sr6);
-    float qqab = qi * (float)pj.w * pc.qfac;
+    float qqab = qi * (float)pj.w;   // qi carries QFACTOR
     float qq = (cls == 0 ? qqab : (cls == 2 ? pc.e14 * qqab : 0.0f)) * pc.inv_epsrf;
     float ez2;
-    float erfckr = nr_erfc(pc.kappa * r, &ez2);
+    float erfckr = nr_erfc(r, r2, pc, &ez2);
     float kpid = pc.kpi2 * r * ez2;
     float rinv3 = rinv * rinv2;
     force += qq * rinv3 * (erfckr + kpid);
@@ -191,7 +194,7 @@ __device__ __forceinline__ void pair_body(int grp, int n, int cap, const uint32_
     double4 pi = s < n ? pos4[s] : make_double4(0.0, 0.0, 0.0, 0.0);
     int ti = s < n ? type[s] : 0;
     const float2 *ljrow = slj + ti * pc.ntypes, *ljrow14 = slj + nt2 + ti * pc.ntypes;
-    float qi = (float)pi.w;
+    float qi = (float)pi.w * pc.qfac;
     // this lane's 64-byte segment of tile 0; tile t is 1024 words further
     const uint4 *seg = (const uint4 *)(nbl + (size_t)grp * cap * 64 + (size_t)lane * 16);
     bool i_in = true;
@@ -315,6 +318,8 @@ static PairConst pair_const(const ttc_ctx *c) {
     pc.rc2f = (float)pc.rc2;
     pc.kappa = (float)c->p.kappa;
     pc.kpi2 = (float)(2.0 * c->p.kappa / sqrt(TTC_PI));
+    pc.half_kappa = (float)(0.5 * c->p.kappa);
+    pc.mk2l2 = (float)(-c->p.kappa * c->p.kappa / log(2.0));
     pc.qfac = (float)TTC_QFACTOR;
     pc.inv_epsrf = (float)(1.0 / c->p.eps_rf);
     pc.e14 = (float)c->p.e14fac;

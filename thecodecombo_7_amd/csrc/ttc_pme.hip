@@ -162,12 +162,20 @@ __device__ __forceinline__ double pbc1(double x, double l) {
 // tile covers every grid point its sites can touch: the brick's cells
 // widened by `delta` (sites drift at most (rclist-rc)/4 plus one step from
 // the cell they were sorted into before the trigger rebuilds the list).
+// cells per brick edge: at most 2, so a brick has at most 4 (y, z) rows
+#define BRICK_ROWS 4
+// LDS bytes per site of one spread chunk: q Mx, My, Mz (3P floats) and the
+// packed stencil base
+#define SPREAD_REC_BYTES(P) (3 * (P) * 4 + 4)
+
 struct Brick {
     int ncx, ncy, ncz;
     int bx, by, bz;
     int nbx, nby, nbz;
     double cwx, cwy, cwz;
     double delta;
+    int chunk;     // sites per pass (their weights sit in LDS after the tile)
+    int rec_off;   // start of the weights, in 8-byte words after the tile
 };
 
 __device__ __forceinline__ void tile_range(int c0, int c1, double cw, double l, int k, double delta, int P,
@@ -224,6 +232,12 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
                                                   const double *__restrict__ pcheck, const KickIn &ki = KickIn{}) {
     extern __shared__ unsigned long long tile[];
     __shared__ double mean[3];
+    // one chunk of sites after the tile: q Mx[1..P], My[1..P], Mz[1..P] per
+    // site and the packed tile-local stencil base (x | y << 10 | z << 20;
+    // -1: site spread straight to the grid)
+    const int chunk = br.chunk;
+    float *rw = (float *)(tile + br.rec_off);
+    int *rb = (int *)(rw + chunk * 3 * P);
     if (KICK) ttc_mean_pme(ki.partials, ki.maxb, ki.nbg, ki.n, mean);
     int ibx = b % br.nbx, iby = (b / br.nbx) % br.nby, ibz = b / (br.nbx * br.nby);
     int cx0 = ibx * br.bx, cy0 = iby * br.by, cz0 = ibz * br.bz;
@@ -234,85 +248,121 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
     tile_range(cz0, cz1, br.cwz, sp.lz, sp.K2, br.delta, P, &oz, &tz);
     int tsz = tx * ty * tz;
     for (int t = threadIdx.x; t < tsz; t += blockDim.x) tile[t] = 0ull;
-    __syncthreads();
-    for (int cz = cz0; cz < cz1; cz++)
-        for (int cy = cy0; cy < cy1; cy++) {
+    // the brick's sites are the contiguous site ranges of its (cy, cz) cell
+    // rows (cells are sorted x fastest); up to BRICK_ROWS rows, taken as one
+    // flat index range
+    const int nyr = cy1 - cy0, nrow = nyr * (cz1 - cz0);
+    int rs0[BRICK_ROWS], rend[BRICK_ROWS];   // row start, cumulative end
+    int tot = 0;
+#pragma unroll
+    for (int r = 0; r < BRICK_ROWS; r++) {
+        rs0[r] = 0;
+        if (r < nrow) {
+            int cy = cy0 + r % nyr, cz = cz0 + r / nyr;
             int c0 = cx0 + br.ncx * (cy + br.ncy * cz);
             int cl = (cx1 - 1) + br.ncx * (cy + br.ncy * cz);
-            int s0 = cell_start[c0], s1 = cell_start[cl] + cell_cnt[cl];
-            for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-                double4 p = pos4[s];
-                if (DRIFT) {
-                    double4 v = vel4[s];
-                    double4 f;
-                    double dth = 0.5 * dt;
-                    double im = v.w;   // 100/m
-                    const int so = ki.orig ? ki.orig[s] : s;
-                    if (KICK) {
-                        f = ttc_combine(s, 7, ki.fpair, ki.g4, mean, so);
-                        f4[s] = f;
-                        v.x = v.x + ki.vscal * dth * v.w * f.x;
-                        v.y = v.y + ki.vscal * dth * v.w * f.y;
-                        v.z = v.z + ki.vscal * dth * v.w * f.z;
-                    } else {
-                        f = f4[s];
-                    }
-                    v.x = vscal * (v.x + dth * im * f.x);
-                    v.y = vscal * (v.y + dth * im * f.y);
-                    v.z = vscal * (v.z + dth * im * f.z);
-                    p.x = pbc1(p.x + dt * v.x, sp.lx);
-                    p.y = pbc1(p.y + dt * v.y, sp.ly);
-                    p.z = pbc1(p.z + dt * v.z, sp.lz);
-                    vel4[s] = v;
-                    pos4[s] = p;
-                    if (ki.snap) ki.snap[so] = p;
-                    double4 a = anchor[s];
-                    double rcut = pcheck[3];
-                    double dx = fabs(p.x - a.x);
-                    if (dx > 0.5 * sp.lx) dx -= sp.lx;
-                    double dy = fabs(p.y - a.y);
-                    if (dy > 0.5 * sp.ly) dy -= sp.ly;
-                    double dz = fabs(p.z - a.z);
-                    if (dz > 0.5 * sp.lz) dz -= sp.lz;
-                    if (dx > rcut || dy > rcut || dz > rcut || ttc_r2_exact(dx, dy, dz) > rcut * rcut)
-                        atomicOr(&status[ST_NLFLAG], 1);
-                }
-                double ux = (double)sp.K0 * (p.x / sp.lx + 0.5), uy = (double)sp.K1 * (p.y / sp.ly + 0.5),
-                       uz = (double)sp.K2 * (p.z / sp.lz + 0.5);
-                double fxd = floor(ux), fyd = floor(uy), fzd = floor(uz);
-                int bxl = tile_base((int)fxd, ox, tx, sp.K0, P);
-                int byl = tile_base((int)fyd, oy, ty, sp.K1, P);
-                int bzl = tile_base((int)fzd, oz, tz, sp.K2, P);
-                if (bxl < 0 || byl < 0 || bzl < 0) {
-                    spread_site<P>(p, sp, qfix);   // outside the tile: straight to the grid
-                    continue;
-                }
-                float Mx[P + 1], My[P + 1], Mz[P + 1], D[P + 1];
-                bspline<P>((float)(ux - fxd), Mx, D);
-                bspline<P>((float)(uy - fyd), My, D);
-                bspline<P>((float)(uz - fzd), Mz, D);
-                float q = (float)p.w;
+            rs0[r] = cell_start[c0];
+            tot += cell_start[cl] + cell_cnt[cl] - rs0[r];
+        }
+        rend[r] = tot;
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < tot; c0 += chunk) {
+        // pass 1: one thread per site (kick, drift, trigger check, weights)
+        for (int i = c0 + threadIdx.x; i < min(tot, c0 + chunk); i += blockDim.x) {
+            int s = 0;
 #pragma unroll
-                for (int xi = 0; xi < P; xi++) {
-                    float qx = q * Mx[xi + 1];
-#pragma unroll
-                    for (int yi = 0; yi < P; yi++) {
-                        float qxy = qx * My[yi + 1];
-                        int rowb = ((bxl + xi) * ty + (byl + yi)) * tz + bzl;
-#pragma unroll
-                        for (int zi = 0; zi < P; zi++) {
-                            long long v = __double2ll_rn((double)(qxy * Mz[zi + 1]) * TTC_QFIX_SCALE);
-                            atomicAdd(&tile[rowb + zi], (unsigned long long)v);
-                        }
-                    }
+            for (int r = BRICK_ROWS - 1; r >= 0; r--)
+                if (i < rend[r]) s = rs0[r] + i - (r ? rend[r - 1] : 0);
+            double4 p = pos4[s];
+            if (DRIFT) {
+                double4 v = vel4[s];
+                double4 f;
+                double dth = 0.5 * dt;
+                double im = v.w;   // 100/m
+                const int so = ki.orig ? ki.orig[s] : s;
+                if (KICK) {
+                    f = ttc_combine(s, 7, ki.fpair, ki.g4, mean, so);
+                    f4[s] = f;
+                    v.x = v.x + ki.vscal * dth * v.w * f.x;
+                    v.y = v.y + ki.vscal * dth * v.w * f.y;
+                    v.z = v.z + ki.vscal * dth * v.w * f.z;
+                } else {
+                    f = f4[s];
                 }
+                v.x = vscal * (v.x + dth * im * f.x);
+                v.y = vscal * (v.y + dth * im * f.y);
+                v.z = vscal * (v.z + dth * im * f.z);
+                p.x = pbc1(p.x + dt * v.x, sp.lx);
+                p.y = pbc1(p.y + dt * v.y, sp.ly);
+                p.z = pbc1(p.z + dt * v.z, sp.lz);
+                vel4[s] = v;
+                pos4[s] = p;
+                if (ki.snap) ki.snap[so] = p;
+                double4 a = anchor[s];
+                double rcut = pcheck[3];
+                double dx = fabs(p.x - a.x);
+                if (dx > 0.5 * sp.lx) dx -= sp.lx;
+                double dy = fabs(p.y - a.y);
+                if (dy > 0.5 * sp.ly) dy -= sp.ly;
+                double dz = fabs(p.z - a.z);
+                if (dz > 0.5 * sp.lz) dz -= sp.lz;
+                if (dx > rcut || dy > rcut || dz > rcut || ttc_r2_exact(dx, dy, dz) > rcut * rcut)
+                    atomicOr(&status[ST_NLFLAG], 1);
+            }
+            double ux = (double)sp.K0 * (p.x / sp.lx + 0.5), uy = (double)sp.K1 * (p.y / sp.ly + 0.5),
+                   uz = (double)sp.K2 * (p.z / sp.lz + 0.5);
+            double fxd = floor(ux), fyd = floor(uy), fzd = floor(uz);
+            int bxl = tile_base((int)fxd, ox, tx, sp.K0, P);
+            int byl = tile_base((int)fyd, oy, ty, sp.K1, P);
+            int bzl = tile_base((int)fzd, oz, tz, sp.K2, P);
+            const int k = i - c0;
+            float *w = rw + k * 3 * P;
+            if (bxl < 0 || byl < 0 || bzl < 0) {
+                spread_site<P>(p, sp, qfix);   // outside the tile: straight to the grid
+                rb[k] = -1;
+                continue;
+            }
+            float Mx[P + 1], My[P + 1], Mz[P + 1], D[P + 1];
+            bspline<P>((float)(ux - fxd), Mx, D);
+            bspline<P>((float)(uy - fyd), My, D);
+            bspline<P>((float)(uz - fzd), Mz, D);
+            float q = (float)p.w;
+#pragma unroll
+            for (int xi = 0; xi < P; xi++) {
+                w[xi] = q * Mx[xi + 1];
+                w[P + xi] = My[xi + 1];
+                w[2 * P + xi] = Mz[xi + 1];
+            }
+            rb[k] = bxl | (byl << 10) | (bzl << 20);
+        }
+        __syncthreads();
+        // pass 2: P*P threads per site, each one (x, y) row of P z points
+        const int nk = min(tot - c0, chunk);
+        for (int it = threadIdx.x; it < nk * P * P; it += blockDim.x) {
+            const int k = it / (P * P), xy = it - k * (P * P), xi = xy / P, yi = xy - xi * P;
+            const int base = rb[k];
+            if (base < 0) continue;
+            const int bxl = base & 1023, byl = (base >> 10) & 1023, bzl = base >> 20;
+            const float *w = rw + k * 3 * P;
+            const float qxy = w[xi] * w[P + yi];
+            const int rowb = ((bxl + xi) * ty + (byl + yi)) * tz + bzl;
+#pragma unroll
+            for (int zi = 0; zi < P; zi++) {
+                long long v = __double2ll_rn((double)(qxy * w[2 * P + zi]) * TTC_QFIX_SCALE);
+                atomicAdd(&tile[rowb + zi], (unsigned long long)v);
             }
         }
-    __syncthreads();
+        __syncthreads();
+    }
+    // flush: one global atomic per touched tile point; t -> (lx, ly, lz) by
+    // multiply-high with ceil(2^32 / d) (exact: t * d < 2^32)
+    const unsigned mz = 0xFFFFFFFFu / (unsigned)tz + 1u, my = 0xFFFFFFFFu / (unsigned)ty + 1u;
     for (int t = threadIdx.x; t < tsz; t += blockDim.x) {
         unsigned long long v = tile[t];
         if (!v) continue;
-        int lz = t % tz, ly = (t / tz) % ty, lx = t / (tz * ty);
+        int r = (int)__umulhi((unsigned)t, mz), lz = t - r * tz;
+        int lx = (int)__umulhi((unsigned)r, my), ly = r - lx * ty;
         int gx = (ox + lx) % sp.K0, gy = (oy + ly) % sp.K1, gz = (oz + lz) % sp.K2;
         if (gx < 0) gx += sp.K0;
         if (gy < 0) gy += sp.K1;
@@ -876,6 +926,9 @@ int ttc_pme_setup(ttc_ctx *c) {
     c->K[0] = c->p.gx; c->K[1] = c->p.gy; c->K[2] = c->p.gz;
     c->ngrid = (size_t)c->K[0] * c->K[1] * c->K[2];
     for (int a = 0; a < 3; a++) {
+        // one FFT workgroup holds two buffers of FFT_LB lines of K complex
+        // values and K twiddles in the default 64 KiB of LDS
+        if (c->K[a] > 248) { ttc_set_error("PME grid dimension above 248 is not supported"); return TTC_ERR_ARG; }
         int r = make_plan(c->plan[a], c->K[a]);
         if (r) { ttc_set_error("PME grid size cannot be planned"); return r; }
     }
@@ -918,7 +971,33 @@ static Brick brick_args(const ttc_ctx *c) {
     br.cwx = c->p.lx / c->ncx; br.cwy = c->p.ly / c->ncy; br.cwz = c->p.lz / c->ncz;
     // the trigger fires at (rclist-rc)/4; one more step moves a site far less
     br.delta = 0.25 * (c->p.rclist - c->p.rc) + 0.25;
+    br.chunk = 0;
+    br.rec_off = 0;
     return br;
+}
+
+static size_t brick_shm(const ttc_ctx *c, const Brick &br);
+
+// workgroups of the spread that fit one CU's LDS (static part: mean force
+// and a margin); at most 8 of 256 threads
+static int spread_occupancy(size_t bytes) { return (int)std::min<size_t>(8, (160 * 1024) / (bytes + 64)); }
+
+// the brick with its LDS layout: tile, then the weights of `chunk` sites,
+// chunk the largest of 256/128/64 that keeps the occupancy of chunk 64
+static Brick spread_brick(const ttc_ctx *c) {
+    Brick br = brick_args(c);
+    size_t tile = brick_shm(c, br);
+    br.rec_off = (int)(tile / 8);
+    br.chunk = 64;
+    const int occ64 = spread_occupancy(tile + 64 * SPREAD_REC_BYTES(c->order));
+    for (int ch : {256, 128})
+        if (spread_occupancy(tile + (size_t)ch * SPREAD_REC_BYTES(c->order)) >= occ64) { br.chunk = ch; break; }
+    return br;
+}
+
+// dynamic LDS of the spread: tile + weights
+static size_t spread_shm(const ttc_ctx *c, const Brick &br) {
+    return (size_t)br.rec_off * 8 + (size_t)br.chunk * SPREAD_REC_BYTES(c->order);
 }
 
 // exact largest tile over all bricks, by the kernel's own formula
@@ -940,7 +1019,7 @@ static size_t brick_shm(const ttc_ctx *c, const Brick &br) {
 }
 
 static DsPack ds_pack(ttc_ctx *c, bool drift, double vscal) {
-    return DsPack{c->pos4, c->vel4, c->f4, drift ? c->p.dt : 0.0, vscal, spread_args(c), brick_args(c),
+    return DsPack{c->pos4, c->vel4, c->f4, drift ? c->p.dt : 0.0, vscal, spread_args(c), spread_brick(c),
                   c->cell_start, c->cell_cnt, c->qfix, c->status, c->anchor, c->ev + EV_N,
                   KickIn{c->fpair, c->g4, c->partials, c->nblk_max, ttc_gather_blocks(c), c->n, vscal}};
 }
@@ -952,8 +1031,8 @@ static int ds_blocks(const ttc_ctx *c) {
 
 static void launch_drift_spread(ttc_ctx *c, bool drift, double vscal) {
     int blocks = ds_blocks(c);
-    size_t shm = brick_shm(c, brick_args(c));
     DsPack a = ds_pack(c, drift, vscal);
+    size_t shm = spread_shm(c, a.br);
 #define DS(P, D) k_drift_spread<P, D><<<blocks, 256, shm, c->stream>>>(a)
     if (c->order == 4) { if (drift) DS(4, true); else DS(4, false); }
     else { if (drift) DS(6, true); else DS(6, false); }
@@ -967,8 +1046,8 @@ void ttc_launch_drift(ttc_ctx *c, double vscal) { launch_drift_spread(c, true, v
 
 void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick) {
     int blocks = ds_blocks(c);
-    size_t shm = brick_shm(c, brick_args(c));
     DsPack a = ds_pack(c, true, vscal);
+    size_t shm = spread_shm(c, a.br);
     a.ki.g4 = c->g4o;
     a.ki.orig = c->orig;
     a.ki.snap = c->snap;
@@ -978,12 +1057,12 @@ void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick) {
 #undef DS
 }
 
-int ttc_spread_shm(const ttc_ctx *c) { return (int)brick_shm(c, brick_args(c)); }
+int ttc_spread_shm(const ttc_ctx *c) { return (int)spread_shm(c, spread_brick(c)); }
 
 // allow the spread tile and the FFT tiles above 64 KiB of LDS (gfx950 has 160 KiB per CU)
 static int set_lds_attrs(ttc_ctx *c) {
-    int shm = (int)brick_shm(c, brick_args(c));
-    if (shm > 160 * 1024) {
+    int shm = (int)spread_shm(c, spread_brick(c));
+    if (spread_occupancy((size_t)shm) < 1) {
         ttc_set_error("PME spread tile exceeds the LDS of one CU");
         return TTC_ERR_ARG;
     }
@@ -1116,9 +1195,9 @@ int ttc_bstage_drift(const std::vector<ttc_ctx *> &r, TtcStage *st) {
         }
         pk.push_back(ds_pack(c, true, 1.0));
         nb.push_back(ds_blocks(c));
-        shm = std::max(shm, brick_shm(c, brick_args(c)));
+        shm = std::max(shm, spread_shm(c, pk.back().br));
     }
-    if (shm > 160 * 1024) {
+    if (spread_occupancy(shm) < 1) {
         ttc_set_error("ttc_batch: PME spread tile exceeds the LDS of one CU");
         return TTC_ERR_ARG;
     }
