@@ -28,6 +28,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ns/day @100k atoms (FP32-mixed vs FP64 plain, 1 GPU) + accepted MC moves/s"
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs, a wave64 VALU instruction every 4
+# cycles per 16-lane SIMD, at the 2.4 GHz peak engine clock
+VALU_PEAK_WINST_PER_S = 256 * 4 / 4 * 2.4e9
 
 
 def cpu_baseline(s, f_gpu0, rebuild_interval):
@@ -316,6 +319,29 @@ def _pmc_per_kernel(path, counter):
     return {k: (sum(v.values()) / len(v), len(v)) for k, v in acc.items()}
 
 
+def valu_summary(valu_csv, out_path):
+    """VALU instructions per pair-kernel launch (tuned variant) from a
+    counter run over SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F32, SQ_WAVES and
+    GRBM_GUI_ACTIVE of --traffic-probe.  SQ_ counters count wave-level
+    instructions over the whole device (SQ_WAVES reads the launch's wave
+    count); GRBM_GUI_ACTIVE sums the busy cycles of the 8 XCDs."""
+    c = {k: _pmc_per_kernel(valu_csv, k) for k in
+         ("SQ_INSTS_VALU", "SQ_INSTS_VALU_TRANS_F32", "SQ_WAVES", "GRBM_GUI_ACTIVE")}
+    # the tuned variant: the pair kernel launched most often
+    name = max((v[1], k) for k, v in c["SQ_INSTS_VALU"].items() if "k_pair<false, false, false" in k)[1]
+    insts, trans = c["SQ_INSTS_VALU"][name][0], c["SQ_INSTS_VALU_TRANS_F32"][name][0]
+    cycles = c["GRBM_GUI_ACTIVE"][name][0] / 8.0
+    cap = 256 * cycles   # wave-instructions the CUs could issue in those cycles
+    res = {"kernel": "k_pair (plain, tuned waves per group)", "workload": "chains-100k after 50 MD steps",
+           "valu_wave_insts_per_launch": insts, "trans_wave_insts_per_launch": trans,
+           "waves_per_launch": c["SQ_WAVES"][name][0], "cycles_per_xcd": cycles,
+           "issue_frac_of_own_cycles": insts / cap,
+           "issue_frac_trans_at_quarter_rate": (insts + 3.0 * trans) / cap}
+    with open(out_path, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(json.dumps(res))
+
+
 def traffic_summary(fetch_csv, write_csv, out_path):
     """HBM bytes per pair-kernel launch from the two counter runs, each
     scaled by the copy kernel's reading against its known COPY_MIB bytes."""
@@ -346,6 +372,8 @@ def main():
                     help="run only the workload of the FETCH_SIZE / WRITE_SIZE counter runs")
     ap.add_argument("--traffic-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
                     help="turn the two counter runs into bytes per pair-kernel launch")
+    ap.add_argument("--valu-summary", nargs=2, metavar=("VALU_CSV", "OUT_JSON"),
+                    help="turn the VALU counter run into instructions per pair-kernel launch")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
@@ -360,6 +388,9 @@ def main():
     args = ap.parse_args()
     if args.traffic_summary:
         traffic_summary(*args.traffic_summary)
+        return
+    if args.valu_summary:
+        valu_summary(*args.valu_summary)
         return
     if args.traffic_probe:
         traffic_probe()
@@ -404,6 +435,16 @@ def main():
         with open(tfile) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
+    valu = None
+    vfile = os.path.join(ROOT, "profiles", "pair_kernel_chains100k_valu.json")
+    if os.path.exists(vfile):
+        with open(vfile) as fh:
+            vi = json.load(fh)["valu_wave_insts_per_launch"]
+        # the counted instructions over this run's HIP-event kernel time
+        rate = vi / (t_pair * 1e-6)
+        valu = {"bound": "valu", "achieved": round(rate / 1e9, 1), "peak": VALU_PEAK_WINST_PER_S / 1e9,
+                "unit": "G wave-instructions/s", "frac": round(rate / VALU_PEAK_WINST_PER_S, 4),
+                "wave_insts_per_launch": vi}
     out = {
         "metric": METRIC, "value": round(ns_day, 3), "unit": "ns/day", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 5),
@@ -420,7 +461,8 @@ def main():
                      "kernel": f"k_pair (LJ + real-space Ewald), {waves} waves per 64-site group",
                      "kernel_us": round(t_pair, 2),
                      "algorithmic_bytes": alg_bytes, "half_list_entries": half_entries,
-                     "spread_us": round(t_spread, 2), "step_us_events": round(t_event, 2)},
+                     "spread_us": round(t_spread, 2), "step_us_events": round(t_event, 2),
+                     "valu": valu},
     }
     if args.drift_steps > 0:
         print("bench: NVE energy drift", file=sys.stderr, flush=True)
