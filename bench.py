@@ -427,6 +427,8 @@ def main():
     # dominant kernel: the pair kernel; HIP events on the context stream
     t_pair = box.timing(0)
     t_spread = box.timing(1)
+    t_fft = box.timing(3)              # the five FFT passes alone
+    t_gather = box.timing(4)           # gather + bonds alone
     alg_bytes = 4.0 * half_entries + 48.0 * s.n
     achieved = alg_bytes / (t_pair * 1e-6) / 1e9
     traffic = None
@@ -461,7 +463,8 @@ def main():
                      "kernel": f"k_pair (LJ + real-space Ewald), {waves} waves per 64-site group",
                      "kernel_us": round(t_pair, 2),
                      "algorithmic_bytes": alg_bytes, "half_list_entries": half_entries,
-                     "spread_us": round(t_spread, 2), "step_us_events": round(t_event, 2),
+                     "spread_us": round(t_spread, 2), "fft_us": round(t_fft, 2),
+                     "gather_us": round(t_gather, 2), "step_us_events": round(t_event, 2),
                      "valu": valu},
     }
     if args.drift_steps > 0:

@@ -762,7 +762,7 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
 extern "C" int ttc_timing(ttc_ctx *c, int which, double *usec) {
     CHECK(c && usec, TTC_ERR_ARG, "ttc_timing: bad argument");
     if (which == 2) { *usec = c->t_step; return TTC_OK; }
-    CHECK(which == 0 || which == 1, TTC_ERR_ARG, "ttc_timing: which must be 0, 1 or 2");
+    CHECK(which >= 0 && which <= 4, TTC_ERR_ARG, "ttc_timing: which must be 0 .. 4");
     int r = need_list(c);
     if (r) return r;
     const int reps = 20;
@@ -770,11 +770,15 @@ extern "C" int ttc_timing(ttc_ctx *c, int which, double *usec) {
     TTC_HIP(hipEventCreate(&e0));
     TTC_HIP(hipEventCreate(&e1));
     // warm-up launch, then timed back-to-back launches on the context stream
-    if (which == 0) ttc_launch_pair(c, false, 0); else ttc_launch_spread(c);
+    auto launch = [&] {
+        if (which == 0) ttc_launch_pair(c, false, 0);
+        else if (which == 1) ttc_launch_spread(c);
+        else if (which == 3) ttc_launch_fft(c, false);   // the five line passes
+        else ttc_launch_gather_orig(c, false);
+    };
+    launch();
     TTC_HIP(hipEventRecord(e0, c->stream));
-    for (int i = 0; i < reps; i++) {
-        if (which == 0) ttc_launch_pair(c, false, 0); else ttc_launch_spread(c);
-    }
+    for (int i = 0; i < reps; i++) launch();
     TTC_HIP(hipEventRecord(e1, c->stream));
     TTC_HIP(hipEventSynchronize(e1));
     float ms = 0.0f;
@@ -786,8 +790,12 @@ extern "C" int ttc_timing(ttc_ctx *c, int which, double *usec) {
         TTC_HIP(hipMemsetAsync(c->qfix, 0, sizeof(long long) * c->ngrid, c->stream));
         TTC_HIP(hipStreamSynchronize(c->stream));
     }
+    // the FFT passes leave the grid cleared and the gather overwrote the PME
+    // forces with those of that grid: forces are recomputed before the next
+    // MD call
+    if (which >= 3) c->f4_valid = false;
     *usec = 1000.0 * ms / reps;
-    if (which == 0) c->t_pair = *usec; else c->t_spread = *usec;
+    if (which == 0) c->t_pair = *usec; else if (which == 1) c->t_spread = *usec;
     return TTC_OK;
 }
 

@@ -403,44 +403,58 @@ struct FftArgs {
     int n;          // line length
     int nrad;
     int rad[16];
+    int ns[16];             // stage st: product of the radices before it
+    int nr[16];             // n / rad[st]
+    unsigned mag[16];       // ceil(2^32 / ns[st]) mod 2^32: j / ns[st] by multiply-high
+    unsigned mag_n;         // ceil(2^32 / n)
     const float2 *tw;
 };
+
+// q = j / d for j >= 0, d >= 1 with j * d < 2^32, where m = ceil(2^32 / d)
+// truncated to 32 bits (so m = 0 stands for d = 1): umulhi(j, m) =
+// floor(j/d + j e / (d 2^32)) with e = m d - 2^32 < d, and j e < 2^32 keeps
+// the second term below 1/d, so the floor is unchanged
+__device__ __forceinline__ int udiv_mag(int j, unsigned m) { return m ? (int)__umulhi((unsigned)j, m) : j; }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ float2 twid(const float2 *tw, int m, int n, int sign) {
-    float2 t = tw[m % n];
+// e^{sign 2 pi i m / n}, 0 <= m < n
+__device__ __forceinline__ float2 twid(const float2 *tw, int m, int sign) {
+    float2 t = tw[m];
     return sign > 0 ? t : make_float2(t.x, -t.y);
 }
 
-// Stockham passes over `nlines` lines held in LDS; element e of line l sits
-// at buf[l*ls + e*es].  Returns the buffer that holds the result.
-__device__ float2 *fft_lds(float2 *A, float2 *B, int nlines, int ls, int es, const FftArgs &f,
-                           const float2 *tw, int sign) {
-    int n = f.n;
-    int Ns = 1;
+// Stockham passes over FFT_LB lines held in LDS; element e of line l sits at
+// buf[l*ls + e*es].  Work item t takes line t % FFT_LB, so consecutive lanes
+// touch consecutive lines: with es = FFT_LB (ls = 1) they read consecutive
+// words, and with es = 1 the line stride ls = n + 1 spreads them over the
+// banks.  Stage constants come from the host (no integer division here).
+// Returns the buffer that holds the result.
+__device__ float2 *fft_lds(float2 *A, float2 *B, int ls, int es, const FftArgs &f, const float2 *tw, int sign) {
+    const int n = f.n;
     float2 *src = A, *dst = B;
     for (int st = 0; st < f.nrad; st++) {
-        int R = f.rad[st];
-        int nR = n / R;
-        int tstride = n / (Ns * R);
-        for (int t = threadIdx.x; t < nlines * nR; t += blockDim.x) {
-            int line = t / nR, j = t - line * nR;
-            int k = j % Ns;
-            int tstep = k * tstride;
+        const int R = f.rad[st], Ns = f.ns[st], nR = f.nr[st];
+        const unsigned mNs = f.mag[st];
+        const int tstride = nR / Ns;   // n / (Ns R)
+        for (int t = threadIdx.x; t < FFT_LB * nR; t += blockDim.x) {
+            const int line = t & (FFT_LB - 1), j = t / FFT_LB;
+            const int q = udiv_mag(j, mNs), k = j - q * Ns;
+            // radix twiddle indices r * k * tstride stay below n for r < R
+            const int tstep = k * tstride;
             const float2 *s0 = src + line * ls;
             float2 *d0 = dst + line * ls;
-            int idxD = (j / Ns) * Ns * R + k;
+            const int idxD = q * Ns * R + k;
             if (R == 2) {
-                float2 a = s0[j * es], b = cmul(s0[(j + nR) * es], twid(tw, tstep, n, sign));
+                float2 a = s0[j * es], b = cmul(s0[(j + nR) * es], twid(tw, tstep, sign));
                 d0[idxD * es] = make_float2(a.x + b.x, a.y + b.y);
                 d0[(idxD + Ns) * es] = make_float2(a.x - b.x, a.y - b.y);
             } else if (R == 4) {
                 float2 a = s0[j * es];
-                float2 b = cmul(s0[(j + nR) * es], twid(tw, tstep, n, sign));
-                float2 c = cmul(s0[(j + 2 * nR) * es], twid(tw, 2 * tstep, n, sign));
-                float2 d = cmul(s0[(j + 3 * nR) * es], twid(tw, 3 * tstep, n, sign));
+                float2 b = cmul(s0[(j + nR) * es], twid(tw, tstep, sign));
+                float2 c = cmul(s0[(j + 2 * nR) * es], twid(tw, 2 * tstep, sign));
+                float2 d = cmul(s0[(j + 3 * nR) * es], twid(tw, 3 * tstep, sign));
                 float2 apc = make_float2(a.x + c.x, a.y + c.y), amc = make_float2(a.x - c.x, a.y - c.y);
                 float2 bpd = make_float2(b.x + d.x, b.y + d.y), bmd = make_float2(b.x - d.x, b.y - d.y);
                 // (sign i) * bmd
@@ -451,8 +465,8 @@ __device__ float2 *fft_lds(float2 *A, float2 *B, int nlines, int ls, int es, con
                 d0[(idxD + 3 * Ns) * es] = make_float2(amc.x - ib.x, amc.y - ib.y);
             } else if (R == 3) {
                 float2 a = s0[j * es];
-                float2 b = cmul(s0[(j + nR) * es], twid(tw, tstep, n, sign));
-                float2 c = cmul(s0[(j + 2 * nR) * es], twid(tw, 2 * tstep, n, sign));
+                float2 b = cmul(s0[(j + nR) * es], twid(tw, tstep, sign));
+                float2 c = cmul(s0[(j + 2 * nR) * es], twid(tw, 2 * tstep, sign));
                 const float h = 0.86602540378443864676f * (float)sign;
                 float2 bpc = make_float2(b.x + c.x, b.y + c.y), bmc = make_float2(b.x - c.x, b.y - c.y);
                 float2 m = make_float2(a.x - 0.5f * bpc.x, a.y - 0.5f * bpc.y);
@@ -461,15 +475,15 @@ __device__ float2 *fft_lds(float2 *A, float2 *B, int nlines, int ls, int es, con
                 d0[(idxD + 2 * Ns) * es] = make_float2(m.x + h * bmc.y, m.y - h * bmc.x);
             } else {
                 // direct DFT of length R: X_q = sum_r src_r w^{r (tstep + q n/R)}
-                for (int q = 0; q < R; q++) {
+                for (int qq = 0; qq < R; qq++) {
                     float2 acc = make_float2(0.0f, 0.0f);
-                    int base = tstep + q * nR;
+                    int base = tstep + qq * nR;
                     for (int r = 0; r < R; r++) {
-                        float2 v = cmul(s0[(j + r * nR) * es], twid(tw, (r * base) % n, n, sign));
+                        float2 v = cmul(s0[(j + r * nR) * es], twid(tw, (r * base) % n, sign));
                         acc.x += v.x;
                         acc.y += v.y;
                     }
-                    d0[(idxD + q * Ns) * es] = acc;
+                    d0[(idxD + qq * Ns) * es] = acc;
                 }
             }
         }
@@ -477,7 +491,6 @@ __device__ float2 *fft_lds(float2 *A, float2 *B, int nlines, int ls, int es, con
         float2 *t = src;
         src = dst;
         dst = t;
-        Ns *= R;
     }
     return src;
 }
@@ -492,8 +505,10 @@ __device__ __forceinline__ void fft_body(int bid, const FftArgs &f, int sign, lo
                                          double *partials, int maxb) {
     extern __shared__ float2 lds[];
     int n = f.n;
-    float2 *A = lds, *B = lds + FFT_LB * n;
-    float2 *tw = B + FFT_LB * n;
+    // buffers of FFT_LB lines; z lines (MODE 0, 3) are padded to n + 1
+    const int lsz = (MODE == 0 || MODE == 3) ? n + 1 : n;
+    float2 *A = lds, *B = lds + FFT_LB * lsz;
+    float2 *tw = B + FFT_LB * lsz;
     for (int m = threadIdx.x; m < n; m += blockDim.x) tw[m] = f.tw[m];
     int nlines, ls, es;
     size_t gbase = 0;
@@ -502,18 +517,22 @@ __device__ __forceinline__ void fft_body(int bid, const FftArgs &f, int sign, lo
         nlines = f.K0 * f.K1;
         int l0 = bid * FFT_LB;
         nl = min(FFT_LB, nlines - l0);
-        ls = n; es = 1;
+        ls = n + 1; es = 1;
         gbase = (size_t)l0 * n;
-        for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
-            float2 v;
-            if (MODE == 0) {
-                long long q = qfix[gbase + idx];
-                qfix[gbase + idx] = 0;
-                v = make_float2((float)((double)q * TTC_QFIX_INV), 0.0f);
-            } else {
-                v = qc[gbase + idx];
+        // lines past the grid's last line are zero-filled and not stored
+        for (int idx = threadIdx.x; idx < FFT_LB * n; idx += blockDim.x) {
+            const int l = udiv_mag(idx, f.mag_n), e = idx - l * n;
+            float2 v = make_float2(0.0f, 0.0f);
+            if (l < nl) {
+                if (MODE == 0) {
+                    long long q = qfix[gbase + idx];
+                    qfix[gbase + idx] = 0;
+                    v = make_float2((float)((double)q * TTC_QFIX_INV), 0.0f);
+                } else {
+                    v = qc[gbase + idx];
+                }
             }
-            A[idx] = v;
+            A[l * ls + e] = v;
         }
     } else {
         // lines along y (MODE 1: fixed x) or x (MODE 2: fixed y), LB z values
@@ -536,7 +555,7 @@ __device__ __forceinline__ void fft_body(int bid, const FftArgs &f, int sign, lo
         nl = FFT_LB;
     }
     __syncthreads();
-    float2 *res = fft_lds(A, B, nl, ls, es, f, tw, MODE == 3 ? -1 : (MODE == 1 ? sign : +1));
+    float2 *res = fft_lds(A, B, ls, es, f, tw, MODE == 3 ? -1 : (MODE == 1 ? sign : +1));
 
     if (MODE == 2) {
         // influence function (pme_calc.c:52-93); element e = k1, line l -> (k2=y0, k3=z0+l)
@@ -585,14 +604,15 @@ __device__ __forceinline__ void fft_body(int bid, const FftArgs &f, int sign, lo
         }
         __syncthreads();
         float2 *other = res == A ? B : A;
-        res = fft_lds(res, other, FFT_LB, ls, es, f, tw, -1);
+        res = fft_lds(res, other, ls, es, f, tw, -1);
     }
 
     // store
     if (MODE == 0 || MODE == 3) {
         for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
-            if (MODE == 0) qc[gbase + idx] = res[idx];
-            else qr[gbase + idx] = res[idx].x;
+            const int l = udiv_mag(idx, f.mag_n), e = idx - l * n;
+            if (MODE == 0) qc[gbase + idx] = res[l * ls + e];
+            else qr[gbase + idx] = res[l * ls + e].x;
         }
     } else {
         int nzl = min(FFT_LB, f.K2 - z0);
@@ -626,7 +646,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft(FftPack a) {
 template <int MODE, bool ENERGY>
 __global__ void __launch_bounds__(FFT_THREADS) kb_fft(const FftPack *pk, const TtcBmap *map) {
     const TtcBmap m = map[blockIdx.x];
-    const FftPack a = pk[m.rep];
+    // the replica's arguments are read in place (a local copy, indexed by
+    // stage at run time, would live in scratch memory)
+    const FftPack &a = pk[m.rep];
     fft_body<MODE, ENERGY>(m.blk, a.f, a.sign, a.qfix, a.qc, a.qr, a.eterm, a.fac, a.lx, a.ly, a.lz, a.partials,
                            a.maxb);
 }
@@ -840,7 +862,15 @@ FftArgs fft_args(const ttc_ctx *c, int axis) {
     const FftPlan &pl = c->plan[axis];
     f.n = pl.n;
     f.nrad = pl.nrad;
-    for (int i = 0; i < 16; i++) f.rad[i] = i < pl.nrad ? pl.rad[i] : 1;
+    int ns = 1;
+    for (int i = 0; i < 16; i++) {
+        f.rad[i] = i < pl.nrad ? pl.rad[i] : 1;
+        f.ns[i] = ns;
+        f.nr[i] = pl.n / f.rad[i];
+        f.mag[i] = (unsigned)((0x100000000ull + ns - 1) / ns);   // 0 for ns = 1
+        ns *= f.rad[i];
+    }
+    f.mag_n = (unsigned)((0x100000000ull + pl.n - 1) / pl.n);
     f.tw = pl.tw;
     return f;
 }
@@ -926,9 +956,9 @@ int ttc_pme_setup(ttc_ctx *c) {
     c->K[0] = c->p.gx; c->K[1] = c->p.gy; c->K[2] = c->p.gz;
     c->ngrid = (size_t)c->K[0] * c->K[1] * c->K[2];
     for (int a = 0; a < 3; a++) {
-        // one FFT workgroup holds two buffers of FFT_LB lines of K complex
-        // values and K twiddles in the default 64 KiB of LDS
-        if (c->K[a] > 248) { ttc_set_error("PME grid dimension above 248 is not supported"); return TTC_ERR_ARG; }
+        // one FFT workgroup holds two buffers of FFT_LB lines of K + 1
+        // complex values and K twiddles in the default 64 KiB of LDS
+        if (c->K[a] > 247) { ttc_set_error("PME grid dimension above 247 is not supported"); return TTC_ERR_ARG; }
         int r = make_plan(c->plan[a], c->K[a]);
         if (r) { ttc_set_error("PME grid size cannot be planned"); return r; }
     }
@@ -1074,7 +1104,8 @@ static int set_lds_attrs(ttc_ctx *c) {
 }
 
 static size_t fft_shm(const ttc_ctx *c, int axis) {
-    return sizeof(float2) * (2 * FFT_LB * c->plan[axis].n + c->plan[axis].n);
+    // two buffers of FFT_LB lines (padded to n + 1) and the twiddles
+    return sizeof(float2) * (2 * FFT_LB * (c->plan[axis].n + 1) + c->plan[axis].n);
 }
 
 // the five line passes of one step: z forward, y forward, x forward +
