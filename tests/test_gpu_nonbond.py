@@ -111,6 +111,30 @@ def test_pme_parity(box3k, oracle3k, sys3k):
     assert np.allclose(vir, virr, rtol=1e-5, atol=2e-6 * np.abs(virr).max())
 
 
+@pytest.mark.parametrize("grid", [25, 27, 30, 34, 35])
+def test_pme_parity_grid_sizes(grid):
+    """PME forces, energy and virial at grid sizes whose line FFTs use
+    odd lengths, radix-3/5/7 stages and a direct length-17 DFT (K = 25 = 5^2,
+    27 = 3^3, 30 = 2 3 5, 34 = 2 17, 35 = 5 7), same tolerances as above."""
+    import pyoracle
+    from thecodecombo_7_amd import system
+    from thecodecombo_7_amd.nonbond import Box
+    print(f"case: PME grid {grid}^3, 600 chains in a 39 A box", flush=True)
+    s = system.make_chain_system(600, 39.0, grid, seed=23)
+    b = Box(s)
+    try:
+        b.nblist()
+        ff, en, vir = b.pme_calc()
+    finally:
+        b.close()
+    ffr, enr, virr, _ = pyoracle.Oracle(s).pme(s.pos)
+    err = rel_rms(ff, ffr)
+    print("grid", grid, "pme force rel rms", err, "energy", en[0], enr[0])
+    assert err < 5e-6
+    assert abs(en[0] - enr[0]) <= 2e-6 * abs(enr[0])
+    assert np.allclose(vir, virr, rtol=1e-5, atol=2e-6 * np.abs(virr).max())
+
+
 def test_cnonbondnl_and_forces_parity(box3k, oracle3k, sys3k):
     o, (cnt, lst, xf), _ = oracle3k
     ff, en, vn, vk = box3k.cnonbondnl()
