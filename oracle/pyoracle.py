@@ -290,4 +290,8 @@ def ref_lib():
     L.ref_erfc.restype = C.c_double; L.ref_erfc.argtypes = [C.c_double]
     L.ref_self_energy.restype = C.c_double
     L.ref_ran2_seq.argtypes = [C.c_long, C.c_int, _dp]
+    L.ref_vverlet.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]
+    L.ref_pbc_all.argtypes = [_dp]
+    L.ref_kinet.restype = C.c_double
+    L.ref_kinet.argtypes = [_dp, _dp, _dp, _dp]
     return L

@@ -226,3 +226,86 @@ void ref_ran2_seq(long seed, int n, double *out)
     idum = seed;
     for (int i = 0; i < n; i++) out[i] = ran2();
 }
+
+/* ---- integrate_vverlet.c, pbc_all.c, kinet.c --------------------------
+ * integrate_vverlet() calls forces(k); the harness defines it as a fixed
+ * linear force field of the wrapped positions that couples the three axes,
+ *   F_x = -fc x + fd y,  F_y = -fc y + fd z,  F_z = -fc z + fd x,
+ * so a test can evaluate the same forces in the oracle's loop bit for bit. */
+static double g_fc = 0.0, g_fd = 0.0;
+
+void forces(int k)
+{
+    for (int i = 0; i < box[k].boxns; i++) {
+        double x = atom[k][i].x, y = atom[k][i].y, z = atom[k][i].z;
+        ff[k][i].x = -g_fc * x + g_fd * y;
+        ff[k][i].y = -g_fc * y + g_fd * z;
+        ff[k][i].z = -g_fc * z + g_fd * x;
+    }
+}
+
+void integrate_vverlet(int, double);
+void pbc_all(int);
+void kinet(int);
+
+static void md_alloc(void)
+{
+    if (vv) return;
+    vv = (struct veloc **)xcalloc(1, sizeof(struct veloc *));
+    uu = (struct veloc **)xcalloc(1, sizeof(struct veloc *));
+    vv[0] = (struct veloc *)xcalloc(g_n, sizeof(struct veloc));
+    uu[0] = (struct veloc *)xcalloc(g_n, sizeof(struct veloc));
+}
+
+/* nsteps calls of integrate_vverlet(0, vscal) after one forces(0) (as
+ * main.c does before the first step), on positions pos (wrapped), nopbc
+ * (unwrapped) and velocities vel; all three are updated in place and uu_out
+ * receives uu.  Needs ref_setup first (box, n). */
+void ref_vverlet(int nsteps, double vscal, double dt, double fc, double fd, const double *mass,
+                 double *pos, double *nopbc, double *vel, double *uu_out)
+{
+    md_alloc();
+    sim.dt = dt;
+    g_fc = fc;
+    g_fd = fd;
+    for (int i = 0; i < g_n; i++) {
+        pott[0][i].mas = mass[i];
+        atom[0][i].x = pos[3 * i]; atom[0][i].y = pos[3 * i + 1]; atom[0][i].z = pos[3 * i + 2];
+        atnopbc[0][i].x = nopbc[3 * i]; atnopbc[0][i].y = nopbc[3 * i + 1]; atnopbc[0][i].z = nopbc[3 * i + 2];
+        vv[0][i].x = vel[3 * i]; vv[0][i].y = vel[3 * i + 1]; vv[0][i].z = vel[3 * i + 2];
+    }
+    forces(0);
+    for (int s = 0; s < nsteps; s++) integrate_vverlet(0, vscal);
+    for (int i = 0; i < g_n; i++) {
+        pos[3 * i] = atom[0][i].x; pos[3 * i + 1] = atom[0][i].y; pos[3 * i + 2] = atom[0][i].z;
+        nopbc[3 * i] = atnopbc[0][i].x; nopbc[3 * i + 1] = atnopbc[0][i].y; nopbc[3 * i + 2] = atnopbc[0][i].z;
+        vel[3 * i] = vv[0][i].x; vel[3 * i + 1] = vv[0][i].y; vel[3 * i + 2] = vv[0][i].z;
+        uu_out[3 * i] = uu[0][i].x; uu_out[3 * i + 1] = uu[0][i].y; uu_out[3 * i + 2] = uu[0][i].z;
+    }
+}
+
+/* pbc_all.c:23-88 on pos (in place) */
+void ref_pbc_all(double *pos)
+{
+    for (int i = 0; i < g_n; i++) {
+        atom[0][i].x = pos[3 * i]; atom[0][i].y = pos[3 * i + 1]; atom[0][i].z = pos[3 * i + 2];
+    }
+    pbc_all(0);
+    for (int i = 0; i < g_n; i++) {
+        pos[3 * i] = atom[0][i].x; pos[3 * i + 1] = atom[0][i].y; pos[3 * i + 2] = atom[0][i].z;
+    }
+}
+
+/* kinet.c:18-68: returns en.kinet, ken6 = en.ken */
+double ref_kinet(const double *mass, const double *vel, const double *uuv, double *ken6)
+{
+    md_alloc();
+    for (int i = 0; i < g_n; i++) {
+        pott[0][i].mas = mass[i];
+        vv[0][i].x = vel[3 * i]; vv[0][i].y = vel[3 * i + 1]; vv[0][i].z = vel[3 * i + 2];
+        uu[0][i].x = uuv[3 * i]; uu[0][i].y = uuv[3 * i + 1]; uu[0][i].z = uuv[3 * i + 2];
+    }
+    kinet(0);
+    for (int i = 0; i < 6; i++) ken6[i] = en[0].ken[i];
+    return en[0].kinet;
+}
