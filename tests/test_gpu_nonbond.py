@@ -111,16 +111,23 @@ def test_pme_parity(box3k, oracle3k, sys3k):
     assert np.allclose(vir, virr, rtol=1e-5, atol=2e-6 * np.abs(virr).max())
 
 
-@pytest.mark.parametrize("grid", [25, 27, 30, 34, 35])
-def test_pme_parity_grid_sizes(grid):
+@pytest.mark.parametrize("grid,nchains,box", [(25, 600, 39.0), (27, 600, 39.0), (30, 600, 39.0), (34, 600, 39.0),
+                                              (35, 600, 39.0), (96, 1500, 96.0), (224, 3000, 215.0), (246, 3000, 240.0),
+                                              (96, 600, 39.0)])
+def test_pme_parity_grid_sizes(grid, nchains, box):
     """PME forces, energy and virial at grid sizes whose line FFTs use
     odd lengths, radix-3/5/7 stages and a direct length-17 DFT (K = 25 = 5^2,
-    27 = 3^3, 30 = 2 3 5, 34 = 2 17, 35 = 5 7), same tolerances as above."""
+    27 = 3^3, 30 = 2 3 5, 34 = 2 17, 35 = 5 7), at the production grids
+    K = 96 = 2^5 3 and K = 224 = 2^5 7 (about 1 A spacing, fewer sites than
+    the production boxes), at the largest supported K = 246 = 2 3 41
+    (a direct length-41 DFT), and at K = 96 in a 39 A box (0.4 A spacing),
+    where the spread's 2x2x2-cell brick would not fit one CU's LDS and a
+    smaller brick is used; same tolerances as above."""
     import pyoracle
     from thecodecombo_7_amd import system
     from thecodecombo_7_amd.nonbond import Box
-    print(f"case: PME grid {grid}^3, 600 chains in a 39 A box", flush=True)
-    s = system.make_chain_system(600, 39.0, grid, seed=23)
+    print(f"case: PME grid {grid}^3, {nchains} chains in a {box} A box", flush=True)
+    s = system.make_chain_system(nchains, box, grid, seed=23)
     b = Box(s)
     try:
         b.nblist()
@@ -253,3 +260,88 @@ def test_md_loop_list_complete(nchains, box, grid):
     print("MD-loop list: rebuilds", nreb, "force rel rms", err)
     assert err < 5e-6
     assert abs(e9["nbond"] - eref["nbond"]) <= 2e-6 * abs(eref["nbond"])
+
+
+def test_nl_check_matches_oracle(sys3k):
+    """ttc_nl_check (nblist.c:607-647 on the device) gives the oracle's
+    decision, which is bit-exact against the reference's own nl_check:
+    random displacements at amplitudes around the trigger distance
+    (rclist - rc)/4 = 0.5 A, over whole, single-site and random [lb,ub)
+    ranges, with sites moved across the periodic boundary by the wrap."""
+    import ctypes as C
+
+    import pyoracle
+    from thecodecombo_7_amd import system
+    from thecodecombo_7_amd.nonbond import Box
+    s = sys3k
+    o = pyoracle.Oracle(s)
+    print("case: chains-3k nl_check vs oracle", flush=True)
+    b = Box(s)
+    try:
+        b.nblist()
+        anchor = np.ascontiguousarray(b.positions())   # anchors = positions of the build
+        rng = np.random.default_rng(44)
+        seen = {0: 0, 1: 0}
+        for amp in (0.05, 0.2, 0.27, 0.29, 0.3, 0.32, 0.4, 0.55):
+            for _ in range(3):
+                d = rng.uniform(-amp, amp, (s.n, 3))
+                b.set_positions(system.wrap(anchor + d, s.box))
+                cur = np.ascontiguousarray(b.positions())
+                ranges = [(0, s.n), (0, 1), (s.n - 1, s.n)]
+                for _ in range(12):
+                    lb = int(rng.integers(0, s.n))
+                    ranges.append((lb, int(rng.integers(lb, min(s.n, lb + 200) + 1))))
+                for lb, ub in ranges:
+                    got = b.nl_check(lb, ub)
+                    ref = o.L.orc_nl_check(C.byref(o.cs), cur.ravel(), anchor.ravel(), lb, ub)
+                    assert got == ref, (amp, lb, ub, got, ref)
+                    seen[ref] += 1
+        print("nl_check decisions (no trigger, trigger):", seen[0], seen[1])
+        assert seen[0] > 20 and seen[1] > 20
+    finally:
+        b.close()
+
+
+def test_md_rebuild_steps_match_oracle(sys3k):
+    """The MD loop's displacement trigger fires at the steps where the
+    reference's nl_check after each step (nve_md.c:369-370) fires: the device
+    runs one step per call and reports whether it rebuilt; the oracle
+    integrates the same trajectory and applies orc_nl_check to the positions
+    each step ends with, re-anchoring where it fires."""
+    import ctypes as C
+
+    import pyoracle
+    from thecodecombo_7_amd.nonbond import Box
+    s = sys3k
+    nsteps = 40
+    print("case: chains-3k, 40 single-step calls, rebuild steps vs oracle", flush=True)
+    b = Box(s)
+    try:
+        b.nblist()
+        dev = [b.integrate_vverlet(1)[1] for _ in range(nsteps)]
+        pos_gpu = b.positions()
+    finally:
+        b.close()
+    o = pyoracle.Oracle(s)
+    pos = s.pos.copy().ravel(); vel = s.vel.copy().ravel(); uu = vel.copy()
+    rc, cnt, lst, xf, *_ = o.nblist(pos.reshape(-1, 3))
+    lists = (cnt, lst, xf)
+    anchor = pos.copy()
+    ff = o.forces(pos.reshape(-1, 3), lists=lists)[0].ravel()
+    ref = []
+    for _ in range(nsteps):
+        o.L.orc_vverlet_a(C.byref(o.cs), s.n, s.mass, s.dt, 1.0, pos, None, vel, ff)
+        fire = o.L.orc_nl_check(C.byref(o.cs), pos, anchor, 0, s.n)
+        if fire:
+            rc, cnt, lst, xf, *_ = o.nblist(pos.reshape(-1, 3))
+            lists = (cnt, lst, xf)
+            anchor = pos.copy()
+        ref.append(int(fire))
+        ff = o.forces(pos.reshape(-1, 3), lists=lists)[0].ravel()
+        o.L.orc_vverlet_b(s.n, s.mass, s.dt, 1.0, vel, uu, ff)
+    dpos = pos.reshape(-1, 3) - pos_gpu
+    dpos -= s.box * np.round(dpos / s.box)
+    print("rebuild steps device", [k + 1 for k, v in enumerate(dev) if v],
+          "oracle", [k + 1 for k, v in enumerate(ref) if v], "max |dx|", np.abs(dpos).max())
+    assert sum(ref) >= 3
+    assert dev == ref

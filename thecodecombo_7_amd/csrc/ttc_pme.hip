@@ -957,8 +957,9 @@ int ttc_pme_setup(ttc_ctx *c) {
     c->ngrid = (size_t)c->K[0] * c->K[1] * c->K[2];
     for (int a = 0; a < 3; a++) {
         // one FFT workgroup holds two buffers of FFT_LB lines of K + 1
-        // complex values and K twiddles in the default 64 KiB of LDS
-        if (c->K[a] > 247) { ttc_set_error("PME grid dimension above 247 is not supported"); return TTC_ERR_ARG; }
+        // complex values and K twiddles (8 (33 K + 32) bytes) and its static
+        // 224-byte reduction array in the default 64 KiB of LDS: K <= 246
+        if (c->K[a] > 246) { ttc_set_error("PME grid dimension above 246 is not supported"); return TTC_ERR_ARG; }
         int r = make_plan(c->plan[a], c->K[a]);
         if (r) { ttc_set_error("PME grid size cannot be planned"); return r; }
     }
@@ -993,24 +994,32 @@ static Spread spread_args(const ttc_ctx *c) {
     return sp;
 }
 
-static Brick brick_args(const ttc_ctx *c) {
-    Brick br;
-    br.ncx = c->ncx; br.ncy = c->ncy; br.ncz = c->ncz;
-    br.bx = c->ncx >= 2 ? 2 : 1; br.by = c->ncy >= 2 ? 2 : 1; br.bz = c->ncz >= 2 ? 2 : 1;
-    br.nbx = (c->ncx + br.bx - 1) / br.bx; br.nby = (c->ncy + br.by - 1) / br.by; br.nbz = (c->ncz + br.bz - 1) / br.bz;
-    br.cwx = c->p.lx / c->ncx; br.cwy = c->p.ly / c->ncy; br.cwz = c->p.lz / c->ncz;
-    // the trigger fires at (rclist-rc)/4; one more step moves a site far less
-    br.delta = 0.25 * (c->p.rclist - c->p.rc) + 0.25;
-    br.chunk = 0;
-    br.rec_off = 0;
-    return br;
-}
-
 static size_t brick_shm(const ttc_ctx *c, const Brick &br);
 
 // workgroups of the spread that fit one CU's LDS (static part: mean force
 // and a margin); at most 8 of 256 threads
 static int spread_occupancy(size_t bytes) { return (int)std::min<size_t>(8, (160 * 1024) / (bytes + 64)); }
+
+// Bricks of 2x2x2 cells; on grids so fine that such a brick's tile (with
+// the weights of 64 sites) does not fit one CU's LDS, the first of 2x2x1,
+// 2x1x1 and 1x1x1 cells that fits.
+static Brick brick_args(const ttc_ctx *c) {
+    static const int shapes[4][3] = {{2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
+    Brick br;
+    for (const auto &sh : shapes) {
+        br.ncx = c->ncx; br.ncy = c->ncy; br.ncz = c->ncz;
+        br.bx = c->ncx >= sh[0] ? sh[0] : 1; br.by = c->ncy >= sh[1] ? sh[1] : 1; br.bz = c->ncz >= sh[2] ? sh[2] : 1;
+        br.nbx = (c->ncx + br.bx - 1) / br.bx; br.nby = (c->ncy + br.by - 1) / br.by;
+        br.nbz = (c->ncz + br.bz - 1) / br.bz;
+        br.cwx = c->p.lx / c->ncx; br.cwy = c->p.ly / c->ncy; br.cwz = c->p.lz / c->ncz;
+        // the trigger fires at (rclist-rc)/4; one more step moves a site far less
+        br.delta = 0.25 * (c->p.rclist - c->p.rc) + 0.25;
+        br.chunk = 0;
+        br.rec_off = 0;
+        if (spread_occupancy(brick_shm(c, br) + 64 * SPREAD_REC_BYTES(c->order)) >= 1) break;
+    }
+    return br;
+}
 
 // the brick with its LDS layout: tile, then the weights of `chunk` sites,
 // chunk the largest of 256/128/64 that keeps the occupancy of chunk 64
