@@ -82,6 +82,17 @@ int  ttc_mc_energy(ttc_mc *m, double *e);
  * E(new) - E(old), as forces_mc(lb,ub,1) - forces_mc(lb,ub,0).          */
 int  ttc_mc_delta_energy(ttc_mc *m, int w, int lb, int ub, const double *newpos, double *de);
 
+/* ttc_mc_delta_energy without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  The new coordinates and the
+ * result pass through a fixed scratch array of the library, one per GPU
+ * (TTC_MC_NOALLOC_MAXBEADS x 3 doubles + 256 B, held until the process
+ * ends), instead of per-call allocations; ub - lb above the limit is
+ * TTC_ERR_ARG.  Calls that overlap on one GPU (several host threads, or
+ * contexts on different streams) share that array and race.              */
+#define TTC_MC_NOALLOC_MAXBEADS 1024
+int  ttc_mc_delta_energy_noalloc(ttc_mc *m, int w, int lb, int ub, const double *newpos, double *de);
+
 /* ntrials Monte Carlo trials per walker.  mode 0: plain Wang-Landau
  * acceptance on the exact energy; mode 1: delayed acceptance, a surrogate
  * energy change from the network screens each proposal and the exact

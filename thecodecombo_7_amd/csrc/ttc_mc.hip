@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -68,6 +69,7 @@ struct ttc_mc {
     long long *ck_hist = nullptr;
     uint64_t *ck_rng = nullptr;
     double t_last = 0.0;
+    int dev = 0;                // device the context was created on
 };
 
 // ---------------------------------------------------------------- device
@@ -1104,6 +1106,32 @@ static uint64_t splitmix(uint64_t &x) {
         }                       \
     } while (0)
 
+// ---- scratch of the entry point that allocates nothing -------------------
+// ttc_mc_delta_energy_noalloc passes the moved beads' new coordinates and
+// the energy change through this array instead of per-call allocations:
+// [0] the result, [NOALLOC_STAGE ...] up to TTC_MC_NOALLOC_MAXBEADS x 3
+// coordinates.  Every GPU holds its own copy until the process ends.  Host
+// code never passes the array's name to a kernel or a copy (in host code it
+// is a stand-in, not a device address): it uses the address
+// hipGetSymbolAddress returns, taken once per device by ttc_mc_create with
+// that device current.
+#define NOALLOC_STAGE 32   // doubles before the staging area (256 B)
+#define NOALLOC_MAXDEV 64
+static __device__ __align__(256) double scratch[NOALLOC_STAGE + 3 * TTC_MC_NOALLOC_MAXBEADS];
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_MAXDEV] = {};
+
+static int scratch_setup(int dev) {
+    MCHECK(dev >= 0 && dev < NOALLOC_MAXDEV, TTC_ERR_ARG, "ttc_mc_create: device index above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return TTC_OK;
+    double *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    MCHECK(ptr, TTC_ERR_HIP, "ttc_mc_create: no device address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+    return TTC_OK;
+}
+
 extern "C" int ttc_mlp_nparams(void) { return NPAR; }
 
 extern "C" int ttc_mc_create(const ttc_mc_params *p, ttc_mc **out) {
@@ -1115,8 +1143,13 @@ extern "C" int ttc_mc_create(const ttc_mc_params *p, ttc_mc **out) {
         ttc_set_error("ttc_mc_create: no HIP device");
         return TTC_ERR_HIP;
     }
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    int rs0 = scratch_setup(dev);
+    if (rs0) return rs0;
     ttc_mc *m = new ttc_mc();
     m->p = *p;
+    m->dev = dev;
     size_t W = p->nwalk, N = p->maxn;
     HIPCHK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
     HIPCHK(hipMalloc(&m->pos, sizeof(double) * W * N * 3));
@@ -1253,6 +1286,29 @@ extern "C" int ttc_mc_delta_energy(ttc_mc *m, int w, int lb, int ub, const doubl
     HIPCHK(hipStreamSynchronize(m->stream));
     hipFree(d);
     hipFree(np);
+    return TTC_OK;
+}
+
+// ttc_mc_delta_energy without allocations: the same kernel, its input and
+// output staged in this GPU's scratch array.  Nothing in the array needs to
+// start at zero: the kernel reads only the coordinates copied in by this
+// call and writes the result before it is read back.
+extern "C" int ttc_mc_delta_energy_noalloc(ttc_mc *m, int w, int lb, int ub, const double *newpos, double *de) {
+    MCHECK(m && newpos && de && w >= 0 && w < m->p.nwalk && lb >= 0 && ub <= m->hn[w] && lb < ub, TTC_ERR_ARG,
+           "ttc_mc_delta_energy_noalloc: bad argument");
+    MCHECK(ub - lb <= TTC_MC_NOALLOC_MAXBEADS, TTC_ERR_ARG,
+           "ttc_mc_delta_energy_noalloc: more than TTC_MC_NOALLOC_MAXBEADS (1024) moved beads");
+    MCHECK(m->dev >= 0 && m->dev < NOALLOC_MAXDEV && g_scratch_ptr[m->dev], TTC_ERR_ARG,
+           "ttc_mc_delta_energy_noalloc: scratch array not set up for this device");
+    double *sp = g_scratch_ptr[m->dev];
+    HIPCHK(hipMemcpyAsync(sp + NOALLOC_STAGE, newpos, sizeof(double) * 3 * (ub - lb), hipMemcpyHostToDevice,
+                          m->stream));
+    size_t shm = (size_t)(6 * m->p.maxn + MC_WAVES) * sizeof(double);
+    k_mc_delta<<<1, MC_THREADS, shm, m->stream>>>(mc_args(m), w, lb, ub, m->pos, m->nbead, m->noff, m->npart,
+                                                  m->nr0, m->ncap, sp + NOALLOC_STAGE, sp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(de, sp, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
     return TTC_OK;
 }
 
