@@ -227,6 +227,25 @@ int  ttc_timing(ttc_ctx *c, int which, double *usec);
  * (usec3[3] optional: the three timings).  No reference counterpart.    */
 int  ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3);
 
+/* The force array ff[k] of the last evaluation at the current positions:
+ * after ttc_md_vverlet the forces the last step computed
+ * (integrate_vverlet.c:88 forces(k)), after ttc_forces the same as it
+ * returned.  Written (not added) in the original site order.              */
+int  ttc_get_forces(ttc_ctx *c, double *ff, int64_t stride);
+
+/* Capacity limits for MD-loop list builds, below the allocated capacities:
+ * a build with a row longer than row_limit or a half-list window wider than
+ * window_limit reports an overflow as if the capacity were that small
+ * (0: no limit).  The loop then grows and redoes the step, which lifts the
+ * limits.  No counterpart in the reference (its row capacity is the fixed
+ * max_num_neighbors of allocate.c:321-322); exercises the overflow path.  */
+int  ttc_set_capacity_limits(ttc_ctx *c, int row_limit, int window_limit);
+
+/* Half-list windows of the velocity-Verlet loop: widest window of the
+ * current list (0 for a full list), window capacity, *half = 1 when the
+ * current list is a half list.  No counterpart in the reference.          */
+int  ttc_window_stats(ttc_ctx *c, int *max_window, int *window_capacity, int *half);
+
 /* Counters of the last list build (an export's scratch build included):
  * entries of the list (each pair in both rows), longest row, row capacity.
  * After ttc_nblist, entries / 2 is the size of the reference's half list. */

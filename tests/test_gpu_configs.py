@@ -59,6 +59,21 @@ def _check_forces_slots(box, o, s, lists):
     assert abs(e9["nbond"] - eref["nbond"]) <= 2e-6 * scale
 
 
+def _check_md_forces(box, o, s, lists):
+    """Forces of one velocity-Verlet step (half-list pair kernel) against
+    the oracle at the step's positions; the step-0 list (skin 2 A) still
+    holds every pair within rc after 1 fs."""
+    box.integrate_vverlet(1)
+    wmax, wcap, half = box.window_stats()
+    ff = box.get_forces()
+    pos = box.positions()
+    fr, _ = o.forces(pos, lists=lists)
+    err = rel_rms(ff, fr)
+    print(f"{s.name}: MD step forces (half list, widest window {wmax} of {wcap}) rel rms {err:.2e}", flush=True)
+    assert half
+    assert err < 5e-6
+
+
 def test_chains100k_step0():
     import pyoracle
     from thecodecombo_7_amd import system
@@ -77,6 +92,7 @@ def test_chains100k_step0():
         del le, xe
         print(f"chains-100k: {npair} half-list pairs identical", flush=True)
         _check_forces_slots(box, o, s, (cnt, lst, xf))
+        _check_md_forces(box, o, s, (cnt, lst, xf))
     finally:
         box.close()
 
@@ -100,6 +116,7 @@ def test_chains1m_step0():
         del gl
         print(f"chains-1M: {npair} half-list pairs identical", flush=True)
         _check_forces_slots(box, o, s, (cnt, lst, xf))
+        _check_md_forces(box, o, s, (cnt, lst, xf))
     finally:
         box.close()
 

@@ -88,6 +88,9 @@ def lib():
     L.ttc_configtemp.argtypes = [vp, dp, dp, dp, dp]
     L.ttc_set_nhc_temperature.argtypes = [vp, C.c_double, C.c_double]
     L.ttc_scale_velocities.argtypes = [vp, C.c_double]
+    L.ttc_get_forces.argtypes = [vp, dp, C.c_int64]
+    L.ttc_set_capacity_limits.argtypes = [vp, C.c_int, C.c_int]
+    L.ttc_window_stats.argtypes = [vp, ip, ip, ip]
     L.ttc_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64, C.POINTER(vp)]
     L.ttc_batch_destroy.argtypes = [vp]
     L.ttc_batch_set_launch.argtypes = [vp, C.c_int, C.c_int]

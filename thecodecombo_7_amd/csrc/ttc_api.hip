@@ -10,7 +10,24 @@
 #include <functional>
 #include <map>
 
+int ttc_pair_half_attrs(ttc_ctx *c);
+int ttc_build_grp_attrs(ttc_ctx *c);
+void ttc_launch_snap(ttc_ctx *c);
+
 static thread_local std::string g_err;
+
+// a pair of timing events destroyed on every return path
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~EventPair() {
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+    }
+    hipError_t create() {
+        hipError_t r = hipEventCreate(&e0);
+        return r != hipSuccess ? r : hipEventCreate(&e1);
+    }
+};
 
 void ttc_set_error(const std::string &msg) { g_err = msg; }
 extern "C" const char *ttc_last_error(void) { return g_err.c_str(); }
@@ -127,7 +144,19 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
     TTC_HIP(hipHostMalloc((void **)&c->h_status, sizeof(int) * ST_NSTAT, hipHostMallocDefault));
     memset(c->h_status, 0, sizeof(int) * ST_NSTAT);
 
-    int r = ttc_pme_setup(c);
+    // half-list state: partner accumulators (cleared by every combine that
+    // reads them), windows sized for this geometry
+    ALLOC(c->fixj, (size_t)3 * n);
+    TTC_HIP(hipMemset(c->fixj, 0, sizeof(long long) * 3 * n));
+    ttc_window_sizes(c, &c->wcap, &c->span, &c->span_pre);
+    ALLOC(c->win, (size_t)c->G * c->wcap);
+    ALLOC(c->wlen, c->G);
+    TTC_HIP(hipMemset(c->wlen, 0, sizeof(int) * c->G));
+    int r = ttc_pair_half_attrs(c);
+    if (r) return r;
+    r = ttc_build_grp_attrs(c);
+    if (r) return r;
+    r = ttc_pme_setup(c);
     if (r) return r;
     r = ttc_pme_attrs(c);
     if (r) return r;
@@ -160,7 +189,7 @@ extern "C" int ttc_destroy(ttc_ctx *c) {
                     c->origb, c->inv, c->fpair, c->g4, c->cell_of, c->cell_cnt, c->cell_start, c->cell_acc,
                     c->cell_members, c->nbl, c->glen, c->status, c->ex_off, c->ex_part, c->ex_flag, c->ljtab,
                     c->b_off, c->b_part, c->b_r0, c->b_k, c->partials, c->ev, c->rows, c->fs, c->fl, c->fsb,
-                    c->flb, c->nhc, c->posfb, c->gcnt, c->snap, c->g4o};
+                    c->flb, c->nhc, c->posfb, c->gcnt, c->snap, c->g4o, c->fixj, c->win, c->wlen};
     for (void *b : bufs)
         if (b) hipFree(b);
     ttc_pme_free(c);
@@ -342,11 +371,73 @@ static int read_status(ttc_ctx *c) {
 }
 
 static int grow_list(ttc_ctx *c, int need) {
+    if (c->row_limit > 0 && need <= c->cap) {
+        // a limit below the capacity reported it: lift the limit
+        c->row_limit = 0;
+        destroy_graphs(c);
+        return TTC_OK;
+    }
     hipFree(c->nbl);
     c->cap = ((int)(need * 1.2) + 127) & ~63;
     ALLOC(c->nbl, (size_t)c->G * c->cap * 64);
+    c->row_limit = 0;
     destroy_graphs(c);
     c->gen++;
+    return TTC_OK;
+}
+
+// wider half-list windows or builder bitmap after a build reported wmax
+// partners in one group, a partner at bitmap bit spanmax - 1, or a partner
+// premax sites before a group's first site
+static int grow_window(ttc_ctx *c, int wmax, int spanmax, int premax) {
+    if (wmax > c->wcap) {
+        hipFree(c->win);
+        c->wcap = ((int)(wmax * 1.2) + 63) & ~63;
+        ALLOC(c->win, (size_t)c->G * c->wcap);
+    }
+    if (spanmax > c->span || premax > c->span_pre) {
+        int below = std::max(c->span_pre, (int)(premax * 1.2));
+        int above = std::max(c->span - c->span_pre, (int)((spanmax - c->span_pre) * 1.2));
+        ttc_bitmap_sizes(c, below, above, &c->span, &c->span_pre);
+    }
+    c->win_limit = 0;
+    size_t ttc_pair_half_shm(const ttc_ctx *c);
+    if (ttc_pair_half_shm(c) > 150 * 1024) {
+        ttc_set_error("half-list window too wide for one CU's LDS (sites far denser than the box average)");
+        return TTC_ERR_LIST_OVERFLOW;
+    }
+    int r = ttc_pair_half_attrs(c);
+    if (r) return r;
+    r = ttc_build_grp_attrs(c);
+    if (r) return r;
+    destroy_graphs(c);
+    c->gen++;
+    return TTC_OK;
+}
+
+// grows whatever the last build reported too small (status already read);
+// returns TTC_ERR_LIST_OVERFLOW for a cell overflow, which growing cannot fix
+static int grow_after_overflow(ttc_ctx *c) {
+    const int ov = c->h_status[ST_OVERFLOW];
+    if (ov & 2) {
+        ttc_set_error("a neighbour cell holds more sites than its capacity");
+        return TTC_ERR_LIST_OVERFLOW;
+    }
+    if (ov & 1) {
+        int r = grow_list(c, c->h_status[ST_MAXROW]);
+        if (r) return r;
+    }
+    if (ov & 8) {
+        int r = grow_window(c, c->h_status[ST_WMAX], c->h_status[ST_SPANMAX], c->h_status[ST_PREMAX]);
+        if (r) return r;
+    }
+    return TTC_OK;
+}
+
+// clears the overflow flag and the maxima the builds report
+static int clear_overflow(ttc_ctx *c) {
+    TTC_HIP(hipMemsetAsync(c->status + ST_OVERFLOW, 0, sizeof(int) * 2, c->stream));
+    TTC_HIP(hipMemsetAsync(c->status + ST_WMAX, 0, sizeof(int) * 3, c->stream));
     return TTC_OK;
 }
 
@@ -373,6 +464,7 @@ extern "C" int ttc_nblist(ttc_ctx *c) {
         }
         c->have_list = true;
         c->list_pruned = false;
+        c->list_half = false;
         c->f4_valid = false;   // the sorted order changed
         return TTC_OK;
     }
@@ -423,19 +515,56 @@ extern "C" int ttc_get_nblist(ttc_ctx *c, int which, int max_nb, int *count, int
 // rc + 2 (rclist-rc)/4 (complete while no site moved more than the DLIST
 // distance); if a site moved further since, the list is rebuilt at the full
 // rclist from the anchors -- the reference's list -- before evaluating.
-static int need_list(ttc_ctx *c) {
+// Kernels other than the plain pair kernel need the full format: a half
+// list is replaced by the reference's list at the anchors unless allow_half.
+static int need_list(ttc_ctx *c, bool allow_half = false) {
     CHECK(c && c->topo, TTC_ERR_ARG, "topology not set");
     if (!c->have_list) return ttc_nblist(c);
-    if (!c->list_pruned) return TTC_OK;
-    int flag = 0;
-    int r = ttc_nl_check_range(c, 0, c->n, &flag);
-    if (r) return r;
-    if (!flag) return TTC_OK;
-    r = ttc_rebuild_full_at_anchors(c);
+    bool rebuild = c->list_half && !allow_half;
+    if (!rebuild && c->list_pruned) {
+        int flag = 0;
+        int r = ttc_nl_check_range(c, 0, c->n, &flag);
+        if (r) return r;
+        rebuild = flag != 0;
+    }
+    if (!rebuild) return TTC_OK;
+    int r = ttc_rebuild_full_at_anchors(c);
     if (r) return r;
     c->list_pruned = false;
+    c->list_half = false;
     return TTC_OK;
 }
+
+// The pruned list of an MD loop in the format the loop's kernels use (half
+// or full), built at the anchors in the current sorted order when the
+// current list has the other format.
+static int ensure_format(ttc_ctx *c, bool half) {
+    if (c->have_list && c->list_half == half) return TTC_OK;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        if (!c->have_list) {
+            int r = ttc_nblist(c);
+            if (r) return r;
+        }
+        int r = clear_overflow(c);
+        if (r) return r;
+        ttc_launch_build_md_at_anchors(c, half);
+        TTC_HIP(hipGetLastError());
+        r = read_status(c);
+        if (r) return r;
+        if (!(c->h_status[ST_OVERFLOW] & 11)) {
+            c->list_half = half;
+            c->list_pruned = true;
+            return TTC_OK;
+        }
+        r = grow_after_overflow(c);
+        if (r) return r;
+    }
+    ttc_set_error("list or window capacity could not be grown");
+    return TTC_ERR_LIST_OVERFLOW;
+}
+
+// half lists for the velocity-Verlet loop from 256 sites on
+static bool half_enabled(const ttc_ctx *c) { return c->n >= 256; }
 
 static int finish(ttc_ctx *c, double *ff, int64_t stride) {
     TTC_HIP(hipGetLastError());
@@ -684,11 +813,59 @@ static int capture(ttc_ctx *c, double vscal, bool energy, bool head, hipGraphExe
     return TTC_OK;
 }
 
+// The tail of a step whose conditional rebuild overflowed: the kernels
+// after that rebuild stopped (the step's drift is done, positions x_k), so
+// the list is grown and built at x_k -- the build the step made -- and the
+// rest of the step runs: spreading x_k again (the step's FFT consumed the
+// grid), pair kernel, FFTs, gather, second half kick and the energy row.
+static int redo_overflowed_step(ttc_ctx *c, double vscal, bool energy) {
+    for (int attempt = 0; attempt < 4; attempt++) {
+        int r = grow_after_overflow(c);
+        if (r) return r;
+        r = clear_overflow(c);
+        if (r) return r;
+        ttc_launch_rebuild_md(c);
+        TTC_HIP(hipGetLastError());
+        r = read_status(c);
+        if (r) return r;
+        if (!(c->h_status[ST_OVERFLOW] & 11)) {
+            ttc_launch_spread(c);
+            ttc_launch_pair_step(c, energy);
+            ttc_launch_fft(c, energy);
+            ttc_launch_gather_orig(c, energy);
+            ttc_launch_kick_tail(c, vscal, energy);
+            if (energy) ttc_launch_reduce(c, true);
+            TTC_HIP(hipGetLastError());
+            return TTC_OK;
+        }
+    }
+    ttc_set_error("ttc_md_vverlet: list capacity could not be grown");
+    return TTC_ERR_LIST_OVERFLOW;
+}
+
+static int capture_vv_graphs(ttc_ctx *c, double vscal) {
+    destroy_graphs(c);
+    for (int h = 0; h < 2; h++)
+        for (int e = 0; e < 2; e++) {
+            int r = capture(c, vscal, e == 1, h == 1, &c->g_vv[h][e]);
+            if (r) return r;
+        }
+    int r = capture(c, vscal, false, true, &c->g_vv_run, TTC_VV_RUN);
+    if (r) return r;
+    c->g_vscal = vscal;
+    return TTC_OK;
+}
+
 extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_every, double *energies,
                               int max_rows, int *nrows, int *rebuilds) {
     CHECK(c && c->topo && nsteps >= 0, TTC_ERR_ARG, "ttc_md_vverlet: bad argument");
     CHECK(energy_every == 0 || (energies && max_rows > 0), TTC_ERR_ARG, "ttc_md_vverlet: no row buffer");
-    int r = need_list(c);
+    const bool half = half_enabled(c);
+    if (c->half_mode != half) {
+        c->half_mode = half;
+        destroy_graphs(c);
+    }
+    int r = need_list(c, half);
     if (r) return r;
     if (energy_every > 0 && max_rows > c->rows_cap) {
         if (c->rows) hipFree(c->rows);
@@ -701,49 +878,56 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
         launch_forces(c, false);
         c->f4_valid = true;
     }
-    if (!c->g_vv[0][0] || c->g_vscal != vscal) {
-        destroy_graphs(c);
-        for (int h = 0; h < 2; h++)
-            for (int e = 0; e < 2; e++) {
-                r = capture(c, vscal, e == 1, h == 1, &c->g_vv[h][e]);
-                if (r) return r;
-            }
-        r = capture(c, vscal, false, true, &c->g_vv_run, TTC_VV_RUN);
-        if (r) return r;
-        c->g_vscal = vscal;
-    }
+    r = ensure_format(c, half);
+    if (r) return r;
     // counters; the trigger for step 0 is checked on the current positions
     TTC_HIP(hipMemsetAsync(c->status, 0, sizeof(int) * ST_NSTAT, c->stream));
     ttc_launch_nlcheck(c);
-    hipEvent_t e0, e1;
-    TTC_HIP(hipEventCreate(&e0));
-    TTC_HIP(hipEventCreate(&e1));
-    TTC_HIP(hipEventRecord(e0, c->stream));
+    EventPair ev;
+    TTC_HIP(ev.create());
+    TTC_HIP(hipEventRecord(ev.e0, c->stream));
     bool kicked = true;   // the state ends a full step: no kick pending
-    for (int step = 1; step <= nsteps;) {
-        // TTC_VV_RUN plain steps in one graph (no launch gap between them)
-        // when a kick is pending and none of them writes an energy row
-        int last = step + TTC_VV_RUN - 1;
-        bool run = !kicked && last <= nsteps &&
-                   (energy_every <= 0 || last / energy_every == (step - 1) / energy_every);
-        if (run) {
-            TTC_HIP(hipGraphLaunch(c->g_vv_run, c->stream));
-            step = last + 1;
-            continue;
+    int step = 1;
+    for (int redo = 0;; redo++) {
+        if (!c->g_vv[0][0] || c->g_vscal != vscal) {
+            r = capture_vv_graphs(c, vscal);
+            if (r) return r;
         }
-        bool en = energy_every > 0 && step % energy_every == 0;
-        TTC_HIP(hipGraphLaunch(c->g_vv[kicked ? 0 : 1][en ? 1 : 0], c->stream));
-        kicked = en;
-        step++;
+        while (step <= nsteps) {
+            // TTC_VV_RUN plain steps in one graph (no launch gap between
+            // them) when a kick is pending and none of them writes an energy row
+            int last = step + TTC_VV_RUN - 1;
+            bool run = !kicked && last <= nsteps &&
+                       (energy_every <= 0 || last / energy_every == (step - 1) / energy_every);
+            if (run) {
+                TTC_HIP(hipGraphLaunch(c->g_vv_run, c->stream));
+                step = last + 1;
+                continue;
+            }
+            bool en = energy_every > 0 && step % energy_every == 0;
+            TTC_HIP(hipGraphLaunch(c->g_vv[kicked ? 0 : 1][en ? 1 : 0], c->stream));
+            kicked = en;
+            step++;
+        }
+        if (!kicked) ttc_launch_kick_tail(c, vscal, false);
+        kicked = true;
+        r = read_status(c);
+        if (r) return r;
+        if (!(c->h_status[ST_OVERFLOW] & TTC_OVF_REDO)) break;
+        // a rebuild overflowed: ST_STEP steps completed, the next one
+        // stopped after its drift; its remaining kernels run here, then the
+        // loop continues after it (graphs are re-captured if buffers grew)
+        CHECK(redo < 16, TTC_ERR_LIST_OVERFLOW, "ttc_md_vverlet: repeated list overflow");
+        const int failed = c->h_status[ST_STEP] + 1;
+        r = redo_overflowed_step(c, vscal, energy_every > 0 && failed % energy_every == 0);
+        if (r) return r;
+        step = failed + 1;
     }
-    if (!kicked) ttc_launch_kick_tail(c, vscal, false);
-    TTC_HIP(hipEventRecord(e1, c->stream));
+    TTC_HIP(hipEventRecord(ev.e1, c->stream));
     r = read_status(c);
     if (r) return r;
     float ms = 0.0f;
-    TTC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    TTC_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
     c->t_step = nsteps > 0 ? 1000.0 * ms / nsteps : 0.0;
     int nr = std::min(c->h_status[ST_NROWS], max_rows);
     if (nrows) *nrows = energy_every > 0 ? nr : 0;
@@ -752,6 +936,10 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
     if (energy_every > 0 && nr > 0)
         TTC_HIP(hipMemcpy(energies, c->rows, sizeof(double) * ROW_LEN * nr, hipMemcpyDeviceToHost));
     c->mts_valid = false;
+    if (c->h_status[ST_OVERFLOW] & 16) {
+        ttc_set_error("ttc_md_vverlet: a pair force beyond 6.7e7 kJ/mol/A (overlapping sites)");
+        return TTC_ERR_ARG;
+    }
     if (c->h_status[ST_OVERFLOW]) {
         ttc_set_error("ttc_md_vverlet: neighbour list overflow during the run (nblist.c:185-196)");
         return TTC_ERR_LIST_OVERFLOW;
@@ -763,12 +951,14 @@ extern "C" int ttc_timing(ttc_ctx *c, int which, double *usec) {
     CHECK(c && usec, TTC_ERR_ARG, "ttc_timing: bad argument");
     if (which == 2) { *usec = c->t_step; return TTC_OK; }
     CHECK(which >= 0 && which <= 4, TTC_ERR_ARG, "ttc_timing: which must be 0 .. 4");
-    int r = need_list(c);
+    int r = need_list(c, true);   // the pair kernel of the current list format
     if (r) return r;
     const int reps = 20;
-    hipEvent_t e0, e1;
-    TTC_HIP(hipEventCreate(&e0));
-    TTC_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    TTC_HIP(ev.create());
+    // the gather in the original numbering reads the positions snapshot an
+    // MD step's drift writes: take it from the current positions
+    if (which == 4) ttc_launch_snap(c);
     // warm-up launch, then timed back-to-back launches on the context stream
     auto launch = [&] {
         if (which == 0) ttc_launch_pair(c, false, 0);
@@ -777,14 +967,17 @@ extern "C" int ttc_timing(ttc_ctx *c, int which, double *usec) {
         else ttc_launch_gather_orig(c, false);
     };
     launch();
-    TTC_HIP(hipEventRecord(e0, c->stream));
+    TTC_HIP(hipEventRecord(ev.e0, c->stream));
     for (int i = 0; i < reps; i++) launch();
-    TTC_HIP(hipEventRecord(e1, c->stream));
-    TTC_HIP(hipEventSynchronize(e1));
+    TTC_HIP(hipEventRecord(ev.e1, c->stream));
+    TTC_HIP(hipEventSynchronize(ev.e1));
     float ms = 0.0f;
-    TTC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    TTC_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
+    if (which == 0) {
+        // the repeated half-list launches accumulated partner forces
+        TTC_HIP(hipMemsetAsync(c->fixj, 0, sizeof(long long) * 3 * c->n, c->stream));
+        TTC_HIP(hipStreamSynchronize(c->stream));
+    }
     if (which == 1) {
         // the repeated spreads accumulated: clear the grid again
         TTC_HIP(hipMemsetAsync(c->qfix, 0, sizeof(long long) * c->ngrid, c->stream));
@@ -805,17 +998,24 @@ extern "C" int ttc_timing(ttc_ctx *c, int which, double *usec) {
 extern "C" int ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3) {
     CHECK(c && c->topo, TTC_ERR_ARG, "ttc_tune: topology not set");
     const int cand[3] = {4, 8, 16};
-    int old = c->pair_waves, best = old;
+    // the pair kernel of the current list (the half-list kernel after a
+    // velocity-Verlet loop, the full-list kernel otherwise)
+    int r0 = need_list(c, true);
+    if (r0) return r0;
+    int &waves = c->list_half ? c->pair_waves_h : c->pair_waves;
+    const int ncand = c->list_half ? 2 : 3;   // the half-list kernel runs 4 or 8 waves
+    int old = waves, best = old;
     double tbest = 1e300;
-    for (int i = 0; i < 3; i++) {
-        c->pair_waves = cand[i];
+    if (usec3) usec3[2] = 0.0;
+    for (int i = 0; i < ncand; i++) {
+        waves = cand[i];
         double t = 0.0;
         int r = ttc_timing(c, 0, &t);
-        if (r) { c->pair_waves = old; return r; }
+        if (r) { waves = old; return r; }
         if (usec3) usec3[i] = t;
         if (t < tbest) { tbest = t; best = cand[i]; }
     }
-    c->pair_waves = best;
+    waves = best;
     c->t_pair = tbest;
     if (best != old) {
         destroy_graphs(c);
@@ -837,6 +1037,35 @@ extern "C" int ttc_list_stats(ttc_ctx *c, int64_t *entries, int *max_row, int *c
     if (entries) *entries = tot;   // entries of the device list (each pair in both rows)
     if (max_row) *max_row = mx;
     if (capacity) *capacity = c->cap;
+    return TTC_OK;
+}
+
+extern "C" int ttc_get_forces(ttc_ctx *c, double *ff, int64_t stride) {
+    CHECK(c && ff && stride >= 3, TTC_ERR_ARG, "ttc_get_forces: bad argument");
+    CHECK(c->f4_valid, TTC_ERR_ARG, "ttc_get_forces: no force of the current positions (run a step or ttc_forces)");
+    return get_xyz(c, ff, stride, c->f4, false);
+}
+
+extern "C" int ttc_set_capacity_limits(ttc_ctx *c, int row_limit, int window_limit) {
+    CHECK(c && row_limit >= 0 && window_limit >= 0, TTC_ERR_ARG, "ttc_set_capacity_limits: bad argument");
+    c->row_limit = row_limit;
+    c->win_limit = window_limit;
+    destroy_graphs(c);
+    return TTC_OK;
+}
+
+extern "C" int ttc_window_stats(ttc_ctx *c, int *max_window, int *window_capacity, int *half) {
+    CHECK(c, TTC_ERR_ARG, "ttc_window_stats: null context");
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    int wl = 0;
+    if (c->list_half) {
+        std::vector<int> w(c->G);
+        TTC_HIP(hipMemcpy(w.data(), c->wlen, sizeof(int) * c->G, hipMemcpyDeviceToHost));
+        for (int g = 0; g < c->G; g++) wl = std::max(wl, w[g]);
+    }
+    if (max_window) *max_window = wl;
+    if (window_capacity) *window_capacity = c->wcap;
+    if (half) *half = c->list_half ? 1 : 0;
     return TTC_OK;
 }
 
@@ -917,21 +1146,18 @@ static int run_graphs(ttc_ctx *c, hipGraphExec_t plain, hipGraphExec_t energy, i
                       double *energies, int max_rows, int *nrows, int *rebuilds) {
     TTC_HIP(hipMemsetAsync(c->status, 0, sizeof(int) * ST_NSTAT, c->stream));
     ttc_launch_nlcheck(c);
-    hipEvent_t e0, e1;
-    TTC_HIP(hipEventCreate(&e0));
-    TTC_HIP(hipEventCreate(&e1));
-    TTC_HIP(hipEventRecord(e0, c->stream));
+    EventPair ev;
+    TTC_HIP(ev.create());
+    TTC_HIP(hipEventRecord(ev.e0, c->stream));
     for (int step = 1; step <= nsteps; step++) {
         bool en = energy_every > 0 && step % energy_every == 0;
         TTC_HIP(hipGraphLaunch(en ? energy : plain, c->stream));
     }
-    TTC_HIP(hipEventRecord(e1, c->stream));
+    TTC_HIP(hipEventRecord(ev.e1, c->stream));
     int r = read_status(c);
     if (r) return r;
     float ms = 0.0f;
-    TTC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    TTC_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
     c->t_step = nsteps > 0 ? 1000.0 * ms / nsteps : 0.0;
     int nr = std::min(c->h_status[ST_NROWS], max_rows);
     if (nrows) *nrows = energy_every > 0 ? nr : 0;
@@ -957,10 +1183,19 @@ static int ensure_rows(ttc_ctx *c, int energy_every, int max_rows) {
     return TTC_OK;
 }
 
+// the NHC and RESPA loops keep full lists
+static void full_mode(ttc_ctx *c) {
+    if (c->half_mode) {
+        c->half_mode = false;
+        destroy_graphs(c);
+    }
+}
+
 extern "C" int ttc_md_mts(ttc_ctx *c, int nouter, int ninner, double dtlong, double vscal, int energy_every,
                           double *energies, int max_rows, int *nrows, int *rebuilds) {
     CHECK(c && c->topo && nouter >= 0 && ninner >= 1 && dtlong > 0, TTC_ERR_ARG, "ttc_md_mts: bad argument");
     CHECK(energy_every == 0 || (energies && max_rows > 0), TTC_ERR_ARG, "ttc_md_mts: no row buffer");
+    full_mode(c);
     int r = need_list(c);
     if (r) return r;
     r = ensure_rows(c, energy_every, max_rows);
@@ -1069,6 +1304,7 @@ void ttc_launch_nhc_step(ttc_ctx *c, bool energy) {
 // current positions
 int ttc_prepare_nhc(ttc_ctx *c, int max_rows) {
     CHECK(c && c->topo && c->nhc_M > 0, TTC_ERR_ARG, "ttc_md_nhc: no topology or no thermostat");
+    full_mode(c);
     int r = need_list(c);
     if (r) return r;
     r = ensure_rows(c, 1, max_rows);
@@ -1085,6 +1321,7 @@ extern "C" int ttc_md_nhc(ttc_ctx *c, int nsteps, int energy_every, double *ener
                           int *rebuilds) {
     CHECK(c && c->topo && nsteps >= 0 && c->nhc_M > 0, TTC_ERR_ARG, "ttc_md_nhc: bad argument or no thermostat");
     CHECK(energy_every == 0 || (energies && max_rows > 0), TTC_ERR_ARG, "ttc_md_nhc: no row buffer");
+    full_mode(c);
     int r = need_list(c);
     if (r) return r;
     r = ensure_rows(c, energy_every, max_rows);

@@ -10,9 +10,11 @@
 //     f4[s]    = {fx, fy, fz, 0}         FP64 total force of the last evaluation
 //     type[s], orig[s] (original index), inv[orig] = s
 //     anchor[s] = position at the last list build (DLIST)
-//     fix[3][N] int64 fixed-point force accumulator (2^-32 kJ/mol/A units)
-//   Neighbour list: full list (each pair in both rows), ELL layout per group
-//   of 64 consecutive sorted sites, tiled by 16 entries:
+//     fixj[3][N] int64 partner forces of the half-list pair kernel (2^-24 kJ/mol/A)
+//   Neighbour list: full list (each pair in both rows) or, in
+//   velocity-Verlet loops, half list (each pair in the row of its lower site,
+//   entries holding window slots instead of j; see ttc_ctx::half_mode), ELL
+//   layout per group of 64 consecutive sorted sites, tiled by 16 entries:
 //   nbl[g][k/16][lane][k%16] (uint32), glen[g] entries per row (cap, the
 //   row capacity, is a multiple of 64).  A row's consecutive entries share a
 //   64-byte segment, so appends by the list build coalesce, and the 16 k of
@@ -40,6 +42,7 @@
 #define TTC_MAX_SITES (1 << 24)
 #define TTC_FIX_SCALE 4294967296.0          /* 2^32 */
 #define TTC_FIX_INV (1.0 / 4294967296.0)
+#define TTC_FIXJ_SHIFT 8                    /* partner forces in 2^-24 units: << 8 gives 2^-32 */
 #define TTC_QFIX_SCALE 1099511627776.0      /* 2^40 */
 #define TTC_QFIX_INV (1.0 / 1099511627776.0)
 #define TTC_MAX_TYPES 64
@@ -52,7 +55,13 @@
 #define TTC_AU 1.6605655E-27
 #define TTC_QFACTOR (8.987551789e19 * 1.602176462e-19 * 1.602176462e-19 * 6.022045E23 * 1.0E-3)
 
-enum { ST_NLFLAG = 0, ST_OVERFLOW = 1, ST_MAXROW = 2, ST_GRIDERR = 3, ST_NSTAT = 8 };
+enum { ST_NLFLAG = 0, ST_OVERFLOW = 1, ST_MAXROW = 2, ST_GRIDERR = 3, ST_NSTAT = 12 };
+// ST_OVERFLOW bits: 1 a row longer than the list capacity, 2 a cell fuller
+// than its capacity, 4 an exported row longer than max_nb, 8 a half-list
+// window wider than its capacity (or a partner beyond the builder's bitmap),
+// 16 a pair force beyond the range of the partner accumulators.  Bits 1 and
+// 8 stop an MD loop's remaining steps until the host has grown the list.
+#define TTC_OVF_REDO (1 | 8)
 
 // energy/virial slots reduced on the device (FP64)
 enum {
@@ -180,11 +189,29 @@ struct ttc_ctx {
     // tuned launch shape: waves per 64-site group of the pair kernel (4, 8, 16)
     int pair_waves = 8;
 
+    // Half list (velocity-Verlet MD loops): each pair once, in the row of
+    // the lower site (minimum-image dz > 0, ties by index).  An entry holds the slot of j in its group's window (the
+    // group's distinct partners, ascending, win[g * wcap + slot] = j) instead
+    // of j; the pair kernel adds the partners' forces into LDS int64
+    // accumulators per window slot and flushes them to fixj.
+    bool half_mode = false;         // conditional rebuilds build half lists
+    bool list_half = false;         // the current list is a half list
+    int *win = nullptr;             // [G][wcap] window of every group
+    int *wlen = nullptr;            // [G] window length
+    int wcap = 0;                   // window capacity (multiple of 64)
+    int span = 0;                   // partner bitmap of the builder, bits (multiple of 64)
+    int span_pre = 0;               // its bit of the group's first site
+    long long *fixj = nullptr;      // [3][n] partner forces, 2^-24 kJ/mol/A; read and cleared by the combine
+    int pair_waves_h = 8;           // waves per group of the half-list pair kernel
+    // limits below the capacities (0: none) at which MD-loop builds report
+    // an overflow; cleared by the next growth (ttc_set_capacity_limits)
+    int row_limit = 0, win_limit = 0;
+
     // timing (microseconds)
     double t_pair = 0, t_spread = 0, t_step = 0;
 };
 
-enum { ST_NREBUILD = 4, ST_STEP = 5, ST_NROWS = 6, ST_TICKET = 7 };
+enum { ST_NREBUILD = 4, ST_STEP = 5, ST_NROWS = 6, ST_TICKET = 7, ST_WMAX = 8, ST_SPANMAX = 9, ST_PREMAX = 10 };
 enum { TTC_NHC_SCALE = 24, TTC_NHC_T = 25, TTC_NHC_Q = 26, TTC_NHC_VMUL = 27, TTC_NHC_LEN = 28 };
 
 #define ROW_LEN 9
@@ -193,6 +220,16 @@ enum { TTC_NHC_SCALE = 24, TTC_NHC_T = 25, TTC_NHC_Q = 26, TTC_NHC_VMUL = 27, TT
 // nblist
 int ttc_launch_rebuild(ttc_ctx *c, bool conditional);   // cells, sort, permute, list
 void ttc_launch_nlcheck(ttc_ctx *c);
+// unconditional group-cooperative build at the current positions (cells,
+// sort, pruned list in the context's half_mode format), as an MD loop's
+// rebuild makes it; the host-side redo after an overflow inside a loop
+int ttc_launch_rebuild_md(ttc_ctx *c);
+// a pruned list (half or full format) at the anchors in the current sorted
+// order, leaving the site state alone
+int ttc_launch_build_md_at_anchors(ttc_ctx *c, bool half);
+// window capacity and builder bitmap size for this context's geometry
+void ttc_window_sizes(const ttc_ctx *c, int *wcap, int *span, int *pre);
+void ttc_bitmap_sizes(const ttc_ctx *c, int below, int above, int *span, int *pre);
 int ttc_export_list(ttc_ctx *c, double cutoff, int max_nb, int *count, int *list, int *xflag);
 int ttc_nl_check_range(ttc_ctx *c, int lb, int ub, int *flag);
 double ttc_list_cutoff(const ttc_ctx *c);
@@ -331,12 +368,24 @@ __device__ __forceinline__ void ttc_mean_pme(const double *partials, int maxb, i
 // fixed point (2^-32 kJ/mol/A) so the total does not depend on the order.
 // The gather force is read at gi (the site's original index where the
 // gather ran in the original numbering), the pair force at s.
+// With mask 1 the partner forces of a half-list pair kernel (fixj, 2^-24
+// units, [3][n]) are added too and cleared for the next evaluation (they are
+// zero after a full-list pair kernel).
 __device__ __forceinline__ double4 ttc_combine(int s, int mask, const double4 *fpair, const double4 *g4,
-                                               const double *mean, int gi = -1) {
+                                               const double *mean, int gi = -1, long long *fixj = nullptr,
+                                               int n = 0) {
     long long fx = 0, fy = 0, fz = 0;
     if (mask & 1) {
         double4 a = fpair[s];
         fx += ttc_to_fix(a.x); fy += ttc_to_fix(a.y); fz += ttc_to_fix(a.z);
+        if (fixj) {
+            fx += fixj[s] << TTC_FIXJ_SHIFT;
+            fy += fixj[n + s] << TTC_FIXJ_SHIFT;
+            fz += fixj[2 * n + s] << TTC_FIXJ_SHIFT;
+            fixj[s] = 0;
+            fixj[n + s] = 0;
+            fixj[2 * n + s] = 0;
+        }
     }
     if (mask & 2) {
         double4 b = g4[gi < 0 ? s : gi];

@@ -12,12 +12,12 @@ namespace {
 
 __global__ void __launch_bounds__(256)
 k_combine(int n, int mask, const double4 *fpair, const double4 *g4, double4 *f4, const double *partials,
-          int maxb, int nbg) {
+          int maxb, int nbg, long long *fixj) {
     __shared__ double mean[3];
     if (mask & 4) ttc_mean_pme(partials, maxb, nbg, n, mean);
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    f4[s] = ttc_combine(s, mask, fpair, g4, mean);
+    f4[s] = ttc_combine(s, mask, fpair, g4, mean, -1, fixj, n);
 }
 
 // kinet.c:18-68 with uu = vv: ken = 0.125 m (v+u)^2 * NA*AU*10
@@ -47,7 +47,11 @@ __device__ __forceinline__ void ken_block(int bid, double m, double4 v, double *
 template <bool ENERGY>
 __device__ __forceinline__ void kick_body(int bid, int n, double4 *vel4, const double4 *fpair, const double4 *g4,
                                           double4 *f4, double dt, double vscal, double *partials, int maxb, int nbg,
-                                          int *status, const int *gorig = nullptr) {
+                                          int *status, const int *gorig = nullptr, long long *fixj = nullptr,
+                                          const int *ovf = nullptr) {
+    // an MD loop whose rebuild overflowed stops here; the host grows the
+    // list and redoes the step
+    if (ovf && (*ovf & TTC_OVF_REDO)) return;
     __shared__ double mean[3];
     ttc_mean_pme(partials, maxb, nbg, n, mean);
     int s = bid * blockDim.x + threadIdx.x;
@@ -58,7 +62,7 @@ __device__ __forceinline__ void kick_body(int bid, int n, double4 *vel4, const d
     double4 v = make_double4(0, 0, 0, 1);
     bool ok = s < n;
     if (ok) {
-        double4 f = ttc_combine(s, 7, fpair, g4, mean, gorig ? gorig[s] : s);
+        double4 f = ttc_combine(s, 7, fpair, g4, mean, gorig ? gorig[s] : s, fixj, n);
         f4[s] = f;
         v = vel4[s];
         double dth = 0.5 * dt;
@@ -80,19 +84,21 @@ struct KickPack {
     int maxb, nbg;
     int *status;
     const int *gorig = nullptr;   // g4 in the original numbering: read at gorig[s]
+    long long *fixj = nullptr;    // partner forces of a half-list pair kernel
+    const int *ovf = nullptr;     // overflow flags of an MD loop (stop when set)
 };
 
 template <bool ENERGY>
 __global__ void __launch_bounds__(256) k_kick(KickPack a) {
     kick_body<ENERGY>(blockIdx.x, a.n, a.vel4, a.fpair, a.g4, a.f4, a.dt, a.vscal, a.partials, a.maxb, a.nbg, a.status,
-                      a.gorig);
+                      a.gorig, a.fixj, a.ovf);
 }
 template <bool ENERGY>
 __global__ void __launch_bounds__(256) kb_kick(const KickPack *pk, const TtcBmap *map) {
     const TtcBmap m = map[blockIdx.x];
     const KickPack a = pk[m.rep];
     kick_body<ENERGY>(m.blk, a.n, a.vel4, a.fpair, a.g4, a.f4, a.dt, a.vscal, a.partials, a.maxb, a.nbg, a.status,
-                      a.gorig);
+                      a.gorig, a.fixj, a.ovf);
 }
 
 __device__ __forceinline__ void kinet_body(int bid, int n, const double4 *vel4, double *partials, int maxb) {
@@ -142,7 +148,8 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs &ra, const double *
     }
     if (threadIdx.x == 0) {
         ev[EV_KIN] = ev[EV_KE0] + ev[EV_KE2] + ev[EV_KE5];
-        if (append) {
+        // no row for a step an overflow stopped (the host's redo appends it)
+        if (append && !(status[ST_OVERFLOW] & TTC_OVF_REDO)) {
             int r = status[ST_NROWS];
             if (r < rows_cap) {
                 double erec = ev[EV_REC];
@@ -191,7 +198,7 @@ int ttc_kick_blocks(ttc_ctx *c) { return (c->n + 255) / 256; }
 void ttc_launch_combine(ttc_ctx *c, int mask) {
     int blocks = (c->n + 255) / 256;
     k_combine<<<blocks, 256, 0, c->stream>>>(c->n, mask, c->fpair, c->g4, c->f4, c->partials, c->nblk_max,
-                                             ttc_gather_blocks(c));
+                                             ttc_gather_blocks(c), c->fixj);
 }
 
 // device constants for the trigger: {lx, ly, lz, (rclist-rc)/4}
@@ -201,7 +208,7 @@ static double *check_consts(ttc_ctx *c) {
 
 static KickPack kick_pack(ttc_ctx *c, double vscal) {
     return KickPack{c->n, c->vel4, c->fpair, c->g4, c->f4, c->p.dt, vscal, c->partials, c->nblk_max,
-                    ttc_gather_blocks(c), c->status};
+                    ttc_gather_blocks(c), c->status, nullptr, c->fixj, nullptr};
 }
 
 void ttc_launch_kick(ttc_ctx *c, double vscal, bool energy) {
@@ -216,6 +223,7 @@ void ttc_launch_kick_tail(ttc_ctx *c, double vscal, bool energy) {
     a.status = nullptr;
     a.g4 = c->g4o;
     a.gorig = c->orig;
+    a.ovf = c->status + ST_OVERFLOW;
     if (energy) k_kick<true><<<blocks, 256, 0, c->stream>>>(a);
     else k_kick<false><<<blocks, 256, 0, c->stream>>>(a);
 }
@@ -252,12 +260,12 @@ namespace {
 
 __global__ void __launch_bounds__(256)
 k_combine_to(int n, int mask, const double4 *fpair, const double4 *g4, double4 *out, const double *partials,
-             int maxb, int nbg) {
+             int maxb, int nbg, long long *fixj) {
     __shared__ double mean[3];
     if (mask & 4) ttc_mean_pme(partials, maxb, nbg, n, mean);
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    out[s] = ttc_combine(s, mask, fpair, g4, mean);
+    out[s] = ttc_combine(s, mask, fpair, g4, mean, -1, fixj, n);
 }
 
 // v = A v + B (100/m) F; the first half kick of integrate_vverlet.c:57-59 is
@@ -407,7 +415,7 @@ __global__ void __launch_bounds__(256) kb_vscale(const VscalePack *pk, const Ttc
 
 void ttc_launch_combine_to(ttc_ctx *c, int mask, double4 *out) {
     k_combine_to<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, mask, c->fpair, c->g4, out, c->partials,
-                                                            c->nblk_max, ttc_gather_blocks(c));
+                                                            c->nblk_max, ttc_gather_blocks(c), c->fixj);
 }
 
 void ttc_launch_vkick(ttc_ctx *c, const double4 *F, double A, double B, bool check, bool ke, bool clear_flag) {

@@ -30,8 +30,11 @@ __device__ __forceinline__ int cell_coord(double x, double h, double l, int nc) 
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
+// a conditional (MD-loop) rebuild runs only when the trigger fired, and not
+// after an earlier rebuild of the same loop overflowed (the host grows the
+// list and redoes that step)
 __device__ __forceinline__ bool rebuild_off(const int *status, int conditional) {
-    return conditional && status[ST_NLFLAG] == 0;
+    return conditional && (status[ST_NLFLAG] == 0 || (status[ST_OVERFLOW] & TTC_OVF_REDO) != 0);
 }
 
 // Counts to cell starts in one workgroup of 256 threads: chunks of
@@ -462,6 +465,28 @@ __device__ void row_ranges(int r, const AxisRange &ay, const AxisRange &az, floa
     }
 }
 
+// Half lists (HalfWin.half != 0): a candidate j is kept in row i only if it
+// lies above i: dz = z_j - z_i (FP32, the wrapped coordinates every row
+// reads alike) taken to the minimum image is > 0, or dz == 0 and j > i.  The
+// computed dz of (j, i) is exactly minus that of (i, j), so every pair is
+// listed once, and each row holds the upper half of its sphere (rows of
+// even length, unlike a split by sorted index, which follows cell planes).
+// Every kept partner sets bit jrel = (j - sb) + pre of an LDS bitmap (sb:
+// the group's first site, j - sb taken cyclically into (-n/2, n/2]; pre
+// covers the partners of the same cell plane below the group); after the
+// rows are complete the bitmap's prefix counts give each distinct partner
+// its window slot (ascending cyclic order from sb - pre), the window
+// win[g][slot] = j is written, and every entry's j is replaced by its slot.
+struct HalfWin {
+    int half;        // build a half list with windows
+    int span;        // bitmap bits (multiple of 64)
+    int pre;         // bitmap bit of the group's first site
+    int wcap;        // window capacity
+    int *win, *wlen;
+    int wlim;        // window length above which an overflow is reported (<= wcap)
+    int rlim;        // row length above which an overflow is reported (<= cap)
+};
+
 __device__ __forceinline__ void build_grp_body(
     int grp, int n, int cap, const GbGeom &g, const float4 *__restrict__ posfb, const double4 *__restrict__ pos4b,
     const double4 *__restrict__ vel4b, const double4 *__restrict__ f4b, const int *__restrict__ typeb,
@@ -469,14 +494,20 @@ __device__ __forceinline__ void build_grp_body(
     const double4 *__restrict__ fsb, const double4 *__restrict__ flb, double4 *fs, double4 *fl,
     const int *__restrict__ cell_cnt, const int *__restrict__ cell_start, const int *__restrict__ ex_off,
     const int *__restrict__ ex_part, const int *__restrict__ ex_flag, int ex_span, uint32_t *nbl, int *glen,
-    int *gcnt, int *status, int conditional) {
+    int *gcnt, int *status, int conditional, int copy_state, const HalfWin &hw) {
     if (rebuild_off(status, conditional)) return;
+    // partner bitmap (span bits) and its per-word prefix counts
+    extern __shared__ unsigned long long hbm[];
+    unsigned int *hpre = (unsigned int *)(hbm + (hw.half ? hw.span / 64 : 0));
+    const int nword = hw.half ? hw.span / 64 : 0;
+    for (int w = threadIdx.x; w < nword; w += blockDim.x) hbm[w] = 0ull;
     __shared__ int srow[64];
     __shared__ int smax;
     // per wave: ring of candidates that passed the box filter (frame
     // position, original index | type bits; sorted index)
     __shared__ float4 sq[GB_WAVES][GB_QCAP];
     __shared__ int sqj[GB_WAVES][GB_QCAP];
+    __shared__ float sqz[GB_WAVES][GB_QCAP];   // wrapped z of the candidates (half lists)
     // exclusion partners of the group's sites (original index) and their
     // class; sxn = count, or -1 where the site has more than GB_EXMAX
     __shared__ int sxo[64][GB_EXMAX];
@@ -499,7 +530,7 @@ __device__ __forceinline__ void build_grp_body(
         sxn[threadIdx.x] = cnt;
     }
     // the new sorted order becomes the current state
-    for (int s = sb + (int)threadIdx.x; s < min(sb + 64, n); s += 64 * GB_WAVES) {
+    for (int s = sb + (int)threadIdx.x; copy_state && s < min(sb + 64, n); s += 64 * GB_WAVES) {
         double4 pi = pos4b[s];
         pos4[s] = pi;
         vel4[s] = vel4b[s];
@@ -526,6 +557,7 @@ __device__ __forceinline__ void build_grp_body(
         const float ux = frame1(pw.x, pref.x, g.lx, g.ilx), uy = frame1(pw.y, pref.y, g.ly, g.ily),
                     uz = frame1(pw.z, pref.z, g.lz, g.ilz);
         const int pwo = __float_as_int(pw.w) & (int)TTC_JMASK;
+        const float rawz = pw.z;   // wrapped z of row lane
         const float xmn = wave_minf(ux), xmx = wave_maxf(ux);
         const float ymn = wave_minf(uy), ymx = wave_maxf(uy);
         const float zmn = wave_minf(uz), zmx = wave_maxf(uz);
@@ -539,6 +571,7 @@ __device__ __forceinline__ void build_grp_body(
         const float cutr2 = g.cutr * g.cutr;   // box filter radius: the test radius plus a margin
         float4 *qp = sq[row0 / GB_ROWS];
         int *qj = sqj[row0 / GB_ROWS];
+        float *qz = sqz[row0 / GB_ROWS];
         int qh = 0, qn = 0;   // ring head and length (wave-uniform)
 
         // test every row against the cnt (<= 64) candidates at the head of the ring
@@ -546,6 +579,12 @@ __device__ __forceinline__ void build_grp_body(
             const int slot = (qh + lane) & (GB_QCAP - 1);
             const float4 pj = qp[slot];
             const int j = qj[slot];
+            const float zj = hw.half ? qz[slot] : 0.0f;
+            // partner's bitmap bit (cyclic offset from the group's first site)
+            int djs = j - sb;
+            if (2 * djs > n) djs -= n;
+            else if (2 * djs <= -n) djs += n;
+            const int jrel = djs + hw.pre;
             const bool ok = lane < cnt;
             const int wj = __float_as_int(pj.w);
             const int oj = wj & (int)TTC_JMASK;                                    // original index
@@ -560,7 +599,25 @@ __device__ __forceinline__ void build_grp_body(
                     dy -= g.ly * rintf(dy * g.ily);
                     dz -= g.lz * rintf(dz * g.ilz);
                 }
-                const bool hit = ok & (dx * dx + dy * dy + dz * dz < g.cut2) & (j != wb + rr);
+                bool hit = ok & (dx * dx + dy * dy + dz * dz < g.cut2) & (j != wb + rr);
+                if (hw.half) {
+                    // the upper half: minimum-image dz > 0 (ties by index)
+                    const float zi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rawz), rr));
+                    const float dzr = zj - zi;
+                    const float dzm = fmaf(-g.lz, rintf(dzr * g.ilz), dzr);
+                    hit = hit & ((dzm > 0.0f) | ((dzm == 0.0f) & (j > wb + rr)));
+                    if (hit) {
+                        if (jrel < 0) {
+                            atomicMax(&status[ST_PREMAX], -djs);
+                            atomicOr(&status[ST_OVERFLOW], 8);
+                        } else if (jrel >= hw.span) {
+                            atomicMax(&status[ST_SPANMAX], jrel + 1);
+                            atomicOr(&status[ST_OVERFLOW], 8);
+                        } else {
+                            atomicOr(&hbm[jrel >> 6], 1ull << (jrel & 63));
+                        }
+                    }
+                }
                 const unsigned long long b = __builtin_amdgcn_ballot_w64(hit);
                 if (b != 0ull) {
                     const int base = __builtin_amdgcn_readlane(rkl, rr);
@@ -640,6 +697,7 @@ __device__ __forceinline__ void build_grp_body(
                                            (GB_QCAP - 1);
                             qp[at] = make_float4(px, py, pz, pj.w);
                             qj[at] = j;
+                            qz[at] = pj.z;
                         }
                         qn += (int)__popcll(m);
                         if (qn >= 64) drain(64);
@@ -664,7 +722,7 @@ __device__ __forceinline__ void build_grp_body(
             smax = full;
             gcnt[grp] = sum;
             glen[grp] = min(full, cap);
-            if (full > cap) atomicOr(&status[ST_OVERFLOW], 1);
+            if (full > hw.rlim) atomicOr(&status[ST_OVERFLOW], 1);
             atomicMax(&status[ST_MAXROW], full);
         }
     }
@@ -673,6 +731,65 @@ __device__ __forceinline__ void build_grp_body(
     for (int r = row0; r < row0 + GB_ROWS; r++) {
         int k0 = min(srow[r], cap);
         for (int k = k0 + lane; k < m; k += 64) grow[ttc_nbl_idx(k, r)] = TTC_PAD;
+    }
+    if (!hw.half) return;
+
+    // window: prefix counts of the bitmap words (each thread a run of
+    // consecutive words, then a scan over the threads), slots in ascending
+    // jrel, win[g][slot] = j
+    __shared__ int wsc[64 * GB_WAVES];
+    const int per = (nword + blockDim.x - 1) / blockDim.x;
+    const int w0 = threadIdx.x * per, w1 = min(nword, w0 + per);
+    int cnt = 0;
+    for (int w = w0; w < w1; w++) cnt += __popcll(hbm[w]);
+    wsc[threadIdx.x] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int q = 0; q < (int)blockDim.x; q++) { int v = wsc[q]; wsc[q] = run; run += v; }
+        hw.wlen[grp] = min(run, hw.wcap);
+        atomicMax(&status[ST_WMAX], run);
+        if (run > hw.wlim) atomicOr(&status[ST_OVERFLOW], 8);
+    }
+    __syncthreads();
+    int run = wsc[threadIdx.x];
+    int *gwin = hw.win + (size_t)grp * hw.wcap;
+    for (int w = w0; w < w1; w++) {
+        unsigned long long bits = hbm[w];
+        hpre[w] = (unsigned int)run;
+        while (bits) {
+            const int bpos = __builtin_ctzll(bits);
+            bits &= bits - 1ull;
+            if (run < hw.wcap) {
+                int j = sb - hw.pre + w * 64 + bpos;
+                j = j < 0 ? j + n : (j >= n ? j - n : j);
+                gwin[run] = j;
+            }
+            run++;
+        }
+    }
+    __syncthreads();
+    // entries: j -> slot (a slot past the capacity is clamped; the overflow
+    // flag makes the host grow the window and build again)
+    for (int rr = 0; rr < GB_ROWS; rr++) {
+        const int r = row0 + rr;
+        if (sb + r >= n) break;
+        const int len = min(srow[r], cap);
+        for (int k = lane; k < len; k += 64) {
+            const uint32_t idx = (uint32_t)(((k >> 4) << 10) + r * 16 + (k & 15));
+            const uint32_t e = grow[idx];
+            int djs = (int)(e & TTC_JMASK) - sb;
+            if (2 * djs > n) djs -= n;
+            else if (2 * djs <= -n) djs += n;
+            const int jrel = djs + hw.pre;
+            int slot = 0;
+            if (jrel >= 0 && jrel < hw.span) {
+                const unsigned long long wbits = hbm[jrel >> 6];
+                slot = (int)hpre[jrel >> 6] + __popcll(wbits & ((1ull << (jrel & 63)) - 1ull));
+            }
+            slot = min(slot, hw.wcap - 1);
+            grow[idx] = (e & ~TTC_JMASK) | (uint32_t)slot;
+        }
     }
 }
 
@@ -692,12 +809,14 @@ struct BuildGrpPack {
     uint32_t *nbl;
     int *glen, *gcnt, *status;
     int conditional;
+    int copy_state;
+    HalfWin hw;
 };
 
 __device__ __forceinline__ void build_grp_pack(int grp, const BuildGrpPack &a) {
     build_grp_body(grp, a.n, a.cap, a.g, a.posfb, a.pos4b, a.vel4b, a.f4b, a.typeb, a.origb, a.pos4, a.vel4, a.f4,
                    a.type, a.orig, a.anchor, a.fsb, a.flb, a.fs, a.fl, a.cell_cnt, a.cell_start, a.ex_off, a.ex_part,
-                   a.ex_flag, a.ex_span, a.nbl, a.glen, a.gcnt, a.status, a.conditional);
+                   a.ex_flag, a.ex_span, a.nbl, a.glen, a.gcnt, a.status, a.conditional, a.copy_state, a.hw);
 }
 __global__ void __launch_bounds__(64 * GB_WAVES) k_build_grp(BuildGrpPack a) { build_grp_pack(blockIdx.x, a); }
 __global__ void __launch_bounds__(64 * GB_WAVES) kb_build_grp(const BuildGrpPack *pk, const TtcBmap *map) {
@@ -805,7 +924,7 @@ static void launch_build(ttc_ctx *c, bool conditional, double cutoff, uint32_t *
 
 // the pruned list of MD loops (cutoff ttc_compute_cutoff), FP32 test with
 // FAST_MARGIN, group-cooperative build
-static BuildGrpPack build_grp_pack_of(ttc_ctx *c) {
+static BuildGrpPack build_grp_pack_of(ttc_ctx *c, bool half = false, int cond = 1, int copy = 1) {
     GbGeom g;
     g.ncx = c->ncx; g.ncy = c->ncy; g.ncz = c->ncz;
     g.lx = (float)c->p.lx; g.ly = (float)c->p.ly; g.lz = (float)c->p.lz;
@@ -817,11 +936,18 @@ static BuildGrpPack build_grp_pack_of(ttc_ctx *c) {
     g.cutr = (float)(cf + FAST_MARGIN);
     return BuildGrpPack{c->n, c->cap, g, c->posfb, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4, c->vel4,
                         c->f4, c->type, c->orig, c->anchor, c->fsb, c->flb, c->fs, c->fl, c->cell_cnt, c->cell_start,
-                        c->ex_off, c->ex_part, c->ex_flag, c->ex_span, c->nbl, c->glen, c->gcnt, c->status, 1};
+                        c->ex_off, c->ex_part, c->ex_flag, c->ex_span, c->nbl, c->glen, c->gcnt, c->status, cond,
+                        copy, HalfWin{half ? 1 : 0, c->span, c->span_pre, c->wcap, c->win, c->wlen,
+                                      c->win_limit > 0 ? std::min(c->win_limit, c->wcap) : c->wcap,
+                                      c->row_limit > 0 ? std::min(c->row_limit, c->cap) : c->cap}};
 }
 
-static void launch_build_grp(ttc_ctx *c) {
-    k_build_grp<<<c->G, 64 * GB_WAVES, 0, c->stream>>>(build_grp_pack_of(c));
+static size_t build_grp_shm(const ttc_ctx *c, bool half) {
+    return half ? (size_t)(c->span / 64) * (sizeof(unsigned long long) + sizeof(unsigned int)) : 0;
+}
+
+static void launch_build_grp(ttc_ctx *c, bool half, int cond, int copy) {
+    k_build_grp<<<c->G, 64 * GB_WAVES, build_grp_shm(c, half), c->stream>>>(build_grp_pack_of(c, half, cond, copy));
 }
 
 static CellAssignPack cell_assign_pack(ttc_ctx *c, int cond) {
@@ -843,10 +969,67 @@ int ttc_launch_rebuild(ttc_ctx *c, bool conditional) {
     // explicit builds (nblist.c:32) make the reference's list; the
     // conditional builds of MD loops make the pruned one
     if (conditional)
-        launch_build_grp(c);
+        launch_build_grp(c, c->half_mode, 1, 1);
     else
         launch_build(c, false, ttc_list_cutoff(c), c->nbl, c->glen, c->cap, true, c->p.rclist);
     return TTC_OK;
+}
+
+// the builder's dynamic LDS (bitmap) may exceed 64 KiB in very large boxes
+int ttc_build_grp_attrs(ttc_ctx *c) {
+    int shm = (int)build_grp_shm(c, true);
+    TTC_HIP(hipFuncSetAttribute((const void *)k_build_grp, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    return TTC_OK;
+}
+
+int ttc_launch_rebuild_md(ttc_ctx *c) {
+    int blocks = (c->n + 255) / 256;
+    k_cell_assign<<<blocks, 256, 0, c->stream>>>(cell_assign_pack(c, 0));
+    k_reorder<<<blocks, 256, 0, c->stream>>>(reorder_pack(c, 0));
+    launch_build_grp(c, c->half_mode, 0, 1);
+    return TTC_OK;
+}
+
+// posfb holds the FP32 positions the current sorted order was made from,
+// i.e. the anchors
+int ttc_launch_build_md_at_anchors(ttc_ctx *c, bool half) {
+    launch_build_grp(c, half, 0, 0);
+    return TTC_OK;
+}
+
+// Window capacity: the distinct partners of a group are the sites within
+// the list radius rp of the group's box above its lowest site, about half of
+// rho (V + S rp + pi (a + b + c) rp^2 + 4/3 pi rp^3) for a box a x b x c
+// (64 consecutive sorted sites: along x over whole cells of the y-z
+// cross-section); 40 % headroom.  Bitmap: a partner lies in the cell plane
+// of its row (z, the slowest cell axis), up to one plane of sites before the
+// group's first site, or in the two planes above.
+void ttc_window_sizes(const ttc_ctx *c, int *wcap, int *span, int *pre) {
+    double vol = c->p.lx * c->p.ly * c->p.lz;
+    double rho = c->n / vol;
+    double cwx = c->p.lx / c->ncx, cwy = c->p.ly / c->ncy, cwz = c->p.lz / c->ncz;
+    double rp = ttc_compute_cutoff(c) + 2.0 * FAST_MARGIN;
+    double a = std::min(c->p.lx, std::max(cwx, 64.0 / (rho * cwy * cwz))), b = cwy, cc = cwz;
+    double vm = a * b * cc + 2.0 * (a * b + b * cc + cc * a) * rp + TTC_PI * (a + b + cc) * rp * rp +
+                4.0 / 3.0 * TTC_PI * rp * rp * rp;
+    double est = 0.5 * rho * vm + 64.0;
+    int w = ((int)(1.4 * est) + 63) & ~63;
+    *wcap = std::max(64, std::min(w, ((c->n + 63) / 64) * 64));
+    ttc_bitmap_sizes(c, (int)(1.3 * c->n / c->ncz) + 64, (int)(3.3 * c->n / c->ncz) + 128, span, pre);
+}
+
+// bitmap of the builder for partners from `below` sites before a group's
+// first site to `above` after it, rounded to 64 bits; in boxes of fewer
+// than 5 cell planes along z (or when that covers the box) every site
+void ttc_bitmap_sizes(const ttc_ctx *c, int below, int above, int *span, int *pre) {
+    long long b = ((long long)below + 63) & ~63LL;
+    long long s = (b + above + 63) & ~63LL;
+    if (c->ncz < 5 || s >= c->n) {
+        b = ((long long)c->n / 2 + 64 + 63) & ~63LL;
+        s = (2 * b + 63) & ~63LL;
+    }
+    *pre = (int)b;
+    *span = (int)s;
 }
 
 // the reference's list (full cutoff) at the anchors, in the current sorted
@@ -950,7 +1133,7 @@ int ttc_bstage_rebuild(const std::vector<ttc_ctx *> &r, TtcStage *st) {
     for (ttc_ctx *c : r) {
         pa.push_back(cell_assign_pack(c, 1));
         pr.push_back(reorder_pack(c, 1));
-        pb.push_back(build_grp_pack_of(c));
+        pb.push_back(build_grp_pack_of(c, false, 1, 1));
         nsite.push_back((c->n + 255) / 256);
         ngrp.push_back(c->G);
     }

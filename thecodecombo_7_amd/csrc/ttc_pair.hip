@@ -7,8 +7,12 @@
 // cutoff test and minimum image run in FP64 on FP64 positions (so exactly
 // the reference's pairs are evaluated); the LJ, erfc and exp arithmetic runs
 // in FP32; each site's force and the energies/virial accumulate in FP64.
-// The list is full (both directions), so no scatter to j and no atomics:
-// every site's force is a fixed-order sum, deterministic by construction.
+// Two list formats:
+//   * full lists (both directions; API calls, NHC and RESPA loops, replica
+//     batches): no scatter to j, every site's force is a fixed-order sum;
+//   * half lists (velocity-Verlet loops, k_pair_half): each pair once, the
+//     partner's force added as int64 fixed point into LDS window slots and
+//     flushed with integer atomics, so the result does not depend on order.
 #include "ttc_common.h"
 
 namespace {
@@ -54,6 +58,7 @@ struct PairAcc {
     float tx, ty, tz;    // this tile's force, added to fx, fy, fz once per 16 entries
     double elj, eljs, er, ers, ei, w0, w1, w2, w3, w4, w5;
     double hx, hy, hz;   // CONFIGT: this site's d2U/dx2, dy2, dz2 (pair terms)
+    float fmax;          // HALF: largest pair-force component (range check of the accumulators)
 };
 
 // One list entry of site i (cnonbond_nblist.c:150-519 for one pair).
@@ -67,12 +72,24 @@ struct RangeSel {
 // site gathered for entry e (site 0 for padding, which is skipped)
 __device__ __forceinline__ int entry_site(uint32_t e) { return e == TTC_PAD ? 0 : (int)(e & TTC_JMASK); }
 
+__device__ __forceinline__ long long fix24(float f) {
+    // f 2^24 rounded to the nearest integer: the FMA with 1.5 2^52 leaves the
+    // integer in the low mantissa bits for |f 2^24| < 2^51; the magic's low
+    // 32 bits are zero, so its removal is one 32-bit subtract on the high word
+    const double t = __builtin_fma((double)f, 16777216.0, 6755399441055744.0);
+    return __double_as_longlong(t) - 0x4338000000000000LL;
+}
+
 // pj: position of the entry's site, gathered by the caller (ahead of time,
-// so the gathers of later entries overlap this one); its type is in the entry
-template <bool ENERGY, bool RANGE = false, bool HES = false>
+// so the gathers of later entries overlap this one); its type is in the entry.
+// HALF: the list holds each pair once and the entry holds j's window slot;
+// the force on j (minus the force on i) is added to that slot in LDS as
+// int64 in 2^-24 kJ/mol/A (jacc = x, y, z accumulator arrays of wcap slots).
+template <bool ENERGY, bool RANGE = false, bool HES = false, bool HALF = false>
 __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const double4 &pi, float qi,
                                            const float2 *ljrow, const float2 *ljrow14, const PairConst &pc, PairAcc &a,
-                                           const RangeSel &rs = RangeSel{}, bool i_in = true) {
+                                           const RangeSel &rs = RangeSel{}, bool i_in = true,
+                                           unsigned long long *jacc = nullptr, int wcap = 0) {
     if (e == TTC_PAD) return;
     int j = (int)(e & TTC_JMASK);
     int tj = (int)((e >> TTC_TSHIFT) & 63u);
@@ -146,6 +163,13 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const 
     a.tx += fxf;
     a.ty += fyf;
     a.tz += fzf;
+    if (HALF) {
+        const int slot = (int)(e & TTC_JMASK);
+        atomicAdd(&jacc[slot], (unsigned long long)fix24(-fxf));
+        atomicAdd(&jacc[wcap + slot], (unsigned long long)fix24(-fyf));
+        atomicAdd(&jacc[2 * wcap + slot], (unsigned long long)fix24(-fzf));
+        a.fmax = fmaxf(a.fmax, fmaxf(fabsf(fxf), fmaxf(fabsf(fyf), fabsf(fzf))));
+    }
     if (ENERGY) {
         float sr2s = sg2 / pc.rc2f;
         float sr6s = sr2s * sr2s * sr2s;
@@ -302,6 +326,159 @@ __global__ void __launch_bounds__(64 * WAVES) kb_pair(const PairPack *pk, const 
                                            a.partials, a.maxb, a.rs);
 }
 
+// Half list: one workgroup per ELL group of 64 sites as above; entries
+// hold window slots.  The group's window (the distinct partners j, win[g])
+// is staged in LDS together with three int64 accumulators per slot; each
+// pair's force is added to i in registers and, negated, to j's slot.  After
+// the row loop the slots are added to fixj[3][n] with one global 64-bit
+// atomic per slot and component (2^-24 kJ/mol/A units: integer adds, so the
+// total does not depend on the order).  Energies and the virial count each
+// pair once.
+struct PairHalfPack {
+    int n, cap, wcap;
+    const uint32_t *nbl;
+    const int *glen;
+    const int *win, *wlen;
+    const double4 *pos4;
+    const int *type;
+    const float4 *ljtab;
+    PairConst pc;
+    double4 *fpair;
+    long long *fixj;
+    double *partials;
+    int maxb;
+    int *status;
+    int step;   // MD step: count the step, clear the consumed trigger flag, stop after an overflow
+};
+
+template <bool ENERGY, int WAVES>
+__device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
+    extern __shared__ unsigned long long hsm[];
+    const int wcap = a.wcap;
+    unsigned long long *jacc = hsm;                       // [3][wcap]
+    float2 *slj = (float2 *)(hsm + 3 * (size_t)wcap);     // LJ halves
+    const int nt2 = a.pc.ntypes * a.pc.ntypes;
+    int *swin = (int *)(slj + 2 * nt2);                   // [wcap]
+    __shared__ double sred[WAVES][64][3];
+    __shared__ double ered[WAVES][11];
+    __shared__ float sfmax[WAVES];
+    const int W = min(a.wlen[grp], wcap);
+    const int *gwin = a.win + (size_t)grp * wcap;
+    // slots past the window (only after an overflow, whose step is redone)
+    // read site 0 and are not flushed
+    for (int k = threadIdx.x; k < wcap; k += blockDim.x) {
+        swin[k] = k < W ? gwin[k] : 0;
+        jacc[k] = 0ull;
+        jacc[wcap + k] = 0ull;
+        jacc[2 * wcap + k] = 0ull;
+    }
+    for (int k = threadIdx.x; k < nt2; k += blockDim.x) {
+        const float4 v = a.ljtab[k];
+        slj[k] = make_float2(v.x, v.y);
+        slj[nt2 + k] = make_float2(v.z, v.w);
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = a.n;
+    const int s = grp * 64 + lane;
+    PairAcc acc = {};
+    const int len = a.glen[grp];
+    const double4 pi = s < n ? a.pos4[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const int ti = s < n ? a.type[s] : 0;
+    const float2 *ljrow = slj + ti * a.pc.ntypes, *ljrow14 = slj + nt2 + ti * a.pc.ntypes;
+    const float qi = (float)pi.w * a.pc.qfac;
+    const uint4 *seg = (const uint4 *)(a.nbl + (size_t)grp * a.cap * 64 + (size_t)lane * 16);
+    const int ntile = (len + 15) >> 4;
+    int t = wave;
+    uint4 q0, q1, q2, q3;
+    if (t < ntile) {
+        const uint4 *p = seg + (size_t)t * 256;
+        q0 = p[0]; q1 = p[1]; q2 = p[2]; q3 = p[3];
+    }
+    for (; t < ntile; t += WAVES) {
+        const uint32_t e[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+        // partner sites of the tile from the window (padding: site 0)
+        int js[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) js[q] = e[q] == TTC_PAD ? 0 : swin[e[q] & TTC_JMASK];
+        double4 pj[16];
+#pragma unroll
+        for (int q = 0; q < PAIR_AHEAD; q++) pj[q] = a.pos4[js[q]];
+        if (t + WAVES < ntile) {
+            const uint4 *p = seg + (size_t)(t + WAVES) * 256;
+            q0 = p[0]; q1 = p[1]; q2 = p[2]; q3 = p[3];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            if (q + PAIR_AHEAD < 16) pj[q + PAIR_AHEAD] = a.pos4[js[q + PAIR_AHEAD]];
+            pair_entry<ENERGY, false, false, true>(e[q], pj[q], pi, qi, ljrow, ljrow14, a.pc, acc, RangeSel{}, true,
+                                                   jacc, wcap);
+        }
+        acc.fx += (double)acc.tx;
+        acc.fy += (double)acc.ty;
+        acc.fz += (double)acc.tz;
+        acc.tx = acc.ty = acc.tz = 0.0f;
+    }
+
+    sred[wave][lane][0] = acc.fx;
+    sred[wave][lane][1] = acc.fy;
+    sred[wave][lane][2] = acc.fz;
+    float fm = acc.fmax;
+    for (int o = 32; o > 0; o >>= 1) fm = fmaxf(fm, __shfl_xor(fm, o, 64));
+    if (lane == 0) sfmax[wave] = fm;
+    if (ENERGY) {
+        double v[11] = {acc.elj, acc.eljs, acc.er, acc.ers, acc.ei, acc.w0, acc.w1, acc.w2, acc.w3, acc.w4, acc.w5};
+        for (int q = 0; q < 11; q++) {
+            double tq = ttc_wave_sum(v[q]);
+            if (lane == 0) ered[wave][q] = tq;
+        }
+    }
+    __syncthreads();
+    // partners' forces to the global accumulators
+    long long *fixj = a.fixj;
+    for (int k = threadIdx.x; k < W; k += blockDim.x) {
+        const int j = swin[k];
+        const long long vx = (long long)jacc[k], vy = (long long)jacc[wcap + k], vz = (long long)jacc[2 * wcap + k];
+        if (vx) atomicAdd((unsigned long long *)&fixj[j], (unsigned long long)vx);
+        if (vy) atomicAdd((unsigned long long *)&fixj[n + j], (unsigned long long)vy);
+        if (vz) atomicAdd((unsigned long long *)&fixj[2 * n + j], (unsigned long long)vz);
+    }
+    if (wave == 0 && s < n) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int w = 0; w < WAVES; w++) {
+            fx += sred[w][lane][0];
+            fy += sred[w][lane][1];
+            fz += sred[w][lane][2];
+        }
+        a.fpair[s] = make_double4(fx, fy, fz, 0.0);
+    }
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int w = 0; w < WAVES; w++) m = fmaxf(m, sfmax[w]);
+        // beyond 2^26 kJ/mol/A a component would leave the 2^51 range of fix24
+        if (!(m < 67108864.0f)) atomicOr(&a.status[ST_OVERFLOW], 16);
+    }
+    if (ENERGY && threadIdx.x < 11) {
+        double tq = 0.0;
+        for (int w = 0; w < WAVES; w++) tq += ered[w][threadIdx.x];
+        a.partials[(size_t)(EV_LJ + threadIdx.x) * a.maxb + grp] = tq;
+    }
+}
+
+template <bool ENERGY, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_pair_half(PairHalfPack a) {
+    if (a.step) {
+        if (a.status[ST_OVERFLOW] & TTC_OVF_REDO) return;   // a rebuild of this loop overflowed
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.status[ST_STEP] += 1;
+            a.status[ST_NLFLAG] = 0;
+        }
+    }
+    pair_half_body<ENERGY, WAVES>(blockIdx.x, a);
+}
+
 }  // namespace
 
 extern double ttc_nr_erfc_host(double x);
@@ -356,13 +533,42 @@ void ttc_launch_pair_hes(ttc_ctx *c) {
     launch_pair_w<true, false, true>(c, RangeSel{});
 }
 
+size_t ttc_pair_half_shm(const ttc_ctx *c) {
+    return (sizeof(unsigned long long) * 3 + sizeof(int)) * (size_t)c->wcap +
+           sizeof(float2) * 2 * (size_t)c->ntypes * c->ntypes;
+}
+
+static void launch_pair_half(ttc_ctx *c, bool energy, bool step) {
+    PairHalfPack a{c->n, c->cap, c->wcap, c->nbl, c->glen, c->win, c->wlen, c->pos4, c->type, c->ljtab,
+                   pair_const(c), c->fpair, c->fixj, c->partials, c->nblk_max, c->status, step ? 1 : 0};
+    size_t shm = ttc_pair_half_shm(c);
+#define PH_LAUNCH(E, W) k_pair_half<E, W><<<c->G, 64 * W, shm, c->stream>>>(a)
+    if (c->pair_waves_h == 4) {
+        if (energy) PH_LAUNCH(true, 4); else PH_LAUNCH(false, 4);
+    } else {
+        if (energy) PH_LAUNCH(true, 8); else PH_LAUNCH(false, 8);
+    }
+#undef PH_LAUNCH
+}
+
+// LDS above 64 KiB for the half-list kernel (gfx950: 160 KiB per CU)
+int ttc_pair_half_attrs(ttc_ctx *c) {
+    int shm = (int)ttc_pair_half_shm(c);
+    const void *fns[] = {(const void *)k_pair_half<true, 4>, (const void *)k_pair_half<false, 4>,
+                         (const void *)k_pair_half<true, 8>, (const void *)k_pair_half<false, 8>};
+    for (const void *f : fns) TTC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    return TTC_OK;
+}
+
 void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
-    if (energy) launch_pair_w<true, false>(c, RangeSel{});
+    if (c->list_half) launch_pair_half(c, energy, false);
+    else if (energy) launch_pair_w<true, false>(c, RangeSel{});
     else launch_pair_w<false, false>(c, RangeSel{});
 }
 
 void ttc_launch_pair_step(ttc_ctx *c, bool energy) {
-    if (energy) launch_pair_w<true, false>(c, RangeSel{}, c->status);
+    if (c->list_half) launch_pair_half(c, energy, true);
+    else if (energy) launch_pair_w<true, false>(c, RangeSel{}, c->status);
     else launch_pair_w<false, false>(c, RangeSel{}, c->status);
 }
 

@@ -210,6 +210,7 @@ struct KickIn {
     // is written to snap[orig[s]] when snap is set
     const int *orig = nullptr;
     double4 *snap = nullptr;
+    long long *fixj = nullptr;   // partner forces of a half-list pair kernel
 };
 
 // kick + drift + pbc (integrate_vverlet.c:52-83) of every site of a brick,
@@ -238,6 +239,9 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
     const int chunk = br.chunk;
     float *rw = (float *)(tile + br.rec_off);
     int *rb = (int *)(rw + chunk * 3 * P);
+    // a velocity-Verlet loop whose rebuild overflowed stops here; the host
+    // grows the list and redoes that step from its drifted positions
+    if (DRIFT && ki.snap && (status[ST_OVERFLOW] & TTC_OVF_REDO)) return;
     if (KICK) ttc_mean_pme(ki.partials, ki.maxb, ki.nbg, ki.n, mean);
     int ibx = b % br.nbx, iby = (b / br.nbx) % br.nby, ibz = b / (br.nbx * br.nby);
     int cx0 = ibx * br.bx, cy0 = iby * br.by, cz0 = ibz * br.bz;
@@ -282,7 +286,7 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
                 double im = v.w;   // 100/m
                 const int so = ki.orig ? ki.orig[s] : s;
                 if (KICK) {
-                    f = ttc_combine(s, 7, ki.fpair, ki.g4, mean, so);
+                    f = ttc_combine(s, 7, ki.fpair, ki.g4, mean, so, ki.fixj, ki.n);
                     f4[s] = f;
                     v.x = v.x + ki.vscal * dth * v.w * f.x;
                     v.y = v.y + ki.vscal * dth * v.w * f.y;
@@ -1060,7 +1064,8 @@ static size_t brick_shm(const ttc_ctx *c, const Brick &br) {
 static DsPack ds_pack(ttc_ctx *c, bool drift, double vscal) {
     return DsPack{c->pos4, c->vel4, c->f4, drift ? c->p.dt : 0.0, vscal, spread_args(c), spread_brick(c),
                   c->cell_start, c->cell_cnt, c->qfix, c->status, c->anchor, c->ev + EV_N,
-                  KickIn{c->fpair, c->g4, c->partials, c->nblk_max, ttc_gather_blocks(c), c->n, vscal}};
+                  KickIn{c->fpair, c->g4, c->partials, c->nblk_max, ttc_gather_blocks(c), c->n, vscal, nullptr,
+                         nullptr, c->fixj}};
 }
 
 static int ds_blocks(const ttc_ctx *c) {
@@ -1168,6 +1173,17 @@ void ttc_launch_gather(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes) 
     }
 #undef GATHER
 }
+
+namespace {
+// snap[orig[s]] = pos4[s]: the snapshot an MD step's drift writes, from the
+// current positions
+__global__ void k_snap(int n, const double4 *__restrict__ pos4, const int *__restrict__ orig, double4 *snap) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) snap[orig[s]] = pos4[s];
+}
+}  // namespace
+
+void ttc_launch_snap(ttc_ctx *c) { k_snap<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, c->pos4, c->orig, c->snap); }
 
 void ttc_launch_gather_orig(ttc_ctx *c, bool energy) {
     double lx = c->p.lx, ly = c->p.ly, lz = c->p.lz;

@@ -137,6 +137,20 @@ class Box:
         keys = ["bond", "nbond", "nbonds", "coulomb", "rewald", "rewalds", "kewald", "sewald", "iewald"]
         return ff, dict(zip(keys, en))
 
+    def get_forces(self):
+        """ff[k] of the last evaluation (after a step: integrate_vverlet.c:88)."""
+        ff = np.zeros((self.n, 3))
+        T.check(self.L.ttc_get_forces(self.h, T.dptr(ff), 3))
+        return ff
+
+    def set_capacity_limits(self, row_limit: int = 0, window_limit: int = 0):
+        T.check(self.L.ttc_set_capacity_limits(self.h, row_limit, window_limit))
+
+    def window_stats(self):
+        m = C.c_int(0); cap = C.c_int(0); h = C.c_int(0)
+        T.check(self.L.ttc_window_stats(self.h, C.byref(m), C.byref(cap), C.byref(h)))
+        return m.value, cap.value, bool(h.value)
+
     def kinet(self):
         k = np.zeros(1); ken = np.zeros(6)
         T.check(self.L.ttc_kinet(self.h, T.dptr(k), T.dptr(ken)))
