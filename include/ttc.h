@@ -217,9 +217,11 @@ int  ttc_batch_autotune(ttc_batch *b, int trial_steps, int *best_mode, int *best
 /* Device time of one kernel class, measured with HIP events on the
  * context's stream around 20 back-to-back launches on the current state:
  * which = 0 pair kernel, 1 spread, 3 the five PME FFT passes (influence
- * function included), 4 PME gather + bonds; which = 2 is the average step
- * of the last ttc_md_vverlet call.  Microseconds.  After 3 or 4 the forces
- * are recomputed before the next MD call.                                 */
+ * function included), 4 PME gather + bonds, 5 / 6 the velocity-Verlet
+ * loop's list build (half / full list, at the anchors, 10 launches);
+ * which = 2 is the average step of the last ttc_md_vverlet call.
+ * Microseconds.  After 3 or 4 the forces are recomputed before the next
+ * MD call.                                                                */
 int  ttc_timing(ttc_ctx *c, int which, double *usec);
 
 /* Autotuner of the launch shape: times the pair kernel at 4, 8 and 16

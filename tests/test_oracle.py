@@ -765,3 +765,17 @@ def test_bond_hessian_is_energy_second_derivative():
     fd = _fd_laplacian(lambda q: o.bonds(q)[0], p, 1e-4).sum(0)
     assert np.allclose(h3, fd, rtol=1e-6, atol=1e-4)
     assert abs(e - o.bonds(p)[0]) == 0.0
+
+
+def test_c0_nve_fixture_is_the_oracle_run():
+    """tests/golden/c0_nve_oracle.npz (written by tests/golden/make_c0_nve.py)
+    is the oracle's own run: its first 15 rows are reproduced exactly."""
+    import importlib.util
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    spec = importlib.util.spec_from_file_location("make_c0_nve", os.path.join(here, "make_c0_nve.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    rows = np.load(os.path.join(here, "c0_nve_oracle.npz"))["rows"]
+    assert rows.shape == (1000, 5)
+    assert np.array_equal(mk.run(15), rows[:15])

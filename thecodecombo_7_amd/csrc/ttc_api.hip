@@ -950,9 +950,28 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
 extern "C" int ttc_timing(ttc_ctx *c, int which, double *usec) {
     CHECK(c && usec, TTC_ERR_ARG, "ttc_timing: bad argument");
     if (which == 2) { *usec = c->t_step; return TTC_OK; }
-    CHECK(which >= 0 && which <= 4, TTC_ERR_ARG, "ttc_timing: which must be 0 .. 4");
+    CHECK(which >= 0 && which <= 6, TTC_ERR_ARG, "ttc_timing: which must be 0 .. 6");
     int r = need_list(c, true);   // the pair kernel of the current list format
     if (r) return r;
+    if (which >= 5) {
+        // the MD loop's list build at the anchors, half (5) or full (6)
+        // format; afterwards the list is rebuilt in its own format
+        const bool had_half = c->list_half;
+        c->list_half = !(which == 5);   // ensure_format builds when the format differs
+        r = ensure_format(c, which == 5);
+        if (r) return r;
+        EventPair ev;
+        TTC_HIP(ev.create());
+        TTC_HIP(hipEventRecord(ev.e0, c->stream));
+        for (int i = 0; i < 10; i++) ttc_launch_build_md_at_anchors(c, which == 5);
+        TTC_HIP(hipEventRecord(ev.e1, c->stream));
+        TTC_HIP(hipEventSynchronize(ev.e1));
+        float ms = 0.0f;
+        TTC_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
+        *usec = 1000.0 * ms / 10;
+        c->list_half = which == 5;
+        return ensure_format(c, had_half);
+    }
     const int reps = 20;
     EventPair ev;
     TTC_HIP(ev.create());

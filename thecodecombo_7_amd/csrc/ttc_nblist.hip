@@ -386,6 +386,7 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__
 #define GB_MI (1 << 13)
 #define GB_QCAP 128   // candidate ring per wave (two batches of 64)
 #define GB_EXMAX 8    // exclusion partners per site held in LDS (more: read from the CSR)
+#define GB_ZMARGIN 1e-3f   // A: far above FP32 rounding of frame coordinates
 
 struct GbGeom {
     int ncx, ncy, ncz;
@@ -562,7 +563,11 @@ __device__ __forceinline__ void build_grp_body(
         const float ymn = wave_minf(uy), ymx = wave_maxf(uy);
         const float zmn = wave_minf(uz), zmx = wave_maxf(uz);
         const AxisRange ay = axis_range(ymn, ymx, g.cutr, g.hy, g.cwy, g.ncy);
-        const AxisRange az = axis_range(zmn, zmx, g.cutr, g.hz, g.cwz, g.ncz);
+        // half lists: a partner lies above some row (minimum-image dz >= 0),
+        // so nothing below the rows' lowest z (less a rounding margin) is a
+        // candidate; that removes about half of the candidate tests
+        const float zlo = zmn - GB_ZMARGIN;
+        const AxisRange az = axis_range(hw.half ? zlo + g.cutr : zmn, zmx, g.cutr, g.hz, g.cwz, g.ncz);
         // whole-ring ranges (small boxes): candidates keep their own image
         // and every pair takes the minimum image; no box filter then
         const bool wmi = ay.mi || az.mi || axis_range(xmn, xmx, g.cutr, g.hx, g.cwx, g.ncx).mi;
@@ -689,7 +694,8 @@ __device__ __forceinline__ void build_grp_body(
                         const float gx = fmaxf(fabsf(px - bcx) - bhx, 0.0f);
                         const float gy = fmaxf(fabsf(py - bcy) - bhy, 0.0f);
                         const float gz = fmaxf(fabsf(pz - bcz) - bhz, 0.0f);
-                        const bool keep = ok & (wmi | (gx * gx + gy * gy + gz * gz < cutr2));
+                        const bool keep = ok & (wmi | ((gx * gx + gy * gy + gz * gz < cutr2) &
+                                                       (!hw.half | (pz >= zlo))));
                         const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
                         if (keep) {
                             const int at = (qh + qn + (int)__builtin_amdgcn_mbcnt_hi(
@@ -770,14 +776,25 @@ __device__ __forceinline__ void build_grp_body(
     }
     __syncthreads();
     // entries: j -> slot (a slot past the capacity is clamped; the overflow
-    // flag makes the host grow the window and build again)
+    // flag makes the host grow the window and build again).  The wave's
+    // rows are read 64 entries at a time each, all GB_ROWS loads in flight
+    // before any is used.
+    int lens[GB_ROWS];
+    int lmax = 0;
+#pragma unroll
     for (int rr = 0; rr < GB_ROWS; rr++) {
-        const int r = row0 + rr;
-        if (sb + r >= n) break;
-        const int len = min(srow[r], cap);
-        for (int k = lane; k < len; k += 64) {
-            const uint32_t idx = (uint32_t)(((k >> 4) << 10) + r * 16 + (k & 15));
-            const uint32_t e = grow[idx];
+        lens[rr] = sb + row0 + rr < n ? min(srow[row0 + rr], cap) : 0;
+        lmax = max(lmax, lens[rr]);
+    }
+    for (int k = lane; k - lane < lmax; k += 64) {
+        uint32_t ev[GB_ROWS];
+#pragma unroll
+        for (int rr = 0; rr < GB_ROWS; rr++)
+            ev[rr] = k < lens[rr] ? grow[(uint32_t)(((k >> 4) << 10) + (row0 + rr) * 16 + (k & 15))] : 0u;
+#pragma unroll
+        for (int rr = 0; rr < GB_ROWS; rr++) {
+            if (k >= lens[rr]) continue;
+            const uint32_t e = ev[rr];
             int djs = (int)(e & TTC_JMASK) - sb;
             if (2 * djs > n) djs -= n;
             else if (2 * djs <= -n) djs += n;
@@ -788,7 +805,7 @@ __device__ __forceinline__ void build_grp_body(
                 slot = (int)hpre[jrel >> 6] + __popcll(wbits & ((1ull << (jrel & 63)) - 1ull));
             }
             slot = min(slot, hw.wcap - 1);
-            grow[idx] = (e & ~TTC_JMASK) | (uint32_t)slot;
+            grow[(uint32_t)(((k >> 4) << 10) + (row0 + rr) * 16 + (k & 15))] = (e & ~TTC_JMASK) | (uint32_t)slot;
         }
     }
 }

@@ -58,7 +58,7 @@ struct PairAcc {
     float tx, ty, tz;    // this tile's force, added to fx, fy, fz once per 16 entries
     double elj, eljs, er, ers, ei, w0, w1, w2, w3, w4, w5;
     double hx, hy, hz;   // CONFIGT: this site's d2U/dx2, dy2, dz2 (pair terms)
-    float fmax;          // HALF: largest pair-force component (range check of the accumulators)
+    float fmax;          // half lists: largest pair-force component (range check of the accumulators)
 };
 
 // One list entry of site i (cnonbond_nblist.c:150-519 for one pair).
@@ -72,24 +72,12 @@ struct RangeSel {
 // site gathered for entry e (site 0 for padding, which is skipped)
 __device__ __forceinline__ int entry_site(uint32_t e) { return e == TTC_PAD ? 0 : (int)(e & TTC_JMASK); }
 
-__device__ __forceinline__ long long fix24(float f) {
-    // f 2^24 rounded to the nearest integer: the FMA with 1.5 2^52 leaves the
-    // integer in the low mantissa bits for |f 2^24| < 2^51; the magic's low
-    // 32 bits are zero, so its removal is one 32-bit subtract on the high word
-    const double t = __builtin_fma((double)f, 16777216.0, 6755399441055744.0);
-    return __double_as_longlong(t) - 0x4338000000000000LL;
-}
-
 // pj: position of the entry's site, gathered by the caller (ahead of time,
-// so the gathers of later entries overlap this one); its type is in the entry.
-// HALF: the list holds each pair once and the entry holds j's window slot;
-// the force on j (minus the force on i) is added to that slot in LDS as
-// int64 in 2^-24 kJ/mol/A (jacc = x, y, z accumulator arrays of wcap slots).
-template <bool ENERGY, bool RANGE = false, bool HES = false, bool HALF = false>
+// so the gathers of later entries overlap this one); its type is in the entry
+template <bool ENERGY, bool RANGE = false, bool HES = false>
 __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const double4 &pi, float qi,
                                            const float2 *ljrow, const float2 *ljrow14, const PairConst &pc, PairAcc &a,
-                                           const RangeSel &rs = RangeSel{}, bool i_in = true,
-                                           unsigned long long *jacc = nullptr, int wcap = 0) {
+                                           const RangeSel &rs = RangeSel{}, bool i_in = true) {
     if (e == TTC_PAD) return;
     int j = (int)(e & TTC_JMASK);
     int tj = (int)((e >> TTC_TSHIFT) & 63u);
@@ -163,13 +151,6 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const 
     a.tx += fxf;
     a.ty += fyf;
     a.tz += fzf;
-    if (HALF) {
-        const int slot = (int)(e & TTC_JMASK);
-        atomicAdd(&jacc[slot], (unsigned long long)fix24(-fxf));
-        atomicAdd(&jacc[wcap + slot], (unsigned long long)fix24(-fyf));
-        atomicAdd(&jacc[2 * wcap + slot], (unsigned long long)fix24(-fzf));
-        a.fmax = fmaxf(a.fmax, fmaxf(fabsf(fxf), fmaxf(fabsf(fyf), fabsf(fzf))));
-    }
     if (ENERGY) {
         float sr2s = sg2 / pc.rc2f;
         float sr6s = sr2s * sr2s * sr2s;
@@ -178,6 +159,80 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const 
         a.er += (double)(qq * erfckr * rinv);
         a.ers += (double)qq * pc.erfc_rc_over_rc;
         if (cls) a.ei += (double)(qi2 * (1.0f - erfckr) * rinv);
+        a.w0 += (double)(fxf * dx);
+        a.w1 += (double)(fyf * dx);
+        a.w2 += (double)(fyf * dy);
+        a.w3 += (double)(fzf * dx);
+        a.w4 += (double)(fzf * dy);
+        a.w5 += (double)(fzf * dz);
+    }
+}
+
+// One entry of a half list, without branches around the pair terms: lanes
+// whose entry is padding or beyond rc compute on a harmless r^2 and their
+// force is selected to zero, so the wave runs one straight sequence per
+// entry (the exec-mask branches and the zeroed values around them cost more
+// than the arithmetic they skip).  The partner's force goes to its window
+// slot as int64 2^-24 units: -f 2^24 is exact in FP32, its FP64 image plus
+// 1.5 2^52 (from an SGPR) rounds it to an integer held in the low mantissa
+// bits, and removing the magic's high word leaves the two's-complement value.
+template <bool ENERGY>
+__device__ __forceinline__ void pair_entry_half(uint32_t e, const double4 &pj, const double4 &pi, float qi,
+                                                const float2 *ljrow, const float2 *ljrow14, const PairConst &pc,
+                                                PairAcc &a, unsigned long long *jacc, int wcap, double magic) {
+    const int tj = (int)((e >> TTC_TSHIFT) & 63u);
+    const uint32_t cls = e >> 30;
+    double ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+    ex = fma(-pc.lx, rint(ex * pc.ilx), ex);
+    ey = fma(-pc.ly, rint(ey * pc.ily), ey);
+    ez = fma(-pc.lz, rint(ez * pc.ilz), ez);
+    const double dr2 = ttc_r2_exact(ex, ey, ez);
+    const bool in = (e != TTC_PAD) & (dr2 <= pc.rc2);
+    const float dx = (float)ex, dy = (float)ey, dz = (float)ez;
+    const float r2 = in ? (float)dr2 : pc.rc2f;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float rinv2 = rinv * rinv;
+    const float r = r2 * rinv;
+    const float2 lj = (cls == 2 ? ljrow14 : ljrow)[tj];
+    const float eps = cls == 1 ? 0.0f : lj.x;
+    const float sr2 = lj.y * rinv2;
+    const float sr6 = sr2 * sr2 * sr2;
+    const float sr12 = sr6 * sr6;
+    float force = 12.0f * eps * rinv2 * (sr12 - sr6);
+    const float qqab = qi * (float)pj.w;
+    const float qq = (cls == 0 ? qqab : (cls == 2 ? pc.e14 * qqab : 0.0f)) * pc.inv_epsrf;
+    float ez2;
+    const float erfckr = nr_erfc(r, r2, pc, &ez2);
+    const float kpid = pc.kpi2 * r * ez2;
+    const float rinv3 = rinv * rinv2;
+    force += qq * rinv3 * (erfckr + kpid);
+    const float qi2 = cls == 0 ? 0.0f : (cls == 2 ? pc.one_m_e14 * qqab : qqab);
+    force -= qi2 * rinv3 * ((1.0f - erfckr) - kpid);
+    force = in ? force : 0.0f;
+    const float fxf = force * dx, fyf = force * dy, fzf = force * dz;
+    a.tx += fxf;
+    a.ty += fyf;
+    a.tz += fzf;
+    a.fmax = fmaxf(a.fmax, fmaxf(fabsf(fxf), fmaxf(fabsf(fyf), fabsf(fzf))));
+    if (in) {
+        const int slot = (int)(e & TTC_JMASK);
+        auto fix = [&](float f) {
+            const double t = (double)(f * -16777216.0f) + magic;
+            return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000LL);
+        };
+        unsigned long long *acc3 = jacc + 3 * slot;
+        atomicAdd(&acc3[0], fix(fxf));
+        atomicAdd(&acc3[1], fix(fyf));
+        atomicAdd(&acc3[2], fix(fzf));
+    }
+    if (ENERGY && in) {
+        const float sr2s = lj.y / pc.rc2f;
+        const float sr6s = sr2s * sr2s * sr2s;
+        a.elj += (double)(eps * (sr12 - 2.0f * sr6));
+        a.eljs += (double)(eps * (sr6s * sr6s - 2.0f * sr6s));
+        a.er += (double)(qq * erfckr * rinv);
+        a.ers += (double)qq * pc.erfc_rc_over_rc;
+        a.ei += (double)(qi2 * (1.0f - erfckr) * rinv);
         a.w0 += (double)(fxf * dx);
         a.w1 += (double)(fyf * dx);
         a.w2 += (double)(fyf * dy);
@@ -349,13 +404,14 @@ struct PairHalfPack {
     int maxb;
     int *status;
     int step;   // MD step: count the step, clear the consumed trigger flag, stop after an overflow
+    double magic;   // 1.5 2^52 (rounding constant of the partner forces)
 };
 
 template <bool ENERGY, int WAVES>
 __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
     extern __shared__ unsigned long long hsm[];
     const int wcap = a.wcap;
-    unsigned long long *jacc = hsm;                       // [3][wcap]
+    unsigned long long *jacc = hsm;                       // [wcap][3]
     float2 *slj = (float2 *)(hsm + 3 * (size_t)wcap);     // LJ halves
     const int nt2 = a.pc.ntypes * a.pc.ntypes;
     int *swin = (int *)(slj + 2 * nt2);                   // [wcap]
@@ -368,9 +424,9 @@ __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
     // read site 0 and are not flushed
     for (int k = threadIdx.x; k < wcap; k += blockDim.x) {
         swin[k] = k < W ? gwin[k] : 0;
-        jacc[k] = 0ull;
-        jacc[wcap + k] = 0ull;
-        jacc[2 * wcap + k] = 0ull;
+        jacc[3 * k] = 0ull;
+        jacc[3 * k + 1] = 0ull;
+        jacc[3 * k + 2] = 0ull;
     }
     for (int k = threadIdx.x; k < nt2; k += blockDim.x) {
         const float4 v = a.ljtab[k];
@@ -413,8 +469,7 @@ __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
 #pragma unroll
         for (int q = 0; q < 16; q++) {
             if (q + PAIR_AHEAD < 16) pj[q + PAIR_AHEAD] = a.pos4[js[q + PAIR_AHEAD]];
-            pair_entry<ENERGY, false, false, true>(e[q], pj[q], pi, qi, ljrow, ljrow14, a.pc, acc, RangeSel{}, true,
-                                                   jacc, wcap);
+            pair_entry_half<ENERGY>(e[q], pj[q], pi, qi, ljrow, ljrow14, a.pc, acc, jacc, wcap, a.magic);
         }
         acc.fx += (double)acc.tx;
         acc.fy += (double)acc.ty;
@@ -440,7 +495,7 @@ __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
     long long *fixj = a.fixj;
     for (int k = threadIdx.x; k < W; k += blockDim.x) {
         const int j = swin[k];
-        const long long vx = (long long)jacc[k], vy = (long long)jacc[wcap + k], vz = (long long)jacc[2 * wcap + k];
+        const long long vx = (long long)jacc[3 * k], vy = (long long)jacc[3 * k + 1], vz = (long long)jacc[3 * k + 2];
         if (vx) atomicAdd((unsigned long long *)&fixj[j], (unsigned long long)vx);
         if (vy) atomicAdd((unsigned long long *)&fixj[n + j], (unsigned long long)vy);
         if (vz) atomicAdd((unsigned long long *)&fixj[2 * n + j], (unsigned long long)vz);
@@ -457,7 +512,8 @@ __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
     if (threadIdx.x == 0) {
         float m = 0.0f;
         for (int w = 0; w < WAVES; w++) m = fmaxf(m, sfmax[w]);
-        // beyond 2^26 kJ/mol/A a component would leave the 2^51 range of fix24
+        // beyond 2^26 kJ/mol/A a component would leave the 2^51 range of the
+        // partner accumulators' rounding
         if (!(m < 67108864.0f)) atomicOr(&a.status[ST_OVERFLOW], 16);
     }
     if (ENERGY && threadIdx.x < 11) {
@@ -540,7 +596,8 @@ size_t ttc_pair_half_shm(const ttc_ctx *c) {
 
 static void launch_pair_half(ttc_ctx *c, bool energy, bool step) {
     PairHalfPack a{c->n, c->cap, c->wcap, c->nbl, c->glen, c->win, c->wlen, c->pos4, c->type, c->ljtab,
-                   pair_const(c), c->fpair, c->fixj, c->partials, c->nblk_max, c->status, step ? 1 : 0};
+                   pair_const(c), c->fpair, c->fixj, c->partials, c->nblk_max, c->status, step ? 1 : 0,
+                   6755399441055744.0};
     size_t shm = ttc_pair_half_shm(c);
 #define PH_LAUNCH(E, W) k_pair_half<E, W><<<c->G, 64 * W, shm, c->stream>>>(a)
     if (c->pair_waves_h == 4) {
