@@ -9,7 +9,8 @@
  *   range energy of a moved part   forces_mc.c:90-300 (o_potens / n_potens),
  *                                  enbnd_mc.c:26-504 (pairs touching [lb,ub))
  *   Go energy terms                cgolik.c:644-948, 12-10 form 1202-1229
- *   Wang-Landau acceptance         pivot.c:236-253, ctdos_pivot.c:246-314
+ *   Wang-Landau acceptance         pivot.c:236-253, ctdos_pivot.c:246-314,
+ *                                  ln g interpolation dos_interpolate.c:107-124
  *   random numbers                 random.c:37-75 (ran2; here a per-walker
  *                                  counter-free generator, see ttc_mc_run)
  * The learned first stage and its training have no reference counterpart.
@@ -93,8 +94,23 @@ int  ttc_mc_delta_energy(ttc_mc *m, int w, int lb, int ub, const double *newpos,
 #define TTC_MC_NOALLOC_MAXBEADS 1024
 int  ttc_mc_delta_energy_noalloc(ttc_mc *m, int w, int lb, int ub, const double *newpos, double *de);
 
+/* Random streams of walker w: 64 xoshiro256** states of 4 words each,
+ * state[4 s .. 4 s + 3] for stream s (ttc_mc_create seeds all walkers'
+ * streams, walker-major, with consecutive splitmix64 outputs of seed).  The
+ * counterpart of the ran2 record of the reference's restart files
+ * (checkpoint.c:304-307).  Trial t of a ttc_mc_run call takes the 8 numbers
+ * stream t mod 64 draws for batch t / 64 (a batch draws 8 from every stream,
+ * at the start of the call and after every 64th trial but the last):
+ * u0 < pivot_frac selects a pivot (bead u1 n, axis u2 3, side u3 2,
+ * clockwise unless u4 2 >= 1, angle u5 pi theta_max), else a displacement
+ * (bead u1 n by (u2 - 0.5, u3 - 0.5, u4 - 0.5) disp_max); u6 decides the
+ * first stage, u7 the acceptance.  An all-zero state is TTC_ERR_ARG.     */
+int  ttc_mc_get_streams(ttc_mc *m, int w, uint64_t *state);
+int  ttc_mc_set_streams(ttc_mc *m, int w, const uint64_t *state);
+
 /* ntrials Monte Carlo trials per walker.  mode 0: plain Wang-Landau
- * acceptance on the exact energy; mode 1: delayed acceptance, a surrogate
+ * acceptance on the exact energy (ln g interpolated at the old and new
+ * energies as dos_interpolate.c:107-124 does); mode 1: delayed acceptance, a surrogate
  * energy change from the network screens each proposal and the exact
  * energy makes the second-stage decision, so the chain keeps the same
  * stationary distribution.  lnf is added to ln g of the visited bin after

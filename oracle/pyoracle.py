@@ -44,6 +44,11 @@ class Ran2State(C.Structure):
     _fields_ = [("idum", C.c_long), ("idum2", C.c_long), ("iy", C.c_long), ("iv", C.c_long * 32)]
 
 
+class WlParams(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("kbond", "r0bond", "eps", "sig_rep", "pivot_frac", "theta_max",
+                                          "disp_max", "emin", "ebin")] + [("nbins", C.c_int)]
+
+
 def build(fast: bool = False) -> None:
     subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
 
@@ -118,6 +123,15 @@ def _declare(L):
     L.orc_cbond2.restype = C.c_double
     L.orc_cbond2.argtypes = [P(OrcSystem), _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
     L.orc_cnonbondnl2.argtypes = [P(OrcSystem), _dp, _ip, _ip, _ip, C.c_int, C.c_double, _dp, _dp, _dp]
+    L.orc_splitmix64.restype = C.c_uint64; L.orc_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
+    L.orc_xoshiro_next.restype = C.c_uint64; L.orc_xoshiro_next.argtypes = [C.POINTER(C.c_uint64)]
+    L.orc_xoshiro_uniform.restype = C.c_double; L.orc_xoshiro_uniform.argtypes = [C.POINTER(C.c_uint64)]
+    L.orc_dos_interp.restype = C.c_double
+    L.orc_dos_interp.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_int, C.c_double]
+    L.orc_wl_run.restype = C.c_int
+    L.orc_wl_run.argtypes = [C.POINTER(WlParams), C.c_int, _dp, C.POINTER(C.c_double), _u8, _dp, _dp,
+                             C.POINTER(C.c_longlong),
+                             C.POINTER(C.c_uint64), C.c_int, C.c_double, C.POINTER(C.c_longlong)]
     L.orc_mlp_nparams.restype = C.c_int; L.orc_mlp_nparams.argtypes = [C.c_int, C.c_int]
     L.orc_mlp_forward.restype = C.c_double
     L.orc_mlp_forward.argtypes = [C.c_int, C.c_int, _dp, _dp]
@@ -295,3 +309,62 @@ def ref_lib():
     L.ref_kinet.restype = C.c_double
     L.ref_kinet.argtypes = [_dp, _dp, _dp, _dp]
     return L
+
+
+REF_DOS_SO = os.path.join(HERE, "_ref", "libttcref_dos.so")
+
+
+def ref_dos_lib():
+    """The reference's own dos_interp (dos_interpolate.c, DOS build)."""
+    L = C.CDLL(REF_DOS_SO)
+    L.ref_dos_interp.restype = C.c_double
+    L.ref_dos_interp.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_int, C.c_double]
+    return L
+
+
+def _u64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def splitmix_seq(seed: int, count: int) -> np.ndarray:
+    """count consecutive splitmix64 outputs from state seed."""
+    L = lib()
+    x = C.c_uint64(seed)
+    return np.array([L.orc_splitmix64(C.byref(x)) for _ in range(count)], dtype=np.uint64)
+
+
+def xoshiro_advance(state: np.ndarray, draws: int) -> np.ndarray:
+    """The (..., 4) xoshiro256** states after `draws` outputs each."""
+    L = lib()
+    s = np.ascontiguousarray(state, dtype=np.uint64).reshape(-1, 4).copy()
+    for row in s:
+        for _ in range(draws):
+            L.orc_xoshiro_next(_u64(row))
+    return s.reshape(state.shape)
+
+
+def go_tables(n, contacts, r0):
+    """n x n native mask and native distances of a Go chain."""
+    nat = np.zeros((n, n), np.uint8)
+    r0m = np.zeros((n, n))
+    for (a, b), r in zip(contacts, r0):
+        nat[a, b] = nat[b, a] = 1
+        r0m[a, b] = r0m[b, a] = r
+    return nat, r0m
+
+
+def wl_run(params: WlParams, pos, ecur, nat, r0m, lng, hist, streams, ntrials, lnf, stats):
+    """orc_wl_run on numpy arrays, all updated in place; returns the new ecur."""
+    L = lib()
+    e = C.c_double(ecur)
+    for a, dt in ((pos, np.float64), (nat, np.uint8), (r0m, np.float64), (lng, np.float64), (hist, np.int64),
+                  (streams, np.uint64), (stats, np.int64)):
+        assert a.dtype == dt and a.flags.c_contiguous
+    assert lng.size == params.nbins and hist.size == params.nbins and streams.size == 256 and stats.size == 7
+    n = pos.size // 3
+    assert nat.size == n * n and r0m.size == n * n
+    rc = L.orc_wl_run(C.byref(params), n, pos, C.byref(e), nat, r0m, lng,
+                      hist.ctypes.data_as(C.POINTER(C.c_longlong)), _u64(streams), ntrials, lnf,
+                      stats.ctypes.data_as(C.POINTER(C.c_longlong)))
+    assert rc == 0
+    return e.value

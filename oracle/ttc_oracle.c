@@ -28,16 +28,29 @@
  *                     (oracle/_ref) and to a direct Ewald sum, by finite
  *                     differences of the energy, and zero net force.
  *   orc_cbond         cbond.c:65-120.  Pinned by finite differences.
- *   orc_vverlet_*     integrate_vverlet.c:36-103.  Pinned by time
- *                     reversibility and NVE energy conservation.
+ *   orc_vverlet_*     integrate_vverlet.c:36-103.  Pinned bit-exactly by the
+ *   orc_pbc_all,      reference's integrate_vverlet.c, pbc_all.c and kinet.c
+ *   orc_kinet         compiled into oracle/_ref; the 100/m factor by the
+ *                     harmonic oscillator's closed form; time reversibility.
  *   orc_nhc           integrate_nhc.c:40-161.  Pinned by conservation of
  *                     the extended Nose-Hoover energy.
- *   orc_kinet         kinet.c:18-68.  Pinned by 0.5*m*v^2 closed form.
  *   orc_go_*          cgolik.c:644-948 / 1202-1229 (12-10 form) as the
  *                     Go-model configuration specifies.  Pinned by E=-eps
  *                     and dE/dr=0 at r=r0 and by range energy identities.
  *   orc_ran2          random.c:37-75 (NR ran2).  Pinned by the published
  *                     Numerical Recipes test values of ran2.
+ *   orc_splitmix64,   the device's random streams (no reference
+ *   orc_xoshiro_*     counterpart).  Pinned by the published output of
+ *                     xoshiro256** from the state {1, 2, 3, 4} and of
+ *                     splitmix64 from 0.
+ *   orc_dos_interp    dos_interpolate.c:107-124.  Pinned bit-exactly by the
+ *                     reference's own dos_interp (DOS build) in oracle/_ref.
+ *   orc_wl_run        pivot.c:236-253 acceptance, mcmove_pivot.c:54-215,
+ *                     mcmove_displ.c:45-99 moves.  Pinned by exactly known
+ *                     distributions of a 4-bead chain with bond energy
+ *                     only: the Boltzmann histogram of its energy and the
+ *                     Wang-Landau ln g(E), both from the closed-form
+ *                     density of states.
  *   orc_mlp_*         no reference counterpart; pinned by central finite
  *                     differences.
  */
@@ -1220,6 +1233,170 @@ double orc_ran2(orc_ran2_state *st)
     if (st->iy < 1) st->iy += R2_IMM1;
     if ((temp = R2_AM * st->iy) > R2_RNMX) return R2_RNMX;
     return temp;
+}
+
+/* ======================================================================== */
+/* Random streams of the sampler (no reference counterpart)                  */
+/* ======================================================================== */
+
+/* splitmix64 (Steele, Lea and Flood, OOPSLA 2014; the seeding generator
+ * Blackman and Vigna recommend for xoshiro): golden-gamma increment, then
+ * the variant-13 finaliser. */
+uint64_t orc_splitmix64(uint64_t *x)
+{
+    uint64_t z = (*x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+/* xoshiro256** 1.0 (Blackman and Vigna, "Scrambled linear pseudorandom
+ * number generators", ACM TOMS 47 (2021)): output rotl(s1 * 5, 7) * 9, then
+ * the xorshift update with shift 17 and rotation 45. */
+static uint64_t orc_rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+uint64_t orc_xoshiro_next(uint64_t s[4])
+{
+    uint64_t out = orc_rotl(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = orc_rotl(s[3], 45);
+    return out;
+}
+
+/* the top 53 bits as a double in [0, 1) */
+double orc_xoshiro_uniform(uint64_t s[4]) { return (double)(orc_xoshiro_next(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+/* ======================================================================== */
+/* Wang-Landau sampling of a Go walker                                       */
+/* ======================================================================== */
+
+/* dos_interpolate.c:107-124 (DOS / CTDOS builds): ln g at energy val inside
+ * bin `bin`, on the line through the bin's midpoint (init.c:345) with the
+ * slope of the neighbouring bins (one-sided in the first and last bin), in
+ * the reference's order of operations.  One bin: ln g of that bin. */
+double orc_dos_interp(int nbins, const double *lng, double e_begin, double e_width, int bin, double val)
+{
+    double mid = e_begin + e_width * (bin + 0.5);
+    if (nbins < 2) return lng[bin];
+    if (bin == 0) return lng[bin] + (lng[bin + 1] - lng[bin]) * (val - mid) / e_width;
+    if (bin == nbins - 1) return lng[bin] + (lng[bin] - lng[bin - 1]) * (val - mid) / e_width;
+    return lng[bin] + (lng[bin + 1] - lng[bin - 1]) * (val - mid) * 0.5 / e_width;
+}
+
+/* pivot.c:236-238: the bin of U, or -1 outside [e_begin, e_begin + nbins w) */
+static int wl_bin(double u, const orc_wl_params *p)
+{
+    if (!(u >= p->emin)) return -1;
+    int b = (int)((u - p->emin) / p->ebin);
+    return b < p->nbins ? b : -1;
+}
+
+/* One walker, ntrials trials, plain acceptance.  Proposals are drawn as the
+ * device draws them: 64 streams (rng[4 l .. 4 l + 3] is stream l), trial t
+ * takes the 8 uniforms u[0..7] that stream t mod 64 produced for batch
+ * floor(t / 64); a batch draws 8 from every stream, at the start of the run
+ * and after every 64th trial that is not the last.
+ *   u0 < pivot_frac (n > 2): pivot (mcmove_pivot.c:54-215) about bead
+ *     p = (int)(u1 n), axis (int)(u2 3), side (int)(u3 2) (0: [0, p),
+ *     1: [p+1, n)), angle u5 pi theta_max, clockwise if (int)(u4 2) == 0
+ *     (the PIVOT matrices of mcmove_pivot.c:156-190);
+ *   otherwise displacement (mcmove_displ.c:45-99) of bead (int)(u1 n) by
+ *     (u2 - 0.5, u3 - 0.5, u4 - 0.5) disp_max;
+ *   u7 decides the acceptance (pivot.c:236-253): arg = interp ln g(old) -
+ *     interp ln g(new) (dos_interp), accepted if arg > 0 or exp(arg) > u7;
+ *     a new energy outside the window is a rejection;
+ *   after every trial ln g of the current bin grows by lnf and its histogram
+ *     count by one (the reference's g_of_e += mod_f, h_of_e++).
+ * An empty moved part (pivot at a chain end, side towards it) is a rejected
+ * trial without an energy evaluation.  The energy change is the range energy
+ * of the moved beads (forces_mc.c:90-300 o_potens / n_potens).
+ * stats: trials, accepted, 0, exact evaluations, outside the window,
+ * pivots, accepted pivots (ttc_mc.h's TTC_MC_STAT_* order).               */
+int orc_wl_run(const orc_wl_params *p, int n, double *pos, double *ecur, const unsigned char *natmask,
+               const double *r0mat, double *lng, long long *hist, uint64_t *rng, int ntrials, double lnf,
+               long long *stats)
+{
+    double u[64][8];
+    double *newp = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    if (!newp) return 11;
+    int bx = wl_bin(*ecur, p);
+    for (int t = 0; t < ntrials; t++) {
+        int l = t % 64;
+        if (l == 0)
+            for (int s = 0; s < 64; s++)
+                for (int q = 0; q < 8; q++) u[s][q] = orc_xoshiro_uniform(rng + 4 * s);
+        const double *v = u[l];
+        int s, e, type, piv = 0, axis = 0;
+        double c = 1.0, sn = 0.0, d[3] = {0.0, 0.0, 0.0};
+        if (v[0] < p->pivot_frac && n > 2) {
+            piv = (int)(v[1] * n);
+            int side = (int)(v[3] * 2.0);
+            double ang = v[5] * ORC_PI * p->theta_max;
+            double th = (int)(v[4] * 2.0) ? -ang : ang;
+            axis = (int)(v[2] * 3.0);
+            c = cos(th);
+            sn = sin(th);
+            type = 1;
+            s = side == 0 ? 0 : piv + 1;
+            e = side == 0 ? piv : n;
+        } else {
+            type = 0;
+            s = (int)(v[1] * n);
+            e = s + 1;
+            d[0] = (v[2] - 0.5) * p->disp_max;
+            d[1] = (v[3] - 0.5) * p->disp_max;
+            d[2] = (v[4] - 0.5) * p->disp_max;
+        }
+        stats[0]++;
+        stats[5] += type;
+        if (e > s) {
+            memcpy(newp, pos, sizeof(double) * 3 * (size_t)n);
+            for (int i = s; i < e; i++) {
+                if (type == 1) {
+                    double px = pos[3 * piv], py = pos[3 * piv + 1], pz = pos[3 * piv + 2];
+                    double x = pos[3 * i] - px, y = pos[3 * i + 1] - py, z = pos[3 * i + 2] - pz;
+                    double nx = x, ny = y, nz = z;
+                    if (axis == 0) { ny = c * y - sn * z; nz = sn * y + c * z; }
+                    else if (axis == 1) { nx = c * x + sn * z; nz = -sn * x + c * z; }
+                    else { nx = c * x + sn * y; ny = -sn * x + c * y; }
+                    newp[3 * i] = nx + px; newp[3 * i + 1] = ny + py; newp[3 * i + 2] = nz + pz;
+                } else {
+                    newp[3 * i] = pos[3 * i] + d[0];
+                    newp[3 * i + 1] = pos[3 * i + 1] + d[1];
+                    newp[3 * i + 2] = pos[3 * i + 2] + d[2];
+                }
+            }
+            double eo = orc_go_range_energy(n, pos, s, e, p->kbond, p->r0bond, natmask, r0mat, p->eps, p->sig_rep, 1e30);
+            double en = orc_go_range_energy(n, newp, s, e, p->kbond, p->r0bond, natmask, r0mat, p->eps, p->sig_rep, 1e30);
+            double ufin = *ecur + (en - eo);
+            stats[3]++;
+            int by = wl_bin(ufin, p);
+            if (by < 0 || bx < 0) {
+                stats[4]++;
+            } else {
+                double arg = orc_dos_interp(p->nbins, lng, p->emin, p->ebin, bx, *ecur) -
+                             orc_dos_interp(p->nbins, lng, p->emin, p->ebin, by, ufin);
+                if (arg > 0.0 || exp(arg) > v[7]) {
+                    memcpy(pos + 3 * s, newp + 3 * s, sizeof(double) * 3 * (size_t)(e - s));
+                    *ecur = ufin;
+                    bx = by;
+                    stats[1]++;
+                    stats[6] += type;
+                }
+            }
+        }
+        if (bx >= 0) {
+            lng[bx] += lnf;
+            hist[bx]++;
+        }
+    }
+    free(newp);
+    return 0;
 }
 
 /* ======================================================================== */

@@ -20,6 +20,8 @@
 #ifndef TTC_ORACLE_H
 #define TTC_ORACLE_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -179,6 +181,24 @@ double orc_go_range_energy(int n, const double *pos, int lb, int ub,
 typedef struct { long idum, idum2, iy; long iv[32]; } orc_ran2_state;
 void   orc_ran2_seed(orc_ran2_state *st, long seed);
 double orc_ran2(orc_ran2_state *st);
+
+/* --- random streams of the sampler and Wang-Landau (no reference RNG) ------- */
+uint64_t orc_splitmix64(uint64_t *x);
+uint64_t orc_xoshiro_next(uint64_t s[4]);
+double   orc_xoshiro_uniform(uint64_t s[4]);
+/* dos_interpolate.c:107-124: interpolated ln g at val in bin `bin` */
+double   orc_dos_interp(int nbins, const double *lng, double e_begin, double e_width, int bin, double val);
+typedef struct {
+    double kbond, r0bond, eps, sig_rep;       /* Go energy terms           */
+    double pivot_frac, theta_max, disp_max;   /* move set                  */
+    double emin, ebin;                        /* window [emin, emin+nbins*ebin) */
+    int nbins;
+} orc_wl_params;
+/* ntrials Wang-Landau trials of one walker (pivot.c:236-253) with the
+ * device's proposal streams rng[64*4]; lng/hist of nbins, stats[7]. */
+int orc_wl_run(const orc_wl_params *p, int n, double *pos, double *ecur, const unsigned char *natmask,
+               const double *r0mat, double *lng, long long *hist, uint64_t *rng, int ntrials, double lnf,
+               long long *stats);
 
 /* --- small MLP (no reference counterpart): FP64 forward / backward --------- */
 /* layers: in -> H (tanh) -> H (tanh) -> 1 (linear); params packed

@@ -7,7 +7,9 @@ together, and the exchange rule of swap_box.c:7-11.
   (2: batched launches of every kernel over all replicas).
 * Per-replica step-0 forces against the FP64 oracle: relative RMS <= 5e-6.
 * Exchanges: a certain swap exchanges the two temperatures and rescales
-  velocities by sqrt(T_new/T_old) exactly; an impossible one changes nothing.
+  velocities by sqrt(T_new/T_old) exactly; an impossible one changes nothing;
+  at a finite energy gap the swap is taken with frequency exp(dbeta dE)
+  (swap_box.c:7-12, kT = R T / 1000 as read_sim.c:563 sets it).
 """
 import numpy as np
 import pytest
@@ -96,3 +98,35 @@ def test_exchange_rule(systems):
     pot, _ = rb.md_nhc(2)
     assert np.isfinite(pot).all()
     rb.close()
+
+
+def test_exchange_frequency_at_finite_gap(systems):
+    from thecodecombo_7_amd.replicas import ReplicaBatch
+    kb, na = 1.380662e-23, 6.022045e23      # defines.h constants
+    rg = kb * na * 0.001
+    rb = ReplicaBatch(systems, TEMPS, seed=11)
+    try:
+        dbeta = 1.0 / (rg * TEMPS[0]) - 1.0 / (rg * TEMPS[1])
+        p_swap = 0.3
+        gap = np.log(p_swap) / dbeta          # E_0 - E_1 with exp(dbeta * gap) = 0.3
+        potens = np.array([gap, 0.0, 0.0])
+        print(f"case: exchange of ranks (0,1), dbeta {dbeta:.4f} mol/kJ, gap {gap:.3f} kJ/mol, "
+              f"expected frequency {p_swap}", flush=True)
+        tries = taken = 0
+        for _ in range(4000):
+            order, *_ = rb.state()
+            forward = order[0] == 0          # replica 0 at 300 K: the swap has probability 0.3
+            acc = rb.exchange(potens, 0)
+            if forward:
+                tries += 1
+                taken += acc
+            else:
+                assert acc == 1              # the reverse swap has exp(-dbeta gap) > 1
+        freq = taken / tries
+        se = np.sqrt(p_swap * (1 - p_swap) / tries)
+        print(f"{taken} of {tries} forward swaps taken: {freq:.4f} +- {se:.4f}", flush=True)
+        assert abs(freq - p_swap) < 4.5 * se
+        order, temps, _, att, accs = rb.state()
+        assert att == 4000 and sorted(temps) == sorted(TEMPS)
+    finally:
+        rb.close()

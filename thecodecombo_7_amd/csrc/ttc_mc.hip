@@ -469,6 +469,20 @@ __device__ void mlp_block(const float *W, const float *xs, float *hs, float *pv)
     __syncthreads();
 }
 
+// ln g at energy en in window bin b (dos_interpolate.c:107-124): on the line
+// through the bin's midpoint e_begin + e_width (b + 0.5) (init.c:345) with
+// the central slope of the neighbouring bins, one-sided in the first and
+// last bin, in the reference's order of operations and without contraction.
+// A one-bin window has no slope.
+__device__ __forceinline__ double lng_interp(const double *lng, int nbins, int b, double en, double emin, double ebin) {
+#pragma clang fp contract(off)
+    double mid = emin + ebin * (b + 0.5);
+    if (nbins < 2) return lng[b];
+    if (b == 0) return lng[b] + (lng[b + 1] - lng[b]) * (en - mid) / ebin;
+    if (b == nbins - 1) return lng[b] + (lng[b] - lng[b - 1]) * (en - mid) / ebin;
+    return lng[b] + (lng[b + 1] - lng[b - 1]) * (en - mid) * 0.5 / ebin;
+}
+
 // u < exp(lr) for u in [0,1): decided in FP32 unless u lies within 1e-5
 // (relative) of the threshold, then in FP64.  Every thread gets the same
 // answer from the same inputs.
@@ -630,6 +644,7 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
     // by at most an ulp of the quotient; every bin here uses the same form)
     const double inv_ebin = 1.0 / a.ebin;
     auto bin_of = [&](double en) { return (int)floor((en - emin) * inv_ebin); };
+    auto lg = [&](int b, double en) { return lng_interp(lng, a.nbins, b, en, emin, a.ebin); };
     int bx = bin_of(ecur);   // bin of the current energy
     double sur = 0.0, a1f = 0.0;   // surrogate dE, log first-stage probability of the trial
     int st_acc = 0, st_s1 = 0, st_ex = 0, st_out = 0, st_piv = 0, st_pacc = 0;   // this launch
@@ -742,14 +757,15 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
         if (w0) st_piv += type;
         if (DA && type == 1 && move) {
             // first stage: surrogate dE of the move (antisymmetric network),
-            // a1 = min(1, g(E_x) / g(E_x + sur)) as log a1
+            // a1 = min(1, g(E_x) / g(E_x + sur)) as log a1, g interpolated
             mlp_block(W, feat, hs, pv);
             if (w0) {
                 float ex = __expf((pv[0] + pv[1]) + (pv[2] + pv[3]));
                 float so = 0.5f * (ex - __builtin_amdgcn_rcpf(ex));   // sinh of the network output
                 sur = escale * (double)so;
                 int by = bin_of(ecur + sur);
-                a1f = (bx >= 0 && bx < a.nbins && by >= 0 && by < a.nbins) ? fmin(0.0, lng[bx] - lng[by]) : -INFINITY;
+                a1f = (bx >= 0 && bx < a.nbins && by >= 0 && by < a.nbins) ? fmin(0.0, lg(bx, ecur) - lg(by, ecur + sur))
+                                                                          : -INFINITY;
                 bool ok = accept_log(a1f, tb[tl]);
                 if (lane == 0) *pass = ok ? 1 : 0;
                 if (!ok) {
@@ -777,13 +793,15 @@ k_mc_run(McArgs a, int ntrials, double lnf, int do_log, double *gpos, const int 
                 if (by < 0 || by >= a.nbins || bx < 0 || bx >= a.nbins) {
                     st_out++;
                 } else {
-                    double lratio = lng[bx] - lng[by];
+                    // pivot.c:236-253: arg = ln g(old) - ln g(new), both
+                    // interpolated at the energies (dos_interp)
+                    double lratio = lg(bx, ecur) - lg(by, ecur + de);
                     if (DA && type == 1) {
                         // second stage: pi(y) a1(y,x) / (pi(x) a1(x,y)), a1(y,x) from
                         // the surrogate of the reverse move, -sur (all as logarithms)
                         int bz = bin_of(ecur + de - sur);
                         double la1r = -INFINITY;
-                        if (bz >= 0 && bz < a.nbins) la1r = fmin(0.0, lng[by] - lng[bz]);
+                        if (bz >= 0 && bz < a.nbins) la1r = fmin(0.0, lg(by, ecur + de) - lg(bz, ecur + de - sur));
                         lratio = lratio + la1r - a1f;
                     }
                     acc = accept_log(lratio, tb[MC_RB + tl]);
@@ -1254,6 +1272,24 @@ extern "C" int ttc_mc_get_walker(ttc_mc *m, int w, double *pos, double *energy) 
     HIPCHK(hipStreamSynchronize(m->stream));
     if (pos) HIPCHK(hipMemcpy(pos, m->pos + (size_t)w * m->p.maxn * 3, sizeof(double) * m->hn[w] * 3, hipMemcpyDeviceToHost));
     if (energy) HIPCHK(hipMemcpy(energy, m->ecur + w, sizeof(double), hipMemcpyDeviceToHost));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_get_streams(ttc_mc *m, int w, uint64_t *state) {
+    MCHECK(m && state && w >= 0 && w < m->p.nwalk, TTC_ERR_ARG, "ttc_mc_get_streams: bad argument");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy(state, m->rng + (size_t)w * MC_STREAMS * 4, sizeof(uint64_t) * MC_STREAMS * 4, hipMemcpyDeviceToHost));
+    return TTC_OK;
+}
+
+extern "C" int ttc_mc_set_streams(ttc_mc *m, int w, const uint64_t *state) {
+    MCHECK(m && state && w >= 0 && w < m->p.nwalk, TTC_ERR_ARG, "ttc_mc_set_streams: bad argument");
+    for (int s = 0; s < MC_STREAMS; s++) {
+        const uint64_t *q = state + 4 * s;
+        MCHECK(q[0] | q[1] | q[2] | q[3], TTC_ERR_ARG, "ttc_mc_set_streams: all-zero stream state");
+    }
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy(m->rng + (size_t)w * MC_STREAMS * 4, state, sizeof(uint64_t) * MC_STREAMS * 4, hipMemcpyHostToDevice));
     return TTC_OK;
 }
 

@@ -43,6 +43,8 @@ def _declare(L):
     L.ttc_mc_destroy.argtypes = [vp]
     L.ttc_mc_set_walker.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int, ip, ip, dp, C.c_double]
     L.ttc_mc_get_walker.argtypes = [vp, C.c_int, dp, dp]
+    L.ttc_mc_get_streams.argtypes = [vp, C.c_int, C.POINTER(C.c_uint64)]
+    L.ttc_mc_set_streams.argtypes = [vp, C.c_int, C.POINTER(C.c_uint64)]
     L.ttc_mc_energy.argtypes = [vp, dp]
     L.ttc_mc_delta_energy.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
     L.ttc_mc_run.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int]
@@ -71,7 +73,10 @@ def _f(a):
 
 class Walkers:
     def __init__(self, chains, *, nbins=512, ebin=0.5 * system.KCAL, seed=2024, pivot_frac=0.5,
-                 theta_max=0.25, disp_max=0.6, window_below=10.0):
+                 theta_max=0.25, disp_max=0.6, window_below=10.0, emin=None, kbond=system.GO_KBOND,
+                 sig_rep=system.GO_SIG_REP, eps=system.GO_EPS):
+        """emin: the windows' lower edges (scalar or one per walker); by
+        default each window starts window_below under the walker's energy."""
         L = T.lib()
         _declare(L)
         self.L = L
@@ -80,18 +85,21 @@ class Walkers:
         self.maxn = max(c.n for c in chains)
         self.nbins = nbins
         self.ebin = ebin
-        self.p = McParams(nwalk=self.nwalk, maxn=self.maxn, nbins=nbins, ebin=ebin, kbond=system.GO_KBOND,
-                          r0bond=system.GO_BOND, eps=system.GO_EPS, sig_rep=system.GO_SIG_REP,
+        self.p = McParams(nwalk=self.nwalk, maxn=self.maxn, nbins=nbins, ebin=ebin, kbond=kbond,
+                          r0bond=system.GO_BOND, eps=eps, sig_rep=sig_rep,
                           pivot_frac=pivot_frac, theta_max=theta_max, disp_max=disp_max, seed=seed)
         h = C.c_void_p()
         T.check(L.ttc_mc_create(C.byref(self.p), C.byref(h)))
         self.h = h
-        self.emin = np.zeros(self.nwalk)
+        if emin is not None:
+            self.emin = np.broadcast_to(np.asarray(emin, dtype=np.float64), (self.nwalk,)).copy()
+        else:
+            self.emin = np.zeros(self.nwalk)
+            for w, c in enumerate(chains):
+                self.set_walker(w, c.pos, c, None)
+            e = self.energy()
+            self.emin = e - window_below
         for w, c in enumerate(chains):
-            self.set_walker(w, c.pos, c, None)
-        e = self.energy()
-        for w, c in enumerate(chains):
-            self.emin[w] = e[w] - window_below
             self.set_walker(w, c.pos, c, self.emin[w])
 
     def set_walker(self, w, pos, chain, emin):
@@ -112,6 +120,16 @@ class Walkers:
             self.close()
         except Exception:
             pass
+
+    def streams(self, w):
+        """The 64 xoshiro256** states of walker w, (64, 4) uint64."""
+        s = np.zeros((64, 4), dtype=np.uint64)
+        T.check(self.L.ttc_mc_get_streams(self.h, w, s.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return s
+
+    def set_streams(self, w, s):
+        s = np.ascontiguousarray(s, dtype=np.uint64).reshape(64, 4)
+        T.check(self.L.ttc_mc_set_streams(self.h, w, s.ctypes.data_as(C.POINTER(C.c_uint64))))
 
     def energy(self):
         e = np.zeros(self.nwalk)
