@@ -110,6 +110,46 @@ int  ttc_pair(ttc_ctx *c, double *ff, int64_t stride, double *e5, double *vir6);
 /* cbond.c:65-120: returns bond energy through *ebond, forces added.       */
 int  ttc_cbond(ttc_ctx *c, double *ff, int64_t stride, double *ebond, double *vir6);
 
+/* ---- angle-type bonded terms ------------------------------------------
+ * read_topology.c's bend / urey / tors / impr arrays of box k:
+ *   bends      (a,b,c) 3 per term, angeq [rad], krbend: cbend.c:63-170
+ *   Urey-Bradley (a,b) 2 per term, Seq, k:              curey.c:57-140
+ *   torsions   (a,b,c,d) 4 per term, 6 Fourier entries each in kphi, nphi,
+ *              delphi (the series ends at the first kphi == 0):
+ *                                                       ctorsion.c:67-236
+ *   impropers  (a,b,c,d), kimpr, angeq [rad]:           cimproper.c:62-225
+ * Any count may be 0; a new call replaces all four sets.  Once set, every
+ * force evaluation that includes bonds includes them (ttc_forces, the MD
+ * loops, the replica batch; RESPA splits them as force_short.c /
+ * force_long.c do: bends and Urey-Bradley short, torsions and impropers
+ * long), and the energy rows' bond column holds bond + bend + urey + tors
+ * + impr (calcvalue.c:44-53).  No counterpart call in the reference (it
+ * reads the arrays from its topology files).                              */
+int  ttc_set_angles(ttc_ctx *c, int nbend, const int *bend_abc, const double *bend_eq, const double *bend_k,
+                    int nurey, const int *urey_ab, const double *urey_s0, const double *urey_k,
+                    int ntors, const int *tors_abcd, const double *kphi, const double *nphi,
+                    const double *delphi, int nimpr, const int *impr_abcd, const double *impr_k,
+                    const double *impr_eq);
+/* cbend.c:25 cbend(k), curey.c curey(k), ctorsion.c:25 ctorsion(k),
+ * cimproper.c cimproper(k): the kind's energy, forces added into ff,
+ * vir6 = pvir[k].bend / .urey / .tors / .impr.  ff / energy / vir6 may be
+ * NULL.                                                                   */
+int  ttc_cbend(ttc_ctx *c, double *ff, int64_t stride, double *ebend, double *vir6);
+int  ttc_curey(ttc_ctx *c, double *ff, int64_t stride, double *eurey, double *vir6);
+int  ttc_ctorsion(ttc_ctx *c, double *ff, int64_t stride, double *etors, double *vir6);
+int  ttc_cimproper(ttc_ctx *c, double *ff, int64_t stride, double *eimpr, double *vir6);
+/* cbend2.c, curey2.c, ctorsion2.c, cimproper2.c (CONFIGT): as above with
+ * hes3 = the kind's sums of d2E/dx_i^2, d2E/dy_i^2, d2E/dz_i^2 (added to
+ * config[k].hesx/y/z by the reference).                                    */
+int  ttc_cbend2(ttc_ctx *c, double *ff, int64_t stride, double *ebend, double *hes3);
+int  ttc_curey2(ttc_ctx *c, double *ff, int64_t stride, double *eurey, double *hes3);
+int  ttc_ctorsion2(ttc_ctx *c, double *ff, int64_t stride, double *etors, double *hes3);
+int  ttc_cimproper2(ttc_ctx *c, double *ff, int64_t stride, double *eimpr, double *hes3);
+/* en[k].bond, .bend, .urey, .tors, .impr of the last evaluation that
+ * computed energies with bonded terms (ttc_forces, ttc_configtemp, an
+ * energy step of an MD loop).                                              */
+int  ttc_bonded_energies(ttc_ctx *c, double *e5);
+
 /* ---- CONFIGT (configurational temperature; used by the CTDOS drivers) ---
  * hes3 = {hesx, hesy, hesz}: sums over sites of d2U/dx_i^2, d2U/dy_i^2,
  * d2U/dz_i^2 (config[k].hesx/y/z).                                       */
@@ -127,9 +167,11 @@ int  ttc_cbond2(ttc_ctx *c, double *ff, int64_t stride, double *ebond, double *h
  * T = num / den / RG / 0.001.  hes3 optional.                            */
 int  ttc_configtemp(ttc_ctx *c, double *num, double *den, double *T, double *hes3);
 
-/* forces.c:118 forces(ibox) for a system whose bonded terms are bonds:
- * zero, cbond, cnonbondnl; en[k] slots (forces.c:204-219) are returned as
- * en9 = {bond, nbond, nbonds, coulomb, rewald, rewalds, kewald, sewald, iewald}. */
+/* forces.c:118 forces(ibox): zero, cbond, the angle-type terms set by
+ * ttc_set_angles (cbend, curey, ctorsion, cimproper), cnonbondnl; en[k]
+ * slots (forces.c:204-219) are returned as en9 = {bond, nbond, nbonds,
+ * coulomb, rewald, rewalds, kewald, sewald, iewald} (bend, urey, tors, impr
+ * through ttc_bonded_energies).                                           */
 int  ttc_forces(ttc_ctx *c, double *ff, int64_t stride, double *en9);
 
 /* integrate_vverlet.c:36-103 repeated nsteps times with nl_check/nblist

@@ -102,6 +102,20 @@ def _declare(L):
     L.orc_cbond.restype = C.c_double
     L.orc_cbond.argtypes = [P(OrcSystem), _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
     L.orc_pbc_all.argtypes = [P(OrcSystem), _dp, C.c_int]
+    L.orc_cbend.restype = C.c_double
+    L.orc_cbend.argtypes = [P(OrcSystem), _dp, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
+    L.orc_curey.restype = C.c_double
+    L.orc_curey.argtypes = [P(OrcSystem), _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
+    L.orc_ctorsion.restype = C.c_double
+    L.orc_ctorsion.argtypes = [P(OrcSystem), _dp, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
+    L.orc_cimproper.restype = C.c_double
+    L.orc_cimproper.argtypes = [P(OrcSystem), _dp, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
+    for nm, args in (("orc_cbend2", [P(OrcSystem), _dp, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp]),
+                     ("orc_curey2", [P(OrcSystem), _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]),
+                     ("orc_ctorsion2", [P(OrcSystem), _dp, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp]),
+                     ("orc_cimproper2", [P(OrcSystem), _dp, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp])):
+        getattr(L, nm).restype = C.c_double
+        getattr(L, nm).argtypes = args
     L.orc_kinet.restype = C.c_double
     L.orc_kinet.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
     L.orc_vverlet_a.argtypes = [P(OrcSystem), C.c_int, _dp, C.c_double, C.c_double, _dp, C.c_void_p, _dp, _dp]
@@ -241,6 +255,36 @@ class Oracle:
 
 
     # ---- CONFIGT (cnonbond_nblist2.c, cbond2.c) ----
+    def angles(self, pos, at, which=("bend", "urey", "tors", "impr"), hessian=False):
+        """Angle-type terms of system.AngleTerms `at` at pos: forces (n,3)
+        summed over the chosen kinds, energies and virials per kind (with
+        hessian: the CONFIGT variants, Hessian diagonal sums per kind)."""
+        sfx = "2" if hessian else ""
+        L = self.L
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        ff = np.zeros(self.s.n * 3)
+        en, vir = {}, {}
+        col = lambda a, k: np.ascontiguousarray(a[:, k], dtype=np.int32)
+        for w in which:
+            v = np.zeros(3 if hessian else 6)
+            if w == "bend":
+                e = getattr(L, 'orc_cbend' + sfx)(C.byref(self.cs), pos, len(at.bend), col(at.bend, 0), col(at.bend, 1),
+                                col(at.bend, 2), np.ascontiguousarray(at.bend_eq), np.ascontiguousarray(at.bend_k), ff, v)
+            elif w == "urey":
+                e = getattr(L, 'orc_curey' + sfx)(C.byref(self.cs), pos, len(at.urey), col(at.urey, 0), col(at.urey, 1),
+                                np.ascontiguousarray(at.urey_s0), np.ascontiguousarray(at.urey_k), ff, v)
+            elif w == "tors":
+                e = getattr(L, 'orc_ctorsion' + sfx)(C.byref(self.cs), pos, len(at.tors), col(at.tors, 0), col(at.tors, 1),
+                                   col(at.tors, 2), col(at.tors, 3), np.ascontiguousarray(at.kphi).ravel(),
+                                   np.ascontiguousarray(at.nphi).ravel(), np.ascontiguousarray(at.delphi).ravel(), ff, v)
+            else:
+                e = getattr(L, 'orc_cimproper' + sfx)(C.byref(self.cs), pos, len(at.impr), col(at.impr, 0), col(at.impr, 1),
+                                    col(at.impr, 2), col(at.impr, 3), np.ascontiguousarray(at.impr_k),
+                                    np.ascontiguousarray(at.impr_eq), ff, v)
+            en[w] = e
+            vir[w] = v
+        return ff.reshape(-1, 3), en, vir
+
     def pair(self, pos, cnt, lst, xf):
         """The pair loop alone: forces and the orc_pair_out fields."""
         n = self.s.n
@@ -308,6 +352,15 @@ def ref_lib():
     L.ref_pbc_all.argtypes = [_dp]
     L.ref_kinet.restype = C.c_double
     L.ref_kinet.argtypes = [_dp, _dp, _dp, _dp]
+    L.ref_bonded_box.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+    L.ref_cbend.restype = C.c_double
+    L.ref_cbend.argtypes = [_dp, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
+    L.ref_curey.restype = C.c_double
+    L.ref_curey.argtypes = [_dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
+    L.ref_ctorsion.restype = C.c_double
+    L.ref_ctorsion.argtypes = [_dp, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
+    L.ref_cimproper.restype = C.c_double
+    L.ref_cimproper.argtypes = [_dp, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
     return L
 
 
@@ -368,3 +421,22 @@ def wl_run(params: WlParams, pos, ecur, nat, r0m, lng, hist, streams, ntrials, l
                       stats.ctypes.data_as(C.POINTER(C.c_longlong)))
     assert rc == 0
     return e.value
+
+
+REF_CT_SO = os.path.join(HERE, "_ref", "libttcref_ct.so")
+
+
+def ref_ct_lib():
+    """The reference's own CONFIGT bonded terms (cbend2.c, curey2.c,
+    ctorsion2.c, cimproper2.c)."""
+    L = C.CDLL(REF_CT_SO)
+    L.ref_ct_box.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+    L.ref_cbend2.restype = C.c_double
+    L.ref_cbend2.argtypes = [_dp, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
+    L.ref_curey2.restype = C.c_double
+    L.ref_curey2.argtypes = [_dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
+    L.ref_ctorsion2.restype = C.c_double
+    L.ref_ctorsion2.argtypes = [_dp, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
+    L.ref_cimproper2.restype = C.c_double
+    L.ref_cimproper2.argtypes = [_dp, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
+    return L

@@ -309,3 +309,112 @@ double ref_kinet(const double *mass, const double *vel, const double *uuv, doubl
     for (int i = 0; i < 6; i++) ken6[i] = en[0].ken[i];
     return en[0].kinet;
 }
+
+/* ---- cbend.c, curey.c, ctorsion.c, cimproper.c ------------------------- */
+double cbend(int);
+double curey(int);
+double ctorsion(int);
+double cimproper(int);
+
+/* box 0 of n sites for the bonded-term calls below (allocate.c's atom / ff
+ * arrays, read_topology.c's term arrays filled by each call) */
+void ref_bonded_box(int n, double lx, double ly, double lz)
+{
+    g_n = n;
+    sim.NB = 1;
+    box[0].boxns = n;
+    box[0].lx = lx; box[0].ly = ly; box[0].lz = lz;
+    box[0].hx = 0.5 * lx; box[0].hy = 0.5 * ly; box[0].hz = 0.5 * lz;
+    if (!atom) atom = (struct atoms **)xcalloc(1, sizeof(struct atoms *));
+    if (!ff) ff = (struct veloc **)xcalloc(1, sizeof(struct veloc *));
+    if (!bend) bend = (struct bends **)xcalloc(1, sizeof(struct bends *));
+    if (!urey) urey = (struct ureys **)xcalloc(1, sizeof(struct ureys *));
+    if (!tors) tors = (struct torsions **)xcalloc(1, sizeof(struct torsions *));
+    if (!impr) impr = (struct impropers **)xcalloc(1, sizeof(struct impropers *));
+    free(atom[0]); free(ff[0]);
+    atom[0] = (struct atoms *)xcalloc(n, sizeof(struct atoms));
+    ff[0] = (struct veloc *)xcalloc(n, sizeof(struct veloc));
+}
+
+static void bonded_in(const double *pos)
+{
+    for (int i = 0; i < g_n; i++) {
+        atom[0][i].x = pos[3 * i]; atom[0][i].y = pos[3 * i + 1]; atom[0][i].z = pos[3 * i + 2];
+        ff[0][i].x = ff[0][i].y = ff[0][i].z = 0.0;
+    }
+}
+
+static void bonded_out(double *ff_out, const double *w, double *vir6)
+{
+    for (int i = 0; i < g_n; i++) {
+        ff_out[3 * i] = ff[0][i].x; ff_out[3 * i + 1] = ff[0][i].y; ff_out[3 * i + 2] = ff[0][i].z;
+    }
+    for (int q = 0; q < 6; q++) vir6[q] = w[q];
+}
+
+double ref_cbend(const double *pos, int nb, const int *ia, const int *ib, const int *ic, const double *angeq,
+                 const double *kb, double *ff_out, double *vir6)
+{
+    free(bend[0]);
+    bend[0] = (struct bends *)xcalloc(nb, sizeof(struct bends));
+    for (int i = 0; i < nb; i++) {
+        bend[0][i].a = ia[i]; bend[0][i].b = ib[i]; bend[0][i].c = ic[i];
+        bend[0][i].angeq = angeq[i]; bend[0][i].krbend = kb[i];
+    }
+    bendN[0] = nb;
+    bonded_in(pos);
+    double e = cbend(0);
+    bonded_out(ff_out, pvir[0].bend, vir6);
+    return e;
+}
+
+double ref_curey(const double *pos, int nu, const int *ia, const int *ib, const double *seq, const double *ku,
+                 double *ff_out, double *vir6)
+{
+    free(urey[0]);
+    urey[0] = (struct ureys *)xcalloc(nu, sizeof(struct ureys));
+    for (int i = 0; i < nu; i++) {
+        urey[0][i].a = ia[i]; urey[0][i].b = ib[i]; urey[0][i].Seq = seq[i]; urey[0][i].k = ku[i];
+    }
+    ureyN[0] = nu;
+    bonded_in(pos);
+    double e = curey(0);
+    bonded_out(ff_out, pvir[0].urey, vir6);
+    return e;
+}
+
+double ref_ctorsion(const double *pos, int nt, const int *ta, const int *tb, const int *tc, const int *td,
+                    const double *kphi, const double *nphi, const double *delphi, double *ff_out, double *vir6)
+{
+    free(tors[0]);
+    tors[0] = (struct torsions *)xcalloc(nt, sizeof(struct torsions));
+    for (int i = 0; i < nt; i++) {
+        tors[0][i].a = ta[i]; tors[0][i].b = tb[i]; tors[0][i].c = tc[i]; tors[0][i].d = td[i];
+        for (int m = 0; m < 6; m++) {
+            tors[0][i].kphi[m] = kphi[6 * i + m];
+            tors[0][i].nphi[m] = nphi[6 * i + m];
+            tors[0][i].delphi[m] = delphi[6 * i + m];
+        }
+    }
+    torsN[0] = nt;
+    bonded_in(pos);
+    double e = ctorsion(0);
+    bonded_out(ff_out, pvir[0].tors, vir6);
+    return e;
+}
+
+double ref_cimproper(const double *pos, int ni, const int *ia, const int *ib, const int *ic, const int *id,
+                     const double *kimpr, const double *angeq, double *ff_out, double *vir6)
+{
+    free(impr[0]);
+    impr[0] = (struct impropers *)xcalloc(ni, sizeof(struct impropers));
+    for (int i = 0; i < ni; i++) {
+        impr[0][i].a = ia[i]; impr[0][i].b = ib[i]; impr[0][i].c = ic[i]; impr[0][i].d = id[i];
+        impr[0][i].kimpr = kimpr[i]; impr[0][i].angeq = angeq[i];
+    }
+    imprN[0] = ni;
+    bonded_in(pos);
+    double e = cimproper(0);
+    bonded_out(ff_out, pvir[0].impr, vir6);
+    return e;
+}

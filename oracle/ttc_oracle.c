@@ -1017,6 +1017,366 @@ double orc_cbond(const orc_system *s, const double *pos, int nb, const int *ia,
     return e;
 }
 
+/* ======================================================================== */
+/* cbend.c, curey.c, ctorsion.c, cimproper.c                                 */
+/* ======================================================================== */
+
+/* the reference's minimum image of one component (cbend.c:85-98 form) */
+static double orc_mi(double d, double h, double l)
+{
+    if (d > h) d -= l;
+    else if (d < -h) d += l;
+    return d;
+}
+
+/* cbend.c:63-170: E = kb (phi - phi0)^2 for the angle phi at b between the
+ * bonds b->a and b->c; forces added, virial of the bends in vir[6]. */
+double orc_cbend(const orc_system *s, const double *pos, int nbend, const int *ia, const int *ib,
+                 const int *ic, const double *angeq, const double *kb, double *ff, double vir[6])
+{
+    double lx = s->lx, ly = s->ly, lz = s->lz, hx = 0.5 * lx, hy = 0.5 * ly, hz = 0.5 * lz;
+    double e = 0.0;
+    for (int q = 0; q < 6; q++) vir[q] = 0.0;
+    for (int i = 0; i < nbend; i++) {
+        int a = ia[i], b = ib[i], c = ic[i];
+        double ax = orc_mi(pos[3 * a] - pos[3 * b], hx, lx), ay = orc_mi(pos[3 * a + 1] - pos[3 * b + 1], hy, ly),
+               az = orc_mi(pos[3 * a + 2] - pos[3 * b + 2], hz, lz);
+        double cx = orc_mi(pos[3 * c] - pos[3 * b], hx, lx), cy = orc_mi(pos[3 * c + 1] - pos[3 * b + 1], hy, ly),
+               cz = orc_mi(pos[3 * c + 2] - pos[3 * b + 2], hz, lz);
+        double ia_ = 1.0 / sqrt(ax * ax + ay * ay + az * az);
+        double uax = ax * ia_, uay = ay * ia_, uaz = az * ia_;
+        double ic_ = 1.0 / sqrt(cx * cx + cy * cy + cz * cz);
+        double ucx = cx * ic_, ucy = cy * ic_, ucz = cz * ic_;
+        double cs = uax * ucx + uay * ucy + uaz * ucz;
+        double phi = acos(cs);
+        double isn = 1.0 / sqrt(1.0 - cs * cs);
+        double dphi = phi - angeq[i];
+        double fr = -kb[i] * dphi * 2.0;
+        e += -fr * dphi * 0.5;
+        double ga = ia_ * isn * fr, gc = ic_ * isn * fr;
+        double fax = (uax * cs - ucx) * ga, fay = (uay * cs - ucy) * ga, faz = (uaz * cs - ucz) * ga;
+        double fcx = (ucx * cs - uax) * gc, fcy = (ucy * cs - uay) * gc, fcz = (ucz * cs - uaz) * gc;
+        ff[3 * a] += fax; ff[3 * a + 1] += fay; ff[3 * a + 2] += faz;
+        ff[3 * b] += -(fax + fcx); ff[3 * b + 1] += -(fay + fcy); ff[3 * b + 2] += -(faz + fcz);
+        ff[3 * c] += fcx; ff[3 * c + 1] += fcy; ff[3 * c + 2] += fcz;
+        vir[0] += fax * ax + fcx * cx; vir[1] += fay * ax + fcy * cx; vir[2] += fay * ay + fcy * cy;
+        vir[3] += faz * ax + fcz * cx; vir[4] += faz * ay + fcz * cy; vir[5] += faz * az + fcz * cz;
+    }
+    return e;
+}
+
+/* curey.c:57-140: Urey-Bradley 1-3 term k (r - S0)^2, as a bond. */
+double orc_curey(const orc_system *s, const double *pos, int nu, const int *ia, const int *ib,
+                 const double *seq, const double *ku, double *ff, double vir[6])
+{
+    return orc_cbond(s, pos, nu, ia, ib, seq, ku, ff, vir);
+}
+
+/* The dihedral of a-b-c-d and what the force transformation of
+ * ctorsion.c:128-236 / cimproper.c needs from it. */
+typedef struct {
+    double d1[3], d2[3], d3[3];           /* b-a, c-b, d-c (minimum image) */
+    double r1, r2, r3, i1, i2, i3;        /* lengths and inverses */
+    double n1[3], n2[3];                  /* e1 x e2, e3 x e2 */
+    double cb, cc, isb2, isc2;            /* -e1.e2, -e3.e2, 1/sin^2 */
+    double tau;
+} orc_dihedral;
+
+static void orc_dihedral_eval(const orc_system *s, const double *pos, int a, int b, int c, int d, orc_dihedral *g)
+{
+    double lx = s->lx, ly = s->ly, lz = s->lz, hx = 0.5 * lx, hy = 0.5 * ly, hz = 0.5 * lz;
+    const double *P[4] = {pos + 3 * a, pos + 3 * b, pos + 3 * c, pos + 3 * d};
+    double *D[3] = {g->d1, g->d2, g->d3};
+    for (int k = 0; k < 3; k++) {
+        D[k][0] = orc_mi(P[k + 1][0] - P[k][0], hx, lx);
+        D[k][1] = orc_mi(P[k + 1][1] - P[k][1], hy, ly);
+        D[k][2] = orc_mi(P[k + 1][2] - P[k][2], hz, lz);
+    }
+    double e1[3], e2[3], e3[3];
+    g->r1 = sqrt(g->d1[0] * g->d1[0] + g->d1[1] * g->d1[1] + g->d1[2] * g->d1[2]);
+    g->i1 = 1.0 / g->r1;
+    for (int k = 0; k < 3; k++) e1[k] = g->d1[k] * g->i1;
+    g->r2 = sqrt(g->d2[0] * g->d2[0] + g->d2[1] * g->d2[1] + g->d2[2] * g->d2[2]);
+    g->i2 = 1.0 / g->r2;
+    for (int k = 0; k < 3; k++) e2[k] = g->d2[k] * g->i2;
+    g->r3 = sqrt(g->d3[0] * g->d3[0] + g->d3[1] * g->d3[1] + g->d3[2] * g->d3[2]);
+    g->i3 = 1.0 / g->r3;
+    for (int k = 0; k < 3; k++) e3[k] = g->d3[k] * g->i3;
+    g->n1[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    g->n1[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    g->n1[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    g->cb = -(e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2]);
+    g->isb2 = 1.0 / (1.0 - g->cb * g->cb);
+    double isb = sqrt(g->isb2);
+    g->n2[0] = e3[1] * e2[2] - e3[2] * e2[1];
+    g->n2[1] = e3[2] * e2[0] - e3[0] * e2[2];
+    g->n2[2] = e3[0] * e2[1] - e3[1] * e2[0];
+    g->cc = -(e3[0] * e2[0] + e3[1] * e2[1] + e3[2] * e2[2]);
+    g->isc2 = 1.0 / (1.0 - g->cc * g->cc);
+    double isc = sqrt(g->isc2);
+    /* sign from (n1 x n2).e2 with the reference's sign convention */
+    double mx = -(g->n1[1] * g->n2[2] - g->n1[2] * g->n2[1]);
+    double my = -(g->n1[2] * g->n2[0] - g->n1[0] * g->n2[2]);
+    double mz = -(g->n1[0] * g->n2[1] - g->n1[1] * g->n2[0]);
+    double sgn = (mx * e2[0] + my * e2[1] + mz * e2[2]) >= 0.0 ? 1.0 : -1.0;
+    double ct = -(g->n1[0] * g->n2[0] + g->n1[1] * g->n2[1] + g->n1[2] * g->n2[2]) * isb * isc;
+    if (ct > 1.0) ct = 1.0;
+    if (ct < -1.0) ct = -1.0;
+    g->tau = sgn * acos(ct);
+}
+
+/* forces of a dihedral term with dE/dtau = -fr on the four sites, and its
+ * virial (ctorsion.c:199-251, the same transformation in cimproper.c) */
+static void orc_dihedral_force(const orc_dihedral *g, double fr, int a, int b, int c, int d, double *ff,
+                               double vir[6])
+{
+    double fa = -fr * g->i1 * g->isb2;
+    double fA[3], fB[3], fC[3], fD[3];
+    double fb1 = fr * (g->r2 - g->r1 * g->cb) * g->i1 * g->i2 * g->isb2;
+    double fb2 = fr * g->cc * g->i2 * g->isc2;
+    double fc1 = fr * (g->r2 - g->r3 * g->cc) * g->i3 * g->i2 * g->isc2;
+    double fc2 = fr * g->cb * g->i2 * g->isb2;
+    double fd = -fr * g->i3 * g->isc2;
+    for (int k = 0; k < 3; k++) {
+        fA[k] = fa * g->n1[k];
+        fB[k] = fb1 * g->n1[k] + fb2 * g->n2[k];
+        fC[k] = fc1 * g->n2[k] + fc2 * g->n1[k];
+        fD[k] = fd * g->n2[k];
+        ff[3 * a + k] += fA[k];
+        ff[3 * b + k] += fB[k];
+        ff[3 * c + k] += fC[k];
+        ff[3 * d + k] += fD[k];
+    }
+    /* virial: sum over sites of f x r relative to a: -fA d1 + fC d2 + fD (d2 + d3) in the
+     * reference's component pairs */
+    int ii[6] = {0, 1, 1, 2, 2, 2}, jj[6] = {0, 0, 1, 0, 1, 2};
+    for (int q = 0; q < 6; q++) {
+        int i = ii[q], j = jj[q];
+        vir[q] += -fA[i] * g->d1[j] + fC[i] * g->d2[j] + fD[i] * (g->d2[j] + g->d3[j]);
+    }
+}
+
+/* ctorsion.c:67-236: E = sum_t kphi_t (1 + cos(nphi_t tau - delphi_t)) over
+ * the leading nonzero kphi (at most 6 per torsion; six entries per torsion
+ * in kphi / nphi / delphi). */
+double orc_ctorsion(const orc_system *s, const double *pos, int nt, const int *ta, const int *tb, const int *tc,
+                    const int *td, const double *kphi, const double *nphi, const double *delphi, double *ff,
+                    double vir[6])
+{
+    double e = 0.0;
+    for (int q = 0; q < 6; q++) vir[q] = 0.0;
+    for (int i = 0; i < nt; i++) {
+        orc_dihedral g;
+        orc_dihedral_eval(s, pos, ta[i], tb[i], tc[i], td[i], &g);
+        double fr = 0.0, et = 0.0;
+        for (int m = 0; m < 6; m++) {
+            double kk = kphi[6 * i + m];
+            if (kk == 0.0) break;
+            double arg = nphi[6 * i + m] * g.tau - delphi[6 * i + m];
+            fr += kk * nphi[6 * i + m] * sin(arg);
+            et += kk * (1 + cos(arg));
+        }
+        e += et;
+        orc_dihedral_force(&g, fr, ta[i], tb[i], tc[i], td[i], ff, vir);
+    }
+    return e;
+}
+
+/* cimproper.c:62-225: E = kimpr (tau - tau0)^2 on the dihedral a-b-c-d. */
+double orc_cimproper(const orc_system *s, const double *pos, int ni, const int *ia, const int *ib, const int *ic,
+                     const int *id, const double *kimpr, const double *angeq, double *ff, double vir[6])
+{
+    double e = 0.0;
+    for (int q = 0; q < 6; q++) vir[q] = 0.0;
+    for (int i = 0; i < ni; i++) {
+        orc_dihedral g;
+        orc_dihedral_eval(s, pos, ia[i], ib[i], ic[i], id[i], &g);
+        double dt = g.tau - angeq[i];
+        double fr = -2.0 * kimpr[i] * dt;
+        e += -fr * dt * 0.5;
+        orc_dihedral_force(&g, fr, ia[i], ib[i], ic[i], id[i], ff, vir);
+    }
+    return e;
+}
+
+/* ---- Hessian diagonals of the angle-type terms (cbend2.c, curey2.c,
+ * ctorsion2.c, cimproper2.c): d2E/dx_i^2 of every coordinate of a term's
+ * sites by second-order forward differentiation (value, first and second
+ * derivative along one coordinate) of the term's energy; dihedrals as
+ * atan2(sin, cos), regular at 0 and pi.                                    */
+typedef struct { double v, d, dd; } oj2;
+static oj2 oj_c(double v) { oj2 r = {v, 0.0, 0.0}; return r; }
+static oj2 oj_add(oj2 a, oj2 b) { oj2 r = {a.v + b.v, a.d + b.d, a.dd + b.dd}; return r; }
+static oj2 oj_sub(oj2 a, oj2 b) { oj2 r = {a.v - b.v, a.d - b.d, a.dd - b.dd}; return r; }
+static oj2 oj_mul(oj2 a, oj2 b)
+{
+    oj2 r = {a.v * b.v, a.v * b.d + a.d * b.v, a.v * b.dd + 2.0 * a.d * b.d + a.dd * b.v};
+    return r;
+}
+static oj2 oj_scale(double s, oj2 a) { oj2 r = {s * a.v, s * a.d, s * a.dd}; return r; }
+static oj2 oj_fun(oj2 a, double f, double f1, double f2)
+{
+    oj2 r = {f, f1 * a.d, f1 * a.dd + f2 * a.d * a.d};
+    return r;
+}
+static oj2 oj_sqrt(oj2 a) { double s = sqrt(a.v); return oj_fun(a, s, 0.5 / s, -0.25 / (s * a.v)); }
+static oj2 oj_rcp(oj2 a) { double r = 1.0 / a.v; return oj_fun(a, r, -r * r, 2.0 * r * r * r); }
+static oj2 oj_acos(oj2 a)
+{
+    double q = 1.0 - a.v * a.v, s = sqrt(q);
+    return oj_fun(a, acos(a.v), -1.0 / s, -a.v / (q * s));
+}
+static oj2 oj_cos(oj2 a) { return oj_fun(a, cos(a.v), -sin(a.v), -cos(a.v)); }
+static oj2 oj_atan2(oj2 y, oj2 x)
+{
+    oj2 r2 = oj_add(oj_mul(x, x), oj_mul(y, y));
+    double n1 = x.v * y.d - y.v * x.d, n1d = x.v * y.dd - y.v * x.dd;
+    oj2 r = {atan2(y.v, x.v), n1 / r2.v, (n1d * r2.v - n1 * r2.d) / (r2.v * r2.v)};
+    return r;
+}
+static oj2 oj_dot(const oj2 *a, const oj2 *b)
+{
+    return oj_add(oj_add(oj_mul(a[0], b[0]), oj_mul(a[1], b[1])), oj_mul(a[2], b[2]));
+}
+static void oj_cross(const oj2 *a, const oj2 *b, oj2 *c)
+{
+    c[0] = oj_sub(oj_mul(a[1], b[2]), oj_mul(a[2], b[1]));
+    c[1] = oj_sub(oj_mul(a[2], b[0]), oj_mul(a[0], b[2]));
+    c[2] = oj_sub(oj_mul(a[0], b[1]), oj_mul(a[1], b[0]));
+}
+/* a vector of constants with one component seeded */
+static void oj_vec(const double *v, int axis, double seed, oj2 *r)
+{
+    for (int k = 0; k < 3; k++) r[k] = oj_c(v[k]);
+    r[axis].d = seed;
+}
+
+/* signed dihedral of b1, b2, b3 (ctorsion.c:171-182 convention) */
+static oj2 oj_dihedral(const oj2 *b1, const oj2 *b2, const oj2 *b3)
+{
+    oj2 n1[3], n2[3], m[3];
+    oj_cross(b1, b2, n1);
+    oj_cross(b2, b3, n2);
+    oj_cross(n1, n2, m);
+    oj2 y = oj_mul(oj_dot(m, b2), oj_rcp(oj_sqrt(oj_dot(b2, b2))));
+    return oj_atan2(y, oj_dot(n1, n2));
+}
+
+static void orc_d3(const orc_system *s, const double *pos, int a, int b, double *d)
+{
+    double lx = s->lx, ly = s->ly, lz = s->lz;
+    d[0] = orc_mi(pos[3 * b] - pos[3 * a], 0.5 * lx, lx);
+    d[1] = orc_mi(pos[3 * b + 1] - pos[3 * a + 1], 0.5 * ly, ly);
+    d[2] = orc_mi(pos[3 * b + 2] - pos[3 * a + 2], 0.5 * lz, lz);
+}
+
+/* cbend2.c: orc_cbend plus the Hessian diagonal sums */
+double orc_cbend2(const orc_system *s, const double *pos, int nbend, const int *ia, const int *ib,
+                  const int *ic, const double *angeq, const double *kb, double *ff, double hes3[3])
+{
+    double vir[6];
+    double e = orc_cbend(s, pos, nbend, ia, ib, ic, angeq, kb, ff, vir);
+    hes3[0] = hes3[1] = hes3[2] = 0.0;
+    for (int i = 0; i < nbend; i++) {
+        double u[3], w[3];
+        orc_d3(s, pos, ib[i], ia[i], u);   /* a - b */
+        orc_d3(s, pos, ib[i], ic[i], w);   /* c - b */
+        for (int ax = 0; ax < 3; ax++)
+            for (int site = 0; site < 3; site++) {
+                /* site a moves u, site c moves w, site b moves both by -1 */
+                oj2 U[3], W[3];
+                oj_vec(u, ax, site == 0 ? 1.0 : (site == 1 ? -1.0 : 0.0), U);
+                oj_vec(w, ax, site == 2 ? 1.0 : (site == 1 ? -1.0 : 0.0), W);
+                oj2 c = oj_mul(oj_dot(U, W), oj_rcp(oj_mul(oj_sqrt(oj_dot(U, U)), oj_sqrt(oj_dot(W, W)))));
+                oj2 dphi = oj_sub(oj_acos(c), oj_c(angeq[i]));
+                hes3[ax] += oj_scale(kb[i], oj_mul(dphi, dphi)).dd;
+            }
+    }
+    return e;
+}
+
+/* curey2.c: orc_curey plus the Hessian diagonal sums */
+double orc_curey2(const orc_system *s, const double *pos, int nu, const int *ia, const int *ib,
+                  const double *seq, const double *ku, double *ff, double hes3[3])
+{
+    double vir[6];
+    double e = orc_curey(s, pos, nu, ia, ib, seq, ku, ff, vir);
+    hes3[0] = hes3[1] = hes3[2] = 0.0;
+    for (int i = 0; i < nu; i++) {
+        double d[3];
+        orc_d3(s, pos, ia[i], ib[i], d);
+        for (int ax = 0; ax < 3; ax++) {
+            oj2 D[3];
+            oj_vec(d, ax, 1.0, D);
+            oj2 dr = oj_sub(oj_sqrt(oj_dot(D, D)), oj_c(seq[i]));
+            hes3[ax] += 2.0 * oj_scale(ku[i], oj_mul(dr, dr)).dd;   /* both ends alike */
+        }
+    }
+    return e;
+}
+
+/* Hessian sums of one dihedral term: Fourier series (nf > 0 entries of
+ * kphi/nphi/delphi) or harmonic k (tau - tau0)^2 with tau on the branch of
+ * tau_ref (the value the force used) */
+static void orc_dihedral_hes(const orc_system *s, const double *pos, int a, int b, int c, int d, const double *kp,
+                             const double *np, const double *dp, int harmonic, double kh, double t0, double tau_ref,
+                             double hes3[3])
+{
+    double b1[3], b2[3], b3[3];
+    orc_d3(s, pos, a, b, b1);
+    orc_d3(s, pos, b, c, b2);
+    orc_d3(s, pos, c, d, b3);
+    for (int site = 0; site < 4; site++)
+        for (int ax = 0; ax < 3; ax++) {
+            oj2 B1[3], B2[3], B3[3];
+            /* site k moves b_k by +1 and b_{k+1} by -1 */
+            oj_vec(b1, ax, site == 1 ? 1.0 : (site == 0 ? -1.0 : 0.0), B1);
+            oj_vec(b2, ax, site == 2 ? 1.0 : (site == 1 ? -1.0 : 0.0), B2);
+            oj_vec(b3, ax, site == 3 ? 1.0 : (site == 2 ? -1.0 : 0.0), B3);
+            oj2 th = oj_dihedral(B1, B2, B3), E = oj_c(0.0);
+            if (harmonic) {
+                oj2 dt = oj_sub(th, oj_c(t0));
+                dt.v = tau_ref - t0;
+                E = oj_scale(kh, oj_mul(dt, dt));
+            } else {
+                for (int m = 0; m < 6; m++) {
+                    if (kp[m] == 0.0) break;
+                    E = oj_add(E, oj_scale(kp[m], oj_cos(oj_sub(oj_scale(np[m], th), oj_c(dp[m])))));
+                }
+            }
+            hes3[ax] += E.dd;
+        }
+}
+
+/* ctorsion2.c: orc_ctorsion plus the Hessian diagonal sums */
+double orc_ctorsion2(const orc_system *s, const double *pos, int nt, const int *ta, const int *tb, const int *tc,
+                     const int *td, const double *kphi, const double *nphi, const double *delphi, double *ff,
+                     double hes3[3])
+{
+    double vir[6];
+    double e = orc_ctorsion(s, pos, nt, ta, tb, tc, td, kphi, nphi, delphi, ff, vir);
+    hes3[0] = hes3[1] = hes3[2] = 0.0;
+    for (int i = 0; i < nt; i++)
+        orc_dihedral_hes(s, pos, ta[i], tb[i], tc[i], td[i], kphi + 6 * i, nphi + 6 * i, delphi + 6 * i, 0, 0.0,
+                         0.0, 0.0, hes3);
+    return e;
+}
+
+/* cimproper2.c: orc_cimproper plus the Hessian diagonal sums */
+double orc_cimproper2(const orc_system *s, const double *pos, int ni, const int *ia, const int *ib, const int *ic,
+                      const int *id, const double *kimpr, const double *angeq, double *ff, double hes3[3])
+{
+    double vir[6];
+    double e = orc_cimproper(s, pos, ni, ia, ib, ic, id, kimpr, angeq, ff, vir);
+    hes3[0] = hes3[1] = hes3[2] = 0.0;
+    for (int i = 0; i < ni; i++) {
+        orc_dihedral g;
+        orc_dihedral_eval(s, pos, ia[i], ib[i], ic[i], id[i], &g);
+        orc_dihedral_hes(s, pos, ia[i], ib[i], ic[i], id[i], NULL, NULL, NULL, 1, kimpr[i], angeq[i], g.tau, hes3);
+    }
+    return e;
+}
+
 void orc_pbc_all(const orc_system *s, double *pos, int n)
 {
     double l[3] = {s->lx, s->ly, s->lz};

@@ -149,6 +149,29 @@ void orc_cnonbondnl(const orc_system *s, const double *pos, const int *count,
 double orc_cbond(const orc_system *s, const double *pos, int nb, const int *ia,
                  const int *ib, const double *req, const double *kb, double *ff,
                  double vir[6]);                             /* cbond.c:65-120 */
+/* Angle-type bonded terms, forces added, virial of the term type in vir[6]:
+ * cbend.c:63-170, curey.c:57-140, ctorsion.c:67-236 (6 Fourier entries per
+ * torsion, leading nonzero kphi used), cimproper.c:62-225.               */
+double orc_cbend(const orc_system *s, const double *pos, int nbend, const int *ia, const int *ib,
+                 const int *ic, const double *angeq, const double *kb, double *ff, double vir[6]);
+double orc_curey(const orc_system *s, const double *pos, int nu, const int *ia, const int *ib,
+                 const double *seq, const double *ku, double *ff, double vir[6]);
+double orc_ctorsion(const orc_system *s, const double *pos, int nt, const int *ta, const int *tb, const int *tc,
+                    const int *td, const double *kphi, const double *nphi, const double *delphi, double *ff,
+                    double vir[6]);
+double orc_cimproper(const orc_system *s, const double *pos, int ni, const int *ia, const int *ib, const int *ic,
+                     const int *id, const double *kimpr, const double *angeq, double *ff, double vir[6]);
+/* cbend2.c, curey2.c, ctorsion2.c, cimproper2.c (CONFIGT): as above, with
+ * the sums over the terms' sites of d2E/dx^2, d2E/dy^2, d2E/dz^2 in hes3. */
+double orc_cbend2(const orc_system *s, const double *pos, int nbend, const int *ia, const int *ib,
+                  const int *ic, const double *angeq, const double *kb, double *ff, double hes3[3]);
+double orc_curey2(const orc_system *s, const double *pos, int nu, const int *ia, const int *ib,
+                  const double *seq, const double *ku, double *ff, double hes3[3]);
+double orc_ctorsion2(const orc_system *s, const double *pos, int nt, const int *ta, const int *tb, const int *tc,
+                     const int *td, const double *kphi, const double *nphi, const double *delphi, double *ff,
+                     double hes3[3]);
+double orc_cimproper2(const orc_system *s, const double *pos, int ni, const int *ia, const int *ib, const int *ic,
+                      const int *id, const double *kimpr, const double *angeq, double *ff, double hes3[3]);
 void orc_pbc_all(const orc_system *s, double *pos, int n);  /* pbc_all.c:23-88 */
 double orc_kinet(int n, const double *mass, const double *vv, const double *uu,
                  double ken[6]);                             /* kinet.c:18-68 */

@@ -779,3 +779,120 @@ def test_c0_nve_fixture_is_the_oracle_run():
     rows = np.load(os.path.join(here, "c0_nve_oracle.npz"))["rows"]
     assert rows.shape == (1000, 5)
     assert np.array_equal(mk.run(15), rows[:15])
+
+
+# ---------------------------------------------------------------- angle-type bonded terms
+@pytest.fixture(scope="module", params=[False, True], ids=["charmm", "generic"])
+def angled(request):
+    s = system.chains_3k()
+    s.pos = system.jitter(s, 0.05)
+    return s, system.angle_terms(s, generic=request.param)
+
+
+def _ref_angles(R, s, at, which):
+    pos = np.ascontiguousarray(s.pos).ravel()
+    ff = np.zeros(3 * s.n)
+    v = np.zeros(6)
+    col = lambda a, k: np.ascontiguousarray(a[:, k], dtype=np.int32)
+    R.ref_bonded_box(s.n, *s.box)
+    if which == "bend":
+        e = R.ref_cbend(pos, len(at.bend), col(at.bend, 0), col(at.bend, 1), col(at.bend, 2), at.bend_eq,
+                        at.bend_k, ff, v)
+    elif which == "urey":
+        e = R.ref_curey(pos, len(at.urey), col(at.urey, 0), col(at.urey, 1), at.urey_s0, at.urey_k, ff, v)
+    elif which == "tors":
+        e = R.ref_ctorsion(pos, len(at.tors), col(at.tors, 0), col(at.tors, 1), col(at.tors, 2), col(at.tors, 3),
+                           np.ascontiguousarray(at.kphi).ravel(), np.ascontiguousarray(at.nphi).ravel(),
+                           np.ascontiguousarray(at.delphi).ravel(), ff, v)
+    else:
+        e = R.ref_cimproper(pos, len(at.impr), col(at.impr, 0), col(at.impr, 1), col(at.impr, 2),
+                            col(at.impr, 3), at.impr_k, at.impr_eq, ff, v)
+    return ff.reshape(-1, 3), e, v
+
+
+@needs_ref
+@pytest.mark.parametrize("which", ["bend", "urey", "tors", "impr"])
+def test_angle_terms_bit_exact_vs_reference(angled, which):
+    s, at = angled
+    R = pyoracle.ref_lib()
+    fr, er, vr = _ref_angles(R, s, at, which)
+    f, e, v = pyoracle.Oracle(s).angles(s.pos, at, (which,))
+    assert e[which] == er and np.array_equal(f, fr) and np.array_equal(v[which], vr), which
+    assert abs(er) > 0 and np.abs(fr).max() > 0
+
+
+@pytest.mark.parametrize("which", ["bend", "urey", "tors", "impr"])
+def test_angle_terms_forces_are_energy_gradient(angled, which):
+    s, at = angled
+    o = pyoracle.Oracle(s)
+    f, e, _ = o.angles(s.pos, at, (which,))
+    rng = np.random.default_rng(4)
+    h = 1e-6
+    for i in rng.choice(s.n, 12, replace=False):
+        for d in range(3):
+            p1 = s.pos.copy(); p1[i, d] += h
+            p2 = s.pos.copy(); p2[i, d] -= h
+            g = (o.angles(p1, at, (which,))[1][which] - o.angles(p2, at, (which,))[1][which]) / (2 * h)
+            assert abs(-g - f[i, d]) <= 2e-5 * max(1.0, abs(f[i, d])), (which, i, d, -g, f[i, d])
+    # translation invariance and zero net force
+    assert np.abs(f.sum(0)).max() < 1e-8 * np.abs(f).max()
+
+
+def _ref_angles2(R, s, at, which):
+    pos = np.ascontiguousarray(s.pos).ravel()
+    ff = np.zeros(3 * s.n)
+    h = np.zeros(3)
+    col = lambda a, k: np.ascontiguousarray(a[:, k], dtype=np.int32)
+    R.ref_ct_box(s.n, *s.box)
+    if which == "bend":
+        e = R.ref_cbend2(pos, len(at.bend), col(at.bend, 0), col(at.bend, 1), col(at.bend, 2), at.bend_eq,
+                         at.bend_k, ff, h)
+    elif which == "urey":
+        e = R.ref_curey2(pos, len(at.urey), col(at.urey, 0), col(at.urey, 1), at.urey_s0, at.urey_k, ff, h)
+    elif which == "tors":
+        e = R.ref_ctorsion2(pos, len(at.tors), col(at.tors, 0), col(at.tors, 1), col(at.tors, 2), col(at.tors, 3),
+                            np.ascontiguousarray(at.kphi).ravel(), np.ascontiguousarray(at.nphi).ravel(),
+                            np.ascontiguousarray(at.delphi).ravel(), ff, h)
+    else:
+        e = R.ref_cimproper2(pos, len(at.impr), col(at.impr, 0), col(at.impr, 1), col(at.impr, 2),
+                             col(at.impr, 3), at.impr_k, at.impr_eq, ff, h)
+    return ff.reshape(-1, 3), e, h
+
+
+@needs_ref
+@pytest.mark.parametrize("which", ["bend", "urey", "tors", "impr"])
+def test_angle_hessians_vs_reference_configt(angled, which):
+    """Hessian diagonal sums against the reference's own cbend2.c, curey2.c,
+    ctorsion2.c, cimproper2.c (analytic second derivatives, another route
+    than the oracle's): within 1e-9 relative; energies and forces as the
+    plain terms give them."""
+    s, at = angled
+    if not os.path.exists(pyoracle.REF_CT_SO):
+        pytest.skip("oracle/_ref/libttcref_ct.so not built")
+    fr, er, hr = _ref_angles2(pyoracle.ref_ct_lib(), s, at, which)
+    f, e, h = pyoracle.Oracle(s).angles(s.pos, at, (which,), hessian=True)
+    print(which, "hes oracle", h[which], "reference", hr)
+    assert abs(e[which] - er) <= 1e-12 * abs(er)
+    assert np.abs(f - fr).max() <= 1e-10 * np.abs(fr).max()
+    assert np.abs(h[which] - hr).max() <= 1e-9 * np.abs(hr).max()
+
+
+@pytest.mark.parametrize("which", ["bend", "urey", "tors", "impr"])
+def test_angle_hessians_are_force_derivatives(which):
+    """The Hessian diagonal sum equals the sum over coordinates of
+    -dF_i/dx_i by central differences of the oracle's forces."""
+    s = system.make_chain_system(30, 31.0, 32, seed=8)
+    s.pos = system.jitter(s, 0.05, seed=3)
+    at = system.angle_terms(s, generic=True)
+    o = pyoracle.Oracle(s)
+    _, _, h = o.angles(s.pos, at, (which,), hessian=True)
+    h = h[which]
+    eps = 1e-5
+    fd = np.zeros(3)
+    for i in range(s.n):
+        for d in range(3):
+            p1 = s.pos.copy(); p1[i, d] += eps
+            p2 = s.pos.copy(); p2[i, d] -= eps
+            fd[d] -= (o.angles(p1, at, (which,))[0][i, d] - o.angles(p2, at, (which,))[0][i, d]) / (2 * eps)
+    print(which, h, fd)
+    assert np.abs(h - fd).max() <= 1e-5 * np.abs(h).max()

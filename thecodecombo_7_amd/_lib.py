@@ -70,6 +70,12 @@ def lib():
     L.ttc_pair.argtypes = [vp, dp, C.c_int64, dp, dp]
     L.ttc_cbond.argtypes = [vp, dp, C.c_int64, dp, dp]
     L.ttc_forces.argtypes = [vp, dp, C.c_int64, dp]
+    L.ttc_set_angles.argtypes = [vp, C.c_int, ip, dp, dp, C.c_int, ip, dp, dp, C.c_int, ip, dp, dp, dp,
+                                 C.c_int, ip, dp, dp]
+    for f in ("ttc_cbend", "ttc_curey", "ttc_ctorsion", "ttc_cimproper", "ttc_cbend2", "ttc_curey2",
+              "ttc_ctorsion2", "ttc_cimproper2"):
+        getattr(L, f).argtypes = [vp, dp, C.c_int64, dp, dp]
+    L.ttc_bonded_energies.argtypes = [vp, dp]
     L.ttc_md_vverlet.argtypes = [vp, C.c_int, C.c_double, C.c_int, dp, C.c_int, ip, ip]
     L.ttc_kinet.argtypes = [vp, dp, dp]
     L.ttc_timing.argtypes = [vp, C.c_int, dp]

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = ["ttc_api.hip", "ttc_nblist.hip", "ttc_pair.hip", "ttc_pme.hip", "ttc_md.hip", "ttc_mc.hip",
-           "ttc_batch.hip"]
+           "ttc_batch.hip", "ttc_angles.hip"]
 LIB = os.path.join(HERE, "libttc.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

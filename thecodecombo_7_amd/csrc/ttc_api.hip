@@ -189,7 +189,8 @@ extern "C" int ttc_destroy(ttc_ctx *c) {
                     c->origb, c->inv, c->fpair, c->g4, c->cell_of, c->cell_cnt, c->cell_start, c->cell_acc,
                     c->cell_members, c->nbl, c->glen, c->status, c->ex_off, c->ex_part, c->ex_flag, c->ljtab,
                     c->b_off, c->b_part, c->b_r0, c->b_k, c->partials, c->ev, c->rows, c->fs, c->fl, c->fsb,
-                    c->flb, c->nhc, c->posfb, c->gcnt, c->snap, c->g4o, c->fixj, c->win, c->wlen};
+                    c->flb, c->nhc, c->posfb, c->gcnt, c->snap, c->g4o, c->fixj, c->win, c->wlen, c->ang_idx,
+                    c->ang_par, c->ang_tpar, c->ang_tf, c->ang_off, c->ang_ref};
     for (void *b : bufs)
         if (b) hipFree(b);
     ttc_pme_free(c);
@@ -593,7 +594,7 @@ extern "C" int ttc_pme_calc(ttc_ctx *c, double *ff, int64_t stride, double *ener
     int r0 = need_list(c);   // the spread walks the sites cell by cell
     if (r0) return r0;
     ttc_launch_spread(c);
-    ttc_launch_pme(c, true, true, false);
+    ttc_launch_pme(c, true, true, 0);
     ttc_launch_reduce(c, false);
     ttc_launch_combine(c, 2 | 4);
     int r = finish(c, ff, stride);
@@ -628,7 +629,7 @@ extern "C" int ttc_cnonbondnl(ttc_ctx *c, double *ff, int64_t stride, double *en
     if (r) return r;
     ttc_launch_pair(c, true, 0);
     ttc_launch_spread(c);
-    ttc_launch_pme(c, true, true, false);
+    ttc_launch_pme(c, true, true, 0);
     ttc_launch_reduce(c, false);
     ttc_launch_combine(c, 1 | 2 | 4);
     r = finish(c, ff, stride);
@@ -642,7 +643,7 @@ extern "C" int ttc_cnonbondnl(ttc_ctx *c, double *ff, int64_t stride, double *en
 
 extern "C" int ttc_cbond(ttc_ctx *c, double *ff, int64_t stride, double *ebond, double *vir6) {
     CHECK(c && c->topo, TTC_ERR_ARG, "ttc_cbond: topology not set");
-    ttc_launch_pme(c, true, false, true);
+    ttc_launch_pme(c, true, false, TTC_BT_BOND);
     ttc_launch_reduce(c, false);
     ttc_launch_combine(c, 2);
     int r = finish(c, ff, stride);
@@ -650,6 +651,171 @@ extern "C" int ttc_cbond(ttc_ctx *c, double *ff, int64_t stride, double *ebond, 
     if (ebond) *ebond = c->h_ev[EV_BOND];
     if (vir6) for (int i = 0; i < 6; i++) vir6[i] = c->h_ev[EV_BV0 + i];
     c->f4_valid = false;
+    return TTC_OK;
+}
+
+// ------------------------------------------------- angle-type bonded terms
+
+extern "C" int ttc_set_angles(ttc_ctx *c, int nbend, const int *bend_abc, const double *bend_eq,
+                              const double *bend_k, int nurey, const int *urey_ab, const double *urey_s0,
+                              const double *urey_k, int ntors, const int *tors_abcd, const double *kphi,
+                              const double *nphi, const double *delphi, int nimpr, const int *impr_abcd,
+                              const double *impr_k, const double *impr_eq) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "ttc_set_angles: topology not set");
+    CHECK(nbend >= 0 && nurey >= 0 && ntors >= 0 && nimpr >= 0, TTC_ERR_ARG, "ttc_set_angles: negative count");
+    CHECK((!nbend || (bend_abc && bend_eq && bend_k)) && (!nurey || (urey_ab && urey_s0 && urey_k)) &&
+              (!ntors || (tors_abcd && kphi && nphi && delphi)) && (!nimpr || (impr_abcd && impr_k && impr_eq)),
+          TTC_ERR_ARG, "ttc_set_angles: null array");
+    const int n = c->n;
+    const int cnt[4] = {nbend, nurey, ntors, nimpr}, ends[4] = {3, 2, 4, 4};
+    const int *sites[4] = {bend_abc, urey_ab, tors_abcd, impr_abcd};
+    int start[5], slot[5];
+    start[0] = 0;
+    slot[0] = 0;
+    for (int k = 0; k < 4; k++) {
+        int pad = (cnt[k] + 63) / 64 * 64;   // whole waves per kind
+        start[k + 1] = start[k] + pad;
+        slot[k + 1] = slot[k] + ends[k] * pad;
+    }
+    CHECK(slot[4] < (1 << 30), TTC_ERR_ARG, "ttc_set_angles: too many terms");
+    const int ntot = start[4];
+    std::vector<int4> idx(ntot, make_int4(-1, -1, -1, -1));
+    std::vector<double2> par(ntot, make_double2(0.0, 0.0));
+    std::vector<double> tpar((size_t)ntors * 18);
+    std::vector<std::vector<int>> per(n);   // slot | kind << 30 per site, term order
+    for (int k = 0; k < 4; k++)
+        for (int i = 0; i < cnt[k]; i++) {
+            int s[4] = {-1, -1, -1, -1};
+            for (int e = 0; e < ends[k]; e++) {
+                s[e] = sites[k][(size_t)ends[k] * i + e];
+                CHECK(s[e] >= 0 && s[e] < n, TTC_ERR_ARG, "ttc_set_angles: site index out of range");
+            }
+            for (int e = 0; e < ends[k]; e++)
+                for (int f = e + 1; f < ends[k]; f++)
+                    CHECK(s[e] != s[f], TTC_ERR_ARG, "ttc_set_angles: a term names one site twice");
+            idx[start[k] + i] = make_int4(s[0], s[1], s[2], s[3]);
+            if (k == 0) par[start[k] + i] = make_double2(bend_eq[i], bend_k[i]);
+            if (k == 1) par[start[k] + i] = make_double2(urey_s0[i], urey_k[i]);
+            if (k == 3) par[start[k] + i] = make_double2(impr_eq[i], impr_k[i]);
+            if (k == 2)
+                for (int m = 0; m < 6; m++) {
+                    tpar[(size_t)i * 18 + m] = kphi[(size_t)6 * i + m];
+                    tpar[(size_t)i * 18 + 6 + m] = nphi[(size_t)6 * i + m];
+                    tpar[(size_t)i * 18 + 12 + m] = delphi[(size_t)6 * i + m];
+                }
+            for (int e = 0; e < ends[k]; e++) per[s[e]].push_back((slot[k] + ends[k] * i + e) | (k << 30));
+        }
+    std::vector<int> off(n + 1, 0), ref;
+    for (int a = 0; a < n; a++) {
+        off[a + 1] = off[a] + (int)per[a].size();
+        ref.insert(ref.end(), per[a].begin(), per[a].end());
+    }
+    void *old[] = {c->ang_idx, c->ang_par, c->ang_tpar, c->ang_tf, c->ang_off, c->ang_ref};
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    for (void *b : old)
+        if (b) hipFree(b);
+    c->ang_idx = nullptr; c->ang_par = nullptr; c->ang_tpar = nullptr;
+    c->ang_tf = nullptr; c->ang_off = nullptr; c->ang_ref = nullptr;
+    for (int k = 0; k < 5; k++) c->ang_start[k] = 0, c->ang_slot[k] = 0;
+    c->nbend = c->nurey = c->ntors = c->nimpr = 0;
+    destroy_graphs(c);
+    c->f4_valid = false;
+    c->mts_valid = false;
+    if (ntot == 0) return TTC_OK;
+    // the reductions need one partial per angle block
+    int blocks = (ntot + 255) / 256;
+    if (blocks > c->nblk_max) {
+        hipFree(c->partials);
+        c->partials = nullptr;
+        c->nblk_max = blocks;
+        ALLOC(c->partials, (size_t)EV_N * c->nblk_max);
+        TTC_HIP(hipMemset(c->partials, 0, sizeof(double) * EV_N * c->nblk_max));
+        c->gen++;
+    }
+    ALLOC(c->ang_idx, ntot);
+    ALLOC(c->ang_par, ntot);
+    ALLOC(c->ang_tpar, std::max<size_t>(tpar.size(), 1));
+    ALLOC(c->ang_tf, slot[4]);
+    ALLOC(c->ang_off, n + 1);
+    ALLOC(c->ang_ref, std::max<size_t>(ref.size(), 1));
+    TTC_HIP(hipMemcpy(c->ang_idx, idx.data(), sizeof(int4) * ntot, hipMemcpyHostToDevice));
+    TTC_HIP(hipMemcpy(c->ang_par, par.data(), sizeof(double2) * ntot, hipMemcpyHostToDevice));
+    if (!tpar.empty()) TTC_HIP(hipMemcpy(c->ang_tpar, tpar.data(), sizeof(double) * tpar.size(), hipMemcpyHostToDevice));
+    TTC_HIP(hipMemset(c->ang_tf, 0, sizeof(double4) * slot[4]));
+    TTC_HIP(hipMemcpy(c->ang_off, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    if (!ref.empty()) TTC_HIP(hipMemcpy(c->ang_ref, ref.data(), sizeof(int) * ref.size(), hipMemcpyHostToDevice));
+    for (int k = 0; k < 5; k++) c->ang_start[k] = start[k], c->ang_slot[k] = slot[k];
+    c->nbend = nbend; c->nurey = nurey; c->ntors = ntors; c->nimpr = nimpr;
+    return TTC_OK;
+}
+
+// forces, energies (bend, urey, tors, impr), virial and (hes3 != null)
+// Hessian diagonal of the angle-type kinds in bterms
+static int angle_call(ttc_ctx *c, int bterms, double *ff, int64_t stride, double *e4, double *vir6, double *hes3) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "ttc angle terms: topology not set");
+    if (hes3) {
+        int r = ttc_pme_theta(c);   // the Hessian gather variant stages theta even without PME
+        if (r) return r;
+        ttc_launch_pme_hes(c, false, bterms);
+    } else {
+        ttc_launch_pme(c, true, false, bterms);
+    }
+    ttc_launch_reduce(c, false);
+    ttc_launch_combine(c, 2);
+    int r = finish(c, ff, stride);
+    if (r) return r;
+    bool any = c->ang_start[4] != 0;
+    for (int k = 0; k < 4; k++)
+        if (e4 && ((bterms >> (k + 1)) & 1)) e4[k] = any ? c->h_ev[EV_EBEND + k] : 0.0;
+    if (vir6) for (int i = 0; i < 6; i++) vir6[i] = any ? c->h_ev[EV_AV0 + i] : 0.0;
+    if (hes3) for (int d = 0; d < 3; d++) hes3[d] = any ? c->h_ev[EV_AHX + d] : 0.0;
+    c->f4_valid = false;
+    return TTC_OK;
+}
+
+static int angle_kind(ttc_ctx *c, int bt, double *ff, int64_t stride, double *e, double *vir6, double *hes3) {
+    double e4[4] = {0.0, 0.0, 0.0, 0.0};
+    int r = angle_call(c, bt, ff, stride, e4, vir6, hes3);
+    if (r) return r;
+    int k = bt == TTC_BT_BEND ? 0 : bt == TTC_BT_UREY ? 1 : bt == TTC_BT_TORS ? 2 : 3;
+    if (e) *e = e4[k];
+    return TTC_OK;
+}
+
+extern "C" int ttc_cbend(ttc_ctx *c, double *ff, int64_t stride, double *ebend, double *vir6) {
+    return angle_kind(c, TTC_BT_BEND, ff, stride, ebend, vir6, nullptr);
+}
+extern "C" int ttc_curey(ttc_ctx *c, double *ff, int64_t stride, double *eurey, double *vir6) {
+    return angle_kind(c, TTC_BT_UREY, ff, stride, eurey, vir6, nullptr);
+}
+extern "C" int ttc_ctorsion(ttc_ctx *c, double *ff, int64_t stride, double *etors, double *vir6) {
+    return angle_kind(c, TTC_BT_TORS, ff, stride, etors, vir6, nullptr);
+}
+extern "C" int ttc_cimproper(ttc_ctx *c, double *ff, int64_t stride, double *eimpr, double *vir6) {
+    return angle_kind(c, TTC_BT_IMPR, ff, stride, eimpr, vir6, nullptr);
+}
+extern "C" int ttc_cbend2(ttc_ctx *c, double *ff, int64_t stride, double *ebend, double *hes3) {
+    CHECK(hes3, TTC_ERR_ARG, "ttc_cbend2: null hes3");
+    return angle_kind(c, TTC_BT_BEND, ff, stride, ebend, nullptr, hes3);
+}
+extern "C" int ttc_curey2(ttc_ctx *c, double *ff, int64_t stride, double *eurey, double *hes3) {
+    CHECK(hes3, TTC_ERR_ARG, "ttc_curey2: null hes3");
+    return angle_kind(c, TTC_BT_UREY, ff, stride, eurey, nullptr, hes3);
+}
+extern "C" int ttc_ctorsion2(ttc_ctx *c, double *ff, int64_t stride, double *etors, double *hes3) {
+    CHECK(hes3, TTC_ERR_ARG, "ttc_ctorsion2: null hes3");
+    return angle_kind(c, TTC_BT_TORS, ff, stride, etors, nullptr, hes3);
+}
+extern "C" int ttc_cimproper2(ttc_ctx *c, double *ff, int64_t stride, double *eimpr, double *hes3) {
+    CHECK(hes3, TTC_ERR_ARG, "ttc_cimproper2: null hes3");
+    return angle_kind(c, TTC_BT_IMPR, ff, stride, eimpr, nullptr, hes3);
+}
+
+extern "C" int ttc_bonded_energies(ttc_ctx *c, double *e5) {
+    CHECK(c && e5, TTC_ERR_ARG, "ttc_bonded_energies: bad argument");
+    bool any = c->ang_start[4] != 0;
+    e5[0] = c->h_ev[EV_BOND];
+    for (int k = 0; k < 4; k++) e5[1 + k] = any ? c->h_ev[EV_EBEND + k] : 0.0;
     return TTC_OK;
 }
 
@@ -662,7 +828,7 @@ extern "C" int ttc_cnonbondnl2(ttc_ctx *c, double *ff, int64_t stride, double *e
     if (r) return r;
     ttc_launch_pair_hes(c);
     ttc_launch_spread(c);
-    ttc_launch_pme_hes(c, true, false);
+    ttc_launch_pme_hes(c, true, 0);
     ttc_launch_reduce(c, false);
     ttc_launch_combine(c, 1 | 2 | 4);
     r = finish(c, ff, stride);
@@ -677,7 +843,7 @@ extern "C" int ttc_cbond2(ttc_ctx *c, double *ff, int64_t stride, double *ebond,
     CHECK(c && c->topo, TTC_ERR_ARG, "ttc_cbond2: topology not set");
     int r = ttc_pme_theta(c);   // the Hessian gather variant stages theta even without PME
     if (r) return r;
-    ttc_launch_pme_hes(c, false, true);
+    ttc_launch_pme_hes(c, false, TTC_BT_BOND);
     ttc_launch_reduce(c, false);
     ttc_launch_combine(c, 2);
     r = finish(c, ff, stride);
@@ -695,7 +861,7 @@ extern "C" int ttc_configtemp(ttc_ctx *c, double *num, double *den, double *T, d
     if (r) return r;
     ttc_launch_pair_hes(c);
     ttc_launch_spread(c);
-    ttc_launch_pme_hes(c, true, true);
+    ttc_launch_pme_hes(c, true, TTC_BT_ALL);
     ttc_launch_reduce(c, false);
     ttc_launch_combine(c, 7);
     TTC_HIP(hipGetLastError());
@@ -706,7 +872,8 @@ extern "C" int ttc_configtemp(ttc_ctx *c, double *num, double *den, double *T, d
     double f2 = 0.0;
     for (const double4 &v : f) f2 += v.x * v.x + v.y * v.y + v.z * v.z;
     double h[3];
-    for (int d = 0; d < 3; d++) h[d] = c->h_ev[EV_PHX + d] + c->h_ev[EV_KHX + d] + c->h_ev[EV_BHX + d];
+    for (int d = 0; d < 3; d++)
+        h[d] = c->h_ev[EV_PHX + d] + c->h_ev[EV_KHX + d] + c->h_ev[EV_BHX + d] + (c->ang_start[4] ? c->h_ev[EV_AHX + d] : 0.0);
     double hr = h[0] + h[1] + h[2];
     if (num) *num = f2;
     if (den) *den = hr;
@@ -720,7 +887,7 @@ extern "C" int ttc_configtemp(ttc_ctx *c, double *num, double *den, double *T, d
 static void launch_forces(ttc_ctx *c, bool energy) {
     ttc_launch_pair(c, energy, 0);
     ttc_launch_spread(c);
-    ttc_launch_pme(c, energy, true, true);
+    ttc_launch_pme(c, energy, true, TTC_BT_ALL);
     ttc_launch_combine(c, 7);
     if (energy) ttc_launch_reduce(c, false);
 }
@@ -1091,14 +1258,14 @@ extern "C" int ttc_window_stats(ttc_ctx *c, int *max_window, int *window_capacit
 // ------------------------------------------------------------------ RESPA
 
 static void launch_force_short(ttc_ctx *c, bool energy) {
-    ttc_launch_pme(c, energy, false, true);   // bonds only
+    ttc_launch_pme(c, energy, false, TTC_BT_SHORT);   // force_short.c: bonds, bends, Urey-Bradley
     ttc_launch_combine_to(c, 2, c->fs);
 }
 
 static void launch_force_long(ttc_ctx *c, bool energy) {
     ttc_launch_pair(c, energy, 0);
     ttc_launch_spread(c);
-    ttc_launch_pme(c, energy, true, false);   // PME only
+    ttc_launch_pme(c, energy, true, TTC_BT_LONG);   // force_long.c: PME, torsions, impropers
     ttc_launch_combine_to(c, 1 | 2 | 4, c->fl);
 }
 
@@ -1306,7 +1473,7 @@ void ttc_launch_nhc_step(ttc_ctx *c, bool energy) {
     ttc_side_fork(c, 1);
     {
         TtcOnSide side(c);
-        ttc_launch_gather(c, energy, true, true, false);
+        ttc_launch_gather(c, energy, true, TTC_BT_ALL, false);
     }
     ttc_launch_pair(c, energy, 0);
     ttc_side_join(c);
@@ -1368,7 +1535,7 @@ extern "C" int ttc_enbnd_mc(ttc_ctx *c, int lb, int ub, double *energy8) {
     if (r) return r;
     ttc_launch_pair_range(c, lb, ub);
     ttc_launch_spread(c);
-    ttc_launch_pme(c, true, true, false);
+    ttc_launch_pme(c, true, true, 0);
     ttc_launch_reduce(c, false);
     r = finish(c, nullptr, 0);
     if (r) return r;

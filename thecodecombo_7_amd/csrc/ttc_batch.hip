@@ -49,6 +49,7 @@ int ttc_bstage_fft(const std::vector<ttc_ctx *> &r, TtcStage *st /* [5] */);
 int ttc_bstage_gather(const std::vector<ttc_ctx *> &r, TtcStage *st);
 int ttc_bstage_rebuild(const std::vector<ttc_ctx *> &r, TtcStage *st /* [3] */);
 int ttc_bstage_pair(const std::vector<ttc_ctx *> &r, TtcStage *st);
+int ttc_bstage_angles(const std::vector<ttc_ctx *> &r, TtcStage *st);
 void ttc_blaunch_kinet(const TtcStage &st, hipStream_t s);
 void ttc_blaunch_nhc(const TtcStage &st, hipStream_t s);
 void ttc_blaunch_vscale(const TtcStage &st, hipStream_t s);
@@ -59,6 +60,7 @@ void ttc_blaunch_fft(const TtcStage *st, bool energy, hipStream_t s);
 void ttc_blaunch_gather(const TtcStage &st, bool energy, hipStream_t s);
 void ttc_blaunch_rebuild(const TtcStage *st, hipStream_t s);
 void ttc_blaunch_pair(const TtcStage &st, bool energy, hipStream_t s);
+void ttc_blaunch_angles(const TtcStage &st, bool energy, hipStream_t s);
 
 int ttc_stage_upload(TtcStage *st, const void *packs, size_t pack_bytes, int nrep, const std::vector<int> &blocks,
                      size_t shm) {
@@ -83,7 +85,7 @@ void ttc_stage_free(TtcStage *st) {
 
 // stages of the batched NHC step
 enum { BS_KINET = 0, BS_NHC, BS_VSCALE, BS_DRIFT, BS_FFT, BS_REBUILD = BS_FFT + 5, BS_GATHER = BS_REBUILD + 3,
-       BS_PAIR, BS_KICK, BS_REDUCE, BS_N };
+       BS_PAIR, BS_KICK, BS_REDUCE, BS_ANGLES, BS_N };
 
 struct ttc_batch {
     std::vector<ttc_ctx *> r;
@@ -209,6 +211,7 @@ int build_stages(ttc_batch *b) {
     if (!r) r = ttc_bstage_pair(b->r, &b->st[BS_PAIR]);
     if (!r) r = ttc_bstage_kick(b->r, &b->st[BS_KICK]);
     if (!r) r = ttc_bstage_reduce(b->r, &b->st[BS_REDUCE]);
+    if (!r) r = ttc_bstage_angles(b->r, &b->st[BS_ANGLES]);
     if (r) { drop_stages(b); return r; }
     b->st_valid = true;
     return TTC_OK;
@@ -236,6 +239,7 @@ int capture_batched(ttc_batch *b, bool energy, hipGraphExec_t *out) {
         ttc_blaunch_rebuild(&st[BS_REBUILD], m);
         if (e == hipSuccess) e = hipEventRecord(b->bev[1], m);
         if (e == hipSuccess) e = hipStreamWaitEvent(sd, b->bev[1], 0);
+        ttc_blaunch_angles(st[BS_ANGLES], en, sd);   // replicas with angle-type terms
         ttc_blaunch_gather(st[BS_GATHER], en, sd);
         ttc_blaunch_pair(st[BS_PAIR], en, m);
         if (e == hipSuccess) e = hipEventRecord(b->bev[2], sd);

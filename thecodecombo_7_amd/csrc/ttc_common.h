@@ -75,8 +75,25 @@ enum {
     EV_PHX, EV_PHY, EV_PHZ,                           // pair terms
     EV_KHX, EV_KHY, EV_KHZ,                           // reciprocal energy
     EV_BHX, EV_BHY, EV_BHZ,                           // bonds
+    // angle-type terms: energies per kind, virial and Hessian diagonal of
+    // the kinds a launch evaluated
+    EV_EBEND, EV_EUREY, EV_ETORS, EV_EIMPR,
+    EV_AV0, EV_AV1, EV_AV2, EV_AV3, EV_AV4, EV_AV5,
+    EV_AHX, EV_AHY, EV_AHZ,
     EV_N
 };
+
+// bonded-term selection of a gather launch: bonds and the angle-type kinds
+// (cbend.c, curey.c, ctorsion.c, cimproper.c); force_short.c evaluates the
+// first three, force_long.c the last two
+#define TTC_BT_BOND 1
+#define TTC_BT_BEND 2
+#define TTC_BT_UREY 4
+#define TTC_BT_TORS 8
+#define TTC_BT_IMPR 16
+#define TTC_BT_ALL 31
+#define TTC_BT_SHORT (TTC_BT_BOND | TTC_BT_BEND | TTC_BT_UREY)
+#define TTC_BT_LONG (TTC_BT_TORS | TTC_BT_IMPR)
 
 struct FftPlan {
     int n = 0;
@@ -169,6 +186,23 @@ struct ttc_ctx {
     // kernel of the reciprocal energy near the origin (CONFIGT Hessian)
     float *theta_near = nullptr;
 
+    // angle-type bonded terms, one thread per term, kinds in this order
+    // (bend, urey, tors, impr), each kind's range padded to whole waves:
+    // ang_start[k] is kind k's first term (ang_start[4] the padded total),
+    // ang_slot[k] its first force slot (3, 2, 4, 4 slots per term: one per
+    // site); ang_tf[slot] holds a term's force on one of its sites, which
+    // the gather adds per site through the CSR ang_off / ang_ref
+    // (slot | kind << 30) in the original site order
+    int nbend = 0, nurey = 0, ntors = 0, nimpr = 0;
+    int ang_start[5] = {0, 0, 0, 0, 0};
+    int ang_slot[5] = {0, 0, 0, 0, 0};
+    int4 *ang_idx = nullptr;        // [ang_start[4]] sites (x < 0: padding)
+    double2 *ang_par = nullptr;     // [ang_start[4]] equilibrium, force constant
+    double *ang_tpar = nullptr;     // [ntors][18] kphi[6], nphi[6], delphi[6]
+    double4 *ang_tf = nullptr;      // [ang_slot[4]]
+    int *ang_off = nullptr;         // [n + 1]
+    int *ang_ref = nullptr;
+
     // reductions: partials[slot * nblk_max + block]
     double *partials = nullptr;
     int nblk_max = 0;
@@ -253,11 +287,16 @@ int ttc_pme_setup(ttc_ctx *c);
 int ttc_pme_attrs(ttc_ctx *c);   // LDS limits of the spread kernel (needs the cell geometry)
 void ttc_pme_free(ttc_ctx *c);
 // gather: pme -> PME forces, bonds -> bond forces (either may be off)
-void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, bool bonds);
+void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, int bterms);   // bterms: TTC_BT_* mask
 // the two halves of ttc_launch_pme: the FFT convolution of the charge grid
 // (charge grid -> qr) and the gather (qr + positions -> g4)
 void ttc_launch_fft(ttc_ctx *c, bool energy);
-void ttc_launch_gather(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes);
+void ttc_launch_gather(ttc_ctx *c, bool energy, bool pme, int bterms, bool hes);
+// angle-type terms of the kinds in bterms (TTC_BT_BEND..IMPR) at pos4 (sorted
+// order, inv map) or, with snap, at the drift's snapshot (original order);
+// the gather that follows on the same stream adds their forces
+void ttc_launch_angles(ttc_ctx *c, bool energy, bool hes, int bterms, bool snap);
+int ttc_angle_blocks(const ttc_ctx *c);
 // captured MD steps: side stream waits for the work queued on the main
 // stream so far (ttc_side_fork, event k), main waits for the side stream
 // (ttc_side_join)
@@ -271,7 +310,7 @@ struct TtcOnSide {
     ~TtcOnSide() { c->stream = saved; }
 };
 // energies + forces + Hessian diagonal (CONFIGT); needs ttc_pme_theta first
-void ttc_launch_pme_hes(ttc_ctx *c, bool pme, bool bonds);
+void ttc_launch_pme_hes(ttc_ctx *c, bool pme, int bterms);
 int ttc_pme_theta(ttc_ctx *c);   // fills theta_near (once per context)
 int ttc_pme_conv_blocks(ttc_ctx *c);
 int ttc_gather_blocks(ttc_ctx *c);

@@ -128,7 +128,8 @@ struct ReduceArgs {
 
 // one workgroup: every slot summed over its producer's blocks in a fixed
 // order; optionally appends an energy row
-// {step, kinet, bond, nbond, nbonds, coulomb, kewald, rewald, iewald}
+// {step, kinet, bond, nbond, nbonds, coulomb, kewald, rewald, iewald}, where
+// bond is bond + bend + urey + tors + impr
 __device__ __forceinline__ void reduce_body(const ReduceArgs &ra, const double *partials, int maxb, double *ev,
                                             double e_self, double fan, double *rows, int rows_cap, int *status,
                                             int append) {
@@ -158,7 +159,8 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs &ra, const double *
                 double *row = rows + (size_t)r * ROW_LEN;
                 row[0] = (double)status[ST_STEP];
                 row[1] = ev[EV_KIN];
-                row[2] = ev[EV_BOND];
+                // calcvalue.c:44-53: the bonded part of potens
+                row[2] = ev[EV_BOND] + ev[EV_EBEND] + ev[EV_EUREY] + ev[EV_ETORS] + ev[EV_EIMPR];
                 row[3] = nbond;
                 row[4] = nbond - ev[EV_LJS];
                 row[5] = ev[EV_REAL] + ktot - ev[EV_INTRA];
@@ -244,6 +246,7 @@ static ReducePack reduce_pack(ttc_ctx *c, bool append_row) {
     for (int i = EV_FSX; i <= EV_FSZ; i++) ra.nb[i] = ttc_gather_blocks(c);
     for (int i = EV_PHX; i <= EV_PHZ; i++) ra.nb[i] = ttc_pair_blocks(c);
     for (int i = EV_KHX; i <= EV_BHZ; i++) ra.nb[i] = ttc_gather_blocks(c);
+    for (int i = EV_EBEND; i <= EV_AHZ; i++) ra.nb[i] = ttc_angle_blocks(c);
     double fan = 0.5 * TTC_QFACTOR / c->p.eps_rf;
     return ReducePack{ra, c->partials, c->nblk_max, c->ev, c->e_self, fan, c->rows, c->rows_cap, c->status,
                       append_row ? 1 : 0};

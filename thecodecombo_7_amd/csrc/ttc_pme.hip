@@ -670,7 +670,10 @@ __device__ __forceinline__ void gather_body(int bid, int n, const double4 *__res
                                             const int *__restrict__ b_off, const int *__restrict__ b_part,
                                             const double *__restrict__ b_r0, const double *__restrict__ b_k,
                                             double4 *__restrict__ g4, double *partials, int maxb, int do_pme,
-                                            int do_bonds, const float *__restrict__ theta_near) {
+                                            int do_bonds, const float *__restrict__ theta_near,
+                                            const int *__restrict__ a_off = nullptr,
+                                            const int *__restrict__ a_ref = nullptr,
+                                            const double4 *__restrict__ a_tf = nullptr, int amask = 0) {
     constexpr int ND = 2 * P - 1;   // grid offsets -(P-1) .. P-1
     __shared__ float sth[HES ? ND * ND * ND : 1];
     if (HES) {
@@ -813,6 +816,17 @@ __device__ __forceinline__ void gather_body(int bid, int n, const double4 *__res
                 v3 += 0.5 * fzb * dx; v4 += 0.5 * fzb * dy; v5 += 0.5 * fzb * dz;
             }
         }
+        // angle-type terms of this site (their kernel ran just before):
+        // the term forces on this site, in the fixed CSR order
+        if (amask) {
+            for (int k = a_off[o]; k < a_off[o + 1]; k++) {
+                unsigned r = (unsigned)a_ref[k];   // slot | kind << 30 (kinds 2, 3 set the sign bit)
+                if ((amask >> (r >> 30)) & 1) {
+                    double4 f = a_tf[r & 0x3FFFFFFFu];
+                    bx += f.x; by += f.y; bz += f.z;
+                }
+            }
+        }
         g4[s] = make_double4(px + bx, py + by, pz + bz, 0.0);
     }
     // per-block sums (PME force sum always; bond energy/virial on demand)
@@ -845,19 +859,24 @@ struct GatherPack {
     double *partials;
     int maxb, do_pme, do_bonds;
     const float *theta_near;
+    const int *a_off, *a_ref;    // angle-type term forces per site (amask: kinds, bit k = kind k)
+    const double4 *a_tf;
+    int amask;
 };
 
 template <int P, bool ENERGY, bool HES = false>
 __global__ void __launch_bounds__(256) k_gather(GatherPack a) {
     gather_body<P, ENERGY, HES>(blockIdx.x, a.n, a.pos4, a.orig, a.inv, a.qr, a.gc, a.b_off, a.b_part, a.b_r0, a.b_k,
-                                a.g4, a.partials, a.maxb, a.do_pme, a.do_bonds, a.theta_near);
+                                a.g4, a.partials, a.maxb, a.do_pme, a.do_bonds, a.theta_near, a.a_off, a.a_ref,
+                                a.a_tf, a.amask);
 }
 template <int P, bool ENERGY>
 __global__ void __launch_bounds__(256) kb_gather(const GatherPack *pk, const TtcBmap *map) {
     const TtcBmap m = map[blockIdx.x];
     const GatherPack a = pk[m.rep];
     gather_body<P, ENERGY, false>(m.blk, a.n, a.pos4, a.orig, a.inv, a.qr, a.gc, a.b_off, a.b_part, a.b_r0, a.b_k,
-                                  a.g4, a.partials, a.maxb, a.do_pme, a.do_bonds, a.theta_near);
+                                  a.g4, a.partials, a.maxb, a.do_pme, a.do_bonds, a.theta_near, a.a_off, a.a_ref,
+                                  a.a_tf, a.amask);
 }
 
 FftArgs fft_args(const ttc_ctx *c, int axis) {
@@ -1154,7 +1173,12 @@ void ttc_launch_fft(ttc_ctx *c, bool energy) {
 #undef FFTL
 }
 
-void ttc_launch_gather(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes) {
+// the gather's angle-kind mask (bit k = kind k) from a TTC_BT_* selection
+static int angle_mask(const ttc_ctx *c, int bterms) { return c->ang_start[4] ? (bterms >> 1) & 15 : 0; }
+
+void ttc_launch_gather(ttc_ctx *c, bool energy, bool pme, int bterms, bool hes) {
+    int am = angle_mask(c, bterms);
+    if (am) ttc_launch_angles(c, energy, hes, bterms, false);
     double lx = c->p.lx, ly = c->p.ly, lz = c->p.lz;
     int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2];
     GatherConst gc;
@@ -1164,7 +1188,8 @@ void ttc_launch_gather(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes) 
     gc.qscale = TTC_QFACTOR / c->p.eps_rf;
     int blocks = ttc_gather_blocks(c);
     GatherPack a{c->n, c->pos4, c->orig, c->inv, c->qr, gc, c->b_off, c->b_part, c->b_r0, c->b_k, c->g4,
-                 c->partials, c->nblk_max, pme ? 1 : 0, bonds ? 1 : 0, c->theta_near};
+                 c->partials, c->nblk_max, pme ? 1 : 0, (bterms & TTC_BT_BOND) ? 1 : 0, c->theta_near,
+                 c->ang_off, c->ang_ref, c->ang_tf, am};
 #define GATHER(P, E, H) k_gather<P, E, H><<<blocks, 256, 0, c->stream>>>(a)
     if (c->order == 4) {
         if (hes) GATHER(4, true, true); else if (energy) GATHER(4, true, false); else GATHER(4, false, false);
@@ -1186,6 +1211,8 @@ __global__ void k_snap(int n, const double4 *__restrict__ pos4, const int *__res
 void ttc_launch_snap(ttc_ctx *c) { k_snap<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, c->pos4, c->orig, c->snap); }
 
 void ttc_launch_gather_orig(ttc_ctx *c, bool energy) {
+    int am = angle_mask(c, TTC_BT_ALL);
+    if (am) ttc_launch_angles(c, energy, false, TTC_BT_ALL, true);
     double lx = c->p.lx, ly = c->p.ly, lz = c->p.lz;
     GatherConst gc;
     gc.lx = lx; gc.ly = ly; gc.lz = lz;
@@ -1194,21 +1221,21 @@ void ttc_launch_gather_orig(ttc_ctx *c, bool energy) {
     gc.qscale = TTC_QFACTOR / c->p.eps_rf;
     int blocks = ttc_gather_blocks(c);
     GatherPack a{c->n, c->snap, nullptr, nullptr, c->qr, gc, c->b_off, c->b_part, c->b_r0, c->b_k, c->g4o,
-                 c->partials, c->nblk_max, 1, 1, c->theta_near};
+                 c->partials, c->nblk_max, 1, 1, c->theta_near, c->ang_off, c->ang_ref, c->ang_tf, am};
 #define GATHER(P, E) k_gather<P, E, false><<<blocks, 256, 0, c->stream>>>(a)
     if (c->order == 4) { if (energy) GATHER(4, true); else GATHER(4, false); }
     else { if (energy) GATHER(6, true); else GATHER(6, false); }
 #undef GATHER
 }
 
-static void launch_pme_impl(ttc_ctx *c, bool energy, bool pme, bool bonds, bool hes) {
+static void launch_pme_impl(ttc_ctx *c, bool energy, bool pme, int bterms, bool hes) {
     if (pme) ttc_launch_fft(c, energy);
-    ttc_launch_gather(c, energy, pme, bonds, hes);
+    ttc_launch_gather(c, energy, pme, bterms, hes);
 }
 
-void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, bool bonds) { launch_pme_impl(c, energy, pme, bonds, false); }
+void ttc_launch_pme(ttc_ctx *c, bool energy, bool pme, int bterms) { launch_pme_impl(c, energy, pme, bterms, false); }
 
-void ttc_launch_pme_hes(ttc_ctx *c, bool pme, bool bonds) { launch_pme_impl(c, true, pme, bonds, true); }
+void ttc_launch_pme_hes(ttc_ctx *c, bool pme, int bterms) { launch_pme_impl(c, true, pme, bterms, true); }
 
 // theta near the origin: the FFT/convolution pipeline run on a unit charge
 // at grid point 0 leaves theta(m) in qr (the kernel the gather convolves
@@ -1303,7 +1330,8 @@ int ttc_bstage_gather(const std::vector<ttc_ctx *> &r, TtcStage *st) {
         gc.K0 = c->K[0]; gc.K1 = c->K[1]; gc.K2 = c->K[2];
         gc.qscale = TTC_QFACTOR / c->p.eps_rf;
         pk.push_back(GatherPack{c->n, c->pos4, c->orig, c->inv, c->qr, gc, c->b_off, c->b_part, c->b_r0, c->b_k, c->g4,
-                                c->partials, c->nblk_max, 1, 1, c->theta_near});
+                                c->partials, c->nblk_max, 1, 1, c->theta_near, c->ang_off, c->ang_ref, c->ang_tf,
+                                angle_mask(c, TTC_BT_ALL)});
         nb.push_back(ttc_gather_blocks(c));
     }
     return ttc_stage_make(st, pk, nb, 0);

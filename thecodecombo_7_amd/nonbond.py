@@ -131,6 +131,32 @@ class Box:
         T.check(self.L.ttc_cbond(self.h, T.dptr(ff), 3, T.dptr(eb), T.dptr(vir)))
         return ff, float(eb[0]), vir
 
+    def set_angles(self, at):
+        """Angle-type terms (system.AngleTerms) on the device."""
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        self._angles = [i32(at.bend), f64(at.bend_eq), f64(at.bend_k), i32(at.urey), f64(at.urey_s0),
+                        f64(at.urey_k), i32(at.tors), f64(at.kphi), f64(at.nphi), f64(at.delphi),
+                        i32(at.impr), f64(at.impr_k), f64(at.impr_eq)]
+        a = self._angles
+        T.check(self.L.ttc_set_angles(self.h, len(at.bend), T.iptr(a[0]), T.dptr(a[1]), T.dptr(a[2]),
+                                      len(at.urey), T.iptr(a[3]), T.dptr(a[4]), T.dptr(a[5]),
+                                      len(at.tors), T.iptr(a[6]), T.dptr(a[7]), T.dptr(a[8]), T.dptr(a[9]),
+                                      len(at.impr), T.iptr(a[10]), T.dptr(a[11]), T.dptr(a[12])))
+
+    def angle_kind(self, kind: str, hessian: bool = False):
+        """cbend / curey / ctorsion / cimproper (or their CONFIGT variants
+        with hessian): forces (n,3), energy, virial (or Hessian sums)."""
+        name = {"bend": "ttc_cbend", "urey": "ttc_curey", "tors": "ttc_ctorsion", "impr": "ttc_cimproper"}[kind]
+        ff = np.zeros((self.n, 3)); e = np.zeros(1); x = np.zeros(3 if hessian else 6)
+        T.check(getattr(self.L, name + ("2" if hessian else ""))(self.h, T.dptr(ff), 3, T.dptr(e), T.dptr(x)))
+        return ff, float(e[0]), x
+
+    def bonded_energies(self):
+        e = np.zeros(5)
+        T.check(self.L.ttc_bonded_energies(self.h, T.dptr(e)))
+        return dict(zip(["bond", "bend", "urey", "tors", "impr"], e))
+
     def forces(self):
         ff = np.zeros((self.n, 3)); en = np.zeros(9)
         T.check(self.L.ttc_forces(self.h, T.dptr(ff), 3, T.dptr(en)))

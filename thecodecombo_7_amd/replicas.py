@@ -17,14 +17,17 @@ KB = 1.380662e-23
 
 
 class ReplicaBatch:
-    def __init__(self, systems, temps, tau: float = 0.1, M: int = 2, seed: int = 1):
+    def __init__(self, systems, temps, tau: float = 0.1, M: int = 2, seed: int = 1, angles=None):
+        """angles: optional system.AngleTerms per replica (None entries: none)."""
         L = T.lib()
         self.L = L
         self.boxes = []
         self.h = None
         try:
-            for s, t in zip(systems, temps):
+            for r, (s, t) in enumerate(zip(systems, temps)):
                 b = Box(s)
+                if angles is not None and angles[r] is not None:
+                    b.set_angles(angles[r])
                 nfree = 3.0 * s.n - 3.0
                 b.set_nhc(float(t), nfree * KB * float(t) * 1e-4 * tau * tau, M=M, nfree=nfree)
                 b.nblist()

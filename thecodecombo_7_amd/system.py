@@ -170,6 +170,89 @@ def make_chain_system(nchains: int, box_len: float, grid: int, *, seed: int = 12
         grid=(grid, grid, grid), temperature=temperature, name=name, meta=self_meta)
 
 
+@dataclass
+class AngleTerms:
+    """Angle-type bonded terms of a chain system (read_topology.c's bend,
+    urey, tors and impr arrays): bends (a,b,c) with angeq [rad] and krbend,
+    Urey-Bradley 1-3 pairs (a,b) with Seq and k, torsions (a,b,c,d) with six
+    Fourier entries each (kphi, nphi, delphi; a zero kphi ends the series),
+    impropers (a,b,c,d) with kimpr and angeq."""
+    bend: np.ndarray           # (nbend,3) int32
+    bend_eq: np.ndarray
+    bend_k: np.ndarray
+    urey: np.ndarray           # (nurey,2) int32
+    urey_s0: np.ndarray
+    urey_k: np.ndarray
+    tors: np.ndarray           # (ntors,4) int32
+    kphi: np.ndarray           # (ntors,6)
+    nphi: np.ndarray
+    delphi: np.ndarray
+    impr: np.ndarray           # (nimpr,4) int32
+    impr_k: np.ndarray
+    impr_eq: np.ndarray
+
+
+def dihedral(pos: np.ndarray, box: np.ndarray, quads: np.ndarray) -> np.ndarray:
+    """Signed dihedral angles of a-b-c-d quads (minimum image), radians, with
+    the sign of (n1 x n2) . b2 as ctorsion.c:171-182 takes it."""
+    def mi(d):
+        return d - box * np.round(d / box)
+    b1 = mi(pos[quads[:, 1]] - pos[quads[:, 0]])
+    b2 = mi(pos[quads[:, 2]] - pos[quads[:, 1]])
+    b3 = mi(pos[quads[:, 3]] - pos[quads[:, 2]])
+    n1, n2 = np.cross(b1, b2), np.cross(b2, b3)
+    y = (np.cross(n1, n2) * b2).sum(1) / np.linalg.norm(b2, axis=1)
+    return np.arctan2(y, (n1 * n2).sum(1))
+
+
+def angle_terms(s: ChainSystem, seed: int = 5, generic: bool = False) -> AngleTerms:
+    """United-atom alkane-like angle terms for every chain of s: a bend and a
+    Urey-Bradley pair on each consecutive triple (angle 112 deg, CHARMM-like
+    constants), a three-term torsion on each consecutive quad, and a
+    planarity improper (i, i+1, i+3, i+2), equilibrium 0, on every third
+    quad.  generic: torsion phases and improper equilibria drawn at random
+    instead (energies that are not even in the dihedral, for tests of its
+    sign)."""
+    rng = np.random.default_rng(seed)
+    nb = len(s.bonds)
+    # chains: runs of consecutive bonds i -> i+1
+    first = [0]
+    for k in range(1, nb):
+        if s.bonds[k, 0] != s.bonds[k - 1, 1]:
+            first.append(k)
+    first.append(nb)
+    tri, quad = [], []
+    for c0, c1 in zip(first[:-1], first[1:]):
+        sites = [int(s.bonds[c0, 0])] + [int(x) for x in s.bonds[c0:c1, 1]]
+        tri += [sites[i:i + 3] for i in range(len(sites) - 2)]
+        quad += [sites[i:i + 4] for i in range(len(sites) - 3)]
+    tri = np.array(tri, np.int32).reshape(-1, 3)
+    quad = np.array(quad, np.int32).reshape(-1, 4)
+    nt = len(quad)
+    kphi = np.zeros((nt, 6)); nphi = np.zeros((nt, 6)); delphi = np.zeros((nt, 6))
+    kphi[:, :3] = KCAL * np.array([0.195, 0.10, 0.05]) * rng.uniform(0.8, 1.2, (nt, 3))
+    nphi[:, :3] = [3.0, 2.0, 1.0]
+    delphi[:, :3] = [0.0, math.pi, 0.0]
+    imp = np.ascontiguousarray(quad[::3][:, [0, 1, 3, 2]])
+    impr_eq = np.zeros(len(imp))
+    if generic:
+        delphi[:, :3] = rng.uniform(-math.pi, math.pi, (nt, 3))
+        impr_eq = rng.uniform(-0.3, 0.3, len(imp))
+    return AngleTerms(
+        bend=tri, bend_eq=np.full(len(tri), BOND_ANGLE), bend_k=np.full(len(tri), 58.35 * KCAL),
+        urey=np.ascontiguousarray(tri[:, [0, 2]]), urey_s0=np.full(len(tri), 2.561),
+        urey_k=np.full(len(tri), 11.16 * KCAL),
+        tors=quad, kphi=kphi, nphi=nphi, delphi=delphi,
+        impr=imp, impr_k=np.full(len(imp), 2.0 * KCAL), impr_eq=impr_eq)
+
+
+def jitter(s: ChainSystem, amp: float, seed: int = 9) -> np.ndarray:
+    """Positions of s displaced by uniform noise of +-amp per component
+    (wrapped): breaks the exact planarity of the all-trans start."""
+    rng = np.random.default_rng(seed)
+    return np.ascontiguousarray(wrap(s.pos + rng.uniform(-amp, amp, s.pos.shape), s.box))
+
+
 def chains_3k(seed: int = 1234) -> ChainSystem:
     """300 chains x 10 sites, 31 A box, PME 32^3."""
     return make_chain_system(300, 31.0, 32, seed=seed, name="chains-3k")
