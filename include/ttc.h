@@ -110,6 +110,15 @@ int  ttc_pair(ttc_ctx *c, double *ff, int64_t stride, double *e5, double *vir6);
 /* cbond.c:65-120: returns bond energy through *ebond, forces added.       */
 int  ttc_cbond(ttc_ctx *c, double *ff, int64_t stride, double *ebond, double *vir6);
 
+/* drift.c:20 drift(k): removes the net momentum, v_i -= (sum_j m_j v_j / N)
+ * / m_i (the reference's non-KONS, non-ZEROAM form).                      */
+int  ttc_drift(ttc_ctx *c);
+/* sim.drift: the MD loops (ttc_md_vverlet, ttc_md_nhc, ttc_md_mts) call
+ * drift after every step whose number within the call is a multiple of
+ * every (nve_md.c:208, nvt_md.c:302), after the step's energy row; 0 (the
+ * default) never.                                                          */
+int  ttc_set_drift(ttc_ctx *c, int every);
+
 /* ---- angle-type bonded terms ------------------------------------------
  * read_topology.c's bend / urey / tors / impr arrays of box k:
  *   bends      (a,b,c) 3 per term, angeq [rad], krbend: cbend.c:63-170

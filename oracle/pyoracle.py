@@ -102,6 +102,7 @@ def _declare(L):
     L.orc_cbond.restype = C.c_double
     L.orc_cbond.argtypes = [P(OrcSystem), _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
     L.orc_pbc_all.argtypes = [P(OrcSystem), _dp, C.c_int]
+    L.orc_drift.argtypes = [C.c_int, _dp, _dp]
     L.orc_cbend.restype = C.c_double
     L.orc_cbend.argtypes = [P(OrcSystem), _dp, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
     L.orc_curey.restype = C.c_double
@@ -353,6 +354,7 @@ def ref_lib():
     L.ref_kinet.restype = C.c_double
     L.ref_kinet.argtypes = [_dp, _dp, _dp, _dp]
     L.ref_bonded_box.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+    L.ref_drift.argtypes = [_dp, _dp]
     L.ref_cbend.restype = C.c_double
     L.ref_cbend.argtypes = [_dp, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
     L.ref_curey.restype = C.c_double

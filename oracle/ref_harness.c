@@ -248,13 +248,18 @@ void integrate_vverlet(int, double);
 void pbc_all(int);
 void kinet(int);
 
+static int g_md_n = 0;   /* sites the velocity arrays hold */
 static void md_alloc(void)
 {
-    if (vv) return;
-    vv = (struct veloc **)xcalloc(1, sizeof(struct veloc *));
-    uu = (struct veloc **)xcalloc(1, sizeof(struct veloc *));
+    if (vv && g_md_n == g_n) return;
+    if (!vv) {
+        vv = (struct veloc **)xcalloc(1, sizeof(struct veloc *));
+        uu = (struct veloc **)xcalloc(1, sizeof(struct veloc *));
+    }
+    free(vv[0]); free(uu[0]);
     vv[0] = (struct veloc *)xcalloc(g_n, sizeof(struct veloc));
     uu[0] = (struct veloc *)xcalloc(g_n, sizeof(struct veloc));
+    g_md_n = g_n;
 }
 
 /* nsteps calls of integrate_vverlet(0, vscal) after one forces(0) (as
@@ -417,4 +422,19 @@ double ref_cimproper(const double *pos, int ni, const int *ia, const int *ib, co
     double e = cimproper(0);
     bonded_out(ff_out, pvir[0].impr, vir6);
     return e;
+}
+
+/* drift.c:20-72 on box 0 (g_n sites): velocities in, corrected velocities out */
+void drift(int);
+void ref_drift(const double *mass, double *vel)
+{
+    md_alloc();
+    for (int i = 0; i < g_n; i++) {
+        pott[0][i].mas = mass[i];
+        vv[0][i].x = vel[3 * i]; vv[0][i].y = vel[3 * i + 1]; vv[0][i].z = vel[3 * i + 2];
+    }
+    drift(0);
+    for (int i = 0; i < g_n; i++) {
+        vel[3 * i] = vv[0][i].x; vel[3 * i + 1] = vv[0][i].y; vel[3 * i + 2] = vv[0][i].z;
+    }
 }

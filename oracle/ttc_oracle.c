@@ -1377,6 +1377,27 @@ double orc_cimproper2(const orc_system *s, const double *pos, int ni, const int 
     return e;
 }
 
+/* drift.c:20-72 (neither KONS nor ZEROAM): net momentum P = sum m v, then
+ * v_i -= (P / N) / m_i. */
+void orc_drift(int n, const double *mass, double *vel)
+{
+    double px = 0.0, py = 0.0, pz = 0.0;
+    for (int i = 0; i < n; i++) {
+        px += mass[i] * vel[3 * i];
+        py += mass[i] * vel[3 * i + 1];
+        pz += mass[i] * vel[3 * i + 2];
+    }
+    px /= (1.0 * n);
+    py /= (1.0 * n);
+    pz /= (1.0 * n);
+    for (int i = 0; i < n; i++) {
+        double im = 1.0 / mass[i];
+        vel[3 * i] -= px * im;
+        vel[3 * i + 1] -= py * im;
+        vel[3 * i + 2] -= pz * im;
+    }
+}
+
 void orc_pbc_all(const orc_system *s, double *pos, int n)
 {
     double l[3] = {s->lx, s->ly, s->lz};

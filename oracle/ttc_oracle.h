@@ -172,6 +172,8 @@ double orc_ctorsion2(const orc_system *s, const double *pos, int nt, const int *
                      double hes3[3]);
 double orc_cimproper2(const orc_system *s, const double *pos, int ni, const int *ia, const int *ib, const int *ic,
                       const int *id, const double *kimpr, const double *angeq, double *ff, double hes3[3]);
+/* drift.c:20-72: removes the net momentum from vel (n x 3) */
+void orc_drift(int n, const double *mass, double *vel);
 void orc_pbc_all(const orc_system *s, double *pos, int n);  /* pbc_all.c:23-88 */
 double orc_kinet(int n, const double *mass, const double *vv, const double *uu,
                  double ken[6]);                             /* kinet.c:18-68 */

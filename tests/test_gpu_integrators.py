@@ -176,3 +176,62 @@ def test_nblist_pivot_gives_fresh_list(small):
     for a in range(s.n):
         assert (gl[a, :cnt[a]] == lst[a, :cnt[a]]).all() and (gx[a, :cnt[a]] == xf[a, :cnt[a]]).all()
     b.close()
+
+
+def test_drift_removes_momentum_as_reference(small):
+    """drift.c: v_i -= (P / N) / m_i, against the oracle (bit-exact to the
+    reference's drift.c); 1e-12 of the speed scale (the device holds 100/m)."""
+    import pyoracle
+    s = small
+    v0 = s.vel + np.array([3.0, -2.0, 1.5])
+    print("case: 1,200 sites, velocities with net momentum, drift()", flush=True)
+    b = _box(s)
+    try:
+        b.set_velocities(v0)
+        b.drift()
+        v1 = b.velocities()
+    finally:
+        b.close()
+    vo = v0.copy().ravel()
+    pyoracle.lib().orc_drift(s.n, np.ascontiguousarray(s.mass), vo)
+    assert np.abs(v1 - vo.reshape(-1, 3)).max() <= 1e-12 * np.abs(v0).max()
+    assert np.abs((s.mass[:, None] * v1).sum(0)).max() < 1e-9 * (s.mass[:, None] * np.abs(v0)).sum()
+
+
+def test_md_with_drift_every_5_steps_matches_oracle(small):
+    """20 velocity-Verlet steps with drift(k) after steps 5, 10, 15, 20
+    (nve_md.c:208) against the oracle's loop."""
+    import pyoracle
+    s = small
+    v0 = s.vel + np.array([3.0, -2.0, 1.5])
+    nsteps = 20
+    b = _box(s)
+    try:
+        b.set_velocities(v0)
+        b.set_drift(5)
+        b.integrate_vverlet(nsteps)
+        pos_gpu, vel_gpu = b.positions(), b.velocities()
+    finally:
+        b.close()
+    o = pyoracle.Oracle(s)
+    pos = s.pos.copy().ravel(); vel = v0.copy().ravel(); uu = vel.copy()
+    anchor = pos.copy()
+    ff = o.forces(pos.reshape(-1, 3))[0].ravel()
+    lists = None
+    for step in range(1, nsteps + 1):
+        if lists is None or o.L.orc_nl_check(C.byref(o.cs), pos, anchor, 0, s.n):
+            rc, cnt, lst, xf, *_ = o.nblist(pos.reshape(-1, 3))
+            lists = (cnt, lst, xf)
+            anchor = pos.copy()
+        o.L.orc_vverlet_a(C.byref(o.cs), s.n, s.mass, s.dt, 1.0, pos, None, vel, ff)
+        ff = o.forces(pos.reshape(-1, 3), lists=lists)[0].ravel()
+        o.L.orc_vverlet_b(s.n, s.mass, s.dt, 1.0, vel, uu, ff)
+        if step % 5 == 0:
+            o.L.orc_drift(s.n, np.ascontiguousarray(s.mass), vel)
+    dpos = pos.reshape(-1, 3) - pos_gpu
+    dpos -= s.box * np.round(dpos / s.box)
+    print("max |dx|", np.abs(dpos).max(), "max |dv|", np.abs(vel.reshape(-1, 3) - vel_gpu).max(), flush=True)
+    assert np.abs(dpos).max() < 1e-6
+    # FP32 pair terms: 1e-7 relative force noise, grown over 20 steps
+    assert np.abs(vel.reshape(-1, 3) - vel_gpu).max() < 5e-5 * np.abs(vel).max()
+    assert np.abs((s.mass[:, None] * vel_gpu).sum(0)).max() < 1e-6 * (s.mass[:, None] * np.abs(vel_gpu)).sum()

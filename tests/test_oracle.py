@@ -896,3 +896,20 @@ def test_angle_hessians_are_force_derivatives(which):
             fd[d] -= (o.angles(p1, at, (which,))[0][i, d] - o.angles(p2, at, (which,))[0][i, d]) / (2 * eps)
     print(which, h, fd)
     assert np.abs(h - fd).max() <= 1e-5 * np.abs(h).max()
+
+
+@needs_ref
+def test_drift_bit_exact_vs_reference():
+    s = system.make_chain_system(40, 20.0, 20, seed=3)
+    R = pyoracle.ref_lib()
+    _ref_setup(R, s, 400)
+    rng = np.random.default_rng(12)
+    mass = rng.uniform(1.0, 40.0, s.n)
+    v = rng.normal(0, 10, 3 * s.n) + 3.0
+    vr, vo = v.copy(), v.copy()
+    R.ref_drift(mass, vr)
+    L.orc_drift(s.n, mass, vo)
+    assert np.array_equal(vr, vo) and not np.array_equal(vo, v)
+    # the net momentum is gone
+    p = (mass[:, None] * vo.reshape(-1, 3)).sum(0)
+    assert np.abs(p).max() < 1e-9 * (mass[:, None] * np.abs(v.reshape(-1, 3))).sum()

@@ -76,6 +76,8 @@ def lib():
               "ttc_ctorsion2", "ttc_cimproper2"):
         getattr(L, f).argtypes = [vp, dp, C.c_int64, dp, dp]
     L.ttc_bonded_energies.argtypes = [vp, dp]
+    L.ttc_drift.argtypes = [vp]
+    L.ttc_set_drift.argtypes = [vp, C.c_int]
     L.ttc_md_vverlet.argtypes = [vp, C.c_int, C.c_double, C.c_int, dp, C.c_int, ip, ip]
     L.ttc_kinet.argtypes = [vp, dp, dp]
     L.ttc_timing.argtypes = [vp, C.c_int, dp]

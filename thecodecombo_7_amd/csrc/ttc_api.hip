@@ -913,6 +913,20 @@ extern "C" int ttc_forces(ttc_ctx *c, double *ff, int64_t stride, double *en9) {
     return TTC_OK;
 }
 
+extern "C" int ttc_drift(ttc_ctx *c) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "ttc_drift: topology not set");
+    ttc_launch_momentum_removal(c);
+    TTC_HIP(hipGetLastError());
+    TTC_HIP(hipStreamSynchronize(c->stream));
+    return TTC_OK;
+}
+
+extern "C" int ttc_set_drift(ttc_ctx *c, int every) {
+    CHECK(c && every >= 0, TTC_ERR_ARG, "ttc_set_drift: bad argument");
+    c->drift_every = every;
+    return TTC_OK;
+}
+
 extern "C" int ttc_kinet(ttc_ctx *c, double *kinet, double *ken6) {
     CHECK(c, TTC_ERR_ARG, "ttc_kinet: null context");
     ttc_launch_kinet(c);
@@ -1064,8 +1078,10 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
             // TTC_VV_RUN plain steps in one graph (no launch gap between
             // them) when a kick is pending and none of them writes an energy row
             int last = step + TTC_VV_RUN - 1;
+            const int de = c->drift_every;
             bool run = !kicked && last <= nsteps &&
-                       (energy_every <= 0 || last / energy_every == (step - 1) / energy_every);
+                       (energy_every <= 0 || last / energy_every == (step - 1) / energy_every) &&
+                       (de <= 0 || last / de == (step - 1) / de);
             if (run) {
                 TTC_HIP(hipGraphLaunch(c->g_vv_run, c->stream));
                 step = last + 1;
@@ -1074,6 +1090,12 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
             bool en = energy_every > 0 && step % energy_every == 0;
             TTC_HIP(hipGraphLaunch(c->g_vv[kicked ? 0 : 1][en ? 1 : 0], c->stream));
             kicked = en;
+            if (de > 0 && step % de == 0) {
+                // nve_md.c:208: drift(k) after the step's kinet / calcvalue
+                if (!kicked) ttc_launch_kick_tail(c, vscal, false);
+                ttc_launch_momentum_removal(c);
+                kicked = true;
+            }
             step++;
         }
         if (!kicked) ttc_launch_kick_tail(c, vscal, false);
@@ -1088,6 +1110,7 @@ extern "C" int ttc_md_vverlet(ttc_ctx *c, int nsteps, double vscal, int energy_e
         const int failed = c->h_status[ST_STEP] + 1;
         r = redo_overflowed_step(c, vscal, energy_every > 0 && failed % energy_every == 0);
         if (r) return r;
+        if (c->drift_every > 0 && failed % c->drift_every == 0) ttc_launch_momentum_removal(c);
         step = failed + 1;
     }
     TTC_HIP(hipEventRecord(ev.e1, c->stream));
@@ -1338,6 +1361,8 @@ static int run_graphs(ttc_ctx *c, hipGraphExec_t plain, hipGraphExec_t energy, i
     for (int step = 1; step <= nsteps; step++) {
         bool en = energy_every > 0 && step % energy_every == 0;
         TTC_HIP(hipGraphLaunch(en ? energy : plain, c->stream));
+        // nvt_md.c:302: drift(k) after the step
+        if (c->drift_every > 0 && step % c->drift_every == 0) ttc_launch_momentum_removal(c);
     }
     TTC_HIP(hipEventRecord(ev.e1, c->stream));
     int r = read_status(c);

@@ -80,6 +80,7 @@ enum {
     EV_EBEND, EV_EUREY, EV_ETORS, EV_EIMPR,
     EV_AV0, EV_AV1, EV_AV2, EV_AV3, EV_AV4, EV_AV5,
     EV_AHX, EV_AHY, EV_AHZ,
+    EV_MOX, EV_MOY, EV_MOZ,                           // net momentum (drift.c)
     EV_N
 };
 
@@ -202,6 +203,10 @@ struct ttc_ctx {
     double4 *ang_tf = nullptr;      // [ang_slot[4]]
     int *ang_off = nullptr;         // [n + 1]
     int *ang_ref = nullptr;
+
+    // drift.c: remove the net momentum after every drift_every-th MD step
+    // (nve_md.c:208, nvt_md.c:302); 0: never
+    int drift_every = 0;
 
     // reductions: partials[slot * nblk_max + block]
     double *partials = nullptr;
@@ -337,6 +342,9 @@ void ttc_launch_nhc(ttc_ctx *c);         // chain update from the KE partials ->
 void ttc_launch_vscale(ttc_ctx *c);      // v *= scale
 void ttc_launch_vmul(ttc_ctx *c);        // v *= nhc[TTC_NHC_VMUL]
 void ttc_launch_kinet(ttc_ctx *c);
+// drift.c:20-72: v_i -= (sum_j m_j v_j / N) / m_i (two launches: block sums
+// of the momentum, then every block sums them in a fixed order and corrects)
+void ttc_launch_momentum_removal(ttc_ctx *c);
 void ttc_launch_reduce(ttc_ctx *c, bool append_row);
 int ttc_kick_blocks(ttc_ctx *c);
 double ttc_nr_erfc_host(double x);

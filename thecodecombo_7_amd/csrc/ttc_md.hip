@@ -108,6 +108,49 @@ __device__ __forceinline__ void kinet_body(int bid, int n, const double4 *vel4, 
     ken_block(bid, 100.0 / v.w, v, partials, maxb, ok);
 }
 
+// drift.c:36-44: block sums of m v (m = 100 / vel4.w)
+__global__ void __launch_bounds__(256) k_momentum(int n, const double4 *vel4, double *partials, int maxb) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double p[3] = {0.0, 0.0, 0.0};
+    if (s < n) {
+        double4 v = vel4[s];
+        double m = 100.0 / v.w;
+        p[0] = m * v.x; p[1] = m * v.y; p[2] = m * v.z;
+    }
+    __shared__ double red[4][3];
+    int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int q = 0; q < 3; q++) {
+        double t = ttc_wave_sum(p[q]);
+        if (lane == 0) red[wid][q] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int q = threadIdx.x;
+        partials[(size_t)(EV_MOX + q) * maxb + blockIdx.x] = red[0][q] + red[1][q] + red[2][q] + red[3][q];
+    }
+}
+
+// drift.c:52-72: every block sums the momentum partials in the same order,
+// then v_i -= (P / N) / m_i
+__global__ void __launch_bounds__(256) k_momentum_remove(int n, double4 *vel4, const double *partials, int maxb,
+                                                         int nb) {
+    __shared__ double P[3];
+    const int d = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (d < 3) {
+        double t = 0.0;
+        for (int b = lane; b < nb; b += 64) t += partials[(size_t)(EV_MOX + d) * maxb + b];
+        t = ttc_wave_sum(t);
+        if (lane == 0) P[d] = t / (1.0 * n);
+    }
+    __syncthreads();
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    double4 v = vel4[s];
+    double im = 1.0 / (100.0 / v.w);
+    v.x -= P[0] * im; v.y -= P[1] * im; v.z -= P[2] * im;
+    vel4[s] = v;
+}
+
 struct KinetPack {
     int n;
     const double4 *vel4;
@@ -236,6 +279,12 @@ void ttc_launch_kinet(ttc_ctx *c) {
     k_kinet<<<ttc_kick_blocks(c), 256, 0, c->stream>>>(kinet_pack(c));
 }
 
+void ttc_launch_momentum_removal(ttc_ctx *c) {
+    int blocks = ttc_kick_blocks(c);
+    k_momentum<<<blocks, 256, 0, c->stream>>>(c->n, c->vel4, c->partials, c->nblk_max);
+    k_momentum_remove<<<blocks, 256, 0, c->stream>>>(c->n, c->vel4, c->partials, c->nblk_max, blocks);
+}
+
 static ReducePack reduce_pack(ttc_ctx *c, bool append_row) {
     ReduceArgs ra;
     for (int i = 0; i < EV_N; i++) ra.nb[i] = 0;
@@ -247,6 +296,7 @@ static ReducePack reduce_pack(ttc_ctx *c, bool append_row) {
     for (int i = EV_PHX; i <= EV_PHZ; i++) ra.nb[i] = ttc_pair_blocks(c);
     for (int i = EV_KHX; i <= EV_BHZ; i++) ra.nb[i] = ttc_gather_blocks(c);
     for (int i = EV_EBEND; i <= EV_AHZ; i++) ra.nb[i] = ttc_angle_blocks(c);
+    for (int i = EV_MOX; i <= EV_MOZ; i++) ra.nb[i] = ttc_kick_blocks(c);
     double fan = 0.5 * TTC_QFACTOR / c->p.eps_rf;
     return ReducePack{ra, c->partials, c->nblk_max, c->ev, c->e_self, fan, c->rows, c->rows_cap, c->status,
                       append_row ? 1 : 0};

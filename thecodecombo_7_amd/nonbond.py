@@ -152,6 +152,14 @@ class Box:
         T.check(getattr(self.L, name + ("2" if hessian else ""))(self.h, T.dptr(ff), 3, T.dptr(e), T.dptr(x)))
         return ff, float(e[0]), x
 
+    def drift(self):
+        """drift.c: remove the net momentum."""
+        T.check(self.L.ttc_drift(self.h))
+
+    def set_drift(self, every: int):
+        """sim.drift: momentum removal after every `every`-th MD step."""
+        T.check(self.L.ttc_set_drift(self.h, every))
+
     def bonded_energies(self):
         e = np.zeros(5)
         T.check(self.L.ttc_bonded_energies(self.h, T.dptr(e)))
