@@ -8,7 +8,24 @@
  * thecodecombo_7_amd/host/ttc_refshim.c and libttc.so.  The reference's
  * forces() therefore calls the shim's cnonbondnl(), which runs on the GPU.
  *
- * Usage: dropin_forces IN.bin OUT.bin
+ * With two more arguments, MD.bin MDOUT.bin, it then runs the reference's
+ * own integrators (integrate_vverlet.c, integrate_nhc.c, integrate_mts.c,
+ * kinet.c, pbc_all.c, also linked unmodified) for a number of steps with
+ * the step order of the reference's drivers, every force call reaching the
+ * GPU through the shim:
+ *   mode 0, nve_md.c:360-390: nl_check(0, n, k), nblist(k) when flagged,
+ *           integrate_vverlet(k, 1.0), kinet(k)
+ *   mode 1, nvt_md.c:528-531: the same list update, then integrate_nhc,
+ *           integrate_vverlet(k, 1.0), integrate_nhc, kinet(k)
+ *   mode 2, nve_md.c:393-420: the list update, integrate_mts(k, 1.0),
+ *           kinet(k) (ninner short steps per long step)
+ *   MD.bin: int32 nsteps, mode, M, Nc, Nys, ninner; float64 T, Q, nfree,
+ *           dtlong, w[Nys], vel[3n]
+ *   MDOUT:  float64 pos[3n] (atom, wrapped), vel[3n], rows[nsteps][2] =
+ *           {en.kinet, en.potens as calcvalue.c:42-53 sums it}, the loop's
+ *           wall seconds, and int32 list rebuilds
+ *
+ * Usage: dropin_forces IN.bin OUT.bin [MD.bin MDOUT.bin]
  *   IN  (little endian): int32 n, ntypes, nx, nbond, order, gx, gy, gz;
  *       float64 lx, ly, lz, rc, rclist, rclist_ew, kappa, eps_rf, dt;
  *       int32 type[n]; float64 q[n], mass[n], pos[3n];
@@ -30,6 +47,10 @@ void forces(int);
 void force_long(int);
 void force_short(int);
 void pme_clean(void);
+void integrate_vverlet(int, double);
+void integrate_nhc(double, int);
+void integrate_mts(int, double);
+void kinet(int);
 
 static void rd(void *p, size_t sz, size_t n, FILE *f)
 {
@@ -43,9 +64,76 @@ static void *xc(size_t n, size_t s)
     return p;
 }
 
+static double wall(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* the reference's integrator loops on box k (see the header comment) */
+static int run_md(const char *in, const char *outp, int n, int k)
+{
+    FILE *f = fopen(in, "rb");
+    if (!f) { perror(in); return 2; }
+    int h[6];
+    double par[4];
+    rd(h, sizeof(int), 6, f);
+    rd(par, sizeof(double), 4, f);
+    int nsteps = h[0], mode = h[1], M = h[2], Nc = h[3], Nys = h[4];
+    double *w = xc(Nys, sizeof(double));
+    rd(w, sizeof(double), Nys, f);
+    vv = xc(1, sizeof *vv); uu = xc(1, sizeof *uu);
+    vv[k] = xc(n, sizeof **vv); uu[k] = xc(n, sizeof **uu);
+    double *v = xc(3 * (size_t)n, sizeof(double));
+    rd(v, sizeof(double), 3 * (size_t)n, f);
+    fclose(f);
+    for (int i = 0; i < n; i++) {
+        vv[k][i].x = v[3 * i]; vv[k][i].y = v[3 * i + 1]; vv[k][i].z = v[3 * i + 2];
+        uu[k][i] = vv[k][i];
+    }
+    sim.T[k] = par[0];
+    box[k].nfree = par[2];
+    sim.dtlong = par[3];
+    sim.nsteps = h[5];
+    nhc[k].M = M; nhc[k].Nc = Nc; nhc[k].Nys = Nys; nhc[k].Q = par[1]; nhc[k].w = w;
+    nhc[k].zeta = xc(M > 0 ? M : 1, sizeof *nhc[k].zeta);
+    if (mode == 2) { force_long(k); force_short(k); }   /* main.c: both before the loop */
+    else forces(k);
+    double *rows = xc(2 * (size_t)(nsteps > 0 ? nsteps : 1), sizeof(double));
+    int rebuilds = 0;
+    double t0 = wall();
+    for (int i = 1; i <= nsteps; i++) {
+        nl_check(0, box[k].boxns, k);
+        if (nl_flag[k] == 1) { nblist(k); rebuilds++; }
+        if (mode == 1) {
+            integrate_nhc(sim.dt, k);
+            integrate_vverlet(k, 1.0);
+            integrate_nhc(sim.dt, k);
+        } else if (mode == 2) {
+            integrate_mts(k, 1.0);
+        } else {
+            integrate_vverlet(k, 1.0);
+        }
+        kinet(k);
+        rows[2 * (i - 1)] = en[k].kinet;
+        rows[2 * (i - 1) + 1] = en[k].bond + en[k].bend + en[k].tors + en[k].nbonds + en[k].impr + en[k].urey;
+    }
+    double secs = wall() - t0;
+    FILE *o = fopen(outp, "wb");
+    if (!o) { perror(outp); return 2; }
+    for (int i = 0; i < n; i++) fwrite(&atom[k][i].x, sizeof(double), 3, o);
+    for (int i = 0; i < n; i++) fwrite(&vv[k][i].x, sizeof(double), 3, o);
+    fwrite(rows, sizeof(double), 2 * (size_t)nsteps, o);
+    fwrite(&secs, sizeof(double), 1, o);
+    fwrite(&rebuilds, sizeof(int), 1, o);
+    fclose(o);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
-    if (argc != 3) { fprintf(stderr, "usage: dropin_forces IN.bin OUT.bin\n"); return 2; }
+    if (argc != 3 && argc != 5) { fprintf(stderr, "usage: dropin_forces IN.bin OUT.bin [MD.bin MDOUT.bin]\n"); return 2; }
     FILE *f = fopen(argv[1], "rb");
     if (!f) { perror(argv[1]); return 2; }
     int hdr[8];
@@ -128,6 +216,10 @@ int main(int argc, char **argv)
     nl_check(0, n, k);
     fwrite(&nl_flag[k], sizeof(int), 1, o);
     fclose(o);
+    if (argc == 5) {
+        int rc = run_md(argv[3], argv[4], n, k);
+        if (rc) return rc;
+    }
     pme_clean();
     return 0;
 }

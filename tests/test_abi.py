@@ -90,6 +90,28 @@ def test_shim_compiles_against_reference_headers(tmp_path):
 
 
 @pytest.mark.skipif(not os.path.isdir(REFDIR), reason="reference tree absent")
+def test_shim_fftw_allocation_pair(tmp_path):
+    """With -DTTC_SHIM_FFTW_ALLOC the binding defines FFTW's allocation pair
+    (fftw_malloc / fftw_free, the only FFTW calls outside the replaced
+    pme_*.c files: allocate.c:500), so an SPME program links without
+    -lfftw3.  (The SPME flag itself is not compiled here: defines.h:41
+    includes <fftw3.h>, which this image does not have.)"""
+    obj = str(tmp_path / "shim3.o")
+    r = subprocess.run(["gcc", "-std=c99", "-O2", "-c", "-DNLIST", "-DDLIST", "-DSTYPE", "-DS14", "-DEWALD",
+                        "-DPRESSURE", "-DTTC_SHIM_FFTW_ALLOC", f"-I{REFDIR}", f"-I{ROOT}/include", SHIM, "-o", obj],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run(["nm", "--defined-only", obj], capture_output=True, text=True).stdout
+    defined = set(re.findall(r" T (\w+)$", out, flags=re.M))
+    assert {"fftw_malloc", "fftw_free"} <= defined
+    # and no other FFTW symbol is used by the reference outside pme_*.c
+    for f in os.listdir(REFDIR):
+        if f.endswith(".c") and not f.startswith("pme_"):
+            calls = set(re.findall(r"\bfftw_\w+(?=\s*\()", open(os.path.join(REFDIR, f), errors="replace").read()))
+            assert calls <= {"fftw_malloc", "fftw_free"}, (f, calls)
+
+
+@pytest.mark.skipif(not os.path.isdir(REFDIR), reason="reference tree absent")
 def test_shim_configt_entry_point(tmp_path):
     """In a CONFIGT build the binding also defines cnonbondnl2
     (cnonbond_nblist2.c:47) over the reference's config[k] struct."""

@@ -25,6 +25,12 @@
  * when ttc_shim_export_lists is set (no replaced routine reads them).
  * Errors end the program with the reference's exit code for the same
  * condition (list overflow 5, net charge 20, grid 1, allocation 11).
+ *
+ * FFTW: in an SPME build defines.h:41 includes <fftw3.h>, so the header is
+ * still needed to compile, and allocate.c:500 still allocates qgrida with
+ * fftw_malloc (nothing reads it once pme_calc runs on the GPU).  Compiled
+ * with -DTTC_SHIM_FFTW_ALLOC this file defines fftw_malloc / fftw_free
+ * itself, and the program links without -lfftw3.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -254,3 +260,12 @@ double erfc(double x)
     double r = t * exp(p);
     return x >= 0.0 ? r : 2.0 - r;
 }
+
+#ifdef TTC_SHIM_FFTW_ALLOC
+/* FFTW's allocation pair (fftw3.h: void *fftw_malloc(size_t); void
+ * fftw_free(void *)) for allocate.c:500, the one FFTW call left outside the
+ * replaced pme_*.c files.  No FFTW plan ever touches the memory, so plain
+ * malloc alignment is enough. */
+void *fftw_malloc(size_t n) { return malloc(n ? n : 1); }
+void fftw_free(void *p) { free(p); }
+#endif
