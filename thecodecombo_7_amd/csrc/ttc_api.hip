@@ -149,6 +149,7 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
     ALLOC(c->fixj, (size_t)3 * n);
     TTC_HIP(hipMemset(c->fixj, 0, sizeof(long long) * 3 * n));
     ttc_window_sizes(c, &c->wcap, &c->span, &c->span_pre);
+    ALLOC(c->posq, n);
     ALLOC(c->win, (size_t)c->G * c->wcap);
     ALLOC(c->wlen, c->G);
     TTC_HIP(hipMemset(c->wlen, 0, sizeof(int) * c->G));
@@ -190,7 +191,7 @@ extern "C" int ttc_destroy(ttc_ctx *c) {
                     c->cell_members, c->nbl, c->glen, c->status, c->ex_off, c->ex_part, c->ex_flag, c->ljtab,
                     c->b_off, c->b_part, c->b_r0, c->b_k, c->partials, c->ev, c->rows, c->fs, c->fl, c->fsb,
                     c->flb, c->nhc, c->posfb, c->gcnt, c->snap, c->g4o, c->fixj, c->win, c->wlen, c->ang_idx,
-                    c->ang_par, c->ang_tpar, c->ang_tf, c->ang_off, c->ang_ref};
+                    c->ang_par, c->ang_tpar, c->ang_tf, c->ang_off, c->ang_ref, c->posq};
     for (void *b : bufs)
         if (b) hipFree(b);
     ttc_pme_free(c);

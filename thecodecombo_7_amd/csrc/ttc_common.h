@@ -235,6 +235,7 @@ struct ttc_ctx {
     // accumulators per window slot and flushes them to fixj.
     bool half_mode = false;         // conditional rebuilds build half lists
     bool list_half = false;         // the current list is a half list
+    int4 *posq = nullptr;           // [N] pos4 as 32-bit fixed point + charge bits (half-list kernel)
     int *win = nullptr;             // [G][wcap] window of every group
     int *wlen = nullptr;            // [G] window length
     int wcap = 0;                   // window capacity (multiple of 64)

@@ -17,6 +17,14 @@
 
 namespace {
 
+// the half-list kernel keeps four waves per SIMD (at most 128 VGPRs): at
+// three it runs 40 % slower on chains-100k
+#ifndef HALF_WPE
+#define HALF_WPE __attribute__((amdgpu_waves_per_eu(4)))
+#endif
+#ifndef HALF_AHEAD
+#define HALF_AHEAD 2   // half-list kernel: entries whose gathers are in flight ahead of the arithmetic
+#endif
 #ifndef PAIR_AHEAD
 #define PAIR_AHEAD 2   // entries whose site gathers are in flight ahead of the arithmetic
 #endif
@@ -32,6 +40,11 @@ struct PairConst {
     float inv_epsrf;
     float e14, one_m_e14;
     int ntypes;
+    // half-list kernel: displacements from 32-bit fixed-point coordinates
+    // (units L / 2^32 per axis), and the band of r^2 around rc^2 inside
+    // which their r^2 cannot decide the cutoff (the FP64 positions do)
+    double ux, uy, uz;
+    double rc2_lo, rc2_hi;
 };
 
 // Numerical Recipes erfcc (ewald.c:817-835) in FP32 at z = kappa r >= 0:
@@ -58,7 +71,7 @@ struct PairAcc {
     float tx, ty, tz;    // this tile's force, added to fx, fy, fz once per 16 entries
     double elj, eljs, er, ers, ei, w0, w1, w2, w3, w4, w5;
     double hx, hy, hz;   // CONFIGT: this site's d2U/dx2, dy2, dz2 (pair terms)
-    float fmax;          // half lists: largest pair-force component (range check of the accumulators)
+    float fmax;          // half lists: largest |force / r| (range check of the accumulators)
 };
 
 // One list entry of site i (cnonbond_nblist.c:150-519 for one pair).
@@ -177,17 +190,32 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const 
 // 1.5 2^52 (from an SGPR) rounds it to an integer held in the low mantissa
 // bits, and removing the magic's high word leaves the two's-complement value.
 template <bool ENERGY>
-__device__ __forceinline__ void pair_entry_half(uint32_t e, const double4 &pj, const double4 &pi, float qi,
-                                                const float2 *ljrow, const float2 *ljrow14, const PairConst &pc,
-                                                PairAcc &a, unsigned long long *jacc, int wcap, double magic) {
+__device__ __forceinline__ void pair_entry_half(uint32_t e, const int4 &pj, int j, const int4 &pi, int s, float qi,
+                                                const double4 *__restrict__ pos4, const float2 *ljrow,
+                                                const float2 *ljrow14, const PairConst &pc, PairAcc &a,
+                                                unsigned long long *jacc, int wcap, double magic) {
     const int tj = (int)((e >> TTC_TSHIFT) & 63u);
     const uint32_t cls = e >> 30;
-    double ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-    ex = fma(-pc.lx, rint(ex * pc.ilx), ex);
-    ey = fma(-pc.ly, rint(ey * pc.ily), ey);
-    ez = fma(-pc.lz, rint(ez * pc.ilz), ez);
-    const double dr2 = ttc_r2_exact(ex, ey, ez);
-    const bool in = (e != TTC_PAD) & (dr2 <= pc.rc2);
+    // i - j from the fixed-point coordinates: the 32-bit difference wraps
+    // modulo the box, which is the minimum image (cnonbond_nblist.c:166-173);
+    // in FP64 it is exact up to the 2^-32 L quantization of the coordinates
+    const double ex = (double)(int)((unsigned)pi.x - (unsigned)pj.x) * pc.ux;
+    const double ey = (double)(int)((unsigned)pi.y - (unsigned)pj.y) * pc.uy;
+    const double ez = (double)(int)((unsigned)pi.z - (unsigned)pj.z) * pc.uz;
+    const double dr2 = fma(ex, ex, fma(ey, ey, ez * ez));
+    const bool real = e != TTC_PAD;
+    bool in = real & (dr2 < pc.rc2_lo);
+    if (real & !(dr2 < pc.rc2_lo) & !(dr2 > pc.rc2_hi)) {
+        // within 1e-8 (relative) of rc^2: the reference's own FP64 test on
+        // the FP64 positions decides (cnonbond_nblist.c:174-176, the sum
+        // rounded op by op)
+        const double4 P = pos4[s], Q = pos4[j];
+        double fx = P.x - Q.x, fy = P.y - Q.y, fz = P.z - Q.z;
+        fx = fma(-pc.lx, rint(fx * pc.ilx), fx);
+        fy = fma(-pc.ly, rint(fy * pc.ily), fy);
+        fz = fma(-pc.lz, rint(fz * pc.ilz), fz);
+        in = ttc_r2_exact(fx, fy, fz) <= pc.rc2;
+    }
     const float dx = (float)ex, dy = (float)ey, dz = (float)ez;
     const float r2 = in ? (float)dr2 : pc.rc2f;
     const float rinv = __builtin_amdgcn_rsqf(r2);
@@ -199,7 +227,7 @@ __device__ __forceinline__ void pair_entry_half(uint32_t e, const double4 &pj, c
     const float sr6 = sr2 * sr2 * sr2;
     const float sr12 = sr6 * sr6;
     float force = 12.0f * eps * rinv2 * (sr12 - sr6);
-    const float qqab = qi * (float)pj.w;
+    const float qqab = qi * __int_as_float(pj.w);
     const float qq = (cls == 0 ? qqab : (cls == 2 ? pc.e14 * qqab : 0.0f)) * pc.inv_epsrf;
     float ez2;
     const float erfckr = nr_erfc(r, r2, pc, &ez2);
@@ -213,7 +241,8 @@ __device__ __forceinline__ void pair_entry_half(uint32_t e, const double4 &pj, c
     a.tx += fxf;
     a.ty += fyf;
     a.tz += fzf;
-    a.fmax = fmaxf(a.fmax, fmaxf(fabsf(fxf), fmaxf(fabsf(fyf), fabsf(fzf))));
+    // |component| <= |force| r <= |force| rc: the range check bounds |force|
+    a.fmax = fmaxf(a.fmax, fabsf(force));
     if (in) {
         const int slot = (int)(e & TTC_JMASK);
         auto fix = [&](float f) {
@@ -395,6 +424,7 @@ struct PairHalfPack {
     const int *glen;
     const int *win, *wlen;
     const double4 *pos4;
+    const int4 *posq;      // fixed-point coordinates and charge bits, as pos4
     const int *type;
     const float4 *ljtab;
     PairConst pc;
@@ -440,10 +470,11 @@ __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
     const int s = grp * 64 + lane;
     PairAcc acc = {};
     const int len = a.glen[grp];
-    const double4 pi = s < n ? a.pos4[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const int si = s < n ? s : 0;
+    const int4 pi = a.posq[si];
     const int ti = s < n ? a.type[s] : 0;
     const float2 *ljrow = slj + ti * a.pc.ntypes, *ljrow14 = slj + nt2 + ti * a.pc.ntypes;
-    const float qi = (float)pi.w * a.pc.qfac;
+    const float qi = __int_as_float(pi.w) * a.pc.qfac;
     const uint4 *seg = (const uint4 *)(a.nbl + (size_t)grp * a.cap * 64 + (size_t)lane * 16);
     const int ntile = (len + 15) >> 4;
     int t = wave;
@@ -459,17 +490,18 @@ __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
         int js[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) js[q] = e[q] == TTC_PAD ? 0 : swin[e[q] & TTC_JMASK];
-        double4 pj[16];
+        int4 pj[16];
 #pragma unroll
-        for (int q = 0; q < PAIR_AHEAD; q++) pj[q] = a.pos4[js[q]];
+        for (int q = 0; q < HALF_AHEAD; q++) pj[q] = a.posq[js[q]];
         if (t + WAVES < ntile) {
             const uint4 *p = seg + (size_t)(t + WAVES) * 256;
             q0 = p[0]; q1 = p[1]; q2 = p[2]; q3 = p[3];
         }
 #pragma unroll
         for (int q = 0; q < 16; q++) {
-            if (q + PAIR_AHEAD < 16) pj[q + PAIR_AHEAD] = a.pos4[js[q + PAIR_AHEAD]];
-            pair_entry_half<ENERGY>(e[q], pj[q], pi, qi, ljrow, ljrow14, a.pc, acc, jacc, wcap, a.magic);
+            if (q + HALF_AHEAD < 16) pj[q + HALF_AHEAD] = a.posq[js[q + HALF_AHEAD]];
+            pair_entry_half<ENERGY>(e[q], pj[q], js[q], pi, si, qi, a.pos4, ljrow, ljrow14, a.pc, acc, jacc, wcap,
+                                    a.magic);
         }
         acc.fx += (double)acc.tx;
         acc.fy += (double)acc.ty;
@@ -513,8 +545,8 @@ __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
         float m = 0.0f;
         for (int w = 0; w < WAVES; w++) m = fmaxf(m, sfmax[w]);
         // beyond 2^26 kJ/mol/A a component would leave the 2^51 range of the
-        // partner accumulators' rounding
-        if (!(m < 67108864.0f)) atomicOr(&a.status[ST_OVERFLOW], 16);
+        // partner accumulators' rounding: |force / r| rc < 2^26
+        if (!(m * m * a.pc.rc2f < 4503599627370496.0f)) atomicOr(&a.status[ST_OVERFLOW], 16);
     }
     if (ENERGY && threadIdx.x < 11) {
         double tq = 0.0;
@@ -524,7 +556,7 @@ __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
 }
 
 template <bool ENERGY, int WAVES>
-__global__ void __launch_bounds__(64 * WAVES) k_pair_half(PairHalfPack a) {
+__global__ void __launch_bounds__(64 * WAVES) HALF_WPE k_pair_half(PairHalfPack a) {
     if (a.step) {
         if (a.status[ST_OVERFLOW] & TTC_OVF_REDO) return;   // a rebuild of this loop overflowed
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -558,6 +590,15 @@ static PairConst pair_const(const ttc_ctx *c) {
     pc.e14 = (float)c->p.e14fac;
     pc.one_m_e14 = (float)(1.0 - c->p.e14fac);
     pc.ntypes = c->ntypes;
+    pc.ux = c->p.lx / 4294967296.0;
+    pc.uy = c->p.ly / 4294967296.0;
+    pc.uz = c->p.lz / 4294967296.0;
+    // the fixed-point r^2 differs from the FP64 positions' r^2 by the
+    // coordinates' quantization, 2 r sqrt(3) 2^-33 L: below 1e-9 of rc^2
+    // for any box the list allows (L <= 1e4 rc); outside +-1e-8 it decides
+    // the cutoff alone
+    pc.rc2_lo = pc.rc2 * (1.0 - 1e-8);
+    pc.rc2_hi = pc.rc2 * (1.0 + 1e-8);
     return pc;
 }
 
@@ -594,8 +635,24 @@ size_t ttc_pair_half_shm(const ttc_ctx *c) {
            sizeof(float2) * 2 * (size_t)c->ntypes * c->ntypes;
 }
 
+namespace {
+// posq[s] = {x, y, z} as 32-bit fixed point (x 2^32 / L, wrapped modulo
+// 2^32, i.e. modulo the box) and the charge's bits, from pos4
+__global__ void __launch_bounds__(256) k_posq(int n, const double4 *__restrict__ pos4, int4 *__restrict__ posq,
+                                              double kx, double ky, double kz) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    double4 p = pos4[s];
+    posq[s] = make_int4((int)(unsigned)__double2ll_rn(p.x * kx), (int)(unsigned)__double2ll_rn(p.y * ky),
+                        (int)(unsigned)__double2ll_rn(p.z * kz), __float_as_int((float)p.w));
+}
+}  // namespace
+
 static void launch_pair_half(ttc_ctx *c, bool energy, bool step) {
-    PairHalfPack a{c->n, c->cap, c->wcap, c->nbl, c->glen, c->win, c->wlen, c->pos4, c->type, c->ljtab,
+    const double two32 = 4294967296.0;
+    k_posq<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, c->pos4, c->posq, two32 / c->p.lx, two32 / c->p.ly,
+                                                       two32 / c->p.lz);
+    PairHalfPack a{c->n, c->cap, c->wcap, c->nbl, c->glen, c->win, c->wlen, c->pos4, c->posq, c->type, c->ljtab,
                    pair_const(c), c->fpair, c->fixj, c->partials, c->nblk_max, c->status, step ? 1 : 0,
                    6755399441055744.0};
     size_t shm = ttc_pair_half_shm(c);
