@@ -1351,9 +1351,14 @@ extern "C" int ttc_mc_delta_energy_noalloc(ttc_mc *m, int w, int lb, int ub, con
 extern "C" int ttc_mc_run(ttc_mc *m, int ntrials, int mode, double lnf, int log) {
     MCHECK(m && ntrials >= 0 && (mode == 0 || mode == 1), TTC_ERR_ARG, "ttc_mc_run: bad argument");
     double escale = 2.5;
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    // timing events destroyed on every return path
+    struct Ev {
+        hipEvent_t e = nullptr;
+        ~Ev() { if (e) hipEventDestroy(e); }
+    } g0, g1;
+    HIPCHK(hipEventCreate(&g0.e));
+    HIPCHK(hipEventCreate(&g1.e));
+    hipEvent_t e0 = g0.e, e1 = g1.e;
     HIPCHK(hipEventRecord(e0, m->stream));
     // at least half the CU's LDS per workgroup: one walker per CU, so no CU
     // holds two walkers while another holds none
@@ -1375,8 +1380,6 @@ extern "C" int ttc_mc_run(ttc_mc *m, int ntrials, int mode, double lnf, int log)
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     m->t_last = 1000.0 * ms;
     return TTC_OK;
 }
