@@ -127,10 +127,12 @@ def test_trr_of_a_device_run_matches_the_oracle_run(tmp_path):
         np.testing.assert_array_equal(fg["box"], fr["box"])
         dx = fg["x"] - fr["x"]
         dx -= fg["box"] * np.round(dx / fg["box"])
-        dv = np.abs(fg["v"] - fr["v"]).max()
+        # velocities reach thousands of A/ps on the strained start: compared
+        # relative to the largest, at the trr's float resolution
+        dv = np.abs(fg["v"] - fr["v"]).max() / np.abs(fr["v"]).max()
         ef = rel_rms(fg["f"], fr["f"])
-        print(f"step {fg['step']}: max |dx| {np.abs(dx).max():.2e} nm, max |dv| {dv:.2e} nm/ps, "
+        print(f"step {fg['step']}: max |dx| {np.abs(dx).max():.2e} nm, max |dv| / max |v| {dv:.2e}, "
               f"force rel rms {ef:.2e}", flush=True)
         assert np.abs(dx).max() < 2e-7
-        assert dv < 1e-5
+        assert dv < 1e-6
         assert ef < 5e-6
