@@ -128,11 +128,12 @@ def test_trr_of_a_device_run_matches_the_oracle_run(tmp_path):
         dx = fg["x"] - fr["x"]
         dx -= fg["box"] * np.round(dx / fg["box"])
         # velocities reach thousands of A/ps on the strained start: compared
-        # relative to the largest, at the trr's float resolution
+        # relative to the largest, to the force tolerance (the FP32 pair
+        # terms' 1e-6 force differences grow along the trajectory)
         dv = np.abs(fg["v"] - fr["v"]).max() / np.abs(fr["v"]).max()
         ef = rel_rms(fg["f"], fr["f"])
         print(f"step {fg['step']}: max |dx| {np.abs(dx).max():.2e} nm, max |dv| / max |v| {dv:.2e}, "
               f"force rel rms {ef:.2e}", flush=True)
         assert np.abs(dx).max() < 2e-7
-        assert dv < 1e-6
+        assert dv < 5e-6
         assert ef < 5e-6

@@ -247,6 +247,89 @@ void enbnd_mc(int lb, int ub, int ibox, double *energy)
     ok(ttc_enbnd_mc(upload(ibox), lb, ub, energy), "enbnd_mc");
 }
 
+/* The angle-type terms of box k (bend[k], urey[k], tors[k], impr[k] as
+ * read_topology.c fills them) on the device, for the device-resident loop;
+ * the replaced cnonbondnl() never evaluates them (forces.c calls cbend.c ..
+ * cimproper.c on the host). */
+static void device_angles(int k, ttc_ctx *c)
+{
+    static int done[NBOXES];
+    if (done[k]) return;
+    done[k] = 1;
+    int nb = bendN[k], nt = torsN[k], ni = imprN[k], nu = 0;
+#ifndef NEUTRAL
+    nu = ureyN[k];
+#endif
+    if (!nb && !nt && !ni && !nu) return;
+    int *bi = xmalloc(sizeof(int) * 3 * (nb + 1)), *ui = xmalloc(sizeof(int) * 2 * (nu + 1));
+    int *ti = xmalloc(sizeof(int) * 4 * (nt + 1)), *ii = xmalloc(sizeof(int) * 4 * (ni + 1));
+    double *beq = xmalloc(sizeof(double) * (nb + 1)), *bk = xmalloc(sizeof(double) * (nb + 1));
+    double *us = xmalloc(sizeof(double) * (nu + 1)), *uk = xmalloc(sizeof(double) * (nu + 1));
+    double *kp = xmalloc(sizeof(double) * 6 * (nt + 1)), *np = xmalloc(sizeof(double) * 6 * (nt + 1));
+    double *dp = xmalloc(sizeof(double) * 6 * (nt + 1));
+    double *ik = xmalloc(sizeof(double) * (ni + 1)), *ieq = xmalloc(sizeof(double) * (ni + 1));
+    for (int i = 0; i < nb; i++) {
+        bi[3 * i] = bend[k][i].a; bi[3 * i + 1] = bend[k][i].b; bi[3 * i + 2] = bend[k][i].c;
+        beq[i] = bend[k][i].angeq; bk[i] = bend[k][i].krbend;
+    }
+#ifndef NEUTRAL
+    for (int i = 0; i < nu; i++) {
+        ui[2 * i] = urey[k][i].a; ui[2 * i + 1] = urey[k][i].b; us[i] = urey[k][i].Seq; uk[i] = urey[k][i].k;
+    }
+#endif
+    for (int i = 0; i < nt; i++) {
+        ti[4 * i] = tors[k][i].a; ti[4 * i + 1] = tors[k][i].b; ti[4 * i + 2] = tors[k][i].c; ti[4 * i + 3] = tors[k][i].d;
+        for (int s = 0; s < 6; s++) {
+            kp[6 * i + s] = tors[k][i].kphi[s]; np[6 * i + s] = tors[k][i].nphi[s]; dp[6 * i + s] = tors[k][i].delphi[s];
+        }
+    }
+    for (int i = 0; i < ni; i++) {
+        ii[4 * i] = impr[k][i].a; ii[4 * i + 1] = impr[k][i].b; ii[4 * i + 2] = impr[k][i].c; ii[4 * i + 3] = impr[k][i].d;
+        ik[i] = impr[k][i].kimpr; ieq[i] = impr[k][i].angeq;
+    }
+    int rc = ttc_set_angles(c, nb, bi, beq, bk, nu, ui, us, uk, nt, ti, kp, np, dp, ni, ii, ik, ieq);
+    free(bi); free(ui); free(ti); free(ii); free(beq); free(bk); free(us); free(uk); free(kp); free(np); free(dp);
+    free(ik); free(ieq);
+    ok(rc, "ttc_set_angles");
+}
+
+void forces(int);
+
+/* integrate_vverlet(k, vscal) repeated nsteps times with the DLIST trigger
+ * and rebuild of nve_md.c:369-370 before each step, as one device-resident
+ * loop (ttc_md_vverlet: state stays in HBM, one captured graph per step)
+ * instead of nsteps host round trips.  Starts from the host state (atom[k],
+ * vv[k]) and leaves it as the reference's own loop would: atom[k] wrapped,
+ * atnopbc[k] advanced by each site's minimum-image displacement over the
+ * call, vv[k] and uu[k] the final velocities, and ff[k], en[k], pvir[k] from
+ * a closing forces(k) at the final positions -- so write_trr.c and the
+ * A/B restart files of checkpoint.c (WriteCheckpoint / ReadCheckpoint) see
+ * the current state.  No reference counterpart as one call. */
+void ttc_shim_md_vverlet(int k, int nsteps, double vscal)
+{
+    ttc_ctx *c = upload(k);
+    int n = box[k].boxns;
+    device_angles(k, c);
+    ok(ttc_set_velocities(c, &vv[k][0].x, 3), "ttc_set_velocities");
+    ok(ttc_md_vverlet(c, nsteps, vscal, 0, NULL, 0, NULL, NULL), "integrate_vverlet");
+    double *x = xmalloc(sizeof(double) * 3 * (size_t)n);
+    ok(ttc_get_positions(c, x, 3), "ttc_get_positions");
+    ok(ttc_get_velocities(c, &vv[k][0].x, 3), "ttc_get_velocities");
+    const double L[3] = {box[k].lx, box[k].ly, box[k].lz};
+    for (int i = 0; i < n; i++) {
+        double *a = &atom[k][i].x, *u = &atnopbc[k][i].x;
+        for (int d = 0; d < 3; d++) {
+            double dd = x[3 * i + d] - a[d];
+            dd -= L[d] * rint(dd / L[d]);
+            u[d] += dd;
+            a[d] = x[3 * i + d];
+        }
+        uu[k][i] = vv[k][i];
+    }
+    free(x);
+    forces(k);
+}
+
 /* ewald.c:817 -- Numerical Recipes erfcc (Chebyshev fit, fractional error
  * below 1.2e-7), kept on the host for the remaining host callers
  * (cgolik.c, enbnd2_mc.c, ewald.c's classic-Ewald routines) */
