@@ -124,6 +124,25 @@ int  ttc_mc_run(ttc_mc *m, int ntrials, int mode, double lnf, int log);
  * (checkpoint.c:146-917); counters, network and log are not part of it.  */
 int  ttc_mc_checkpoint(ttc_mc *m, int restore);
 
+/* swap_mpi.c:24-188 (a DOS run with one energy window per MPI rank) on the
+ * batch: walkers are the windows [emin, emin + nbins*ebin), and each pair
+ * (w, w+1) with w = parity mod 2 exchanges configurations when each energy
+ * lies inside the other's window (swap_mpi.c:147-149, the
+ * msg_swap_request / msg_swap_accept exchange of defines.h:1846-1874
+ * becomes device reads of the partner's state).  Every walker of a pair
+ * then counts the attempt in the bin of its energy after it: h += 1,
+ * ln g += lnf (swap_mpi.c:78-80, 131-133, 175-177).  The pairs must hold
+ * the same chain (bead count and native contacts), else TTC_ERR_ARG.
+ * accepted (optional, nwalk ints): 1 for both walkers of a swapped pair.  */
+int  ttc_mc_swap_windows(ttc_mc *m, int parity, double lnf, int *accepted);
+/* swap_mpi.c:190-296 merge_gofe: ln g of walker w (left window) and w + 1
+ * (right) made to agree on their overlap (averaged slopes integrated from
+ * the bin before the overlap, the right window's upper bins shifted to join,
+ * histograms raised by (int)(change / lnf) floored at zero).  The windows
+ * share the bin grid and overlap by 2 .. nbins-1 bins; the caller decides
+ * when (the reference merges when mod_f < MERGE_F on its swap schedule).  */
+int  ttc_mc_merge_windows(ttc_mc *m, int w, double lnf);
+
 int  ttc_mc_get_stats(ttc_mc *m, int64_t *stats /* nwalk * TTC_MC_NSTAT */);
 int  ttc_mc_reset_stats(ttc_mc *m);
 int  ttc_mc_get_dos(ttc_mc *m, int w, double *lng, int64_t *hist);

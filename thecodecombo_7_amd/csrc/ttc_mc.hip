@@ -64,6 +64,7 @@ struct ttc_mc {
     int logcap = 0;
     hipStream_t stream = nullptr;
     std::vector<int> hn;        // host copy of bead counts
+    std::vector<uint64_t> chain_sig; // per walker: hash of bead count and native contacts
     // device copy of the sampling state taken by ttc_mc_checkpoint
     double *ck_pos = nullptr, *ck_ecur = nullptr, *ck_lng = nullptr;
     long long *ck_hist = nullptr;
@@ -1191,6 +1192,7 @@ extern "C" int ttc_mc_create(const ttc_mc_params *p, ttc_mc **out) {
     HIPCHK(hipMemset(m->stats, 0, sizeof(long long) * W * TTC_MC_NSTAT));
     HIPCHK(hipMemset(m->adam, 0, sizeof(float) * 2 * NPAR));
     m->hn.assign(W, 0);
+    m->chain_sig.assign(W, 0);
     // network: small uniform init (seeded)
     std::vector<float> w0(NPAR);
     uint64_t sx = p->seed ^ 0x5DEECE66Dull;
@@ -1259,6 +1261,17 @@ extern "C" int ttc_mc_set_walker(ttc_mc *m, int w, int n, const double *pos, int
     HIPCHK(hipMemcpy(m->nbead + w, &n, sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m->emin + w, &emin, sizeof(double), hipMemcpyHostToDevice));
     m->hn[w] = n;
+    {   // FNV-1a over the chain's identity: bead count, CSR offsets, partners, r0
+        uint64_t h = 1469598103934665603ull;
+        auto mix = [&h](const void *p, size_t nb) {
+            const unsigned char *c = (const unsigned char *)p;
+            for (size_t i = 0; i < nb; i++) { h ^= c[i]; h *= 1099511628211ull; }
+        };
+        mix(&n, sizeof n);
+        mix(off.data(), sizeof(int) * off.size());
+        if (!part.empty()) { mix(part.data(), sizeof(int) * part.size()); mix(rr.data(), sizeof(double) * rr.size()); }
+        m->chain_sig[w] = h;
+    }
     // current energy of the walker
     std::vector<double> e(m->p.nwalk);
     int r = ttc_mc_energy(m, e.data());
@@ -1419,6 +1432,127 @@ extern "C" int ttc_mc_reset_stats(ttc_mc *m) {
     MCHECK(m, TTC_ERR_ARG, "ttc_mc_reset_stats: null");
     HIPCHK(hipMemsetAsync(m->stats, 0, sizeof(long long) * m->p.nwalk * TTC_MC_NSTAT, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
+    return TTC_OK;
+}
+
+// swap_mpi.c:24-188 as one kernel over the batch: block p takes the pair
+// (w, w+1), w = 2p + parity.  The sender's request (energy, window) and the
+// receiver's answer (accept, energy) of msg_swap_request / msg_swap_accept
+// (defines.h:1846-1874) are reads of the partner's device state.
+__global__ void k_mc_swap(int parity, int nwalk, int maxn, int nbins, double ebin, double lnf, double *pos,
+                          const int *nbead, const double *emin, double *ecur, double *lng, long long *hist,
+                          int *accepted) {
+    const int w = 2 * blockIdx.x + parity;
+    if (w + 1 >= nwalk) return;
+    __shared__ int acc;
+    const double e1 = ecur[w], e2 = ecur[w + 1];
+    const double top = nbins * ebin;
+    if (threadIdx.x == 0) {
+        // swap_mpi.c:147-149: each energy inside the other's window
+        acc = (e2 >= emin[w] && e2 < emin[w] + top) && (e1 >= emin[w + 1] && e1 < emin[w + 1] + top);
+    }
+    __syncthreads();
+    if (acc) {
+        double *p1 = pos + (size_t)w * maxn * 3, *p2 = pos + (size_t)(w + 1) * maxn * 3;
+        for (int i = threadIdx.x; i < 3 * nbead[w]; i += blockDim.x) {
+            double t = p1[i];
+            p1[i] = p2[i];
+            p2[i] = t;
+        }
+    }
+    if (threadIdx.x == 0) {
+        const double ew[2] = {acc ? e2 : e1, acc ? e1 : e2};
+        for (int s = 0; s < 2; s++) {
+            // the trial counts in the bin of the walker's energy after the
+            // attempt (swap_mpi.c:78-80, 131-133, 175-177), binned as the
+            // walkers' own trials are
+            const int v = w + s;
+            const int b = (int)floor((ew[s] - emin[v]) * (1.0 / ebin));
+            if (b >= 0 && b < nbins) {
+                hist[(size_t)v * nbins + b] += 1;
+                lng[(size_t)v * nbins + b] += lnf;
+            }
+            ecur[v] = ew[s];
+            if (accepted) accepted[v] = acc;
+        }
+    }
+}
+
+extern "C" int ttc_mc_swap_windows(ttc_mc *m, int parity, double lnf, int *accepted) {
+    MCHECK(m && (parity == 0 || parity == 1) && lnf >= 0.0, TTC_ERR_ARG, "ttc_mc_swap_windows: bad argument");
+    const int W = m->p.nwalk;
+    for (int w = parity; w + 1 < W; w += 2)
+        MCHECK(m->hn[w] > 0 && m->chain_sig[w] == m->chain_sig[w + 1], TTC_ERR_ARG,
+               "ttc_mc_swap_windows: neighbouring windows must hold the same chain");
+    const int npair = (W - parity) / 2;
+    if (npair <= 0) return TTC_OK;
+    int *dacc = nullptr;
+    if (accepted) HIPCHK(hipMallocAsync((void **)&dacc, sizeof(int) * W, m->stream));
+    if (accepted) HIPCHK(hipMemsetAsync(dacc, 0, sizeof(int) * W, m->stream));
+    k_mc_swap<<<npair, 256, 0, m->stream>>>(parity, W, m->p.maxn, m->p.nbins, m->p.ebin, lnf, m->pos, m->nbead,
+                                             m->emin, m->ecur, m->lng, m->hist, dacc);
+    HIPCHK(hipGetLastError());
+    if (accepted) {
+        HIPCHK(hipMemcpyAsync(accepted, dacc, sizeof(int) * W, hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipFreeAsync(dacc, m->stream));
+    }
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return TTC_OK;
+}
+
+// swap_mpi.c:190-296 merge_gofe(k, e_begin, e_end, mod_f): walker w is the
+// left window (the sender's dos_hist, received as dos_hist_temp), w + 1 the
+// right one.  ln g of both is made to agree on the overlap: the slopes of
+// the two windows' ln g (central differences, one-sided at the overlap's
+// ends) are averaged and integrated from the bin before the overlap in the
+// left window; the right window's bins above the overlap are shifted to
+// join; each adjusted bin's histogram gains (int)(change / lnf), floored at
+// zero.  The windows must share the bin grid, overlap by 2 to nbins - 1
+// bins and start the overlap at least one bin into the left window (the
+// reference reads past the histogram otherwise).
+extern "C" int ttc_mc_merge_windows(ttc_mc *m, int w, double lnf) {
+    MCHECK(m && w >= 0 && w + 1 < m->p.nwalk && lnf > 0.0, TTC_ERR_ARG, "ttc_mc_merge_windows: bad argument");
+    const int B = m->p.nbins;
+    const double eb = m->p.ebin;
+    double em[2];
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy(em, m->emin + w, sizeof(double) * 2, hipMemcpyDeviceToHost));
+    const double e_begin = em[0], e_end = em[0] + B * eb;
+    const int e_bins = (int)((e_end - e_begin) / eb + 0.5);
+    const int o_bins = (int)((e_end - em[1]) / eb);
+    const int j = e_bins - o_bins;
+    MCHECK(B >= 3 && o_bins >= 2 && o_bins < B && j >= 1, TTC_ERR_ARG,
+           "ttc_mc_merge_windows: windows must overlap by 2 to nbins - 1 bins, the right one starting inside the left");
+    std::vector<double> gl(B), gr(B);
+    std::vector<long long> hl(B), hr(B);
+    HIPCHK(hipMemcpy(gl.data(), m->lng + (size_t)w * B, sizeof(double) * B, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(gr.data(), m->lng + (size_t)(w + 1) * B, sizeof(double) * B, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hl.data(), m->hist + (size_t)w * B, sizeof(long long) * B, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hr.data(), m->hist + (size_t)(w + 1) * B, sizeof(long long) * B, hipMemcpyDeviceToHost));
+    std::vector<double> tl(o_bins), tr(o_bins), tm(o_bins);
+    for (int i = 0; i < o_bins; i++) {
+        // slopes over two bins; the left window's last bin repeats the slope
+        // before it, the right window's first bin looks two bins ahead
+        tl[i] = (i == o_bins - 1) ? tl[i - 1] : gl[j + i + 1] - gl[j + i - 1];
+        tr[i] = (i == 0) ? gr[2] - gr[0] : gr[i + 1] - gr[i - 1];
+        tm[i] = (tl[i] + tr[i]) / 4.0;
+    }
+    for (int i = 0; i < o_bins; i++) {
+        const double old = gl[j + i];
+        gl[j + i] = gl[j + i - 1] + tm[i];
+        hl[j + i] = std::max(0LL, hl[j + i] + (long long)(int)((gl[j + i] - old) / lnf));
+    }
+    const double shift = gr[o_bins - 1] - gl[j + o_bins - 1];
+    for (int i = o_bins; i < B; i++) gr[i] -= shift;
+    for (int i = 0; i < o_bins; i++) {
+        const double old = gr[i];
+        gr[i] = gl[j + i];
+        hr[i] = std::max(0LL, hr[i] + (long long)(int)((gr[i] - old + shift) / lnf));
+    }
+    HIPCHK(hipMemcpy(m->lng + (size_t)w * B, gl.data(), sizeof(double) * B, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m->lng + (size_t)(w + 1) * B, gr.data(), sizeof(double) * B, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m->hist + (size_t)w * B, hl.data(), sizeof(long long) * B, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m->hist + (size_t)(w + 1) * B, hr.data(), sizeof(long long) * B, hipMemcpyHostToDevice));
     return TTC_OK;
 }
 

@@ -54,6 +54,8 @@ def _declare(L):
     L.ttc_mc_get_dos.argtypes = [vp, C.c_int, dp, i64p]
     L.ttc_mc_set_dos.argtypes = [vp, C.c_int, dp]
     L.ttc_mc_reset_hist.argtypes = [vp]
+    L.ttc_mc_swap_windows.argtypes = [vp, C.c_int, C.c_double, ip]
+    L.ttc_mc_merge_windows.argtypes = [vp, C.c_int, C.c_double]
     L.ttc_mc_get_log.argtypes = [vp, C.c_int, fp, dp, ip]
     L.ttc_mc_clear_log.argtypes = [vp]
     L.ttc_mc_last_time.argtypes = [vp, dp]
@@ -182,6 +184,18 @@ class Walkers:
 
     def reset_hist(self):
         T.check(self.L.ttc_mc_reset_hist(self.h))
+
+    def swap_windows(self, parity, lnf=0.0):
+        """swap_mpi.c: configuration exchange between neighbouring energy
+        windows (pairs (w, w+1), w = parity mod 2); returns the per-walker
+        accepted flags."""
+        acc = np.zeros(self.nwalk, np.int32)
+        T.check(self.L.ttc_mc_swap_windows(self.h, int(parity), float(lnf), T.iptr(acc)))
+        return acc
+
+    def merge_windows(self, w, lnf):
+        """swap_mpi.c merge_gofe of windows w (left) and w + 1 (right)."""
+        T.check(self.L.ttc_mc_merge_windows(self.h, int(w), float(lnf)))
 
     def flat_histogram(self, w, tol=0.8):
         """ctdos.c:268 flat_histogram: every visited bin at least tol x mean."""
