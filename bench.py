@@ -113,47 +113,57 @@ def mc_leg(ntrials: int, cpu: bool):
     }
     out["da_gain_accepted_per_s"] = round(out["da_accepted_moves_per_s"] / out["plain_accepted_moves_per_s"], 3)
     if cpu:
-        out["cpu_baseline"] = mc_cpu_baseline(chains[:4], st0[:, 1].sum() / st0[:, 0].sum())
+        w.restore()                                  # the plain run's start state, re-run to its end
+        w.run(ntrials, mode=0, lnf=lnf)
+        out["cpu_baseline"] = mc_cpu_baseline(w, chains, lnf, w.emin)
     w.close()
     return out
 
 
-def mc_cpu_baseline(chains, acc_ratio):
-    """Plain exact MC on one host core: the oracle's FP64 range energy
-    (forces_mc semantics) for old and new state of each trial."""
+def mc_cpu_baseline(w, chains, lnf, emin):
+    """Plain exact Wang-Landau MC on one host core: the oracle's orc_wl_run
+    (the restatement of the reference's pivot / displacement trial with
+    pivot.c:236-253 acceptance, FP64 energy changes; built with the
+    reference's flags) continues eight walkers of the batch, every 32nd by
+    size so the sample has the batch's size mix, from their state after the
+    GPU's plain run (positions, ln g, histograms, random streams)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from thecodecombo_7_amd import system
-    L = pyoracle.lib(fast=True)
-    rng = np.random.default_rng(1)
-    ntr = 0
-    t0 = time.perf_counter()
-    for ch in chains:
-        n = ch.n
-        nat = np.zeros((n, n), np.uint8); r0 = np.zeros((n, n))
-        for (a, b), r in zip(ch.contacts, ch.r0):
-            nat[a, b] = nat[b, a] = 1
-            r0[a, b] = r0[b, a] = r
-        pos = ch.pos.copy()
-        for _ in range(60):
-            p = int(rng.integers(1, n - 1))
-            lb, ub = (0, p) if rng.random() < 0.5 else (p + 1, n)
-            th = rng.random() * np.pi * 0.25
-            c, s = np.cos(th), np.sin(th)
-            R = np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
-            new = pos.copy()
-            new[lb:ub] = (pos[lb:ub] - pos[p]) @ R.T + pos[p]
-            e = lambda q: L.orc_go_range_energy(n, np.ascontiguousarray(q).ravel(), lb, ub, system.GO_KBOND,
-                                                system.GO_BOND, nat.ravel(), r0.ravel(), system.GO_EPS,
-                                                system.GO_SIG_REP, 1e30)
-            de = e(new) - e(pos)
-            if de < 0:
-                pos = new
-            ntr += 1
-    dt = time.perf_counter() - t0
-    return {"value": round(ntr / dt * acc_ratio, 2), "unit": "accepted moves/s", "cores": 1, "kind": "port",
-            "sample": f"{ntr} pivot trials on {len(chains)} walkers ({sum(c.n for c in chains)} beads), "
-                      "oracle FP64 range energy old+new per trial; scaled by the GPU plain acceptance ratio"}
+    order = np.argsort([c.n for c in chains])
+    pick = order[16::32]
+    states = []
+    for k in pick:
+        ch = chains[k]
+        pos, e = w.walker(int(k))
+        lng, hist = w.dos(int(k))
+        nat, r0m = pyoracle.go_tables(ch.n, ch.contacts, ch.r0)
+        states.append([np.ascontiguousarray(pos.ravel()), e, nat, r0m, lng.copy(), hist.copy(),
+                       np.ascontiguousarray(w.streams(int(k))), ch.n, float(emin[k])])
+
+    def params(em):
+        return pyoracle.WlParams(kbond=system.GO_KBOND, r0bond=system.GO_BOND, eps=system.GO_EPS,
+                                 sig_rep=system.GO_SIG_REP, pivot_frac=w.p.pivot_frac, theta_max=w.p.theta_max,
+                                 disp_max=w.p.disp_max, emin=em, ebin=w.ebin, nbins=w.nbins)
+
+    def sweep(ntr):
+        acc = trials = 0
+        t0 = time.perf_counter()
+        for st in states:
+            stats = np.zeros(7, np.int64)
+            st[1] = pyoracle.wl_run(params(st[8]), st[0], st[1], st[2], st[3], st[4], st[5], st[6], ntr, lnf,
+                                    stats, fast=True)
+            acc += int(stats[1]); trials += int(stats[0])
+        return acc, trials, time.perf_counter() - t0
+
+    _, _, t_cal = sweep(64)                    # calibration: about 10 s of work in the timed sweep
+    ntr = int(min(20000, max(256, 64 * 10.0 / max(t_cal, 1e-3))))
+    acc, trials, dt = sweep(ntr)
+    return {"value": round(acc / dt, 2), "unit": "accepted moves/s", "cores": 1, "kind": "port",
+            "sample": f"{trials} Wang-Landau trials ({ntr} per walker) on 8 walkers of the batch "
+                      f"({', '.join(str(int(st[7])) for st in states)} beads) from their state after the GPU's "
+                      f"plain run: oracle orc_wl_run (FP64, reference flags, 1 thread), {acc} accepted in "
+                      f"{dt:.1f} s"}
 
 
 def large_leg(steps: int, cpu: bool):
@@ -229,23 +239,31 @@ def replica_leg(steps: int, cpu: bool):
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
-        errs, t_cpu = [], 0.0
-        for b, x in zip(rb.boxes, systems):
+        import ctypes as C
+        errs, t_list, t_step = [], np.zeros(nrep), np.zeros(nrep)
+        for r_, (b, x) in enumerate(zip(rb.boxes, systems)):
             ff, _ = b.forces()
             o = pyoracle.Oracle(x, fast=True)
             t0 = time.perf_counter()
             rc, cnt, lst, xf = o.nblist(x.pos, 1500, cells=True)
-            fr, _ = o.forces(x.pos, lists=(cnt, lst, xf))
-            t_cpu += time.perf_counter() - t0
+            t_list[r_] = time.perf_counter() - t0
             if rc:
                 raise RuntimeError("oracle list overflow in the replica leg")
+            t0 = time.perf_counter()
+            fr, _ = o.forces(x.pos, lists=(cnt, lst, xf))
+            # one NHC step's integration (nvt_md.c:528-531): thermostat half,
+            # velocity-Verlet halves, thermostat half
+            p = x.pos.ravel().copy(); v = x.vel.ravel().copy(); u = v.copy(); f = fr.ravel().copy()
+            zeta = np.zeros(2 * 4)
+            w = np.ones(1)
+            nfree = 3.0 * x.n - 3.0
+            o.L.orc_nhc(x.n, v, u, x.mass, nfree, float(temps[r_]), 1e-18, 2, 1, 1, w, zeta, x.dt)
+            o.L.orc_vverlet_a(C.byref(o.cs), x.n, x.mass, x.dt, 1.0, p, None, v, f)
+            o.L.orc_vverlet_b(x.n, x.mass, x.dt, 1.0, v, u, f)
+            o.L.orc_nhc(x.n, v, u, x.mass, nfree, float(temps[r_]), 1e-18, 2, 1, 1, w, zeta, x.dt)
+            t_step[r_] = time.perf_counter() - t0
             errs.append(float(np.sqrt(((ff - fr) ** 2).sum() / (fr ** 2).sum())))
         out["force_rel_rms_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs)), "min": min(errs)}
-        out["cpu_baseline"] = {"value": round(nrep * 0.0864 / t_cpu, 5), "unit": "aggregate ns/day", "cores": 1,
-                               "kind": "port",
-                               "sample": "oracle (reference flags, 1 thread): cell-list build + one force "
-                                         "evaluation of every replica at step 0, summed, as one step of all "
-                                         "replicas (integration and thermostat not counted)"}
     mode, spl, tt = rb.autotune(50)
     rb.run(500, exchange_every=500)
     t0 = time.perf_counter()
@@ -256,6 +274,19 @@ def replica_leg(steps: int, cpu: bool):
                 "aggregate_ns_per_day": round(nrep * 0.0864 / wall, 2),
                 "site_ns_per_day": round(out["sites"] * 0.0864 / wall, 1),
                 "exchange_attempts": int(r["attempts"]), "exchange_accepts": int(r["accepts"])})
+    if cpu:
+        # one step of all replicas on one core: every replica's force
+        # evaluation and NHC integration, plus its cell-list build amortised
+        # at the rebuild rate the GPU run observed for it
+        reb = np.asarray(r["rebuilds"], np.float64) / steps
+        t_cpu = float((t_step + t_list * reb).sum())
+        out["cpu_baseline"] = {"value": round(nrep * 0.0864 / t_cpu, 5), "unit": "aggregate ns/day", "cores": 1,
+                               "kind": "port",
+                               "sample": "oracle (reference flags, 1 thread) on every replica's step-0 state: "
+                                         "one force evaluation (pair loop, PME, bonds) and one NHC step's "
+                                         "integration (orc_nhc, orc_vverlet_a/b, orc_nhc) per replica, plus its "
+                                         f"cell-list build ({t_list.sum():.2f} s for all) amortised at the "
+                                         f"GPU run's rebuild rate ({reb.mean():.3f} per step on average)"}
     rb.close()
     return out
 

@@ -408,9 +408,10 @@ def go_tables(n, contacts, r0):
     return nat, r0m
 
 
-def wl_run(params: WlParams, pos, ecur, nat, r0m, lng, hist, streams, ntrials, lnf, stats):
-    """orc_wl_run on numpy arrays, all updated in place; returns the new ecur."""
-    L = lib()
+def wl_run(params: WlParams, pos, ecur, nat, r0m, lng, hist, streams, ntrials, lnf, stats, fast=False):
+    """orc_wl_run on numpy arrays, all updated in place; returns the new ecur.
+    fast: the build with the reference's own compiler flags (CPU baseline)."""
+    L = lib(fast)
     e = C.c_double(ecur)
     for a, dt in ((pos, np.float64), (nat, np.uint8), (r0m, np.float64), (lng, np.float64), (hist, np.int64),
                   (streams, np.uint64), (stats, np.int64)):
