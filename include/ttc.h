@@ -279,6 +279,16 @@ int  ttc_timing(ttc_ctx *c, int which, double *usec);
  * waves per 64-site group on the current list and keeps the fastest
  * (usec3[3] optional: the three timings).  No reference counterpart.    */
 int  ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3);
+/* Autotuner of the charge spreading: sites per pass of the spread kernel
+ * (64, 128, 256, those that fit the LDS) timed on the current positions;
+ * the fastest stays (usec3 optional, 0 for a chunk that does not fit).    */
+int  ttc_tune_spread(ttc_ctx *c, int *chunk, double *usec3);
+/* Both tuners behind a cache file: a text line per context shape (site
+ * count, box, grid, spline order, cutoffs, list format, type count, GPU
+ * architecture and CU count) with the choices; a matching line is applied
+ * without timing (*from_cache = 1), otherwise the tuners run and a line is
+ * appended.  No reference counterpart.                                    */
+int  ttc_tune_cached(ttc_ctx *c, const char *path, int *pair_waves, int *spread_chunk, int *from_cache);
 
 /* The force array ff[k] of the last evaluation at the current positions:
  * after ttc_md_vverlet the forces the last step computed

@@ -91,6 +91,8 @@ def lib():
     L.ttc_enbnd_mc.argtypes = [vp, C.c_int, C.c_int, dp]
     L.ttc_nblist_pivot.argtypes = [vp, C.c_int]
     L.ttc_tune.argtypes = [vp, ip, dp]
+    L.ttc_tune_spread.argtypes = [vp, ip, dp]
+    L.ttc_tune_cached.argtypes = [vp, C.c_char_p, ip, ip, ip]
     L.ttc_cnonbondnl2.argtypes = [vp, dp, C.c_int64, dp, dp]
     L.ttc_cbond2.argtypes = [vp, dp, C.c_int64, dp, dp]
     L.ttc_configtemp.argtypes = [vp, dp, dp, dp, dp]

@@ -1205,6 +1205,32 @@ extern "C" int ttc_timing(ttc_ctx *c, int which, double *usec) {
 // Times the pair kernel at 4, 8 and 16 waves per 64-site group on the
 // current list (ttc_timing(0): HIP events around 20 launches) and keeps the
 // fastest; captured graphs are re-made if the choice changes.
+int ttc_spread_chunk_ok(const ttc_ctx *c, int chunk);
+int ttc_spread_attrs(ttc_ctx *c);
+
+// the spread chunk: 64, 128 and 256 sites per pass where they fit, timed
+// with ttc_timing(1); the fastest stays
+static int tune_spread(ttc_ctx *c, double *usec3) {
+    const int cand[3] = {64, 128, 256};
+    const int old = c->spread_chunk;
+    int best = old;
+    double tbest = 1e300;
+    for (int i = 0; i < 3; i++) {
+        if (usec3) usec3[i] = 0.0;
+        if (!ttc_spread_chunk_ok(c, cand[i])) continue;
+        c->spread_chunk = cand[i];
+        int r = ttc_spread_attrs(c);
+        double t = 0.0;
+        if (!r) r = ttc_timing(c, 1, &t);
+        if (r) { c->spread_chunk = old; ttc_spread_attrs(c); return r; }
+        if (usec3) usec3[i] = t;
+        if (t < tbest) { tbest = t; best = cand[i]; }
+    }
+    c->spread_chunk = best;
+    c->t_spread = tbest;
+    return ttc_spread_attrs(c);
+}
+
 extern "C" int ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3) {
     CHECK(c && c->topo, TTC_ERR_ARG, "ttc_tune: topology not set");
     const int cand[3] = {4, 8, 16};
@@ -1232,6 +1258,85 @@ extern "C" int ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3) {
         c->gen++;
     }
     if (pair_waves) *pair_waves = best;
+    return TTC_OK;
+}
+
+extern "C" int ttc_tune_spread(ttc_ctx *c, int *chunk, double *usec3) {
+    CHECK(c && c->topo, TTC_ERR_ARG, "ttc_tune_spread: topology not set");
+    const int old = c->spread_chunk;
+    int r = tune_spread(c, usec3);
+    if (r) return r;
+    if (c->spread_chunk != old) {
+        destroy_graphs(c);
+        c->gen++;
+    }
+    if (chunk) *chunk = c->spread_chunk;
+    return TTC_OK;
+}
+
+// A line of the tuning cache: the key of a context (sizes, box, grid,
+// cutoffs, list format, type count, GPU) and the choices made for it.
+static std::string tune_key(ttc_ctx *c) {
+    hipDeviceProp_t prop;
+    int dev = 0;
+    std::string arch = "unknown";
+    int cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        arch = prop.gcnArchName;
+        cus = prop.multiProcessorCount;
+    }
+    for (char &ch : arch)
+        if (ch == ' ' || ch == '\t') ch = '_';
+    char buf[512];
+    snprintf(buf, sizeof buf, "n=%d box=%.9g,%.9g,%.9g grid=%d,%d,%d order=%d rc=%.9g rclist=%.9g half=%d ntypes=%d gpu=%s/%d",
+             c->n, c->p.lx, c->p.ly, c->p.lz, c->p.gx, c->p.gy, c->p.gz, c->p.order, c->p.rc, c->p.rclist,
+             c->list_half ? 1 : 0, c->ntypes, arch.c_str(), cus);
+    return buf;
+}
+
+extern "C" int ttc_tune_cached(ttc_ctx *c, const char *path, int *pair_waves, int *spread_chunk, int *from_cache) {
+    CHECK(c && c->topo && path, TTC_ERR_ARG, "ttc_tune_cached: bad argument");
+    int r = need_list(c, true);
+    if (r) return r;
+    const std::string key = tune_key(c);
+    int w = 0, ch = 0, hit = 0;
+    if (FILE *f = fopen(path, "r")) {
+        char line[1024];
+        while (fgets(line, sizeof line, f)) {
+            char *tab = strchr(line, '\t');
+            if (!tab) continue;
+            *tab = 0;
+            int lw = 0, lc = 0;
+            if (key == line && sscanf(tab + 1, "%d %d", &lw, &lc) == 2 && (lw == 4 || lw == 8 || lw == 16) &&
+                (lc == 64 || lc == 128 || lc == 256) && !(c->list_half && lw == 16) && ttc_spread_chunk_ok(c, lc)) {
+                w = lw; ch = lc; hit = 1;   // the last valid line of the key wins
+            }
+        }
+        fclose(f);
+    }
+    if (hit) {
+        int &waves = c->list_half ? c->pair_waves_h : c->pair_waves;
+        const bool changed = waves != w || c->spread_chunk != ch;
+        waves = w;
+        c->spread_chunk = ch;
+        r = ttc_spread_attrs(c);
+        if (r) return r;
+        if (changed) {
+            destroy_graphs(c);
+            c->gen++;
+        }
+    } else {
+        r = ttc_tune(c, &w, nullptr);
+        if (!r) r = ttc_tune_spread(c, &ch, nullptr);
+        if (r) return r;
+        if (FILE *f = fopen(path, "a")) {
+            fprintf(f, "%s\t%d %d\n", key.c_str(), w, ch);
+            fclose(f);
+        }
+    }
+    if (pair_waves) *pair_waves = w;
+    if (spread_chunk) *spread_chunk = ch;
+    if (from_cache) *from_cache = hit;
     return TTC_OK;
 }
 

@@ -243,6 +243,7 @@ struct ttc_ctx {
     int span_pre = 0;               // its bit of the group's first site
     long long *fixj = nullptr;      // [3][n] partner forces, 2^-24 kJ/mol/A; read and cleared by the combine
     int pair_waves_h = 8;           // waves per group of the half-list pair kernel
+    int spread_chunk = 0;           // sites per spread pass (0: the occupancy rule)
     // limits below the capacities (0: none) at which MD-loop builds report
     // an overflow; cleared by the next growth (ttc_set_capacity_limits)
     int row_limit = 0, win_limit = 0;

@@ -189,10 +189,15 @@ __device__ __forceinline__ void pair_entry(uint32_t e, const double4 &pj, const 
 // slot as int64 2^-24 units: -f 2^24 is exact in FP32, its FP64 image plus
 // 1.5 2^52 (from an SGPR) rounds it to an integer held in the low mantissa
 // bits, and removing the magic's high word leaves the two's-complement value.
+// Per entry class (e >> 30: 0 plain, 1 excluded, 2 one-four, 3 padding) one
+// LDS record replaces the class selects: {eps factor, charge factor of the
+// real-space term (1/eps_RF, E14FAC/eps_RF for 1-4 pairs), charge factor of
+// the excluded-pair correction (1, 1 - E14FAC), offset of the 1-4 LJ table
+// in float2 entries (int bits)}; padding has every factor zero.
 template <bool ENERGY>
 __device__ __forceinline__ void pair_entry_half(uint32_t e, const int4 &pj, int j, const int4 &pi, int s, float qi,
                                                 const double4 *__restrict__ pos4, const float2 *ljrow,
-                                                const float2 *ljrow14, const PairConst &pc, PairAcc &a,
+                                                const float4 *ctab, const PairConst &pc, PairAcc &a,
                                                 unsigned long long *jacc, int wcap, double magic) {
     const int tj = (int)((e >> TTC_TSHIFT) & 63u);
     const uint32_t cls = e >> 30;
@@ -217,26 +222,30 @@ __device__ __forceinline__ void pair_entry_half(uint32_t e, const int4 &pj, int 
         in = ttc_r2_exact(fx, fy, fz) <= pc.rc2;
     }
     const float dx = (float)ex, dy = (float)ey, dz = (float)ez;
-    const float r2 = in ? (float)dr2 : pc.rc2f;
+    // lanes out of range take the padding record (every factor zero, so
+    // their force is zero) and compute on r^2 = 4, an inline constant; the
+    // record's index depends on the cutoff test, which also keeps its load
+    // next to its use
+    const float4 cf = ctab[in ? cls : 3u];
+    const float r2 = in ? (float)dr2 : 4.0f;
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float rinv2 = rinv * rinv;
     const float r = r2 * rinv;
-    const float2 lj = (cls == 2 ? ljrow14 : ljrow)[tj];
-    const float eps = cls == 1 ? 0.0f : lj.x;
+    const float2 lj = ljrow[tj + __float_as_int(cf.w)];
+    const float eps = lj.x * cf.x;
     const float sr2 = lj.y * rinv2;
     const float sr6 = sr2 * sr2 * sr2;
     const float sr12 = sr6 * sr6;
     float force = 12.0f * eps * rinv2 * (sr12 - sr6);
     const float qqab = qi * __int_as_float(pj.w);
-    const float qq = (cls == 0 ? qqab : (cls == 2 ? pc.e14 * qqab : 0.0f)) * pc.inv_epsrf;
+    const float qq = qqab * cf.y;
     float ez2;
     const float erfckr = nr_erfc(r, r2, pc, &ez2);
     const float kpid = pc.kpi2 * r * ez2;
     const float rinv3 = rinv * rinv2;
     force += qq * rinv3 * (erfckr + kpid);
-    const float qi2 = cls == 0 ? 0.0f : (cls == 2 ? pc.one_m_e14 * qqab : qqab);
+    const float qi2 = qqab * cf.z;
     force -= qi2 * rinv3 * ((1.0f - erfckr) - kpid);
-    force = in ? force : 0.0f;
     const float fxf = force * dx, fyf = force * dy, fzf = force * dz;
     a.tx += fxf;
     a.ty += fyf;
@@ -448,6 +457,13 @@ __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
     __shared__ double sred[WAVES][64][3];
     __shared__ double ered[WAVES][11];
     __shared__ float sfmax[WAVES];
+    __shared__ float4 ctab[4];
+    if (threadIdx.x == 0) {
+        ctab[0] = make_float4(1.0f, a.pc.inv_epsrf, 0.0f, __int_as_float(0));
+        ctab[1] = make_float4(0.0f, 0.0f, 1.0f, __int_as_float(0));
+        ctab[2] = make_float4(1.0f, a.pc.e14 * a.pc.inv_epsrf, a.pc.one_m_e14, __int_as_float(nt2));
+        ctab[3] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
+    }
     const int W = min(a.wlen[grp], wcap);
     const int *gwin = a.win + (size_t)grp * wcap;
     // slots past the window (only after an overflow, whose step is redone)
@@ -473,7 +489,7 @@ __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
     const int si = s < n ? s : 0;
     const int4 pi = a.posq[si];
     const int ti = s < n ? a.type[s] : 0;
-    const float2 *ljrow = slj + ti * a.pc.ntypes, *ljrow14 = slj + nt2 + ti * a.pc.ntypes;
+    const float2 *ljrow = slj + ti * a.pc.ntypes;   // the 1-4 row is nt2 entries further
     const float qi = __int_as_float(pi.w) * a.pc.qfac;
     const uint4 *seg = (const uint4 *)(a.nbl + (size_t)grp * a.cap * 64 + (size_t)lane * 16);
     const int ntile = (len + 15) >> 4;
@@ -500,7 +516,7 @@ __device__ __forceinline__ void pair_half_body(int grp, const PairHalfPack &a) {
 #pragma unroll
         for (int q = 0; q < 16; q++) {
             if (q + HALF_AHEAD < 16) pj[q + HALF_AHEAD] = a.posq[js[q + HALF_AHEAD]];
-            pair_entry_half<ENERGY>(e[q], pj[q], js[q], pi, si, qi, a.pos4, ljrow, ljrow14, a.pc, acc, jacc, wcap,
+            pair_entry_half<ENERGY>(e[q], pj[q], js[q], pi, si, qi, a.pos4, ljrow, ctab, a.pc, acc, jacc, wcap,
                                     a.magic);
         }
         acc.fx += (double)acc.tx;

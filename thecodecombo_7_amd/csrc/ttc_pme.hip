@@ -997,6 +997,7 @@ int ttc_pme_setup(ttc_ctx *c) {
     return TTC_OK;
 }
 
+int ttc_spread_attrs(ttc_ctx *c) { return set_lds_attrs(c); }
 int ttc_pme_attrs(ttc_ctx *c) { return set_lds_attrs(c);
 }
 
@@ -1046,15 +1047,28 @@ static Brick brick_args(const ttc_ctx *c) {
 
 // the brick with its LDS layout: tile, then the weights of `chunk` sites,
 // chunk the largest of 256/128/64 that keeps the occupancy of chunk 64
+// (ttc_tune may set the chunk instead: c->spread_chunk, kept only while it
+// fits the LDS)
 static Brick spread_brick(const ttc_ctx *c) {
     Brick br = brick_args(c);
     size_t tile = brick_shm(c, br);
     br.rec_off = (int)(tile / 8);
     br.chunk = 64;
+    if (c->spread_chunk > 0 && spread_occupancy(tile + (size_t)c->spread_chunk * SPREAD_REC_BYTES(c->order)) >= 1) {
+        br.chunk = c->spread_chunk;
+        return br;
+    }
     const int occ64 = spread_occupancy(tile + 64 * SPREAD_REC_BYTES(c->order));
     for (int ch : {256, 128})
         if (spread_occupancy(tile + (size_t)ch * SPREAD_REC_BYTES(c->order)) >= occ64) { br.chunk = ch; break; }
     return br;
+}
+
+// the chunks the tuner may try (64, 128, 256, those that fit the LDS)
+int ttc_spread_chunk_ok(const ttc_ctx *c, int chunk) {
+    Brick br = brick_args(c);
+    size_t tile = brick_shm(c, br);
+    return spread_occupancy(tile + (size_t)chunk * SPREAD_REC_BYTES(c->order)) >= 1;
 }
 
 // dynamic LDS of the spread: tile + weights
@@ -1122,13 +1136,22 @@ void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick) {
 
 int ttc_spread_shm(const ttc_ctx *c) { return (int)spread_shm(c, spread_brick(c)); }
 
-// allow the spread tile and the FFT tiles above 64 KiB of LDS (gfx950 has 160 KiB per CU)
+// allow the spread tile and the FFT tiles above 64 KiB of LDS (gfx950 has
+// 160 KiB per CU).  The attribute belongs to the kernel, not the context:
+// it only grows, so a context with a smaller tile (or a tuned chunk) never
+// lowers it below what another context launches with.
 static int set_lds_attrs(ttc_ctx *c) {
+    static int attr_max[64] = {0};   // per device
     int shm = (int)spread_shm(c, spread_brick(c));
     if (spread_occupancy((size_t)shm) < 1) {
         ttc_set_error("PME spread tile exceeds the LDS of one CU");
         return TTC_ERR_ARG;
     }
+    int dev = 0;
+    TTC_HIP(hipGetDevice(&dev));
+    int &amax = attr_max[dev & 63];
+    if (shm <= amax) return TTC_OK;
+    amax = shm;
     const void *fns[] = {(const void *)k_drift_spread<4, true>, (const void *)k_drift_spread<4, false>,
                          (const void *)k_drift_spread<6, true>, (const void *)k_drift_spread<6, false>,
                          (const void *)k_drift_spread<4, true, true>, (const void *)k_drift_spread<6, true, true>};

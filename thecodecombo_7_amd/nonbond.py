@@ -18,6 +18,7 @@ behaviour (errors carry the reference's exit codes, see include/ttc.h):
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -271,6 +272,19 @@ class Box:
         w = C.c_int(0); t = np.zeros(3)
         T.check(self.L.ttc_tune(self.h, C.byref(w), T.dptr(t)))
         return w.value, t
+
+    def tune_spread(self):
+        """Pick the spread kernel's sites per pass by timing (ttc_tune_spread)."""
+        ch = C.c_int(0); t = np.zeros(3)
+        T.check(self.L.ttc_tune_spread(self.h, C.byref(ch), T.dptr(t)))
+        return ch.value, t
+
+    def tune_cached(self, path):
+        """Both tuners behind a cache file (ttc_tune_cached): returns
+        (pair waves, spread chunk, taken from the cache)."""
+        w = C.c_int(0); ch = C.c_int(0); hit = C.c_int(0)
+        T.check(self.L.ttc_tune_cached(self.h, os.fsencode(path), C.byref(w), C.byref(ch), C.byref(hit)))
+        return w.value, ch.value, bool(hit.value)
 
     def list_stats(self):
         e = C.c_int64(0); m = C.c_int(0); cap = C.c_int(0)
