@@ -103,6 +103,14 @@ struct FftPlan {
     float2 *tw = nullptr;   // e^{+2 pi i m / n}, m = 0..n-1
 };
 
+// x 2^32 / L per axis, wrapped modulo 2^32 (modulo the box), and the
+// charge's FP32 bits: the half-list pair kernel's coordinates; the 32-bit
+// difference of two of them is the minimum-image displacement
+__device__ __forceinline__ int4 ttc_posq(const double4 &p, double kx, double ky, double kz) {
+    return make_int4((int)(unsigned)__double2ll_rn(p.x * kx), (int)(unsigned)__double2ll_rn(p.y * ky),
+                     (int)(unsigned)__double2ll_rn(p.z * kz), __float_as_int((float)p.w));
+}
+
 struct ttc_ctx {
     ttc_params p{};
     int n = 0, G = 0, cap = 0, ntypes = 0;
@@ -235,7 +243,8 @@ struct ttc_ctx {
     // accumulators per window slot and flushes them to fixj.
     bool half_mode = false;         // conditional rebuilds build half lists
     bool list_half = false;         // the current list is a half list
-    int4 *posq = nullptr;           // [N] pos4 as 32-bit fixed point + charge bits (half-list kernel)
+    int4 *posq = nullptr;           // [N] pos4 as 32-bit fixed point + charge bits (half-list kernel),
+                                    // written by every MD drift and by the MD-loop rebuild's copy
     int *win = nullptr;             // [G][wcap] window of every group
     int *wlen = nullptr;            // [G] window length
     int wcap = 0;                   // window capacity (multiple of 64)

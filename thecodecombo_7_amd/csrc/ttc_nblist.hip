@@ -495,7 +495,8 @@ __device__ __forceinline__ void build_grp_body(
     const double4 *__restrict__ fsb, const double4 *__restrict__ flb, double4 *fs, double4 *fl,
     const int *__restrict__ cell_cnt, const int *__restrict__ cell_start, const int *__restrict__ ex_off,
     const int *__restrict__ ex_part, const int *__restrict__ ex_flag, int ex_span, uint32_t *nbl, int *glen,
-    int *gcnt, int *status, int conditional, int copy_state, const HalfWin &hw) {
+    int *gcnt, int *status, int conditional, int copy_state, const HalfWin &hw, int4 *posq = nullptr,
+    double qkx = 0.0, double qky = 0.0, double qkz = 0.0) {
     if (rebuild_off(status, conditional)) return;
     // partner bitmap (span bits) and its per-word prefix counts
     extern __shared__ unsigned long long hbm[];
@@ -534,6 +535,7 @@ __device__ __forceinline__ void build_grp_body(
     for (int s = sb + (int)threadIdx.x; copy_state && s < min(sb + 64, n); s += 64 * GB_WAVES) {
         double4 pi = pos4b[s];
         pos4[s] = pi;
+        if (posq) posq[s] = ttc_posq(pi, qkx, qky, qkz);
         vel4[s] = vel4b[s];
         f4[s] = f4b[s];
         fs[s] = fsb[s];
@@ -828,12 +830,15 @@ struct BuildGrpPack {
     int conditional;
     int copy_state;
     HalfWin hw;
+    int4 *posq;              // fixed-point coordinates of the new order (MD-loop builds)
+    double qkx, qky, qkz;
 };
 
 __device__ __forceinline__ void build_grp_pack(int grp, const BuildGrpPack &a) {
     build_grp_body(grp, a.n, a.cap, a.g, a.posfb, a.pos4b, a.vel4b, a.f4b, a.typeb, a.origb, a.pos4, a.vel4, a.f4,
                    a.type, a.orig, a.anchor, a.fsb, a.flb, a.fs, a.fl, a.cell_cnt, a.cell_start, a.ex_off, a.ex_part,
-                   a.ex_flag, a.ex_span, a.nbl, a.glen, a.gcnt, a.status, a.conditional, a.copy_state, a.hw);
+                   a.ex_flag, a.ex_span, a.nbl, a.glen, a.gcnt, a.status, a.conditional, a.copy_state, a.hw,
+                   a.posq, a.qkx, a.qky, a.qkz);
 }
 __global__ void __launch_bounds__(64 * GB_WAVES) k_build_grp(BuildGrpPack a) { build_grp_pack(blockIdx.x, a); }
 __global__ void __launch_bounds__(64 * GB_WAVES) kb_build_grp(const BuildGrpPack *pk, const TtcBmap *map) {
@@ -956,7 +961,8 @@ static BuildGrpPack build_grp_pack_of(ttc_ctx *c, bool half = false, int cond = 
                         c->ex_off, c->ex_part, c->ex_flag, c->ex_span, c->nbl, c->glen, c->gcnt, c->status, cond,
                         copy, HalfWin{half ? 1 : 0, c->span, c->span_pre, c->wcap, c->win, c->wlen,
                                       c->win_limit > 0 ? std::min(c->win_limit, c->wcap) : c->wcap,
-                                      c->row_limit > 0 ? std::min(c->row_limit, c->cap) : c->cap}};
+                                      c->row_limit > 0 ? std::min(c->row_limit, c->cap) : c->cap},
+                        c->posq, 4294967296.0 / c->p.lx, 4294967296.0 / c->p.ly, 4294967296.0 / c->p.lz};
 }
 
 static size_t build_grp_shm(const ttc_ctx *c, bool half) {

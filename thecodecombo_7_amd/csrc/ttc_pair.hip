@@ -658,16 +658,20 @@ __global__ void __launch_bounds__(256) k_posq(int n, const double4 *__restrict__
                                               double kx, double ky, double kz) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    double4 p = pos4[s];
-    posq[s] = make_int4((int)(unsigned)__double2ll_rn(p.x * kx), (int)(unsigned)__double2ll_rn(p.y * ky),
-                        (int)(unsigned)__double2ll_rn(p.z * kz), __float_as_int((float)p.w));
+    posq[s] = ttc_posq(pos4[s], kx, ky, kz);
 }
 }  // namespace
 
-static void launch_pair_half(ttc_ctx *c, bool energy, bool step) {
+void ttc_launch_posq(ttc_ctx *c) {
     const double two32 = 4294967296.0;
     k_posq<<<(c->n + 255) / 256, 256, 0, c->stream>>>(c->n, c->pos4, c->posq, two32 / c->p.lx, two32 / c->p.ly,
                                                        two32 / c->p.lz);
+}
+
+// an MD step's drift (and its loop's rebuild) keep posq current; other
+// callers refresh it from pos4 first (refresh)
+static void launch_pair_half(ttc_ctx *c, bool energy, bool step, bool refresh = true) {
+    if (refresh && !step) ttc_launch_posq(c);
     PairHalfPack a{c->n, c->cap, c->wcap, c->nbl, c->glen, c->win, c->wlen, c->pos4, c->posq, c->type, c->ljtab,
                    pair_const(c), c->fpair, c->fixj, c->partials, c->nblk_max, c->status, step ? 1 : 0,
                    6755399441055744.0};
@@ -688,6 +692,13 @@ int ttc_pair_half_attrs(ttc_ctx *c) {
                          (const void *)k_pair_half<true, 8>, (const void *)k_pair_half<false, 8>};
     for (const void *f : fns) TTC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
     return TTC_OK;
+}
+
+// the pair kernel alone on the current posq (timing loops: ttc_launch_posq
+// once before them)
+void ttc_launch_pair_only(ttc_ctx *c) {
+    if (c->list_half) launch_pair_half(c, false, false, false);
+    else launch_pair_w<false, false>(c, RangeSel{});
 }
 
 void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {

@@ -211,6 +211,8 @@ struct KickIn {
     const int *orig = nullptr;
     double4 *snap = nullptr;
     long long *fixj = nullptr;   // partner forces of a half-list pair kernel
+    int4 *posq = nullptr;        // fixed-point coordinates of the new positions (ttc_posq), when set
+    double qkx = 0.0, qky = 0.0, qkz = 0.0;
 };
 
 // kick + drift + pbc (integrate_vverlet.c:52-83) of every site of a brick,
@@ -303,6 +305,7 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
                 vel4[s] = v;
                 pos4[s] = p;
                 if (ki.snap) ki.snap[so] = p;
+                if (ki.posq) ki.posq[s] = ttc_posq(p, ki.qkx, ki.qky, ki.qkz);
                 double4 a = anchor[s];
                 double rcut = pcheck[3];
                 double dx = fabs(p.x - a.x);
@@ -1128,6 +1131,10 @@ void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick) {
     a.ki.g4 = c->g4o;
     a.ki.orig = c->orig;
     a.ki.snap = c->snap;
+    a.ki.posq = c->posq;
+    a.ki.qkx = 4294967296.0 / c->p.lx;
+    a.ki.qky = 4294967296.0 / c->p.ly;
+    a.ki.qkz = 4294967296.0 / c->p.lz;
 #define DS(P, K) k_drift_spread<P, true, K><<<blocks, 256, shm, c->stream>>>(a)
     if (c->order == 4) { if (kick) DS(4, true); else DS(4, false); }
     else { if (kick) DS(6, true); else DS(6, false); }
