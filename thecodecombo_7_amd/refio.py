@@ -195,10 +195,13 @@ def read_trr(path: str):
         raw = fh.read()
     out, p = [], 0
     while p < len(raw):
-        hdr = np.frombuffer(raw, ">i4", 16, p)
-        if hdr[0] != 1993:
+        head = np.frombuffer(raw, ">i4", 3, p)
+        if head[0] != 1993:
             raise ValueError(f"{path}: bad magic at byte {p}")
-        p += 64 + int(hdr[2])
+        # magic, version, title length, the title's bytes, then 13 sizes
+        p += 12 + int(head[2])
+        hdr = np.concatenate([head, np.frombuffer(raw, ">i4", 13, p)])
+        p += 52
         t, lam = np.frombuffer(raw, ">f8", 2, p)
         p += 16
         bx = np.frombuffer(raw, ">f8", 9, p).reshape(3, 3)
