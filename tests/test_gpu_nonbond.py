@@ -233,20 +233,42 @@ def test_md_step_graph_runs_bit_identical(sys3k):
     assert np.array_equal(v1, v2)
 
 
-@pytest.mark.parametrize("nchains,box,grid", [(300, 31.0, 32), (2000, 58.3, 60)])
-def test_md_loop_list_complete(nchains, box, grid):
+def _renumbered(s, seed):
+    """The same system with its sites numbered in a random order: bonded
+    partners end up far apart in the numbering."""
+    import dataclasses
+    perm = np.random.default_rng(seed).permutation(s.n)     # new index k holds old site perm[k]
+    inv = np.empty(s.n, dtype=np.int64)
+    inv[perm] = np.arange(s.n)
+    xint = s.xint.copy()
+    xint[:, :2] = inv[s.xint[:, :2]]
+    return dataclasses.replace(
+        s, pos=np.ascontiguousarray(s.pos[perm]), vel=np.ascontiguousarray(s.vel[perm]), mass=s.mass[perm],
+        q=s.q[perm], type=np.ascontiguousarray(s.type[perm]), bonds=inv[s.bonds].astype(np.int32),
+        xint=xint.astype(np.int32))
+
+
+@pytest.mark.parametrize("nchains,box,grid,renumber", [(300, 31.0, 32, False), (2000, 58.3, 60, False),
+                                                       (2000, 58.3, 60, True)])
+def test_md_loop_list_complete(nchains, box, grid, renumber):
     """The list an MD loop rebuilds (pruned to rc + half the skin, FP32 test
     with a margin, group-cooperative build) holds every pair within rc once
     and with its exclusion class: forces and energies evaluated from it
     after the loop match the oracle's from a fresh reference list at the
     same positions.  The 31 A box takes the minimum-image path of the build
     (candidate ranges wrap onto themselves), the 58 A box the periodic-shift
-    path."""
+    path.  Chains number their bonded partners within 3 of each other (the
+    build's per-site class masks); the renumbered case puts them far apart
+    (the exclusion lists read per pair)."""
     import pyoracle
     from thecodecombo_7_amd import system
     from thecodecombo_7_amd.nonbond import Box
     s = system.make_chain_system(nchains, box, grid, seed=17)
-    print(f"case: MD-loop list, {s.n} sites, box {box}", flush=True)
+    if renumber:
+        s = _renumbered(s, 5)
+        span = int(np.abs(s.xint[:, 0].astype(np.int64) - s.xint[:, 1]).max())
+        assert span > 15, span
+    print(f"case: MD-loop list, {s.n} sites, box {box}, renumbered {renumber}", flush=True)
     b = Box(s)
     b.nblist()
     _, nreb = b.integrate_vverlet(30)

@@ -386,6 +386,7 @@ k_build(int n, int cap, CellGeom g, double rcut, double rcut2, const double4 *__
 #define GB_MI (1 << 13)
 #define GB_QCAP 128   // candidate ring per wave (two batches of 64)
 #define GB_EXMAX 8    // exclusion partners per site held in LDS (more: read from the CSR)
+#define GB_XM_SPAN 15 // original-index offsets +-15 fit a 2-bit class mask of 62 bits
 #define GB_ZMARGIN 1e-3f   // A: far above FP32 rounding of frame coordinates
 
 struct GbGeom {
@@ -393,7 +394,13 @@ struct GbGeom {
     float lx, ly, lz, hx, hy, hz, cwx, cwy, cwz, ilx, ily, ilz;
     float cut2;   // FP32 test radius squared
     float cutr;   // radius of the cell-range enumeration (test radius + rounding margin)
+    float qzk;    // 2^32 / lz: the z key of the half-list test
 };
+
+// z key of a site for the half-list test: z in units of lz / 2^32, kept mod
+// 2^32, so the wrap-around difference of two keys is the minimum-image dz
+// (to a few 1e-6 A) and exactly minus the difference taken the other way
+__device__ __forceinline__ int zkey(float z, float qzk) { return (int)(unsigned long long)__float2ll_rn(z * qzk); }
 
 __device__ __forceinline__ float wave_minf(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
@@ -467,11 +474,12 @@ __device__ void row_ranges(int r, const AxisRange &ay, const AxisRange &az, floa
 }
 
 // Half lists (HalfWin.half != 0): a candidate j is kept in row i only if it
-// lies above i: dz = z_j - z_i (FP32, the wrapped coordinates every row
-// reads alike) taken to the minimum image is > 0, or dz == 0 and j > i.  The
-// computed dz of (j, i) is exactly minus that of (i, j), so every pair is
-// listed once, and each row holds the upper half of its sphere (rows of
-// even length, unlike a split by sorted index, which follows cell planes).
+// lies above i: the wrap-around difference of the z keys (zkey: z in units
+// of lz / 2^32, mod 2^32; its minimum-image dz to a few 1e-6 A) is > 0, or
+// a tie (0 or -2^31) and j > i.  The key difference of (j, i) is exactly minus that of
+// (i, j), so every pair is listed once, and each row holds the upper half of
+// its sphere (rows of even length, unlike a split by sorted index, which
+// follows cell planes).
 // Every kept partner sets bit jrel = (j - sb) + pre of an LDS bitmap (sb:
 // the group's first site, j - sb taken cyclically into (-n/2, n/2]; pre
 // covers the partners of the same cell plane below the group); after the
@@ -509,27 +517,43 @@ __device__ __forceinline__ void build_grp_body(
     // position, original index | type bits; sorted index)
     __shared__ float4 sq[GB_WAVES][GB_QCAP];
     __shared__ int sqj[GB_WAVES][GB_QCAP];
-    __shared__ float sqz[GB_WAVES][GB_QCAP];   // wrapped z of the candidates (half lists)
+    __shared__ int sqz[GB_WAVES][GB_QCAP];   // z keys of the candidates (half lists)
     // exclusion partners of the group's sites (original index) and their
     // class; sxn = count, or -1 where the site has more than GB_EXMAX
     __shared__ int sxo[64][GB_EXMAX];
     __shared__ int sxc[64][GB_EXMAX];
     __shared__ int sxn[64];
+    // where every exclusion partner lies within GB_XM_SPAN of its site in the
+    // original numbering: the class of partner oj of site oi is 2 bits at
+    // 2 * (oj - oi + GB_XM_SPAN) of the site's mask (0 where not listed)
+    __shared__ unsigned long long sxm[64];
+    const bool xmask = ex_span <= GB_XM_SPAN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sb = grp * 64;
     if (threadIdx.x < 64) {
         int s = sb + threadIdx.x, cnt = 0;
+        unsigned long long cm = 0ull;
         if (s < n) {
             int oi = origb[s];
             int x0 = ex_off[oi], x1 = ex_off[oi + 1];
-            cnt = x1 - x0 > GB_EXMAX ? -1 : x1 - x0;
-            for (int q = 0; q < cnt; q++) {
-                int f = ex_flag[x0 + q];
-                sxo[threadIdx.x][q] = ex_part[x0 + q];
-                sxc[threadIdx.x][q] = f == 4 ? 2 : (f ? 1 : 0);
+            if (xmask) {
+                // later entries of the same partner replace earlier ones
+                for (int q = x0; q < x1; q++) {
+                    const int f = ex_flag[q];
+                    const int sh = 2 * (ex_part[q] - oi + GB_XM_SPAN);
+                    cm = (cm & ~(3ull << sh)) | ((unsigned long long)(f == 4 ? 2 : (f ? 1 : 0)) << sh);
+                }
+            } else {
+                cnt = x1 - x0 > GB_EXMAX ? -1 : x1 - x0;
+                for (int q = 0; q < cnt; q++) {
+                    int f = ex_flag[x0 + q];
+                    sxo[threadIdx.x][q] = ex_part[x0 + q];
+                    sxc[threadIdx.x][q] = f == 4 ? 2 : (f ? 1 : 0);
+                }
             }
         }
         sxn[threadIdx.x] = cnt;
+        sxm[threadIdx.x] = cm;
     }
     // the new sorted order becomes the current state
     for (int s = sb + (int)threadIdx.x; copy_state && s < min(sb + 64, n); s += 64 * GB_WAVES) {
@@ -549,6 +573,8 @@ __device__ __forceinline__ void build_grp_body(
     const int row0 = __builtin_amdgcn_readfirstlane(wave) * GB_ROWS;   // this wave's first row in the group
     const int wb = sb + row0;          // and its sorted index
     const int nrw = max(0, min(GB_ROWS, n - wb));
+    // lane r < GB_ROWS: the exclusion class mask of row r
+    const unsigned long long rcm = sxm[row0 + min(lane, GB_ROWS - 1)];
     int rkl = 0;   // entry count of row lane (lanes below GB_ROWS)
     uint32_t *grow = nbl + (size_t)grp * cap * 64;
     if (nrw > 0) {
@@ -560,7 +586,7 @@ __device__ __forceinline__ void build_grp_body(
         const float ux = frame1(pw.x, pref.x, g.lx, g.ilx), uy = frame1(pw.y, pref.y, g.ly, g.ily),
                     uz = frame1(pw.z, pref.z, g.lz, g.ilz);
         const int pwo = __float_as_int(pw.w) & (int)TTC_JMASK;
-        const float rawz = pw.z;   // wrapped z of row lane
+        const int rowzk = zkey(pw.z, g.qzk);   // z key of row lane
         const float xmn = wave_minf(ux), xmx = wave_maxf(ux);
         const float ymn = wave_minf(uy), ymx = wave_maxf(uy);
         const float zmn = wave_minf(uz), zmx = wave_maxf(uz);
@@ -578,25 +604,26 @@ __device__ __forceinline__ void build_grp_body(
         const float cutr2 = g.cutr * g.cutr;   // box filter radius: the test radius plus a margin
         float4 *qp = sq[row0 / GB_ROWS];
         int *qj = sqj[row0 / GB_ROWS];
-        float *qz = sqz[row0 / GB_ROWS];
+        int *qz = sqz[row0 / GB_ROWS];
         int qh = 0, qn = 0;   // ring head and length (wave-uniform)
 
-        // test every row against the cnt (<= 64) candidates at the head of the ring
-        auto consume = [&](int cnt, auto MI) {
+        // test every row against the cnt (<= 64) candidates at the head of the ring.
+        // The rows are unrolled (lane r holds row r's coordinates and entry
+        // count, read wave-uniform), and a candidate's bitmap bit is set
+        // once for all rows that keep it.
+        auto consume = [&](int cnt, auto MI, auto HALF) {
             const int slot = (qh + lane) & (GB_QCAP - 1);
             const float4 pj = qp[slot];
             const int j = qj[slot];
-            const float zj = hw.half ? qz[slot] : 0.0f;
-            // partner's bitmap bit (cyclic offset from the group's first site)
-            int djs = j - sb;
-            if (2 * djs > n) djs -= n;
-            else if (2 * djs <= -n) djs += n;
-            const int jrel = djs + hw.pre;
+            const int zj = decltype(HALF)::value ? qz[slot] : 0;
             const bool ok = lane < cnt;
             const int wj = __float_as_int(pj.w);
             const int oj = wj & (int)TTC_JMASK;                                    // original index
             const uint32_t ej = (uint32_t)j | ((uint32_t)wj & ~TTC_JMASK);         // sorted index | type bits
-            for (int rr = 0; rr < nrw; rr++) {
+            bool kept = false;   // some row keeps this candidate (half lists)
+#pragma unroll
+            for (int rr = 0; rr < GB_ROWS; rr++) {
+                if (rr >= nrw) break;
                 const float rwx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ux), rr));
                 const float rwy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(uy), rr));
                 const float rwz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(uz), rr));
@@ -606,24 +633,15 @@ __device__ __forceinline__ void build_grp_body(
                     dy -= g.ly * rintf(dy * g.ily);
                     dz -= g.lz * rintf(dz * g.ilz);
                 }
-                bool hit = ok & (dx * dx + dy * dy + dz * dz < g.cut2) & (j != wb + rr);
-                if (hw.half) {
-                    // the upper half: minimum-image dz > 0 (ties by index)
-                    const float zi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rawz), rr));
-                    const float dzr = zj - zi;
-                    const float dzm = fmaf(-g.lz, rintf(dzr * g.ilz), dzr);
-                    hit = hit & ((dzm > 0.0f) | ((dzm == 0.0f) & (j > wb + rr)));
-                    if (hit) {
-                        if (jrel < 0) {
-                            atomicMax(&status[ST_PREMAX], -djs);
-                            atomicOr(&status[ST_OVERFLOW], 8);
-                        } else if (jrel >= hw.span) {
-                            atomicMax(&status[ST_SPANMAX], jrel + 1);
-                            atomicOr(&status[ST_OVERFLOW], 8);
-                        } else {
-                            atomicOr(&hbm[jrel >> 6], 1ull << (jrel & 63));
-                        }
-                    }
+                bool hit = ok & (dx * dx + dy * dy + dz * dz < g.cut2);
+                if (decltype(HALF)::value) {
+                    // the upper half: key difference > 0; ties (0, and -2^31,
+                    // its own negative) by index (a site is not above itself)
+                    const int dq = zj - __builtin_amdgcn_readlane(rowzk, rr);
+                    hit = hit & ((dq > 0) | (((unsigned)dq << 1 == 0u) & (j > wb + rr)));
+                    kept |= hit;
+                } else {
+                    hit = hit & (j != wb + rr);
                 }
                 const unsigned long long b = __builtin_amdgcn_ballot_w64(hit);
                 if (b != 0ull) {
@@ -637,7 +655,13 @@ __device__ __forceinline__ void build_grp_body(
                     const int oi = __builtin_amdgcn_readlane(pwo, rr);
                     const bool nearo = hit & ((unsigned)(oj - oi + ex_span) <= (unsigned)(2 * ex_span));
                     if (__builtin_amdgcn_ballot_w64(nearo) != 0ull) {
-                        if (nearo) {
+                        if (xmask) {
+                            const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)rcm, rr);
+                            const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(rcm >> 32), rr);
+                            const unsigned long long cm = ((unsigned long long)mhi << 32) | mlo;
+                            const unsigned d = (unsigned)(oj - oi + GB_XM_SPAN);
+                            cls = nearo ? (uint32_t)(cm >> (2u * d)) & 3u : 0u;
+                        } else if (nearo) {
                             const int rw = row0 + rr;
                             const int xc = sxn[rw];
                             for (int q2 = 0; q2 < xc; q2++)
@@ -656,13 +680,34 @@ __device__ __forceinline__ void build_grp_body(
                         grow[(uint32_t)(((pos >> 4) << 10) + (row0 + rr) * 16 + (pos & 15))] = ej | (cls << 30);
                 }
             }
+            if (decltype(HALF)::value && kept) {
+                // partner's bitmap bit (cyclic offset from the group's first site)
+                int djs = j - sb;
+                if (2 * djs > n) djs -= n;
+                else if (2 * djs <= -n) djs += n;
+                const int jrel = djs + hw.pre;
+                if (jrel < 0) {
+                    atomicMax(&status[ST_PREMAX], -djs);
+                    atomicOr(&status[ST_OVERFLOW], 8);
+                } else if (jrel >= hw.span) {
+                    atomicMax(&status[ST_SPANMAX], jrel + 1);
+                    atomicOr(&status[ST_OVERFLOW], 8);
+                } else {
+                    atomicOr(&hbm[jrel >> 6], 1ull << (jrel & 63));
+                }
+            }
             qh = (qh + cnt) & (GB_QCAP - 1);
             qn -= cnt;
         };
         auto drain = [&](int cnt) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // ring writes before the reads
-            if (wmi) consume(cnt, std::true_type{});
-            else consume(cnt, std::false_type{});
+            if (hw.half) {
+                if (wmi) consume(cnt, std::true_type{}, std::true_type{});
+                else consume(cnt, std::false_type{}, std::true_type{});
+            } else {
+                if (wmi) consume(cnt, std::true_type{}, std::false_type{});
+                else consume(cnt, std::false_type{}, std::false_type{});
+            }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // reads before the next writes
         };
 
@@ -705,7 +750,7 @@ __device__ __forceinline__ void build_grp_body(
                                            (GB_QCAP - 1);
                             qp[at] = make_float4(px, py, pz, pj.w);
                             qj[at] = j;
-                            qz[at] = pj.z;
+                            qz[at] = zkey(pj.z, g.qzk);
                         }
                         qn += (int)__popcll(m);
                         if (qn >= 64) drain(64);
@@ -745,38 +790,49 @@ __device__ __forceinline__ void build_grp_body(
     // window: prefix counts of the bitmap words (each thread a run of
     // consecutive words, then a scan over the threads), slots in ascending
     // jrel, win[g][slot] = j
-    __shared__ int wsc[64 * GB_WAVES];
+    __shared__ int wtot[GB_WAVES];
     const int per = (nword + blockDim.x - 1) / blockDim.x;
     const int w0 = threadIdx.x * per, w1 = min(nword, w0 + per);
     int cnt = 0;
     for (int w = w0; w < w1; w++) cnt += __popcll(hbm[w]);
-    wsc[threadIdx.x] = cnt;
+    // exclusive prefix over the threads: inclusive scan in each wave, then
+    // the totals of the waves before
+    int inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) wtot[wave] = inc;
     __syncthreads();
+    int run = inc - cnt, total = 0;
+#pragma unroll
+    for (int q = 0; q < GB_WAVES; q++) {
+        if (q < wave) run += wtot[q];
+        total += wtot[q];
+    }
     if (threadIdx.x == 0) {
-        int run = 0;
-        for (int q = 0; q < (int)blockDim.x; q++) { int v = wsc[q]; wsc[q] = run; run += v; }
-        hw.wlen[grp] = min(run, hw.wcap);
-        atomicMax(&status[ST_WMAX], run);
-        if (run > hw.wlim) atomicOr(&status[ST_OVERFLOW], 8);
+        hw.wlen[grp] = min(total, hw.wcap);
+        atomicMax(&status[ST_WMAX], total);
+        if (total > hw.wlim) atomicOr(&status[ST_OVERFLOW], 8);
+    }
+    for (int w = w0; w < w1; w++) {
+        hpre[w] = (unsigned int)run;
+        run += __popcll(hbm[w]);
     }
     __syncthreads();
-    int run = wsc[threadIdx.x];
+    // win[g][slot] = j: one word per wave at a time, lane b its bit b
     int *gwin = hw.win + (size_t)grp * hw.wcap;
-    for (int w = w0; w < w1; w++) {
-        unsigned long long bits = hbm[w];
-        hpre[w] = (unsigned int)run;
-        while (bits) {
-            const int bpos = __builtin_ctzll(bits);
-            bits &= bits - 1ull;
-            if (run < hw.wcap) {
-                int j = sb - hw.pre + w * 64 + bpos;
-                j = j < 0 ? j + n : (j >= n ? j - n : j);
-                gwin[run] = j;
-            }
-            run++;
+    for (int w = wave; w < nword; w += GB_WAVES) {
+        const unsigned long long bits = hbm[w];
+        if (!((bits >> lane) & 1ull)) continue;
+        const int slot = (int)hpre[w] + __popcll(bits & ((1ull << lane) - 1ull));
+        if (slot < hw.wcap) {
+            int j = sb - hw.pre + w * 64 + lane;
+            j = j < 0 ? j + n : (j >= n ? j - n : j);
+            gwin[slot] = j;
         }
     }
-    __syncthreads();
     // entries: j -> slot (a slot past the capacity is clamped; the overflow
     // flag makes the host grow the window and build again).  The wave's
     // rows are read 64 entries at a time each, all GB_ROWS loads in flight
@@ -840,8 +896,10 @@ __device__ __forceinline__ void build_grp_pack(int grp, const BuildGrpPack &a) {
                    a.ex_flag, a.ex_span, a.nbl, a.glen, a.gcnt, a.status, a.conditional, a.copy_state, a.hw,
                    a.posq, a.qkx, a.qky, a.qkz);
 }
-__global__ void __launch_bounds__(64 * GB_WAVES) k_build_grp(BuildGrpPack a) { build_grp_pack(blockIdx.x, a); }
-__global__ void __launch_bounds__(64 * GB_WAVES) kb_build_grp(const BuildGrpPack *pk, const TtcBmap *map) {
+// at most 72 VGPRs: 7 workgroups per CU, so that the groups of about 1800
+// x 64 sites (115k) run as one round on 256 CUs
+__global__ void __launch_bounds__(64 * GB_WAVES, 7) k_build_grp(BuildGrpPack a) { build_grp_pack(blockIdx.x, a); }
+__global__ void __launch_bounds__(64 * GB_WAVES, 7) kb_build_grp(const BuildGrpPack *pk, const TtcBmap *map) {
     const TtcBmap m = map[blockIdx.x];
     const BuildGrpPack a = pk[m.rep];
     build_grp_pack(m.blk, a);
@@ -956,6 +1014,7 @@ static BuildGrpPack build_grp_pack_of(ttc_ctx *c, bool half = false, int cond = 
     double cf = ttc_compute_cutoff(c) + FAST_MARGIN;
     g.cut2 = (float)(cf * cf);
     g.cutr = (float)(cf + FAST_MARGIN);
+    g.qzk = (float)(4294967296.0 / c->p.lz);
     return BuildGrpPack{c->n, c->cap, g, c->posfb, c->pos4b, c->vel4b, c->f4b, c->typeb, c->origb, c->pos4, c->vel4,
                         c->f4, c->type, c->orig, c->anchor, c->fsb, c->flb, c->fs, c->fl, c->cell_cnt, c->cell_start,
                         c->ex_off, c->ex_part, c->ex_flag, c->ex_span, c->nbl, c->glen, c->gcnt, c->status, cond,

@@ -23,6 +23,10 @@
 
 #define FFT_LB 16
 #define FFT_THREADS 256
+// threads of one spread workgroup (one brick)
+#ifndef SPREAD_THREADS
+#define SPREAD_THREADS 512
+#endif
 
 namespace {
 
@@ -393,12 +397,12 @@ struct DsPack {
 };
 
 template <int P, bool DRIFT, bool KICK = false>
-__global__ void __launch_bounds__(256) k_drift_spread(DsPack a) {
+__global__ void __launch_bounds__(SPREAD_THREADS) k_drift_spread(DsPack a) {
     drift_spread_body<P, DRIFT, KICK>(blockIdx.x, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start,
                                       a.cell_cnt, a.qfix, a.status, a.anchor, a.pcheck, a.ki);
 }
 template <int P, bool DRIFT>
-__global__ void __launch_bounds__(256) kb_drift_spread(const DsPack *pk, const TtcBmap *map) {
+__global__ void __launch_bounds__(SPREAD_THREADS) kb_drift_spread(const DsPack *pk, const TtcBmap *map) {
     const TtcBmap m = map[blockIdx.x];
     const DsPack a = pk[m.rep];
     drift_spread_body<P, DRIFT>(m.blk, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start, a.cell_cnt,
@@ -1024,8 +1028,10 @@ static Spread spread_args(const ttc_ctx *c) {
 static size_t brick_shm(const ttc_ctx *c, const Brick &br);
 
 // workgroups of the spread that fit one CU's LDS (static part: mean force
-// and a margin); at most 8 of 256 threads
-static int spread_occupancy(size_t bytes) { return (int)std::min<size_t>(8, (160 * 1024) / (bytes + 64)); }
+// and a margin); at most 32 waves
+static int spread_occupancy(size_t bytes) {
+    return (int)std::min<size_t>(2048 / SPREAD_THREADS, (160 * 1024) / (bytes + 64));
+}
 
 // Bricks of 2x2x2 cells; on grids so fine that such a brick's tile (with
 // the weights of 64 sites) does not fit one CU's LDS, the first of 2x2x1,
@@ -1113,7 +1119,7 @@ static void launch_drift_spread(ttc_ctx *c, bool drift, double vscal) {
     int blocks = ds_blocks(c);
     DsPack a = ds_pack(c, drift, vscal);
     size_t shm = spread_shm(c, a.br);
-#define DS(P, D) k_drift_spread<P, D><<<blocks, 256, shm, c->stream>>>(a)
+#define DS(P, D) k_drift_spread<P, D><<<blocks, SPREAD_THREADS, shm, c->stream>>>(a)
     if (c->order == 4) { if (drift) DS(4, true); else DS(4, false); }
     else { if (drift) DS(6, true); else DS(6, false); }
 #undef DS
@@ -1135,7 +1141,7 @@ void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick) {
     a.ki.qkx = 4294967296.0 / c->p.lx;
     a.ki.qky = 4294967296.0 / c->p.ly;
     a.ki.qkz = 4294967296.0 / c->p.lz;
-#define DS(P, K) k_drift_spread<P, true, K><<<blocks, 256, shm, c->stream>>>(a)
+#define DS(P, K) k_drift_spread<P, true, K><<<blocks, SPREAD_THREADS, shm, c->stream>>>(a)
     if (c->order == 4) { if (kick) DS(4, true); else DS(4, false); }
     else { if (kick) DS(6, true); else DS(6, false); }
 #undef DS
@@ -1319,7 +1325,7 @@ int ttc_bstage_drift(const std::vector<ttc_ctx *> &r, TtcStage *st) {
     return ttc_stage_make(st, pk, nb, shm);
 }
 void ttc_blaunch_drift(const TtcStage &st, hipStream_t s) {
-    kb_drift_spread<4, true><<<st.nblk, 256, st.shm, s>>>((const DsPack *)st.pk, st.map);
+    kb_drift_spread<4, true><<<st.nblk, SPREAD_THREADS, st.shm, s>>>((const DsPack *)st.pk, st.map);
 }
 
 // st[FP_N]: the five FFT passes
