@@ -418,6 +418,8 @@ struct FftArgs {
     int nr[16];             // n / rad[st]
     unsigned mag[16];       // ceil(2^32 / ns[st]) mod 2^32: j / ns[st] by multiply-high
     unsigned mag_n;         // ceil(2^32 / n)
+    int H;                  // kz values kept: K2 / 2 + 1 (the grid is real)
+    unsigned mag_h;         // ceil(2^32 / H)
     const float2 *tw;
 };
 
@@ -506,87 +508,113 @@ __device__ float2 *fft_lds(float2 *A, float2 *B, int ls, int es, const FftArgs &
     return src;
 }
 
-// MODE 0: z lines, forward, input = fixed-point grid (cleared after reading)
-// MODE 1: y lines, forward (sign +1) / inverse (sign -1), in place
-// MODE 2: x lines, forward + influence function + inverse, in place
-// MODE 3: z lines, inverse, output = real part
+// The charge grid is real, so its transform is Hermitian, S(-k) = conj S(k),
+// and only kz = 0 .. K2/2 (H values) is kept: grid qc[K0][K1][H].
+// MODE 0: z lines, forward: two real lines of the fixed-point grid (cleared
+//         after reading) as the real and imaginary parts of one complex
+//         line, whose transform Z gives both half spectra:
+//         A(k) = (Z(k) + conj Z(n-k)) / 2, B(k) = -i (Z(k) - conj Z(n-k)) / 2
+// MODE 1: y lines of the half grid, forward (sign +1) / inverse (sign -1)
+// MODE 2: x lines of the half grid, forward + influence function + inverse
+// MODE 3: z lines, inverse: two half spectra completed by Hermitian symmetry
+//         as Z = A + i B, whose inverse holds the two real lines in its real
+//         and imaginary parts
+// The y and x passes take FFT_LB lines per workgroup from the flat list of
+// (fixed coordinate, kz) lines.
 template <int MODE, bool ENERGY>
 __device__ __forceinline__ void fft_body(int bid, const FftArgs &f, int sign, long long *qfix, float2 *qc, float *qr,
                                          const float *eterm, double fac, double lx, double ly, double lz,
                                          double *partials, int maxb) {
     extern __shared__ float2 lds[];
-    int n = f.n;
+    const int n = f.n, H = f.H;
     // buffers of FFT_LB lines; z lines (MODE 0, 3) are padded to n + 1
     const int lsz = (MODE == 0 || MODE == 3) ? n + 1 : n;
     float2 *A = lds, *B = lds + FFT_LB * lsz;
     float2 *tw = B + FFT_LB * lsz;
     for (int m = threadIdx.x; m < n; m += blockDim.x) tw[m] = f.tw[m];
-    int nlines, ls, es;
-    size_t gbase = 0;
-    int x0 = 0, y0 = 0, z0 = 0, nl = 0;
+    int ls, es;
+    size_t r0 = 0;   // first real z line of the workgroup (MODE 0, 3)
+    int nreal = 0;   // its real z lines, at most 2 FFT_LB
+    // MODE 1, 2: this thread's line (the workgroup's line threadIdx.x % FFT_LB)
+    int fix = 0, kz = 0;
+    bool lok = false;
     if (MODE == 0 || MODE == 3) {
-        nlines = f.K0 * f.K1;
-        int l0 = bid * FFT_LB;
-        nl = min(FFT_LB, nlines - l0);
+        r0 = (size_t)bid * 2 * FFT_LB;
+        nreal = min(2 * FFT_LB, f.K0 * f.K1 - (int)r0);
         ls = n + 1; es = 1;
-        gbase = (size_t)l0 * n;
-        // lines past the grid's last line are zero-filled and not stored
         for (int idx = threadIdx.x; idx < FFT_LB * n; idx += blockDim.x) {
             const int l = udiv_mag(idx, f.mag_n), e = idx - l * n;
+            const int ra = 2 * l, rb = 2 * l + 1;
             float2 v = make_float2(0.0f, 0.0f);
-            if (l < nl) {
-                if (MODE == 0) {
-                    long long q = qfix[gbase + idx];
-                    qfix[gbase + idx] = 0;
-                    v = make_float2((float)((double)q * TTC_QFIX_INV), 0.0f);
-                } else {
-                    v = qc[gbase + idx];
+            if (MODE == 0) {
+                if (ra < nreal) {
+                    const size_t g = (r0 + ra) * n + e;
+                    const long long q = qfix[g];
+                    qfix[g] = 0;
+                    v.x = (float)((double)q * TTC_QFIX_INV);
                 }
+                if (rb < nreal) {
+                    const size_t g = (r0 + rb) * n + e;
+                    const long long q = qfix[g];
+                    qfix[g] = 0;
+                    v.y = (float)((double)q * TTC_QFIX_INV);
+                }
+            } else {
+                // element e >= H of a half spectrum is conj of element n - e
+                const bool lo = e < H;
+                const int k = lo ? e : n - e;
+                float2 a = ra < nreal ? qc[(r0 + ra) * H + k] : make_float2(0.0f, 0.0f);
+                float2 b = rb < nreal ? qc[(r0 + rb) * H + k] : make_float2(0.0f, 0.0f);
+                if (!lo) { a.y = -a.y; b.y = -b.y; }
+                v = make_float2(a.x - b.y, a.y + b.x);
             }
             A[l * ls + e] = v;
         }
     } else {
-        // lines along y (MODE 1: fixed x) or x (MODE 2: fixed y), LB z values
-        int nzb = (f.K2 + FFT_LB - 1) / FFT_LB;
-        int fixed = bid / nzb;
-        z0 = (bid - fixed * nzb) * FFT_LB;
-        nl = min(FFT_LB, f.K2 - z0);
+        const int nfix = MODE == 1 ? f.K0 : f.K1;
+        const int fl = bid * FFT_LB + (int)(threadIdx.x & (FFT_LB - 1));
+        lok = fl < nfix * H;
+        if (lok) {
+            fix = udiv_mag(fl, f.mag_h);
+            kz = fl - fix * H;
+        }
         ls = 1; es = FFT_LB;
-        if (MODE == 1) x0 = fixed; else y0 = fixed;
+        // lines along y (MODE 1: fixed x) or x (MODE 2: fixed y); blockDim is
+        // a multiple of FFT_LB, so idx % FFT_LB is this thread's line
         for (int idx = threadIdx.x; idx < n * FFT_LB; idx += blockDim.x) {
-            int e = idx / FFT_LB, l = idx - e * FFT_LB;
+            const int e = idx / FFT_LB;
             float2 v = make_float2(0.0f, 0.0f);
-            if (l < nl) {
-                size_t g = MODE == 1 ? ((size_t)x0 * f.K1 + e) * f.K2 + z0 + l
-                                     : ((size_t)e * f.K1 + y0) * f.K2 + z0 + l;
+            if (lok) {
+                const size_t g = MODE == 1 ? ((size_t)fix * f.K1 + e) * H + kz : ((size_t)e * f.K1 + fix) * H + kz;
                 v = qc[g];
             }
             A[idx] = v;
         }
-        nl = FFT_LB;
     }
     __syncthreads();
     float2 *res = fft_lds(A, B, ls, es, f, tw, MODE == 3 ? -1 : (MODE == 1 ? sign : +1));
 
     if (MODE == 2) {
-        // influence function (pme_calc.c:52-93); element e = k1, line l -> (k2=y0, k3=z0+l)
+        // influence function (pme_calc.c:52-93); element e = k1, line -> (k2 = fix, k3 = kz).
+        // Energy and virial: kz and -kz give equal terms (every term is even
+        // in m), so the kept 0 < kz < K2/2 count twice
         double en = 0.0, w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
-        int nzl = min(FFT_LB, f.K2 - z0);
+        const double wgt = (kz == 0 || 2 * kz == f.K2) ? 1.0 : 2.0;
         for (int idx = threadIdx.x; idx < n * FFT_LB; idx += blockDim.x) {
-            int e = idx / FFT_LB, l = idx - e * FFT_LB;
-            if (l >= nzl) continue;
-            size_t g = ((size_t)e * f.K1 + y0) * f.K2 + z0 + l;
+            if (!lok) continue;
+            const int e = idx / FFT_LB;
+            const size_t g = ((size_t)e * f.K1 + fix) * H + kz;
             float et = eterm[g];
             float2 v = res[idx];
             if (ENERGY && g != 0) {
-                int k1 = e, k2 = y0, k3 = z0 + l;
+                int k1 = e, k2 = fix, k3 = kz;
                 int mx = k1 - (k1 > f.K0 / 2 ? f.K0 : 0);
                 int my = k2 - (k2 > f.K1 / 2 ? f.K1 : 0);
-                int mz = k3 - (k3 > f.K2 / 2 ? f.K2 : 0);
+                int mz = k3;
                 double hx = mx / lx, hy = my / ly, hz = mz / lz;
                 double m2 = hx * hx + hy * hy + hz * hz;
                 double st2 = (double)v.x * v.x + (double)v.y * v.y;
-                double es2 = (double)et * st2;
+                double es2 = wgt * (double)et * st2;
                 double wt = -2.0 * (fac * m2 + 1.0) / m2;
                 en += es2;
                 w0 += es2 * (wt * hx * hx + 1.0);
@@ -619,19 +647,28 @@ __device__ __forceinline__ void fft_body(int bid, const FftArgs &f, int sign, lo
     }
 
     // store
-    if (MODE == 0 || MODE == 3) {
-        for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
+    if (MODE == 0) {
+        // both half spectra of complex line l
+        for (int idx = threadIdx.x; idx < FFT_LB * H; idx += blockDim.x) {
+            const int l = udiv_mag(idx, f.mag_h), k = idx - l * H;
+            const int ra = 2 * l, rb = 2 * l + 1;
+            const float2 zk = res[l * ls + k], zm = res[l * ls + (k ? n - k : 0)];
+            if (ra < nreal) qc[(r0 + ra) * H + k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+            if (rb < nreal) qc[(r0 + rb) * H + k] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+        }
+    } else if (MODE == 3) {
+        for (int idx = threadIdx.x; idx < FFT_LB * n; idx += blockDim.x) {
             const int l = udiv_mag(idx, f.mag_n), e = idx - l * n;
-            if (MODE == 0) qc[gbase + idx] = res[l * ls + e];
-            else qr[gbase + idx] = res[l * ls + e].x;
+            const int ra = 2 * l, rb = 2 * l + 1;
+            const float2 z = res[l * ls + e];
+            if (ra < nreal) qr[(r0 + ra) * n + e] = z.x;
+            if (rb < nreal) qr[(r0 + rb) * n + e] = z.y;
         }
     } else {
-        int nzl = min(FFT_LB, f.K2 - z0);
         for (int idx = threadIdx.x; idx < n * FFT_LB; idx += blockDim.x) {
-            int e = idx / FFT_LB, l = idx - e * FFT_LB;
-            if (l >= nzl) continue;
-            size_t g = MODE == 1 ? ((size_t)x0 * f.K1 + e) * f.K2 + z0 + l
-                                 : ((size_t)e * f.K1 + y0) * f.K2 + z0 + l;
+            if (!lok) continue;
+            const int e = idx / FFT_LB;
+            const size_t g = MODE == 1 ? ((size_t)fix * f.K1 + e) * H + kz : ((size_t)e * f.K1 + fix) * H + kz;
             qc[g] = res[idx];
         }
     }
@@ -901,6 +938,8 @@ FftArgs fft_args(const ttc_ctx *c, int axis) {
         ns *= f.rad[i];
     }
     f.mag_n = (unsigned)((0x100000000ull + pl.n - 1) / pl.n);
+    f.H = c->K[2] / 2 + 1;
+    f.mag_h = (unsigned)((0x100000000ull + f.H - 1) / f.H);
     f.tw = pl.tw;
     return f;
 }
@@ -947,11 +986,13 @@ static void influence_host(const ttc_ctx *c, std::vector<float> &et) {
     dftmod(K0, bx); dftmod(K1, by); dftmod(K2, bz);
     double lx = c->p.lx, ly = c->p.ly, lz = c->p.lz, vol = lx * ly * lz;
     double fac = TTC_PI * TTC_PI / c->p.kappa / c->p.kappa;
-    et.assign((size_t)K0 * K1 * K2, 0.0f);
+    // the half grid of the transforms: kz = 0 .. K2/2
+    const int H = K2 / 2 + 1;
+    et.assign((size_t)K0 * K1 * H, 0.0f);
     for (int k1 = 0; k1 < K0; k1++)
         for (int k2 = 0; k2 < K1; k2++)
-            for (int k3 = 0; k3 < K2; k3++) {
-                size_t ind = ((size_t)k1 * K1 + k2) * K2 + k3;
+            for (int k3 = 0; k3 < H; k3++) {
+                size_t ind = ((size_t)k1 * K1 + k2) * H + k3;
                 if (ind == 0) continue;   // m = 0 carries no energy; its value drops out of the force
                 int mx = k1 - (k1 > K0 / 2 ? K0 : 0);
                 int my = k2 - (k2 > K1 / 2 ? K1 : 0);
@@ -1000,7 +1041,7 @@ int ttc_pme_setup(ttc_ctx *c) {
     TTC_HIP(hipMalloc(&c->eterm, sizeof(float) * c->ngrid));
     std::vector<float> et;
     influence_host(c, et);
-    TTC_HIP(hipMemcpy(c->eterm, et.data(), sizeof(float) * c->ngrid, hipMemcpyHostToDevice));
+    TTC_HIP(hipMemcpy(c->eterm, et.data(), sizeof(float) * et.size(), hipMemcpyHostToDevice));
     return TTC_OK;
 }
 
@@ -1015,7 +1056,7 @@ void ttc_pme_free(ttc_ctx *c) {
     for (int a = 0; a < 3; a++) hipFree(c->plan[a].tw);
 }
 
-int ttc_pme_conv_blocks(ttc_ctx *c) { return c->K[1] * ((c->K[2] + FFT_LB - 1) / FFT_LB); }
+int ttc_pme_conv_blocks(ttc_ctx *c) { return (c->K[1] * (c->K[2] / 2 + 1) + FFT_LB - 1) / FFT_LB; }
 int ttc_gather_blocks(ttc_ctx *c) { return (c->n + 255) / 256; }
 
 static Spread spread_args(const ttc_ctx *c) {
@@ -1189,10 +1230,10 @@ static FftPack fft_pack(const ttc_ctx *c, int pass) {
 }
 
 static int fft_blocks(const ttc_ctx *c, int pass) {
-    int nzb = (c->K[2] + FFT_LB - 1) / FFT_LB;
-    if (pass == FP_Z_FWD || pass == FP_Z_INV) return (c->K[0] * c->K[1] + FFT_LB - 1) / FFT_LB;
-    if (pass == FP_X_CONV) return c->K[1] * nzb;
-    return c->K[0] * nzb;
+    const int H = c->K[2] / 2 + 1;
+    if (pass == FP_Z_FWD || pass == FP_Z_INV) return (c->K[0] * c->K[1] + 2 * FFT_LB - 1) / (2 * FFT_LB);
+    if (pass == FP_X_CONV) return (c->K[1] * H + FFT_LB - 1) / FFT_LB;
+    return (c->K[0] * H + FFT_LB - 1) / FFT_LB;
 }
 
 // FFT passes and influence function.  Expects the spread grid in qfix;
