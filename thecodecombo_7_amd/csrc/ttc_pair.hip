@@ -211,9 +211,9 @@ __device__ __forceinline__ void pair_entry_half(uint32_t e, const int4 &pj, int 
     const bool real = e != TTC_PAD;
     bool in = real & (dr2 < pc.rc2_lo);
     if (real & !(dr2 < pc.rc2_lo) & !(dr2 > pc.rc2_hi)) {
-        // within 1e-8 (relative) of rc^2: the reference's own FP64 test on
-        // the FP64 positions decides (cnonbond_nblist.c:174-176, the sum
-        // rounded op by op)
+        // near rc^2 (the band: 1e-8 plus the coordinates' quantization):
+        // the reference's own FP64 test on the FP64 positions decides
+        // (cnonbond_nblist.c:174-176, the sum rounded op by op)
         const double4 P = pos4[s], Q = pos4[j];
         double fx = P.x - Q.x, fy = P.y - Q.y, fz = P.z - Q.z;
         fx = fma(-pc.lx, rint(fx * pc.ilx), fx);
@@ -610,11 +610,16 @@ static PairConst pair_const(const ttc_ctx *c) {
     pc.uy = c->p.ly / 4294967296.0;
     pc.uz = c->p.lz / 4294967296.0;
     // the fixed-point r^2 differs from the FP64 positions' r^2 by the
-    // coordinates' quantization, 2 r sqrt(3) 2^-33 L: below 1e-9 of rc^2
-    // for any box the list allows (L <= 1e4 rc); outside +-1e-8 it decides
-    // the cutoff alone
-    pc.rc2_lo = pc.rc2 * (1.0 - 1e-8);
-    pc.rc2_hi = pc.rc2 * (1.0 + 1e-8);
+    // coordinates' quantization: each coordinate is rounded to L / 2^32, so
+    // a difference is within L / 2^32 of the FP64 one and |r_q^2 - r^2| <=
+    // 2 r sqrt(3) Lmax 2^-32 (plus its negligible square), 2 sqrt(3) Lmax /
+    // (rc 2^32) of rc^2 at r ~ rc (1.6e-8 for a 200 A box at rc = 10 A).
+    // Outside twice that, plus 1e-8 for the FP64 roundings, it decides the
+    // cutoff alone
+    const double lmax = std::max(c->p.lx, std::max(c->p.ly, c->p.lz));
+    const double band = 1e-8 + 4.0 * sqrt(3.0) * lmax / (c->p.rc * 4294967296.0);
+    pc.rc2_lo = pc.rc2 * (1.0 - band);
+    pc.rc2_hi = pc.rc2 * (1.0 + band);
     return pc;
 }
 
