@@ -21,7 +21,9 @@
 
 #include <algorithm>
 
-#define FFT_LB 16
+// FFT lines per workgroup: 8 keep a workgroup's LDS (13 KiB at K = 96)
+// small enough to sit beside two pair-kernel groups on a CU
+#define FFT_LB 8
 #define FFT_THREADS 256
 // threads of one spread workgroup (one brick)
 #ifndef SPREAD_THREADS
@@ -1028,8 +1030,10 @@ int ttc_pme_setup(ttc_ctx *c) {
     c->ngrid = (size_t)c->K[0] * c->K[1] * c->K[2];
     for (int a = 0; a < 3; a++) {
         // one FFT workgroup holds two buffers of FFT_LB lines of K + 1
-        // complex values and K twiddles (8 (33 K + 32) bytes) and its static
-        // 224-byte reduction array in the default 64 KiB of LDS: K <= 246
+        // complex values and K twiddles (8 (2 FFT_LB (K + 1) + K) bytes)
+        // and its static 224-byte reduction array, far inside the default
+        // 64 KiB of LDS; grids are limited to 246 points per axis (the
+        // sizes tested reach 224)
         if (c->K[a] > 246) { ttc_set_error("PME grid dimension above 246 is not supported"); return TTC_ERR_ARG; }
         int r = make_plan(c->plan[a], c->K[a]);
         if (r) { ttc_set_error("PME grid size cannot be planned"); return r; }
