@@ -608,9 +608,9 @@ __device__ __forceinline__ void build_grp_body(
         int qh = 0, qn = 0;   // ring head and length (wave-uniform)
 
         // test every row against the cnt (<= 64) candidates at the head of the ring.
-        // The rows are unrolled (lane r holds row r's coordinates and entry
-        // count, read wave-uniform), and a candidate's bitmap bit is set
-        // once for all rows that keep it.
+        // Lane r holds row r's coordinates and entry count, read wave-uniform;
+        // the rows' ballots collect the candidates some row keeps, whose
+        // bitmap bits are set once after the rows.
         auto consume = [&](int cnt, auto MI, auto HALF) {
             const int slot = (qh + lane) & (GB_QCAP - 1);
             const float4 pj = qp[slot];
@@ -620,10 +620,9 @@ __device__ __forceinline__ void build_grp_body(
             const int wj = __float_as_int(pj.w);
             const int oj = wj & (int)TTC_JMASK;                                    // original index
             const uint32_t ej = (uint32_t)j | ((uint32_t)wj & ~TTC_JMASK);         // sorted index | type bits
-            bool kept = false;   // some row keeps this candidate (half lists)
-#pragma unroll
-            for (int rr = 0; rr < GB_ROWS; rr++) {
-                if (rr >= nrw) break;
+            unsigned long long kmask = 0ull;   // candidates some row keeps (half lists)
+#pragma unroll 2
+            for (int rr = 0; rr < nrw; rr++) {
                 const float rwx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ux), rr));
                 const float rwy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(uy), rr));
                 const float rwz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(uz), rr));
@@ -639,11 +638,11 @@ __device__ __forceinline__ void build_grp_body(
                     // its own negative) by index (a site is not above itself)
                     const int dq = zj - __builtin_amdgcn_readlane(rowzk, rr);
                     hit = hit & ((dq > 0) | (((unsigned)dq << 1 == 0u) & (j > wb + rr)));
-                    kept |= hit;
                 } else {
                     hit = hit & (j != wb + rr);
                 }
                 const unsigned long long b = __builtin_amdgcn_ballot_w64(hit);
+                kmask |= b;
                 if (b != 0ull) {
                     const int base = __builtin_amdgcn_readlane(rkl, rr);
                     const int pos = base + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
@@ -680,7 +679,7 @@ __device__ __forceinline__ void build_grp_body(
                         grow[(uint32_t)(((pos >> 4) << 10) + (row0 + rr) * 16 + (pos & 15))] = ej | (cls << 30);
                 }
             }
-            if (decltype(HALF)::value && kept) {
+            if (decltype(HALF)::value && ((kmask >> lane) & 1ull)) {
                 // partner's bitmap bit (cyclic offset from the group's first site)
                 int djs = j - sb;
                 if (2 * djs > n) djs -= n;
