@@ -568,7 +568,7 @@ static int ensure_format(ttc_ctx *c, bool half) {
 }
 
 // half lists for the velocity-Verlet loop from 256 sites on
-static bool half_enabled(const ttc_ctx *c) { return c->n >= 256; }
+static bool half_enabled(const ttc_ctx *c) { return ttc_half_enabled(c); }
 
 static int finish(ttc_ctx *c, double *ff, int64_t stride) {
     TTC_HIP(hipGetLastError());
@@ -1610,7 +1610,7 @@ void ttc_launch_nhc_step(ttc_ctx *c, bool energy) {
         TtcOnSide side(c);
         ttc_launch_gather(c, energy, true, TTC_BT_ALL, false);
     }
-    ttc_launch_pair(c, energy, 0);
+    ttc_launch_pair_drifted(c, energy);
     ttc_side_join(c);
     ttc_launch_kick(c, 1.0, true);   // kinetic partials for the second thermostat half
     ttc_launch_nhc(c);
@@ -1621,19 +1621,54 @@ void ttc_launch_nhc_step(ttc_ctx *c, bool energy) {
     }
 }
 
-// state a captured NHC step needs: list, row buffer, forces at the
-// current positions
-int ttc_prepare_nhc(ttc_ctx *c, int max_rows) {
-    CHECK(c && c->topo && c->nhc_M > 0, TTC_ERR_ARG, "ttc_md_nhc: no topology or no thermostat");
-    full_mode(c);
-    int r = need_list(c);
-    if (r) return r;
-    r = ensure_rows(c, 1, max_rows);
+// The list format of the NHC loop: half lists (each pair once, partner
+// forces through the windows) where the context is large enough, as in the
+// velocity-Verlet loop.  The half-list format is made after the forces of
+// the current positions, which the full list gives.
+// These loops redo no step after an overflow (a run that overflows
+// reports it at the end), so their half lists get margins first: the
+// partner bitmap over the whole ring where that costs at most 16 KiB of
+// LDS, and windows at least 30 % above the largest one the build at the
+// anchors makes.
+static int nhc_list(ttc_ctx *c, bool half) {
+    if (c->half_mode != half) {
+        c->half_mode = half;
+        destroy_graphs(c);
+        c->gen++;
+    }
+    int r = need_list(c, half);
     if (r) return r;
     if (!c->f4_valid) {
         launch_forces(c, false);
         c->f4_valid = true;
     }
+    if (half && !c->list_half && (size_t)(c->n + 128) / 64 * 12 <= 16384 && c->span < c->n) {
+        ttc_bitmap_sizes(c, c->n / 2 + 64, c->n / 2 + 64, &c->span, &c->span_pre);
+        r = ttc_build_grp_attrs(c);
+        if (r) return r;
+        destroy_graphs(c);
+        c->gen++;
+    }
+    r = ensure_format(c, half);
+    if (r || !half) return r;
+    const int wmax = c->h_status[ST_WMAX];
+    if (wmax * 13 / 10 > c->wcap) {
+        r = grow_window(c, wmax * 13 / 10, 0, 0);
+        if (r) return r;
+        c->list_half = false;   // build the half list again with the wider windows
+        r = ensure_format(c, true);
+    }
+    return r;
+}
+
+// state a captured NHC step needs: list (half lists where `half`), row
+// buffer, forces at the current positions
+int ttc_prepare_nhc(ttc_ctx *c, int max_rows, bool half) {
+    CHECK(c && c->topo && c->nhc_M > 0, TTC_ERR_ARG, "ttc_md_nhc: no topology or no thermostat");
+    int r = ensure_rows(c, 1, max_rows);
+    if (r) return r;
+    r = nhc_list(c, half && half_enabled(c));
+    if (r) return r;
     c->mts_valid = false;
     return TTC_OK;
 }
@@ -1642,15 +1677,10 @@ extern "C" int ttc_md_nhc(ttc_ctx *c, int nsteps, int energy_every, double *ener
                           int *rebuilds) {
     CHECK(c && c->topo && nsteps >= 0 && c->nhc_M > 0, TTC_ERR_ARG, "ttc_md_nhc: bad argument or no thermostat");
     CHECK(energy_every == 0 || (energies && max_rows > 0), TTC_ERR_ARG, "ttc_md_nhc: no row buffer");
-    full_mode(c);
-    int r = need_list(c);
+    int r = ensure_rows(c, energy_every, max_rows);
     if (r) return r;
-    r = ensure_rows(c, energy_every, max_rows);
+    r = nhc_list(c, half_enabled(c));
     if (r) return r;
-    if (!c->f4_valid) {
-        launch_forces(c, false);
-        c->f4_valid = true;
-    }
     if (!c->g_nhc_plain) {
         r = capture_fn(c, &c->g_nhc_plain, [&] { ttc_launch_nhc_step(c, false); });
         if (r) return r;

@@ -32,7 +32,7 @@
 #include <chrono>
 #include <cmath>
 
-int ttc_prepare_nhc(ttc_ctx *c, int max_rows);
+int ttc_prepare_nhc(ttc_ctx *c, int max_rows, bool half);
 void ttc_launch_nhc_step(ttc_ctx *c, bool energy);
 extern "C" int ttc_set_nhc_temperature(ttc_ctx *c, double T, double Q);
 extern "C" int ttc_tune(ttc_ctx *c, int *pair_waves, double *usec3);
@@ -355,12 +355,20 @@ extern "C" int ttc_batch_set_launch(ttc_batch *b, int mode, int steps_per_launch
     return TTC_OK;
 }
 
+// half lists for the NHC steps when every replica takes them (the batched
+// pair stage runs one kernel variant for all)
+static bool batch_half(const ttc_batch *b) {
+    for (const ttc_ctx *c : b->r)
+        if (!ttc_half_enabled(c)) return false;
+    return true;
+}
+
 extern "C" int ttc_batch_md_nhc(ttc_batch *b, int nsteps, double *potens, int *rebuilds) {
     BCHECK(b && nsteps >= 0 && nsteps % b->spl == 0, TTC_ERR_ARG,
            "ttc_batch_md_nhc: nsteps must be a multiple of the steps per launch");
     size_t R = b->r.size();
     for (ttc_ctx *c : b->r) {
-        int r = ttc_prepare_nhc(c, 1);
+        int r = ttc_prepare_nhc(c, 1, batch_half(b));
         if (r) return r;
     }
     int r = ensure_graphs(b);
@@ -390,7 +398,10 @@ extern "C" int ttc_batch_md_nhc(ttc_batch *b, int nsteps, double *potens, int *r
         TTC_HIP(hipMemcpy(c->h_status, c->status, sizeof(int) * ST_NSTAT, hipMemcpyDeviceToHost));
         if (c->h_status[ST_OVERFLOW]) {
             ttc_set_error("ttc_batch_md_nhc: neighbour list overflow in replica " + std::to_string(i) +
-                          " (nblist.c:185-196); call ttc_nblist on it and run again");
+                          " (nblist.c:185-196; flags " + std::to_string(c->h_status[ST_OVERFLOW]) +
+                          ", longest row " + std::to_string(c->h_status[ST_MAXROW]) + " of " +
+                          std::to_string(c->cap) + ", window " + std::to_string(c->h_status[ST_WMAX]) + " of " +
+                          std::to_string(c->wcap) + "); call ttc_nblist on it and run again");
             return TTC_ERR_LIST_OVERFLOW;
         }
         if (rebuilds) rebuilds[i] = c->h_status[ST_NREBUILD];
@@ -453,7 +464,7 @@ extern "C" int ttc_batch_state(ttc_batch *b, int *order, double *temps, double *
 extern "C" int ttc_batch_autotune(ttc_batch *b, int trial_steps, int *best_mode, int *best_spl, double *timings) {
     BCHECK(b && trial_steps >= 1, TTC_ERR_ARG, "ttc_batch_autotune: bad argument");
     for (ttc_ctx *c : b->r) {
-        int r = ttc_prepare_nhc(c, 1);
+        int r = ttc_prepare_nhc(c, 1, batch_half(b));
         if (!r) r = ttc_tune(c, nullptr, nullptr);
         if (r) return r;
     }

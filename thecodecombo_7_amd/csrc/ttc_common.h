@@ -295,6 +295,9 @@ void ttc_launch_pair(ttc_ctx *c, bool energy, int blk_off);
 // pair kernel of an MD step: also counts the step and clears the trigger
 // flag the step's conditional rebuild has consumed
 void ttc_launch_pair_step(ttc_ctx *c, bool energy);
+void ttc_launch_pair_drifted(ttc_ctx *c, bool energy);
+// MD loops keep half lists (each pair once) from 256 sites on
+inline bool ttc_half_enabled(const ttc_ctx *c) { return c->n >= 256; }
 void ttc_launch_pair_range(ttc_ctx *c, int lb, int ub);
 void ttc_launch_pair_hes(ttc_ctx *c);    // energies + Hessian diagonal (CONFIGT)
 int ttc_pair_blocks(ttc_ctx *c);
@@ -469,6 +472,7 @@ struct TtcStage {
     TtcBmap *map = nullptr;    // device array, one entry per block
     int nblk = 0;
     size_t shm = 0;            // dynamic LDS per block (largest over the contexts)
+    int kind = 0;              // variant of the stage's kernel (pair: 1 = half lists)
 };
 int ttc_stage_upload(TtcStage *st, const void *packs, size_t pack_bytes, int nrep, const std::vector<int> &blocks,
                      size_t shm);

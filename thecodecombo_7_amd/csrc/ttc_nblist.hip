@@ -1211,21 +1211,24 @@ int ttc_bstage_rebuild(const std::vector<ttc_ctx *> &r, TtcStage *st) {
     std::vector<ReorderPack> pr;
     std::vector<BuildGrpPack> pb;
     std::vector<int> nsite, ngrp;
+    size_t shm = 0;   // half lists: the largest partner bitmap
     for (ttc_ctx *c : r) {
         pa.push_back(cell_assign_pack(c, 1));
         pr.push_back(reorder_pack(c, 1));
-        pb.push_back(build_grp_pack_of(c, false, 1, 1));
+        pb.push_back(build_grp_pack_of(c, c->half_mode, 1, 1));
         nsite.push_back((c->n + 255) / 256);
         ngrp.push_back(c->G);
+        shm = std::max(shm, build_grp_shm(c, c->half_mode));
     }
+    TTC_HIP(hipFuncSetAttribute((const void *)kb_build_grp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     int e = ttc_stage_make(&st[0], pa, nsite, 0);
     if (!e) e = ttc_stage_make(&st[1], pr, nsite, 0);
-    if (!e) e = ttc_stage_make(&st[2], pb, ngrp, 0);
+    if (!e) e = ttc_stage_make(&st[2], pb, ngrp, shm);
     return e;
 }
 
 void ttc_blaunch_rebuild(const TtcStage *st, hipStream_t s) {
     kb_cell_assign<<<st[0].nblk, 256, 0, s>>>((const CellAssignPack *)st[0].pk, st[0].map);
     kb_reorder<<<st[1].nblk, 256, 0, s>>>((const ReorderPack *)st[1].pk, st[1].map);
-    kb_build_grp<<<st[2].nblk, 64 * GB_WAVES, 0, s>>>((const BuildGrpPack *)st[2].pk, st[2].map);
+    kb_build_grp<<<st[2].nblk, 64 * GB_WAVES, st[2].shm, s>>>((const BuildGrpPack *)st[2].pk, st[2].map);
 }

@@ -583,6 +583,14 @@ __global__ void __launch_bounds__(64 * WAVES) HALF_WPE k_pair_half(PairHalfPack 
     pair_half_body<ENERGY, WAVES>(blockIdx.x, a);
 }
 
+// batched (ttc_batch.hip): block `blk` of context `rep`, its argument pack
+// read in place
+template <bool ENERGY, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) HALF_WPE kb_pair_half(const PairHalfPack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    pair_half_body<ENERGY, WAVES>(m.blk, pk[m.rep]);
+}
+
 }  // namespace
 
 extern double ttc_nr_erfc_host(double x);
@@ -712,6 +720,14 @@ void ttc_launch_pair(ttc_ctx *c, bool energy, int /*blk_off*/) {
     else launch_pair_w<false, false>(c, RangeSel{});
 }
 
+// after a drift (which wrote the fixed-point coordinates) outside the
+// velocity-Verlet loop: the current list's kernel, no step counting
+void ttc_launch_pair_drifted(ttc_ctx *c, bool energy) {
+    if (c->list_half) launch_pair_half(c, energy, false, false);
+    else if (energy) launch_pair_w<true, false>(c, RangeSel{});
+    else launch_pair_w<false, false>(c, RangeSel{});
+}
+
 void ttc_launch_pair_step(ttc_ctx *c, bool energy) {
     if (c->list_half) launch_pair_half(c, energy, true);
     else if (energy) launch_pair_w<true, false>(c, RangeSel{}, c->status);
@@ -719,20 +735,45 @@ void ttc_launch_pair_step(ttc_ctx *c, bool energy) {
 }
 
 // ---- batched stage of the NHC step (ttc_batch.hip): every context's pair
-// kernel with 8 waves per group (the untuned default of a context)
+// kernel with 8 waves per group (the untuned default of a context), the
+// half-list kernel when every context holds a half list
 int ttc_bstage_pair(const std::vector<ttc_ctx *> &r, TtcStage *st) {
-    std::vector<PairPack> pk;
+    bool half = !r.empty();
+    for (ttc_ctx *c : r) half = half && c->list_half;
     std::vector<int> nb;
     size_t shm = 0;
+    if (half) {
+        std::vector<PairHalfPack> pk;
+        for (ttc_ctx *c : r) {
+            pk.push_back(PairHalfPack{c->n, c->cap, c->wcap, c->nbl, c->glen, c->win, c->wlen, c->pos4, c->posq,
+                                      c->type, c->ljtab, pair_const(c), c->fpair, c->fixj, c->partials, c->nblk_max,
+                                      c->status, 0, 6755399441055744.0});
+            nb.push_back(c->G);
+            shm = std::max(shm, ttc_pair_half_shm(c));
+        }
+        const void *fns[] = {(const void *)kb_pair_half<true, 8>, (const void *)kb_pair_half<false, 8>};
+        for (const void *f : fns) TTC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        int e = ttc_stage_make(st, pk, nb, shm);
+        st->kind = 1;
+        return e;
+    }
+    std::vector<PairPack> pk;
     for (ttc_ctx *c : r) {
         pk.push_back(pair_pack(c, RangeSel{}));
         nb.push_back(ttc_pair_blocks(c));
         shm = std::max(shm, sizeof(float4) * c->ntypes * c->ntypes);
     }
-    return ttc_stage_make(st, pk, nb, shm);
+    int e = ttc_stage_make(st, pk, nb, shm);
+    st->kind = 0;
+    return e;
 }
 
 void ttc_blaunch_pair(const TtcStage &st, bool energy, hipStream_t s) {
+    if (st.kind == 1) {
+        if (energy) kb_pair_half<true, 8><<<st.nblk, 64 * 8, st.shm, s>>>((const PairHalfPack *)st.pk, st.map);
+        else kb_pair_half<false, 8><<<st.nblk, 64 * 8, st.shm, s>>>((const PairHalfPack *)st.pk, st.map);
+        return;
+    }
     if (energy) kb_pair<true, 8><<<st.nblk, 64 * 8, st.shm, s>>>((const PairPack *)st.pk, st.map);
     else kb_pair<false, 8><<<st.nblk, 64 * 8, st.shm, s>>>((const PairPack *)st.pk, st.map);
 }

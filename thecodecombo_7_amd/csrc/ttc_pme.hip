@@ -408,7 +408,7 @@ __global__ void __launch_bounds__(SPREAD_THREADS) kb_drift_spread(const DsPack *
     const TtcBmap m = map[blockIdx.x];
     const DsPack a = pk[m.rep];
     drift_spread_body<P, DRIFT>(m.blk, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start, a.cell_cnt,
-                                a.qfix, a.status, a.anchor, a.pcheck);
+                                a.qfix, a.status, a.anchor, a.pcheck, a.ki);
 }
 
 struct FftArgs {
@@ -1149,10 +1149,19 @@ static size_t brick_shm(const ttc_ctx *c, const Brick &br) {
 }
 
 static DsPack ds_pack(ttc_ctx *c, bool drift, double vscal) {
-    return DsPack{c->pos4, c->vel4, c->f4, drift ? c->p.dt : 0.0, vscal, spread_args(c), spread_brick(c),
-                  c->cell_start, c->cell_cnt, c->qfix, c->status, c->anchor, c->ev + EV_N,
-                  KickIn{c->fpair, c->g4, c->partials, c->nblk_max, ttc_gather_blocks(c), c->n, vscal, nullptr,
-                         nullptr, c->fixj}};
+    DsPack a{c->pos4, c->vel4, c->f4, drift ? c->p.dt : 0.0, vscal, spread_args(c), spread_brick(c),
+             c->cell_start, c->cell_cnt, c->qfix, c->status, c->anchor, c->ev + EV_N,
+             KickIn{c->fpair, c->g4, c->partials, c->nblk_max, ttc_gather_blocks(c), c->n, vscal, nullptr,
+                    nullptr, c->fixj}};
+    if (drift) {
+        // the drifted positions' fixed-point coordinates, which the
+        // half-list pair kernel reads
+        a.ki.posq = c->posq;
+        a.ki.qkx = 4294967296.0 / c->p.lx;
+        a.ki.qky = 4294967296.0 / c->p.ly;
+        a.ki.qkz = 4294967296.0 / c->p.lz;
+    }
+    return a;
 }
 
 static int ds_blocks(const ttc_ctx *c) {
@@ -1182,10 +1191,6 @@ void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick) {
     a.ki.g4 = c->g4o;
     a.ki.orig = c->orig;
     a.ki.snap = c->snap;
-    a.ki.posq = c->posq;
-    a.ki.qkx = 4294967296.0 / c->p.lx;
-    a.ki.qky = 4294967296.0 / c->p.ly;
-    a.ki.qkz = 4294967296.0 / c->p.lz;
 #define DS(P, K) k_drift_spread<P, true, K><<<blocks, SPREAD_THREADS, shm, c->stream>>>(a)
     if (c->order == 4) { if (kick) DS(4, true); else DS(4, false); }
     else { if (kick) DS(6, true); else DS(6, false); }
