@@ -1504,7 +1504,7 @@ static int ensure_rows(ttc_ctx *c, int energy_every, int max_rows) {
     return TTC_OK;
 }
 
-// the NHC and RESPA loops keep full lists
+// the RESPA loop keeps full lists
 static void full_mode(ttc_ctx *c) {
     if (c->half_mode) {
         c->half_mode = false;

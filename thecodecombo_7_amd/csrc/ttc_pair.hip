@@ -8,11 +8,12 @@
 // the reference's pairs are evaluated); the LJ, erfc and exp arithmetic runs
 // in FP32; each site's force and the energies/virial accumulate in FP64.
 // Two list formats:
-//   * full lists (both directions; API calls, NHC and RESPA loops, replica
-//     batches): no scatter to j, every site's force is a fixed-order sum;
-//   * half lists (velocity-Verlet loops, k_pair_half): each pair once, the
-//     partner's force added as int64 fixed point into LDS window slots and
-//     flushed with integer atomics, so the result does not depend on order.
+//   * full lists (both directions; API calls, RESPA loops, small contexts):
+//     no scatter to j, every site's force is a fixed-order sum;
+//   * half lists (velocity-Verlet and NHC loops, replica batches;
+//     k_pair_half, kb_pair_half): each pair once, the partner's force added
+//     as int64 fixed point into LDS window slots and flushed with integer
+//     atomics, so the result does not depend on order.
 #include "ttc_common.h"
 
 namespace {
