@@ -4,9 +4,11 @@
 // One lane per sorted site, one workgroup per ELL group of 64 sites.  Each
 // step of the row loop reads one entry per lane from the group's current
 // 16-entry list tile (4 KB, shared by the group's waves).  The
-// cutoff test and minimum image run in FP64 on FP64 positions (so exactly
-// the reference's pairs are evaluated); the LJ, erfc and exp arithmetic runs
-// in FP32; each site's force and the energies/virial accumulate in FP64.
+// cutoff test and minimum image run in FP64 on FP64 positions (the
+// half-list kernel: on 32-bit fixed-point coordinates, with the FP64
+// positions deciding inside a band around rc^2), so exactly the reference's
+// pairs are evaluated; the LJ, erfc and exp arithmetic runs in FP32; each
+// site's force and the energies/virial accumulate in FP64.
 // Two list formats:
 //   * full lists (both directions; API calls, RESPA loops, small contexts):
 //     no scatter to j, every site's force is a fixed-order sum;
