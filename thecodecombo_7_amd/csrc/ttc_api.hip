@@ -134,6 +134,11 @@ extern "C" int ttc_create(const ttc_params *p, ttc_ctx **out) {
     ALLOC(c->cell_of, n); ALLOC(c->cell_cnt, c->ncell); ALLOC(c->cell_start, c->ncell);
     ALLOC(c->cell_acc, c->ncell);
     TTC_HIP(hipMemset(c->cell_acc, 0, sizeof(int) * c->ncell));
+    // the MD spread's copies start empty (no sites in any cell)
+    ALLOC(c->spos, n); ALLOC(c->scs, c->ncell); ALLOC(c->scc, c->ncell);
+    TTC_HIP(hipMemset(c->spos, 0, sizeof(double4) * n));
+    TTC_HIP(hipMemset(c->scs, 0, sizeof(int) * c->ncell));
+    TTC_HIP(hipMemset(c->scc, 0, sizeof(int) * c->ncell));
     ALLOC(c->cell_members, (size_t)c->ncell * ttc_cellcap(c));
 
     c->cap = list_capacity(c, ttc_list_cutoff(c));
@@ -193,7 +198,7 @@ extern "C" int ttc_destroy(ttc_ctx *c) {
                     c->cell_members, c->nbl, c->glen, c->status, c->ex_off, c->ex_part, c->ex_flag, c->ljtab,
                     c->b_off, c->b_part, c->b_r0, c->b_k, c->partials, c->ev, c->rows, c->fs, c->fl, c->fsb,
                     c->flb, c->nhc, c->posfb, c->gcnt, c->snap, c->g4o, c->fixj, c->win, c->wlen, c->ang_idx,
-                    c->ang_par, c->ang_tpar, c->ang_tf, c->ang_off, c->ang_ref, c->posq};
+                    c->ang_par, c->ang_tpar, c->ang_tf, c->ang_off, c->ang_ref, c->posq, c->spos, c->scs, c->scc};
     for (void *b : bufs)
         if (b) hipFree(b);
     ttc_pme_free(c);
@@ -941,16 +946,17 @@ extern "C" int ttc_kinet(ttc_ctx *c, double *kinet, double *ken6) {
     return TTC_OK;
 }
 
-// One velocity-Verlet step as launched (and captured): conditional rebuild,
-// kick + drift + spread, pair, PME (+ bonds), kick + trigger.
+// One velocity-Verlet step as launched (and captured): kick + drift,
+// conditional rebuild, pair, spread + PME (+ bonds), kick + trigger.
 // integrate_vverlet.c:36-103 with the DLIST trigger of nve_md.c:369-370:
 // the drift checks the displacement and the conditional rebuild runs
 // before this step's forces, with the same anchors (this step's positions)
 // the reference rebuilds at after the step.
-// The FFTs need only the charge grid the drift spread, and the gather works
-// in the original numbering on a snapshot the drift writes, so both run on
-// the side stream while the main stream rebuilds the list and runs the pair
-// kernel: one fork and one join per step.
+// The spread reads copies of the drifted positions and of the cell ranges
+// the drift writes, the FFTs only the charge grid, and the gather works in
+// the original numbering on a snapshot the drift writes, so all three run
+// on the side stream while the main stream rebuilds the list and runs the
+// pair kernel: one fork and one join per step.
 void ttc_side_fork(ttc_ctx *c, int k) {
     hipEventRecord(c->sev[k], c->stream);
     hipStreamWaitEvent(c->side, c->sev[k], 0);
@@ -977,6 +983,7 @@ static void launch_step(ttc_ctx *c, double vscal, bool energy, bool head) {
     hipStreamWaitEvent(c->side, c->sev[0], 0);
     {
         TtcOnSide side(c);
+        ttc_launch_md_spread(c);
         ttc_launch_fft(c, energy);
         ttc_launch_gather_orig(c, energy);
     }

@@ -165,6 +165,11 @@ struct ttc_ctx {
     // and the gather's output by original index, so the gather does not
     // wait for the step's rebuild
     double4 *snap = nullptr, *g4o = nullptr;
+    // MD step: the spread's inputs as the drift leaves them (positions +
+    // charge in the sorted order, the cell ranges of that order), read by
+    // the spread on the side stream while a rebuild rewrites the originals
+    double4 *spos = nullptr;
+    int *scs = nullptr, *scc = nullptr;
     bool f4_valid = false;          // f4 holds the force at the current positions
     // RESPA short (bonded) and long (nonbonded) forces, sorted order, permuted
     // with every rebuild like f4
@@ -335,9 +340,12 @@ int ttc_pme_conv_blocks(ttc_ctx *c);
 int ttc_gather_blocks(ttc_ctx *c);
 void ttc_launch_spread(ttc_ctx *c);
 void ttc_launch_drift(ttc_ctx *c, double vscal);
-// drift of an MD step: writes the original-order snapshot for the gather;
-// kick = the previous step's second half kick fused ahead of it
+// drift of an MD step: writes the original-order snapshot for the gather
+// and the spread's inputs; kick = the previous step's second half kick fused
+// ahead of it
 void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick);
+// spread of an MD step from the drift's copies (spos, scs, scc)
+void ttc_launch_md_spread(ttc_ctx *c);
 // the gather in the original numbering (snapshot in, g4o out)
 void ttc_launch_gather_orig(ttc_ctx *c, bool energy);
 // md / combine / reductions

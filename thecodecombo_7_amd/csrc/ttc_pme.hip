@@ -6,6 +6,9 @@
 //                   26-113): B-spline weights (pme_spline.c:10-49) in FP32,
 //                   q*Mx*My*Mz spread with int64 fixed-point atomics, so the
 //                   grid is the same whatever order the atomics land in.
+//   k_md_drift      the velocity-Verlet loop's kick + drift alone, one
+//                   thread per site, with copies of the spread's inputs, so
+//                   the spread of that loop runs on the side stream.
 //   k_fft<MODE>     3-D complex FFT as three line passes in LDS (Stockham,
 //                   mixed radix 2/3/4 + direct DFT for other primes).  The
 //                   z pass reads the fixed-point grid and clears it for the
@@ -221,26 +224,69 @@ struct KickIn {
     double qkx = 0.0, qky = 0.0, qkz = 0.0;
 };
 
-// kick + drift + pbc (integrate_vverlet.c:52-83) of every site of a brick,
-// the DLIST displacement check of nblist.c:607-647 on the new position
-// (pcheck = {lx, ly, lz, (rclist-rc)/4}; it sets the flag the conditional
-// rebuild that follows in the same step reads, so the forces of this step
-// are evaluated with a list valid for these positions), then charge_grid
-// (pme_qgrid.c:77-113) into an LDS tile of int64 fixed point, flushed to
-// the global fixed-point grid with one atomic per touched grid point.
-// Integer adds make the grid independent of order.
-//
+// kick + drift + pbc of one site (integrate_vverlet.c:52-83), its
+// original-order snapshot and fixed-point coordinates when asked for, and the
+// DLIST displacement check of nblist.c:607-647 on the new position (pcheck =
+// {lx, ly, lz, (rclist-rc)/4}; it sets the flag the conditional rebuild that
+// follows in the same step reads, so the forces of this step are evaluated
+// with a list valid for these positions).
 // KICK: the previous step's second half kick (integrate_vverlet.c:93-102)
 // first, on the force combined here from the pair and gather outputs (kept
 // in f4), exactly as the separate kick kernel computes it.
-template <int P, bool DRIFT, bool KICK = false>
+template <bool KICK>
+__device__ __forceinline__ double4 drift_site(int s, double4 *pos4, double4 *vel4, double4 *f4, double dt,
+                                              double vscal, const Spread &sp, int *status,
+                                              const double4 *__restrict__ anchor, const double *__restrict__ pcheck,
+                                              const KickIn &ki, const double *mean) {
+    double4 p = pos4[s];
+    double4 v = vel4[s];
+    double4 f;
+    double dth = 0.5 * dt;
+    double im = v.w;   // 100/m
+    const int so = ki.orig ? ki.orig[s] : s;
+    if (KICK) {
+        f = ttc_combine(s, 7, ki.fpair, ki.g4, mean, so, ki.fixj, ki.n);
+        f4[s] = f;
+        v.x = v.x + ki.vscal * dth * v.w * f.x;
+        v.y = v.y + ki.vscal * dth * v.w * f.y;
+        v.z = v.z + ki.vscal * dth * v.w * f.z;
+    } else {
+        f = f4[s];
+    }
+    v.x = vscal * (v.x + dth * im * f.x);
+    v.y = vscal * (v.y + dth * im * f.y);
+    v.z = vscal * (v.z + dth * im * f.z);
+    p.x = pbc1(p.x + dt * v.x, sp.lx);
+    p.y = pbc1(p.y + dt * v.y, sp.ly);
+    p.z = pbc1(p.z + dt * v.z, sp.lz);
+    vel4[s] = v;
+    pos4[s] = p;
+    if (ki.snap) ki.snap[so] = p;
+    if (ki.posq) ki.posq[s] = ttc_posq(p, ki.qkx, ki.qky, ki.qkz);
+    double4 a = anchor[s];
+    double rcut = pcheck[3];
+    double dx = fabs(p.x - a.x);
+    if (dx > 0.5 * sp.lx) dx -= sp.lx;
+    double dy = fabs(p.y - a.y);
+    if (dy > 0.5 * sp.ly) dy -= sp.ly;
+    double dz = fabs(p.z - a.z);
+    if (dz > 0.5 * sp.lz) dz -= sp.lz;
+    if (dx > rcut || dy > rcut || dz > rcut || ttc_r2_exact(dx, dy, dz) > rcut * rcut)
+        atomicOr(&status[ST_NLFLAG], 1);
+    return p;
+}
+
+// DRIFT: drift_site of every site of a brick; then charge_grid
+// (pme_qgrid.c:77-113) into an LDS tile of int64 fixed point, flushed to
+// the global fixed-point grid with one atomic per touched grid point.
+// Integer adds make the grid independent of order.
+template <int P, bool DRIFT>
 __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 *vel4, double4 *f4, double dt,
                                                   double vscal, const Spread &sp, const Brick &br,
                                                   const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
                                                   long long *qfix, int *status, const double4 *__restrict__ anchor,
                                                   const double *__restrict__ pcheck, const KickIn &ki = KickIn{}) {
     extern __shared__ unsigned long long tile[];
-    __shared__ double mean[3];
     // one chunk of sites after the tile: q Mx[1..P], My[1..P], Mz[1..P] per
     // site and the packed tile-local stencil base (x | y << 10 | z << 20;
     // -1: site spread straight to the grid)
@@ -250,7 +296,6 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
     // a velocity-Verlet loop whose rebuild overflowed stops here; the host
     // grows the list and redoes that step from its drifted positions
     if (DRIFT && ki.snap && (status[ST_OVERFLOW] & TTC_OVF_REDO)) return;
-    if (KICK) ttc_mean_pme(ki.partials, ki.maxb, ki.nbg, ki.n, mean);
     int ibx = b % br.nbx, iby = (b / br.nbx) % br.nby, ibz = b / (br.nbx * br.nby);
     int cx0 = ibx * br.bx, cy0 = iby * br.by, cz0 = ibz * br.bz;
     int cx1 = min(cx0 + br.bx, br.ncx), cy1 = min(cy0 + br.by, br.ncy), cz1 = min(cz0 + br.bz, br.ncz);
@@ -286,43 +331,9 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
 #pragma unroll
             for (int r = BRICK_ROWS - 1; r >= 0; r--)
                 if (i < rend[r]) s = rs0[r] + i - (r ? rend[r - 1] : 0);
-            double4 p = pos4[s];
-            if (DRIFT) {
-                double4 v = vel4[s];
-                double4 f;
-                double dth = 0.5 * dt;
-                double im = v.w;   // 100/m
-                const int so = ki.orig ? ki.orig[s] : s;
-                if (KICK) {
-                    f = ttc_combine(s, 7, ki.fpair, ki.g4, mean, so, ki.fixj, ki.n);
-                    f4[s] = f;
-                    v.x = v.x + ki.vscal * dth * v.w * f.x;
-                    v.y = v.y + ki.vscal * dth * v.w * f.y;
-                    v.z = v.z + ki.vscal * dth * v.w * f.z;
-                } else {
-                    f = f4[s];
-                }
-                v.x = vscal * (v.x + dth * im * f.x);
-                v.y = vscal * (v.y + dth * im * f.y);
-                v.z = vscal * (v.z + dth * im * f.z);
-                p.x = pbc1(p.x + dt * v.x, sp.lx);
-                p.y = pbc1(p.y + dt * v.y, sp.ly);
-                p.z = pbc1(p.z + dt * v.z, sp.lz);
-                vel4[s] = v;
-                pos4[s] = p;
-                if (ki.snap) ki.snap[so] = p;
-                if (ki.posq) ki.posq[s] = ttc_posq(p, ki.qkx, ki.qky, ki.qkz);
-                double4 a = anchor[s];
-                double rcut = pcheck[3];
-                double dx = fabs(p.x - a.x);
-                if (dx > 0.5 * sp.lx) dx -= sp.lx;
-                double dy = fabs(p.y - a.y);
-                if (dy > 0.5 * sp.ly) dy -= sp.ly;
-                double dz = fabs(p.z - a.z);
-                if (dz > 0.5 * sp.lz) dz -= sp.lz;
-                if (dx > rcut || dy > rcut || dz > rcut || ttc_r2_exact(dx, dy, dz) > rcut * rcut)
-                    atomicOr(&status[ST_NLFLAG], 1);
-            }
+            const double4 p = DRIFT ? drift_site<false>(s, pos4, vel4, f4, dt, vscal, sp, status, anchor, pcheck, ki,
+                                                        nullptr)
+                                    : pos4[s];
             double ux = (double)sp.K0 * (p.x / sp.lx + 0.5), uy = (double)sp.K1 * (p.y / sp.ly + 0.5),
                    uz = (double)sp.K2 * (p.z / sp.lz + 0.5);
             double fxd = floor(ux), fyd = floor(uy), fzd = floor(uz);
@@ -398,9 +409,9 @@ struct DsPack {
     KickIn ki;
 };
 
-template <int P, bool DRIFT, bool KICK = false>
+template <int P, bool DRIFT>
 __global__ void __launch_bounds__(SPREAD_THREADS) k_drift_spread(DsPack a) {
-    drift_spread_body<P, DRIFT, KICK>(blockIdx.x, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start,
+    drift_spread_body<P, DRIFT>(blockIdx.x, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start,
                                       a.cell_cnt, a.qfix, a.status, a.anchor, a.pcheck, a.ki);
 }
 template <int P, bool DRIFT>
@@ -409,6 +420,43 @@ __global__ void __launch_bounds__(SPREAD_THREADS) kb_drift_spread(const DsPack *
     const DsPack a = pk[m.rep];
     drift_spread_body<P, DRIFT>(m.blk, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start, a.cell_cnt,
                                 a.qfix, a.status, a.anchor, a.pcheck, a.ki);
+}
+
+// The velocity-Verlet loop's drift alone, one thread per site of the sorted
+// order (drift_site with the fused kick), which also copies what this step's
+// spread reads: the new position + charge (spos) and the cell ranges of the
+// current order (scs, scc).  The spread then runs on the side stream beside
+// the step's conditional rebuild, which reorders pos4 and rewrites the cell
+// ranges, and the pair kernel waits for the drift only.
+struct MdDriftPack {
+    double4 *pos4, *vel4, *f4;
+    double dt, vscal;
+    Spread sp;
+    int n, ncell;
+    const int *cell_start, *cell_cnt;
+    int *scs, *scc;
+    double4 *spos;
+    int *status;
+    const double4 *anchor;
+    const double *pcheck;
+    KickIn ki;
+};
+
+template <bool KICK>
+__global__ void __launch_bounds__(256) k_md_drift(MdDriftPack a) {
+    __shared__ double mean[3];
+    // a velocity-Verlet loop whose rebuild overflowed stops here; the host
+    // grows the list and redoes that step from its drifted positions
+    if (a.status[ST_OVERFLOW] & TTC_OVF_REDO) return;
+    if (KICK) ttc_mean_pme(a.ki.partials, a.ki.maxb, a.ki.nbg, a.ki.n, mean);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.ncell) {
+        a.scs[t] = a.cell_start[t];
+        a.scc[t] = a.cell_cnt[t];
+    }
+    if (t >= a.n) return;
+    a.spos[t] = drift_site<KICK>(t, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.status, a.anchor, a.pcheck, a.ki,
+                                 mean);
 }
 
 struct FftArgs {
@@ -1185,16 +1233,26 @@ void ttc_launch_spread(ttc_ctx *c) { launch_drift_spread(c, false, 1.0); }
 void ttc_launch_drift(ttc_ctx *c, double vscal) { launch_drift_spread(c, true, vscal); }
 
 void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick) {
-    int blocks = ds_blocks(c);
-    DsPack a = ds_pack(c, true, vscal);
-    size_t shm = spread_shm(c, a.br);
+    const DsPack d = ds_pack(c, true, vscal);
+    MdDriftPack a{c->pos4, c->vel4, c->f4, d.dt, vscal, d.sp, c->n, c->ncell, c->cell_start, c->cell_cnt,
+                  c->scs, c->scc, c->spos, c->status, c->anchor, d.pcheck, d.ki};
     a.ki.g4 = c->g4o;
     a.ki.orig = c->orig;
     a.ki.snap = c->snap;
-#define DS(P, K) k_drift_spread<P, true, K><<<blocks, SPREAD_THREADS, shm, c->stream>>>(a)
-    if (c->order == 4) { if (kick) DS(4, true); else DS(4, false); }
-    else { if (kick) DS(6, true); else DS(6, false); }
-#undef DS
+    const int blocks = (std::max(c->n, c->ncell) + 255) / 256;
+    if (kick) k_md_drift<true><<<blocks, 256, 0, c->stream>>>(a);
+    else k_md_drift<false><<<blocks, 256, 0, c->stream>>>(a);
+}
+
+void ttc_launch_md_spread(ttc_ctx *c) {
+    int blocks = ds_blocks(c);
+    DsPack a = ds_pack(c, false, 1.0);
+    a.pos4 = c->spos;
+    a.cell_start = c->scs;
+    a.cell_cnt = c->scc;
+    size_t shm = spread_shm(c, a.br);
+    if (c->order == 4) k_drift_spread<4, false><<<blocks, SPREAD_THREADS, shm, c->stream>>>(a);
+    else k_drift_spread<6, false><<<blocks, SPREAD_THREADS, shm, c->stream>>>(a);
 }
 
 int ttc_spread_shm(const ttc_ctx *c) { return (int)spread_shm(c, spread_brick(c)); }
@@ -1216,8 +1274,7 @@ static int set_lds_attrs(ttc_ctx *c) {
     if (shm <= amax) return TTC_OK;
     amax = shm;
     const void *fns[] = {(const void *)k_drift_spread<4, true>, (const void *)k_drift_spread<4, false>,
-                         (const void *)k_drift_spread<6, true>, (const void *)k_drift_spread<6, false>,
-                         (const void *)k_drift_spread<4, true, true>, (const void *)k_drift_spread<6, true, true>};
+                         (const void *)k_drift_spread<6, true>, (const void *)k_drift_spread<6, false>};
     for (const void *f : fns) TTC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
     return TTC_OK;
 }
