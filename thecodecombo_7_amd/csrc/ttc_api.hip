@@ -1605,10 +1605,11 @@ void ttc_launch_nhc_step(ttc_ctx *c, bool energy) {
     ttc_launch_kinet(c);
     ttc_launch_nhc(c);
     ttc_launch_vscale(c);
-    ttc_launch_drift(c, 1.0);
+    ttc_launch_nhc_drift(c);
     ttc_side_fork(c, 0);
     {
         TtcOnSide side(c);
+        ttc_launch_md_spread(c);
         ttc_launch_fft(c, energy);
     }
     ttc_launch_rebuild(c, true);

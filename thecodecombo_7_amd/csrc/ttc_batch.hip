@@ -45,6 +45,7 @@ int ttc_bstage_vscale(const std::vector<ttc_ctx *> &r, TtcStage *st);
 int ttc_bstage_kick(const std::vector<ttc_ctx *> &r, TtcStage *st);
 int ttc_bstage_reduce(const std::vector<ttc_ctx *> &r, TtcStage *st);
 int ttc_bstage_drift(const std::vector<ttc_ctx *> &r, TtcStage *st);
+int ttc_bstage_spread(const std::vector<ttc_ctx *> &r, TtcStage *st);
 int ttc_bstage_fft(const std::vector<ttc_ctx *> &r, TtcStage *st /* [5] */);
 int ttc_bstage_gather(const std::vector<ttc_ctx *> &r, TtcStage *st);
 int ttc_bstage_rebuild(const std::vector<ttc_ctx *> &r, TtcStage *st /* [3] */);
@@ -56,6 +57,7 @@ void ttc_blaunch_vscale(const TtcStage &st, hipStream_t s);
 void ttc_blaunch_kick(const TtcStage &st, bool energy, hipStream_t s);
 void ttc_blaunch_reduce(const TtcStage &st, hipStream_t s);
 void ttc_blaunch_drift(const TtcStage &st, hipStream_t s);
+void ttc_blaunch_spread(const TtcStage &st, hipStream_t s);
 void ttc_blaunch_fft(const TtcStage *st, bool energy, hipStream_t s);
 void ttc_blaunch_gather(const TtcStage &st, bool energy, hipStream_t s);
 void ttc_blaunch_rebuild(const TtcStage *st, hipStream_t s);
@@ -84,7 +86,7 @@ void ttc_stage_free(TtcStage *st) {
 }
 
 // stages of the batched NHC step
-enum { BS_KINET = 0, BS_NHC, BS_VSCALE, BS_DRIFT, BS_FFT, BS_REBUILD = BS_FFT + 5, BS_GATHER = BS_REBUILD + 3,
+enum { BS_KINET = 0, BS_NHC, BS_VSCALE, BS_DRIFT, BS_SPREAD, BS_FFT, BS_REBUILD = BS_FFT + 5, BS_GATHER = BS_REBUILD + 3,
        BS_PAIR, BS_KICK, BS_REDUCE, BS_ANGLES, BS_N };
 
 struct ttc_batch {
@@ -205,6 +207,7 @@ int build_stages(ttc_batch *b) {
     if (!r) r = ttc_bstage_nhc(b->r, &b->st[BS_NHC]);
     if (!r) r = ttc_bstage_vscale(b->r, &b->st[BS_VSCALE]);
     if (!r) r = ttc_bstage_drift(b->r, &b->st[BS_DRIFT]);
+    if (!r) r = ttc_bstage_spread(b->r, &b->st[BS_SPREAD]);
     if (!r) r = ttc_bstage_fft(b->r, &b->st[BS_FFT]);
     if (!r) r = ttc_bstage_rebuild(b->r, &b->st[BS_REBUILD]);
     if (!r) r = ttc_bstage_gather(b->r, &b->st[BS_GATHER]);
@@ -218,9 +221,9 @@ int build_stages(ttc_batch *b) {
 }
 
 // one graph of `spl` NHC steps of the whole batch as batched launches, the
-// same sequence as ttc_launch_nhc_step: thermostat half, kick + drift +
-// spread, FFTs (side stream) beside the conditional rebuild, gather (side)
-// beside the pair kernel, kick, thermostat half (+ energies)
+// same sequence as ttc_launch_nhc_step: thermostat half, drift, spread and
+// FFTs (side stream) beside the conditional rebuild, gather (side) beside
+// the pair kernel, kick, thermostat half (+ energies)
 int capture_batched(ttc_batch *b, bool energy, hipGraphExec_t *out) {
     hipGraph_t g = nullptr;
     const TtcStage *st = b->st;
@@ -235,6 +238,7 @@ int capture_batched(ttc_batch *b, bool energy, hipGraphExec_t *out) {
         ttc_blaunch_drift(st[BS_DRIFT], m);
         e = hipEventRecord(b->bev[0], m);
         if (e == hipSuccess) e = hipStreamWaitEvent(sd, b->bev[0], 0);
+        ttc_blaunch_spread(st[BS_SPREAD], sd);
         ttc_blaunch_fft(&st[BS_FFT], en, sd);
         ttc_blaunch_rebuild(&st[BS_REBUILD], m);
         if (e == hipSuccess) e = hipEventRecord(b->bev[1], m);

@@ -339,11 +339,12 @@ int ttc_pme_theta(ttc_ctx *c);   // fills theta_near (once per context)
 int ttc_pme_conv_blocks(ttc_ctx *c);
 int ttc_gather_blocks(ttc_ctx *c);
 void ttc_launch_spread(ttc_ctx *c);
-void ttc_launch_drift(ttc_ctx *c, double vscal);
 // drift of an MD step: writes the original-order snapshot for the gather
 // and the spread's inputs; kick = the previous step's second half kick fused
 // ahead of it
 void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick);
+// drift of an NHC step (no fused kick) with the spread's copies
+void ttc_launch_nhc_drift(ttc_ctx *c);
 // spread of an MD step from the drift's copies (spos, scs, scc)
 void ttc_launch_md_spread(ttc_ctx *c);
 // the gather in the original numbering (snapshot in, g4o out)

@@ -1,14 +1,14 @@
 // ttc_pme.hip -- smooth PME on the device (replaces pme_qgrid.c,
-// pme_calc.c, pme_spline.c) plus the fused kick/drift that feeds it.
+// pme_calc.c, pme_spline.c) plus the MD loops' kick/drift that feeds it.
 //
-//   k_drift_spread  first half of integrate_vverlet.c:52-83 (kick, drift,
-//                   pbc_all.c:23-88) fused with charge_grid (pme_qgrid.c:
-//                   26-113): B-spline weights (pme_spline.c:10-49) in FP32,
-//                   q*Mx*My*Mz spread with int64 fixed-point atomics, so the
-//                   grid is the same whatever order the atomics land in.
-//   k_md_drift      the velocity-Verlet loop's kick + drift alone, one
-//                   thread per site, with copies of the spread's inputs, so
-//                   the spread of that loop runs on the side stream.
+//   k_md_drift      first half of integrate_vverlet.c:52-83 (kick, drift,
+//                   pbc_all.c:23-88) of the MD loops, one thread per site,
+//                   with copies of the spread's inputs, so the spread runs
+//                   on the side stream beside the step's rebuild.
+//   k_spread        charge_grid (pme_qgrid.c:26-113): B-spline weights
+//                   (pme_spline.c:10-49) in FP32, q*Mx*My*Mz spread with
+//                   int64 fixed-point atomics, so the grid is the same
+//                   whatever order the atomics land in.
 //   k_fft<MODE>     3-D complex FFT as three line passes in LDS (Stockham,
 //                   mixed radix 2/3/4 + direct DFT for other primes).  The
 //                   z pass reads the fixed-point grid and clears it for the
@@ -276,16 +276,14 @@ __device__ __forceinline__ double4 drift_site(int s, double4 *pos4, double4 *vel
     return p;
 }
 
-// DRIFT: drift_site of every site of a brick; then charge_grid
-// (pme_qgrid.c:77-113) into an LDS tile of int64 fixed point, flushed to
-// the global fixed-point grid with one atomic per touched grid point.
-// Integer adds make the grid independent of order.
-template <int P, bool DRIFT>
-__device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 *vel4, double4 *f4, double dt,
-                                                  double vscal, const Spread &sp, const Brick &br,
-                                                  const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
-                                                  long long *qfix, int *status, const double4 *__restrict__ anchor,
-                                                  const double *__restrict__ pcheck, const KickIn &ki = KickIn{}) {
+// charge_grid (pme_qgrid.c:77-113) of every site of a brick into an LDS
+// tile of int64 fixed point, flushed to the global fixed-point grid with
+// one atomic per touched grid point.  Integer adds make the grid
+// independent of order.
+template <int P>
+__device__ __forceinline__ void spread_body(int b, const double4 *__restrict__ pos4, const Spread &sp,
+                                            const Brick &br, const int *__restrict__ cell_start,
+                                            const int *__restrict__ cell_cnt, long long *qfix) {
     extern __shared__ unsigned long long tile[];
     // one chunk of sites after the tile: q Mx[1..P], My[1..P], Mz[1..P] per
     // site and the packed tile-local stencil base (x | y << 10 | z << 20;
@@ -293,9 +291,6 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
     const int chunk = br.chunk;
     float *rw = (float *)(tile + br.rec_off);
     int *rb = (int *)(rw + chunk * 3 * P);
-    // a velocity-Verlet loop whose rebuild overflowed stops here; the host
-    // grows the list and redoes that step from its drifted positions
-    if (DRIFT && ki.snap && (status[ST_OVERFLOW] & TTC_OVF_REDO)) return;
     int ibx = b % br.nbx, iby = (b / br.nbx) % br.nby, ibz = b / (br.nbx * br.nby);
     int cx0 = ibx * br.bx, cy0 = iby * br.by, cz0 = ibz * br.bz;
     int cx1 = min(cx0 + br.bx, br.ncx), cy1 = min(cy0 + br.by, br.ncy), cz1 = min(cz0 + br.bz, br.ncz);
@@ -325,15 +320,13 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
     }
     __syncthreads();
     for (int c0 = 0; c0 < tot; c0 += chunk) {
-        // pass 1: one thread per site (kick, drift, trigger check, weights)
+        // pass 1: one thread per site (B-spline weights)
         for (int i = c0 + threadIdx.x; i < min(tot, c0 + chunk); i += blockDim.x) {
             int s = 0;
 #pragma unroll
             for (int r = BRICK_ROWS - 1; r >= 0; r--)
                 if (i < rend[r]) s = rs0[r] + i - (r ? rend[r - 1] : 0);
-            const double4 p = DRIFT ? drift_site<false>(s, pos4, vel4, f4, dt, vscal, sp, status, anchor, pcheck, ki,
-                                                        nullptr)
-                                    : pos4[s];
+            const double4 p = pos4[s];
             double ux = (double)sp.K0 * (p.x / sp.lx + 0.5), uy = (double)sp.K1 * (p.y / sp.ly + 0.5),
                    uz = (double)sp.K2 * (p.z / sp.lz + 0.5);
             double fxd = floor(ux), fyd = floor(uy), fzd = floor(uz);
@@ -395,39 +388,32 @@ __device__ __forceinline__ void drift_spread_body(int b, double4 *pos4, double4 
     }
 }
 
-struct DsPack {
-    double4 *pos4, *vel4;
-    double4 *f4;
-    double dt, vscal;
+struct SpreadPack {
+    const double4 *pos4;
     Spread sp;
     Brick br;
     const int *cell_start, *cell_cnt;
     long long *qfix;
-    int *status;
-    const double4 *anchor;
-    const double *pcheck;
-    KickIn ki;
 };
 
-template <int P, bool DRIFT>
-__global__ void __launch_bounds__(SPREAD_THREADS) k_drift_spread(DsPack a) {
-    drift_spread_body<P, DRIFT>(blockIdx.x, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start,
-                                      a.cell_cnt, a.qfix, a.status, a.anchor, a.pcheck, a.ki);
+template <int P>
+__global__ void __launch_bounds__(SPREAD_THREADS) k_spread(SpreadPack a) {
+    spread_body<P>(blockIdx.x, a.pos4, a.sp, a.br, a.cell_start, a.cell_cnt, a.qfix);
 }
-template <int P, bool DRIFT>
-__global__ void __launch_bounds__(SPREAD_THREADS) kb_drift_spread(const DsPack *pk, const TtcBmap *map) {
+template <int P>
+__global__ void __launch_bounds__(SPREAD_THREADS) kb_spread(const SpreadPack *pk, const TtcBmap *map) {
     const TtcBmap m = map[blockIdx.x];
-    const DsPack a = pk[m.rep];
-    drift_spread_body<P, DRIFT>(m.blk, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.br, a.cell_start, a.cell_cnt,
-                                a.qfix, a.status, a.anchor, a.pcheck, a.ki);
+    const SpreadPack a = pk[m.rep];
+    spread_body<P>(m.blk, a.pos4, a.sp, a.br, a.cell_start, a.cell_cnt, a.qfix);
 }
 
-// The velocity-Verlet loop's drift alone, one thread per site of the sorted
-// order (drift_site with the fused kick), which also copies what this step's
-// spread reads: the new position + charge (spos) and the cell ranges of the
-// current order (scs, scc).  The spread then runs on the side stream beside
-// the step's conditional rebuild, which reorders pos4 and rewrites the cell
-// ranges, and the pair kernel waits for the drift only.
+// An MD loop's drift alone, one thread per site of the sorted order
+// (drift_site; with the fused kick in the velocity-Verlet loop), which also
+// copies what this step's spread reads: the new position + charge (spos)
+// and the cell ranges of the current order (scs, scc).  The spread then
+// runs on the side stream beside the step's conditional rebuild, which
+// reorders pos4 and rewrites the cell ranges, and the pair kernel waits for
+// the drift only.
 struct MdDriftPack {
     double4 *pos4, *vel4, *f4;
     double dt, vscal;
@@ -443,13 +429,14 @@ struct MdDriftPack {
 };
 
 template <bool KICK>
-__global__ void __launch_bounds__(256) k_md_drift(MdDriftPack a) {
+__device__ __forceinline__ void md_drift_body(int blk, const MdDriftPack &a) {
     __shared__ double mean[3];
     // a velocity-Verlet loop whose rebuild overflowed stops here; the host
-    // grows the list and redoes that step from its drifted positions
-    if (a.status[ST_OVERFLOW] & TTC_OVF_REDO) return;
+    // grows the list and redoes that step from its drifted positions (the
+    // NHC loops, without a snapshot, redo no step)
+    if (a.ki.snap && (a.status[ST_OVERFLOW] & TTC_OVF_REDO)) return;
     if (KICK) ttc_mean_pme(a.ki.partials, a.ki.maxb, a.ki.nbg, a.ki.n, mean);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t < a.ncell) {
         a.scs[t] = a.cell_start[t];
         a.scc[t] = a.cell_cnt[t];
@@ -457,6 +444,15 @@ __global__ void __launch_bounds__(256) k_md_drift(MdDriftPack a) {
     if (t >= a.n) return;
     a.spos[t] = drift_site<KICK>(t, a.pos4, a.vel4, a.f4, a.dt, a.vscal, a.sp, a.status, a.anchor, a.pcheck, a.ki,
                                  mean);
+}
+
+template <bool KICK>
+__global__ void __launch_bounds__(256) k_md_drift(MdDriftPack a) { md_drift_body<KICK>(blockIdx.x, a); }
+
+// batched (ttc_batch.hip): the NHC step's drift, block `blk` of context `rep`
+__global__ void __launch_bounds__(256) kb_md_drift(const MdDriftPack *pk, const TtcBmap *map) {
+    const TtcBmap m = map[blockIdx.x];
+    md_drift_body<false>(m.blk, pk[m.rep]);
 }
 
 struct FftArgs {
@@ -1196,64 +1192,69 @@ static size_t brick_shm(const ttc_ctx *c, const Brick &br) {
     return t * sizeof(unsigned long long);
 }
 
-static DsPack ds_pack(ttc_ctx *c, bool drift, double vscal) {
-    DsPack a{c->pos4, c->vel4, c->f4, drift ? c->p.dt : 0.0, vscal, spread_args(c), spread_brick(c),
-             c->cell_start, c->cell_cnt, c->qfix, c->status, c->anchor, c->ev + EV_N,
-             KickIn{c->fpair, c->g4, c->partials, c->nblk_max, ttc_gather_blocks(c), c->n, vscal, nullptr,
-                    nullptr, c->fixj}};
-    if (drift) {
-        // the drifted positions' fixed-point coordinates, which the
-        // half-list pair kernel reads
-        a.ki.posq = c->posq;
-        a.ki.qkx = 4294967296.0 / c->p.lx;
-        a.ki.qky = 4294967296.0 / c->p.ly;
-        a.ki.qkz = 4294967296.0 / c->p.lz;
-    }
-    return a;
+// the spread of the current positions (sorted order, the cell ranges of
+// the last rebuild)
+static SpreadPack spread_pack(ttc_ctx *c) {
+    return SpreadPack{c->pos4, spread_args(c), spread_brick(c), c->cell_start, c->cell_cnt, c->qfix};
 }
 
-static int ds_blocks(const ttc_ctx *c) {
+static int spread_blocks(const ttc_ctx *c) {
     Brick br = brick_args(c);
     return br.nbx * br.nby * br.nbz;
 }
 
-static void launch_drift_spread(ttc_ctx *c, bool drift, double vscal) {
-    int blocks = ds_blocks(c);
-    DsPack a = ds_pack(c, drift, vscal);
-    size_t shm = spread_shm(c, a.br);
-#define DS(P, D) k_drift_spread<P, D><<<blocks, SPREAD_THREADS, shm, c->stream>>>(a)
-    if (c->order == 4) { if (drift) DS(4, true); else DS(4, false); }
-    else { if (drift) DS(6, true); else DS(6, false); }
-#undef DS
+static void launch_spread_pack(ttc_ctx *c, const SpreadPack &a) {
+    const size_t shm = spread_shm(c, a.br);
+    if (c->order == 4) k_spread<4><<<spread_blocks(c), SPREAD_THREADS, shm, c->stream>>>(a);
+    else k_spread<6><<<spread_blocks(c), SPREAD_THREADS, shm, c->stream>>>(a);
 }
 
-// spread only (ttc_pme_calc / ttc_cnonbondnl): positions unchanged
-void ttc_launch_spread(ttc_ctx *c) { launch_drift_spread(c, false, 1.0); }
+// spread only (ttc_pme_calc / ttc_cnonbondnl, the redo of an overflowed step)
+void ttc_launch_spread(ttc_ctx *c) { launch_spread_pack(c, spread_pack(c)); }
 
-void ttc_launch_drift(ttc_ctx *c, double vscal) { launch_drift_spread(c, true, vscal); }
+// the drift alone with the spread's copies and the drifted positions'
+// fixed-point coordinates (which the half-list pair kernel reads); the
+// velocity-Verlet loop's (vv) also writes the original-order snapshot and
+// takes the gather's output in the original numbering
+static MdDriftPack md_drift_pack(ttc_ctx *c, double vscal, bool vv) {
+    MdDriftPack a{c->pos4, c->vel4, c->f4, c->p.dt, vscal, spread_args(c), c->n, c->ncell, c->cell_start,
+                  c->cell_cnt, c->scs, c->scc, c->spos, c->status, c->anchor, c->ev + EV_N,
+                  KickIn{c->fpair, c->g4, c->partials, c->nblk_max, ttc_gather_blocks(c), c->n, vscal, nullptr,
+                         nullptr, c->fixj}};
+    a.ki.posq = c->posq;
+    a.ki.qkx = 4294967296.0 / c->p.lx;
+    a.ki.qky = 4294967296.0 / c->p.ly;
+    a.ki.qkz = 4294967296.0 / c->p.lz;
+    if (vv) {
+        a.ki.g4 = c->g4o;
+        a.ki.orig = c->orig;
+        a.ki.snap = c->snap;
+    }
+    return a;
+}
+
+static int md_drift_blocks(const ttc_ctx *c) { return (std::max(c->n, c->ncell) + 255) / 256; }
 
 void ttc_launch_md_drift(ttc_ctx *c, double vscal, bool kick) {
-    const DsPack d = ds_pack(c, true, vscal);
-    MdDriftPack a{c->pos4, c->vel4, c->f4, d.dt, vscal, d.sp, c->n, c->ncell, c->cell_start, c->cell_cnt,
-                  c->scs, c->scc, c->spos, c->status, c->anchor, d.pcheck, d.ki};
-    a.ki.g4 = c->g4o;
-    a.ki.orig = c->orig;
-    a.ki.snap = c->snap;
-    const int blocks = (std::max(c->n, c->ncell) + 255) / 256;
-    if (kick) k_md_drift<true><<<blocks, 256, 0, c->stream>>>(a);
-    else k_md_drift<false><<<blocks, 256, 0, c->stream>>>(a);
+    const MdDriftPack a = md_drift_pack(c, vscal, true);
+    if (kick) k_md_drift<true><<<md_drift_blocks(c), 256, 0, c->stream>>>(a);
+    else k_md_drift<false><<<md_drift_blocks(c), 256, 0, c->stream>>>(a);
 }
 
-void ttc_launch_md_spread(ttc_ctx *c) {
-    int blocks = ds_blocks(c);
-    DsPack a = ds_pack(c, false, 1.0);
+void ttc_launch_nhc_drift(ttc_ctx *c) {
+    k_md_drift<false><<<md_drift_blocks(c), 256, 0, c->stream>>>(md_drift_pack(c, 1.0, false));
+}
+
+// the spread from the drift's copies
+static SpreadPack md_spread_pack(ttc_ctx *c) {
+    SpreadPack a = spread_pack(c);
     a.pos4 = c->spos;
     a.cell_start = c->scs;
     a.cell_cnt = c->scc;
-    size_t shm = spread_shm(c, a.br);
-    if (c->order == 4) k_drift_spread<4, false><<<blocks, SPREAD_THREADS, shm, c->stream>>>(a);
-    else k_drift_spread<6, false><<<blocks, SPREAD_THREADS, shm, c->stream>>>(a);
+    return a;
 }
+
+void ttc_launch_md_spread(ttc_ctx *c) { launch_spread_pack(c, md_spread_pack(c)); }
 
 int ttc_spread_shm(const ttc_ctx *c) { return (int)spread_shm(c, spread_brick(c)); }
 
@@ -1273,8 +1274,7 @@ static int set_lds_attrs(ttc_ctx *c) {
     int &amax = attr_max[dev & 63];
     if (shm <= amax) return TTC_OK;
     amax = shm;
-    const void *fns[] = {(const void *)k_drift_spread<4, true>, (const void *)k_drift_spread<4, false>,
-                         (const void *)k_drift_spread<6, true>, (const void *)k_drift_spread<6, false>};
+    const void *fns[] = {(const void *)k_spread<4>, (const void *)k_spread<6>};
     for (const void *f : fns) TTC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
     return TTC_OK;
 }
@@ -1411,7 +1411,20 @@ int ttc_pme_theta(ttc_ctx *c) {
 // ---- batched stages of the NHC step (ttc_batch.hip) ------------------------
 
 int ttc_bstage_drift(const std::vector<ttc_ctx *> &r, TtcStage *st) {
-    std::vector<DsPack> pk;
+    std::vector<MdDriftPack> pk;
+    std::vector<int> nb;
+    for (ttc_ctx *c : r) {
+        pk.push_back(md_drift_pack(c, 1.0, false));
+        nb.push_back(md_drift_blocks(c));
+    }
+    return ttc_stage_make(st, pk, nb, 0);
+}
+void ttc_blaunch_drift(const TtcStage &st, hipStream_t s) {
+    kb_md_drift<<<st.nblk, 256, 0, s>>>((const MdDriftPack *)st.pk, st.map);
+}
+
+int ttc_bstage_spread(const std::vector<ttc_ctx *> &r, TtcStage *st) {
+    std::vector<SpreadPack> pk;
     std::vector<int> nb;
     size_t shm = 0;
     for (ttc_ctx *c : r) {
@@ -1419,20 +1432,20 @@ int ttc_bstage_drift(const std::vector<ttc_ctx *> &r, TtcStage *st) {
             ttc_set_error("ttc_batch: batched launches need spline order 4 in every replica");
             return TTC_ERR_ARG;
         }
-        pk.push_back(ds_pack(c, true, 1.0));
-        nb.push_back(ds_blocks(c));
+        pk.push_back(md_spread_pack(c));
+        nb.push_back(spread_blocks(c));
         shm = std::max(shm, spread_shm(c, pk.back().br));
     }
     if (spread_occupancy(shm) < 1) {
         ttc_set_error("ttc_batch: PME spread tile exceeds the LDS of one CU");
         return TTC_ERR_ARG;
     }
-    TTC_HIP(hipFuncSetAttribute((const void *)kb_drift_spread<4, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    TTC_HIP(hipFuncSetAttribute((const void *)kb_spread<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)shm));
     return ttc_stage_make(st, pk, nb, shm);
 }
-void ttc_blaunch_drift(const TtcStage &st, hipStream_t s) {
-    kb_drift_spread<4, true><<<st.nblk, SPREAD_THREADS, st.shm, s>>>((const DsPack *)st.pk, st.map);
+void ttc_blaunch_spread(const TtcStage &st, hipStream_t s) {
+    kb_spread<4><<<st.nblk, SPREAD_THREADS, st.shm, s>>>((const SpreadPack *)st.pk, st.map);
 }
 
 // st[FP_N]: the five FFT passes
